@@ -1,0 +1,37 @@
+/*
+ * scenes.h — synthetic benchmark scenes (configurations 1-5) built through the
+ * tracer:: factory API, and a C ABI to fetch them as tr_scene_desc.
+ *
+ * There is no reference counterpart: the reference hard-codes its scenes in
+ * src/main.cpp run*() (src/main.cpp:4-313) and loads external OBJ assets.
+ * These generators play that role for the benchmark configurations.  All
+ * randomness is PCG32 with a fixed draw order, so a (config, seed) pair always
+ * gives the same scene.
+ */
+#ifndef TRACER_SCENES_H
+#define TRACER_SCENES_H
+
+#include <stdint.h>
+#include "tracer_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct trs_scene trs_scene;
+
+/* config: 1..5.  seed: PCG32 seed (the configurations use seed = config - 1).
+ * detail: 1.0 = full size; < 1 shrinks the heavy parts (sphere count,
+ * terrain resolution) for quick tests.  Returns NULL on failure and sets
+ * *err (static storage) when err is not NULL. */
+trs_scene* trs_create(int32_t config, uint64_t seed, double detail, const char** err);
+const tr_scene_desc* trs_desc(const trs_scene* s);
+/* Film, spp and depths of the configuration (PTRendererParams + film). */
+void trs_render_defaults(const trs_scene* s, tr_render_params* out);
+int32_t trs_max_leaf(const trs_scene* s);
+void trs_destroy(trs_scene* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TRACER_SCENES_H */

@@ -1,0 +1,231 @@
+/*
+ * tracer_hip.h — C ABI of libtracer_hip.so, the MI355X (gfx950) wavefront path
+ * tracer.  Plain C types, plain pointers and sizes; no exceptions cross it.
+ *
+ * The reference (wyzwzz/Tracer) has no FFI: its public surface is the C++
+ * factory API in namespace tracer (the headers under src/factory/) and the abstract
+ * Renderer/Scene classes (src/core/renderer.hpp:12-18, src/core/scene.hpp:11-33).
+ * This ABI is what that API flattens to; the C++ facade in include/tracer/
+ * keeps the reference's names and signatures and calls into it.  Each entry
+ * point below names the reference interface it replaces.
+ *
+ * Conventions
+ *   - Every call returns tr_status (0 = TR_OK).  tr_last_error(ctx) returns a
+ *     message for the last failure on that context.
+ *   - One HIP stream per context; calls on one context are serialised, and
+ *     separate contexts are independent.  A context is bound to one device.
+ *   - Host pointers are caller-owned and only read (or written) during the
+ *     call.  Functions whose name ends in _device take device pointers and
+ *     enqueue on the context's stream without synchronising.
+ */
+#ifndef TRACER_HIP_H
+#define TRACER_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t tr_status;
+enum {
+    TR_OK = 0,
+    TR_ERR_INVALID = 1,   /* bad argument or inconsistent scene */
+    TR_ERR_HIP = 2,       /* a HIP runtime call failed */
+    TR_ERR_OOM = 3,       /* device allocation failed */
+    TR_ERR_STATE = 4      /* call order violated (e.g. render before upload) */
+};
+
+typedef struct tr_ctx tr_ctx;
+
+/* Material kinds.  TR_MAT_LAMBERT is the reference's Phong material with
+ * ks = 0 (src/material/phong.cpp:29-70): base_color is kd.  TR_MAT_DISNEY is
+ * create_disney (src/factory/material.hpp:37-52, src/material/disney.cpp)
+ * with every texture constant. */
+enum { TR_MAT_LAMBERT = 0, TR_MAT_DISNEY = 1 };
+
+typedef struct tr_material {
+    int32_t type;
+    float base_color[3];
+    float metallic, roughness, transmission, transmission_roughness, ior;
+    float specular_scale[3];
+    float specular_tint, anisotropic, sheen, sheen_tint, clearcoat, clearcoat_gloss;
+} tr_material;
+
+/* create_homogeneous_medium(sigma_a, sigma_s, g, max_scattering_count),
+ * src/factory/medium.hpp:21-26, src/medium/homogeneous.cpp:12-18. */
+typedef struct tr_medium {
+    float sigma_a[3];
+    float sigma_s[3];
+    float g;
+    int32_t max_scattering_count;
+} tr_medium;
+
+/* create_thin_lens_camera(aspect, pos, target, up, fov, lens_radius,
+ * focal_dist), src/factory/camera.hpp:9-11; fov is the vertical field of view
+ * in radians (src/camera/thin_lens_camera.cpp:31). */
+typedef struct tr_camera {
+    float pos[3];
+    float target[3];
+    float up[3];
+    float fov_y;
+    float lens_radius;
+    float focal_distance;
+    float aspect;
+} tr_camera;
+
+/* A flattened scene: what create_triangle_mesh + create_geometric_primitive +
+ * create_bvh_accel::build + create_general_scene (+ lights, environment
+ * light, camera) describe in the reference.  Triangle i is one
+ * GeometricPrimitive (src/shape/geometry.cpp:10-75): its material, its
+ * MediumInterface (-1 = vacuum) and, if emission is non-zero, a one-sided
+ * DiffuseLight (src/light/diffuse_light.cpp) on the side of its geometric
+ * normal cross(B-A, C-A). */
+typedef struct tr_scene_desc {
+    int64_t num_vertices;
+    const float* positions;        /* 3 * num_vertices, world space */
+    const float* normals;          /* 3 * num_vertices, or NULL (flat shading) */
+    const float* uvs;              /* 2 * num_vertices, or NULL */
+    int64_t num_triangles;
+    const int32_t* indices;        /* 3 * num_triangles */
+    const int32_t* material_ids;   /* num_triangles */
+    const float* emission;         /* 3 * num_triangles, or NULL */
+    const int32_t* medium_inside;  /* num_triangles, or NULL (all vacuum) */
+    const int32_t* medium_outside; /* num_triangles, or NULL (all vacuum) */
+    int32_t num_materials;
+    int32_t num_media;
+    const tr_material* materials;
+    const tr_medium* media;
+    /* Constant environment light: create_ibl_light(create_constant_texture2d(L)). */
+    int32_t has_env;
+    float env_radiance[3];
+    /* Directional light (no reference counterpart): direction points from the
+     * scene towards the sun; irradiance is measured normal to it. */
+    int32_t has_sun;
+    float sun_direction[3];
+    float sun_irradiance[3];
+    tr_camera camera;
+} tr_scene_desc;
+
+/* BVH options.  max_leaf_primitives mirrors create_bvh_accel(int)
+ * (src/factory/accelerator.hpp:12); the device layout is always the 4-wide
+ * node format described in the library. */
+typedef struct tr_bvh_opts {
+    int32_t max_leaf_primitives;  /* leaf size bound for the binary SAH pass (reference default 3) */
+    int32_t num_threads;          /* host build threads, 0 = all */
+} tr_bvh_opts;
+
+enum { TR_FILTER_BOX = 0, TR_FILTER_GAUSSIAN = 1 };
+enum { TR_TR_ANALYTIC = 0, TR_TR_RATIO = 1 };
+
+/* PTRendererParams (src/factory/renderer.hpp:12-19) plus the film and the
+ * wavefront/guide controls that have no reference counterpart. */
+typedef struct tr_render_params {
+    int32_t width, height;
+    int32_t spp;
+    int32_t min_depth;                 /* Russian roulette (survival 0.9) when depth > min_depth */
+    int32_t max_depth;
+    int32_t direct_light_sample_num;   /* light samples per vertex (reference default 1) */
+    int32_t filter_type;               /* TR_FILTER_* (reference: Gaussian only) */
+    float filter_radius;               /* <= 1 */
+    float filter_alpha;
+    int32_t shadow_transmittance;      /* TR_TR_* */
+    uint64_t seed;
+    int32_t sample_offset;             /* index of the first sample, for progressive rendering */
+    int32_t samples_per_pass;          /* samples per pixel in flight at once, 0 = auto */
+    int32_t guide;                     /* 0 = plain path tracer, 1 = learned path guiding */
+    int32_t guide_train_passes;        /* passes that train the guide before it is used (guide=1) */
+    float guide_fraction;              /* one-sample MIS probability of the guide (0.5) */
+    float guide_lr;                    /* Adam learning rate (1e-3) */
+    int32_t profile;                   /* 1 = time kernels with HIP events */
+    int32_t reserved;
+} tr_render_params;
+
+typedef struct tr_render_stats {
+    uint64_t camera_paths;
+    uint64_t extension_rays;      /* closest-hit rays traced */
+    uint64_t shadow_rays;         /* any-hit rays traced */
+    uint64_t nonfinite_dropped;   /* samples dropped for NaN/Inf (pixel_sampler_renderer.cpp:83-87) */
+    uint64_t guide_fallbacks;     /* guided vertices that fell back to BSDF-only sampling */
+    uint64_t guide_train_samples;
+    double ms_total;              /* wall time of the render on the device (HIP events) */
+    double ms_extend;             /* k_extend total (profile = 1) */
+    double ms_shade;
+    double ms_shadow;
+    double ms_other;
+    double ms_guide;              /* guide inference + training (profile = 1) */
+    uint64_t extend_launches;
+    uint64_t pad;
+} tr_render_stats;
+
+/* ---- context ---------------------------------------------------------- */
+tr_status tr_create(tr_ctx** out, int32_t device);
+void tr_destroy(tr_ctx* ctx);
+const char* tr_last_error(const tr_ctx* ctx);
+/* Returns a static string describing the library build (arch, version). */
+const char* tr_version(void);
+
+/* ---- scene (replaces Aggregate::build + create_general_scene,
+ *      src/core/aggregate.hpp:19, src/factory/scene.hpp:14) ---------------- */
+tr_status tr_scene_upload(tr_ctx* ctx, const tr_scene_desc* scene);
+/* BVHAccel::build (src/accelerator/bvh.cpp:78-105): host SAH build, 4-wide
+ * collapse, upload.  Requires tr_scene_upload first. */
+tr_status tr_bvh_build(tr_ctx* ctx, const tr_bvh_opts* opts);
+/* Number of nodes / bytes of the device BVH and triangle arrays. */
+tr_status tr_bvh_info(tr_ctx* ctx, int64_t* num_nodes, int64_t* node_bytes, int64_t* tri_bytes);
+
+/* Host-only build of the device BVH (no GPU needed): 4-wide nodes of 128 bytes
+ * and 48-byte leaf-ordered triangles, as uploaded by tr_bvh_build.  Call with
+ * nodes_out = tris_out = NULL to get *num_nodes first.  For inspection and
+ * tests. */
+tr_status tr_bvh4_export(const tr_scene_desc* scene, const tr_bvh_opts* opts, void* nodes_out, int64_t nodes_cap,
+                         void* tris_out, int64_t* num_nodes, int32_t* depth);
+/* The first n outputs of the PCG32 stream the renderer draws for camera
+ * sample `sample` of pixel `pixel` (host evaluation of the device code). */
+void tr_rng_sequence(uint64_t seed, uint32_t pixel, uint32_t sample, int64_t n, uint32_t* out);
+
+/* ---- render (replaces Renderer::render, src/core/renderer.hpp:16, for the
+ *      PathTraceRenderer of src/renderer/path_trace_renderer.cpp) ---------- */
+/* film_sums: W*H*4 doubles (R, G, B, filter weight), accumulated onto what the
+ * caller passes in when accumulate != 0 (progressive), else overwritten.
+ * image: W*H*3 floats, sums / weight (Film::write_render_target,
+ * src/core/render.hpp:105-115).  Either may be NULL. */
+tr_status tr_render(tr_ctx* ctx, const tr_render_params* params, double* film_sums,
+                    int32_t accumulate, float* image, tr_render_stats* stats);
+
+/* ---- ray queries (replace Aggregate::intersect_p / intersect,
+ *      src/accelerator/bvh.cpp:112-185, with the watertight test) --------- */
+/* rays: n * 8 floats (ox, oy, oz, tmin, dx, dy, dz, tmax); directions need not
+ * be normalised.  hits: n * 4 floats (t, prim as int32 bits, b1, b2); prim =
+ * -1 on a miss; b1/b2 weight the triangle's 2nd/3rd vertex.  Ties in t go to
+ * the lowest primitive index. */
+tr_status tr_trace_closest(tr_ctx* ctx, const float* rays, float* hits, int64_t n);
+tr_status tr_trace_any(tr_ctx* ctx, const float* rays, int32_t* occluded, int64_t n);
+/* Device-pointer forms, enqueued on the context stream. */
+tr_status tr_trace_closest_device(tr_ctx* ctx, const float* d_rays, float* d_hits, int64_t n);
+tr_status tr_trace_any_device(tr_ctx* ctx, const float* d_rays, int32_t* d_occluded, int64_t n);
+/* Closest-hit with per-ray visit counters (nodes, triangles) for the
+ * algorithmic-bytes model; slower, never used by tr_render. */
+tr_status tr_trace_closest_counted(tr_ctx* ctx, const float* rays, float* hits,
+                                   uint32_t* node_visits, uint32_t* tri_tests, int64_t n);
+tr_status tr_stream(tr_ctx* ctx, void** hip_stream);
+tr_status tr_synchronize(tr_ctx* ctx);
+
+/* ---- BSDF batches (replace BSDF::eval / pdf / sample of DisneyBSDF and the
+ *      Phong/Lambert AggregateBSDF, src/core/bsdf.hpp:95-112) ------------- */
+/* All vectors in the shading frame (shading normal = +z).  ng: geometric
+ * normal per entry, or NULL (ng = +z).  f: n*3, pdf: n. */
+tr_status tr_bsdf_eval_batch(tr_ctx* ctx, const tr_material* mats, int32_t num_mats,
+                             const int32_t* mat_ids, const float* wi, const float* wo,
+                             const float* ng, int64_t n, float* f, float* pdf);
+/* u: n*3 uniforms (lobe selector, u, v).  wi: n*3, f: n*3, pdf: n
+ * (pdf = 0 for an invalid sample). */
+tr_status tr_bsdf_sample_batch(tr_ctx* ctx, const tr_material* mats, int32_t num_mats,
+                               const int32_t* mat_ids, const float* wo, const float* ng,
+                               const float* u, int64_t n, float* wi, float* f, float* pdf);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TRACER_HIP_H */

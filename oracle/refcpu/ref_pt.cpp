@@ -1,0 +1,978 @@
+// refcpu — TEST INFRASTRUCTURE ONLY (see ref_math.hpp header).
+//
+// Scene, lights, medium, camera, film and the path-tracing loop of the
+// reference, plus the extern "C" entry points the tests and the CPU-baseline
+// leg of bench.py call through ctypes.
+//
+//   PixelSamplerRenderer::render   src/renderer/pixel_sampler_renderer.cpp:24-109
+//   PathTraceRenderer::eval_pixel_li  src/renderer/path_trace_renderer.cpp:27-210
+//   NEE / MIS helpers             src/renderer/direct_illumination.cpp:11-264
+//   Triangle surface frames       src/shape/triangle.cpp:178-213, :61-97
+//   DiffuseLight                  src/light/diffuse_light.cpp:17-60
+//   IBL with a constant texture   src/light/environment_light.cpp:20-196
+//   HomogeneousMedium / vacuum    src/medium/homogeneous.cpp:12-66, src/medium/vacuum.cpp:8-29
+//   ThinLensCamera::generate_ray  src/camera/thin_lens_camera.cpp:23-59, src/utility/transform.cpp:96-124
+//   GaussianFilter / Film tiles   src/filter/gaussian_filter.cpp:9-35, src/core/render.hpp:37-115
+//   parallel_for_2d               src/utility/parallel.hpp:65-107
+//
+// Modes
+//   quirks = 1 (Reference): the reference's estimator as written, including
+//     Q1 (medium segment sampled from the camera-ray origin), Q4 (scatter
+//     weight 1), Q5 (MIS-BSDF light hit attenuated by Tr(ray.o, isect.pos)),
+//     Q6 (no environment NEE at medium vertices) and the Disney Q2/Q3/Q21.
+//   quirks = 0 (Fixed): the same loop with those defects corrected.
+//   nee = 0: every light sampled at every vertex (reference); nee = 1: one
+//     light picked by power (compute_light_power_distribution,
+//     direct_illumination.cpp:257-264).
+// The RNG is PCG32 seeded per (pixel, sample) so renders are reproducible
+// (the reference seeds one stream per thread, sampler.hpp:145-147).
+#include "ref_math.hpp"
+#include "ref_rng.hpp"
+#include "ref_triangle.hpp"
+#include "ref_bvh.hpp"
+#include "ref_bsdf.hpp"
+#include "tracer_hip.h"
+
+#include <atomic>
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+namespace refcpu {
+
+struct Sampler {
+    Pcg32 rng;
+    float get() { return rng.next_float(); }
+};
+
+struct Isect {
+    V3f pos, wo;
+    Coord<float> geo, shd;
+    int prim = -1;
+    float t = 0;
+    V3f eps_offset(const V3f& dir) const {   // intersection.hpp:19-23
+        if (dot(geo.z, dir) > 0) return pos + geo.z * kEps;
+        return pos - geo.z * kEps;
+    }
+};
+
+struct Scene {
+    std::vector<V3f> P, N;
+    std::vector<float> UV;
+    std::vector<int> idx, mat, med_in, med_out;
+    std::vector<float> emis;
+    std::vector<tr_material> mats;
+    std::vector<tr_medium> media;
+    RefBVH bvh;
+    std::vector<int> lights;          // emissive triangles, one DiffuseLight each
+    bool has_env = false;
+    RGB<float> env;
+    bool has_sun = false;
+    V3f sun_dir;
+    RGB<float> sun_E;
+    tr_camera cam;
+    Bounds3 world;
+    V3f wcenter;
+    float wradius = 0;
+    // power distribution over [area lights..., env, sun]
+    std::vector<float> lpmf, lcdf;
+    int n_light_entries = 0;
+
+    V3f vtx(int t, int k) const { return P[idx[3 * t + k]]; }
+    float tri_area(int t) const {   // triangle.cpp:54-59
+        V3f A = vtx(t, 0), B = vtx(t, 1), C = vtx(t, 2);
+        return float(0.5 * cross(B - A, C - A).length());
+    }
+    RGB<float> tri_emission(int t) const {
+        if (emis.empty()) return RGB<float>(0);
+        return RGB<float>(emis[3 * t], emis[3 * t + 1], emis[3 * t + 2]);
+    }
+};
+
+// triangle.cpp:178-213 — surface interaction for a hit (alpha -> B, beta -> C)
+static void fill_isect(const Scene& s, int prim, const V3f& o, const V3f& d, float t, float alpha, float beta, Isect& is) {
+    const int i0 = s.idx[3 * prim], i1 = s.idx[3 * prim + 1], i2 = s.idx[3 * prim + 2];
+    const V3f A = s.P[i0], B = s.P[i1], C = s.P[i2];
+    const V3f AB = B - A, AC = C - A;
+    float ua = 0, va = 0, ub = 0, vb = 0, uc = 0, vc = 0;
+    if (!s.UV.empty()) {
+        ua = s.UV[2 * i0]; va = s.UV[2 * i0 + 1];
+        ub = s.UV[2 * i1]; vb = s.UV[2 * i1 + 1];
+        uc = s.UV[2 * i2]; vc = s.UV[2 * i2 + 1];
+    }
+    V3f gn = cross(AB, AC).normalize();
+    V3f nA = gn, nB = gn, nC = gn;
+    if (!s.N.empty()) { nA = s.N[i0].normalize(); nB = s.N[i1].normalize(); nC = s.N[i2].normalize(); }
+    is.wo = -d;
+    is.pos = o + d * t;
+    is.prim = prim;
+    is.t = t;
+    V3f dpdu, dpdv;
+    compute_ss_ts(AB, AC, ub - ua, vb - va, uc - ua, vc - va, gn, dpdu, dpdv);
+    dpdu = cross(dpdv, gn);
+    dpdv = cross(gn, dpdu);
+    is.geo = Coord<float>(dpdu, dpdv, gn);
+    V3f sn = nA + (nB - nA) * alpha + (nC - nA) * beta;
+    V3f dndu, dndv;
+    compute_ss_ts(nB - nA, nC - nA, ub - ua, vb - va, uc - ua, vc - va, sn, dndu, dndv);
+    if (dndu.length_squared() < kEps || dndv.length_squared() < kEps) coordinate_system(sn, dndu, dndv);
+    dndu = cross(dndv, sn);
+    dndv = cross(sn, dndu);
+    is.shd = Coord<float>(dndu, dndv, sn);
+}
+
+// Reference closest hit: BVHAccel::intersect_p with Moller-Trumbore; the
+// surface interaction is formed for every accepted candidate as in
+// geometry.cpp:49-60 / triangle.cpp:175-213.
+static bool intersect_ref(const Scene& s, const V3f& o, const V3f& dn, float tmin, float tmax, Isect* is) {
+    float tm = tmax;
+    return s.bvh.traverse(o, dn, tm, [&](int p, float& tcur) {
+        MTHit h;
+        if (!mt_intersect(o, dn, tmin, tcur, s.vtx(p, 0), s.vtx(p, 1), s.vtx(p, 2), &h)) return false;
+        tcur = h.t;
+        if (is) fill_isect(s, p, o, dn, h.t, h.alpha, h.beta, *is);
+        return true;
+    }, false);
+}
+static bool occluded_ref(const Scene& s, const V3f& o, const V3f& dn, float tmin, float tmax) {
+    float tm = tmax;
+    return s.bvh.traverse(o, dn, tm, [&](int p, float& tcur) {
+        return mt_intersect(o, dn, tmin, tcur, s.vtx(p, 0), s.vtx(p, 1), s.vtx(p, 2), nullptr);
+    }, true);
+}
+
+// ----------------------------------------------------------------------------
+// Lights
+struct LightSample { V3f pos; RGB<float> L; float pdf = 0; bool valid = false; };
+
+// diffuse_light.cpp:44-60 with triangle.cpp:61-97 (b.x weights A, b.y weights B)
+static LightSample sample_area(const Scene& s, int tri, const V3f& ref, float u, float v) {
+    LightSample ls;
+    float su0 = std::sqrt(u);
+    float bx = 1 - su0, by = v * su0;
+    V3f A = s.vtx(tri, 0), B = s.vtx(tri, 1), C = s.vtx(tri, 2);
+    V3f pos = A * bx + B * by + C * (1 - bx - by);
+    V3f gn = cross(B - A, C - A).normalize();
+    if (dot(gn, ref - pos) <= 0) return ls;
+    V3f sp2ref = ref - pos;
+    float dist2 = float(sp2ref.length_squared());
+    float area_pdf = 1 / s.tri_area(tri);
+    ls.pdf = area_pdf * dist2 / std::abs(float(dot(gn, sp2ref) / sp2ref.length()));
+    ls.pos = pos;
+    ls.L = s.tri_emission(tri);
+    ls.valid = true;
+    return ls;
+}
+// diffuse_light.cpp:31-42
+static float pdf_area(const Scene& s, int tri, const V3f& ref, const V3f& pos) {
+    V3f A = s.vtx(tri, 0), B = s.vtx(tri, 1), C = s.vtx(tri, 2);
+    V3f n = cross(B - A, C - A).normalize();
+    if (dot(n, ref - pos) <= 0) return 0;
+    float area = s.tri_area(tri);
+    V3f p2r = ref - pos;
+    float d2 = float(p2r.length_squared());
+    float ac = std::abs(float(dot(n, p2r) / (n.length() * p2r.length())));
+    return d2 / (area * ac);
+}
+// environment_light.cpp:92-119,179-196 for a 1x1 constant texture: the 2D
+// distribution is uniform, theta = pi v, phi = 2 pi u, pdf = 1/(2 pi^2 sin).
+static LightSample sample_env(const Scene& s, const V3f& ref, float u, float v) {
+    LightSample ls;
+    float theta = v * kPi<float>, phi = u * 2 * kPi<float>;
+    float st = std::sin(theta), ct = std::cos(theta);
+    V3f wi = V3f(st * std::cos(phi), st * std::sin(phi), ct).normalize();
+    if (st == 0) return ls;
+    ls.pdf = 1 / (2 * kPi<float> * kPi<float> * st);
+    ls.L = s.env;
+    ls.pos = ref + wi * (2 * s.wradius);
+    ls.valid = true;
+    return ls;
+}
+static float pdf_env(const V3f& dir) {
+    float theta = std::acos(std::clamp<float>(dir.z / float(dir.length()), -1, 1));
+    float st = std::sin(theta);
+    if (st == 0) return 0;
+    return 1 / (2 * kPi<float> * kPi<float> * st);
+}
+
+// ----------------------------------------------------------------------------
+// Medium: homogeneous.cpp:26-66 (quirks) / spectral-MIS free flight (fixed)
+struct MediumSample { bool scattered = false; V3f pos; RGB<float> w{1, 1, 1}; };
+static RGB<float> medium_tr(const tr_medium* m, const V3f& a, const V3f& b) {
+    if (!m) return RGB<float>(1);
+    float d = float((a - b).length());
+    return RGB<float>(std::exp(-(m->sigma_a[0] + m->sigma_s[0]) * d), std::exp(-(m->sigma_a[1] + m->sigma_s[1]) * d),
+                      std::exp(-(m->sigma_a[2] + m->sigma_s[2]) * d));
+}
+static MediumSample medium_sample(const tr_medium* m, const V3f& a, const V3f& b, Sampler& sp, bool quirks) {
+    MediumSample r;
+    if (!m) return r;
+    RGB<float> ss(m->sigma_s[0], m->sigma_s[1], m->sigma_s[2]);
+    RGB<float> st(m->sigma_a[0] + ss.r, m->sigma_a[1] + ss.g, m->sigma_a[2] + ss.b);
+    if (ss.is_black()) { r.w = medium_tr(m, a, b); return r; }
+    float u = sp.get(), v = sp.get();
+    int ch = std::min<int>(int(u * 3), 2);
+    float dist = -std::log(1 - v) / st[ch];
+    float tmax = float((b - a).length());
+    V3f dir = (b - a).normalize();
+    bool scat = dist < tmax;
+    RGB<float> tr;
+    for (int i = 0; i < 3; ++i) tr[i] = std::exp(-st[i] * std::min(dist, tmax));
+    RGB<float> density = scat ? (quirks ? ss * tr : st * tr) : tr;
+    float pdf = (density.r + density.g + density.b) / 3;
+    if (quirks) pdf = std::max(pdf, kEps);
+    r.w = tr / pdf;
+    if (scat) r.w *= ss;
+    if (scat) { r.scattered = true; r.pos = a + dir * dist; }
+    return r;
+}
+
+// ----------------------------------------------------------------------------
+// BSDF dispatch (float, reference precision)
+struct SurfBSDF {
+    int type = 0;
+    std::unique_ptr<DisneyRef<float>> dis;
+    std::unique_ptr<LambertRef<float>> lam;
+    RGB<float> eval(const V3f& wi, const V3f& wo) const { return type ? dis->eval(wi, wo) : lam->eval(wi, wo); }
+    float pdf(const V3f& wi, const V3f& wo) const { return type ? dis->pdf(wi, wo) : lam->pdf(wi, wo); }
+    BSDFSample<float> sample(const V3f& wo, float a, float b, float c) const { return type ? dis->sample(wo, a, b, c) : lam->sample(wo, a, b, c); }
+};
+static DisneyParams to_params(const tr_material& m) {
+    DisneyParams p;
+    for (int i = 0; i < 3; ++i) { p.base_color[i] = m.base_color[i]; p.specular_scale[i] = m.specular_scale[i]; }
+    p.metallic = m.metallic; p.roughness = m.roughness; p.transmission = m.transmission;
+    p.transmission_roughness = m.transmission_roughness; p.ior = m.ior; p.specular_tint = m.specular_tint;
+    p.anisotropic = m.anisotropic; p.sheen = m.sheen; p.sheen_tint = m.sheen_tint; p.clearcoat = m.clearcoat;
+    p.clearcoat_gloss = m.clearcoat_gloss;
+    return p;
+}
+static void make_bsdf(const Scene& s, const Isect& is, bool quirks, SurfBSDF& b) {
+    const tr_material& m = s.mats[s.mat[is.prim]];
+    b.type = m.type;
+    if (m.type == TR_MAT_DISNEY) b.dis.reset(new DisneyRef<float>(is.geo, is.shd, to_params(m), !quirks));
+    else b.lam.reset(new LambertRef<float>(is.geo, is.shd, RGB<float>(m.base_color[0], m.base_color[1], m.base_color[2]), !quirks));
+}
+
+// ----------------------------------------------------------------------------
+struct RenderCfg {
+    int W, H, spp, min_depth, max_depth, nls;
+    int filter; float fr, fa;
+    uint64_t seed;
+    int sample_offset;
+    bool quirks;
+    int nee;  // 0 all lights, 1 one by power
+};
+
+struct Renderer {
+    const Scene& s;
+    const RenderCfg& c;
+    std::atomic<uint64_t> ext_rays{0}, shadow_rays{0};
+    Renderer(const Scene& sc, const RenderCfg& cf) : s(sc), c(cf) {}
+
+    const tr_medium* med(int id) const { return id < 0 ? nullptr : &s.media[id]; }
+    int wo_medium_id(const Isect& is) const {   // intersection.hpp:36-41
+        return dot(is.wo, is.geo.z) >= 0 ? s.med_out[is.prim] : s.med_in[is.prim];
+    }
+    int medium_id(const Isect& is, const V3f& d) const { return dot(d, is.geo.z) >= 0 ? s.med_out[is.prim] : s.med_in[is.prim]; }
+
+    bool intersect(const V3f& o, const V3f& d, float tmin, float tmax, Isect* is) {
+        ext_rays++;
+        return intersect_ref(s, o, d, tmin, tmax, is);
+    }
+    bool occluded(const V3f& o, const V3f& d, float tmin, float tmax) {
+        shadow_rays++;
+        return occluded_ref(s, o, d, tmin, tmax);
+    }
+
+    // One light strategy evaluated at a shading point.  `li` indexes the light
+    // list: [0, nA) area lights, nA = env, nA+1 = sun.  Returns radiance
+    // contribution without the light-selection probability.
+    template <class EvalF, class PdfF>
+    RGB<float> sample_one_light(int li, const V3f& p, const V3f& ng, bool surface, int medium_towards_light_id_surface,
+                                const tr_medium* cur_med, EvalF&& evalf, PdfF&& pdff, float sel_pdf, Sampler& sp) {
+        const int nA = int(s.lights.size());
+        float u0 = sp.get(), u1 = sp.get(); sp.get(); sp.get(); sp.get();   // Sample5
+        if (li < nA) {
+            const int tri = s.lights[li];
+            LightSample ls = sample_area(s, tri, p, u0, u1);
+            if (!ls.valid || ls.L.is_black() || !ls.pdf) return RGB<float>(0);
+            ls.pdf *= sel_pdf;
+            V3f to = ls.pos - p;
+            float dist = float(to.length()) - kEps;
+            if (dist <= kEps) return RGB<float>(0);
+            V3f dir = to.normalize();
+            if (surface) {
+                if (occluded(p, dir, kEps, dist)) return RGB<float>(0);
+            } else {
+                // scene.visible(light_pos, p): Ray(p, dir, eps, |p-q| - eps)
+                float dd = float((ls.pos - p).length());
+                if (occluded(p, dir, kEps, dd - kEps)) return RGB<float>(0);
+            }
+            RGB<float> f = evalf(dir);
+            if (f.is_black()) return RGB<float>(0);
+            float bp = pdff(dir);
+            const tr_medium* m = surface ? med(medium_towards_light_id_surface) : cur_med;
+            RGB<float> tr = medium_tr(m, ls.pos, p);
+            RGB<float> val = tr * ls.L * f;
+            if (surface) val *= std::abs(cosv(dir, ng));
+            float w = power_heuristic(ls.pdf, bp);
+            return val * (w / ls.pdf);
+        }
+        if (li == nA) {   // environment
+            if (!surface && c.quirks) return RGB<float>(0);   // Q6
+            LightSample ls = sample_env(s, p, u0, u1);
+            if (!ls.valid || ls.L.is_black()) return RGB<float>(0);
+            ls.pdf *= sel_pdf;
+            V3f to = ls.pos - p;
+            float dist = float(to.length()) - kEps;
+            if (dist <= kEps) return RGB<float>(0);
+            V3f dir = to.normalize();
+            if (occluded(p, dir, kEps, dist)) return RGB<float>(0);
+            RGB<float> f = evalf(dir);
+            if (f.is_black()) return RGB<float>(0);
+            float bp = pdff(dir);
+            RGB<float> val = ls.L * f;
+            if (surface) val *= std::abs(cosv(dir, ng));
+            return val * (power_heuristic(ls.pdf, bp) / ls.pdf);
+        }
+        // sun (delta light, no MIS)
+        V3f dir = s.sun_dir.normalize();
+        if (occluded(p, dir, kEps, 2 * s.wradius)) return RGB<float>(0);
+        RGB<float> f = evalf(dir);
+        if (f.is_black()) return RGB<float>(0);
+        RGB<float> val = s.sun_E * f;
+        if (surface) val *= std::abs(cosv(dir, ng));
+        return val / sel_pdf;
+    }
+
+    float light_select_pdf(int li) const { return c.nee == 1 ? s.lpmf[li] : 1.0f; }
+
+    // Direct lighting at a vertex: all lights, or one by power.
+    template <class EvalF, class PdfF>
+    RGB<float> direct(const V3f& p, const V3f& ng, bool surface, const Isect* is, const tr_medium* cur_med,
+                      EvalF&& evalf, PdfF&& pdff, Sampler& sp) {
+        RGB<float> acc;
+        if (c.nee == 0) {
+            const int nA = int(s.lights.size());
+            for (int li = 0; li < nA; ++li)
+                acc += sample_light_generic(li, p, ng, surface, is, cur_med, evalf, pdff, 1.0f, sp);
+            if (s.has_env) acc += sample_light_generic(nA, p, ng, surface, is, cur_med, evalf, pdff, 1.0f, sp);
+            if (s.has_sun) acc += sample_light_generic(nA + 1, p, ng, surface, is, cur_med, evalf, pdff, 1.0f, sp);
+        } else if (s.n_light_entries > 0) {
+            float u = sp.get();
+            int li = int(std::upper_bound(s.lcdf.begin(), s.lcdf.end(), u) - s.lcdf.begin());
+            li = std::min(li, s.n_light_entries - 1);
+            while (li > 0 && s.lpmf[li] == 0) --li;
+            acc += sample_light_generic(li, p, ng, surface, is, cur_med, evalf, pdff, s.lpmf[li], sp);
+        }
+        return acc;
+    }
+    template <class EvalF, class PdfF>
+    RGB<float> sample_light_generic(int li, const V3f& p, const V3f& ng, bool surface, const Isect* is, const tr_medium* cur_med,
+                                    EvalF&& evalf, PdfF&& pdff, float sel, Sampler& sp) {
+        // Surface + area light: the medium between the point and the light is
+        // the one on the light's side of the surface (direct_illumination.cpp:55),
+        // known only once the light point is sampled.
+        if (surface && li < int(s.lights.size())) {
+            float u0 = sp.get(), u1 = sp.get(); sp.get(); sp.get(); sp.get();
+            const int tri = s.lights[li];
+            LightSample ls = sample_area(s, tri, p, u0, u1);
+            if (!ls.valid || ls.L.is_black() || !ls.pdf) return RGB<float>(0);
+            ls.pdf *= sel;
+            V3f to = ls.pos - p;
+            float dist = float(to.length()) - kEps;
+            if (dist <= kEps) return RGB<float>(0);
+            V3f dir = to.normalize();
+            if (occluded(p, dir, kEps, dist)) return RGB<float>(0);
+            RGB<float> f = evalf(dir);
+            if (f.is_black()) return RGB<float>(0);
+            float bp = pdff(dir);
+            RGB<float> tr = medium_tr(med(medium_id(*is, dir)), ls.pos, p);
+            RGB<float> val = tr * ls.L * f * std::abs(cosv(dir, ng));
+            return val * (power_heuristic(ls.pdf, bp) / ls.pdf);
+        }
+        return sample_one_light(li, p, ng, surface, -1, cur_med, evalf, pdff, sel, sp);
+    }
+
+    // light pdf of the strategy that would have sampled `dir` towards a hit on
+    // area light triangle `tri` / the environment (including selection).
+    float area_light_pdf(int tri, const V3f& ref, const V3f& pos) const {
+        float p = pdf_area(s, tri, ref, pos);
+        if (c.nee == 1) {
+            int li = light_index_of_tri(tri);
+            p *= li >= 0 ? s.lpmf[li] : 0.0f;
+        }
+        return p;
+    }
+    float env_light_pdf(const V3f& dir) const {
+        float p = pdf_env(dir);
+        if (c.nee == 1) p *= s.lpmf[s.lights.size()];
+        return p;
+    }
+    std::vector<int> tri_to_light;
+    int light_index_of_tri(int tri) const { return tri_to_light[tri]; }
+
+    // direct_illumination.cpp:144-198 (surface) / :200-255 (medium)
+    template <class SampleF>
+    RGB<float> sample_bsdf_mis(const V3f& origin, const V3f& ng, bool surface, const Isect* is, const tr_medium* cur_med,
+                               SampleF&& samplef, Sampler& sp) {
+        float a = sp.get(), b = sp.get(), cc = sp.get();
+        BSDFSample<float> bs = samplef(a, b, cc);
+        if (!bs.valid()) return RGB<float>(0);
+        V3f wi = bs.wi.normalize();
+        V3f o = surface ? is->eps_offset(wi) : origin;
+        Isect t;
+        bool hit = intersect(o, wi, 0, std::numeric_limits<float>::max(), &t);
+        if (!hit) {
+            if (!s.has_env) return RGB<float>(0);
+            RGB<float> Le = s.env;
+            RGB<float> f = Le * bs.f;
+            if (surface) f *= std::abs(cosv(ng, wi));
+            float lp = env_light_pdf(wi);
+            return f * (power_heuristic(bs.pdf, lp) / bs.pdf);
+        }
+        RGB<float> Le = s.tri_emission(t.prim);
+        if (Le.is_black()) return RGB<float>(0);
+        if (!(dot(t.geo.z, t.wo) > 0)) return RGB<float>(0);   // one-sided light_emit
+        RGB<float> tr;
+        if (surface) {
+            const tr_medium* m = med(medium_id(*is, bs.wi));
+            tr = c.quirks ? medium_tr(m, o, is->pos) : medium_tr(m, o, t.pos);   // Q5
+        } else {
+            tr = medium_tr(cur_med, o, t.pos);
+        }
+        RGB<float> f = tr * Le * bs.f;
+        if (surface) f *= std::abs(cosv(ng, wi));
+        float lp = area_light_pdf(t.prim, o, t.pos);
+        return f * (power_heuristic(bs.pdf, lp) / bs.pdf);
+    }
+
+    // path_trace_renderer.cpp:27-210
+    RGB<float> Li(V3f ro, V3f rd, Sampler& sp) {
+        RGB<float> coef(1), L(0);
+        const V3f cam_o = ro;
+        V3f o = ro, d = rd.normalize();
+        bool specular = false;
+        int scattering = 0;
+        for (int depth = 0; depth < c.max_depth; ++depth) {
+            if (depth > c.min_depth) {
+                if (sp.get() > 0.9f) break;
+                coef = coef / 0.9f;
+            }
+            if (!coef.is_finite()) break;
+            Isect is;
+            bool found = intersect(o, d, 0, std::numeric_limits<float>::max(), &is);
+            if (depth == 0 || specular) {
+                if (found) {
+                    RGB<float> Le = s.tri_emission(is.prim);
+                    if (!Le.is_black() && dot(is.geo.z, -d) > 0) L += coef * Le;
+                } else if (s.has_env) {
+                    L += coef * s.env;
+                }
+            }
+            if (!found) break;
+            const int mid = wo_medium_id(is);
+            const tr_medium* m = med(mid);
+            const int max_sc = m ? m->max_scattering_count : std::numeric_limits<int>::max();
+            if (scattering < max_sc) {
+                MediumSample ms = medium_sample(m, c.quirks ? cam_o : o, is.pos, sp, c.quirks);   // Q1
+                coef *= ms.w;
+                if (ms.scattered) {
+                    scattering++;
+                    HGRef<float> hg{std::clamp<float>(m->g, -1 + kEps, 1 - kEps)};
+                    const V3f wo = -((is.pos - (c.quirks ? cam_o : o)).normalize());
+                    auto evalf = [&](const V3f& wi) { return RGB<float>(hg.eval(wi, wo)); };
+                    auto pdff = [&](const V3f& wi) { return hg.eval(wi, wo); };
+                    auto samplef = [&](float a, float b, float) { return hg.sample(wo, a, b); };
+                    RGB<float> di;
+                    for (int i = 0; i < c.nls; ++i) {
+                        di += coef * direct(ms.pos, V3f(0, 0, 1), false, nullptr, m, evalf, pdff, sp);
+                        di += coef * sample_bsdf_mis(ms.pos, V3f(0, 0, 1), false, nullptr, m, samplef, sp);
+                    }
+                    L += di * (1.0f / c.nls);
+                    float a = sp.get(), b = sp.get(); sp.get();
+                    BSDFSample<float> ps = hg.sample(wo, a, b);
+                    if (!ps.valid()) break;
+                    o = ms.pos;
+                    d = ps.wi.normalize();
+                    coef *= ps.f / ps.pdf;
+                    specular = false;
+                    continue;
+                }
+            } else {
+                coef *= medium_tr(m, o, is.pos);
+                scattering = 0;
+            }
+            SurfBSDF bsdf;
+            make_bsdf(s, is, c.quirks, bsdf);
+            float a = sp.get(), b = sp.get(), cc = sp.get();
+            BSDFSample<float> bs = bsdf.sample(is.wo, a, b, cc);
+            if (bs.f.is_black() || bs.pdf < kEps) break;
+            specular = false;   // Disney/Lambert are never delta
+            auto evalf = [&](const V3f& wi) { return bsdf.eval(wi, is.wo); };
+            auto pdff = [&](const V3f& wi) { return bsdf.pdf(wi, is.wo); };
+            auto samplef = [&](float x, float y, float z) { return bsdf.sample(is.wo, x, y, z); };
+            RGB<float> di;
+            for (int i = 0; i < c.nls; ++i) {
+                di += coef * direct(is.pos, is.geo.z, true, &is, nullptr, evalf, pdff, sp);
+                di += coef * sample_bsdf_mis(is.pos, is.geo.z, true, &is, nullptr, samplef, sp);
+            }
+            L += di * (1.0f / c.nls);
+            coef *= bs.f * (std::abs(cosv(is.geo.z, bs.wi)) / bs.pdf);
+            if (!coef.is_finite()) break;
+            V3f nd = bs.wi.normalize();
+            o = is.eps_offset(bs.wi);
+            d = nd;
+        }
+        if (!L.is_finite()) return RGB<float>(0);
+        return L;
+    }
+
+    // thin_lens_camera.cpp:40-59 with look_at (transform.cpp:96-124)
+    void camera_ray(float fx, float fy, float lu, float lv, V3f& o, V3f& d) const {
+        const tr_camera& cm = s.cam;
+        V3f pos(cm.pos[0], cm.pos[1], cm.pos[2]), tgt(cm.target[0], cm.target[1], cm.target[2]), up(cm.up[0], cm.up[1], cm.up[2]);
+        V3f dir = (tgt - pos).normalize();
+        V3f right = cross(up.normalize(), dir).normalize();
+        V3f nup = cross(dir, right);
+        float Hf = 2 * cm.focal_distance * std::tan(cm.fov_y * 0.5f);
+        float Wf = cm.aspect * Hf;
+        V3f film((0.5f - fx) * Wf, (0.5f - fy) * Hf, cm.focal_distance);
+        float dx, dy;
+        concentric_sample_disk(lu, lv, dx, dy);
+        V3f lens(cm.lens_radius * dx, cm.lens_radius * dy, 0);
+        V3f ld = (film - lens).normalize();
+        o = pos + right * lens.x + nup * lens.y;
+        d = right * ld.x + nup * ld.y + dir * ld.z;
+    }
+
+    float filter_w(float dx, float dy) const {
+        if (c.filter == TR_FILTER_BOX) return (std::abs(dx) <= c.fr && std::abs(dy) <= c.fr) ? 1.0f : 0.0f;
+        float e = std::exp(-c.fa * c.fr * c.fr);
+        auto g = [&](float x) { return std::max(0.0f, std::exp(-c.fa * x * x) - e); };
+        return g(dx) * g(dy);
+    }
+
+    void render(double* sums, int threads) {
+        const int W = c.W, H = c.H;
+        const int TS = 16;
+        const int tx = (W + TS - 1) / TS, ty = (H + TS - 1) / TS;
+        std::atomic<int> next{0};
+        std::mutex mtx;
+        auto worker = [&]() {
+            std::vector<double> tile;
+            for (;;) {
+                int t = next++;
+                if (t >= tx * ty) break;
+                int x0 = (t % tx) * TS, y0 = (t / tx) * TS;
+                int x1 = std::min(W, x0 + TS), y1 = std::min(H, y0 + TS);
+                // tile padded by the filter radius (render.hpp:79-88)
+                int px0 = std::max(0, x0 - 1), py0 = std::max(0, y0 - 1);
+                int px1 = std::min(W, x1 + 1), py1 = std::min(H, y1 + 1);
+                int tw = px1 - px0;
+                tile.assign(size_t(tw) * (py1 - py0) * 4, 0.0);
+                for (int y = y0; y < y1; ++y)
+                    for (int x = x0; x < x1; ++x) {
+                        const uint64_t pix = uint64_t(y) * W + x;
+                        for (int i = 0; i < c.spp; ++i) {
+                            const uint64_t sidx = uint64_t(c.sample_offset) + i;
+                            Sampler sp;
+                            sp.rng.seed(mix64(c.seed ^ mix64(pix * 0x9E3779B97F4A7C15ULL + sidx)), pix);
+                            float fu = sp.get(), fv = sp.get(), lu = sp.get(), lv = sp.get();
+                            float pxf = x + fu, pyf = y + fv;
+                            V3f o, d;
+                            camera_ray(pxf / W, pyf / H, lu, lv, o, d);
+                            RGB<float> L = Li(o, d, sp);
+                            if (!L.is_finite()) continue;
+                            if (c.filter == TR_FILTER_BOX && c.fr <= 0.5f) {
+                                double* tp = &tile[(size_t(y - py0) * tw + (x - px0)) * 4];
+                                tp[0] += L.r; tp[1] += L.g; tp[2] += L.b; tp[3] += 1.0;
+                                continue;
+                            }
+                            float dxp = pxf - 0.5f, dyp = pyf - 0.5f;
+                            int qx0 = std::max(px0, int(std::ceil(dxp - c.fr))), qx1 = std::min(px1, int(std::floor(dxp + c.fr)) + 1);
+                            int qy0 = std::max(py0, int(std::ceil(dyp - c.fr))), qy1 = std::min(py1, int(std::floor(dyp + c.fr)) + 1);
+                            for (int qy = qy0; qy < qy1; ++qy)
+                                for (int qx = qx0; qx < qx1; ++qx) {
+                                    float w = filter_w(pxf - (qx + 0.5f), pyf - (qy + 0.5f));
+                                    double* tp = &tile[(size_t(qy - py0) * tw + (qx - px0)) * 4];
+                                    tp[0] += double(L.r) * w; tp[1] += double(L.g) * w; tp[2] += double(L.b) * w; tp[3] += w;
+                                }
+                        }
+                    }
+                std::lock_guard<std::mutex> lk(mtx);   // render.hpp:90-100
+                for (int y = py0; y < py1; ++y)
+                    for (int x = px0; x < px1; ++x) {
+                        const double* tp = &tile[(size_t(y - py0) * tw + (x - px0)) * 4];
+                        double* fp = &sums[(size_t(y) * W + x) * 4];
+                        for (int k = 0; k < 4; ++k) fp[k] += tp[k];
+                    }
+            }
+        };
+        std::vector<std::thread> pool;
+        for (int i = 0; i < threads; ++i) pool.emplace_back(worker);
+        for (auto& t : pool) t.join();
+    }
+};
+
+static Scene* build_scene(const tr_scene_desc* d, int max_leaf) {
+    Scene* s = new Scene();
+    const int64_t nv = d->num_vertices, nt = d->num_triangles;
+    s->P.resize(nv);
+    for (int64_t i = 0; i < nv; ++i) s->P[i] = V3f(d->positions[3 * i], d->positions[3 * i + 1], d->positions[3 * i + 2]);
+    if (d->normals) {
+        s->N.resize(nv);
+        for (int64_t i = 0; i < nv; ++i) s->N[i] = V3f(d->normals[3 * i], d->normals[3 * i + 1], d->normals[3 * i + 2]);
+    }
+    if (d->uvs) s->UV.assign(d->uvs, d->uvs + 2 * nv);
+    s->idx.assign(d->indices, d->indices + 3 * nt);
+    s->mat.assign(d->material_ids, d->material_ids + nt);
+    if (d->emission) s->emis.assign(d->emission, d->emission + 3 * nt);
+    s->med_in.assign(nt, -1);
+    s->med_out.assign(nt, -1);
+    if (d->medium_inside) s->med_in.assign(d->medium_inside, d->medium_inside + nt);
+    if (d->medium_outside) s->med_out.assign(d->medium_outside, d->medium_outside + nt);
+    s->mats.assign(d->materials, d->materials + d->num_materials);
+    if (d->num_media) s->media.assign(d->media, d->media + d->num_media);
+    std::vector<Bounds3> pb(nt);
+    for (int64_t t = 0; t < nt; ++t) {
+        Bounds3 b;
+        b.grow(s->vtx(int(t), 0)); b.grow(s->vtx(int(t), 1)); b.grow(s->vtx(int(t), 2));
+        pb[t] = b;
+        s->world.grow(b);
+        if (!s->tri_emission(int(t)).is_black()) s->lights.push_back(int(t));
+    }
+    s->bvh.build(pb, max_leaf);
+    s->has_env = d->has_env != 0;
+    s->env = RGB<float>(d->env_radiance[0], d->env_radiance[1], d->env_radiance[2]);
+    s->has_sun = d->has_sun != 0;
+    s->sun_dir = V3f(d->sun_direction[0], d->sun_direction[1], d->sun_direction[2]);
+    s->sun_E = RGB<float>(d->sun_irradiance[0], d->sun_irradiance[1], d->sun_irradiance[2]);
+    s->cam = d->camera;
+    // Bounds3::bounding_sphere (geometry.hpp:752-755)
+    s->wcenter = (s->world.lo + s->world.hi) * 0.5f;
+    s->wradius = float((s->world.hi - s->wcenter).length());
+    // power distribution: DiffuseLight pi*L*A, IBL (L*4pi)*R^2, sun E*pi*R^2
+    const int nA = int(s->lights.size());
+    s->lpmf.assign(nA + 2, 0.0f);
+    for (int i = 0; i < nA; ++i) s->lpmf[i] = kPi<float> * s->tri_emission(s->lights[i]).lum() * s->tri_area(s->lights[i]);
+    if (s->has_env) s->lpmf[nA] = s->env.lum() * 4 * kPi<float> * s->wradius * s->wradius;
+    if (s->has_sun) s->lpmf[nA + 1] = s->sun_E.lum() * kPi<float> * s->wradius * s->wradius;
+    double tot = 0;
+    for (float p : s->lpmf) tot += p;
+    s->lcdf.assign(nA + 2, 1.0f);
+    double acc = 0;
+    for (int i = 0; i < nA + 2; ++i) {
+        s->lpmf[i] = tot > 0 ? float(s->lpmf[i] / tot) : 0.0f;
+        acc += s->lpmf[i];
+        s->lcdf[i] = float(acc);
+    }
+    s->n_light_entries = nA + 2;
+    return s;
+}
+
+}  // namespace refcpu
+
+using namespace refcpu;
+
+extern "C" {
+
+// ---- scene -------------------------------------------------------------
+void* ref_scene_create(const tr_scene_desc* d, int max_leaf) { return build_scene(d, max_leaf); }
+void ref_scene_destroy(void* h) { delete static_cast<Scene*>(h); }
+int64_t ref_bvh_node_count(void* h) { return int64_t(static_cast<Scene*>(h)->bvh.nodes.size()); }
+// Copies the flattened 32-byte nodes and the reordered primitive indices.
+void ref_bvh_export(void* h, void* nodes_out, int32_t* prims_out) {
+    Scene* s = static_cast<Scene*>(h);
+    std::memcpy(nodes_out, s->bvh.nodes.data(), s->bvh.nodes.size() * sizeof(RefNode));
+    std::memcpy(prims_out, s->bvh.prims.data(), s->bvh.prims.size() * sizeof(int));
+}
+void ref_light_pmf(void* h, float* pmf_out) {
+    Scene* s = static_cast<Scene*>(h);
+    std::memcpy(pmf_out, s->lpmf.data(), s->lpmf.size() * sizeof(float));
+}
+
+// ---- ray queries --------------------------------------------------------
+// rays: n*8 (ox,oy,oz,tmin,dx,dy,dz,tmax); hits: n*4 (t, prim bits, b1, b2)
+static void par_for(int64_t n, int threads, const std::function<void(int64_t, int64_t)>& fn) {
+    if (threads <= 1 || n < 1024) { fn(0, n); return; }
+    std::vector<std::thread> pool;
+    int64_t chunk = (n + threads - 1) / threads;
+    for (int t = 0; t < threads; ++t) {
+        int64_t a = t * chunk, b = std::min(n, a + chunk);
+        if (a < b) pool.emplace_back(fn, a, b);
+    }
+    for (auto& th : pool) th.join();
+}
+static inline void put_hit(float* h, float t, int prim, float b1, float b2) {
+    h[0] = t; std::memcpy(&h[1], &prim, 4); h[2] = b1; h[3] = b2;
+}
+
+// Reference semantics: BVHAccel::intersect_p + Moller-Trumbore (normalised d).
+void ref_trace_closest_mt(void* hs, const float* rays, float* hits, int64_t n, int threads) {
+    Scene* s = static_cast<Scene*>(hs);
+    par_for(n, threads, [&](int64_t a, int64_t b) {
+        for (int64_t i = a; i < b; ++i) {
+            const float* r = rays + 8 * i;
+            V3f o(r[0], r[1], r[2]), d = V3f(r[4], r[5], r[6]).normalize();
+            float tm = r[7];
+            int best = -1; MTHit bh{0, 0, 0};
+            s->bvh.traverse(o, d, tm, [&](int p, float& tcur) {
+                MTHit h;
+                if (!mt_intersect(o, d, r[3], tcur, s->vtx(p, 0), s->vtx(p, 1), s->vtx(p, 2), &h)) return false;
+                tcur = h.t; best = p; bh = h;
+                return true;
+            }, false);
+            put_hit(hits + 4 * i, best >= 0 ? bh.t : 0.0f, best, bh.alpha, bh.beta);
+        }
+    });
+}
+
+// Brute force over all triangles with the watertight test; lowest id wins ties.
+void ref_trace_closest_wt_brute(void* hs, const float* rays, float* hits, int64_t n, int threads) {
+    Scene* s = static_cast<Scene*>(hs);
+    const int nt = int(s->mat.size());
+    par_for(n, threads, [&](int64_t a, int64_t b) {
+        for (int64_t i = a; i < b; ++i) {
+            const float* r = rays + 8 * i;
+            V3f o(r[0], r[1], r[2]), d(r[4], r[5], r[6]);
+            WTRay wr = wt_setup(o, d);
+            float best_t = r[7]; int best = -1; WTHit bh{0, 0, 0};
+            for (int p = 0; p < nt; ++p) {
+                WTHit h;
+                if (!wt_intersect(wr, r[3], best_t, s->vtx(p, 0), s->vtx(p, 1), s->vtx(p, 2), &h)) continue;
+                if (h.t < best_t || best < 0 || p < best) { best_t = h.t; best = p; bh = h; }
+            }
+            put_hit(hits + 4 * i, best >= 0 ? bh.t : 0.0f, best, bh.b1, bh.b2);
+        }
+    });
+}
+
+// Same result as the brute force, through the reference BVH with a
+// conservative slab test (Ize, "Robust BVH Ray Traversal", JCGT 2013:
+// interval widened by 1 +/- 2*gamma(3)) so no box holding a hit is culled.
+static inline bool slab_conservative(const Bounds3& b, const V3f& o, const V3f& inv, float tmin, float tmax, float& tnear) {
+    const float g = 2.0f * (3 * 5.96046448e-08f) / (1 - 3 * 5.96046448e-08f);
+    float t0 = tmin, t1 = tmax;
+    for (int k = 0; k < 3; ++k) {
+        float tn = (b.lo[k] - o[k]) * inv[k];
+        float tf = (b.hi[k] - o[k]) * inv[k];
+        if (tn > tf) std::swap(tn, tf);
+        tn *= 1 - g;
+        tf *= 1 + g;
+        t0 = tn > t0 ? tn : t0;
+        t1 = tf < t1 ? tf : t1;
+        if (t0 > t1) return false;
+    }
+    tnear = t0;
+    return true;
+}
+static inline V3f safe_inv(const V3f& d) {
+    auto f = [](float x) { float ax = std::fabs(x) < 1e-30f ? std::copysign(1e-30f, x) : x; return 1.0f / ax; };
+    return V3f(f(d.x), f(d.y), f(d.z));
+}
+void ref_trace_closest_wt_bvh(void* hs, const float* rays, float* hits, int64_t n, int threads) {
+    Scene* s = static_cast<Scene*>(hs);
+    par_for(n, threads, [&](int64_t a, int64_t b) {
+        std::vector<int> stack;
+        for (int64_t i = a; i < b; ++i) {
+            const float* r = rays + 8 * i;
+            V3f o(r[0], r[1], r[2]), d(r[4], r[5], r[6]);
+            WTRay wr = wt_setup(o, d);
+            V3f inv = safe_inv(d);
+            float best_t = r[7]; int best = -1; WTHit bh{0, 0, 0};
+            stack.clear();
+            if (!s->bvh.nodes.empty()) stack.push_back(0);
+            while (!stack.empty()) {
+                int ni = stack.back(); stack.pop_back();
+                const RefNode& nd = s->bvh.nodes[ni];
+                float tn;
+                if (!slab_conservative(nd.b, o, inv, r[3], best_t, tn)) continue;
+                if (nd.count > 0) {
+                    for (int k = 0; k < nd.count; ++k) {
+                        int p = s->bvh.prims[nd.offset + k];
+                        WTHit h;
+                        if (!wt_intersect(wr, r[3], best_t, s->vtx(p, 0), s->vtx(p, 1), s->vtx(p, 2), &h)) continue;
+                        if (h.t < best_t || best < 0 || p < best) { best_t = h.t; best = p; bh = h; }
+                    }
+                } else {
+                    stack.push_back(nd.offset);
+                    stack.push_back(ni + 1);
+                }
+            }
+            put_hit(hits + 4 * i, best >= 0 ? bh.t : 0.0f, best, bh.b1, bh.b2);
+        }
+    });
+}
+void ref_trace_any_wt(void* hs, const float* rays, int32_t* occ, int64_t n, int threads) {
+    Scene* s = static_cast<Scene*>(hs);
+    const int nt = int(s->mat.size());
+    par_for(n, threads, [&](int64_t a, int64_t b) {
+        std::vector<int> stack;
+        for (int64_t i = a; i < b; ++i) {
+            const float* r = rays + 8 * i;
+            V3f o(r[0], r[1], r[2]), d(r[4], r[5], r[6]);
+            WTRay wr = wt_setup(o, d);
+            V3f inv = safe_inv(d);
+            int hit = 0;
+            stack.clear();
+            if (!s->bvh.nodes.empty()) stack.push_back(0);
+            while (!stack.empty() && !hit) {
+                int ni = stack.back(); stack.pop_back();
+                const RefNode& nd = s->bvh.nodes[ni];
+                float tn;
+                if (!slab_conservative(nd.b, o, inv, r[3], r[7], tn)) continue;
+                if (nd.count > 0) {
+                    for (int k = 0; k < nd.count && !hit; ++k) {
+                        int p = s->bvh.prims[nd.offset + k];
+                        if (wt_intersect(wr, r[3], r[7], s->vtx(p, 0), s->vtx(p, 1), s->vtx(p, 2), nullptr)) hit = 1;
+                    }
+                } else {
+                    stack.push_back(nd.offset);
+                    stack.push_back(ni + 1);
+                }
+            }
+            occ[i] = hit;
+            (void)nt;
+        }
+    });
+}
+// Single-triangle tests, for known-answer fixtures.
+int ref_mt_single(const float* ray, const float* tri, float* out) {
+    MTHit h;
+    V3f o(ray[0], ray[1], ray[2]), d = V3f(ray[4], ray[5], ray[6]).normalize();
+    if (!mt_intersect(o, d, ray[3], ray[7], V3f(tri[0], tri[1], tri[2]), V3f(tri[3], tri[4], tri[5]), V3f(tri[6], tri[7], tri[8]), &h)) return 0;
+    out[0] = h.t; out[1] = h.alpha; out[2] = h.beta;
+    return 1;
+}
+int ref_wt_single(const float* ray, const float* tri, float* out) {
+    WTHit h;
+    WTRay wr = wt_setup(V3f(ray[0], ray[1], ray[2]), V3f(ray[4], ray[5], ray[6]));
+    if (!wt_intersect(wr, ray[3], ray[7], V3f(tri[0], tri[1], tri[2]), V3f(tri[3], tri[4], tri[5]), V3f(tri[6], tri[7], tri[8]), &h)) return 0;
+    out[0] = h.t; out[1] = h.b1; out[2] = h.b2;
+    return 1;
+}
+
+// ---- BSDF batches ---------------------------------------------------------
+// Vectors in the shading frame (shading normal +z); ng per entry or NULL.
+// precision: 0 float (reference), 1 double (FP64 twin).  fixed: quirk switch.
+void ref_bsdf_eval(const tr_material* mats, const int32_t* ids, const float* wi, const float* wo, const float* ng,
+                   int64_t n, int precision, int fixed, double* f, double* pdf) {
+    for (int64_t i = 0; i < n; ++i) {
+        const tr_material& m = mats[ids[i]];
+        auto run = [&](auto zero) {
+            using T = decltype(zero);
+            V3<T> vi(wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]), vo(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]);
+            V3<T> g = ng ? V3<T>(ng[3 * i], ng[3 * i + 1], ng[3 * i + 2]) : V3<T>(0, 0, 1);
+            Coord<T> shd;
+            Coord<T> geo = Coord<T>::from_z(g);
+            RGB<T> fv; T pv;
+            if (m.type == TR_MAT_DISNEY) {
+                DisneyRef<T> b(geo, shd, to_params(m), fixed != 0);
+                fv = b.eval(vi, vo); pv = b.pdf(vi, vo);
+            } else {
+                LambertRef<T> b(geo, shd, RGB<T>(m.base_color[0], m.base_color[1], m.base_color[2]), fixed != 0);
+                fv = b.eval(vi, vo); pv = b.pdf(vi, vo);
+            }
+            f[3 * i] = fv.r; f[3 * i + 1] = fv.g; f[3 * i + 2] = fv.b; pdf[i] = pv;
+        };
+        if (precision) run(double(0)); else run(float(0));
+    }
+}
+void ref_bsdf_sample(const tr_material* mats, const int32_t* ids, const float* wo, const float* ng, const float* u,
+                     int64_t n, int precision, int fixed, double* wi, double* f, double* pdf) {
+    for (int64_t i = 0; i < n; ++i) {
+        const tr_material& m = mats[ids[i]];
+        auto run = [&](auto zero) {
+            using T = decltype(zero);
+            V3<T> vo(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]);
+            V3<T> g = ng ? V3<T>(ng[3 * i], ng[3 * i + 1], ng[3 * i + 2]) : V3<T>(0, 0, 1);
+            Coord<T> shd;
+            Coord<T> geo = Coord<T>::from_z(g);
+            BSDFSample<T> s;
+            if (m.type == TR_MAT_DISNEY) {
+                DisneyRef<T> b(geo, shd, to_params(m), fixed != 0);
+                s = b.sample(vo, T(u[3 * i]), T(u[3 * i + 1]), T(u[3 * i + 2]));
+            } else {
+                LambertRef<T> b(geo, shd, RGB<T>(m.base_color[0], m.base_color[1], m.base_color[2]), fixed != 0);
+                s = b.sample(vo, T(u[3 * i]), T(u[3 * i + 1]), T(u[3 * i + 2]));
+            }
+            wi[3 * i] = s.wi.x; wi[3 * i + 1] = s.wi.y; wi[3 * i + 2] = s.wi.z;
+            f[3 * i] = s.f.r; f[3 * i + 1] = s.f.g; f[3 * i + 2] = s.f.b;
+            pdf[i] = s.valid() ? s.pdf : 0;
+        };
+        if (precision) run(double(0)); else run(float(0));
+    }
+}
+// HG phase: value for cos between wi and wo, and sampling.
+double ref_hg_eval(double g, double cos_wi_wo) { HGRef<double> h{g}; return h.phase(-cos_wi_wo); }
+void ref_hg_sample(float g, const float* wo, const float* u, int64_t n, float* wi, float* pdf) {
+    HGRef<float> h{g};
+    for (int64_t i = 0; i < n; ++i) {
+        BSDFSample<float> s = h.sample(V3f(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]), u[2 * i], u[2 * i + 1]);
+        wi[3 * i] = s.wi.x; wi[3 * i + 1] = s.wi.y; wi[3 * i + 2] = s.wi.z; pdf[i] = s.pdf;
+    }
+}
+void ref_camera_rays(void* hs, const float* film_uv, const float* lens_uv, int64_t n, float* rays) {
+    Scene* s = static_cast<Scene*>(hs);
+    RenderCfg c{};
+    Renderer r(*s, c);
+    for (int64_t i = 0; i < n; ++i) {
+        V3f o, d;
+        r.camera_ray(film_uv[2 * i], film_uv[2 * i + 1], lens_uv[2 * i], lens_uv[2 * i + 1], o, d);
+        d = d.normalize();
+        float* q = rays + 8 * i;
+        q[0] = o.x; q[1] = o.y; q[2] = o.z; q[3] = 0; q[4] = d.x; q[5] = d.y; q[6] = d.z; q[7] = std::numeric_limits<float>::max();
+    }
+}
+
+// ---- render ----------------------------------------------------------------
+// sums: W*H*4 doubles, accumulated.  Returns milliseconds of render() only;
+// rays_out (optional) gets extension and shadow ray counts.
+double ref_render(void* hs, const tr_render_params* p, int quirks, int nee, int threads, double* sums, uint64_t* rays_out) {
+    Scene* s = static_cast<Scene*>(hs);
+    RenderCfg c;
+    c.W = p->width; c.H = p->height; c.spp = p->spp; c.min_depth = p->min_depth; c.max_depth = p->max_depth;
+    c.nls = std::max(1, p->direct_light_sample_num);
+    c.filter = p->filter_type; c.fr = p->filter_radius; c.fa = p->filter_alpha;
+    c.seed = p->seed; c.sample_offset = p->sample_offset; c.quirks = quirks != 0; c.nee = nee;
+    Renderer r(*s, c);
+    r.tri_to_light.assign(s->mat.size(), -1);
+    for (size_t i = 0; i < s->lights.size(); ++i) r.tri_to_light[s->lights[i]] = int(i);
+    if (threads <= 0) threads = int(std::thread::hardware_concurrency());
+    auto t0 = std::chrono::steady_clock::now();
+    r.render(sums, threads);
+    auto t1 = std::chrono::steady_clock::now();
+    if (rays_out) { rays_out[0] = r.ext_rays.load(); rays_out[1] = r.shadow_rays.load(); }
+    return std::chrono::duration<double, std::milli>(t1 - t0).count();
+}
+
+}  // extern "C"
+
+// ---- closed-form helpers exposed for known-answer tests --------------------
+extern "C" {
+void ref_pcg32(uint64_t state, uint64_t stream, int default_stream, int64_t n, uint32_t* out) {
+    Pcg32 g;
+    if (default_stream) g.seed_default_stream(state);
+    else g.seed(state, stream);
+    for (int64_t i = 0; i < n; ++i) out[i] = g.next_u32();
+}
+double ref_fresnel(double eta_i, double eta_o, double cos_i) { return dielectric_fresnel<double>(eta_i, eta_o, cos_i); }
+// anisotropic GTR2 and GTR1 distributions of a unit half vector, double precision
+double ref_D_gtr2(double hx, double hy, double hz, double ax, double ay) {
+    V3d h(hx, hy, hz);
+    double sphi, cphi;
+    double phi = local_phi(h);
+    sphi = std::sin(phi); cphi = std::cos(phi);
+    return mf::anisotropic_gtr2(sphi, cphi, local_cos_to_sin(hz), hz, ax, ay);
+}
+double ref_D_gtr1(double hz, double alpha) { return mf::gtr1(local_cos_to_sin(hz), hz, alpha); }
+double ref_G1_gtr2(double wx, double wy, double wz, double ax, double ay) {
+    V3d w(wx, wy, wz);
+    double phi = local_phi(w);
+    return mf::smith_anisotropic_gtr2(std::cos(phi), std::sin(phi), ax, ay, local_tan_theta(w));
+}
+}

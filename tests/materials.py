@@ -1,0 +1,39 @@
+"""Seeded material sets and direction batches for the BSDF tests."""
+from __future__ import annotations
+
+import numpy as np
+
+from tracer_amd.abi import TR_MAT_DISNEY, TR_MAT_LAMBERT, make_material
+
+
+def c2_materials(seed: int, n: int):
+    """Disney parameters drawn like configuration 2, plus corner cases and a Lambert."""
+    rng = np.random.default_rng(seed)
+    mats = []
+    for _ in range(n):
+        r = float(rng.uniform(0.05, 1.0))
+        mats.append(make_material(type=TR_MAT_DISNEY, base_color=tuple(rng.uniform(0, 1, 3)),
+                                  metallic=float(rng.uniform() < 0.3), roughness=r, transmission=float(rng.uniform() < 0.1),
+                                  transmission_roughness=r, ior=1.5, specular_scale=1.0, clearcoat=float(rng.uniform()),
+                                  sheen=float(rng.uniform(0, 0.5)), sheen_tint=0.5, clearcoat_gloss=1.0))
+    mats.append(make_material(type=TR_MAT_DISNEY, base_color=(0.8, 0.3, 0.2), roughness=0.05, transmission=1.0,
+                              transmission_roughness=0.05, clearcoat=1.0, sheen=0.5))
+    mats.append(make_material(type=TR_MAT_DISNEY, base_color=(0.9, 0.9, 0.9), metallic=1.0, roughness=0.3, anisotropic=0.6))
+    mats.append(make_material(type=TR_MAT_DISNEY, base_color=(0.2, 0.6, 0.3), roughness=1.0, transmission=0.5,
+                              transmission_roughness=0.7, specular_tint=0.7, sheen=0.3, sheen_tint=0.2, clearcoat=0.4,
+                              clearcoat_gloss=0.3, ior=2.0))
+    mats.append(make_material(type=TR_MAT_LAMBERT, base_color=(0.7, 0.5, 0.3)))
+    return mats
+
+
+def sphere_dirs(rng, n):
+    z = rng.uniform(-1, 1, n)
+    phi = rng.uniform(0, 2 * np.pi, n)
+    r = np.sqrt(1 - z * z)
+    return np.stack([r * np.cos(phi), r * np.sin(phi), z], axis=1).astype(np.float32)
+
+
+def tilted_normals(rng, n, max_tilt=0.3):
+    t = rng.uniform(-max_tilt, max_tilt, (n, 2))
+    g = np.concatenate([t, np.ones((n, 1))], axis=1)
+    return (g / np.linalg.norm(g, axis=1, keepdims=True)).astype(np.float32)

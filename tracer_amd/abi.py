@@ -1,0 +1,140 @@
+"""ctypes mirror of the plain-data types in include/tracer_hip.h.
+
+Pure layout declarations (no library is loaded here); the product bindings in
+tracer_amd.native and the test harness both use them.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+TR_OK, TR_ERR_INVALID, TR_ERR_HIP, TR_ERR_OOM, TR_ERR_STATE = range(5)
+TR_MAT_LAMBERT, TR_MAT_DISNEY = 0, 1
+TR_FILTER_BOX, TR_FILTER_GAUSSIAN = 0, 1
+TR_TR_ANALYTIC, TR_TR_RATIO = 0, 1
+
+f3 = C.c_float * 3
+
+
+class TrMaterial(C.Structure):
+    _fields_ = [
+        ("type", C.c_int32),
+        ("base_color", f3),
+        ("metallic", C.c_float), ("roughness", C.c_float), ("transmission", C.c_float),
+        ("transmission_roughness", C.c_float), ("ior", C.c_float),
+        ("specular_scale", f3),
+        ("specular_tint", C.c_float), ("anisotropic", C.c_float), ("sheen", C.c_float),
+        ("sheen_tint", C.c_float), ("clearcoat", C.c_float), ("clearcoat_gloss", C.c_float),
+    ]
+
+
+class TrMedium(C.Structure):
+    _fields_ = [("sigma_a", f3), ("sigma_s", f3), ("g", C.c_float), ("max_scattering_count", C.c_int32)]
+
+
+class TrCamera(C.Structure):
+    _fields_ = [("pos", f3), ("target", f3), ("up", f3), ("fov_y", C.c_float), ("lens_radius", C.c_float),
+                ("focal_distance", C.c_float), ("aspect", C.c_float)]
+
+
+class TrSceneDesc(C.Structure):
+    _fields_ = [
+        ("num_vertices", C.c_int64),
+        ("positions", C.POINTER(C.c_float)),
+        ("normals", C.POINTER(C.c_float)),
+        ("uvs", C.POINTER(C.c_float)),
+        ("num_triangles", C.c_int64),
+        ("indices", C.POINTER(C.c_int32)),
+        ("material_ids", C.POINTER(C.c_int32)),
+        ("emission", C.POINTER(C.c_float)),
+        ("medium_inside", C.POINTER(C.c_int32)),
+        ("medium_outside", C.POINTER(C.c_int32)),
+        ("num_materials", C.c_int32),
+        ("num_media", C.c_int32),
+        ("materials", C.POINTER(TrMaterial)),
+        ("media", C.POINTER(TrMedium)),
+        ("has_env", C.c_int32),
+        ("env_radiance", f3),
+        ("has_sun", C.c_int32),
+        ("sun_direction", f3),
+        ("sun_irradiance", f3),
+        ("camera", TrCamera),
+    ]
+
+
+class TrBvhOpts(C.Structure):
+    _fields_ = [("max_leaf_primitives", C.c_int32), ("num_threads", C.c_int32)]
+
+
+class TrRenderParams(C.Structure):
+    _fields_ = [
+        ("width", C.c_int32), ("height", C.c_int32), ("spp", C.c_int32),
+        ("min_depth", C.c_int32), ("max_depth", C.c_int32), ("direct_light_sample_num", C.c_int32),
+        ("filter_type", C.c_int32), ("filter_radius", C.c_float), ("filter_alpha", C.c_float),
+        ("shadow_transmittance", C.c_int32),
+        ("seed", C.c_uint64),
+        ("sample_offset", C.c_int32), ("samples_per_pass", C.c_int32),
+        ("guide", C.c_int32), ("guide_train_passes", C.c_int32),
+        ("guide_fraction", C.c_float), ("guide_lr", C.c_float),
+        ("profile", C.c_int32), ("reserved", C.c_int32),
+    ]
+
+
+class TrRenderStats(C.Structure):
+    _fields_ = [
+        ("camera_paths", C.c_uint64), ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
+        ("nonfinite_dropped", C.c_uint64), ("guide_fallbacks", C.c_uint64), ("guide_train_samples", C.c_uint64),
+        ("ms_total", C.c_double), ("ms_extend", C.c_double), ("ms_shade", C.c_double), ("ms_shadow", C.c_double),
+        ("ms_other", C.c_double), ("ms_guide", C.c_double),
+        ("extend_launches", C.c_uint64), ("pad", C.c_uint64),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+
+
+def fptr(a: np.ndarray | None):
+    if a is None:
+        return C.POINTER(C.c_float)()
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def iptr(a: np.ndarray | None):
+    if a is None:
+        return C.POINTER(C.c_int32)()
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def dptr(a: np.ndarray | None):
+    if a is None:
+        return C.POINTER(C.c_double)()
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+MATERIAL_FIELDS = ["type", "base_color", "metallic", "roughness", "transmission", "transmission_roughness", "ior",
+                   "specular_scale", "specular_tint", "anisotropic", "sheen", "sheen_tint", "clearcoat",
+                   "clearcoat_gloss"]
+
+
+def make_material(**kw) -> TrMaterial:
+    m = TrMaterial()
+    m.type = kw.get("type", TR_MAT_DISNEY)
+    for k in ("base_color", "specular_scale"):
+        v = kw.get(k, (1.0, 1.0, 1.0) if k == "specular_scale" else (0.5, 0.5, 0.5))
+        if np.isscalar(v):
+            v = (v, v, v)
+        setattr(m, k, f3(*map(float, v)))
+    defaults = dict(metallic=0.0, roughness=0.5, transmission=0.0, transmission_roughness=0.5, ior=1.5,
+                    specular_tint=0.0, anisotropic=0.0, sheen=0.0, sheen_tint=0.5, clearcoat=0.0, clearcoat_gloss=1.0)
+    for k, v in defaults.items():
+        setattr(m, k, float(kw.get(k, v)))
+    return m
+
+
+def material_array(mats: list[TrMaterial]):
+    arr = (TrMaterial * len(mats))(*mats)
+    return arr

@@ -1,0 +1,270 @@
+// Binned-SAH binary build + collapse to 4-wide nodes.  See bvh_build.h.
+#include "bvh_build.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <future>
+#include <limits>
+#include <stdexcept>
+#include <thread>
+
+namespace trhip {
+namespace {
+
+constexpr float kInf = std::numeric_limits<float>::infinity();
+constexpr int kBins = 16;
+
+struct Box {
+    float lo[3] = {kInf, kInf, kInf};
+    float hi[3] = {-kInf, -kInf, -kInf};
+    void grow(const Box& b) {
+        for (int k = 0; k < 3; ++k) { lo[k] = std::min(lo[k], b.lo[k]); hi[k] = std::max(hi[k], b.hi[k]); }
+    }
+    void grow(const float* p) {
+        for (int k = 0; k < 3; ++k) { lo[k] = std::min(lo[k], p[k]); hi[k] = std::max(hi[k], p[k]); }
+    }
+    float area() const {
+        float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+        if (dx < 0 || dy < 0 || dz < 0) return 0.0f;
+        return 2.0f * (dx * dy + dx * dz + dy * dz);
+    }
+};
+
+struct PrimRef {
+    Box b;
+    float c[3];
+};
+
+struct BNode {
+    Box b;
+    int left = -1, right = -1;
+    int first = 0, count = 0;   // count > 0: leaf over ids[first, first+count)
+};
+
+struct Builder {
+    const std::vector<PrimRef>& prims;
+    std::vector<int>& ids;
+    std::vector<BNode>& nodes;
+    std::atomic<int> next_node{0};
+    int max_leaf;
+    int par_depth;
+
+    Builder(const std::vector<PrimRef>& p, std::vector<int>& i, std::vector<BNode>& n, int ml, int pd)
+        : prims(p), ids(i), nodes(n), max_leaf(ml), par_depth(pd) {}
+
+    int alloc() {
+        int id = next_node.fetch_add(1);
+        if (id >= int(nodes.size())) throw std::runtime_error("bvh: node pool exhausted");
+        return id;
+    }
+
+    void make_leaf(int id, int start, int end, const Box& b) {
+        BNode& n = nodes[id];
+        n.b = b;
+        n.first = start;
+        n.count = end - start;
+    }
+
+    void build(int id, int start, int end, int depth) {
+        Box b, cb;
+        for (int i = start; i < end; ++i) {
+            const PrimRef& p = prims[ids[i]];
+            b.grow(p.b);
+            cb.grow(p.c);
+        }
+        const int count = end - start;
+        if (count <= max_leaf) { make_leaf(id, start, end, b); return; }
+
+        // best split over three axes
+        int best_axis = -1, best_bin = -1;
+        float best_cost = kInf;
+        for (int axis = 0; axis < 3; ++axis) {
+            const float ext = cb.hi[axis] - cb.lo[axis];
+            if (!(ext > 0)) continue;
+            const float scale = kBins / ext;
+            Box bb[kBins];
+            int bc[kBins] = {0};
+            for (int i = start; i < end; ++i) {
+                const PrimRef& p = prims[ids[i]];
+                int k = int((p.c[axis] - cb.lo[axis]) * scale);
+                k = std::min(std::max(k, 0), kBins - 1);
+                bc[k]++;
+                bb[k].grow(p.b);
+            }
+            float ra[kBins];
+            int rc[kBins];
+            Box acc;
+            int n = 0;
+            for (int k = kBins - 1; k > 0; --k) {
+                acc.grow(bb[k]);
+                n += bc[k];
+                ra[k] = acc.area();
+                rc[k] = n;
+            }
+            acc = Box();
+            n = 0;
+            for (int k = 0; k < kBins - 1; ++k) {
+                acc.grow(bb[k]);
+                n += bc[k];
+                if (n == 0 || rc[k + 1] == 0) continue;
+                const float cost = acc.area() * n + ra[k + 1] * rc[k + 1];
+                if (cost < best_cost) { best_cost = cost; best_axis = axis; best_bin = k; }
+            }
+        }
+
+        int mid;
+        if (best_axis < 0) {
+            mid = start + count / 2;   // identical centroids: split by index
+        } else {
+            const float scale = kBins / (cb.hi[best_axis] - cb.lo[best_axis]);
+            const float lo = cb.lo[best_axis];
+            auto it = std::partition(ids.begin() + start, ids.begin() + end, [&](int pi) {
+                int k = int((prims[pi].c[best_axis] - lo) * scale);
+                k = std::min(std::max(k, 0), kBins - 1);
+                return k <= best_bin;
+            });
+            mid = int(it - ids.begin());
+            if (mid == start || mid == end) mid = start + count / 2;
+        }
+        const int l = alloc(), r = alloc();
+        BNode& n = nodes[id];
+        n.b = b;
+        n.left = l;
+        n.right = r;
+        n.count = 0;
+        if (depth < par_depth && count > 32768) {
+            auto fut = std::async(std::launch::async, [&, l, start, mid, depth] { build(l, start, mid, depth + 1); });
+            build(r, mid, end, depth + 1);
+            fut.get();
+        } else {
+            build(l, start, mid, depth + 1);
+            build(r, mid, end, depth + 1);
+        }
+    }
+};
+
+struct Collapser {
+    const std::vector<BNode>& bn;
+    std::vector<Node4>& out;
+    int max_depth = 0;
+
+    int collapse(int root, int depth) {
+        max_depth = std::max(max_depth, depth + 1);
+        int kids[4];
+        int nk = 0;
+        kids[nk++] = bn[root].left;
+        kids[nk++] = bn[root].right;
+        while (nk < 4) {
+            int pick = -1;
+            float best = -1.0f;
+            for (int i = 0; i < nk; ++i) {
+                const BNode& c = bn[kids[i]];
+                if (c.count == 0 && c.b.area() > best) { best = c.b.area(); pick = i; }
+            }
+            if (pick < 0) break;
+            const int opened = kids[pick];
+            kids[pick] = bn[opened].left;
+            kids[nk++] = bn[opened].right;
+        }
+        const int me = int(out.size());
+        out.emplace_back();
+        Node4 tmp;
+        for (int s = 0; s < 4; ++s) {
+            tmp.lox[s] = tmp.loy[s] = tmp.loz[s] = kInf;
+            tmp.hix[s] = tmp.hiy[s] = tmp.hiz[s] = -kInf;
+            tmp.child[s] = -1;
+            tmp.pad[s] = 0;
+        }
+        for (int s = 0; s < nk; ++s) {
+            const BNode& c = bn[kids[s]];
+            tmp.lox[s] = c.b.lo[0]; tmp.loy[s] = c.b.lo[1]; tmp.loz[s] = c.b.lo[2];
+            tmp.hix[s] = c.b.hi[0]; tmp.hiy[s] = c.b.hi[1]; tmp.hiz[s] = c.b.hi[2];
+            if (c.count > 0) {
+                if (c.count > 15) throw std::runtime_error("bvh: leaf larger than 15 triangles");
+                tmp.child[s] = ~((c.first << 4) | c.count);
+            } else {
+                tmp.child[s] = collapse(kids[s], depth + 1);
+            }
+        }
+        out[me] = tmp;
+        return me;
+    }
+};
+
+double sah_cost(const std::vector<BNode>& bn, int id, float root_area) {
+    const BNode& n = bn[id];
+    const double rel = root_area > 0 ? double(n.b.area()) / root_area : 0.0;
+    if (n.count > 0) return rel * n.count;
+    return rel + sah_cost(bn, n.left, root_area) + sah_cost(bn, n.right, root_area);
+}
+
+}  // namespace
+
+BvhBuildResult build_bvh4(const float* P, int64_t nv, const int32_t* idx, int64_t nt, const BvhBuildOptions& opts) {
+    if (nt <= 0) throw std::runtime_error("bvh: scene has no triangles");
+    if (nt >= (int64_t(1) << 27)) throw std::runtime_error("bvh: too many triangles for the 4-wide leaf encoding");
+    std::vector<PrimRef> prims(static_cast<size_t>(nt));
+    for (int64_t t = 0; t < nt; ++t) {
+        PrimRef& p = prims[size_t(t)];
+        for (int k = 0; k < 3; ++k) {
+            const int32_t vi = idx[3 * t + k];
+            if (vi < 0 || vi >= nv) throw std::runtime_error("bvh: vertex index out of range");
+            p.b.grow(P + 3 * int64_t(vi));
+        }
+        for (int k = 0; k < 3; ++k) p.c[k] = 0.5f * p.b.lo[k] + 0.5f * p.b.hi[k];
+    }
+    std::vector<int> ids(static_cast<size_t>(nt));
+    for (int64_t i = 0; i < nt; ++i) ids[size_t(i)] = int(i);
+    std::vector<BNode> bnodes(size_t(2 * nt + 1));
+    int threads = opts.threads > 0 ? opts.threads : int(std::max(1u, std::thread::hardware_concurrency()));
+    int par_depth = 0;
+    while ((1 << par_depth) < threads) ++par_depth;
+    par_depth += 1;
+    const int max_leaf = std::min(15, std::max(1, opts.max_leaf));
+    Builder b(prims, ids, bnodes, max_leaf, par_depth);
+    const int root = b.alloc();
+    b.build(root, 0, int(nt), 0);
+    bnodes.resize(size_t(b.next_node.load()));
+
+    BvhBuildResult res;
+    res.sah_cost = sah_cost(bnodes, root, bnodes[root].b.area());
+    res.tris.resize(size_t(nt));
+    for (int64_t i = 0; i < nt; ++i) {
+        const int t = ids[size_t(i)];
+        TriPacked& q = res.tris[size_t(i)];
+        const float* a = P + 3 * int64_t(idx[3 * int64_t(t)]);
+        const float* bb = P + 3 * int64_t(idx[3 * int64_t(t) + 1]);
+        const float* c = P + 3 * int64_t(idx[3 * int64_t(t) + 2]);
+        std::memcpy(q.v0, a, 12);
+        std::memcpy(q.v1, bb, 12);
+        std::memcpy(q.v2, c, 12);
+        q.prim = t;
+        q.pad1 = q.pad2 = 0;
+    }
+    if (bnodes[root].count > 0) {
+        // Single leaf: wrap it in one 4-wide node.
+        Node4 n;
+        for (int s = 0; s < 4; ++s) {
+            n.lox[s] = n.loy[s] = n.loz[s] = kInf;
+            n.hix[s] = n.hiy[s] = n.hiz[s] = -kInf;
+            n.child[s] = -1;
+            n.pad[s] = 0;
+        }
+        const BNode& r = bnodes[root];
+        n.lox[0] = r.b.lo[0]; n.loy[0] = r.b.lo[1]; n.loz[0] = r.b.lo[2];
+        n.hix[0] = r.b.hi[0]; n.hiy[0] = r.b.hi[1]; n.hiz[0] = r.b.hi[2];
+        n.child[0] = ~((r.first << 4) | r.count);
+        res.nodes.push_back(n);
+        res.depth = 1;
+        return res;
+    }
+    res.nodes.reserve(size_t(nt / 2 + 8));
+    Collapser col{bnodes, res.nodes};
+    col.collapse(root, 0);
+    res.depth = col.max_depth;
+    return res;
+}
+
+}  // namespace trhip

@@ -1,0 +1,759 @@
+// C ABI of libtracer_hip.so (include/tracer_hip.h): context, scene upload,
+// BVH build, render orchestration and the batch query entry points.
+#include "tracer_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "bvh_build.h"
+#include "tr_kernels.h"
+
+using namespace trhip;
+
+namespace {
+
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct InvalidArg : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct StateError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+inline void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Device allocation that grows on demand and never shrinks.
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    void reserve(size_t n) {
+        if (n <= bytes) return;
+        release();
+        if (n == 0) return;
+        hipError_t e = hipMalloc(&ptr, n);
+        if (e != hipSuccess) {
+            ptr = nullptr;
+            bytes = 0;
+            (void)hipGetLastError();
+            throw std::bad_alloc();
+        }
+        bytes = n;
+    }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    template <class T> T* as() const { return static_cast<T*>(ptr); }
+    ~DevBuf() { release(); }
+};
+
+struct HostScene {
+    std::vector<float> P, N;
+    std::vector<int32_t> idx, mat, med_in, med_out;
+    std::vector<float> emission;
+    std::vector<tr_material> mats;
+    std::vector<tr_medium> media;
+    int has_env = 0, has_sun = 0;
+    float env[3] = {0, 0, 0}, sun_dir[3] = {0, 0, 1}, sun_E[3] = {0, 0, 0};
+    tr_camera cam{};
+    float lo[3], hi[3];
+};
+
+}  // namespace
+
+struct tr_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    HostScene hs;
+    bool have_scene = false, have_bvh = false;
+    // scene on device
+    DevBuf nodes, tris, tri_shade, mats, media, lights, light_cdf, light_pmf;
+    int64_t num_nodes = 0;
+    int max_stack = 0;
+    int bvh_depth = 0;
+    DevScene ds{};
+    // traversal spill area
+    DevBuf ovf_node, ovf_t;
+    int ovf_threads = 0;
+    // wavefront buffers
+    DevBuf ray_o, ray_d, hit, thr, rad, rng, info, q0, q1, sh_o, sh_d, sh_c, counters, stats, film;
+    int64_t path_capacity = 0;
+    // scratch for batch calls
+    DevBuf s_in, s_out, s_aux, s_aux2, s_aux3, s_aux4;
+    std::vector<hipEvent_t> events;
+};
+
+namespace {
+
+tr_status fail(tr_ctx* ctx, tr_status code, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+template <class F> tr_status guarded(tr_ctx* ctx, F&& f) {
+    if (!ctx) return TR_ERR_INVALID;
+    try {
+        hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+        f();
+        ctx->err.clear();
+        return TR_OK;
+    } catch (const InvalidArg& e) {
+        return fail(ctx, TR_ERR_INVALID, e.what());
+    } catch (const StateError& e) {
+        return fail(ctx, TR_ERR_STATE, e.what());
+    } catch (const HipError& e) {
+        return fail(ctx, TR_ERR_HIP, e.what());
+    } catch (const std::bad_alloc&) {
+        return fail(ctx, TR_ERR_OOM, "device or host allocation failed");
+    } catch (const std::exception& e) {
+        return fail(ctx, TR_ERR_INVALID, e.what());
+    }
+}
+
+void upload(DevBuf& b, const void* src, size_t bytes, hipStream_t st) {
+    b.reserve(std::max<size_t>(bytes, 16));
+    if (bytes) hip_check(hipMemcpyAsync(b.ptr, src, bytes, hipMemcpyHostToDevice, st), "hipMemcpyAsync H2D");
+}
+
+// DisneyBSDF constructor (src/material/disney.cpp:347-399) evaluated once per material.
+DevMaterial make_material(const tr_material& m) {
+    DevMaterial d{};
+    d.type = m.type == TR_MAT_DISNEY ? MAT_DISNEY : MAT_LAMBERT;
+    d.C = mk(m.base_color[0], m.base_color[1], m.base_color[2]);
+    const float l = lum(d.C);
+    d.Ctint = l > 0.0f ? d.C * (1.0f / l) : f3s(1.0f);
+    d.metallic = m.metallic;
+    d.roughness = m.roughness;
+    d.transmission = m.transmission;
+    d.ior = std::max(1.01f, m.ior);
+    d.spec_scale = mk(m.specular_scale[0], m.specular_scale[1], m.specular_scale[2]);
+    d.spec_tint = m.specular_tint;
+    d.sheen = m.sheen;
+    d.sheen_tint = m.sheen_tint;
+    d.clearcoat = m.clearcoat;
+    const float aspect = m.anisotropic > 0.0f ? std::sqrt(1.0f - 0.9f * m.anisotropic) : 1.0f;
+    d.ax = std::max(0.001f, m.roughness * m.roughness / aspect);
+    d.ay = std::max(0.001f, m.roughness * m.roughness * aspect);
+    d.tax = std::max(0.001f, m.transmission_roughness * m.transmission_roughness / aspect);
+    d.tay = std::max(0.001f, m.transmission_roughness * m.transmission_roughness * aspect);
+    float cr = (1.0f - m.clearcoat_gloss) * 0.1f + m.clearcoat_gloss * 0.0f;
+    cr *= cr;
+    d.cc_rough = std::max(cr, 0.0001f);
+    const float A = std::min(std::max(l * (1.0f - m.metallic), 0.3f), 0.7f);
+    const float B = 1.0f - A;
+    d.w_diffuse = A * (1.0f - m.transmission);
+    d.w_trans = A * m.transmission;
+    d.w_spec = B * 2.0f / (2.0f + m.clearcoat);
+    d.w_cc = B * m.clearcoat / (2.0f + m.clearcoat);
+    if (d.type == MAT_LAMBERT) {
+        d.metallic = 0.0f;
+        d.transmission = 0.0f;
+    }
+    return d;
+}
+
+DevMedium make_medium(const tr_medium& m) {
+    DevMedium d{};
+    d.sigma_s = mk(m.sigma_s[0], m.sigma_s[1], m.sigma_s[2]);
+    const f3 sa = mk(m.sigma_a[0], m.sigma_a[1], m.sigma_a[2]);
+    d.sigma_t = sa + d.sigma_s;
+    // HomogeneousMedium clamps g into (-1, 1) (src/medium/homogeneous.cpp:16-17)
+    d.g = std::min(std::max(m.g, -1.0f + kEps), 1.0f - kEps);
+    d.majorant = max3(d.sigma_t);
+    d.max_scatter = m.max_scattering_count > 0 ? m.max_scattering_count : 0x7fffffff;
+    if (d.majorant > 0.0f) {
+        d.p_absorb = (sa.x + sa.y + sa.z) / (3.0f * d.majorant);
+        d.p_scatter = (d.sigma_s.x + d.sigma_s.y + d.sigma_s.z) / (3.0f * d.majorant);
+        d.p_null = 1.0f - d.p_absorb - d.p_scatter;
+        if (d.p_null < 1e-6f) {
+            d.p_null = 0.0f;
+            d.p_scatter = 1.0f - d.p_absorb;
+        }
+    } else {
+        d.majorant = 1e-30f;
+        d.p_absorb = 0.0f;
+        d.p_scatter = 0.0f;
+        d.p_null = 1.0f;
+    }
+    return d;
+}
+
+void validate_scene(const tr_scene_desc* d) {
+    if (!d) throw InvalidArg("scene is NULL");
+    if (d->num_vertices <= 0 || !d->positions) throw InvalidArg("scene has no vertices");
+    if (d->num_triangles <= 0 || !d->indices || !d->material_ids) throw InvalidArg("scene has no triangles");
+    if (d->num_materials <= 0 || !d->materials) throw InvalidArg("scene has no materials");
+    if (d->num_media < 0 || (d->num_media > 0 && !d->media)) throw InvalidArg("media array missing");
+    for (int64_t t = 0; t < d->num_triangles; ++t) {
+        for (int k = 0; k < 3; ++k) {
+            const int32_t v = d->indices[3 * t + k];
+            if (v < 0 || v >= d->num_vertices) throw InvalidArg("triangle " + std::to_string(t) + " has a vertex index out of range");
+        }
+        const int32_t m = d->material_ids[t];
+        if (m < 0 || m >= d->num_materials) throw InvalidArg("triangle " + std::to_string(t) + " has an invalid material id");
+        if (d->medium_inside) {
+            const int32_t mi = d->medium_inside[t];
+            if (mi < -1 || mi >= d->num_media) throw InvalidArg("invalid medium_inside id");
+        }
+        if (d->medium_outside) {
+            const int32_t mo = d->medium_outside[t];
+            if (mo < -1 || mo >= d->num_media) throw InvalidArg("invalid medium_outside id");
+        }
+    }
+    for (int i = 0; i < d->num_materials; ++i) {
+        const int ty = d->materials[i].type;
+        if (ty != TR_MAT_LAMBERT && ty != TR_MAT_DISNEY) throw InvalidArg("unknown material type");
+    }
+    for (int i = 0; i < d->num_media; ++i) {
+        const tr_medium& m = d->media[i];
+        if (!(m.g > -1.0f && m.g < 1.0f)) throw InvalidArg("medium g must lie in (-1, 1)");  // homogeneous.cpp:16
+        for (int k = 0; k < 3; ++k)
+            if (m.sigma_a[k] < 0 || m.sigma_s[k] < 0) throw InvalidArg("medium coefficients must be >= 0");
+    }
+}
+
+void build_device_scene(tr_ctx* c, const tr_bvh_opts* opts) {
+    HostScene& h = c->hs;
+    const int64_t nt = int64_t(h.mat.size());
+    const int64_t nv = int64_t(h.P.size() / 3);
+    BvhBuildOptions bo;
+    if (opts) {
+        bo.max_leaf = opts->max_leaf_primitives > 0 ? opts->max_leaf_primitives : 4;
+        bo.threads = opts->num_threads;
+    }
+    BvhBuildResult r = build_bvh4(h.P.data(), nv, h.idx.data(), nt, bo);
+    c->num_nodes = int64_t(r.nodes.size());
+    c->bvh_depth = r.depth;
+    c->max_stack = 3 * r.depth + 4;
+
+    // leaf-ordered shading records and light table
+    std::vector<float4> shade(size_t(nt) * 3);
+    std::vector<int> tri_to_light(size_t(nt), -1);
+    std::vector<int> light_tris;   // leaf-ordered index of each emissive triangle
+    std::vector<float4> lights;
+    for (int64_t i = 0; i < nt; ++i) {
+        const int prim = r.tris[size_t(i)].prim;
+        const bool em = !h.emission.empty() &&
+                        (h.emission[3 * prim] != 0.0f || h.emission[3 * prim + 1] != 0.0f || h.emission[3 * prim + 2] != 0.0f);
+        if (em) {
+            tri_to_light[size_t(i)] = int(light_tris.size());
+            light_tris.push_back(int(i));
+        }
+    }
+    const int nA = int(light_tris.size());
+    std::vector<double> power(size_t(nA) + 2, 0.0);
+    // world bounding sphere (Bounds3::bounding_sphere, geometry.hpp:752-755)
+    const f3 wc = mk(0.5f * (h.lo[0] + h.hi[0]), 0.5f * (h.lo[1] + h.hi[1]), 0.5f * (h.lo[2] + h.hi[2]));
+    const float wr = length(mk(h.hi[0], h.hi[1], h.hi[2]) - wc);
+    for (int64_t i = 0; i < nt; ++i) {
+        const TriPacked& tp = r.tris[size_t(i)];
+        const int prim = tp.prim;
+        f3 n[3];
+        const f3 A = mk(tp.v0[0], tp.v0[1], tp.v0[2]), B = mk(tp.v1[0], tp.v1[1], tp.v1[2]), C = mk(tp.v2[0], tp.v2[1], tp.v2[2]);
+        const f3 cr = cross(B - A, C - A);
+        const f3 gn = length(cr) > 0.0f ? normalize(cr) : mk(0.0f, 0.0f, 1.0f);
+        for (int k = 0; k < 3; ++k) {
+            if (!h.N.empty()) {
+                const int v = h.idx[3 * size_t(prim) + k];
+                f3 nn = mk(h.N[3 * size_t(v)], h.N[3 * size_t(v) + 1], h.N[3 * size_t(v) + 2]);
+                const float ln = length(nn);
+                n[k] = ln > 0.0f ? nn * (1.0f / ln) : gn;
+            } else {
+                n[k] = gn;
+            }
+        }
+        const int mi = h.med_in.empty() ? -1 : h.med_in[size_t(prim)];
+        const int mo = h.med_out.empty() ? -1 : h.med_out[size_t(prim)];
+        const int med = (mi & 0xffff) | (mo << 16);
+        shade[3 * size_t(i)] = f4(n[0], __builtin_bit_cast(float, int32_t(h.mat[size_t(prim)])));
+        shade[3 * size_t(i) + 1] = f4(n[1], __builtin_bit_cast(float, int32_t(tri_to_light[size_t(i)])));
+        shade[3 * size_t(i) + 2] = f4(n[2], __builtin_bit_cast(float, int32_t(med)));
+        const int li = tri_to_light[size_t(i)];
+        if (li >= 0) {
+            const f3 L = mk(h.emission[3 * prim], h.emission[3 * prim + 1], h.emission[3 * prim + 2]);
+            const double area = 0.5 * std::sqrt(double(cr.x) * cr.x + double(cr.y) * cr.y + double(cr.z) * cr.z);
+            lights.push_back(f4(L, float(area)));
+            lights.push_back(make_float4(__builtin_bit_cast(float, int32_t(i)), 0.0f, 0.0f, 0.0f));
+            power[size_t(li)] = double(kPi) * lum(L) * area;   // DiffuseLight::power, diffuse_light.cpp:17-20
+        }
+    }
+    if (h.has_env) power[size_t(nA)] = double(lum(mk(h.env[0], h.env[1], h.env[2]))) * 4.0 * kPi * wr * wr;
+    if (h.has_sun) power[size_t(nA) + 1] = double(lum(mk(h.sun_E[0], h.sun_E[1], h.sun_E[2]))) * kPi * wr * wr;
+    double tot = 0.0;
+    for (double p : power) tot += p;
+    std::vector<float> pmf(power.size(), 0.0f), cdf(power.size(), 1.0f);
+    double acc = 0.0;
+    for (size_t i = 0; i < power.size(); ++i) {
+        pmf[i] = tot > 0.0 ? float(power[i] / tot) : 0.0f;
+        acc += tot > 0.0 ? power[i] / tot : 0.0;
+        cdf[i] = float(acc);
+    }
+    for (int i = 0; i < nA; ++i) lights[2 * size_t(i) + 1].y = pmf[size_t(i)];
+
+    std::vector<DevMaterial> dm(h.mats.size());
+    for (size_t i = 0; i < h.mats.size(); ++i) dm[i] = make_material(h.mats[i]);
+    std::vector<DevMedium> dmed(h.media.size());
+    for (size_t i = 0; i < h.media.size(); ++i) dmed[i] = make_medium(h.media[i]);
+
+    hipStream_t st = c->stream;
+    upload(c->nodes, r.nodes.data(), r.nodes.size() * sizeof(Node4), st);
+    upload(c->tris, r.tris.data(), r.tris.size() * sizeof(TriPacked), st);
+    upload(c->tri_shade, shade.data(), shade.size() * sizeof(float4), st);
+    upload(c->mats, dm.data(), dm.size() * sizeof(DevMaterial), st);
+    upload(c->media, dmed.data(), dmed.size() * sizeof(DevMedium), st);
+    upload(c->lights, lights.data(), lights.size() * sizeof(float4), st);
+    upload(c->light_cdf, cdf.data(), cdf.size() * sizeof(float), st);
+    upload(c->light_pmf, pmf.data(), pmf.size() * sizeof(float), st);
+    hip_check(hipStreamSynchronize(st), "scene upload");
+
+    // traversal spill area: (max_stack - LDS entries) per thread of the largest grid
+    const int threads = trace_grid_threads(int64_t(1) << 40);
+    const int extra = std::max(1, c->max_stack - kLdsStack);
+    c->ovf_node.reserve(size_t(extra) * threads * sizeof(int));
+    c->ovf_t.reserve(size_t(extra) * threads * sizeof(float));
+    c->ovf_threads = threads;
+
+    DevScene& s = c->ds;
+    s.tr.nodes = c->nodes.as<float4>();
+    s.tr.tris = c->tris.as<float4>();
+    s.tr.max_stack = c->max_stack;
+    s.tri_shade = c->tri_shade.as<float4>();
+    s.mats = c->mats.as<DevMaterial>();
+    s.media = c->media.as<DevMedium>();
+    s.lights = c->lights.as<float4>();
+    s.light_cdf = c->light_cdf.as<float>();
+    s.light_pmf = c->light_pmf.as<float>();
+    s.num_area_lights = nA;
+    s.num_light_entries = tot > 0.0 ? nA + 2 : 0;
+    s.has_env = h.has_env;
+    s.has_sun = h.has_sun;
+    s.env_L = mk(h.env[0], h.env[1], h.env[2]);
+    const f3 sd = mk(h.sun_dir[0], h.sun_dir[1], h.sun_dir[2]);
+    s.sun_dir = length(sd) > 0.0f ? normalize(sd) : mk(0.0f, 0.0f, 1.0f);
+    s.sun_E = mk(h.sun_E[0], h.sun_E[1], h.sun_E[2]);
+    s.wcenter = wc;
+    s.wradius = wr;
+    // camera (thin_lens_camera.cpp:23-36, transform.cpp:96-124)
+    const tr_camera& cm = h.cam;
+    const f3 pos = mk(cm.pos[0], cm.pos[1], cm.pos[2]);
+    const f3 dir = normalize(mk(cm.target[0], cm.target[1], cm.target[2]) - pos);
+    const f3 right = normalize(cross(normalize(mk(cm.up[0], cm.up[1], cm.up[2])), dir));
+    const f3 up = cross(dir, right);
+    s.cam.pos = pos;
+    s.cam.dir = dir;
+    s.cam.right = right;
+    s.cam.up = up;
+    s.cam.focal = cm.focal_distance > 0.0f ? cm.focal_distance : 1.0f;
+    s.cam.film_h = 2.0f * s.cam.focal * std::tan(cm.fov_y * 0.5f);
+    s.cam.film_w = (cm.aspect > 0.0f ? cm.aspect : 1.0f) * s.cam.film_h;
+    s.cam.lens_radius = cm.lens_radius;
+    c->have_bvh = true;
+}
+
+void ensure_paths(tr_ctx* c, int64_t P) {
+    if (P <= c->path_capacity) return;
+    const size_t n = size_t(P);
+    c->ray_o.reserve(n * 16); c->ray_d.reserve(n * 16); c->hit.reserve(n * 16);
+    c->thr.reserve(n * 16); c->rad.reserve(n * 16); c->rng.reserve(n * 8); c->info.reserve(n * 8);
+    c->q0.reserve(n * 4); c->q1.reserve(n * 4);
+    c->sh_o.reserve(n * 16); c->sh_d.reserve(n * 16); c->sh_c.reserve(n * 16);
+    c->path_capacity = P;
+}
+
+DevPaths make_paths(tr_ctx* c) {
+    DevPaths p{};
+    p.ray_o = c->ray_o.as<float4>(); p.ray_d = c->ray_d.as<float4>(); p.hit = c->hit.as<float4>();
+    p.thr = c->thr.as<float4>(); p.rad = c->rad.as<float4>(); p.rng = c->rng.as<uint2>(); p.info = c->info.as<uint2>();
+    p.queue[0] = c->q0.as<int>(); p.queue[1] = c->q1.as<int>();
+    p.sh_o = c->sh_o.as<float4>(); p.sh_d = c->sh_d.as<float4>(); p.sh_c = c->sh_c.as<float4>();
+    p.counters = c->counters.as<unsigned int>();
+    p.stats = c->stats.as<unsigned long long>();
+    p.ovf_node = c->ovf_node.as<int>();
+    p.ovf_t = c->ovf_t.as<float>();
+    p.ovf_threads = c->ovf_threads;
+    return p;
+}
+
+hipEvent_t ev(tr_ctx* c, size_t i) {
+    while (c->events.size() <= i) {
+        hipEvent_t e;
+        hip_check(hipEventCreate(&e), "hipEventCreate");
+        c->events.push_back(e);
+    }
+    return c->events[i];
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* tr_version(void) { return "tracer_hip 0.1 (gfx950, wavefront path tracer, 4-wide BVH)"; }
+
+tr_status tr_bvh4_export(const tr_scene_desc* d, const tr_bvh_opts* opts, void* nodes_out, int64_t nodes_cap,
+                         void* tris_out, int64_t* num_nodes, int32_t* depth) {
+    try {
+        validate_scene(d);
+        BvhBuildOptions bo;
+        if (opts) {
+            bo.max_leaf = opts->max_leaf_primitives > 0 ? opts->max_leaf_primitives : 4;
+            bo.threads = opts->num_threads;
+        }
+        BvhBuildResult r = build_bvh4(d->positions, d->num_vertices, d->indices, d->num_triangles, bo);
+        if (num_nodes) *num_nodes = int64_t(r.nodes.size());
+        if (depth) *depth = r.depth;
+        if (nodes_out) {
+            if (nodes_cap < int64_t(r.nodes.size())) return TR_ERR_INVALID;
+            std::memcpy(nodes_out, r.nodes.data(), r.nodes.size() * sizeof(Node4));
+        }
+        if (tris_out) std::memcpy(tris_out, r.tris.data(), r.tris.size() * sizeof(TriPacked));
+        return TR_OK;
+    } catch (const std::exception&) {
+        return TR_ERR_INVALID;
+    }
+}
+
+void tr_rng_sequence(uint64_t seed, uint32_t pixel, uint32_t sample, int64_t n, uint32_t* out) {
+    Pcg r = pcg_for_sample(seed, pixel, sample);
+    for (int64_t i = 0; i < n; ++i) out[i] = r.next();
+}
+
+tr_status tr_create(tr_ctx** out, int32_t device) {
+    if (!out) return TR_ERR_INVALID;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return TR_ERR_HIP;
+    if (device < 0 || device >= count) return TR_ERR_INVALID;
+    tr_ctx* c = new tr_ctx();
+    c->device = device;
+    tr_status s = guarded(c, [&] {
+        hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+        c->counters.reserve(64);
+        c->stats.reserve(64);
+    });
+    if (s != TR_OK) {
+        delete c;
+        return s;
+    }
+    *out = c;
+    return TR_OK;
+}
+
+void tr_destroy(tr_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* tr_last_error(const tr_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+tr_status tr_scene_upload(tr_ctx* c, const tr_scene_desc* d) {
+    return guarded(c, [&] {
+        validate_scene(d);
+        HostScene& h = c->hs;
+        const size_t nv = size_t(d->num_vertices), nt = size_t(d->num_triangles);
+        h.P.assign(d->positions, d->positions + 3 * nv);
+        if (d->normals) h.N.assign(d->normals, d->normals + 3 * nv); else h.N.clear();
+        h.idx.assign(d->indices, d->indices + 3 * nt);
+        h.mat.assign(d->material_ids, d->material_ids + nt);
+        if (d->emission) h.emission.assign(d->emission, d->emission + 3 * nt); else h.emission.clear();
+        if (d->medium_inside) h.med_in.assign(d->medium_inside, d->medium_inside + nt); else h.med_in.clear();
+        if (d->medium_outside) h.med_out.assign(d->medium_outside, d->medium_outside + nt); else h.med_out.clear();
+        h.mats.assign(d->materials, d->materials + d->num_materials);
+        if (d->num_media) h.media.assign(d->media, d->media + d->num_media); else h.media.clear();
+        h.has_env = d->has_env;
+        h.has_sun = d->has_sun;
+        std::memcpy(h.env, d->env_radiance, sizeof h.env);
+        std::memcpy(h.sun_dir, d->sun_direction, sizeof h.sun_dir);
+        std::memcpy(h.sun_E, d->sun_irradiance, sizeof h.sun_E);
+        h.cam = d->camera;
+        for (int k = 0; k < 3; ++k) { h.lo[k] = INFINITY; h.hi[k] = -INFINITY; }
+        for (size_t t = 0; t < nt; ++t)
+            for (int k = 0; k < 3; ++k) {
+                const float* v = &h.P[3 * size_t(h.idx[3 * t + k])];
+                for (int a = 0; a < 3; ++a) { h.lo[a] = std::min(h.lo[a], v[a]); h.hi[a] = std::max(h.hi[a], v[a]); }
+            }
+        c->have_scene = true;
+        c->have_bvh = false;
+    });
+}
+
+tr_status tr_bvh_build(tr_ctx* c, const tr_bvh_opts* opts) {
+    return guarded(c, [&] {
+        if (!c->have_scene) throw StateError("tr_bvh_build before tr_scene_upload");
+        build_device_scene(c, opts);
+    });
+}
+
+tr_status tr_bvh_info(tr_ctx* c, int64_t* num_nodes, int64_t* node_bytes, int64_t* tri_bytes) {
+    return guarded(c, [&] {
+        if (!c->have_bvh) throw StateError("no BVH");
+        if (num_nodes) *num_nodes = c->num_nodes;
+        if (node_bytes) *node_bytes = c->num_nodes * int64_t(sizeof(Node4));
+        if (tri_bytes) *tri_bytes = int64_t(c->hs.mat.size()) * int64_t(sizeof(TriPacked));
+    });
+}
+
+tr_status tr_stream(tr_ctx* c, void** s) {
+    return guarded(c, [&] {
+        if (!s) throw InvalidArg("null out pointer");
+        *s = c->stream;
+    });
+}
+
+tr_status tr_synchronize(tr_ctx* c) {
+    return guarded(c, [&] { hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize"); });
+}
+
+tr_status tr_trace_closest_device(tr_ctx* c, const float* d_rays, float* d_hits, int64_t n) {
+    return guarded(c, [&] {
+        if (!c->have_bvh) throw StateError("tr_trace_closest before tr_bvh_build");
+        if (n < 0 || (n > 0 && (!d_rays || !d_hits))) throw InvalidArg("bad ray batch");
+        launch_trace_closest(c->ds.tr, reinterpret_cast<const float4*>(d_rays), reinterpret_cast<float4*>(d_hits), n,
+                             c->ovf_node.as<int>(), c->ovf_t.as<float>(), c->ovf_threads, c->stream);
+        hip_check(hipGetLastError(), "k_trace_closest launch");
+    });
+}
+
+tr_status tr_trace_any_device(tr_ctx* c, const float* d_rays, int32_t* d_occ, int64_t n) {
+    return guarded(c, [&] {
+        if (!c->have_bvh) throw StateError("tr_trace_any before tr_bvh_build");
+        if (n < 0 || (n > 0 && (!d_rays || !d_occ))) throw InvalidArg("bad ray batch");
+        launch_trace_any(c->ds.tr, reinterpret_cast<const float4*>(d_rays), d_occ, n, c->ovf_node.as<int>(),
+                         c->ovf_t.as<float>(), c->ovf_threads, c->stream);
+        hip_check(hipGetLastError(), "k_trace_any launch");
+    });
+}
+
+tr_status tr_trace_closest(tr_ctx* c, const float* rays, float* hits, int64_t n) {
+    return guarded(c, [&] {
+        if (!c->have_bvh) throw StateError("tr_trace_closest before tr_bvh_build");
+        if (n < 0 || (n > 0 && (!rays || !hits))) throw InvalidArg("bad ray batch");
+        if (n == 0) return;
+        upload(c->s_in, rays, size_t(n) * 32, c->stream);
+        c->s_out.reserve(size_t(n) * 16);
+        launch_trace_closest(c->ds.tr, c->s_in.as<float4>(), c->s_out.as<float4>(), n, c->ovf_node.as<int>(),
+                             c->ovf_t.as<float>(), c->ovf_threads, c->stream);
+        hip_check(hipGetLastError(), "k_trace_closest launch");
+        hip_check(hipMemcpyAsync(hits, c->s_out.ptr, size_t(n) * 16, hipMemcpyDeviceToHost, c->stream), "D2H");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+    });
+}
+
+tr_status tr_trace_any(tr_ctx* c, const float* rays, int32_t* occ, int64_t n) {
+    return guarded(c, [&] {
+        if (!c->have_bvh) throw StateError("tr_trace_any before tr_bvh_build");
+        if (n < 0 || (n > 0 && (!rays || !occ))) throw InvalidArg("bad ray batch");
+        if (n == 0) return;
+        upload(c->s_in, rays, size_t(n) * 32, c->stream);
+        c->s_out.reserve(size_t(n) * 4);
+        launch_trace_any(c->ds.tr, c->s_in.as<float4>(), c->s_out.as<int>(), n, c->ovf_node.as<int>(), c->ovf_t.as<float>(),
+                         c->ovf_threads, c->stream);
+        hip_check(hipGetLastError(), "k_trace_any launch");
+        hip_check(hipMemcpyAsync(occ, c->s_out.ptr, size_t(n) * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+    });
+}
+
+tr_status tr_trace_closest_counted(tr_ctx* c, const float* rays, float* hits, uint32_t* nvis, uint32_t* ntri, int64_t n) {
+    return guarded(c, [&] {
+        if (!c->have_bvh) throw StateError("tr_trace_closest_counted before tr_bvh_build");
+        if (n < 0 || (n > 0 && (!rays || !hits || !nvis || !ntri))) throw InvalidArg("bad ray batch");
+        if (n == 0) return;
+        upload(c->s_in, rays, size_t(n) * 32, c->stream);
+        c->s_out.reserve(size_t(n) * 16);
+        c->s_aux.reserve(size_t(n) * 4);
+        c->s_aux2.reserve(size_t(n) * 4);
+        launch_trace_counted(c->ds.tr, c->s_in.as<float4>(), c->s_out.as<float4>(), c->s_aux.as<uint32_t>(),
+                             c->s_aux2.as<uint32_t>(), n, c->ovf_node.as<int>(), c->ovf_t.as<float>(), c->ovf_threads,
+                             c->stream);
+        hip_check(hipGetLastError(), "k_trace_counted launch");
+        hip_check(hipMemcpyAsync(hits, c->s_out.ptr, size_t(n) * 16, hipMemcpyDeviceToHost, c->stream), "D2H");
+        hip_check(hipMemcpyAsync(nvis, c->s_aux.ptr, size_t(n) * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+        hip_check(hipMemcpyAsync(ntri, c->s_aux2.ptr, size_t(n) * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+    });
+}
+
+static void bsdf_common(tr_ctx* c, const tr_material* mats, int32_t num_mats, const int32_t* ids, int64_t n) {
+    if (!mats || num_mats <= 0 || !ids || n < 0) throw InvalidArg("bad BSDF batch");
+    std::vector<DevMaterial> dm(static_cast<size_t>(num_mats));
+    for (int i = 0; i < num_mats; ++i) {
+        if (mats[i].type != TR_MAT_LAMBERT && mats[i].type != TR_MAT_DISNEY) throw InvalidArg("unknown material type");
+        dm[size_t(i)] = make_material(mats[i]);
+    }
+    for (int64_t i = 0; i < n; ++i)
+        if (ids[i] < 0 || ids[i] >= num_mats) throw InvalidArg("material id out of range");
+    upload(c->s_aux3, dm.data(), dm.size() * sizeof(DevMaterial), c->stream);
+    upload(c->s_aux4, ids, size_t(n) * 4, c->stream);
+}
+
+tr_status tr_bsdf_eval_batch(tr_ctx* c, const tr_material* mats, int32_t num_mats, const int32_t* ids, const float* wi,
+                             const float* wo, const float* ng, int64_t n, float* f, float* pdf) {
+    return guarded(c, [&] {
+        if (!wi || !wo || !f || !pdf) throw InvalidArg("bad BSDF batch");
+        bsdf_common(c, mats, num_mats, ids, n);
+        if (n == 0) return;
+        const size_t v3 = size_t(n) * 12;
+        c->s_in.reserve(3 * v3);
+        char* in = c->s_in.as<char>();
+        hip_check(hipMemcpyAsync(in, wi, v3, hipMemcpyHostToDevice, c->stream), "H2D");
+        hip_check(hipMemcpyAsync(in + v3, wo, v3, hipMemcpyHostToDevice, c->stream), "H2D");
+        if (ng) hip_check(hipMemcpyAsync(in + 2 * v3, ng, v3, hipMemcpyHostToDevice, c->stream), "H2D");
+        c->s_out.reserve(v3 + size_t(n) * 4);
+        char* out = c->s_out.as<char>();
+        launch_bsdf_eval(c->s_aux3.as<DevMaterial>(), c->s_aux4.as<int>(), reinterpret_cast<float*>(in),
+                         reinterpret_cast<float*>(in + v3), ng ? reinterpret_cast<float*>(in + 2 * v3) : nullptr, n,
+                         reinterpret_cast<float*>(out), reinterpret_cast<float*>(out + v3), c->stream);
+        hip_check(hipGetLastError(), "k_bsdf_eval launch");
+        hip_check(hipMemcpyAsync(f, out, v3, hipMemcpyDeviceToHost, c->stream), "D2H");
+        hip_check(hipMemcpyAsync(pdf, out + v3, size_t(n) * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+    });
+}
+
+tr_status tr_bsdf_sample_batch(tr_ctx* c, const tr_material* mats, int32_t num_mats, const int32_t* ids, const float* wo,
+                               const float* ng, const float* u, int64_t n, float* wi, float* f, float* pdf) {
+    return guarded(c, [&] {
+        if (!wo || !u || !wi || !f || !pdf) throw InvalidArg("bad BSDF batch");
+        bsdf_common(c, mats, num_mats, ids, n);
+        if (n == 0) return;
+        const size_t v3 = size_t(n) * 12;
+        c->s_in.reserve(3 * v3);
+        char* in = c->s_in.as<char>();
+        hip_check(hipMemcpyAsync(in, wo, v3, hipMemcpyHostToDevice, c->stream), "H2D");
+        hip_check(hipMemcpyAsync(in + v3, u, v3, hipMemcpyHostToDevice, c->stream), "H2D");
+        if (ng) hip_check(hipMemcpyAsync(in + 2 * v3, ng, v3, hipMemcpyHostToDevice, c->stream), "H2D");
+        c->s_out.reserve(2 * v3 + size_t(n) * 4);
+        char* out = c->s_out.as<char>();
+        launch_bsdf_sample(c->s_aux3.as<DevMaterial>(), c->s_aux4.as<int>(), reinterpret_cast<float*>(in),
+                           ng ? reinterpret_cast<float*>(in + 2 * v3) : nullptr, reinterpret_cast<float*>(in + v3), n,
+                           reinterpret_cast<float*>(out), reinterpret_cast<float*>(out + v3),
+                           reinterpret_cast<float*>(out + 2 * v3), c->stream);
+        hip_check(hipGetLastError(), "k_bsdf_sample launch");
+        hip_check(hipMemcpyAsync(wi, out, v3, hipMemcpyDeviceToHost, c->stream), "D2H");
+        hip_check(hipMemcpyAsync(f, out + v3, v3, hipMemcpyDeviceToHost, c->stream), "D2H");
+        hip_check(hipMemcpyAsync(pdf, out + 2 * v3, size_t(n) * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+    });
+}
+
+tr_status tr_render(tr_ctx* c, const tr_render_params* prm, double* film_sums, int32_t accumulate, float* image,
+                    tr_render_stats* stats) {
+    return guarded(c, [&] {
+        if (!c->have_bvh) throw StateError("tr_render before tr_bvh_build");
+        if (!prm) throw InvalidArg("params is NULL");
+        const tr_render_params& P = *prm;
+        if (P.width <= 0 || P.height <= 0 || P.spp <= 0) throw InvalidArg("width, height and spp must be positive");
+        if (P.max_depth < 0 || P.min_depth < 0) throw InvalidArg("depths must be >= 0");
+        if (P.direct_light_sample_num > 1) throw InvalidArg("direct_light_sample_num > 1 is not supported (use 1)");
+        if (P.filter_type != TR_FILTER_BOX && P.filter_type != TR_FILTER_GAUSSIAN) throw InvalidArg("unknown filter");
+        if (!(P.filter_radius > 0.0f && P.filter_radius <= 1.0f)) throw InvalidArg("filter radius must lie in (0, 1]");
+        if (P.guide != 0) throw InvalidArg("guided rendering is driven through tr_render_guided");
+        const int64_t npix = int64_t(P.width) * P.height;
+        if (npix > (int64_t(1) << 28)) throw InvalidArg("image too large");
+        int spp_pp = P.samples_per_pass > 0 ? P.samples_per_pass : int(std::max<int64_t>(1, (int64_t(1) << 21) / npix));
+        spp_pp = std::min(spp_pp, P.spp);
+        const int64_t maxP = npix * spp_pp;
+        if (maxP > (int64_t(1) << 30)) throw InvalidArg("too many paths per pass");
+        ensure_paths(c, maxP);
+        c->film.reserve(size_t(npix) * sizeof(double4));
+        hipStream_t st = c->stream;
+        if (accumulate && film_sums)
+            hip_check(hipMemcpyAsync(c->film.ptr, film_sums, size_t(npix) * 32, hipMemcpyHostToDevice, st), "H2D film");
+        else
+            hip_check(hipMemsetAsync(c->film.ptr, 0, size_t(npix) * 32, st), "film clear");
+        hip_check(hipMemsetAsync(c->stats.ptr, 0, 64, st), "stats clear");
+        DevPaths p = make_paths(c);
+        RenderConsts rc{};
+        rc.width = P.width; rc.height = P.height;
+        rc.min_depth = P.min_depth; rc.max_depth = P.max_depth;
+        rc.filter = P.filter_type; rc.filter_radius = P.filter_radius; rc.filter_alpha = P.filter_alpha;
+        rc.shadow_tr = P.shadow_transmittance;
+        rc.nls = 1;
+        rc.seed = P.seed;
+        size_t evi = 0;
+        const bool prof = P.profile != 0;
+        double ms_ext = 0, ms_sh = 0, ms_shd = 0, ms_oth = 0;
+        std::vector<std::pair<size_t, int>> marks;   // (event index, kind)
+        hipEvent_t e0 = ev(c, evi++), e1 = ev(c, evi++);
+        hip_check(hipEventRecord(e0, st), "event");
+        auto timed = [&](int kind, auto&& launch) {
+            if (!prof) { launch(); return; }
+            const size_t a = evi++, b = evi++;
+            hip_check(hipEventRecord(ev(c, a), st), "event");
+            launch();
+            hip_check(hipEventRecord(ev(c, b), st), "event");
+            marks.push_back({a, kind});
+        };
+        uint64_t launches = 0;
+        for (int done = 0; done < P.spp; done += spp_pp) {
+            const int k = std::min(spp_pp, P.spp - done);
+            rc.spp_in_pass = k;
+            rc.sample_base = P.sample_offset + done;
+            const int num_paths = int(npix * k);
+            int cur = 0;
+            timed(3, [&] { launch_generate(c->ds, p, rc, num_paths, cur, st); });
+            for (int b = 0; b <= P.max_depth; ++b) {
+                timed(0, [&] { launch_extend(c->ds.tr, p, cur, num_paths, st); });
+                timed(1, [&] { launch_shade(c->ds, p, rc, cur, num_paths, st); });
+                timed(2, [&] { launch_shadow(c->ds.tr, p, num_paths, 0, st); });
+                ++launches;
+                cur = 1 - cur;
+            }
+            timed(3, [&] { launch_accumulate(p, rc, c->film.as<double4>(), st); });
+            hip_check(hipGetLastError(), "render launch");
+        }
+        hip_check(hipEventRecord(e1, st), "event");
+        if (film_sums)
+            hip_check(hipMemcpyAsync(film_sums, c->film.ptr, size_t(npix) * 32, hipMemcpyDeviceToHost, st), "D2H film");
+        if (image) {
+            c->s_out.reserve(size_t(npix) * 12);
+            launch_resolve(c->film.as<double4>(), c->s_out.as<float>(), int(npix), st);
+            hip_check(hipMemcpyAsync(image, c->s_out.ptr, size_t(npix) * 12, hipMemcpyDeviceToHost, st), "D2H image");
+        }
+        unsigned long long hs[8] = {0};
+        hip_check(hipMemcpyAsync(hs, c->stats.ptr, sizeof hs, hipMemcpyDeviceToHost, st), "D2H stats");
+        hip_check(hipStreamSynchronize(st), "render sync");
+        if (stats) {
+            std::memset(stats, 0, sizeof *stats);
+            stats->camera_paths = uint64_t(npix) * uint64_t(P.spp);
+            stats->extension_rays = hs[0];
+            stats->shadow_rays = hs[1];
+            stats->nonfinite_dropped = hs[2];
+            float ms = 0.0f;
+            hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+            stats->ms_total = ms;
+            for (auto& m : marks) {
+                float t = 0.0f;
+                hip_check(hipEventElapsedTime(&t, ev(c, m.first), ev(c, m.first + 1)), "elapsed");
+                if (m.second == 0) ms_ext += t;
+                else if (m.second == 1) ms_sh += t;
+                else if (m.second == 2) ms_shd += t;
+                else ms_oth += t;
+            }
+            stats->ms_extend = ms_ext;
+            stats->ms_shade = ms_sh;
+            stats->ms_shadow = ms_shd;
+            stats->ms_other = ms_oth;
+            stats->extend_launches = launches;
+        }
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,432 @@
+// Wavefront path-tracing kernels: camera ray generation, shading (surface and
+// medium vertices, next-event estimation, MIS, Russian roulette), film
+// accumulation in FP64 and resolve.
+//
+// Replaces PixelSamplerRenderer::render (src/renderer/pixel_sampler_renderer.cpp:24-109)
+// and PathTraceRenderer::eval_pixel_li (src/renderer/path_trace_renderer.cpp:27-210)
+// with the NEE/MIS helpers of src/renderer/direct_illumination.cpp.  The
+// recursion of eval_pixel_li is unrolled into one kernel launch per bounce
+// over a compacted queue of live paths:
+//   * the reference traces, per vertex, a continuation ray and a separate
+//     BSDF-sampled MIS ray; here the continuation ray is the MIS ray (its hit
+//     on an emitter is weighted against the light-sampling pdf), which has the
+//     same expectation with one closest-hit ray per vertex;
+//   * NEE samples one light chosen by power instead of every light
+//     (path_trace_renderer.cpp:141-153 loops over scene.lights);
+//   * homogeneous media use delta tracking (spectral tracking for
+//     chromatic coefficients) with the medium of a segment decided at its end
+//     as in the reference (SurfaceIntersection::wo_medium,
+//     src/core/intersection.hpp:36-41);
+//   * the reference defects Q1, Q4, Q5, Q6 and Q23 (direct emission added
+//     before the medium sample) are not reproduced.
+#include "tr_kernels.h"
+
+namespace trhip {
+
+// info.y layout: depth (8 bits) | scatter count (24 bits) << 8
+TR_D uint32_t pack_info(int depth, int scatter) { return uint32_t(depth) | (uint32_t(scatter) << 8); }
+
+TR_D Pcg load_rng(const DevPaths& p, int slot, uint32_t pixel) {
+    const uint2 s = p.rng[slot];
+    Pcg r;
+    r.state = (uint64_t(s.y) << 32) | s.x;
+    r.inc = (uint64_t(pixel) << 1) | 1u;
+    return r;
+}
+TR_D void store_rng(const DevPaths& p, int slot, const Pcg& r) {
+    p.rng[slot] = make_uint2(uint32_t(r.state), uint32_t(r.state >> 32));
+}
+
+// Wave-aggregated append: one atomic per wavefront, lanes get consecutive
+// slots in lane order.
+TR_D unsigned int wave_append(unsigned int* counter, bool pred) {
+    const unsigned long long mask = __ballot(pred);
+    if (mask == 0ull) return 0u;
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)mask) - 1;
+    unsigned int base = 0u;
+    if (lane == leader) base = atomicAdd(counter, (unsigned int)__popcll(mask));
+    base = __shfl(base, leader);
+    const unsigned long long below = lane == 0 ? 0ull : (mask & ((~0ull) >> (64 - lane)));
+    return base + (unsigned int)__popcll(below);
+}
+
+__global__ __launch_bounds__(256) void k_generate(DevScene s, DevPaths p, RenderConsts c, int num_paths, int cur) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        p.counters[cur] = (unsigned int)num_paths;
+        p.counters[1 - cur] = 0u;
+        p.counters[2] = 0u;
+    }
+    if (i >= num_paths) return;
+    const int npix = c.width * c.height;
+    const uint32_t pixel = uint32_t(i % npix);
+    const uint32_t sample = uint32_t(c.sample_base + i / npix);
+    Pcg rng = pcg_for_sample(c.seed, pixel, sample);
+    const float fu = rng.uniform(), fv = rng.uniform(), lu = rng.uniform(), lv = rng.uniform();
+    const int px = int(pixel % uint32_t(c.width)), py = int(pixel / uint32_t(c.width));
+    // pixel_sampler_renderer.cpp:62-71: film position in [0,1)^2
+    const float fx = (float(px) + fu) / float(c.width), fy = (float(py) + fv) / float(c.height);
+    f3 o, d;
+    camera_ray(s.cam, fx, fy, lu, lv, o, d);
+    p.ray_o[i] = f4(o, 0.0f);
+    p.ray_d[i] = f4(d, 3.0e38f);
+    p.thr[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    p.rad[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    p.info[i] = make_uint2(pixel, pack_info(0, 0));
+    store_rng(p, i, rng);
+    p.queue[cur][i] = i;
+}
+
+// Transmittance of a homogeneous medium over a shadow segment: closed form,
+// or a ratio-tracking estimate (Novak et al. 2014) with the medium majorant.
+TR_D f3 shadow_tr(const DevMedium& m, float dist, int mode, Pcg& rng) {
+    if (mode == 0) return mk(expf(-m.sigma_t.x * dist), expf(-m.sigma_t.y * dist), expf(-m.sigma_t.z * dist));
+    f3 T = f3s(1.0f);
+    float t = 0.0f;
+    const f3 ratio = mk(1.0f - m.sigma_t.x / m.majorant, 1.0f - m.sigma_t.y / m.majorant, 1.0f - m.sigma_t.z / m.majorant);
+    for (int k = 0; k < 4096; ++k) {
+        t -= logf(1.0f - rng.uniform()) / m.majorant;
+        if (t >= dist) break;
+        T *= ratio;
+        if (max3(T) <= 0.0f) break;
+    }
+    return T;
+}
+
+struct ShadowOut {
+    bool emit;
+    f3 o;
+    float tmin;
+    f3 d;
+    float tmax;
+    f3 contrib;
+};
+
+__global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene s, DevPaths p, RenderConsts c, int cur) {
+    const unsigned int n = p.counters[cur];
+    const int* q = p.queue[cur];
+    int* qn = p.queue[1 - cur];
+    const unsigned int stride = gridDim.x * blockDim.x;
+    // Loop bound is uniform across the wavefront so wave_append stays convergent.
+    const unsigned int base0 = blockIdx.x * blockDim.x;
+    for (unsigned int base = base0; base < n; base += stride) {
+        const unsigned int i = base + threadIdx.x;
+        const bool active = i < n;
+        bool cont = false;
+        ShadowOut sh;
+        sh.emit = false;
+        int slot = 0;
+        float4 next_o = make_float4(0, 0, 0, 0), next_d = make_float4(0, 0, 0, 0);
+        if (active) {
+            slot = q[i];
+            const float4 ro = p.ray_o[slot], rd = p.ray_d[slot], h = p.hit[slot];
+            float4 tp = p.thr[slot];
+            float4 rp = p.rad[slot];
+            const uint2 info = p.info[slot];
+            const int depth = int(info.y & 0xffu);
+            int scatter = int(info.y >> 8);
+            Pcg rng = load_rng(p, slot, info.x);
+            f3 T = xyz(tp);
+            const float prev_pdf = tp.w;
+            f3 L = xyz(rp);
+            const f3 o = xyz(ro), d = xyz(rd);
+            const int tri = __float_as_int(h.y);
+            bool alive = true;
+            if (tri < 0) {
+                // miss: constant environment (environment_light.cpp:179-188)
+                if (s.has_env) {
+                    float w = 1.0f;
+                    if (prev_pdf > 0.0f) w = power_heuristic(prev_pdf, s.light_pmf[s.num_area_lights] * (1.0f / (4.0f * kPi)));
+                    L += T * s.env_L * w;
+                }
+                alive = false;
+            } else {
+                const float t = h.x;
+                const float4* tv = s.tr.tris + size_t(tri) * 3;
+                const f3 v0 = xyz(tv[0]), v1 = xyz(tv[1]), v2 = xyz(tv[2]);
+                const f3 ng = normalize(cross(v1 - v0, v2 - v0));
+                const TriShade sr = load_shade(s, tri);
+                const float dn = dot(d, ng);
+                // segment medium: the side of the hit surface the ray arrives from
+                const int mid = dn <= 0.0f ? sr.med_out : sr.med_in;
+                bool scattered = false;
+                f3 sp_pos;
+                if (mid >= 0) {
+                    const DevMedium m = s.media[mid];
+                    if (depth >= c.max_depth || scatter >= m.max_scatter) {
+                        T *= mk(expf(-m.sigma_t.x * t), expf(-m.sigma_t.y * t), expf(-m.sigma_t.z * t));
+                        if (scatter >= m.max_scatter) scatter = 0;
+                    } else {
+                        // delta / spectral tracking with the majorant
+                        float tt = 0.0f;
+                        for (int k = 0; k < 100000; ++k) {
+                            tt -= logf(1.0f - rng.uniform()) / m.majorant;
+                            if (tt >= t) break;
+                            const float xi = rng.uniform();
+                            if (xi < m.p_absorb) { alive = false; break; }
+                            if (xi < m.p_absorb + m.p_scatter) {
+                                T *= m.sigma_s * (1.0f / (m.majorant * m.p_scatter));
+                                scattered = true;
+                                sp_pos = o + d * tt;
+                                break;
+                            }
+                            T *= (f3s(m.majorant) - m.sigma_t) * (1.0f / (m.majorant * m.p_null));
+                        }
+                    }
+                }
+                if (!alive) {
+                } else if (scattered) {
+                    // ---- medium vertex (path_trace_renderer.cpp:94-115) ----
+                    const DevMedium m = s.media[mid];
+                    ++scatter;
+                    const f3 wo = -d;
+                    if (s.num_light_entries > 0) {
+                        float pmf;
+                        const int e = pick_light(s, rng.uniform(), pmf);
+                        const float lu = rng.uniform(), lv = rng.uniform();
+                        const LightSampleD ls = sample_light(s, e, sp_pos, lu, lv);
+                        if (ls.pdf != 0.0f && pmf > 0.0f && !is_black(ls.L)) {
+                            const float ph = hg_phase(m.g, -dot(ls.wi, wo));
+                            float w = 1.0f;
+                            float lp = pmf;
+                            if (ls.pdf > 0.0f) { lp *= ls.pdf; w = power_heuristic(lp, ph); }
+                            f3 trs = f3s(1.0f);
+                            if (ls.area) trs = shadow_tr(m, ls.dist, c.shadow_tr, rng);
+                            const f3 contrib = T * ls.L * trs * (ph * w / lp);
+                            if (max3(contrib) > 0.0f && finite3(contrib)) {
+                                sh.emit = true;
+                                sh.o = sp_pos;
+                                sh.tmin = kEps;
+                                sh.d = ls.wi;
+                                sh.tmax = ls.area ? ls.dist - kEps : ls.dist;
+                                sh.contrib = contrib;
+                            }
+                        }
+                    }
+                    float ppdf;
+                    const f3 wi = hg_sample(m.g, wo, rng.uniform(), rng.uniform(), ppdf);
+                    // f / pdf = 1 for the phase function
+                    next_o = f4(sp_pos, 0.0f);
+                    next_d = f4(wi, 3.0e38f);
+                    tp = f4(T, ppdf);
+                    cont = true;
+                } else {
+                    // ---- surface vertex ----
+                    if (sr.light >= 0 && dn < 0.0f) {
+                        // emission with MIS against light sampling (direct_illumination.cpp:176-197)
+                        const float4 l0 = s.lights[2 * sr.light], l1 = s.lights[2 * sr.light + 1];
+                        float w = 1.0f;
+                        if (prev_pdf > 0.0f) {
+                            const float lp = l1.y * t * t / (l0.w * (-dn));
+                            w = power_heuristic(prev_pdf, lp);
+                        }
+                        L += T * xyz(l0) * w;
+                    }
+                    if (depth >= c.max_depth) {
+                        alive = false;
+                    } else {
+                        const float b1 = h.z, b2 = h.w;
+                        const f3 pos = o + d * t;
+                        const f3 ns = normalize(sr.n0 * (1.0f - b1 - b2) + sr.n1 * b1 + sr.n2 * b2);
+                        f3 tx, ty;
+                        onb(ns, tx, ty);
+                        auto to_local = [&](f3 v) { return mk(dot(v, tx), dot(v, ty), dot(v, ns)); };
+                        const DevMaterial mat = s.mats[sr.material];
+                        const f3 wo = -d;
+                        const f3 lwo = to_local(wo), lng = to_local(ng);
+                        const float su = rng.uniform(), u1 = rng.uniform(), u2 = rng.uniform();
+                        const BsdfSample bs = bsdf_sample(mat, lwo, lng, su, u1, u2);
+                        if (is_black(bs.f) || bs.pdf < kEps) {
+                            alive = false;
+                        } else {
+                            const f3 wi = normalize(tx * bs.wi.x + ty * bs.wi.y + ns * bs.wi.z);
+                            // next-event estimation (direct_illumination.cpp:35-109)
+                            if (s.num_light_entries > 0) {
+                                float pmf;
+                                const int e = pick_light(s, rng.uniform(), pmf);
+                                const float lu = rng.uniform(), lv = rng.uniform();
+                                const LightSampleD ls = sample_light(s, e, pos, lu, lv);
+                                if (ls.pdf != 0.0f && pmf > 0.0f && !is_black(ls.L)) {
+                                    const float dist = ls.area ? ls.dist - kEps : ls.dist;
+                                    if (dist > kEps) {
+                                        const BsdfEval be = bsdf_eval(mat, to_local(ls.wi), lwo, lng);
+                                        if (!is_black(be.f)) {
+                                            float w = 1.0f;
+                                            float lp = pmf;
+                                            if (ls.pdf > 0.0f) { lp *= ls.pdf; w = power_heuristic(lp, be.pdf); }
+                                            f3 trs = f3s(1.0f);
+                                            if (ls.area) {
+                                                const int lm = dot(ls.wi, ng) >= 0.0f ? sr.med_out : sr.med_in;
+                                                if (lm >= 0) trs = shadow_tr(s.media[lm], ls.dist, c.shadow_tr, rng);
+                                            }
+                                            const float cosg = fabsf(dot(ls.wi, ng));
+                                            const f3 contrib = T * ls.L * be.f * trs * (cosg * w / lp);
+                                            if (max3(contrib) > 0.0f && finite3(contrib)) {
+                                                sh.emit = true;
+                                                sh.o = pos;
+                                                sh.tmin = kEps;
+                                                sh.d = ls.wi;
+                                                sh.tmax = dist;
+                                                sh.contrib = contrib;
+                                            }
+                                        }
+                                    }
+                                }
+                            }
+                            // continuation (path_trace_renderer.cpp:155-163)
+                            T *= bs.f * (fabsf(dot(ng, wi)) / bs.pdf);
+                            const f3 org = dot(ng, wi) > 0.0f ? pos + ng * kEps : pos - ng * kEps;
+                            next_o = f4(org, 0.0f);
+                            next_d = f4(wi, 3.0e38f);
+                            tp = f4(T, bs.pdf);
+                            cont = finite3(T);
+                        }
+                    }
+                }
+            }
+            if (cont) {
+                // Russian roulette before the next segment (path_trace_renderer.cpp:54-58)
+                if (depth + 1 > c.min_depth) {
+                    if (rng.uniform() > 0.9f) cont = false;
+                    else { tp.x /= 0.9f; tp.y /= 0.9f; tp.z /= 0.9f; }
+                }
+            }
+            p.rad[slot] = f4(L, rp.w);
+            if (cont) {
+                p.ray_o[slot] = next_o;
+                p.ray_d[slot] = next_d;
+                p.thr[slot] = tp;
+                p.info[slot] = make_uint2(info.x, pack_info(depth + 1, scatter));
+                store_rng(p, slot, rng);
+            }
+        }
+        const unsigned int qi = wave_append(&p.counters[1 - cur], cont);
+        if (cont) qn[qi] = slot;
+        const unsigned int si = wave_append(&p.counters[2], sh.emit);
+        if (sh.emit) {
+            p.sh_o[si] = f4(sh.o, sh.tmin);
+            p.sh_d[si] = f4(sh.d, sh.tmax);
+            p.sh_c[si] = f4(sh.contrib, __int_as_float(slot));
+        }
+    }
+}
+
+// Film accumulation for one pass (Tile::add_sample + Film::merge_film_tile,
+// src/core/render.hpp:37-57, 90-100): every pixel gathers the samples of the
+// pass that its filter footprint covers, in a fixed order, into FP64 sums —
+// deterministic without atomics.  Sample positions are re-derived from the
+// sample's RNG stream (its first two numbers).
+__global__ __launch_bounds__(256) void k_accumulate(DevPaths p, RenderConsts c, double4* film) {
+    const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+    const int npix = c.width * c.height;
+    if (pix >= npix) return;
+    const int px = pix % c.width, py = pix / c.width;
+    double4 acc = film[pix];
+    unsigned long long bad = 0;
+    const bool own_only = c.filter == 0 ? c.filter_radius <= 0.5f : c.filter_radius <= 0.5f;
+    const float e = expf(-c.filter_alpha * c.filter_radius * c.filter_radius);
+    for (int sidx = 0; sidx < c.spp_in_pass; ++sidx) {
+        for (int dy = -1; dy <= 1; ++dy) {
+            for (int dx = -1; dx <= 1; ++dx) {
+                if (own_only && (dx != 0 || dy != 0)) continue;
+                const int qx = px + dx, qy = py + dy;
+                if (qx < 0 || qy < 0 || qx >= c.width || qy >= c.height) continue;
+                const int qpix = qy * c.width + qx;
+                const int slot = sidx * npix + qpix;
+                const float4 L = p.rad[slot];
+                const bool fin = isfinite(L.x) && isfinite(L.y) && isfinite(L.z);
+                if (dx == 0 && dy == 0 && !fin) ++bad;
+                if (!fin) continue;
+                float w = 1.0f;
+                if (!(c.filter == 0 && own_only)) {
+                    Pcg rng = pcg_for_sample(c.seed, uint32_t(qpix), uint32_t(c.sample_base + sidx));
+                    const float fu = rng.uniform(), fv = rng.uniform();
+                    const float ox = float(qx) + fu - (float(px) + 0.5f), oy = float(qy) + fv - (float(py) + 0.5f);
+                    if (c.filter == 0) {
+                        w = (fabsf(ox) <= c.filter_radius && fabsf(oy) <= c.filter_radius) ? 1.0f : 0.0f;
+                    } else {
+                        // GaussianFilter::eval (src/filter/gaussian_filter.cpp:20-30)
+                        const float gx = fmaxf(0.0f, expf(-c.filter_alpha * ox * ox) - e);
+                        const float gy = fmaxf(0.0f, expf(-c.filter_alpha * oy * oy) - e);
+                        w = (fabsf(ox) <= c.filter_radius && fabsf(oy) <= c.filter_radius) ? gx * gy : 0.0f;
+                    }
+                }
+                if (w == 0.0f) continue;
+                acc.x += double(L.x) * w;
+                acc.y += double(L.y) * w;
+                acc.z += double(L.z) * w;
+                acc.w += double(w);
+            }
+        }
+    }
+    film[pix] = acc;
+    if (bad) atomicAdd(&p.stats[2], bad);
+}
+
+__global__ void k_resolve(const double4* film, float* image, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double4 f = film[i];
+    float r = 0.0f, g = 0.0f, b = 0.0f;
+    if (f.w > 0.0) { r = float(f.x / f.w); g = float(f.y / f.w); b = float(f.z / f.w); }
+    image[3 * i] = r; image[3 * i + 1] = g; image[3 * i + 2] = b;
+}
+
+__global__ void k_bsdf_eval(const DevMaterial* mats, const int* ids, const float* wi, const float* wo, const float* ng,
+                            int64_t n, float* f, float* pdf) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const DevMaterial m = mats[ids[i]];
+    const f3 g = ng ? mk(ng[3 * i], ng[3 * i + 1], ng[3 * i + 2]) : mk(0.0f, 0.0f, 1.0f);
+    const BsdfEval e = bsdf_eval(m, mk(wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]), mk(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]), g);
+    f[3 * i] = e.f.x; f[3 * i + 1] = e.f.y; f[3 * i + 2] = e.f.z;
+    pdf[i] = e.pdf;
+}
+
+__global__ void k_bsdf_sample(const DevMaterial* mats, const int* ids, const float* wo, const float* ng, const float* u,
+                              int64_t n, float* wi, float* f, float* pdf) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const DevMaterial m = mats[ids[i]];
+    const f3 g = ng ? mk(ng[3 * i], ng[3 * i + 1], ng[3 * i + 2]) : mk(0.0f, 0.0f, 1.0f);
+    const BsdfSample s = bsdf_sample(m, mk(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]), g, u[3 * i], u[3 * i + 1], u[3 * i + 2]);
+    const bool ok = s.pdf > 0.0f && !is_black(s.f);
+    wi[3 * i] = s.wi.x; wi[3 * i + 1] = s.wi.y; wi[3 * i + 2] = s.wi.z;
+    f[3 * i] = s.f.x; f[3 * i + 1] = s.f.y; f[3 * i + 2] = s.f.z;
+    pdf[i] = ok ? s.pdf : 0.0f;
+}
+
+static int shade_blocks(int max_paths) {
+    int b = (max_paths + kShadeBlock - 1) / kShadeBlock;
+    if (b < 1) b = 1;
+    if (b > 4096) b = 4096;
+    return b;
+}
+
+void launch_generate(const DevScene& s, const DevPaths& p, const RenderConsts& c, int num_paths, int cur, hipStream_t st) {
+    const int blocks = (num_paths + 255) / 256;
+    hipLaunchKernelGGL(k_generate, dim3(blocks > 0 ? blocks : 1), dim3(256), 0, st, s, p, c, num_paths, cur);
+}
+void launch_shade(const DevScene& s, const DevPaths& p, const RenderConsts& c, int cur, int max_paths, hipStream_t st) {
+    hipLaunchKernelGGL(k_shade, dim3(shade_blocks(max_paths)), dim3(kShadeBlock), 0, st, s, p, c, cur);
+}
+void launch_accumulate(const DevPaths& p, const RenderConsts& c, double4* film, hipStream_t st) {
+    const int n = c.width * c.height;
+    hipLaunchKernelGGL(k_accumulate, dim3((n + 255) / 256), dim3(256), 0, st, p, c, film);
+}
+void launch_resolve(const double4* film, float* image, int num_pixels, hipStream_t st) {
+    hipLaunchKernelGGL(k_resolve, dim3((num_pixels + 255) / 256), dim3(256), 0, st, film, image, num_pixels);
+}
+void launch_bsdf_eval(const DevMaterial* mats, const int* ids, const float* wi, const float* wo, const float* ng, int64_t n,
+                      float* f, float* pdf, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_bsdf_eval, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, mats, ids, wi, wo, ng, n, f, pdf);
+}
+void launch_bsdf_sample(const DevMaterial* mats, const int* ids, const float* wo, const float* ng, const float* u, int64_t n,
+                        float* wi, float* f, float* pdf, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_bsdf_sample, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, mats, ids, wo, ng, u, n, wi, f, pdf);
+}
+
+}  // namespace trhip

@@ -1,0 +1,357 @@
+// Materials on the device: Disney BSDF and Lambert, evaluated in the shading
+// frame (shading normal = +z) with the geometric normal given in that frame.
+//
+// Replaces DisneyBSDF (src/material/disney.cpp:14-548) with its helpers
+// (src/material/utility/microfacet.cpp, src/utility/reflection.hpp) and the
+// Phong(ks = 0) Lambert AggregateBSDF (src/material/phong.cpp:29-70,
+// src/material/aggregate_bsdf.hpp, src/material/bxdf/diffuse_bxdf.cpp), plus
+// the black-fringe handling of LocalBSDF (src/core/bsdf.hpp:114-165).
+//
+// The formulas are the reference's, rewritten for FP32 accuracy and without
+// transcendental round trips:
+//   * sin/cos of a direction's azimuth come from x/r and y/r instead of
+//     sin(atan2(y, x)) and cos(atan2(y, x)) (disney.cpp:74-89, algebraically
+//     identical);
+//   * sin^2(theta) comes from x^2 + y^2 instead of 1 - cos^2 (no
+//     cancellation near the pole), so D and G keep their relative accuracy
+//     at roughness 0.05 (alpha = 0.0025).
+// Sampling is exclusive lobe selection with the macro-Fresnel split for
+// lwo.z < 0, transmission pdf uses cos(theta_h), refraction is about the
+// sampled microfacet normal, and Lambert eval carries the shading-normal
+// correction that its sample() applies — the corrections that make the
+// returned pdf the sampling density (reference defects Q2, Q3, Q21, Q22).
+#pragma once
+#include "tr_device.h"
+
+namespace trhip {
+
+enum : int { MAT_LAMBERT = 0, MAT_DISNEY = 1 };
+
+// Per-material constants, precomputed on the host from the create_disney
+// parameters exactly as the DisneyBSDF constructor does (disney.cpp:347-399).
+struct DevMaterial {
+    int type;
+    float metallic, roughness, transmission;
+    f3 C; float ior;
+    f3 Ctint; float sheen;
+    f3 spec_scale; float sheen_tint;
+    float spec_tint, clearcoat, ax, ay;
+    float tax, tay, cc_rough, w_diffuse;
+    float w_trans, w_spec, w_cc, pad;
+};
+
+struct BsdfSample {
+    f3 wi;
+    f3 f;
+    float pdf;
+};
+
+TR_D float pow5(float x) { const float t2 = x * x; return t2 * t2 * x; }
+TR_D float one_minus_5(float c) { return pow5(1.0f - c); }
+
+// reflection.hpp:16-40
+TR_D float fresnel_dielectric(float eta_i, float eta_o, float cos_i) {
+    if (cos_i < 0.0f) { const float t = eta_i; eta_i = eta_o; eta_o = t; cos_i = -cos_i; }
+    const float sin_i = sqrtf(fmaxf(0.0f, 1.0f - cos_i * cos_i));
+    const float sin_t = eta_o / eta_i * sin_i;
+    if (sin_t >= 1.0f) return 1.0f;
+    const float cos_t = sqrtf(fmaxf(0.0f, 1.0f - sin_t * sin_t));
+    const float para = (eta_i * cos_i - eta_o * cos_t) / (eta_i * cos_i + eta_o * cos_t);
+    const float perp = (eta_o * cos_i - eta_i * cos_t) / (eta_o * cos_i + eta_i * cos_t);
+    return 0.5f * (para * para + perp * perp);
+}
+
+// Anisotropic GTR2 (GGX) normal distribution, microfacet.cpp:15-21, for a
+// unit half vector h: 1 / (pi ax ay (hx^2/ax^2 + hy^2/ay^2 + hz^2)^2).
+TR_D float D_gtr2(f3 h, float ax, float ay) {
+    const float x = h.x / ax, y = h.y / ay;
+    const float s = x * x + y * y + h.z * h.z;
+    return 1.0f / (kPi * ax * ay * s * s);
+}
+// Smith G1 for anisotropic GTR2, microfacet.cpp:38-43:
+// 2 / (1 + sqrt(1 + (ax^2 x^2 + ay^2 y^2) / z^2)).
+TR_D float G1_gtr2(f3 w, float ax, float ay) {
+    const float r2 = w.x * w.x + w.y * w.y;
+    if (r2 <= 0.0f) return 1.0f;
+    const float a = ax * w.x, b = ay * w.y;
+    const float t = (a * a + b * b) / (w.z * w.z);
+    return 2.0f / (1.0f + sqrtf(1.0f + t));
+}
+// GTR1 (clearcoat) distribution, microfacet.cpp:29-35.
+TR_D float D_gtr1(f3 h, float alpha) {
+    if (alpha >= 1.0f) return kInvPi;
+    const float a2 = alpha * alpha;
+    const float sin2 = h.x * h.x + h.y * h.y;
+    return (a2 - 1.0f) / (2.0f * kPi * logf(alpha) * (a2 * h.z * h.z + sin2));
+}
+// Smith G for GTR2 with fixed alpha, microfacet.cpp:46-51.
+TR_D float G1_smith(f3 w, float alpha) {
+    const float r2 = w.x * w.x + w.y * w.y;
+    if (r2 <= 0.0f) return 1.0f;
+    const float t2 = r2 / (w.z * w.z);
+    return 2.0f / (1.0f + sqrtf(1.0f + alpha * alpha * t2));
+}
+
+// microfacet.cpp:58-80 (Heitz 2014 visible-normal sampling)
+TR_D f3 sample_vndf(f3 ve, float ax, float ay, float u, float v) {
+    const f3 vh = normalize(mk(ax * ve.x, ay * ve.y, ve.z));
+    const float lensq = vh.x * vh.x + vh.y * vh.y;
+    const f3 t1 = lensq > kEps ? mk(-vh.y, vh.x, 0.0f) * (1.0f / sqrtf(lensq)) : mk(1.0f, 0.0f, 0.0f);
+    const f3 t2 = cross(vh, t1);
+    const float r = sqrtf(u);
+    float sp, cp;
+    sincosf(2.0f * kPi * v, &sp, &cp);
+    const float p1 = r * cp;
+    const float p2r = r * sp;
+    const float s = 0.5f * (1.0f + vh.z);
+    const float p2 = (1.0f - s) * sqrtf(1.0f - p1 * p1) + s * p2r;
+    const f3 nh = t1 * p1 + t2 * p2 + vh * sqrtf(fmaxf(0.0f, 1.0f - p1 * p1 - p2 * p2));
+    return normalize(mk(ax * nh.x, ay * nh.y, fmaxf(0.0f, nh.z)));
+}
+// microfacet.cpp:92-109 (D * cos sampling of anisotropic GTR2)
+TR_D f3 sample_gtr2_dcos(float ax, float ay, float u, float v) {
+    float s, c;
+    sincosf(2.0f * kPi * u, &s, &c);
+    float sp = ay * s, cp = ax * c;
+    const float nor = 1.0f / sqrtf(sp * sp + cp * cp);
+    sp *= nor;
+    cp *= nor;
+    const float A = (cp / ax) * (cp / ax) + (sp / ay) * (sp / ay);
+    const float ct = sqrtf(A * (1.0f - v) / ((1.0f - A) * v + A));
+    const float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+    return normalize(mk(st * cp, st * sp, ct));
+}
+// microfacet.cpp:125-134
+TR_D f3 sample_gtr1(float alpha, float u, float v) {
+    float s, c;
+    sincosf(2.0f * kPi * u, &s, &c);
+    const float a2 = alpha * alpha;
+    const float ct = sqrtf((exp2f((2.0f - 2.0f * v) * log2f(alpha)) - 1.0f) / (a2 - 1.0f));
+    const float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+    return normalize(mk(st * c, st * s, ct));
+}
+
+// ---- Disney lobes (local frame) -------------------------------------------
+// disney.cpp:60-109
+TR_D f3 dis_f_transmission(const DevMaterial& m, f3 wi, f3 wo) {
+    const float ci = wi.z, co = wo.z;
+    const float eta = co > 0.0f ? m.ior : 1.0f / m.ior;
+    f3 h = normalize(wo + wi * eta);
+    if (h.z < 0.0f) h = -h;
+    const float dwo = dot(wo, h), dwi = dot(wi, h);
+    const float F = fresnel_dielectric(m.ior, 1.0f, dwo);
+    const float D = D_gtr2(h, m.tax, m.tay);
+    const float G = G1_gtr2(wi, m.tax, m.tay) * G1_gtr2(wo, m.tax, m.tay);
+    const float sdem = dwo + eta * dwi;
+    // eta^2 * (1/eta)^2 = 1 for radiance transport (disney.cpp:98,102-105)
+    const float val = (1.0f - F) * D * G * dwi * dwo / (ci * co * sdem * sdem);
+    const float tf = co > 0.0f ? m.transmission : 1.0f;
+    const float k = (1.0f - m.metallic) * tf * fabsf(val);
+    return mk(sqrtf(m.C.x), sqrtf(m.C.y), sqrtf(m.C.z)) * k;
+}
+// disney.cpp:110-145
+TR_D f3 dis_f_inner(const DevMaterial& m, f3 wi, f3 wo) {
+    const f3 h = -normalize(wi + wo);
+    const float F = fresnel_dielectric(m.ior, 1.0f, dot(wo, h));
+    const float D = D_gtr2(h, m.tax, m.tay);
+    const float G = G1_gtr2(wi, m.tax, m.tay) * G1_gtr2(wo, m.tax, m.tay);
+    return m.C * (m.transmission * fabsf(F * D * G / (4.0f * wi.z * wo.z)));
+}
+// disney.cpp:146-209 (outside reflection: diffuse + retro, sheen, specular, clearcoat)
+TR_D f3 dis_f_reflect(const DevMaterial& m, f3 wi, f3 wo) {
+    const float ci = wi.z, co = wo.z;
+    const f3 h = normalize(wi + wo);
+    const float cd = dot(wi, h);
+    const float Fcd = one_minus_5(cd);
+    f3 diffuse = f3s(0.0f), sheen = f3s(0.0f);
+    if (m.metallic < 1.0f) {
+        const float FL = one_minus_5(ci), FV = one_minus_5(co);
+        const float RR = 2.0f * m.roughness * cd * cd;
+        const float k = kInvPi * ((1.0f - 0.5f * FL) * (1.0f - 0.5f * FV) + RR * (FL + FV + FL * FV * (RR - 1.0f)));
+        diffuse = m.C * k;
+        if (m.sheen > 0.0f) sheen = lerp3(f3s(1.0f), m.Ctint, m.sheen_tint) * (4.0f * m.sheen * Fcd);
+    }
+    const f3 Cspec = lerp3(lerp3(f3s(1.0f), m.Ctint, m.spec_tint), m.C, m.metallic);
+    const float Fd = fresnel_dielectric(m.ior, 1.0f, cd);
+    const f3 cond = Cspec + (f3s(1.0f) - Cspec) * Fcd;
+    const f3 F = lerp3(m.spec_scale * Cspec * Fd, cond, m.metallic);
+    const float D = D_gtr2(h, m.ax, m.ay);
+    const float G = G1_gtr2(wi, m.ax, m.ay) * G1_gtr2(wo, m.ax, m.ay);
+    f3 r = (diffuse + sheen) * ((1.0f - m.metallic) * (1.0f - m.transmission)) + F * (D * G / fabsf(4.0f * ci * co));
+    if (m.clearcoat > 0.0f) {
+        const float Dc = D_gtr1(h, m.cc_rough);
+        const float Fc = 0.04f + 0.96f * Fcd;
+        const float Gc = G1_smith(wi, 0.25f) * G1_smith(wo, 0.25f);
+        r += f3s(m.clearcoat * Dc * Fc * Gc / fabsf(4.0f * ci * co));
+    }
+    return r;
+}
+// disney.cpp:210-235, with cos(theta_h) as the D * cos sampling density
+TR_D float dis_pdf_transmission(const DevMaterial& m, f3 wi, f3 wo) {
+    const float eta = wo.z > 0.0f ? m.ior : 1.0f / m.ior;
+    f3 h = normalize(wo + wi * eta);
+    if (h.z < 0.0f) h = -h;
+    const float dwo = dot(wo, h), dwi = dot(wi, h);
+    if (((wo.z > 0.0f) != (dwo > 0.0f)) || ((wi.z > 0.0f) != (dwi > 0.0f))) return 0.0f;
+    const float sdem = dwo + eta * dwi;
+    const float jac = eta * eta * dwi / (sdem * sdem);
+    return fabsf(h.z * D_gtr2(h, m.tax, m.tay) * jac);
+}
+// disney.cpp:236-251
+TR_D float dis_pdf_inner(const DevMaterial& m, f3 wi, f3 wo) {
+    const f3 h = -normalize(wi + wo);
+    return fabsf(h.z * D_gtr2(h, m.tax, m.tay) / (4.0f * dot(wi, h)));
+}
+// disney.cpp:499-534 (local, |z| >= eps already checked)
+TR_D float dis_pdf_local(const DevMaterial& m, f3 wi, f3 wo) {
+    if (wo.z < 0.0f) {
+        if (m.transmission == 0.0f) return 0.0f;
+        const float F = fminf(fmaxf(fresnel_dielectric(m.ior, 1.0f, wo.z), 0.1f), 0.9f);
+        if (wi.z > 0.0f) return (1.0f - F) * dis_pdf_transmission(m, wi, wo);
+        return F * dis_pdf_inner(m, wi, wo);
+    }
+    if (wi.z < 0.0f) return m.w_trans * dis_pdf_transmission(m, wi, wo);
+    const f3 h = normalize(wi + wo);
+    const float pd = wi.z * kInvPi;
+    const float ps = G1_gtr2(wo, m.ax, m.ay) * D_gtr2(h, m.ax, m.ay) / (4.0f * wo.z);
+    const float pc = h.z * D_gtr1(h, m.cc_rough) / (4.0f * dot(wi, h));
+    return m.w_diffuse * pd + m.w_spec * ps + m.w_cc * pc;
+}
+TR_D f3 dis_f_local(const DevMaterial& m, f3 wi, f3 wo) {
+    if (wi.z * wo.z < 0.0f) {
+        if (m.transmission == 0.0f) return f3s(0.0f);
+        return dis_f_transmission(m, wi, wo);
+    }
+    if (wi.z < 0.0f && wo.z < 0.0f) {
+        if (m.transmission == 0.0f) return f3s(0.0f);
+        return dis_f_inner(m, wi, wo);
+    }
+    return dis_f_reflect(m, wi, wo);
+}
+// disney.cpp:285-305 (refraction about the sampled normal)
+TR_D bool dis_sample_transmission(const DevMaterial& m, f3 wo, float u, float v, f3& wi) {
+    const f3 h = sample_gtr2_dcos(m.tax, m.tay, u, v);
+    if (h.z <= 0.0f) return false;
+    const float dho = dot(h, wo);
+    if ((wo.z > 0.0f) != (dho > 0.0f)) return false;
+    const float eta = wo.z > 0.0f ? 1.0f / m.ior : m.ior;
+    const f3 n = dho > 0.0f ? h : -h;
+    const float ci = fabsf(dho);
+    const float st2 = eta * eta * fmaxf(0.0f, 1.0f - ci * ci);
+    if (st2 >= 1.0f) return false;
+    const float ct = sqrtf(1.0f - st2);
+    wi = normalize(n * (eta * ci - ct) - wo * eta);
+    if (wi.z * wo.z > 0.0f || ((wi.z > 0.0f) != (dot(h, wi) > 0.0f))) return false;
+    return true;
+}
+TR_D bool dis_sample_local(const DevMaterial& m, f3 wo, float su, float u, float v, f3& wi) {
+    if (wo.z < 0.0f) {
+        if (m.transmission == 0.0f) return false;
+        const float F = fminf(fmaxf(fresnel_dielectric(m.ior, 1.0f, wo.z), 0.1f), 0.9f);
+        if (su < F) {
+            const f3 h = sample_gtr2_dcos(m.tax, m.tay, u, v);
+            if (h.z <= 0.0f) return false;
+            const f3 w = h * (2.0f * dot(wo, h)) - wo;
+            if (w.z > 0.0f) return false;
+            wi = normalize(w);
+            return true;
+        }
+        return dis_sample_transmission(m, wo, u, v, wi);
+    }
+    float sel = su;
+    if (sel < m.w_diffuse) { wi = cosine_hemisphere(u, v); return true; }
+    sel -= m.w_diffuse;
+    if (sel < m.w_trans) return dis_sample_transmission(m, wo, u, v, wi);
+    sel -= m.w_trans;
+    f3 h;
+    if (sel < m.w_spec) h = normalize(sample_vndf(wo, m.ax, m.ay, u, v));
+    else h = sample_gtr1(m.cc_rough, u, v);
+    if (h.z <= 0.0f) return false;
+    wi = normalize(h * (2.0f * dot(wo, h)) - wo);
+    return wi.z > 0.0f;
+}
+
+// ---- full BSDF: fringe handling, local normalisation, normal correction ----
+TR_D bool fringe(f3 w, f3 ng) { return (dot(w, ng) > 0.0f) != (w.z > 0.0f); }
+// coordinate.hpp:43-47 with the shading normal = +z
+TR_D float normal_correct(f3 ng, f3 wi) {
+    const float lw = length(wi);
+    const float dem = fabsf(dot(ng, wi) / (length(ng) * lw));
+    return dem < kEps ? 1.0f : fabsf(wi.z / lw) / dem;
+}
+
+struct BsdfEval {
+    f3 f;
+    float pdf;
+};
+
+// eval + pdf for directions wi, wo (shading frame), geometric normal ng.
+TR_D BsdfEval bsdf_eval(const DevMaterial& m, f3 wi, f3 wo, f3 ng) {
+    BsdfEval r{f3s(0.0f), 0.0f};
+    if (fringe(wi, ng) || fringe(wo, ng)) {
+        const bool gi = dot(wi, ng) > 0.0f, go = dot(wo, ng) > 0.0f;
+        if (gi && go) r.f = m.C * kInvPi;
+        if (gi == go) r.pdf = dot(wi, ng) / (length(wi) * length(ng)) * kInvPi;
+        return r;
+    }
+    const f3 lwi = normalize(wi), lwo = normalize(wo);
+    if (m.type == MAT_LAMBERT) {
+        if (lwi.z <= 0.0f || lwo.z <= 0.0f) return r;
+        r.f = m.C * (kInvPi * normal_correct(ng, wi));
+        r.pdf = lwi.z * kInvPi;
+        return r;
+    }
+    if (fabsf(lwi.z) < kEps || fabsf(lwo.z) < kEps) return r;
+    r.f = dis_f_local(m, lwi, lwo) * normal_correct(ng, wi);
+    r.pdf = dis_pdf_local(m, lwi, lwo);
+    return r;
+}
+
+// Sample wi given wo; returns pdf = 0 when no valid direction was produced.
+// (gt, gb): a tangent frame around ng for the fringe fallback.
+TR_D BsdfSample bsdf_sample(const DevMaterial& m, f3 wo, f3 ng, float su, float u, float v) {
+    BsdfSample s{f3s(0.0f), f3s(0.0f), 0.0f};
+    if (fringe(wo, ng)) {
+        // bsdf.hpp:141-153: cosine sampling about the geometric normal
+        if (!(dot(wo, ng) > 0.0f)) return s;
+        const f3 l = cosine_hemisphere(u, v);
+        const float pd = l.z * kInvPi;
+        if (pd < kEps) return s;
+        const f3 n = normalize(ng);
+        f3 t, b;
+        onb(n, t, b);
+        const f3 wi = normalize(t * l.x + b * l.y + n * l.z);
+        s.wi = wi;
+        s.f = m.C * (kInvPi * normal_correct(ng, wi));
+        s.pdf = pd;
+        return s;
+    }
+    const f3 lwo = normalize(wo);
+    f3 wi;
+    if (m.type == MAT_LAMBERT) {
+        if (lwo.z <= 0.0f) return s;
+        wi = cosine_hemisphere(u, v);
+        const float pd = wi.z * kInvPi;
+        if (pd < 0.001f) return s;   // diffuse_bxdf.cpp:25-26
+        wi = normalize(wi);
+        if (fringe(wi, ng)) {
+            // both hemispheres disagree for wi: evaluate through the fringe path
+            const BsdfEval e = bsdf_eval(m, wi, wo, ng);
+            s.wi = wi; s.f = e.f; s.pdf = e.pdf;
+            return s;
+        }
+        s.wi = wi;
+        s.f = m.C * (kInvPi * normal_correct(ng, wi));
+        s.pdf = pd;
+        return s;
+    }
+    if (fabsf(lwo.z) < kEps) return s;
+    if (!dis_sample_local(m, lwo, su, u, v, wi)) return s;
+    const BsdfEval e = bsdf_eval(m, wi, wo, ng);
+    s.wi = wi;
+    s.f = e.f;
+    s.pdf = e.pdf;
+    return s;
+}
+
+}  // namespace trhip

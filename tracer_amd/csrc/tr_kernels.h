@@ -1,0 +1,68 @@
+// Host-side launchers for the kernels in k_trace.hip and k_render.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include "tr_scene.h"
+
+namespace trhip {
+
+constexpr int kTraceBlock = 256;
+constexpr int kLdsStack = 16;
+constexpr int kShadeBlock = 256;
+
+// Path state of the wavefront (one slot per camera sample in flight).
+struct DevPaths {
+    float4* ray_o;    // (origin, tmin)
+    float4* ray_d;    // (direction, tmax)
+    float4* hit;      // (t, leaf-ordered triangle, b1, b2)
+    float4* thr;      // (throughput rgb, pdf of the sampled direction; 0 = camera ray)
+    float4* rad;      // (radiance rgb, -)
+    uint2* rng;       // PCG32 state
+    uint2* info;      // (pixel, depth | scatter count << 8)
+    int* queue[2];    // active path slots for the current / next bounce
+    float4* sh_o;     // shadow rays: (origin, tmin)
+    float4* sh_d;     //              (direction, tmax)
+    float4* sh_c;     //              (contribution rgb, path slot)
+    unsigned int* counters;   // [0..1] queue lengths, [2] shadow length
+    unsigned long long* stats;  // [0] extension rays, [1] shadow rays, [2] non-finite samples
+    int* ovf_node;    // traversal stack spill area
+    float* ovf_t;
+    int ovf_threads;  // threads the spill area is sized for
+};
+
+struct RenderConsts {
+    int width, height;
+    int min_depth, max_depth;
+    int spp_in_pass;     // samples per pixel in this pass
+    int sample_base;     // sample index of the pass's first sample
+    int filter;          // TR_FILTER_*
+    float filter_radius, filter_alpha;
+    int shadow_tr;       // TR_TR_*
+    int nls;             // light samples per vertex
+    unsigned long long seed;
+};
+
+// k_trace.hip
+void launch_trace_closest(const TraceScene& sc, const float4* rays, float4* hits, int64_t n, int* ovf_node,
+                          float* ovf_t, int ovf_threads, hipStream_t st);
+void launch_trace_any(const TraceScene& sc, const float4* rays, int* occ, int64_t n, int* ovf_node, float* ovf_t,
+                      int ovf_threads, hipStream_t st);
+void launch_trace_counted(const TraceScene& sc, const float4* rays, float4* hits, uint32_t* nvis, uint32_t* ntri,
+                          int64_t n, int* ovf_node, float* ovf_t, int ovf_threads, hipStream_t st);
+// wavefront: extend the current queue, trace the shadow queue
+void launch_extend(const TraceScene& sc, const DevPaths& p, int cur, int max_paths, hipStream_t st);
+void launch_shadow(const TraceScene& sc, const DevPaths& p, int max_shadow, int accumulate_atomic, hipStream_t st);
+// Grid size the spill area must cover for launches of up to n rays.
+int trace_grid_threads(int64_t n);
+
+// k_render.hip
+void launch_generate(const DevScene& s, const DevPaths& p, const RenderConsts& c, int num_paths, int cur, hipStream_t st);
+void launch_shade(const DevScene& s, const DevPaths& p, const RenderConsts& c, int cur, int max_paths, hipStream_t st);
+void launch_accumulate(const DevPaths& p, const RenderConsts& c, double4* film, hipStream_t st);
+void launch_resolve(const double4* film, float* image, int num_pixels, hipStream_t st);
+void launch_bsdf_eval(const DevMaterial* mats, const int* ids, const float* wi, const float* wo, const float* ng, int64_t n,
+                      float* f, float* pdf, hipStream_t st);
+void launch_bsdf_sample(const DevMaterial* mats, const int* ids, const float* wo, const float* ng, const float* u,
+                        int64_t n, float* wi, float* f, float* pdf, hipStream_t st);
+
+}  // namespace trhip

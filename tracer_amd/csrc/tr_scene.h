@@ -1,0 +1,177 @@
+// Device-resident scene: geometry for traversal, per-triangle shading
+// records, materials, media, lights and camera.  One struct, passed to the
+// kernels by value.
+#pragma once
+#include "tr_bsdf.h"
+#include "tr_trace.h"
+
+namespace trhip {
+
+// Homogeneous medium (src/medium/homogeneous.cpp:12-18) prepared for delta
+// tracking: majorant = max channel sigma_t, collision-type probabilities from
+// the channel-averaged coefficients (spectral tracking; for RGB-equal media
+// this is plain analogue free-flight sampling).
+struct DevMedium {
+    f3 sigma_s;
+    float g;
+    f3 sigma_t;
+    float majorant;
+    float p_scatter, p_absorb, p_null;
+    int max_scatter;
+};
+
+// ThinLensCamera (src/camera/thin_lens_camera.cpp:23-36) with look_at's
+// left-handed basis (src/utility/transform.cpp:96-124).
+struct DevCamera {
+    f3 pos, right, up, dir;
+    float film_w, film_h, focal, lens_radius;
+};
+
+struct DevScene {
+    TraceScene tr;
+    const float4* tri_shade;   // 3 float4 per leaf-ordered triangle: (n0, material), (n1, light), (n2, media)
+    const DevMaterial* mats;
+    const DevMedium* media;
+    const float4* lights;      // 2 float4 per area light: (L, area), (tri, pmf, -, -)
+    const float* light_cdf;    // over [area lights..., env, sun]
+    const float* light_pmf;
+    int num_area_lights;
+    int num_light_entries;
+    int has_env, has_sun;
+    f3 env_L;
+    f3 sun_dir;
+    f3 sun_E;
+    f3 wcenter;
+    float wradius;
+    DevCamera cam;
+};
+
+struct TriShade {
+    f3 n0, n1, n2;
+    int material, light, med_in, med_out;
+};
+TR_D TriShade load_shade(const DevScene& s, int tri) {
+    const float4* p = s.tri_shade + size_t(tri) * 3;
+    const float4 a = p[0], b = p[1], c = p[2];
+    TriShade r;
+    r.n0 = xyz(a); r.n1 = xyz(b); r.n2 = xyz(c);
+    r.material = __float_as_int(a.w);
+    r.light = __float_as_int(b.w);
+    const int med = __float_as_int(c.w);
+    r.med_in = (med << 16) >> 16;   // sign-extended low half
+    r.med_out = med >> 16;
+    return r;
+}
+
+// Light selection by power (compute_light_power_distribution,
+// src/renderer/direct_illumination.cpp:257-264; Distribution1D::sample_discrete,
+// src/utility/distribution.hpp:49-67): binary search in the CDF.
+TR_D int pick_light(const DevScene& s, float u, float& pmf) {
+    int lo = 0, hi = s.num_light_entries - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (s.light_cdf[mid] > u) hi = mid;
+        else lo = mid + 1;
+    }
+    while (lo > 0 && s.light_pmf[lo] == 0.0f) --lo;
+    pmf = s.light_pmf[lo];
+    return lo;
+}
+
+struct LightSampleD {
+    f3 pos;     // point on the light (area) or a far point (env / sun)
+    f3 wi;      // unit direction from the reference point
+    float dist; // distance to pos
+    f3 L;
+    float pdf;  // solid-angle pdf without selection; 0 = invalid; < 0 = delta light
+    bool area;
+};
+
+// DiffuseLight::sample_li (src/light/diffuse_light.cpp:44-60) over a triangle
+// (Triangle::sample, src/shape/triangle.cpp:61-97: b.x weights A, b.y weights B);
+// the environment is sampled uniformly over the sphere and the sun is a
+// delta direction.
+TR_D LightSampleD sample_light(const DevScene& s, int entry, f3 ref, float u, float v) {
+    LightSampleD r;
+    r.pdf = 0.0f;
+    r.area = false;
+    if (entry < s.num_area_lights) {
+        const float4 l0 = s.lights[2 * entry], l1 = s.lights[2 * entry + 1];
+        const int tri = __float_as_int(l1.x);
+        const float4* tp = s.tr.tris + size_t(tri) * 3;
+        const f3 A = xyz(tp[0]), B = xyz(tp[1]), C = xyz(tp[2]);
+        const float su = sqrtf(u);
+        const float bx = 1.0f - su, by = v * su;
+        const f3 pos = A * bx + B * by + C * (1.0f - bx - by);
+        const f3 n = normalize(cross(B - A, C - A));
+        const f3 to_ref = ref - pos;
+        const float d2 = dot(to_ref, to_ref);
+        const float d = sqrtf(d2);
+        const float cosl = dot(n, to_ref) / d;
+        if (!(cosl > 0.0f)) return r;
+        r.pos = pos;
+        r.wi = to_ref * (-1.0f / d);
+        r.dist = d;
+        r.L = xyz(l0);
+        r.pdf = d2 / (l0.w * cosl);
+        r.area = true;
+        return r;
+    }
+    if (entry == s.num_area_lights) {
+        const float z = 1.0f - 2.0f * u;
+        const float rr = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+        float sp, cp;
+        sincosf(2.0f * kPi * v, &sp, &cp);
+        r.wi = mk(rr * cp, rr * sp, z);
+        r.dist = 2.0f * s.wradius;
+        r.pos = ref + r.wi * r.dist;
+        r.L = s.env_L;
+        r.pdf = 1.0f / (4.0f * kPi);
+        return r;
+    }
+    r.wi = s.sun_dir;
+    r.dist = 2.0f * s.wradius;
+    r.pos = ref + r.wi * r.dist;
+    r.L = s.sun_E;
+    r.pdf = -1.0f;
+    return r;
+}
+
+// Henyey-Greenstein phase function (src/medium/phase_function.hpp:12-77);
+// u = -cos(wi, wo) as in the reference.
+TR_D float hg_phase(float g, float u) {
+    const float g2 = g * g;
+    const float dem = 1.0f + g2 - 2.0f * g * u;
+    return (1.0f - g2) / (4.0f * kPi * dem * sqrtf(dem));
+}
+TR_D f3 hg_sample(float g, f3 wo, float u1, float u2, float& pdf) {
+    const float s = 2.0f * u1 - 1.0f;
+    float u;
+    if (fabsf(g) < 0.001f) u = s;
+    else {
+        const float g2 = g * g;
+        const float q = (1.0f - g2) / (1.0f + g * s);
+        u = (1.0f + g2 - q * q) / (2.0f * g);
+    }
+    const float ct = -u;
+    const float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+    float sp, cp;
+    sincosf(2.0f * kPi * u2, &sp, &cp);
+    f3 t, b;
+    const f3 n = normalize(wo);
+    onb(n, t, b);
+    pdf = hg_phase(g, u);
+    return normalize(t * (st * sp) + b * (st * cp) + n * ct);
+}
+
+TR_D void camera_ray(const DevCamera& c, float fx, float fy, float lu, float lv, f3& o, f3& d) {
+    const f3 film = mk((0.5f - fx) * c.film_w, (0.5f - fy) * c.film_h, c.focal);
+    float dx = 0.0f, dy = 0.0f;
+    if (c.lens_radius > 0.0f) concentric_disk(lu, lv, dx, dy);
+    const f3 lens = mk(c.lens_radius * dx, c.lens_radius * dy, 0.0f);
+    const f3 ld = normalize(film - lens);
+    o = c.pos + c.right * lens.x + c.up * lens.y;
+    d = normalize(c.right * ld.x + c.up * ld.y + c.dir * ld.z);
+}
+
+}  // namespace trhip

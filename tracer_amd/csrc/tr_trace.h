@@ -1,0 +1,222 @@
+// Ray traversal on gfx950: 4-wide BVH (bvh_build.h), one ray per lane,
+// near-first order, traversal stack in LDS with a global-memory spill area.
+//
+// Replaces BVHAccel::intersect_p / intersect (src/accelerator/bvh.cpp:112-185)
+// and Triangle::intersect_p / intersect (src/shape/triangle.cpp:117-175).
+// Differences from the reference, by design:
+//   * the triangle test is the watertight test of Woop, Benthin and Wald
+//     (JCGT 2013) instead of Moller-Trumbore, so rays through shared edges
+//     and vertices never fall through;
+//   * the box test widens the slab interval (Ize, "Robust BVH Ray
+//     Traversal", JCGT 2013) so no box holding a triangle hit is culled, and
+//     the closest hit equals a brute-force loop over all triangles;
+//   * equal t goes to the lowest primitive index (the reference keeps the
+//     last-tested primitive, which depends on traversal order).
+#pragma once
+#include "tr_device.h"
+
+namespace trhip {
+
+struct TraceScene {
+    const float4* nodes;   // Node4 as 8 float4 per node
+    const float4* tris;    // TriPacked as 3 float4 per triangle
+    int max_stack;         // stack entries needed by the deepest path
+};
+
+struct TraceHit {
+    float t;
+    int tri;      // leaf-ordered triangle index, -1 on a miss
+    int prim;     // original primitive index, -1 on a miss
+    float b1, b2; // weights of vertex 1 and vertex 2
+};
+
+struct RayWT {
+    f3 o;
+    float Sx, Sy, Sz;
+    int kx, ky, kz;
+};
+
+TR_D float comp(f3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+TR_D RayWT wt_setup(f3 o, f3 d) {
+    RayWT r;
+    r.o = o;
+    const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    int kz = (ax >= ay) ? ((ax >= az) ? 0 : 2) : ((ay >= az) ? 1 : 2);
+    int kx = kz + 1; if (kx == 3) kx = 0;
+    int ky = kx + 1; if (ky == 3) ky = 0;
+    const float dkz = comp(d, kz);
+    if (dkz < 0.0f) { int t = kx; kx = ky; ky = t; }
+    r.kx = kx; r.ky = ky; r.kz = kz;
+    r.Sx = comp(d, kx) / dkz;
+    r.Sy = comp(d, ky) / dkz;
+    r.Sz = 1.0f / dkz;
+    return r;
+}
+
+// Watertight ray/triangle test.  The operation order, the explicit fmaf and
+// the double-precision edge fallback match the CPU restatement exactly, so
+// results are bit-identical; this function must not be contracted.
+TR_D bool wt_test(const RayWT& r, float tmin, float tmax, f3 v0, f3 v1, f3 v2, float& t_out, float& b1, float& b2) {
+#pragma clang fp contract(off)
+    const f3 A = v0 - r.o, B = v1 - r.o, C = v2 - r.o;
+    const float Akz = comp(A, r.kz), Bkz = comp(B, r.kz), Ckz = comp(C, r.kz);
+    const float Ax = fmaf(-r.Sx, Akz, comp(A, r.kx));
+    const float Ay = fmaf(-r.Sy, Akz, comp(A, r.ky));
+    const float Bx = fmaf(-r.Sx, Bkz, comp(B, r.kx));
+    const float By = fmaf(-r.Sy, Bkz, comp(B, r.ky));
+    const float Cx = fmaf(-r.Sx, Ckz, comp(C, r.kx));
+    const float Cy = fmaf(-r.Sy, Ckz, comp(C, r.ky));
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        U = float(double(Cx) * double(By) - double(Cy) * double(Bx));
+        V = float(double(Ax) * double(Cy) - double(Ay) * double(Cx));
+        W = float(double(Bx) * double(Ay) - double(By) * double(Ax));
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    const float det = U + V + W;
+    if (det == 0.0f) return false;
+    const float Az = r.Sz * Akz, Bz = r.Sz * Bkz, Cz = r.Sz * Ckz;
+    const float T = U * Az + V * Bz + W * Cz;
+    const float rcp = 1.0f / det;
+    const float t = T * rcp;
+    if (!(t >= tmin && t <= tmax)) return false;
+    t_out = t;
+    b1 = V * rcp;
+    b2 = W * rcp;
+    return true;
+}
+
+TR_D float safe_rcp(float x) {
+    const float ax = fabsf(x) < 1e-30f ? copysignf(1e-30f, x) : x;
+    return 1.0f / ax;
+}
+
+// Per-block LDS stack, [entry][thread] so that consecutive lanes use
+// consecutive banks.
+template <int BLOCK, int LDS_STACK>
+struct TraceStack {
+    int node[LDS_STACK][BLOCK];
+    float tn[LDS_STACK][BLOCK];
+};
+
+// Traverse one ray.  ANY: stop at the first hit in [tmin, tmax].
+// ovf_node/ovf_t: this thread's spill slots, `ovf_stride` apart.
+// COUNT: accumulate node visits and triangle tests.
+template <bool ANY, bool COUNT, int BLOCK, int LDS_STACK>
+TR_D bool trace(const TraceScene& sc, f3 o, float tmin, f3 d, float tmax, TraceStack<BLOCK, LDS_STACK>& st, int tid,
+                int* ovf_node, float* ovf_t, int ovf_stride, TraceHit& hit, uint32_t* nvis = nullptr,
+                uint32_t* ntri = nullptr) {
+    const RayWT wr = wt_setup(o, d);
+    const f3 inv = {safe_rcp(d.x), safe_rcp(d.y), safe_rcp(d.z)};
+    const f3 oinv = {o.x, o.y, o.z};
+    float best_t = tmax;
+    int best_prim = 0x7fffffff;
+    int best_tri = -1;
+    float bb1 = 0.0f, bb2 = 0.0f;
+    // Interval widening: Ize's 2*gamma(3) plus margin for the triangle's own
+    // t rounding.
+    const float lo_scale = 1.0f - 2e-6f, hi_scale = 1.0f + 2e-6f;
+    int sp = 0;
+    int cur = 0;
+    bool have = true;
+    while (true) {
+        if (!have) {
+            // pop, skipping entries farther than the current closest hit
+            bool found = false;
+            while (sp > 0) {
+                --sp;
+                int n;
+                float t;
+                if (sp < LDS_STACK) { n = st.node[sp][tid]; t = st.tn[sp][tid]; }
+                else { n = ovf_node[(sp - LDS_STACK) * ovf_stride]; t = ovf_t[(sp - LDS_STACK) * ovf_stride]; }
+                if (t <= best_t) { cur = n; found = true; break; }
+            }
+            if (!found) break;
+        }
+        if (COUNT) ++*nvis;
+        const float4* np = sc.nodes + size_t(cur) * 8;
+        const float4 lox = np[0], hix = np[1], loy = np[2], hiy = np[3], loz = np[4], hiz = np[5];
+        const int4 ch = *reinterpret_cast<const int4*>(np + 6);
+        float tn[4];
+        bool hm[4];
+        {
+            const float lx[4] = {lox.x, lox.y, lox.z, lox.w}, hx[4] = {hix.x, hix.y, hix.z, hix.w};
+            const float ly[4] = {loy.x, loy.y, loy.z, loy.w}, hy[4] = {hiy.x, hiy.y, hiy.z, hiy.w};
+            const float lz[4] = {loz.x, loz.y, loz.z, loz.w}, hz[4] = {hiz.x, hiz.y, hiz.z, hiz.w};
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const float x0 = (lx[c] - oinv.x) * inv.x, x1 = (hx[c] - oinv.x) * inv.x;
+                const float y0 = (ly[c] - oinv.y) * inv.y, y1 = (hy[c] - oinv.y) * inv.y;
+                const float z0 = (lz[c] - oinv.z) * inv.z, z1 = (hz[c] - oinv.z) * inv.z;
+                const float tnear = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin)) * lo_scale;
+                const float tfar = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1)) * hi_scale;
+                hm[c] = tnear <= tfar && tnear <= best_t && tfar >= tmin;
+                tn[c] = tnear;
+            }
+        }
+        const int chv[4] = {ch.x, ch.y, ch.z, ch.w};
+        // leaves first: they can only shrink best_t
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            if (!hm[c] || chv[c] >= 0) continue;
+            const int code = ~chv[c];
+            const int first = code >> 4, cnt = code & 15;
+            for (int k = 0; k < cnt; ++k) {
+                const int ti = first + k;
+                const float4* tp = sc.tris + size_t(ti) * 3;
+                const float4 a = tp[0], b = tp[1], c2 = tp[2];
+                if (COUNT) ++*ntri;
+                float t, u1, u2;
+                if (!wt_test(wr, tmin, best_t, xyz(a), xyz(b), xyz(c2), t, u1, u2)) continue;
+                const int prim = __float_as_int(a.w);
+                if (ANY) {
+                    hit.t = t; hit.tri = ti; hit.prim = prim; hit.b1 = u1; hit.b2 = u2;
+                    return true;
+                }
+                if (t < best_t || prim < best_prim) {
+                    best_t = t; best_prim = prim; best_tri = ti; bb1 = u1; bb2 = u2;
+                }
+            }
+        }
+        // inner children, nearest first
+        int ids[4];
+        float ts[4];
+        int m = 0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            if (hm[c] && chv[c] >= 0 && tn[c] <= best_t) { ids[m] = chv[c]; ts[m] = tn[c]; ++m; }
+        }
+        if (m == 0) { have = false; continue; }
+        // insertion sort, descending (far first) so the nearest ends last
+        for (int i = 1; i < m; ++i) {
+            const int ki = ids[i];
+            const float kt = ts[i];
+            int j = i - 1;
+            while (j >= 0 && ts[j] < kt) { ids[j + 1] = ids[j]; ts[j + 1] = ts[j]; --j; }
+            ids[j + 1] = ki;
+            ts[j + 1] = kt;
+        }
+        for (int i = 0; i < m - 1; ++i) {
+            if (sp < LDS_STACK) { st.node[sp][tid] = ids[i]; st.tn[sp][tid] = ts[i]; }
+            else if (sp < sc.max_stack) {
+                ovf_node[(sp - LDS_STACK) * ovf_stride] = ids[i];
+                ovf_t[(sp - LDS_STACK) * ovf_stride] = ts[i];
+            }
+            ++sp;
+        }
+        if (sp > sc.max_stack) sp = sc.max_stack;   // unreachable by construction (3 * depth)
+        cur = ids[m - 1];
+        have = true;
+    }
+    if (best_tri < 0) {
+        hit.t = 0.0f; hit.tri = -1; hit.prim = -1; hit.b1 = hit.b2 = 0.0f;
+        return false;
+    }
+    hit.t = best_t; hit.tri = best_tri; hit.prim = best_prim; hit.b1 = bb1; hit.b2 = bb2;
+    return true;
+}
+
+}  // namespace trhip

@@ -1,0 +1,496 @@
+// Implementation of the tracer:: factory API (include/tracer/tracer.hpp) and
+// of the GPU-backed PathTraceRenderer.
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <unordered_map>
+
+#include "facade_impl.h"
+
+namespace tracer {
+
+using namespace detail;
+
+// ---- Transform (src/utility/transform.cpp) --------------------------------
+static void identity(real m[4][4]) {
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) m[i][j] = i == j ? 1.0f : 0.0f;
+}
+static bool invert4(const real a[4][4], real out[4][4]) {
+    double m[4][8];
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 8; ++j) m[i][j] = j < 4 ? a[i][j] : (j - 4 == i ? 1.0 : 0.0);
+    for (int c = 0; c < 4; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < 4; ++r)
+            if (std::fabs(m[r][c]) > std::fabs(m[piv][c])) piv = r;
+        if (m[piv][c] == 0.0) return false;
+        if (piv != c)
+            for (int j = 0; j < 8; ++j) std::swap(m[c][j], m[piv][j]);
+        const double inv = 1.0 / m[c][c];
+        for (int j = 0; j < 8; ++j) m[c][j] *= inv;
+        for (int r = 0; r < 4; ++r) {
+            if (r == c) continue;
+            const double f = m[r][c];
+            for (int j = 0; j < 8; ++j) m[r][j] -= f * m[c][j];
+        }
+    }
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) out[i][j] = real(m[i][j + 4]);
+    return true;
+}
+Transform::Transform() {
+    identity(m);
+    identity(inv);
+}
+Transform::Transform(const real mat[4][4]) {
+    std::memcpy(m, mat, sizeof m);
+    if (!invert4(m, inv)) throw std::runtime_error("Transform: singular matrix");
+}
+Transform::Transform(const real mat[4][4], const real inv_mat[4][4]) {
+    std::memcpy(m, mat, sizeof m);
+    std::memcpy(inv, inv_mat, sizeof inv);
+}
+Point3f Transform::operator()(const Point3f& p) const {
+    const real x = m[0][0] * p.x + m[0][1] * p.y + m[0][2] * p.z + m[0][3];
+    const real y = m[1][0] * p.x + m[1][1] * p.y + m[1][2] * p.z + m[1][3];
+    const real z = m[2][0] * p.x + m[2][1] * p.y + m[2][2] * p.z + m[2][3];
+    const real w = m[3][0] * p.x + m[3][1] * p.y + m[3][2] * p.z + m[3][3];
+    if (w == 1.0f) return {x, y, z};
+    return {x / w, y / w, z / w};
+}
+Vector3f Transform::operator()(const Vector3f& v) const {
+    return {m[0][0] * v.x + m[0][1] * v.y + m[0][2] * v.z, m[1][0] * v.x + m[1][1] * v.y + m[1][2] * v.z,
+            m[2][0] * v.x + m[2][1] * v.y + m[2][2] * v.z};
+}
+Normal3f Transform::operator()(const Normal3f& n) const {   // transform.hpp:83-88 (inverse transpose)
+    return {inv[0][0] * n.x + inv[1][0] * n.y + inv[2][0] * n.z, inv[0][1] * n.x + inv[1][1] * n.y + inv[2][1] * n.z,
+            inv[0][2] * n.x + inv[1][2] * n.y + inv[2][2] * n.z};
+}
+Transform operator*(const Transform& a, const Transform& b) {
+    real m[4][4], inv[4][4];
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) {
+            real s = 0, t = 0;
+            for (int k = 0; k < 4; ++k) {
+                s += a.m[i][k] * b.m[k][j];
+                t += b.inv[i][k] * a.inv[k][j];
+            }
+            m[i][j] = s;
+            inv[i][j] = t;
+        }
+    return Transform(m, inv);
+}
+Transform inverse(const Transform& t) { return Transform(t.inv, t.m); }
+Transform translate(const Vector3f& d) {
+    real m[4][4], inv[4][4];
+    identity(m);
+    identity(inv);
+    m[0][3] = d.x; m[1][3] = d.y; m[2][3] = d.z;
+    inv[0][3] = -d.x; inv[1][3] = -d.y; inv[2][3] = -d.z;
+    return Transform(m, inv);
+}
+Transform scale(real x, real y, real z) {
+    real m[4][4], inv[4][4];
+    identity(m);
+    identity(inv);
+    m[0][0] = x; m[1][1] = y; m[2][2] = z;
+    inv[0][0] = 1 / x; inv[1][1] = 1 / y; inv[2][2] = 1 / z;
+    return Transform(m, inv);
+}
+static Transform rot(int axis, real th) {
+    const real s = std::sin(th), c = std::cos(th);
+    real m[4][4];
+    identity(m);
+    const int a = (axis + 1) % 3, b = (axis + 2) % 3;
+    m[a][a] = c; m[a][b] = -s;
+    m[b][a] = s; m[b][b] = c;
+    real t[4][4];
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) t[i][j] = m[j][i];
+    return Transform(m, t);
+}
+Transform rotate_x(real th) { return rot(0, th); }
+Transform rotate_y(real th) { return rot(1, th); }   // transform.cpp:147-153
+Transform rotate_z(real th) { return rot(2, th); }
+Transform look_at(const Point3f& pos, const Point3f& target, const Vector3f& up) {   // transform.cpp:96-124
+    const Vector3f dir = (target - pos).normalize();
+    const Vector3f right = cross(up.normalize(), dir).normalize();
+    const Vector3f nup = cross(dir, right);
+    real c2w[4][4] = {{right.x, nup.x, dir.x, pos.x}, {right.y, nup.y, dir.y, pos.y}, {right.z, nup.z, dir.z, pos.z}, {0, 0, 0, 1}};
+    Transform cw(c2w);
+    return inverse(cw);
+}
+
+// ---- factories -------------------------------------------------------------
+RC<Texture2D> create_constant_texture2d(const Spectrum& c) { return newRC<ConstantTexture2D>(c); }
+
+RC<Material> create_disney(RC<const Texture2D> base_color, RC<const Texture2D> metallic, RC<const Texture2D> roughness,
+                           RC<const Texture2D> transmission, RC<const Texture2D> transmission_roughness,
+                           RC<const Texture2D> ior, RC<const Texture2D> specular_scale, RC<const Texture2D> specular_tint,
+                           RC<const Texture2D> anisotropic, RC<const Texture2D> sheen, RC<const Texture2D> sheen_tint,
+                           RC<const Texture2D> clearcoat, RC<const Texture2D> clearcoat_gloss,
+                           Box<const NormalMapper> normal_mapper, RC<const BSSRDFSurface> bssrdf) {
+    // disney.cpp:621,638 dereference both; a normal map would need textures on the GPU path.
+    if (!normal_mapper) throw std::runtime_error("create_disney: normal_mapper must not be null");
+    if (normal_mapper->map()) throw std::runtime_error("create_disney: normal maps are not supported by the GPU path");
+    (void)bssrdf;
+    auto m = newRC<DisneyMaterial>();
+    m->base_color = base_color; m->metallic = metallic; m->roughness = roughness; m->transmission = transmission;
+    m->transmission_roughness = transmission_roughness; m->ior = ior; m->specular_scale = specular_scale;
+    m->specular_tint = specular_tint; m->anisotropic = anisotropic; m->sheen = sheen; m->sheen_tint = sheen_tint;
+    m->clearcoat = clearcoat; m->clearcoat_gloss = clearcoat_gloss;
+    const RC<const Texture2D>* all[] = {&m->base_color, &m->metallic, &m->roughness, &m->transmission,
+                                        &m->transmission_roughness, &m->ior, &m->specular_scale, &m->specular_tint,
+                                        &m->anisotropic, &m->sheen, &m->sheen_tint, &m->clearcoat, &m->clearcoat_gloss};
+    for (auto* t : all)
+        if (!*t) throw std::runtime_error("create_disney: texture must not be null");
+    return m;
+}
+
+RC<Material> create_phong_material(RC<const Texture2D> ka, RC<const Texture2D> kd, RC<const Texture2D> ks,
+                                   RC<const Texture2D> ns) {
+    if (!kd || !ks || !ns) throw std::runtime_error("create_phong_material: texture must not be null");
+    auto m = newRC<PhongMaterial>();
+    m->ka = ka; m->kd = kd; m->ks = ks; m->ns = ns;
+    return m;
+}
+
+// triangle.cpp:216-236
+std::vector<RC<Shape>> create_triangle_mesh(const mesh_t& mesh, const Transform& t) {
+    if (mesh.indices.size() % 3 != 0) throw std::runtime_error("create_triangle_mesh: index count not a multiple of 3");
+    auto tm = newRC<TriangleMesh>();
+    const size_t nv = mesh.vertices.size();
+    if (!nv || mesh.indices.empty()) throw std::runtime_error("create_triangle_mesh: empty mesh");
+    tm->p.resize(nv);
+    tm->n.resize(nv);
+    tm->uv.resize(nv);
+    for (size_t i = 0; i < nv; ++i) {
+        tm->p[i] = t(mesh.vertices[i].pos);
+        tm->n[i] = t(mesh.vertices[i].n).normalize();
+        tm->uv[i] = mesh.vertices[i].uv;
+    }
+    for (int v : mesh.indices)
+        if (v < 0 || size_t(v) >= nv) throw std::runtime_error("create_triangle_mesh: index out of range");
+    tm->indices = mesh.indices;
+    std::vector<RC<Shape>> out;
+    out.reserve(mesh.indices.size() / 3);
+    for (size_t i = 0; i < mesh.indices.size() / 3; ++i) out.push_back(newRC<TriangleShape>(tm, int(i)));
+    return out;
+}
+
+RC<Medium> create_homogeneous_medium(const Spectrum& sa, const Spectrum& ss, real g, int max_sc) {
+    if (!(g > -1 && g < 1)) throw std::runtime_error("create_homogeneous_medium: g must lie in (-1, 1)");
+    auto m = newRC<HomogeneousMedium>();
+    m->sigma_a = sa; m->sigma_s = ss; m->g = g; m->max_scattering_count = max_sc;
+    return m;
+}
+RC<Medium> create_vacuum() {   // vacuum.cpp:26-29: one shared instance
+    static RC<Medium> v = newRC<VacuumMedium>();
+    return v;
+}
+
+RC<EnvironmentLight> create_ibl_light(RC<const Texture2D> tex, const Transform&) {
+    if (!tex) throw std::runtime_error("create_ibl_light: texture must not be null");
+    if (!tex->is_constant()) throw std::runtime_error("create_ibl_light: only constant textures are supported by the GPU path");
+    auto l = newRC<IBLConstant>();
+    l->tex = tex;
+    return l;
+}
+RC<Light> create_directional_light(const Vector3f& dir, const Spectrum& E) {
+    auto l = newRC<DirectionalLight>();
+    l->direction = dir.normalize();
+    l->irradiance = E;
+    return l;
+}
+
+RC<Primitive> create_geometric_primitive(const RC<Shape>& shape, const RC<Material>& material, const MediumInterface& mi,
+                                         const Spectrum& emission) {
+    if (!mi.inside || !mi.outside) throw std::runtime_error("create_geometric_primitive: MediumInterface members must be set");
+    if (!std::dynamic_pointer_cast<TriangleShape>(shape)) throw std::runtime_error("create_geometric_primitive: triangle shapes only");
+    if (!material) throw std::runtime_error("create_geometric_primitive: material must not be null");
+    auto p = newRC<GeometricPrimitive>();
+    p->shape = shape;
+    p->material = material;
+    p->mi = mi;
+    p->emission = emission;
+    if (!emission.is_back()) {   // geometry.cpp:39-42
+        p->light = newBox<DiffuseLight>();
+        p->light->emission = emission;
+    }
+    return p;
+}
+
+RC<Aggregate> create_bvh_accel(int max_leaf) { return newRC<BVHAccel>(max_leaf); }
+
+RC<Camera> create_thin_lens_camera(real aspect, const Point3f& pos, const Point3f& target, const Vector3f& up, real fov,
+                                   real lens_radius, real focal_dist) {
+    auto c = newRC<ThinLensCamera>();
+    c->aspect = aspect; c->pos = pos; c->target = target; c->up = up; c->fov = fov;
+    c->lens_radius = lens_radius; c->focal_dist = focal_dist;
+    return c;
+}
+RC<Filter> create_gaussin_filter(real radius, real alpha) {
+    auto f = newRC<GaussianFilter>();
+    f->r = radius; f->alpha = alpha;
+    return f;
+}
+RC<Filter> create_box_filter(real radius) {
+    auto f = newRC<BoxFilter>();
+    f->r = radius;
+    return f;
+}
+RC<Scene> create_general_scene(const RC<Aggregate>& accel) {
+    if (!std::dynamic_pointer_cast<BVHAccel>(accel)) throw std::runtime_error("create_general_scene: aggregate must come from create_bvh_accel");
+    return newRC<GeneralScene>(accel);
+}
+
+// ---- flattening --------------------------------------------------------------
+namespace detail {
+
+tr_scene_desc FlatScene::desc() const {
+    tr_scene_desc d{};
+    d.num_vertices = int64_t(P.size() / 3);
+    d.positions = P.data();
+    d.normals = N.empty() ? nullptr : N.data();
+    d.uvs = UV.empty() ? nullptr : UV.data();
+    d.num_triangles = int64_t(mat.size());
+    d.indices = idx.data();
+    d.material_ids = mat.data();
+    d.emission = emission.empty() ? nullptr : emission.data();
+    d.medium_inside = med_in.empty() ? nullptr : med_in.data();
+    d.medium_outside = med_out.empty() ? nullptr : med_out.data();
+    d.num_materials = int32_t(mats.size());
+    d.num_media = int32_t(media.size());
+    d.materials = mats.data();
+    d.media = media.empty() ? nullptr : media.data();
+    d.has_env = has_env;
+    std::memcpy(d.env_radiance, env, sizeof env);
+    d.has_sun = has_sun;
+    std::memcpy(d.sun_direction, sun_dir, sizeof sun_dir);
+    std::memcpy(d.sun_irradiance, sun_E, sizeof sun_E);
+    d.camera = cam;
+    return d;
+}
+
+static float tex_s(const RC<const Texture2D>& t) { return t->evaluate_s(Point2f(0, 0)); }
+static void tex_rgb(const RC<const Texture2D>& t, float out[3]) {
+    const Spectrum s = t->evaluate(Point2f(0, 0));
+    out[0] = s.r; out[1] = s.g; out[2] = s.b;
+}
+
+static tr_material convert_material(const Material* m) {
+    tr_material r{};
+    if (auto d = dynamic_cast<const DisneyMaterial*>(m)) {
+        const RC<const Texture2D>* all[] = {&d->base_color, &d->metallic, &d->roughness, &d->transmission,
+                                            &d->transmission_roughness, &d->ior, &d->specular_scale, &d->specular_tint,
+                                            &d->anisotropic, &d->sheen, &d->sheen_tint, &d->clearcoat, &d->clearcoat_gloss};
+        for (auto* t : all)
+            if (!(*t)->is_constant()) throw std::runtime_error("Disney: only constant textures are supported by the GPU path");
+        r.type = TR_MAT_DISNEY;
+        tex_rgb(d->base_color, r.base_color);
+        r.metallic = tex_s(d->metallic);
+        r.roughness = tex_s(d->roughness);
+        r.transmission = tex_s(d->transmission);
+        r.transmission_roughness = tex_s(d->transmission_roughness);
+        r.ior = tex_s(d->ior);
+        tex_rgb(d->specular_scale, r.specular_scale);
+        r.specular_tint = tex_s(d->specular_tint);
+        r.anisotropic = tex_s(d->anisotropic);
+        r.sheen = tex_s(d->sheen);
+        r.sheen_tint = tex_s(d->sheen_tint);
+        r.clearcoat = tex_s(d->clearcoat);
+        r.clearcoat_gloss = tex_s(d->clearcoat_gloss);
+        return r;
+    }
+    if (auto p = dynamic_cast<const PhongMaterial*>(m)) {
+        if (!p->kd->is_constant() || !p->ks->is_constant()) throw std::runtime_error("Phong: only constant textures are supported");
+        const Spectrum kd = p->kd->evaluate(Point2f(0, 0)), ks = p->ks->evaluate(Point2f(0, 0));
+        if (!ks.is_back()) throw std::runtime_error("Phong: the GPU path supports the Lambert case (ks = 0) only");
+        // phong.cpp:36-41: energy conservation divides by max(1, max_i(kd_i + ks_i))
+        const float dem = std::max({1.0f, kd.r + ks.r, kd.g + ks.g, kd.b + ks.b});
+        r.type = TR_MAT_LAMBERT;
+        r.base_color[0] = kd.r / dem; r.base_color[1] = kd.g / dem; r.base_color[2] = kd.b / dem;
+        return r;
+    }
+    throw std::runtime_error("unsupported material");
+}
+
+FlatScene flatten(const Scene& scene) {
+    const auto* gs = dynamic_cast<const GeneralScene*>(&scene);
+    if (!gs) throw std::runtime_error("render: scene must come from create_general_scene");
+    const auto* bvh = dynamic_cast<const BVHAccel*>(gs->accel.get());
+    if (!bvh || bvh->prims.empty()) throw std::runtime_error("render: aggregate has no primitives (call build)");
+    const auto* cam = dynamic_cast<const ThinLensCamera*>(gs->camera.get());
+    if (!cam) throw std::runtime_error("render: scene has no camera");
+    FlatScene f;
+    f.max_leaf = bvh->max_leaf;
+    std::unordered_map<const TriangleMesh*, int32_t> mesh_base;
+    std::unordered_map<const Material*, int32_t> mat_id;
+    std::unordered_map<const Medium*, int32_t> med_id;
+    auto medium_index = [&](const RC<const Medium>& m) -> int32_t {
+        if (dynamic_cast<const VacuumMedium*>(m.get())) return -1;
+        auto it = med_id.find(m.get());
+        if (it != med_id.end()) return it->second;
+        const auto* h = dynamic_cast<const HomogeneousMedium*>(m.get());
+        if (!h) throw std::runtime_error("render: only homogeneous media and vacuum are supported");
+        tr_medium tm{};
+        tm.sigma_a[0] = h->sigma_a.r; tm.sigma_a[1] = h->sigma_a.g; tm.sigma_a[2] = h->sigma_a.b;
+        tm.sigma_s[0] = h->sigma_s.r; tm.sigma_s[1] = h->sigma_s.g; tm.sigma_s[2] = h->sigma_s.b;
+        tm.g = h->g;
+        tm.max_scattering_count = h->max_scattering_count;
+        const int32_t id = int32_t(f.media.size());
+        f.media.push_back(tm);
+        med_id[m.get()] = id;
+        return id;
+    };
+    const size_t np = bvh->prims.size();
+    f.idx.reserve(3 * np);
+    f.mat.reserve(np);
+    f.med_in.reserve(np);
+    f.med_out.reserve(np);
+    f.emission.reserve(3 * np);
+    bool any_medium = false, any_emission = false;
+    for (const auto& prim : bvh->prims) {
+        const auto* gp = dynamic_cast<const GeometricPrimitive*>(prim.get());
+        if (!gp) throw std::runtime_error("render: unsupported primitive");
+        const auto* tri = static_cast<const TriangleShape*>(gp->shape.get());
+        const TriangleMesh* mesh = tri->mesh.get();
+        auto it = mesh_base.find(mesh);
+        int32_t base;
+        if (it == mesh_base.end()) {
+            base = int32_t(f.P.size() / 3);
+            for (size_t v = 0; v < mesh->p.size(); ++v) {
+                f.P.insert(f.P.end(), {mesh->p[v].x, mesh->p[v].y, mesh->p[v].z});
+                f.N.insert(f.N.end(), {mesh->n[v].x, mesh->n[v].y, mesh->n[v].z});
+                f.UV.insert(f.UV.end(), {mesh->uv[v].x, mesh->uv[v].y});
+            }
+            mesh_base[mesh] = base;
+        } else {
+            base = it->second;
+        }
+        for (int k = 0; k < 3; ++k) f.idx.push_back(base + mesh->indices[3 * size_t(tri->index) + k]);
+        auto mit = mat_id.find(gp->material.get());
+        int32_t mid;
+        if (mit == mat_id.end()) {
+            mid = int32_t(f.mats.size());
+            f.mats.push_back(convert_material(gp->material.get()));
+            mat_id[gp->material.get()] = mid;
+        } else {
+            mid = mit->second;
+        }
+        f.mat.push_back(mid);
+        const int32_t mi = medium_index(gp->mi.inside), mo = medium_index(gp->mi.outside);
+        any_medium |= (mi >= 0 || mo >= 0);
+        f.med_in.push_back(mi);
+        f.med_out.push_back(mo);
+        f.emission.insert(f.emission.end(), {gp->emission.r, gp->emission.g, gp->emission.b});
+        any_emission |= !gp->emission.is_back();
+    }
+    if (!any_medium) { f.med_in.clear(); f.med_out.clear(); }
+    if (!any_emission) f.emission.clear();
+    if (gs->environment_light) {
+        const auto* ibl = dynamic_cast<const IBLConstant*>(gs->environment_light.get());
+        if (!ibl) throw std::runtime_error("render: unsupported environment light");
+        tex_rgb(ibl->tex, f.env);
+        f.has_env = 1;
+    }
+    for (const Light* l : gs->lights) {
+        if (const auto* dl = dynamic_cast<const DirectionalLight*>(l)) {
+            if (f.has_sun) throw std::runtime_error("render: at most one directional light is supported");
+            f.has_sun = 1;
+            f.sun_dir[0] = dl->direction.x; f.sun_dir[1] = dl->direction.y; f.sun_dir[2] = dl->direction.z;
+            f.sun_E[0] = dl->irradiance.r; f.sun_E[1] = dl->irradiance.g; f.sun_E[2] = dl->irradiance.b;
+        }
+    }
+    tr_camera& c = f.cam;
+    c.pos[0] = cam->pos.x; c.pos[1] = cam->pos.y; c.pos[2] = cam->pos.z;
+    c.target[0] = cam->target.x; c.target[1] = cam->target.y; c.target[2] = cam->target.z;
+    c.up[0] = cam->up.x; c.up[1] = cam->up.y; c.up[2] = cam->up.z;
+    c.fov_y = cam->fov;
+    c.lens_radius = cam->lens_radius;
+    c.focal_distance = cam->focal_dist;
+    c.aspect = cam->aspect;
+    return f;
+}
+
+}  // namespace detail
+
+// ---- the GPU path tracer behind create_pt_renderer ---------------------------
+namespace {
+
+class GpuPathTraceRenderer : public Renderer {
+public:
+    GpuPathTraceRenderer(const PTRendererParams& p, const GpuRenderOptions& g) : params(p), gpu(g) {
+        if (p.spp <= 0 || p.max_depth < 0 || p.min_depth < 0) throw std::runtime_error("create_pt_renderer: invalid params");
+    }
+    ~GpuPathTraceRenderer() override {
+        if (ctx) tr_destroy(ctx);
+    }
+    RenderTarget render(const Scene& scene, Film film) override {
+        std::lock_guard<std::mutex> lk(mtx);
+        auto check = [&](tr_status s, const char* what) {
+            if (s != TR_OK) throw std::runtime_error(std::string(what) + ": " + (ctx ? tr_last_error(ctx) : "no context"));
+        };
+        if (!ctx) check(tr_create(&ctx, gpu.device), "tr_create");
+        if (&scene != last_scene) {
+            FlatScene f = flatten(scene);
+            const tr_scene_desc d = f.desc();
+            check(tr_scene_upload(ctx, &d), "tr_scene_upload");
+            tr_bvh_opts bo{};
+            bo.max_leaf_primitives = std::min(std::max(f.max_leaf, 1), 15);
+            bo.num_threads = 0;
+            check(tr_bvh_build(ctx, &bo), "tr_bvh_build");
+            last_scene = &scene;
+        }
+        tr_render_params rp{};
+        rp.width = film.width();
+        rp.height = film.height();
+        rp.spp = params.spp;
+        rp.min_depth = params.min_depth;
+        rp.max_depth = params.max_depth;
+        rp.direct_light_sample_num = params.direct_light_sample_num;
+        const Filter* flt = film.get_filter().get();
+        if (const auto* gf = dynamic_cast<const GaussianFilter*>(flt)) {
+            rp.filter_type = TR_FILTER_GAUSSIAN;
+            rp.filter_radius = gf->r;
+            rp.filter_alpha = gf->alpha;
+        } else if (const auto* bf = dynamic_cast<const BoxFilter*>(flt)) {
+            rp.filter_type = TR_FILTER_BOX;
+            rp.filter_radius = bf->r;
+        } else {
+            throw std::runtime_error("render: unsupported filter");
+        }
+        rp.shadow_transmittance = gpu.shadow_transmittance;
+        rp.seed = gpu.seed;
+        rp.samples_per_pass = gpu.samples_per_pass;
+        rp.guide = gpu.guide;
+        rp.guide_train_passes = gpu.guide_train_passes;
+        rp.guide_fraction = gpu.guide_fraction;
+        rp.guide_lr = gpu.guide_lr;
+        RenderTarget rt;
+        rt.color = Image2D<Spectrum>(rp.width, rp.height);
+        static_assert(sizeof(Spectrum) == 12, "Spectrum is three floats");
+        tr_render_stats st{};
+        check(tr_render(ctx, &rp, nullptr, 0, reinterpret_cast<float*>(rt.color.get_raw_data()), &st), "tr_render");
+        if (gpu.stats_out) *gpu.stats_out = st;
+        return rt;
+    }
+
+private:
+    PTRendererParams params;
+    GpuRenderOptions gpu;
+    tr_ctx* ctx = nullptr;
+    const Scene* last_scene = nullptr;
+    std::mutex mtx;
+};
+
+}  // namespace
+
+RC<Renderer> create_pt_renderer(const PTRendererParams& params) { return newRC<GpuPathTraceRenderer>(params, GpuRenderOptions{}); }
+RC<Renderer> create_pt_renderer(const PTRendererParams& params, const GpuRenderOptions& gpu) {
+    return newRC<GpuPathTraceRenderer>(params, gpu);
+}
+
+}  // namespace tracer

@@ -1,0 +1,131 @@
+// Internal classes behind the tracer:: factory API and the scene flattener
+// that turns a tracer::Scene into a tr_scene_desc for libtracer_hip.so.
+#pragma once
+#include <string>
+#include <vector>
+
+#include "tracer/tracer.hpp"
+#include "tracer_hip.h"
+
+namespace tracer {
+namespace detail {
+
+class ConstantTexture2D : public Texture2D {
+public:
+    explicit ConstantTexture2D(const Spectrum& c) : c(c) {}
+    Spectrum evaluate(const Point2f&) const override { return c; }
+    bool is_constant() const override { return true; }
+    Spectrum c;
+};
+
+class DisneyMaterial : public Material {
+public:
+    RC<const Texture2D> base_color, metallic, roughness, transmission, transmission_roughness, ior, specular_scale,
+        specular_tint, anisotropic, sheen, sheen_tint, clearcoat, clearcoat_gloss;
+};
+class PhongMaterial : public Material {
+public:
+    RC<const Texture2D> ka, kd, ks, ns;
+};
+
+// triangle.cpp:15-33 — world-space vertices of one mesh
+struct TriangleMesh {
+    std::vector<Point3f> p;
+    std::vector<Normal3f> n;
+    std::vector<Point2f> uv;
+    std::vector<int> indices;
+};
+class TriangleShape : public Shape {
+public:
+    TriangleShape(RC<const TriangleMesh> m, int i) : mesh(std::move(m)), index(i) {}
+    RC<const TriangleMesh> mesh;
+    int index;
+};
+
+class HomogeneousMedium : public Medium {
+public:
+    Spectrum sigma_a, sigma_s;
+    real g = 0;
+    int max_scattering_count = 0;
+};
+class VacuumMedium : public Medium {};
+
+class DiffuseLight : public AreaLight {
+public:
+    Spectrum emission;
+};
+class IBLConstant : public EnvironmentLight {
+public:
+    RC<const Texture2D> tex;
+};
+class DirectionalLight : public Light {
+public:
+    Vector3f direction;
+    Spectrum irradiance;
+};
+
+class GeometricPrimitive : public Primitive {
+public:
+    RC<Shape> shape;
+    RC<Material> material;
+    MediumInterface mi;
+    Spectrum emission;
+    Box<DiffuseLight> light;
+    const AreaLight* as_area_light() const noexcept override { return light.get(); }
+};
+
+class BVHAccel : public Aggregate {
+public:
+    explicit BVHAccel(int max_leaf) : max_leaf(max_leaf) {}
+    void build(std::vector<RC<Primitive>> p) override { prims = std::move(p); }
+    int max_leaf;
+    std::vector<RC<Primitive>> prims;
+};
+
+class ThinLensCamera : public Camera {
+public:
+    real aspect = 1, fov = 0, lens_radius = 0, focal_dist = 1;
+    Point3f pos, target;
+    Vector3f up;
+};
+
+class GaussianFilter : public Filter {
+public:
+    real r = 0.5f, alpha = 2.0f;
+    real radius() const noexcept override { return r; }
+};
+class BoxFilter : public Filter {
+public:
+    real r = 0.5f;
+    real radius() const noexcept override { return r; }
+};
+
+class GeneralScene : public Scene {
+public:
+    explicit GeneralScene(RC<Aggregate> a) : accel(std::move(a)) {}
+    void set_camera(RC<const Camera> c) override { camera = std::move(c); }
+    RC<const Camera> get_camera() const noexcept override { return camera; }
+    void prepare_to_render() override {}
+    RC<Aggregate> accel;
+    RC<const Camera> camera;
+};
+
+// A scene flattened to plain arrays; desc() points into them.
+struct FlatScene {
+    std::vector<float> P, N, UV;
+    std::vector<int32_t> idx, mat, med_in, med_out;
+    std::vector<float> emission;
+    std::vector<tr_material> mats;
+    std::vector<tr_medium> media;
+    int has_env = 0, has_sun = 0;
+    float env[3] = {0, 0, 0}, sun_dir[3] = {0, 0, 1}, sun_E[3] = {0, 0, 0};
+    tr_camera cam{};
+    int max_leaf = 3;
+    tr_scene_desc desc() const;
+};
+
+// Throws std::runtime_error for scene content the GPU path does not support.
+FlatScene flatten(const Scene& scene);
+
+}  // namespace detail
+}  // namespace tracer

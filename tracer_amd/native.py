@@ -1,0 +1,320 @@
+"""Python bindings over the C ABI (include/tracer_hip.h, include/tracer/scenes.h).
+
+The GPU path always runs through libtracer_hip.so; there is no fallback.  If
+the library is missing, loading raises NativeLibraryMissing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from pathlib import Path
+
+import numpy as np
+
+from . import abi
+from .abi import TrMaterial, TrMedium, TrRenderParams, TrRenderStats, TrSceneDesc, dptr, fptr, iptr
+
+LIB_DIR = Path(__file__).resolve().parent / "lib"
+
+
+class NativeLibraryMissing(RuntimeError):
+    pass
+
+
+class TracerError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"status {status}: {msg}")
+        self.status = status
+
+
+_hip = None
+_scenes = None
+
+
+def hip_lib() -> C.CDLL:
+    global _hip
+    if _hip is None:
+        p = LIB_DIR / "libtracer_hip.so"
+        if not p.exists():
+            raise NativeLibraryMissing(f"{p} is not built (run python -m tracer_amd.build)")
+        lib = C.CDLL(str(p))
+        P, I64, I32, VP = C.POINTER, C.c_int64, C.c_int32, C.c_void_p
+        lib.tr_create.argtypes = [P(VP), I32]
+        lib.tr_destroy.argtypes = [VP]
+        lib.tr_destroy.restype = None
+        lib.tr_last_error.argtypes = [VP]
+        lib.tr_last_error.restype = C.c_char_p
+        lib.tr_version.restype = C.c_char_p
+        lib.tr_scene_upload.argtypes = [VP, P(TrSceneDesc)]
+        lib.tr_bvh_build.argtypes = [VP, P(abi.TrBvhOpts)]
+        lib.tr_bvh_info.argtypes = [VP, P(I64), P(I64), P(I64)]
+        lib.tr_render.argtypes = [VP, P(TrRenderParams), P(C.c_double), I32, P(C.c_float), P(TrRenderStats)]
+        lib.tr_trace_closest.argtypes = [VP, P(C.c_float), P(C.c_float), I64]
+        lib.tr_trace_any.argtypes = [VP, P(C.c_float), P(C.c_int32), I64]
+        lib.tr_trace_closest_device.argtypes = [VP, VP, VP, I64]
+        lib.tr_trace_any_device.argtypes = [VP, VP, VP, I64]
+        lib.tr_trace_closest_counted.argtypes = [VP, P(C.c_float), P(C.c_float), P(C.c_uint32), P(C.c_uint32), I64]
+        lib.tr_stream.argtypes = [VP, P(VP)]
+        lib.tr_synchronize.argtypes = [VP]
+        lib.tr_bsdf_eval_batch.argtypes = [VP, P(TrMaterial), I32, P(C.c_int32), P(C.c_float), P(C.c_float),
+                                           P(C.c_float), I64, P(C.c_float), P(C.c_float)]
+        lib.tr_bsdf_sample_batch.argtypes = [VP, P(TrMaterial), I32, P(C.c_int32), P(C.c_float), P(C.c_float),
+                                             P(C.c_float), I64, P(C.c_float), P(C.c_float), P(C.c_float)]
+        _hip = lib
+    return _hip
+
+
+def scenes_lib() -> C.CDLL:
+    global _scenes
+    if _scenes is None:
+        hip_lib()  # libtracer.so links against it
+        p = LIB_DIR / "libtracer.so"
+        if not p.exists():
+            raise NativeLibraryMissing(f"{p} is not built (run python -m tracer_amd.build)")
+        lib = C.CDLL(str(p))
+        lib.trs_create.argtypes = [C.c_int32, C.c_uint64, C.c_double, C.POINTER(C.c_char_p)]
+        lib.trs_create.restype = C.c_void_p
+        lib.trs_desc.argtypes = [C.c_void_p]
+        lib.trs_desc.restype = C.POINTER(TrSceneDesc)
+        lib.trs_render_defaults.argtypes = [C.c_void_p, C.POINTER(TrRenderParams)]
+        lib.trs_render_defaults.restype = None
+        lib.trs_max_leaf.argtypes = [C.c_void_p]
+        lib.trs_destroy.argtypes = [C.c_void_p]
+        lib.trs_destroy.restype = None
+        _scenes = lib
+    return _scenes
+
+
+def loaded_libraries() -> list[str]:
+    return [str(LIB_DIR / n) for n, l in (("libtracer_hip.so", _hip), ("libtracer.so", _scenes)) if l is not None]
+
+
+@dataclass
+class Scene:
+    """A flattened scene held in numpy arrays; desc() gives the C view."""
+    positions: np.ndarray
+    indices: np.ndarray
+    material_ids: np.ndarray
+    materials: list
+    normals: np.ndarray | None = None
+    uvs: np.ndarray | None = None
+    emission: np.ndarray | None = None
+    medium_inside: np.ndarray | None = None
+    medium_outside: np.ndarray | None = None
+    media: list = field(default_factory=list)
+    env: tuple | None = None
+    sun: tuple | None = None          # (direction, irradiance)
+    camera: abi.TrCamera | None = None
+    defaults: TrRenderParams | None = None
+    max_leaf: int = 3
+
+    def __post_init__(self):
+        self._keep = []
+
+    @property
+    def num_triangles(self) -> int:
+        return int(self.indices.size // 3)
+
+    def desc(self) -> TrSceneDesc:
+        d = TrSceneDesc()
+        self.positions = np.ascontiguousarray(self.positions, np.float32).reshape(-1)
+        self.indices = np.ascontiguousarray(self.indices, np.int32).reshape(-1)
+        self.material_ids = np.ascontiguousarray(self.material_ids, np.int32).reshape(-1)
+        d.num_vertices = self.positions.size // 3
+        d.positions = fptr(self.positions)
+        if self.normals is not None:
+            self.normals = np.ascontiguousarray(self.normals, np.float32).reshape(-1)
+        d.normals = fptr(self.normals)
+        if self.uvs is not None:
+            self.uvs = np.ascontiguousarray(self.uvs, np.float32).reshape(-1)
+        d.uvs = fptr(self.uvs)
+        d.num_triangles = self.indices.size // 3
+        d.indices = iptr(self.indices)
+        d.material_ids = iptr(self.material_ids)
+        if self.emission is not None:
+            self.emission = np.ascontiguousarray(self.emission, np.float32).reshape(-1)
+        d.emission = fptr(self.emission)
+        for k in ("medium_inside", "medium_outside"):
+            v = getattr(self, k)
+            if v is not None:
+                setattr(self, k, np.ascontiguousarray(v, np.int32).reshape(-1))
+        d.medium_inside = iptr(self.medium_inside)
+        d.medium_outside = iptr(self.medium_outside)
+        mats = (TrMaterial * len(self.materials))(*self.materials)
+        d.num_materials = len(self.materials)
+        d.materials = C.cast(mats, C.POINTER(TrMaterial))
+        media = (TrMedium * max(1, len(self.media)))(*self.media) if self.media else None
+        d.num_media = len(self.media)
+        d.media = C.cast(media, C.POINTER(TrMedium)) if media is not None else C.POINTER(TrMedium)()
+        if self.env is not None:
+            d.has_env = 1
+            d.env_radiance = abi.f3(*self.env)
+        if self.sun is not None:
+            d.has_sun = 1
+            d.sun_direction = abi.f3(*self.sun[0])
+            d.sun_irradiance = abi.f3(*self.sun[1])
+        d.camera = self.camera if self.camera is not None else abi.TrCamera()
+        self._keep = [mats, media]
+        return d
+
+    def render_params(self, **kw) -> TrRenderParams:
+        p = TrRenderParams()
+        if self.defaults is not None:
+            C.memmove(C.byref(p), C.byref(self.defaults), C.sizeof(p))
+        else:
+            p.width, p.height, p.spp, p.min_depth, p.max_depth = 64, 64, 1, 3, 8
+            p.direct_light_sample_num, p.filter_type, p.filter_radius = 1, abi.TR_FILTER_BOX, 0.5
+            p.guide_fraction, p.guide_lr = 0.5, 1e-3
+        for k, v in kw.items():
+            setattr(p, k, v)
+        return p
+
+
+def config_scene(config: int, seed: int | None = None, detail: float = 1.0) -> Scene:
+    """Synthetic configuration 1..5 built through the C++ factory API."""
+    lib = scenes_lib()
+    err = C.c_char_p()
+    h = lib.trs_create(config, config - 1 if seed is None else seed, detail, C.byref(err))
+    if not h:
+        raise TracerError(abi.TR_ERR_INVALID, (err.value or b"").decode())
+    try:
+        d = lib.trs_desc(h).contents
+        nv, nt = d.num_vertices, d.num_triangles
+
+        def arr(ptr, n, dt):
+            if not ptr:
+                return None
+            return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dt, copy=True)
+
+        mats = [TrMaterial.from_buffer_copy(d.materials[i]) for i in range(d.num_materials)]
+        media = [TrMedium.from_buffer_copy(d.media[i]) for i in range(d.num_media)]
+        defaults = TrRenderParams()
+        lib.trs_render_defaults(h, C.byref(defaults))
+        s = Scene(
+            positions=arr(d.positions, 3 * nv, np.float32), indices=arr(d.indices, 3 * nt, np.int32),
+            material_ids=arr(d.material_ids, nt, np.int32), materials=mats,
+            normals=arr(d.normals, 3 * nv, np.float32), uvs=arr(d.uvs, 2 * nv, np.float32),
+            emission=arr(d.emission, 3 * nt, np.float32), medium_inside=arr(d.medium_inside, nt, np.int32),
+            medium_outside=arr(d.medium_outside, nt, np.int32), media=media,
+            env=tuple(d.env_radiance) if d.has_env else None,
+            sun=(tuple(d.sun_direction), tuple(d.sun_irradiance)) if d.has_sun else None,
+            camera=abi.TrCamera.from_buffer_copy(d.camera), defaults=defaults, max_leaf=lib.trs_max_leaf(h))
+        return s
+    finally:
+        lib.trs_destroy(h)
+
+
+class Context:
+    """One device context (tr_ctx): a stream, an uploaded scene and its BVH."""
+
+    def __init__(self, device: int = 0):
+        self.lib = hip_lib()
+        self.h = C.c_void_p()
+        st = self.lib.tr_create(C.byref(self.h), device)
+        if st != abi.TR_OK:
+            raise TracerError(st, f"tr_create failed on device {device}")
+
+    def close(self):
+        if self.h:
+            self.lib.tr_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st: int):
+        if st != abi.TR_OK:
+            raise TracerError(st, self.lib.tr_last_error(self.h).decode())
+
+    def load_scene(self, scene: Scene, max_leaf: int | None = None, threads: int = 0):
+        d = scene.desc()
+        self._check(self.lib.tr_scene_upload(self.h, C.byref(d)))
+        o = abi.TrBvhOpts(max_leaf if max_leaf is not None else 4, threads)
+        self._check(self.lib.tr_bvh_build(self.h, C.byref(o)))
+
+    def bvh_info(self) -> dict:
+        n, nb, tb = C.c_int64(), C.c_int64(), C.c_int64()
+        self._check(self.lib.tr_bvh_info(self.h, C.byref(n), C.byref(nb), C.byref(tb)))
+        return {"nodes": n.value, "node_bytes": nb.value, "tri_bytes": tb.value}
+
+    def render(self, params: TrRenderParams, film: np.ndarray | None = None, want_image: bool = True):
+        W, H = params.width, params.height
+        acc = film is not None
+        sums = film if acc else np.zeros((H, W, 4), np.float64)
+        img = np.zeros((H, W, 3), np.float32) if want_image else None
+        st = TrRenderStats()
+        self._check(self.lib.tr_render(self.h, C.byref(params), dptr(sums), 1 if acc else 0,
+                                       fptr(img) if img is not None else fptr(None), C.byref(st)))
+        return sums, img, st
+
+    def trace_closest(self, rays: np.ndarray) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = rays.shape[0]
+        hits = np.zeros((n, 4), np.float32)
+        self._check(self.lib.tr_trace_closest(self.h, fptr(rays), fptr(hits), n))
+        return hits
+
+    def trace_closest_counted(self, rays: np.ndarray):
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = rays.shape[0]
+        hits = np.zeros((n, 4), np.float32)
+        nv = np.zeros(n, np.uint32)
+        nt = np.zeros(n, np.uint32)
+        self._check(self.lib.tr_trace_closest_counted(self.h, fptr(rays), fptr(hits),
+                                                      nv.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                                      nt.ctypes.data_as(C.POINTER(C.c_uint32)), n))
+        return hits, nv, nt
+
+    def trace_any(self, rays: np.ndarray) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = rays.shape[0]
+        occ = np.zeros(n, np.int32)
+        self._check(self.lib.tr_trace_any(self.h, fptr(rays), iptr(occ), n))
+        return occ
+
+    def bsdf_eval(self, mats: list, ids: np.ndarray, wi: np.ndarray, wo: np.ndarray, ng: np.ndarray | None = None):
+        n = len(ids)
+        arr = (TrMaterial * len(mats))(*mats)
+        ids = np.ascontiguousarray(ids, np.int32)
+        wi = np.ascontiguousarray(wi, np.float32)
+        wo = np.ascontiguousarray(wo, np.float32)
+        ngc = np.ascontiguousarray(ng, np.float32) if ng is not None else None
+        f = np.zeros((n, 3), np.float32)
+        pdf = np.zeros(n, np.float32)
+        self._check(self.lib.tr_bsdf_eval_batch(self.h, arr, len(mats), iptr(ids), fptr(wi), fptr(wo), fptr(ngc), n,
+                                                fptr(f), fptr(pdf)))
+        return f, pdf
+
+    def bsdf_sample(self, mats: list, ids: np.ndarray, wo: np.ndarray, u: np.ndarray, ng: np.ndarray | None = None):
+        n = len(ids)
+        arr = (TrMaterial * len(mats))(*mats)
+        ids = np.ascontiguousarray(ids, np.int32)
+        wo = np.ascontiguousarray(wo, np.float32)
+        u = np.ascontiguousarray(u, np.float32)
+        ngc = np.ascontiguousarray(ng, np.float32) if ng is not None else None
+        wi = np.zeros((n, 3), np.float32)
+        f = np.zeros((n, 3), np.float32)
+        pdf = np.zeros(n, np.float32)
+        self._check(self.lib.tr_bsdf_sample_batch(self.h, arr, len(mats), iptr(ids), fptr(wo), fptr(ngc), fptr(u), n,
+                                                  fptr(wi), fptr(f), fptr(pdf)))
+        return wi, f, pdf
+
+
+def gpu_available() -> bool:
+    try:
+        import torch  # noqa: F401  (device discovery only)
+        return torch.cuda.device_count() > 0
+    except Exception:
+        return False
+
+
+def version() -> str:
+    return hip_lib().tr_version().decode()
+
+
+__all__ = ["Context", "Scene", "config_scene", "hip_lib", "scenes_lib", "NativeLibraryMissing", "TracerError",
+           "gpu_available", "version", "loaded_libraries"]
+
