@@ -138,7 +138,8 @@ typedef struct tr_render_params {
     int32_t guide_train_passes;        /* passes that train the guide before it is used (guide=1) */
     float guide_fraction;              /* one-sample MIS probability of the guide (0.5) */
     float guide_lr;                    /* Adam learning rate (1e-3) */
-    int32_t profile;                   /* 1 = time kernels with HIP events */
+    int32_t profile;                   /* 1 = time kernels with HIP events; 2 = also count BVH
+                                          node visits / triangle tests (slower) */
     int32_t reserved;
 } tr_render_params;
 
@@ -156,6 +157,8 @@ typedef struct tr_render_stats {
     double ms_other;
     double ms_guide;              /* guide inference + training (profile = 1) */
     uint64_t extend_launches;
+    uint64_t node_visits;         /* profile = 2: BVH nodes visited by extension rays */
+    uint64_t tri_tests;           /* profile = 2: triangle tests by extension rays */
     uint64_t pad;
 } tr_render_stats;
 

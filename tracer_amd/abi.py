@@ -87,7 +87,8 @@ class TrRenderStats(C.Structure):
         ("nonfinite_dropped", C.c_uint64), ("guide_fallbacks", C.c_uint64), ("guide_train_samples", C.c_uint64),
         ("ms_total", C.c_double), ("ms_extend", C.c_double), ("ms_shade", C.c_double), ("ms_shadow", C.c_double),
         ("ms_other", C.c_double), ("ms_guide", C.c_double),
-        ("extend_launches", C.c_uint64), ("pad", C.c_uint64),
+        ("extend_launches", C.c_uint64), ("node_visits", C.c_uint64), ("tri_tests", C.c_uint64),
+        ("pad", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:

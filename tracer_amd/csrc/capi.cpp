@@ -710,7 +710,7 @@ tr_status tr_render(tr_ctx* c, const tr_render_params* prm, double* film_sums, i
             int cur = 0;
             timed(3, [&] { launch_generate(c->ds, p, rc, num_paths, cur, st); });
             for (int b = 0; b <= P.max_depth; ++b) {
-                timed(0, [&] { launch_extend(c->ds.tr, p, cur, num_paths, st); });
+                timed(0, [&] { launch_extend(c->ds.tr, p, cur, num_paths, P.profile == 2, st); });
                 timed(1, [&] { launch_shade(c->ds, p, rc, cur, num_paths, st); });
                 timed(2, [&] { launch_shadow(c->ds.tr, p, num_paths, 0, st); });
                 ++launches;
@@ -752,6 +752,8 @@ tr_status tr_render(tr_ctx* c, const tr_render_params* prm, double* film_sums, i
             stats->ms_shadow = ms_shd;
             stats->ms_other = ms_oth;
             stats->extend_launches = launches;
+            stats->node_visits = hs[3];
+            stats->tri_tests = hs[4];
         }
     });
 }

@@ -67,6 +67,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_counted(TraceScene sc, co
 }
 
 // Wavefront extension: closest hit for every path slot in the current queue.
+// COUNT: also accumulate node visits and triangle tests into stats[3..4].
+template <bool COUNT>
 __global__ __launch_bounds__(kTraceBlock) void k_extend(TraceScene sc, DevPaths p, int cur) {
     __shared__ TraceStack<kTraceBlock, kLdsStack> st;
     const int tid = threadIdx.x;
@@ -83,9 +85,14 @@ __global__ __launch_bounds__(kTraceBlock) void k_extend(TraceScene sc, DevPaths 
         const int slot = q[i];
         const float4 a = p.ray_o[slot], b = p.ray_d[slot];
         TraceHit h;
-        trace<false, false, kTraceBlock, kLdsStack>(sc, xyz(a), a.w, xyz(b), b.w, st, tid, p.ovf_node + gt,
-                                                    p.ovf_t + gt, stride, h);
+        uint32_t cv = 0, ct = 0;
+        trace<false, COUNT, kTraceBlock, kLdsStack>(sc, xyz(a), a.w, xyz(b), b.w, st, tid, p.ovf_node + gt,
+                                                    p.ovf_t + gt, stride, h, &cv, &ct);
         p.hit[slot] = make_float4(h.t, __int_as_float(h.tri), h.b1, h.b2);
+        if (COUNT) {
+            atomicAdd(&p.stats[3], (unsigned long long)cv);
+            atomicAdd(&p.stats[4], (unsigned long long)ct);
+        }
     }
 }
 
@@ -139,8 +146,11 @@ void launch_trace_counted(const TraceScene& sc, const float4* rays, float4* hits
     hipLaunchKernelGGL(k_trace_counted, dim3(trace_blocks(n)), dim3(kTraceBlock), 0, st, sc, rays, hits, nvis, ntri, n,
                        ovf_node, ovf_t);
 }
-void launch_extend(const TraceScene& sc, const DevPaths& p, int cur, int max_paths, hipStream_t st) {
-    hipLaunchKernelGGL(k_extend, dim3(trace_blocks(max_paths)), dim3(kTraceBlock), 0, st, sc, p, cur);
+void launch_extend(const TraceScene& sc, const DevPaths& p, int cur, int max_paths, bool count, hipStream_t st) {
+    if (count)
+        hipLaunchKernelGGL(k_extend<true>, dim3(trace_blocks(max_paths)), dim3(kTraceBlock), 0, st, sc, p, cur);
+    else
+        hipLaunchKernelGGL(k_extend<false>, dim3(trace_blocks(max_paths)), dim3(kTraceBlock), 0, st, sc, p, cur);
 }
 void launch_shadow(const TraceScene& sc, const DevPaths& p, int max_shadow, int accumulate_atomic, hipStream_t st) {
     hipLaunchKernelGGL(k_shadow, dim3(trace_blocks(max_shadow)), dim3(kTraceBlock), 0, st, sc, p, accumulate_atomic);

@@ -46,11 +46,11 @@ struct BsdfSample {
     float pdf;
 };
 
-TR_D float pow5(float x) { const float t2 = x * x; return t2 * t2 * x; }
-TR_D float one_minus_5(float c) { return pow5(1.0f - c); }
+TR_HD float pow5(float x) { const float t2 = x * x; return t2 * t2 * x; }
+TR_HD float one_minus_5(float c) { return pow5(1.0f - c); }
 
 // reflection.hpp:16-40
-TR_D float fresnel_dielectric(float eta_i, float eta_o, float cos_i) {
+TR_HD float fresnel_dielectric(float eta_i, float eta_o, float cos_i) {
     if (cos_i < 0.0f) { const float t = eta_i; eta_i = eta_o; eta_o = t; cos_i = -cos_i; }
     const float sin_i = sqrtf(fmaxf(0.0f, 1.0f - cos_i * cos_i));
     const float sin_t = eta_o / eta_i * sin_i;
@@ -61,16 +61,26 @@ TR_D float fresnel_dielectric(float eta_i, float eta_o, float cos_i) {
     return 0.5f * (para * para + perp * perp);
 }
 
+// 1 - fresnel_dielectric(eta_i, eta_o, cos_i) when the transmitted cosine
+// cos_t is known, formed without cancellation: 1 - r^2 = (1 - r)(1 + r) for
+// the s and p amplitudes, i.e.
+// T = 2 ni no ci ct (1/(ni ci + no ct)^2 + 1/(no ci + ni ct)^2).
+TR_HD float fresnel_transmittance(float eta_i, float eta_o, float cos_i, float cos_t) {
+    if (cos_i < 0.0f) { const float t = eta_i; eta_i = eta_o; eta_o = t; cos_i = -cos_i; }
+    const float a = eta_i * cos_i + eta_o * cos_t, b = eta_o * cos_i + eta_i * cos_t;
+    return 2.0f * eta_i * eta_o * cos_i * cos_t * (1.0f / (a * a) + 1.0f / (b * b));
+}
+
 // Anisotropic GTR2 (GGX) normal distribution, microfacet.cpp:15-21, for a
 // unit half vector h: 1 / (pi ax ay (hx^2/ax^2 + hy^2/ay^2 + hz^2)^2).
-TR_D float D_gtr2(f3 h, float ax, float ay) {
+TR_HD float D_gtr2(f3 h, float ax, float ay) {
     const float x = h.x / ax, y = h.y / ay;
     const float s = x * x + y * y + h.z * h.z;
     return 1.0f / (kPi * ax * ay * s * s);
 }
 // Smith G1 for anisotropic GTR2, microfacet.cpp:38-43:
 // 2 / (1 + sqrt(1 + (ax^2 x^2 + ay^2 y^2) / z^2)).
-TR_D float G1_gtr2(f3 w, float ax, float ay) {
+TR_HD float G1_gtr2(f3 w, float ax, float ay) {
     const float r2 = w.x * w.x + w.y * w.y;
     if (r2 <= 0.0f) return 1.0f;
     const float a = ax * w.x, b = ay * w.y;
@@ -78,14 +88,14 @@ TR_D float G1_gtr2(f3 w, float ax, float ay) {
     return 2.0f / (1.0f + sqrtf(1.0f + t));
 }
 // GTR1 (clearcoat) distribution, microfacet.cpp:29-35.
-TR_D float D_gtr1(f3 h, float alpha) {
+TR_HD float D_gtr1(f3 h, float alpha) {
     if (alpha >= 1.0f) return kInvPi;
     const float a2 = alpha * alpha;
     const float sin2 = h.x * h.x + h.y * h.y;
     return (a2 - 1.0f) / (2.0f * kPi * logf(alpha) * (a2 * h.z * h.z + sin2));
 }
 // Smith G for GTR2 with fixed alpha, microfacet.cpp:46-51.
-TR_D float G1_smith(f3 w, float alpha) {
+TR_HD float G1_smith(f3 w, float alpha) {
     const float r2 = w.x * w.x + w.y * w.y;
     if (r2 <= 0.0f) return 1.0f;
     const float t2 = r2 / (w.z * w.z);
@@ -93,7 +103,7 @@ TR_D float G1_smith(f3 w, float alpha) {
 }
 
 // microfacet.cpp:58-80 (Heitz 2014 visible-normal sampling)
-TR_D f3 sample_vndf(f3 ve, float ax, float ay, float u, float v) {
+TR_HD f3 sample_vndf(f3 ve, float ax, float ay, float u, float v) {
     const f3 vh = normalize(mk(ax * ve.x, ay * ve.y, ve.z));
     const float lensq = vh.x * vh.x + vh.y * vh.y;
     const f3 t1 = lensq > kEps ? mk(-vh.y, vh.x, 0.0f) * (1.0f / sqrtf(lensq)) : mk(1.0f, 0.0f, 0.0f);
@@ -109,7 +119,7 @@ TR_D f3 sample_vndf(f3 ve, float ax, float ay, float u, float v) {
     return normalize(mk(ax * nh.x, ay * nh.y, fmaxf(0.0f, nh.z)));
 }
 // microfacet.cpp:92-109 (D * cos sampling of anisotropic GTR2)
-TR_D f3 sample_gtr2_dcos(float ax, float ay, float u, float v) {
+TR_HD f3 sample_gtr2_dcos(float ax, float ay, float u, float v) {
     float s, c;
     sincosf(2.0f * kPi * u, &s, &c);
     float sp = ay * s, cp = ax * c;
@@ -122,7 +132,7 @@ TR_D f3 sample_gtr2_dcos(float ax, float ay, float u, float v) {
     return normalize(mk(st * cp, st * sp, ct));
 }
 // microfacet.cpp:125-134
-TR_D f3 sample_gtr1(float alpha, float u, float v) {
+TR_HD f3 sample_gtr1(float alpha, float u, float v) {
     float s, c;
     sincosf(2.0f * kPi * u, &s, &c);
     const float a2 = alpha * alpha;
@@ -133,24 +143,34 @@ TR_D f3 sample_gtr1(float alpha, float u, float v) {
 
 // ---- Disney lobes (local frame) -------------------------------------------
 // disney.cpp:60-109
-TR_D f3 dis_f_transmission(const DevMaterial& m, f3 wi, f3 wo) {
+TR_HD f3 dis_f_transmission(const DevMaterial& m, f3 wi, f3 wo) {
     const float ci = wi.z, co = wo.z;
     const float eta = co > 0.0f ? m.ior : 1.0f / m.ior;
-    f3 h = normalize(wo + wi * eta);
-    if (h.z < 0.0f) h = -h;
+    const f3 v = wo + wi * eta;
+    const float lv = length(v);
+    f3 h = v * (1.0f / lv);
+    // sdem = dot(wo, h) + eta dot(wi, h) = dot(v, h) = +-|v| (no cancellation)
+    float sdem = lv;
+    if (h.z < 0.0f) { h = -h; sdem = -lv; }
     const float dwo = dot(wo, h), dwi = dot(wi, h);
-    const float F = fresnel_dielectric(m.ior, 1.0f, dwo);
+    // With h along wo + eta wi, Snell's law holds about h for any unit pair,
+    // so the transmitted cosine of fresnel_dielectric(ior, 1, dwo) is |dwi|
+    // exactly when wo lies on h's side in the orientation the Fresnel term
+    // assumes; computing it as sqrt(1 - sin_t^2) loses ~1/cos_t^2 digits
+    // near the critical angle.
+    const bool snell = (dwo > 0.0f) == (co > 0.0f);
+    const float T = snell ? fresnel_transmittance(m.ior, 1.0f, dwo, fabsf(dwi))
+                          : 1.0f - fresnel_dielectric(m.ior, 1.0f, dwo);
     const float D = D_gtr2(h, m.tax, m.tay);
     const float G = G1_gtr2(wi, m.tax, m.tay) * G1_gtr2(wo, m.tax, m.tay);
-    const float sdem = dwo + eta * dwi;
     // eta^2 * (1/eta)^2 = 1 for radiance transport (disney.cpp:98,102-105)
-    const float val = (1.0f - F) * D * G * dwi * dwo / (ci * co * sdem * sdem);
+    const float val = T * D * G * dwi * dwo / (ci * co * sdem * sdem);
     const float tf = co > 0.0f ? m.transmission : 1.0f;
     const float k = (1.0f - m.metallic) * tf * fabsf(val);
     return mk(sqrtf(m.C.x), sqrtf(m.C.y), sqrtf(m.C.z)) * k;
 }
 // disney.cpp:110-145
-TR_D f3 dis_f_inner(const DevMaterial& m, f3 wi, f3 wo) {
+TR_HD f3 dis_f_inner(const DevMaterial& m, f3 wi, f3 wo) {
     const f3 h = -normalize(wi + wo);
     const float F = fresnel_dielectric(m.ior, 1.0f, dot(wo, h));
     const float D = D_gtr2(h, m.tax, m.tay);
@@ -158,7 +178,7 @@ TR_D f3 dis_f_inner(const DevMaterial& m, f3 wi, f3 wo) {
     return m.C * (m.transmission * fabsf(F * D * G / (4.0f * wi.z * wo.z)));
 }
 // disney.cpp:146-209 (outside reflection: diffuse + retro, sheen, specular, clearcoat)
-TR_D f3 dis_f_reflect(const DevMaterial& m, f3 wi, f3 wo) {
+TR_HD f3 dis_f_reflect(const DevMaterial& m, f3 wi, f3 wo) {
     const float ci = wi.z, co = wo.z;
     const f3 h = normalize(wi + wo);
     const float cd = dot(wi, h);
@@ -187,23 +207,25 @@ TR_D f3 dis_f_reflect(const DevMaterial& m, f3 wi, f3 wo) {
     return r;
 }
 // disney.cpp:210-235, with cos(theta_h) as the D * cos sampling density
-TR_D float dis_pdf_transmission(const DevMaterial& m, f3 wi, f3 wo) {
+TR_HD float dis_pdf_transmission(const DevMaterial& m, f3 wi, f3 wo) {
     const float eta = wo.z > 0.0f ? m.ior : 1.0f / m.ior;
-    f3 h = normalize(wo + wi * eta);
-    if (h.z < 0.0f) h = -h;
+    const f3 v = wo + wi * eta;
+    const float lv = length(v);
+    f3 h = v * (1.0f / lv);
+    float sdem = lv;
+    if (h.z < 0.0f) { h = -h; sdem = -lv; }
     const float dwo = dot(wo, h), dwi = dot(wi, h);
     if (((wo.z > 0.0f) != (dwo > 0.0f)) || ((wi.z > 0.0f) != (dwi > 0.0f))) return 0.0f;
-    const float sdem = dwo + eta * dwi;
     const float jac = eta * eta * dwi / (sdem * sdem);
     return fabsf(h.z * D_gtr2(h, m.tax, m.tay) * jac);
 }
 // disney.cpp:236-251
-TR_D float dis_pdf_inner(const DevMaterial& m, f3 wi, f3 wo) {
+TR_HD float dis_pdf_inner(const DevMaterial& m, f3 wi, f3 wo) {
     const f3 h = -normalize(wi + wo);
     return fabsf(h.z * D_gtr2(h, m.tax, m.tay) / (4.0f * dot(wi, h)));
 }
 // disney.cpp:499-534 (local, |z| >= eps already checked)
-TR_D float dis_pdf_local(const DevMaterial& m, f3 wi, f3 wo) {
+TR_HD float dis_pdf_local(const DevMaterial& m, f3 wi, f3 wo) {
     if (wo.z < 0.0f) {
         if (m.transmission == 0.0f) return 0.0f;
         const float F = fminf(fmaxf(fresnel_dielectric(m.ior, 1.0f, wo.z), 0.1f), 0.9f);
@@ -217,7 +239,7 @@ TR_D float dis_pdf_local(const DevMaterial& m, f3 wi, f3 wo) {
     const float pc = h.z * D_gtr1(h, m.cc_rough) / (4.0f * dot(wi, h));
     return m.w_diffuse * pd + m.w_spec * ps + m.w_cc * pc;
 }
-TR_D f3 dis_f_local(const DevMaterial& m, f3 wi, f3 wo) {
+TR_HD f3 dis_f_local(const DevMaterial& m, f3 wi, f3 wo) {
     if (wi.z * wo.z < 0.0f) {
         if (m.transmission == 0.0f) return f3s(0.0f);
         return dis_f_transmission(m, wi, wo);
@@ -229,7 +251,7 @@ TR_D f3 dis_f_local(const DevMaterial& m, f3 wi, f3 wo) {
     return dis_f_reflect(m, wi, wo);
 }
 // disney.cpp:285-305 (refraction about the sampled normal)
-TR_D bool dis_sample_transmission(const DevMaterial& m, f3 wo, float u, float v, f3& wi) {
+TR_HD bool dis_sample_transmission(const DevMaterial& m, f3 wo, float u, float v, f3& wi) {
     const f3 h = sample_gtr2_dcos(m.tax, m.tay, u, v);
     if (h.z <= 0.0f) return false;
     const float dho = dot(h, wo);
@@ -244,7 +266,7 @@ TR_D bool dis_sample_transmission(const DevMaterial& m, f3 wo, float u, float v,
     if (wi.z * wo.z > 0.0f || ((wi.z > 0.0f) != (dot(h, wi) > 0.0f))) return false;
     return true;
 }
-TR_D bool dis_sample_local(const DevMaterial& m, f3 wo, float su, float u, float v, f3& wi) {
+TR_HD bool dis_sample_local(const DevMaterial& m, f3 wo, float su, float u, float v, f3& wi) {
     if (wo.z < 0.0f) {
         if (m.transmission == 0.0f) return false;
         const float F = fminf(fmaxf(fresnel_dielectric(m.ior, 1.0f, wo.z), 0.1f), 0.9f);
@@ -272,9 +294,9 @@ TR_D bool dis_sample_local(const DevMaterial& m, f3 wo, float su, float u, float
 }
 
 // ---- full BSDF: fringe handling, local normalisation, normal correction ----
-TR_D bool fringe(f3 w, f3 ng) { return (dot(w, ng) > 0.0f) != (w.z > 0.0f); }
+TR_HD bool fringe(f3 w, f3 ng) { return (dot(w, ng) > 0.0f) != (w.z > 0.0f); }
 // coordinate.hpp:43-47 with the shading normal = +z
-TR_D float normal_correct(f3 ng, f3 wi) {
+TR_HD float normal_correct(f3 ng, f3 wi) {
     const float lw = length(wi);
     const float dem = fabsf(dot(ng, wi) / (length(ng) * lw));
     return dem < kEps ? 1.0f : fabsf(wi.z / lw) / dem;
@@ -286,7 +308,7 @@ struct BsdfEval {
 };
 
 // eval + pdf for directions wi, wo (shading frame), geometric normal ng.
-TR_D BsdfEval bsdf_eval(const DevMaterial& m, f3 wi, f3 wo, f3 ng) {
+TR_HD BsdfEval bsdf_eval(const DevMaterial& m, f3 wi, f3 wo, f3 ng) {
     BsdfEval r{f3s(0.0f), 0.0f};
     if (fringe(wi, ng) || fringe(wo, ng)) {
         const bool gi = dot(wi, ng) > 0.0f, go = dot(wo, ng) > 0.0f;
@@ -309,7 +331,7 @@ TR_D BsdfEval bsdf_eval(const DevMaterial& m, f3 wi, f3 wo, f3 ng) {
 
 // Sample wi given wo; returns pdf = 0 when no valid direction was produced.
 // (gt, gb): a tangent frame around ng for the fringe fallback.
-TR_D BsdfSample bsdf_sample(const DevMaterial& m, f3 wo, f3 ng, float su, float u, float v) {
+TR_HD BsdfSample bsdf_sample(const DevMaterial& m, f3 wo, f3 ng, float su, float u, float v) {
     BsdfSample s{f3s(0.0f), f3s(0.0f), 0.0f};
     if (fringe(wo, ng)) {
         // bsdf.hpp:141-153: cosine sampling about the geometric normal

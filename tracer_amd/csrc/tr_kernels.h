@@ -24,7 +24,8 @@ struct DevPaths {
     float4* sh_d;     //              (direction, tmax)
     float4* sh_c;     //              (contribution rgb, path slot)
     unsigned int* counters;   // [0..1] queue lengths, [2] shadow length
-    unsigned long long* stats;  // [0] extension rays, [1] shadow rays, [2] non-finite samples
+    unsigned long long* stats;  // [0] extension rays, [1] shadow rays, [2] non-finite samples,
+                                // [3] node visits, [4] triangle tests
     int* ovf_node;    // traversal stack spill area
     float* ovf_t;
     int ovf_threads;  // threads the spill area is sized for
@@ -50,7 +51,7 @@ void launch_trace_any(const TraceScene& sc, const float4* rays, int* occ, int64_
 void launch_trace_counted(const TraceScene& sc, const float4* rays, float4* hits, uint32_t* nvis, uint32_t* ntri,
                           int64_t n, int* ovf_node, float* ovf_t, int ovf_threads, hipStream_t st);
 // wavefront: extend the current queue, trace the shadow queue
-void launch_extend(const TraceScene& sc, const DevPaths& p, int cur, int max_paths, hipStream_t st);
+void launch_extend(const TraceScene& sc, const DevPaths& p, int cur, int max_paths, bool count, hipStream_t st);
 void launch_shadow(const TraceScene& sc, const DevPaths& p, int max_shadow, int accumulate_atomic, hipStream_t st);
 // Grid size the spill area must cover for launches of up to n rays.
 int trace_grid_threads(int64_t n);
