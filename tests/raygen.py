@@ -54,61 +54,66 @@ def hit_prims(h):
     return h[:, 1].copy().view(np.int32)
 
 
-def compare_with_mt(scene, rays, wt_hits, mt_hits, min_cos=1e-2):
-    """Agreement of watertight hits with the reference Moller-Trumbore hits.
-
-    Returns (id mismatch rate, fraction of mismatches that are edge cases,
-    max relative t difference over hits with |cos(ray, n_g)| >= min_cos).  A
-    mismatch is an edge case when the two triangles share a vertex, when
-    either hit lies within 1e-3 (barycentric) of its triangle's edge — where
-    Moller-Trumbore can leak through to a farther triangle — when either hit
-    is grazing (|cos| < min_cos), or when the two t agree to 1e-5.  Below min_cos the FP32 t of any
-    ray/triangle test is ill-conditioned (error ~ eps / cos)."""
+def exact_crossing(scene, rays, prims):
+    """FP64 crossing of each ray with the plane of the given triangle:
+    (min barycentric, |cos(ray, n_g)|, t, |o - A|, shortest edge)."""
     P = scene.positions.reshape(-1, 3).astype(np.float64)
     I = scene.indices.reshape(-1, 3)
+    v0, v1, v2 = P[I[prims, 0]], P[I[prims, 1]], P[I[prims, 2]]
+    o = rays[:, 0:3].astype(np.float64)
+    d = rays[:, 4:7].astype(np.float64)
+    e1, e2 = v1 - v0, v2 - v0
+    p = np.cross(d, e2)
+    det = (e1 * p).sum(1)
+    tv = o - v0
+    q = np.cross(tv, e1)
+    u = (tv * p).sum(1) / det
+    w = (d * q).sum(1) / det
+    t = (e2 * q).sum(1) / det
+    n = np.cross(e1, e2)
+    cos = np.abs((n * d).sum(1)) / (np.linalg.norm(n, axis=1) * np.linalg.norm(d, axis=1))
+    emin = np.minimum(np.minimum(np.linalg.norm(e1, axis=1), np.linalg.norm(e2, axis=1)),
+                      np.linalg.norm(v2 - v1, axis=1))
+    return np.minimum(np.minimum(u, w), 1 - u - w), cos, t * np.linalg.norm(d, axis=1), np.linalg.norm(tv, axis=1), emin
+
+
+EPS32 = 2.0 ** -24
+
+
+def well_conditioned(scene, rays, prims):
+    """A hit is well conditioned when the ray crosses the triangle inside its
+    edges by more than FP32 can resolve from the ray origin (exact barycentric
+    margin >= max(1e-3, 16 eps |o - A| / shortest edge)), at |cos| >= 0.02,
+    with t condition number |o - A| / (t cos) <= 50."""
+    out = np.zeros(len(prims), bool)
+    k = prims >= 0
+    if k.any():
+        mg, cs, tt, dist, emin = exact_crossing(scene, rays[k], prims[k])
+        need = np.maximum(1e-3, 16 * EPS32 * dist / emin)
+        out[k] = (mg >= need) & (cs >= 0.02) & (dist / np.maximum(tt * cs, 1e-30) <= 50)
+    return out
+
+
+def compare_with_mt(scene, rays, wt_hits, mt_hits):
+    """Agreement of watertight closest hits with the reference's
+    Moller-Trumbore closest hits (src/shape/triangle.cpp:145-175).
+
+    On the well-conditioned suite (tests.raygen.well_conditioned holds for the
+    watertight hit) both tests must name the same triangle unless the other
+    test's triangle is itself not well conditioned (an edge or grazing hit
+    that FP32 cannot resolve), and t must agree within 1e-5 relative.
+    Everything else is reported, not asserted.  Returns a dict."""
     a, b = hit_prims(wt_hits), hit_prims(mt_hits)
-    mism = a != b
-    adj = 0
-    def near_edge(i, prim):
-        # exact (FP64) barycentrics of the ray's crossing with the triangle's plane
-        v = P[I[prim]]
-        o = rays[i, 0:3].astype(np.float64)
-        d = rays[i, 4:7].astype(np.float64)
-        e1, e2 = v[1] - v[0], v[2] - v[0]
-        p = np.cross(d, e2)
-        det = e1 @ p
-        if det == 0:
-            return True
-        tv = o - v[0]
-        u = (tv @ p) / det
-        w = (d @ np.cross(tv, e1)) / det
-        return min(u, w, 1.0 - u - w) < 1e-3
-
-    def grazing(i, prim):
-        v = P[I[prim]]
-        n = np.cross(v[1] - v[0], v[2] - v[0])
-        d = rays[i, 4:7].astype(np.float64)
-        return abs(n @ d) / (np.linalg.norm(n) * np.linalg.norm(d)) < min_cos
-
-    for i in np.nonzero(mism)[0]:
-        if (a[i] >= 0 and (near_edge(i, a[i]) or grazing(i, a[i]))) or \
-                (b[i] >= 0 and (near_edge(i, b[i]) or grazing(i, b[i]))):
-            adj += 1
-            continue
-        if a[i] < 0 or b[i] < 0:
-            continue
-        if set(I[a[i]].tolist()) & set(I[b[i]].tolist()):
-            adj += 1
-        elif abs(wt_hits[i, 0] - mt_hits[i, 0]) <= 1e-5 * abs(wt_hits[i, 0]):
-            adj += 1
-    ok = (~mism) & (a >= 0)
-    pr = a[ok]
-    A, B, C = P[I[pr, 0]], P[I[pr, 1]], P[I[pr, 2]]
-    n = np.cross(B - A, C - A)
-    n /= np.linalg.norm(n, axis=1, keepdims=True)
-    d = rays[ok, 4:7].astype(np.float64)
-    d /= np.linalg.norm(d, axis=1, keepdims=True)
-    cos = np.abs((n * d).sum(1))
+    well = well_conditioned(scene, rays, a)
+    other_well = well_conditioned(scene, rays, b)
+    same = a == b
+    genuine = well & ~same & (other_well | (b < 0))
+    ok = well & same
     rel = np.abs(wt_hits[ok, 0] - mt_hits[ok, 0]) / np.abs(mt_hits[ok, 0])
-    k = cos >= min_cos
-    return mism.mean(), (adj / max(1, mism.sum()) if mism.any() else 1.0), (rel[k].max() if k.any() else 0.0), k.mean()
+    return {
+        "well_fraction": float(well.mean()),
+        "well_id_mismatch": int(genuine.sum()),
+        "well_id_mismatch_fp32_unresolvable": int((well & ~same).sum() - genuine.sum()),
+        "well_t_rel_max": float(rel.max()) if rel.size else 0.0,
+        "all_id_mismatch_rate": float((~same).mean()),
+    }

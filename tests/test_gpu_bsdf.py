@@ -78,5 +78,6 @@ def test_sample_directions_match_fp64(gpu_ctx):
     print(f"valid agreement {agree.mean():.6f}, both valid {both.sum()}")
     assert agree.mean() > 0.999
     ang = np.linalg.norm(wi[both] - wir[both], axis=1)
-    print(f"direction max abs diff {ang.max():.3g}")
-    assert np.quantile(ang, 0.999) < 1e-4
+    print(f"direction abs diff: median {np.median(ang):.3g}, p99 {np.quantile(ang, 0.99):.3g}, max {ang.max():.3g}")
+    assert np.quantile(ang, 0.99) < 1e-5
+    assert np.quantile(ang, 0.9999) < 1e-3
