@@ -118,7 +118,10 @@ TR_HD f3 sample_vndf(f3 ve, float ax, float ay, float u, float v) {
     const f3 nh = t1 * p1 + t2 * p2 + vh * sqrtf(fmaxf(0.0f, 1.0f - p1 * p1 - p2 * p2));
     return normalize(mk(ax * nh.x, ay * nh.y, fmaxf(0.0f, nh.z)));
 }
-// microfacet.cpp:92-109 (D * cos sampling of anisotropic GTR2)
+// microfacet.cpp:92-109 (D * cos sampling of anisotropic GTR2).  With
+// A = cos^2(phi)/ax^2 + sin^2(phi)/ay^2 and D = (1 - A) v + A:
+// cos^2(theta) = A (1 - v) / D and sin^2(theta) = v / D, both formed
+// directly (sqrt(1 - cos^2) loses half the digits near the pole).
 TR_HD f3 sample_gtr2_dcos(float ax, float ay, float u, float v) {
     float s, c;
     sincosf(2.0f * kPi * u, &s, &c);
@@ -127,17 +130,23 @@ TR_HD f3 sample_gtr2_dcos(float ax, float ay, float u, float v) {
     sp *= nor;
     cp *= nor;
     const float A = (cp / ax) * (cp / ax) + (sp / ay) * (sp / ay);
-    const float ct = sqrtf(A * (1.0f - v) / ((1.0f - A) * v + A));
-    const float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+    const float D = (1.0f - A) * v + A;
+    const float ct = sqrtf(A * (1.0f - v) / D);
+    const float st = sqrtf(v / D);
     return normalize(mk(st * cp, st * sp, ct));
 }
-// microfacet.cpp:125-134
+// microfacet.cpp:125-134.  cos^2(theta) = (alpha^(2-2v) - 1) / (alpha^2 - 1);
+// both it and sin^2(theta) = alpha^2 (alpha^(-2v) - 1) / (1 - alpha^2) are
+// formed with expm1 so the sharp lobe (alpha = 1e-4) keeps its digits.
 TR_HD f3 sample_gtr1(float alpha, float u, float v) {
     float s, c;
     sincosf(2.0f * kPi * u, &s, &c);
+    const float la = logf(alpha);
     const float a2 = alpha * alpha;
-    const float ct = sqrtf((exp2f((2.0f - 2.0f * v) * log2f(alpha)) - 1.0f) / (a2 - 1.0f));
-    const float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+    const float one_m_a2 = 1.0f - a2;
+    const float ct2 = -expm1f((2.0f - 2.0f * v) * la) / one_m_a2;
+    const float st2 = a2 * expm1f(-2.0f * v * la) / one_m_a2;
+    const float ct = sqrtf(fmaxf(0.0f, ct2)), st = sqrtf(fmaxf(0.0f, st2));
     return normalize(mk(st * c, st * s, ct));
 }
 
