@@ -140,7 +140,7 @@ typedef struct tr_render_params {
     float guide_lr;                    /* Adam learning rate (1e-3) */
     int32_t profile;                   /* 1 = time kernels with HIP events; 2 = also count BVH
                                           node visits / triangle tests (slower) */
-    int32_t reserved;
+    int32_t guide_max_depth;           /* guide only vertices of depth < this (0 = every depth) */
 } tr_render_params;
 
 typedef struct tr_render_stats {
@@ -159,7 +159,7 @@ typedef struct tr_render_stats {
     uint64_t extend_launches;
     uint64_t node_visits;         /* profile = 2: BVH nodes visited by extension rays */
     uint64_t tri_tests;           /* profile = 2: triangle tests by extension rays */
-    uint64_t pad;
+    uint64_t guide_vertices;      /* surface vertices sampled with the guide mixture */
 } tr_render_stats;
 
 /* ---- context ---------------------------------------------------------- */
@@ -227,6 +227,60 @@ tr_status tr_bsdf_eval_batch(tr_ctx* ctx, const tr_material* mats, int32_t num_m
 tr_status tr_bsdf_sample_batch(tr_ctx* ctx, const tr_material* mats, int32_t num_mats,
                                const int32_t* mat_ids, const float* wo, const float* ng,
                                const float* u, int64_t n, float* wi, float* f, float* pdf);
+
+/* ---- learned path guiding (no reference counterpart) ---------------------
+ * MLP: 96 input features (81 used: position in the scene box, its sin/cos at
+ * 12 octaves per axis, shading normal, outgoing direction) -> 4 hidden ReLU
+ * layers of `hidden` units -> 48 outputs (40 used: per lobe k of 8, logit,
+ * mean (3, normalised), log concentration clamped to [-4, 8]).  bf16 weights
+ * and activations on MFMA with FP32 accumulation; FP32 master weights and
+ * Adam.  Parameter layout (floats): for each layer l = 0..4 its weights
+ * W_l[in][out] row-major (in = 96 for l = 0, else hidden; out = 48 for l = 4,
+ * else hidden), then the biases of layers 0..4. */
+typedef struct tr_guide_config {
+    int32_t hidden;            /* 64 or 128 */
+    int32_t batch;             /* training samples per Adam step (1 << 18) */
+    int32_t steps_per_pass;    /* Adam steps after each training pass (16) */
+    int32_t record_capacity;   /* path vertices recorded per training pass (1 << 20) */
+    float lr, beta1, beta2, eps;   /* Adam (1e-3, 0.9, 0.99, 1e-8) */
+    float weight_clamp;        /* training weights are clamped at this multiple of their mean (1000) */
+    int32_t reserved0;
+    uint64_t seed;             /* parameter initialisation */
+} tr_guide_config;
+
+/* Fills *cfg with the defaults above. */
+void tr_guide_default_config(tr_guide_config* cfg);
+/* (Re)initialises the network and its Adam state; cfg NULL = defaults.
+ * tr_render with guide = 1 calls this with defaults if it was never called. */
+tr_status tr_guide_init(tr_ctx* ctx, const tr_guide_config* cfg);
+tr_status tr_guide_num_params(tr_ctx* ctx, int64_t* n);
+tr_status tr_guide_get_params(tr_ctx* ctx, float* params, int64_t n);
+/* Replaces the master parameters (and their bf16 copies); Adam state is reset. */
+tr_status tr_guide_set_params(tr_ctx* ctx, const float* params, int64_t n);
+/* Box of the position encoding; by default (and after tr_guide_init) the
+ * bounds of the uploaded scene. */
+tr_status tr_guide_set_box(tr_ctx* ctx, const float lo[3], const float hi[3]);
+/* points: n * 9 floats (x, n, wo); raw: n * 48 head outputs. */
+tr_status tr_guide_forward(tr_ctx* ctx, const float* points, int64_t n, float* raw);
+/* One training batch from host records, n * 16 floats each (x, n, wo, wi,
+ * weight, 3 unused).  flags bit 0: take an Adam step; bit 1: normalise the
+ * weights (w * n / sum(w), clamped at weight_clamp, / n) as training during
+ * rendering does, else use them as given.  loss: sum of w * -log(q(wi) + 1e-3)
+ * over the batch; grad: num_params floats, its gradient.  Either may be NULL. */
+tr_status tr_guide_train_batch(tr_ctx* ctx, const float* records, int64_t n, int32_t flags, float* loss,
+                               float* grad);
+/* The mixture head as the renderer uses it, on given head outputs raw (n*48):
+ * samples (n*3) drawn with uniforms u (n*3), the density at them (pdf_samples)
+ * and at the unit directions dirs (n*3) (pdf_dirs); both -1 when raw is not
+ * finite (the renderer's BSDF-only fallback). */
+tr_status tr_guide_mixture_eval(tr_ctx* ctx, const float* raw, const float* dirs, const float* u, int64_t n,
+                                float* samples, float* pdf_samples, float* pdf_dirs);
+/* Weighted training samples of the most recent training pass of tr_render
+ * (the layout tr_guide_train_batch takes, with the sampling pdf in float 14):
+ * copies min(cap, count) of them to out and the count to *n. */
+tr_status tr_guide_last_samples(tr_ctx* ctx, float* out, int64_t cap, int64_t* n);
+/* MFMA layout check: D (16x16) = bf16(A) (16x32) * bf16(B) (32x16), row-major. */
+tr_status tr_guide_mfma_selftest(tr_ctx* ctx, const float* A, const float* B, float* D);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,302 @@
+"""The guide network on the GPU (bf16 MFMA kernels) against the CPU oracle
+(oracle/guide_ref.py), and guided rendering: unbiasedness against the plain
+estimator, the BSDF-only fallback, and the gain of a trained guide.  The
+guide has no counterpart in the reference renderer."""
+import numpy as np
+import pytest
+
+from oracle import guide_ref as G
+
+pytestmark = pytest.mark.gpu
+
+BOX_LO, BOX_HI = np.array([-2.0, -1.0, -3.0], np.float32), np.array([2.0, 3.0, 1.0], np.float32)
+
+
+def _unit(rng, n):
+    v = rng.normal(size=(n, 3))
+    return (v / np.linalg.norm(v, axis=1, keepdims=True)).astype(np.float32)
+
+
+def _points(rng, n):
+    x = rng.uniform(BOX_LO, BOX_HI, (n, 3)).astype(np.float32)
+    return np.concatenate([x, _unit(rng, n), _unit(rng, n)], axis=1)
+
+
+def _records(rng, n):
+    pts = _points(rng, n)
+    rec = np.zeros((n, 16), np.float32)
+    rec[:, :9] = pts
+    rec[:, 9:12] = _unit(rng, n)
+    rec[:, 12] = rng.uniform(0.1, 2.0, n)
+    return rec
+
+
+def _feats(pts):
+    lo, inv = G.guide_box(BOX_LO, BOX_HI)
+    return G.encode(pts[:, 0:3], pts[:, 3:6], pts[:, 6:9], lo, inv)
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+@pytest.fixture
+def guide(gpu_ctx):
+    def make(H, seed=1):
+        gpu_ctx.guide_init(hidden=H, seed=seed)
+        gpu_ctx.guide_set_box(BOX_LO, BOX_HI)
+        rng = np.random.default_rng(seed)
+        # perturb the initial parameters so no structure of the initialisation hides an error
+        p = gpu_ctx.guide_params().astype(np.float64)
+        Ws, bs = G.unpack_params(p, H)
+        for l in range(5):
+            bs[l] = bs[l] + rng.normal(0, 0.05, bs[l].shape)
+        Ws[4] = Ws[4] * 5.0
+        Ws[0][~G.input_used()] = 0.0
+        p = G.pack_params(Ws, bs).astype(np.float32)
+        gpu_ctx.guide_set_params(p)
+        return p
+    return make
+
+
+def test_mfma_layout(gpu_ctx):
+    rng = np.random.default_rng(0)
+    A = rng.normal(size=(16, 32)).astype(np.float32)
+    B = rng.normal(size=(32, 16)).astype(np.float32)
+    D = gpu_ctx.mfma_selftest(A, B)
+    ref = G.bf16(A) @ G.bf16(B)
+    assert np.abs(D - ref).max() <= 1e-5 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("H", [64, 128])
+def test_forward_matches_oracle(gpu_ctx, guide, H):
+    p = guide(H)
+    assert gpu_ctx.guide_num_params() == G.num_params(H)
+    assert np.array_equal(gpu_ctx.guide_params(), p)
+    rng = np.random.default_rng(10 + H)
+    pts = _points(rng, 4099)   # not a multiple of the tile
+    raw = gpu_ctx.guide_forward(pts)
+    ref, _ = G.forward(p, H, _feats(pts), bf16_emul=True)
+    err = _rel(raw[:, :40], ref[:, :40])
+    print(f"H={H}: forward rel L2 error vs bf16-emulating oracle {err:.2e}, "
+          f"vs float64 oracle {_rel(raw[:, :40], G.forward(p, H, _feats(pts), bf16_emul=False)[0][:, :40]):.2e}")
+    assert err < 5e-3
+    assert np.abs(raw[:, :40] - ref[:, :40]).max() < 0.05 * np.abs(ref[:, :40]).max()
+
+
+@pytest.mark.parametrize("H", [64, 128])
+@pytest.mark.parametrize("normalize", [False, True])
+def test_gradients_match_oracle(gpu_ctx, guide, H, normalize):
+    p = guide(H, seed=2)
+    rng = np.random.default_rng(20 + H)
+    n = 5000
+    rec = _records(rng, n)
+    rec[::7, 12] = 0.0   # zero-weight records carry no gradient
+    loss, grad = gpu_ctx.guide_train_batch(rec, update=False, normalize=normalize)
+    w = rec[:, 12].astype(np.float64)
+    if normalize:
+        w = G.normalized_weights(w, gpu_ctx.guide_config().weight_clamp)
+    L, g = G.backward(p, H, _feats(rec[:, :9]), rec[:, 9:12], w, bf16_emul=True)
+    assert loss == pytest.approx(L, rel=2e-3)
+    Wsz = [fi * fo for fi, fo in G.layer_dims(H)]
+    offs = np.cumsum([0] + Wsz)
+    errs = [_rel(grad[offs[l]:offs[l + 1]], g[offs[l]:offs[l + 1]]) for l in range(5)]
+    berr = _rel(grad[offs[5]:], g[offs[5]:])
+    print(f"H={H} normalize={normalize}: per-layer dW rel err {[f'{e:.1e}' for e in errs]}, db {berr:.1e}")
+    assert max(errs) < 2e-2 and berr < 2e-2
+    assert _rel(grad, g) < 1e-2
+    # padded inputs and outputs get exactly zero gradient
+    Wg, bg = G.unpack_params(grad, H)
+    assert np.all(Wg[0][~G.input_used()] == 0) and np.all(Wg[4][:, 40:] == 0) and np.all(bg[4][40:] == 0)
+
+
+def test_adam_step_matches_oracle(gpu_ctx, guide):
+    H = 64
+    p = guide(H, seed=3)
+    rng = np.random.default_rng(30)
+    rec = _records(rng, 2048)
+    _, g = gpu_ctx.guide_train_batch(rec, update=True)
+    new = gpu_ctx.guide_params()
+    cfg = gpu_ctx.guide_config()
+    ref, _, _ = G.adam(p.astype(np.float64), g.astype(np.float64), 0.0, 0.0, 1, lr=cfg.lr, b1=cfg.beta1, b2=cfg.beta2,
+                       eps=cfg.eps)
+    assert np.abs(new - ref).max() < 1e-6
+    # the bf16 copies follow the master weights: a forward pass sees the new parameters
+    pts = _points(rng, 256)
+    raw = gpu_ctx.guide_forward(pts)
+    exp, _ = G.forward(new, H, _feats(pts), bf16_emul=True)
+    assert _rel(raw[:, :40], exp[:, :40]) < 5e-3
+
+
+def test_device_mixture_matches_oracle(gpu_ctx):
+    rng = np.random.default_rng(40)
+    n = 20000
+    raw = rng.normal(0, 1, (n, 48)).astype(np.float32)
+    raw[:, 4:40:5] = rng.uniform(-4.5, 8.5, (n, 8))
+    raw[:, 40:] = 0
+    dirs = _unit(rng, n)
+    u = rng.random((n, 3)).astype(np.float32)
+    raw[5, 7] = np.nan
+    smp, ps, pd = gpu_ctx.guide_mixture(raw, dirs, u)
+    assert pd[5] == -1 and ps[5] == -1
+    ok = np.ones(n, bool)
+    ok[5] = False
+    ref_pd = G.mixture_pdf(raw[ok], dirs[ok])
+    # FP32 conditioning: exp(kappa (mu.w - 1)) turns the rounding of the dot
+    # product (a few 1e-8) into a relative error of kappa times that
+    _, _, kap = G.mixture(raw[ok])
+    tol = 1e-5 + 1e-6 * kap.max(axis=1)
+    rel_pd = np.abs(pd[ok] - ref_pd) / np.maximum(ref_pd, 1e-30)
+    print(f"pdf rel error: median {np.median(rel_pd):.1e}, max/tolerance {np.max(rel_pd / tol):.2f}")
+    assert np.all((rel_pd <= tol) | (np.abs(pd[ok] - ref_pd) < 1e-7))
+    # sampled directions: the same inverse-CDF map of the same uniforms
+    ref_s = G.mixture_sample(raw[ok].astype(np.float64), u[ok].astype(np.float64))
+    dev = np.linalg.norm(smp[ok] - ref_s, axis=1)
+    print(f"sample direction differences: median {np.median(dev):.1e}, p99.9 {np.quantile(dev, 0.999):.1e}")
+    assert np.quantile(dev, 0.999) < 1e-3
+    # the density reported for a sample is the density the renderer uses for it
+    ref_ps = G.mixture_pdf(raw[ok], smp[ok])
+    rel_ps = np.abs(ps[ok] - ref_ps) / np.maximum(ref_ps, 1e-30)
+    assert np.all((rel_ps <= tol) | (np.abs(ps[ok] - ref_ps) < 1e-7))
+
+
+def test_device_sampler_chi_square(gpu_ctx):
+    from test_oracle_guide import _chi2_sphere
+    rng = np.random.default_rng(41)
+    raw = rng.normal(0, 1, 48).astype(np.float32)
+    raw[4:40:5] = rng.uniform(0.0, 3.0, 8)
+    n = 1 << 19
+    R = np.repeat(raw[None], n, 0)
+    smp, _, _ = gpu_ctx.guide_mixture(R, _unit(rng, n), rng.random((n, 3)).astype(np.float32))
+    p, _, _ = _chi2_sphere(smp.astype(np.float64), lambda x: G.mixture_pdf(np.repeat(raw[None], len(x), 0), x))
+    print(f"chi-square p = {p:.3g}")
+    assert p > 1e-3
+
+
+def test_training_fits_a_known_distribution(gpu_ctx, guide):
+    """Directions drawn from a position-dependent vMF lobe: training with the
+    normalised weights raises the mixture density of held-out samples."""
+    H = 64
+    guide(H, seed=4)
+    rng = np.random.default_rng(50)
+
+    def batch(n):
+        rec = np.zeros((n, 16), np.float32)
+        x = rng.uniform(BOX_LO, BOX_HI, (n, 3)).astype(np.float32)
+        target = np.array([0.0, 2.0, -1.0], np.float32)
+        mu = target - x
+        mu /= np.linalg.norm(mu, axis=1, keepdims=True)
+        raw = np.zeros((n, 48))
+        raw[:, 1:4] = mu
+        raw[:, 4] = np.log(20.0)
+        raw[:, 5:40:5] = -30.0   # other lobes off
+        d = G.mixture_sample(raw, rng.random((n, 3)))
+        rec[:, 0:3] = x
+        rec[:, 3:6] = _unit(rng, n)
+        rec[:, 6:9] = _unit(rng, n)
+        rec[:, 9:12] = d
+        rec[:, 12] = 1.0
+        return rec
+
+    test = batch(8192)
+
+    def mean_log_q():
+        raw = gpu_ctx.guide_forward(test[:, :9])
+        return float(np.mean(np.log(G.mixture_pdf(raw.astype(np.float64), test[:, 9:12].astype(np.float64)))))
+
+    before = mean_log_q()
+    losses = []
+    for _ in range(300):
+        l, _ = gpu_ctx.guide_train_batch(batch(4096), update=True, normalize=True)
+        losses.append(l)
+    after = mean_log_q()
+    # the target lobe has mean log density log(20 / 2pi) - 1 = 0.158 (entropy of a vMF, kappa = 20)
+    print(f"mean log q: {before:.3f} -> {after:.3f}; loss {np.mean(losses[:10]):.3f} -> {np.mean(losses[-10:]):.3f}")
+    assert after > before + 1.5
+    assert np.mean(losses[-10:]) < np.mean(losses[:10])
+
+
+# ---- guided rendering ----------------------------------------------------------
+def _tile_means(sums, tile):
+    img = sums[..., :3] / np.maximum(sums[..., 3:], 1e-12)
+    H, W, _ = img.shape
+    return img.reshape(H // tile, tile, W // tile, tile, 3).mean(axis=(1, 3))
+
+
+def test_guided_render_is_unbiased(gpu_ctx):
+    """Configuration 1 with a guide trained during the render: per-tile means
+    agree with the plain estimator (independent seeds on both sides)."""
+    from tracer_amd import native
+    s = native.config_scene(1)
+    gpu_ctx.load_scene(s)
+    W = H = 64
+    tile = 8
+    g_t, p_t = [], []
+    for seed in range(8):
+        gpu_ctx.guide_init(hidden=64, seed=seed + 1, batch=1 << 14)
+        pg = s.render_params(width=W, height=H, spp=256, seed=100 + seed, guide=1, guide_train_passes=4,
+                             samples_per_pass=16)
+        sums, _, st = gpu_ctx.render(pg, want_image=False)
+        assert st.guide_vertices > 0 and st.guide_train_samples > 0 and st.nonfinite_dropped == 0
+        g_t.append(_tile_means(sums, tile))
+        sums, _, _ = gpu_ctx.render(s.render_params(width=W, height=H, spp=256, seed=200 + seed), want_image=False)
+        p_t.append(_tile_means(sums, tile))
+    g, p = np.stack(g_t), np.stack(p_t)
+    mg, mp = g.mean(0), p.mean(0)
+    se = np.sqrt(g.var(0, ddof=1) / len(g) + p.var(0, ddof=1) / len(p))
+    z = (mg - mp) / np.maximum(se, 1e-6)
+    rel = abs(mg.mean() - mp.mean()) / mp.mean()
+    print(f"guided vs plain: max |z| {np.abs(z).max():.2f}, image-mean rel diff {rel:.4f}")
+    assert np.abs(z).max() < 5.0
+    assert rel < 0.01
+
+
+def test_nan_guide_falls_back_to_plain(gpu_ctx):
+    """With non-finite network parameters every surface vertex falls back to
+    BSDF sampling, drawing the plain estimator's random numbers: the image is
+    the plain image bit for bit."""
+    from tracer_amd import native
+    s = native.config_scene(1)
+    gpu_ctx.load_scene(s)
+    gpu_ctx.guide_init(hidden=64)
+    p = gpu_ctx.guide_params()
+    p[G.num_params(64) - 48 + 3] = np.nan   # one output bias
+    gpu_ctx.guide_set_params(p)
+    pg = s.render_params(width=32, height=32, spp=16, seed=5, guide=1, guide_train_passes=0)
+    a, _, st = gpu_ctx.render(pg, want_image=False)
+    b, _, sp = gpu_ctx.render(s.render_params(width=32, height=32, spp=16, seed=5), want_image=False)
+    assert st.guide_vertices == 0 and st.guide_fallbacks > 0
+    assert np.array_equal(a, b)
+    assert st.extension_rays == sp.extension_rays
+
+
+def test_trained_guide_lowers_error_on_occluded_light(gpu_ctx):
+    """Configuration 4 (the only light shines through a small aperture) at a
+    reduced film: after training, the guided estimator's relMSE against a
+    high-sample plain reference is below the plain estimator's at the same
+    sample count.  Both estimators have rare fireflies (paths that reach the
+    emitter close to it), so the comparison uses the median over seeds."""
+    from tracer_amd import native
+    s = native.config_scene(4, detail=1 / 16)
+    gpu_ctx.load_scene(s)
+    W = H = 64
+    ref, _, _ = gpu_ctx.render(s.render_params(width=W, height=H, spp=32768, seed=1), want_image=False)
+    R = ref[..., :3] / ref[..., 3:]
+
+    def relmse(sums):
+        I = sums[..., :3] / sums[..., 3:]
+        return float(np.mean((I - R) ** 2 / (R ** 2 + 1e-2)))
+
+    gpu_ctx.guide_init(hidden=64, seed=7, batch=1 << 16, steps_per_pass=32)
+    # train during a render, then measure fresh guided renders with the trained network
+    _, _, st = gpu_ctx.render(s.render_params(width=W, height=H, spp=1024, seed=2, guide=1, guide_train_passes=64,
+                                              samples_per_pass=16), want_image=False)
+    assert st.guide_train_samples > 0
+    spp, seeds = 64, range(100, 108)
+    e_plain = [relmse(gpu_ctx.render(s.render_params(width=W, height=H, spp=spp, seed=k), want_image=False)[0])
+               for k in seeds]
+    e_guided = [relmse(gpu_ctx.render(s.render_params(width=W, height=H, spp=spp, seed=k, guide=1,
+                                                      guide_train_passes=0), want_image=False)[0]) for k in seeds]
+    mp, mg = float(np.median(e_plain)), float(np.median(e_guided))
+    print(f"median relMSE at {spp} spp over {len(seeds)} seeds: plain {mp:.4g}, trained guide {mg:.4g}, gain {mp / mg:.2f}")
+    assert mg < 0.8 * mp

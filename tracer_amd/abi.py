@@ -77,7 +77,7 @@ class TrRenderParams(C.Structure):
         ("sample_offset", C.c_int32), ("samples_per_pass", C.c_int32),
         ("guide", C.c_int32), ("guide_train_passes", C.c_int32),
         ("guide_fraction", C.c_float), ("guide_lr", C.c_float),
-        ("profile", C.c_int32), ("reserved", C.c_int32),
+        ("profile", C.c_int32), ("guide_max_depth", C.c_int32),
     ]
 
 
@@ -88,11 +88,21 @@ class TrRenderStats(C.Structure):
         ("ms_total", C.c_double), ("ms_extend", C.c_double), ("ms_shade", C.c_double), ("ms_shadow", C.c_double),
         ("ms_other", C.c_double), ("ms_guide", C.c_double),
         ("extend_launches", C.c_uint64), ("node_visits", C.c_uint64), ("tri_tests", C.c_uint64),
-        ("pad", C.c_uint64),
+        ("guide_vertices", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class TrGuideConfig(C.Structure):
+    """tr_guide_config (include/tracer_hip.h)."""
+    _fields_ = [
+        ("hidden", C.c_int32), ("batch", C.c_int32), ("steps_per_pass", C.c_int32), ("record_capacity", C.c_int32),
+        ("lr", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float),
+        ("weight_clamp", C.c_float), ("reserved0", C.c_int32),
+        ("seed", C.c_uint64),
+    ]
 
 
 def fptr(a: np.ndarray | None):

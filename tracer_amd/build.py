@@ -57,8 +57,6 @@ def build_hip(jobs: int = 8) -> Path:
     LIB.mkdir(parents=True, exist_ok=True)
     cmds, objs = [], []
     for src, extra in HIP_SOURCES.items():
-        if not (CSRC / src).exists():
-            continue
         obj = OBJ / (src + ".o")
         objs.append(obj)
         cmds.append([HIPCC, *COMMON, *HIP_FLAGS, *extra, "-c", str(CSRC / src), "-o", str(obj)])

@@ -13,120 +13,12 @@
 #include <vector>
 
 #include "bvh_build.h"
-#include "tr_kernels.h"
+#include "capi_internal.h"
 
 using namespace trhip;
+using namespace trcapi;
 
 namespace {
-
-struct HipError : std::runtime_error {
-    using std::runtime_error::runtime_error;
-};
-struct InvalidArg : std::runtime_error {
-    using std::runtime_error::runtime_error;
-};
-struct StateError : std::runtime_error {
-    using std::runtime_error::runtime_error;
-};
-
-inline void hip_check(hipError_t e, const char* what) {
-    if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
-}
-
-// Device allocation that grows on demand and never shrinks.
-struct DevBuf {
-    void* ptr = nullptr;
-    size_t bytes = 0;
-    void reserve(size_t n) {
-        if (n <= bytes) return;
-        release();
-        if (n == 0) return;
-        hipError_t e = hipMalloc(&ptr, n);
-        if (e != hipSuccess) {
-            ptr = nullptr;
-            bytes = 0;
-            (void)hipGetLastError();
-            throw std::bad_alloc();
-        }
-        bytes = n;
-    }
-    void release() {
-        if (ptr) (void)hipFree(ptr);
-        ptr = nullptr;
-        bytes = 0;
-    }
-    template <class T> T* as() const { return static_cast<T*>(ptr); }
-    ~DevBuf() { release(); }
-};
-
-struct HostScene {
-    std::vector<float> P, N;
-    std::vector<int32_t> idx, mat, med_in, med_out;
-    std::vector<float> emission;
-    std::vector<tr_material> mats;
-    std::vector<tr_medium> media;
-    int has_env = 0, has_sun = 0;
-    float env[3] = {0, 0, 0}, sun_dir[3] = {0, 0, 1}, sun_E[3] = {0, 0, 0};
-    tr_camera cam{};
-    float lo[3], hi[3];
-};
-
-}  // namespace
-
-struct tr_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::string err;
-    HostScene hs;
-    bool have_scene = false, have_bvh = false;
-    // scene on device
-    DevBuf nodes, tris, tri_shade, mats, media, lights, light_cdf, light_pmf;
-    int64_t num_nodes = 0;
-    int max_stack = 0;
-    int bvh_depth = 0;
-    DevScene ds{};
-    // traversal spill area
-    DevBuf ovf_node, ovf_t;
-    int ovf_threads = 0;
-    // wavefront buffers
-    DevBuf ray_o, ray_d, hit, thr, rad, rng, info, q0, q1, sh_o, sh_d, sh_c, counters, stats, film;
-    int64_t path_capacity = 0;
-    // scratch for batch calls
-    DevBuf s_in, s_out, s_aux, s_aux2, s_aux3, s_aux4;
-    std::vector<hipEvent_t> events;
-};
-
-namespace {
-
-tr_status fail(tr_ctx* ctx, tr_status code, const std::string& msg) {
-    if (ctx) ctx->err = msg;
-    return code;
-}
-
-template <class F> tr_status guarded(tr_ctx* ctx, F&& f) {
-    if (!ctx) return TR_ERR_INVALID;
-    try {
-        hip_check(hipSetDevice(ctx->device), "hipSetDevice");
-        f();
-        ctx->err.clear();
-        return TR_OK;
-    } catch (const InvalidArg& e) {
-        return fail(ctx, TR_ERR_INVALID, e.what());
-    } catch (const StateError& e) {
-        return fail(ctx, TR_ERR_STATE, e.what());
-    } catch (const HipError& e) {
-        return fail(ctx, TR_ERR_HIP, e.what());
-    } catch (const std::bad_alloc&) {
-        return fail(ctx, TR_ERR_OOM, "device or host allocation failed");
-    } catch (const std::exception& e) {
-        return fail(ctx, TR_ERR_INVALID, e.what());
-    }
-}
-
-void upload(DevBuf& b, const void* src, size_t bytes, hipStream_t st) {
-    b.reserve(std::max<size_t>(bytes, 16));
-    if (bytes) hip_check(hipMemcpyAsync(b.ptr, src, bytes, hipMemcpyHostToDevice, st), "hipMemcpyAsync H2D");
-}
 
 // DisneyBSDF constructor (src/material/disney.cpp:347-399) evaluated once per material.
 DevMaterial make_material(const tr_material& m) {
@@ -664,7 +556,10 @@ tr_status tr_render(tr_ctx* c, const tr_render_params* prm, double* film_sums, i
         if (P.direct_light_sample_num > 1) throw InvalidArg("direct_light_sample_num > 1 is not supported (use 1)");
         if (P.filter_type != TR_FILTER_BOX && P.filter_type != TR_FILTER_GAUSSIAN) throw InvalidArg("unknown filter");
         if (!(P.filter_radius > 0.0f && P.filter_radius <= 1.0f)) throw InvalidArg("filter radius must lie in (0, 1]");
-        if (P.guide != 0) throw InvalidArg("guided rendering is driven through tr_render_guided");
+        if (P.guide != 0 && P.guide != 1) throw InvalidArg("guide must be 0 or 1");
+        if (P.guide && !(P.guide_fraction >= 0.0f && P.guide_fraction < 1.0f))
+            throw InvalidArg("guide_fraction must lie in [0, 1) (0 = the default 0.5)");
+        if (P.guide && P.guide_train_passes < 0) throw InvalidArg("guide_train_passes must be >= 0");
         const int64_t npix = int64_t(P.width) * P.height;
         if (npix > (int64_t(1) << 28)) throw InvalidArg("image too large");
         int spp_pp = P.samples_per_pass > 0 ? P.samples_per_pass : int(std::max<int64_t>(1, (int64_t(1) << 21) / npix));
@@ -679,7 +574,11 @@ tr_status tr_render(tr_ctx* c, const tr_render_params* prm, double* film_sums, i
         else
             hip_check(hipMemsetAsync(c->film.ptr, 0, size_t(npix) * 32, st), "film clear");
         hip_check(hipMemsetAsync(c->stats.ptr, 0, 64, st), "stats clear");
+        const bool guided = P.guide != 0;
+        const bool any_training = guided && P.guide_train_passes > 0;
+        if (guided) guide_prepare(c, maxP, any_training);
         DevPaths p = make_paths(c);
+        if (guided) guide_fill_paths(c, p, any_training);
         RenderConsts rc{};
         rc.width = P.width; rc.height = P.height;
         rc.min_depth = P.min_depth; rc.max_depth = P.max_depth;
@@ -687,9 +586,13 @@ tr_status tr_render(tr_ctx* c, const tr_render_params* prm, double* film_sums, i
         rc.shadow_tr = P.shadow_transmittance;
         rc.nls = 1;
         rc.seed = P.seed;
+        rc.guide_frac = guided ? (P.guide_fraction > 0.0f ? P.guide_fraction : 0.5f) : 0.0f;
+        rc.guide_max_depth = P.guide_max_depth;
+        const float guide_lr = P.guide_lr > 0.0f ? P.guide_lr : c->guide.cfg.lr;
+        uint64_t trained = 0;
         size_t evi = 0;
         const bool prof = P.profile != 0;
-        double ms_ext = 0, ms_sh = 0, ms_shd = 0, ms_oth = 0;
+        double ms_ext = 0, ms_sh = 0, ms_shd = 0, ms_oth = 0, ms_gd = 0;
         std::vector<std::pair<size_t, int>> marks;   // (event index, kind)
         hipEvent_t e0 = ev(c, evi++), e1 = ev(c, evi++);
         hip_check(hipEventRecord(e0, st), "event");
@@ -702,22 +605,33 @@ tr_status tr_render(tr_ctx* c, const tr_render_params* prm, double* film_sums, i
             marks.push_back({a, kind});
         };
         uint64_t launches = 0;
-        for (int done = 0; done < P.spp; done += spp_pp) {
+        int pass = 0;
+        for (int done = 0; done < P.spp; done += spp_pp, ++pass) {
             const int k = std::min(spp_pp, P.spp - done);
             rc.spp_in_pass = k;
             rc.sample_base = P.sample_offset + done;
             const int num_paths = int(npix * k);
+            const bool training = guided && pass < P.guide_train_passes;
+            rc.guide_record = training ? 1 : 0;
+            if (training) {
+                rc.rec_threshold = guide_record_threshold(c, num_paths);
+                rc.rec_seed = uint32_t(mix64(P.seed ^ (0x7261696eull + uint64_t(rc.sample_base))));
+                guide_begin_pass(c);
+            }
             int cur = 0;
             timed(3, [&] { launch_generate(c->ds, p, rc, num_paths, cur, st); });
             for (int b = 0; b <= P.max_depth; ++b) {
                 timed(0, [&] { launch_extend(c->ds.tr, p, cur, num_paths, P.profile == 2, st); });
-                timed(1, [&] { launch_shade(c->ds, p, rc, cur, num_paths, st); });
-                timed(2, [&] { launch_shadow(c->ds.tr, p, num_paths, 0, st); });
+                if (guided) timed(4, [&] { guide_infer(c, p, cur, num_paths); });
+                timed(1, [&] { launch_shade(c->ds, p, rc, cur, num_paths, guided, st); });
+                timed(2, [&] { launch_shadow(c->ds.tr, p, num_paths, 0, rc.guide_record, st); });
                 ++launches;
                 cur = 1 - cur;
             }
             timed(3, [&] { launch_accumulate(p, rc, c->film.as<double4>(), st); });
             hip_check(hipGetLastError(), "render launch");
+            if (training)
+                timed(5, [&] { trained += uint64_t(guide_train_pass(c, p, P.seed + uint64_t(pass), guide_lr)); });
         }
         hip_check(hipEventRecord(e1, st), "event");
         if (film_sums)
@@ -745,15 +659,20 @@ tr_status tr_render(tr_ctx* c, const tr_render_params* prm, double* film_sums, i
                 if (m.second == 0) ms_ext += t;
                 else if (m.second == 1) ms_sh += t;
                 else if (m.second == 2) ms_shd += t;
+                else if (m.second >= 4) ms_gd += t;
                 else ms_oth += t;
             }
             stats->ms_extend = ms_ext;
             stats->ms_shade = ms_sh;
             stats->ms_shadow = ms_shd;
             stats->ms_other = ms_oth;
+            stats->ms_guide = ms_gd;
             stats->extend_launches = launches;
             stats->node_visits = hs[3];
             stats->tri_tests = hs[4];
+            stats->guide_fallbacks = hs[5];
+            stats->guide_vertices = hs[6];
+            stats->guide_train_samples = trained;
         }
     });
 }

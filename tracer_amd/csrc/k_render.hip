@@ -19,6 +19,7 @@
 //     src/core/intersection.hpp:36-41);
 //   * the reference defects Q1, Q4, Q5, Q6 and Q23 (direct emission added
 //     before the medium sample) are not reproduced.
+#include "tr_guide.h"
 #include "tr_kernels.h"
 
 namespace trhip {
@@ -35,20 +36,6 @@ TR_D Pcg load_rng(const DevPaths& p, int slot, uint32_t pixel) {
 }
 TR_D void store_rng(const DevPaths& p, int slot, const Pcg& r) {
     p.rng[slot] = make_uint2(uint32_t(r.state), uint32_t(r.state >> 32));
-}
-
-// Wave-aggregated append: one atomic per wavefront, lanes get consecutive
-// slots in lane order.
-TR_D unsigned int wave_append(unsigned int* counter, bool pred) {
-    const unsigned long long mask = __ballot(pred);
-    if (mask == 0ull) return 0u;
-    const int lane = __lane_id();
-    const int leader = __ffsll((long long)mask) - 1;
-    unsigned int base = 0u;
-    if (lane == leader) base = atomicAdd(counter, (unsigned int)__popcll(mask));
-    base = __shfl(base, leader);
-    const unsigned long long below = lane == 0 ? 0ull : (mask & ((~0ull) >> (64 - lane)));
-    return base + (unsigned int)__popcll(below);
 }
 
 __global__ __launch_bounds__(256) void k_generate(DevScene s, DevPaths p, RenderConsts c, int num_paths, int cur) {
@@ -76,6 +63,7 @@ __global__ __launch_bounds__(256) void k_generate(DevScene s, DevPaths p, Render
     p.info[i] = make_uint2(pixel, pack_info(0, 0));
     store_rng(p, i, rng);
     p.queue[cur][i] = i;
+    if (p.grec) p.grec[i] = -1;
 }
 
 // Transmittance of a homogeneous medium over a shadow segment: closed form,
@@ -103,6 +91,36 @@ struct ShadowOut {
     f3 contrib;
 };
 
+// Training-record selection: a path records its surface vertices when this
+// hash of its slot falls below the pass threshold (murmur3 finaliser).
+TR_D unsigned int record_hash(unsigned int slot, unsigned int seed) {
+    unsigned int h = slot * 0x9E3779B1u ^ seed;
+    h ^= h >> 16; h *= 0x85EBCA6Bu;
+    h ^= h >> 13; h *= 0xC2B2AE35u;
+    h ^= h >> 16;
+    return h;
+}
+
+// Guide mixture of a path slot from the inference kernel's head outputs.
+TR_D bool load_mixture(const DevPaths& p, int slot, Mixture& mix) {
+    float raw[5 * kGuideLobes];
+    const float4* g = reinterpret_cast<const float4*>(p.graw + size_t(slot) * kGuideOut);
+#pragma unroll
+    for (int k = 0; k < 10; ++k) {
+        const float4 v = g[k];
+        raw[4 * k] = v.x; raw[4 * k + 1] = v.y; raw[4 * k + 2] = v.z; raw[4 * k + 3] = v.w;
+    }
+    if (!mixture_raw_valid(raw)) return false;
+    mix = mixture_from_raw(raw);
+    return true;
+}
+
+// GUIDE: surface vertices sample the continuation direction by one-sample MIS
+// between the guide mixture (probability c.guide_frac) and the BSDF, and
+// weight it, light samples and emitter hits with the combined pdf.  A vertex
+// whose head outputs are not finite uses the plain BSDF estimator, drawing
+// the same random numbers as the plain kernel.
+template <bool GUIDE>
 __global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene s, DevPaths p, RenderConsts c, int cur) {
     const unsigned int n = p.counters[cur];
     const int* q = p.queue[cur];
@@ -118,6 +136,10 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene s, DevPaths p, R
         sh.emit = false;
         int slot = 0;
         float4 next_o = make_float4(0, 0, 0, 0), next_d = make_float4(0, 0, 0, 0);
+        bool g_used = false, g_fallback = false, want_rec = false, want_nee_rec = false;
+        float4 rec0{}, rec1{}, rec2{}, rec3{}, aux0{};
+        float4 nrec0{}, nrec1{}, nrec2{}, nrec3{};
+        float nee_weight = 0.0f;
         if (active) {
             slot = q[i];
             const float4 ro = p.ray_o[slot], rd = p.ray_d[slot], h = p.hit[slot];
@@ -235,12 +257,43 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene s, DevPaths p, R
                         const DevMaterial mat = s.mats[sr.material];
                         const f3 wo = -d;
                         const f3 lwo = to_local(wo), lng = to_local(ng);
+                        Mixture mix;
+                        bool use_guide = false;
+                        if (GUIDE && c.guide_frac > 0.0f && (c.guide_max_depth <= 0 || depth < c.guide_max_depth)) {
+                            use_guide = load_mixture(p, slot, mix);
+                            g_fallback = !use_guide;
+                            g_used = use_guide;
+                        }
+                        const float frac = c.guide_frac;
+                        const bool rec_path =
+                            GUIDE && c.guide_record && record_hash(uint32_t(slot), c.rec_seed) < c.rec_threshold;
                         const float su = rng.uniform(), u1 = rng.uniform(), u2 = rng.uniform();
-                        const BsdfSample bs = bsdf_sample(mat, lwo, lng, su, u1, u2);
-                        if (is_black(bs.f) || bs.pdf < kEps) {
-                            alive = false;
+                        f3 wi, fs;
+                        float pdf;
+                        if (use_guide) {
+                            float pdf_b;
+                            if (rng.uniform() < frac) {
+                                wi = mixture_sample(mix, su, u1, u2);
+                                const BsdfEval be = bsdf_eval(mat, to_local(wi), lwo, lng);
+                                fs = be.f;
+                                pdf_b = be.pdf;
+                            } else {
+                                const BsdfSample bs = bsdf_sample(mat, lwo, lng, su, u1, u2);
+                                wi = normalize(tx * bs.wi.x + ty * bs.wi.y + ns * bs.wi.z);
+                                fs = bs.f;
+                                pdf_b = bs.pdf;
+                            }
+                            pdf = is_black(fs) ? 0.0f : frac * mixture_pdf(mix, wi) + (1.0f - frac) * pdf_b;
                         } else {
-                            const f3 wi = normalize(tx * bs.wi.x + ty * bs.wi.y + ns * bs.wi.z);
+                            const BsdfSample bs = bsdf_sample(mat, lwo, lng, su, u1, u2);
+                            wi = normalize(tx * bs.wi.x + ty * bs.wi.y + ns * bs.wi.z);
+                            fs = bs.f;
+                            pdf = bs.pdf;
+                        }
+                        // Next-event estimation does not depend on the continuation sample: the
+                        // reference estimates direct light before it samples the continuation
+                        // and ends the path only afterwards (path_trace_renderer.cpp:141-163).
+                        {
                             // next-event estimation (direct_illumination.cpp:35-109)
                             if (s.num_light_entries > 0) {
                                 float pmf;
@@ -254,7 +307,13 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene s, DevPaths p, R
                                         if (!is_black(be.f)) {
                                             float w = 1.0f;
                                             float lp = pmf;
-                                            if (ls.pdf > 0.0f) { lp *= ls.pdf; w = power_heuristic(lp, be.pdf); }
+                                            if (ls.pdf > 0.0f) {
+                                                lp *= ls.pdf;
+                                                const float sp = use_guide ? frac * mixture_pdf(mix, ls.wi) +
+                                                                                 (1.0f - frac) * be.pdf
+                                                                           : be.pdf;
+                                                w = power_heuristic(lp, sp);
+                                            }
                                             f3 trs = f3s(1.0f);
                                             if (ls.area) {
                                                 const int lm = dot(ls.wi, ng) >= 0.0f ? sr.med_out : sr.med_in;
@@ -269,18 +328,45 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene s, DevPaths p, R
                                                 sh.d = ls.wi;
                                                 sh.tmax = dist;
                                                 sh.contrib = contrib;
+                                                if (rec_path) {
+                                                    // light-sample record: MIS-weighted f |cos| L / p_light,
+                                                    // kept if the shadow ray reaches the light
+                                                    nee_weight = lum(ls.L * be.f * trs) * (cosg * w / lp);
+                                                    want_nee_rec = nee_weight > 0.0f && isfinite(nee_weight);
+                                                    nrec0 = make_float4(pos.x, pos.y, pos.z, ns.x);
+                                                    nrec1 = make_float4(ns.y, ns.z, wo.x, wo.y);
+                                                    nrec2 = make_float4(wo.z, ls.wi.x, ls.wi.y, ls.wi.z);
+                                                    nrec3 = make_float4(0.0f, __int_as_float(slot), lp, 0.0f);
+                                                }
                                             }
                                         }
                                     }
                                 }
                             }
+                        }
+                        if (is_black(fs) || !(pdf >= kEps)) {
+                            alive = false;
+                        } else {
                             // continuation (path_trace_renderer.cpp:155-163)
-                            T *= bs.f * (fabsf(dot(ng, wi)) / bs.pdf);
+                            const f3 Tv = T;
+                            T *= fs * (fabsf(dot(ng, wi)) / pdf);
                             const f3 org = dot(ng, wi) > 0.0f ? pos + ng * kEps : pos - ng * kEps;
                             next_o = f4(org, 0.0f);
                             next_d = f4(wi, 3.0e38f);
-                            tp = f4(T, bs.pdf);
+                            tp = f4(T, pdf);
                             cont = finite3(T);
+                            if (rec_path && cont) {
+                                // record (x, n, wo, wi); its weight lum((L_end - L_next) * rr / Tv)
+                                // estimates f |cos| L_i / pdf, with emission MIS-weighted, once the
+                                // path has ended
+                                want_rec = true;
+                                rec0 = make_float4(pos.x, pos.y, pos.z, ns.x);
+                                rec1 = make_float4(ns.y, ns.z, wo.x, wo.y);
+                                rec2 = make_float4(wo.z, wi.x, wi.y, wi.z);
+                                rec3 = make_float4(0.0f, __int_as_float(slot), pdf, 0.0f);
+                                aux0 = make_float4(Tv.x > 0.0f ? 1.0f / Tv.x : 0.0f, Tv.y > 0.0f ? 1.0f / Tv.y : 0.0f,
+                                                   Tv.z > 0.0f ? 1.0f / Tv.z : 0.0f, 0.0f);
+                            }
                         }
                     }
                 }
@@ -289,8 +375,17 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene s, DevPaths p, R
                 // Russian roulette before the next segment (path_trace_renderer.cpp:54-58)
                 if (depth + 1 > c.min_depth) {
                     if (rng.uniform() > 0.9f) cont = false;
-                    else { tp.x /= 0.9f; tp.y /= 0.9f; tp.z /= 0.9f; }
+                    else {
+                        tp.x /= 0.9f; tp.y /= 0.9f; tp.z /= 0.9f;
+                        aux0.x *= 0.9f; aux0.y *= 0.9f; aux0.z *= 0.9f;
+                    }
                 }
+            }
+            want_rec = want_rec && cont;
+            if (GUIDE && c.guide_record) {
+                // radiance gathered before this vertex closes the previous vertex's record
+                const int pr = p.grec[slot];
+                if (pr >= 0) p.rec_aux[2 * pr + 1] = make_float4(rp.x, rp.y, rp.z, 1.0f);
             }
             p.rad[slot] = f4(L, rp.w);
             if (cont) {
@@ -301,6 +396,39 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene s, DevPaths p, R
                 store_rng(p, slot, rng);
             }
         }
+        if (GUIDE) {
+            const unsigned long long mu = __ballot(g_used), mf = __ballot(g_fallback);
+            if (__lane_id() == 0) {
+                if (mu) atomicAdd(&p.stats[6], (unsigned long long)__popcll(mu));
+                if (mf) atomicAdd(&p.stats[5], (unsigned long long)__popcll(mf));
+            }
+            if (c.guide_record) {
+                const unsigned int ri = wave_append(p.rec_count, want_rec);
+                const bool stored = want_rec && ri < (unsigned int)p.rec_cap;
+                if (stored) {
+                    p.rec[4 * ri] = rec0;
+                    p.rec[4 * ri + 1] = rec1;
+                    p.rec[4 * ri + 2] = rec2;
+                    p.rec[4 * ri + 3] = rec3;
+                    p.rec_aux[2 * ri] = aux0;
+                    p.rec_aux[2 * ri + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                }
+                if (cont) p.grec[slot] = stored ? int(ri) : -1;
+            }
+        }
+        int nee_ri = -1;
+        if (GUIDE && c.guide_record) {
+            const unsigned int ri = wave_append(p.rec_count, want_nee_rec);
+            if (want_nee_rec && ri < (unsigned int)p.rec_cap) {
+                nee_ri = int(ri);
+                p.rec[4 * ri] = nrec0;
+                p.rec[4 * ri + 1] = nrec1;
+                p.rec[4 * ri + 2] = nrec2;
+                p.rec[4 * ri + 3] = nrec3;
+                p.rec_aux[2 * ri] = make_float4(nee_weight, 0.0f, 0.0f, 0.0f);
+                p.rec_aux[2 * ri + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            }
+        }
         const unsigned int qi = wave_append(&p.counters[1 - cur], cont);
         if (cont) qn[qi] = slot;
         const unsigned int si = wave_append(&p.counters[2], sh.emit);
@@ -308,6 +436,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene s, DevPaths p, R
             p.sh_o[si] = f4(sh.o, sh.tmin);
             p.sh_d[si] = f4(sh.d, sh.tmax);
             p.sh_c[si] = f4(sh.contrib, __int_as_float(slot));
+            if (GUIDE && c.guide_record) p.sh_rec[si] = nee_ri;
         }
     }
 }
@@ -408,8 +537,12 @@ void launch_generate(const DevScene& s, const DevPaths& p, const RenderConsts& c
     const int blocks = (num_paths + 255) / 256;
     hipLaunchKernelGGL(k_generate, dim3(blocks > 0 ? blocks : 1), dim3(256), 0, st, s, p, c, num_paths, cur);
 }
-void launch_shade(const DevScene& s, const DevPaths& p, const RenderConsts& c, int cur, int max_paths, hipStream_t st) {
-    hipLaunchKernelGGL(k_shade, dim3(shade_blocks(max_paths)), dim3(kShadeBlock), 0, st, s, p, c, cur);
+void launch_shade(const DevScene& s, const DevPaths& p, const RenderConsts& c, int cur, int max_paths, bool guide,
+                  hipStream_t st) {
+    if (guide)
+        hipLaunchKernelGGL(k_shade<true>, dim3(shade_blocks(max_paths)), dim3(kShadeBlock), 0, st, s, p, c, cur);
+    else
+        hipLaunchKernelGGL(k_shade<false>, dim3(shade_blocks(max_paths)), dim3(kShadeBlock), 0, st, s, p, c, cur);
 }
 void launch_accumulate(const DevPaths& p, const RenderConsts& c, double4* film, hipStream_t st) {
     const int n = c.width * c.height;

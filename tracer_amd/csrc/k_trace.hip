@@ -98,7 +98,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_extend(TraceScene sc, DevPaths 
 
 // Wavefront shadow rays: any hit; unoccluded contributions are added to the
 // radiance of their path.
-__global__ __launch_bounds__(kTraceBlock) void k_shadow(TraceScene sc, DevPaths p, int atomic_add) {
+__global__ __launch_bounds__(kTraceBlock) void k_shadow(TraceScene sc, DevPaths p, int atomic_add, int record) {
     __shared__ TraceStack<kTraceBlock, kLdsStack> st;
     const int tid = threadIdx.x;
     const int gt = blockIdx.x * kTraceBlock + tid;
@@ -111,6 +111,10 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(TraceScene sc, DevPaths 
         const bool occ = trace<true, false, kTraceBlock, kLdsStack>(sc, xyz(a), a.w, xyz(b), b.w, st, tid,
                                                                     p.ovf_node + gt, p.ovf_t + gt, stride, h);
         if (!occ) {
+            if (record) {
+                const int ri = p.sh_rec[i];
+                if (ri >= 0) p.rec_aux[2 * ri + 1] = make_float4(0.0f, 0.0f, 0.0f, 2.0f);
+            }
             const float4 c = p.sh_c[i];
             const int slot = __float_as_int(c.w);
             if (atomic_add) {
@@ -152,8 +156,10 @@ void launch_extend(const TraceScene& sc, const DevPaths& p, int cur, int max_pat
     else
         hipLaunchKernelGGL(k_extend<false>, dim3(trace_blocks(max_paths)), dim3(kTraceBlock), 0, st, sc, p, cur);
 }
-void launch_shadow(const TraceScene& sc, const DevPaths& p, int max_shadow, int accumulate_atomic, hipStream_t st) {
-    hipLaunchKernelGGL(k_shadow, dim3(trace_blocks(max_shadow)), dim3(kTraceBlock), 0, st, sc, p, accumulate_atomic);
+void launch_shadow(const TraceScene& sc, const DevPaths& p, int max_shadow, int accumulate_atomic, int record,
+                   hipStream_t st) {
+    hipLaunchKernelGGL(k_shadow, dim3(trace_blocks(max_shadow)), dim3(kTraceBlock), 0, st, sc, p, accumulate_atomic,
+                       record);
 }
 
 }  // namespace trhip

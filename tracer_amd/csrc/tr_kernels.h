@@ -10,6 +10,20 @@ constexpr int kTraceBlock = 256;
 constexpr int kLdsStack = 16;
 constexpr int kShadeBlock = 256;
 
+// Wave-aggregated append: one atomic per wavefront, lanes get consecutive
+// slots in lane order.
+TR_D unsigned int wave_append(unsigned int* counter, bool pred) {
+    const unsigned long long mask = __ballot(pred);
+    if (mask == 0ull) return 0u;
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)mask) - 1;
+    unsigned int base = 0u;
+    if (lane == leader) base = atomicAdd(counter, (unsigned int)__popcll(mask));
+    base = __shfl(base, leader);
+    const unsigned long long below = lane == 0 ? 0ull : (mask & ((~0ull) >> (64 - lane)));
+    return base + (unsigned int)__popcll(below);
+}
+
 // Path state of the wavefront (one slot per camera sample in flight).
 struct DevPaths {
     float4* ray_o;    // (origin, tmin)
@@ -29,6 +43,17 @@ struct DevPaths {
     int* ovf_node;    // traversal stack spill area
     float* ovf_t;
     int ovf_threads;  // threads the spill area is sized for
+    // path guiding (null without the guide); stats[5] vertices that fell back
+    // to BSDF sampling, stats[6] vertices sampled with the guide mixture
+    const float* graw;        // [slot][48] head outputs of the guide MLP
+    int* grec;                // training record waiting for the radiance of its continuation, -1 = none
+    float4* rec;              // training records [cap][4]: (x, n.x), (n.yz, wo.xy), (wo.z, wi), (-, slot, pdf, -)
+    float4* rec_aux;          // [cap][2]: continuation records (rr / throughput before the vertex, -),
+                              //   (radiance before the next vertex, 1); light-sample records
+                              //   (weight, -, -, -), (-, -, -, 2 once the shadow ray found the light)
+    unsigned int* rec_count;
+    int rec_cap;
+    int* sh_rec;              // light-sample record of each shadow ray, -1 = none
 };
 
 struct RenderConsts {
@@ -41,6 +66,11 @@ struct RenderConsts {
     int shadow_tr;       // TR_TR_*
     int nls;             // light samples per vertex
     unsigned long long seed;
+    float guide_frac;    // probability of sampling the guide mixture at a surface vertex
+    int guide_max_depth; // guide only vertices of depth < this (0 = every depth)
+    int guide_record;    // collect training records in this pass
+    unsigned int rec_threshold;   // a path records when hash(slot, rec_seed) < rec_threshold
+    unsigned int rec_seed;
 };
 
 // k_trace.hip
@@ -52,13 +82,16 @@ void launch_trace_counted(const TraceScene& sc, const float4* rays, float4* hits
                           int64_t n, int* ovf_node, float* ovf_t, int ovf_threads, hipStream_t st);
 // wavefront: extend the current queue, trace the shadow queue
 void launch_extend(const TraceScene& sc, const DevPaths& p, int cur, int max_paths, bool count, hipStream_t st);
-void launch_shadow(const TraceScene& sc, const DevPaths& p, int max_shadow, int accumulate_atomic, hipStream_t st);
+// record != 0: validate the light-sample training records of unoccluded shadow rays
+void launch_shadow(const TraceScene& sc, const DevPaths& p, int max_shadow, int accumulate_atomic, int record,
+                   hipStream_t st);
 // Grid size the spill area must cover for launches of up to n rays.
 int trace_grid_threads(int64_t n);
 
 // k_render.hip
 void launch_generate(const DevScene& s, const DevPaths& p, const RenderConsts& c, int num_paths, int cur, hipStream_t st);
-void launch_shade(const DevScene& s, const DevPaths& p, const RenderConsts& c, int cur, int max_paths, hipStream_t st);
+void launch_shade(const DevScene& s, const DevPaths& p, const RenderConsts& c, int cur, int max_paths, bool guide,
+                  hipStream_t st);
 void launch_accumulate(const DevPaths& p, const RenderConsts& c, double4* film, hipStream_t st);
 void launch_resolve(const double4* film, float* image, int num_pixels, hipStream_t st);
 void launch_bsdf_eval(const DevMaterial* mats, const int* ids, const float* wi, const float* wo, const float* ng, int64_t n,

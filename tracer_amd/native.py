@@ -60,6 +60,19 @@ def hip_lib() -> C.CDLL:
                                            P(C.c_float), I64, P(C.c_float), P(C.c_float)]
         lib.tr_bsdf_sample_batch.argtypes = [VP, P(TrMaterial), I32, P(C.c_int32), P(C.c_float), P(C.c_float),
                                              P(C.c_float), I64, P(C.c_float), P(C.c_float), P(C.c_float)]
+        F = P(C.c_float)
+        lib.tr_guide_default_config.argtypes = [P(abi.TrGuideConfig)]
+        lib.tr_guide_default_config.restype = None
+        lib.tr_guide_init.argtypes = [VP, P(abi.TrGuideConfig)]
+        lib.tr_guide_num_params.argtypes = [VP, P(I64)]
+        lib.tr_guide_get_params.argtypes = [VP, F, I64]
+        lib.tr_guide_set_params.argtypes = [VP, F, I64]
+        lib.tr_guide_set_box.argtypes = [VP, F, F]
+        lib.tr_guide_forward.argtypes = [VP, F, I64, F]
+        lib.tr_guide_train_batch.argtypes = [VP, F, I64, I32, F, F]
+        lib.tr_guide_mixture_eval.argtypes = [VP, F, F, F, I64, F, F, F]
+        lib.tr_guide_last_samples.argtypes = [VP, F, I64, P(I64)]
+        lib.tr_guide_mfma_selftest.argtypes = [VP, F, F, F]
         _hip = lib
     return _hip
 
@@ -301,6 +314,84 @@ class Context:
         self._check(self.lib.tr_bsdf_sample_batch(self.h, arr, len(mats), iptr(ids), fptr(wo), fptr(ngc), fptr(u), n,
                                                   fptr(wi), fptr(f), fptr(pdf)))
         return wi, f, pdf
+
+
+    # ---- learned path guiding --------------------------------------------
+    def guide_config(self, **kw) -> abi.TrGuideConfig:
+        cfg = abi.TrGuideConfig()
+        self.lib.tr_guide_default_config(C.byref(cfg))
+        for k, v in kw.items():
+            setattr(cfg, k, v)
+        return cfg
+
+    def guide_init(self, **kw):
+        cfg = self.guide_config(**kw)
+        self._check(self.lib.tr_guide_init(self.h, C.byref(cfg)))
+
+    def guide_num_params(self) -> int:
+        n = C.c_int64()
+        self._check(self.lib.tr_guide_num_params(self.h, C.byref(n)))
+        return n.value
+
+    def guide_params(self) -> np.ndarray:
+        p = np.zeros(self.guide_num_params(), np.float32)
+        self._check(self.lib.tr_guide_get_params(self.h, fptr(p), p.size))
+        return p
+
+    def guide_set_params(self, p: np.ndarray):
+        p = np.ascontiguousarray(p, np.float32)
+        self._check(self.lib.tr_guide_set_params(self.h, fptr(p), p.size))
+
+    def guide_set_box(self, lo, hi):
+        lo = np.ascontiguousarray(lo, np.float32)
+        hi = np.ascontiguousarray(hi, np.float32)
+        self._check(self.lib.tr_guide_set_box(self.h, fptr(lo), fptr(hi)))
+
+    def guide_forward(self, points: np.ndarray) -> np.ndarray:
+        """points: (n, 9) = (x, n, wo) -> (n, 48) head outputs."""
+        pts = np.ascontiguousarray(points, np.float32)
+        out = np.zeros((pts.shape[0], 48), np.float32)
+        self._check(self.lib.tr_guide_forward(self.h, fptr(pts), pts.shape[0], fptr(out)))
+        return out
+
+    def guide_train_batch(self, records: np.ndarray, update: bool = False, normalize: bool = False):
+        """records: (n, 16) = (x, n, wo, wi, weight, 3 unused) -> (loss, gradient)."""
+        rec = np.ascontiguousarray(records, np.float32)
+        grad = np.zeros(self.guide_num_params(), np.float32)
+        loss = C.c_float()
+        flags = (1 if update else 0) | (2 if normalize else 0)
+        self._check(self.lib.tr_guide_train_batch(self.h, fptr(rec), rec.shape[0], flags, C.byref(loss), fptr(grad)))
+        return loss.value, grad
+
+    def guide_mixture(self, raw: np.ndarray, dirs: np.ndarray, u: np.ndarray):
+        """Device mixture head: (samples, pdf at samples, pdf at dirs)."""
+        raw = np.ascontiguousarray(raw, np.float32)
+        dirs = np.ascontiguousarray(dirs, np.float32)
+        u = np.ascontiguousarray(u, np.float32)
+        n = raw.shape[0]
+        smp = np.zeros((n, 3), np.float32)
+        ps = np.zeros(n, np.float32)
+        pd = np.zeros(n, np.float32)
+        self._check(self.lib.tr_guide_mixture_eval(self.h, fptr(raw), fptr(dirs), fptr(u), n, fptr(smp), fptr(ps),
+                                                   fptr(pd)))
+        return smp, ps, pd
+
+    def guide_last_samples(self, cap: int = 1 << 20) -> np.ndarray:
+        """Weighted training samples of the latest training pass, (n, 16)."""
+        n = C.c_int64()
+        self._check(self.lib.tr_guide_last_samples(self.h, fptr(None), 0, C.byref(n)))
+        k = min(cap, n.value)
+        out = np.zeros((k, 16), np.float32)
+        if k:
+            self._check(self.lib.tr_guide_last_samples(self.h, fptr(out), k, C.byref(n)))
+        return out
+
+    def mfma_selftest(self, A: np.ndarray, B: np.ndarray) -> np.ndarray:
+        A = np.ascontiguousarray(A, np.float32)
+        B = np.ascontiguousarray(B, np.float32)
+        D = np.zeros((16, 16), np.float32)
+        self._check(self.lib.tr_guide_mfma_selftest(self.h, fptr(A), fptr(B), fptr(D)))
+        return D
 
 
 def gpu_available() -> bool:
