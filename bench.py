@@ -1,24 +1,30 @@
 """Benchmark of the wavefront path tracer on one MI355X.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--config 2] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-relmse]
 
-One step renders the configuration's full frame (configuration 2: 64
-displaced level-5 icospheres, 1,310,754 triangles, Disney materials, 16 quad
-lights + constant environment, 1024x1024, 64 spp, max depth 16) with the
-scene and BVH already resident in HBM.  The timed region is K steps between
-device synchronisations; value = camera samples per second over it.
+Throughput (the reported value).  One step renders configuration 2's full
+frame (64 displaced level-5 icospheres, 1,310,754 triangles, Disney
+materials, 16 quad lights + constant environment, 1024x1024, 64 spp, max depth
+16) with the scene and BVH resident in HBM; value = camera samples per second
+over K steps between device synchronisations.
 
-Also reported:
-  roofline   k_extend (closest-hit traversal, the dominant kernel): algorithmic
-             bytes per extension ray = 32 (ray) + 16 (hit) + 128 * nodes
-             visited + 48 * triangles tested, with visit counts from an
-             instrumented replay of the same frame (profile = 2); divided by
-             the k_extend time measured with HIP events on the library's
-             stream inside the timed region.  traffic (HBM bytes per launch)
-             comes from the committed rocprofv3 --pmc summary when present.
-  cpu_baseline  the CPU restatement of the reference renderer (all lights per
-             vertex, binary SAH BVH, std::thread over 16x16 tiles) on the
-             same scene at a reduced film, timed on this host.
+Also reported on the same JSON line:
+  roofline       k_extend (closest-hit traversal, the dominant kernel):
+                 algorithmic bytes per extension ray = 32 (ray) + 16 (hit) +
+                 128 * nodes visited + 48 * triangles tested, visit counts from
+                 an instrumented replay of the same frame; divided by the
+                 k_extend time measured with HIP events on the library's
+                 stream inside the timed region.  traffic: HBM bytes per launch
+                 from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary.
+  equal_time     configuration 4 (the only light shines through a 0.5 x 0.5
+                 aperture) at a reduced film: relMSE against a 65536-spp plain
+                 reference of the guided renderer (training included) and of
+                 the plain renderer given the same device time.
+  guide_network  GFLOP/s of guide inference and training against the bf16
+                 MFMA peak.
+  cpu_baseline   the CPU restatement of the reference renderer (all lights
+                 per vertex, binary SAH BVH, std::thread over 16x16 tiles) on
+                 configuration 2 at a reduced film, timed on this host.
 """
 from __future__ import annotations
 
@@ -29,11 +35,14 @@ import sys
 import time
 from pathlib import Path
 
+import numpy as np
+
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 METRIC = "equal-time relMSE vs 64k-spp ref (plain vs guided); Mrays/s; samples/s"
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
 NODE_BYTES, TRI_BYTES, RAY_BYTES, HIT_BYTES = 128, 48, 32, 16
 
 
@@ -47,7 +56,7 @@ def cpu_model() -> str:
     return "unknown"
 
 
-def cpu_baseline(scene, params, threads: int, target_s: float = 15.0) -> dict:
+def cpu_baseline(scene, threads: int, target_s: float = 15.0) -> dict:
     """The oracle's reference-mode renderer on a reduced film of the same scene."""
     from oracle import refcpu
     t0 = time.time()
@@ -66,7 +75,7 @@ def cpu_baseline(scene, params, threads: int, target_s: float = 15.0) -> dict:
         "cores": threads,
         "kind": "port",
         "sample": (f"reference-mode CPU restatement (all-lights NEE + MIS BSDF ray, binary SAH BVH leaf 3, "
-                   f"16x16 tiles) on the same scene, {W}x{H} film at {spp} spp ({samples} samples, "
+                   f"16x16 tiles) on configuration 2, {W}x{H} film at {spp} spp ({samples} samples, "
                    f"{ms / 1e3:.1f} s, {(rays[0] + rays[1]) / (ms / 1e3) / 1e6:.1f} Mrays/s); BVH build "
                    f"{build_s:.1f} s not timed; CPU: {cpu_model()}"),
     }
@@ -74,10 +83,118 @@ def cpu_baseline(scene, params, threads: int, target_s: float = 15.0) -> dict:
 
 def read_traffic(path: Path):
     try:
-        d = json.loads(path.read_text())
-        return d.get("bytes_per_launch")
+        return json.loads(path.read_text()).get("bytes_per_launch")
     except (OSError, ValueError):
         return None
+
+
+def guide_flops(H: int) -> tuple[float, float]:
+    """Algorithmic flops of one network evaluation and of one training sample
+    (81 used inputs, 40 used outputs): forward; forward + input gradients of
+    layers 4..1 + weight gradients."""
+    fwd = 2.0 * (81 * H + 3 * H * H + H * 40)
+    bwd_in = 2.0 * (3 * H * H + H * 40)
+    return fwd, fwd + bwd_in + fwd
+
+
+def throughput(ctx, scene, steps: int, warmup: int) -> dict:
+    base = scene.render_params(seed=0)
+    W, H, spp = base.width, base.height, base.spp
+    for i in range(warmup):
+        ctx.render(scene.render_params(seed=1000 + i), want_image=False)
+    tot = dict(ext=0, shadow=0, ms_extend=0.0, ms_shade=0.0, ms_shadow=0.0, ms_other=0.0, ms_dev=0.0, launches=0)
+    t0 = time.perf_counter()
+    for k in range(steps):
+        _, _, st = ctx.render(scene.render_params(seed=k, profile=1), want_image=False)
+        tot["ext"] += st.extension_rays
+        tot["shadow"] += st.shadow_rays
+        tot["ms_extend"] += st.ms_extend
+        tot["ms_shade"] += st.ms_shade
+        tot["ms_shadow"] += st.ms_shadow
+        tot["ms_other"] += st.ms_other
+        tot["ms_dev"] += st.ms_total
+        tot["launches"] += st.extend_launches
+    elapsed = time.perf_counter() - t0
+    return {"W": W, "H": H, "spp": spp, "max_depth": base.max_depth, "elapsed": elapsed, **tot}
+
+
+def equal_time(res: int, ref_spp: int, spp: int, seeds: int, hidden: int) -> dict:
+    """Configuration 4 at res x res: guided (training included) vs plain at the
+    same device time, both against a ref_spp plain reference."""
+    from tracer_amd import native
+    scene = native.config_scene(4)
+    ctx = native.Context(0)
+    ctx.load_scene(scene)
+    t0 = time.perf_counter()
+    chunks = 8   # progressive chunks of the reference, accumulated on one FP64 film
+    ref = None
+    for c in range(chunks):
+        p = scene.render_params(width=res, height=res, spp=ref_spp // chunks, seed=1, sample_offset=c * (ref_spp // chunks))
+        ref, _, _ = ctx.render(p, film=ref, want_image=False)
+        print(f"[bench] reference chunk {c + 1}/{chunks} ({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+    ref_s = time.perf_counter() - t0
+    R = ref[..., :3] / ref[..., 3:]
+
+    def relmse(sums):
+        img = sums[..., :3] / sums[..., 3:]
+        return float(np.mean((img - R) ** 2 / (R ** 2 + 1e-2)))
+
+    train_spp, train_pp = spp // 2, 8   # training phase: passes of 8 spp, Adam steps after each
+    g_err, g_ms, g_train_ms, g_inf_ms, g_inf, g_smp = [], [], [], [], 0, 0
+    for k in range(seeds):
+        ctx.guide_init(hidden=hidden, seed=11 + k, batch=1 << 16, steps_per_pass=32)
+        s1, _, a = ctx.render(scene.render_params(width=res, height=res, spp=train_spp, seed=200 + k, guide=1,
+                                                  guide_train_passes=train_spp // train_pp, samples_per_pass=train_pp,
+                                                  profile=1), want_image=False)
+        s2, _, b = ctx.render(scene.render_params(width=res, height=res, spp=spp - train_spp, seed=200 + k,
+                                                  sample_offset=train_spp, guide=1, guide_train_passes=0, profile=1),
+                              film=s1, want_image=False)
+        g_err.append(relmse(s2))
+        g_ms.append(a.ms_total + b.ms_total)
+        g_train_ms.append(a.ms_guide_train)
+        g_inf_ms.append(a.ms_guide + b.ms_guide)
+        g_inf += a.guide_inferences + b.guide_inferences
+        g_smp += a.guide_train_samples
+    p_ms = []
+    for k in range(seeds):
+        _, _, st = ctx.render(scene.render_params(width=res, height=res, spp=spp, seed=300 + k), want_image=False)
+        p_ms.append(st.ms_total)
+    spp_eq = max(1, int(round(spp * np.mean(g_ms) / np.mean(p_ms))))
+    p_err, p_err_same = [], []
+    for k in range(seeds):
+        sums, _, _ = ctx.render(scene.render_params(width=res, height=res, spp=spp_eq, seed=400 + k), want_image=False)
+        p_err.append(relmse(sums))
+        sums, _, _ = ctx.render(scene.render_params(width=res, height=res, spp=spp, seed=500 + k), want_image=False)
+        p_err_same.append(relmse(sums))
+    ctx.close()
+    fwd, trn = guide_flops(hidden)
+    inf_tflops = g_inf * fwd / (np.sum(g_inf_ms) / 1e3) / 1e12 if np.sum(g_inf_ms) > 0 else 0.0
+    trn_tflops = g_smp * trn / (np.sum(g_train_ms) / 1e3) / 1e12 if np.sum(g_train_ms) > 0 else 0.0
+    return {
+        "equal_time": {
+            "workload": (f"configuration 4 ({scene.num_triangles} triangles, light through a 0.5x0.5 aperture) at "
+                         f"{res}x{res}, max depth 16; reference: plain GPU path, {ref_spp} spp, FP64 film "
+                         f"({ref_s:.1f} s); {seeds} seeds each"),
+            "guided": {"spp": spp, "ms": float(np.mean(g_ms)), "relmse": float(np.mean(g_err)),
+                       "relmse_median": float(np.median(g_err)),
+                       "training": f"{train_spp} spp in passes of {train_pp} spp sampled without the guide "
+                                   f"(recording only), 32 Adam steps of 65536 samples after each (lr 1e-3), then "
+                                   f"{spp - train_spp} spp guided (one-sample MIS 0.5 guide / 0.5 BSDF)",
+                       "ms_training": float(np.mean(g_train_ms)), "ms_inference": float(np.mean(g_inf_ms))},
+            "plain_equal_time": {"spp": spp_eq, "ms": float(np.mean(p_ms)) * spp_eq / spp,
+                                 "relmse": float(np.mean(p_err)), "relmse_median": float(np.median(p_err))},
+            "plain_equal_spp": {"spp": spp, "ms": float(np.mean(p_ms)), "relmse": float(np.mean(p_err_same))},
+            "ratio_plain_over_guided": float(np.mean(p_err) / np.mean(g_err)),
+            "ratio_median": float(np.median(p_err) / np.median(g_err)),
+        },
+        "guide_network": {
+            "hidden": hidden, "flops_per_inference": fwd, "flops_per_training_sample": trn,
+            "inference_tflops": inf_tflops, "training_tflops": trn_tflops, "peak_tflops_bf16": BF16_PEAK_TFLOPS,
+            "inference_frac_of_peak": inf_tflops / BF16_PEAK_TFLOPS,
+            "training_frac_of_peak": trn_tflops / BF16_PEAK_TFLOPS,
+            "inferences": int(g_inf), "training_samples": int(g_smp),
+        },
+    }
 
 
 def main() -> int:
@@ -88,6 +205,11 @@ def main() -> int:
     ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-relmse", action="store_true")
+    ap.add_argument("--relmse-res", type=int, default=256)
+    ap.add_argument("--ref-spp", type=int, default=65536)
+    ap.add_argument("--relmse-spp", type=int, default=256)
+    ap.add_argument("--relmse-seeds", type=int, default=6)
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one 1 GiB device-to-device copy (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -102,8 +224,8 @@ def main() -> int:
         print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4}))
         return 0
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (rendering shards by sample index; "
-              "see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: this build has no multi-GPU path (rendering shards by sample index "
+              "across GPUs, as DESIGN.md describes). Run with --gpus 1.", file=sys.stderr)
         return 2
 
     from tracer_amd import native
@@ -114,41 +236,25 @@ def main() -> int:
     ctx.load_scene(scene)
     bvh_s = time.time() - t0
     info = ctx.bvh_info()
-    base = scene.render_params(seed=0)
-    W, H, spp = base.width, base.height, base.spp
-
-    for i in range(args.warmup):
-        ctx.render(scene.render_params(seed=1000 + i), want_image=False)
-
-    # timed region: K full frames, profile = 1 (HIP events around every launch)
-    totals = dict(ext=0, shadow=0, ms_extend=0.0, ms_shade=0.0, ms_shadow=0.0, ms_other=0.0, ms_dev=0.0, launches=0)
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        _, _, st = ctx.render(scene.render_params(seed=k, profile=1), want_image=False)
-        totals["ext"] += st.extension_rays
-        totals["shadow"] += st.shadow_rays
-        totals["ms_extend"] += st.ms_extend
-        totals["ms_shade"] += st.ms_shade
-        totals["ms_shadow"] += st.ms_shadow
-        totals["ms_other"] += st.ms_other
-        totals["ms_dev"] += st.ms_total
-        totals["launches"] += st.extend_launches
-    elapsed = time.perf_counter() - t0
-
-    samples = args.steps * W * H * spp
-    value = samples / elapsed
-    mrays = (totals["ext"] + totals["shadow"]) / elapsed / 1e6
+    print(f"[bench] configuration {args.config} loaded ({scene.num_triangles} triangles, BVH {bvh_s:.2f} s)",
+          file=sys.stderr, flush=True)
+    tp = throughput(ctx, scene, args.steps, args.warmup)
+    samples = args.steps * tp["W"] * tp["H"] * tp["spp"]
+    value = samples / tp["elapsed"]
+    mrays = (tp["ext"] + tp["shadow"]) / tp["elapsed"] / 1e6
+    print(f"[bench] throughput {value / 1e6:.1f} M samples/s, {mrays:.0f} Mrays/s", file=sys.stderr, flush=True)
 
     # instrumented replay of one frame: node visits and triangle tests per extension ray
     _, _, cst = ctx.render(scene.render_params(seed=0, profile=2), want_image=False)
     nodes_per_ray = cst.node_visits / max(1, cst.extension_rays)
     tris_per_ray = cst.tri_tests / max(1, cst.extension_rays)
     bytes_per_ray = RAY_BYTES + HIT_BYTES + NODE_BYTES * nodes_per_ray + TRI_BYTES * tris_per_ray
-    launches = max(1, totals["launches"])
-    bytes_per_launch = bytes_per_ray * totals["ext"] / launches
-    ms_per_launch = totals["ms_extend"] / launches
+    launches = max(1, tp["launches"])
+    bytes_per_launch = bytes_per_ray * tp["ext"] / launches
+    ms_per_launch = tp["ms_extend"] / launches
     achieved = bytes_per_launch / (ms_per_launch / 1e3) / 1e9 if ms_per_launch > 0 else 0.0
     traffic = read_traffic(ROOT / "profiles" / f"k_extend_c{args.config}_traffic.json")
+    ctx.close()
 
     out = {
         "metric": METRIC,
@@ -157,22 +263,24 @@ def main() -> int:
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed * 1e3 / args.steps,
+        "ms_per_step": tp["elapsed"] * 1e3 / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": (f"configuration {args.config}: {scene.num_triangles} triangles, {W}x{H} film, {spp} spp, "
-                         f"max depth {base.max_depth}, plain wavefront path tracer (one light by power + MIS)"),
-            "film": f"{W}x{H}", "spp": spp, "max_depth": base.max_depth, "triangles": scene.num_triangles,
+            "workload": (f"configuration {args.config}: {scene.num_triangles} triangles, {tp['W']}x{tp['H']} film, "
+                         f"{tp['spp']} spp, max depth {tp['max_depth']}, plain wavefront path tracer (one light by "
+                         f"power + MIS)"),
+            "film": f"{tp['W']}x{tp['H']}", "spp": tp["spp"], "max_depth": tp["max_depth"],
+            "triangles": scene.num_triangles,
         },
         "mrays_per_s": mrays,
         "samples_per_s": value,
-        "rays": {"extension": totals["ext"], "shadow": totals["shadow"]},
-        "kernel_ms": {k: round(totals[k], 3) for k in ("ms_extend", "ms_shade", "ms_shadow", "ms_other")},
-        "device_ms_per_step": totals["ms_dev"] / args.steps,
+        "rays": {"extension": tp["ext"], "shadow": tp["shadow"]},
+        "kernel_ms": {k: round(tp[k], 3) for k in ("ms_extend", "ms_shade", "ms_shadow", "ms_other")},
+        "device_ms_per_step": tp["ms_dev"] / args.steps,
         "bvh": {"build_s": round(bvh_s, 3), **info, "nodes_per_ray": nodes_per_ray, "tris_per_ray": tris_per_ray},
         "roofline": {
             "kernel": "k_extend",
@@ -187,10 +295,16 @@ def main() -> int:
             "ms_per_launch": ms_per_launch,
         },
     }
+    if not args.no_relmse:
+        print("[bench] equal-time relMSE on configuration 4 ...", file=sys.stderr, flush=True)
+        out.update(equal_time(args.relmse_res, args.ref_spp, args.relmse_spp, args.relmse_seeds, 64))
+        print(f"[bench] equal-time ratio {out['equal_time']['ratio_plain_over_guided']:.2f}", file=sys.stderr,
+              flush=True)
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline(scene, base, threads)
-    ctx.close()
+        print(f"[bench] CPU baseline on {threads} threads ...", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(scene, threads)
+        out["gpu_over_cpu_samples_per_s"] = value / out["cpu_baseline"]["value"]
     print(json.dumps(out))
     return 0
 

@@ -291,6 +291,8 @@ struct GpuRenderOptions {
     int guide_train_passes = 8;
     float guide_fraction = 0.5f;
     float guide_lr = 1e-3f;
+    float guide_train_fraction = 0.0f;      // guide probability during training passes
+    int guide_max_depth = 0;                // 0 = guide every depth
     tr_render_stats* stats_out = nullptr;   // optional
 };
 RC<Renderer> create_pt_renderer(const PTRendererParams& params, const GpuRenderOptions& gpu);

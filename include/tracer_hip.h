@@ -141,6 +141,10 @@ typedef struct tr_render_params {
     int32_t profile;                   /* 1 = time kernels with HIP events; 2 = also count BVH
                                           node visits / triangle tests (slower) */
     int32_t guide_max_depth;           /* guide only vertices of depth < this (0 = every depth) */
+    float guide_train_fraction;        /* MIS probability of the guide during training passes (0: they
+                                          sample the BSDF alone and only record, so their image samples
+                                          have the plain estimator's variance) */
+    int32_t reserved;
 } tr_render_params;
 
 typedef struct tr_render_stats {
@@ -155,11 +159,13 @@ typedef struct tr_render_stats {
     double ms_shade;
     double ms_shadow;
     double ms_other;
-    double ms_guide;              /* guide inference + training (profile = 1) */
+    double ms_guide;              /* guide inference (profile = 1) */
     uint64_t extend_launches;
     uint64_t node_visits;         /* profile = 2: BVH nodes visited by extension rays */
     uint64_t tri_tests;           /* profile = 2: triangle tests by extension rays */
     uint64_t guide_vertices;      /* surface vertices sampled with the guide mixture */
+    uint64_t guide_inferences;    /* network evaluations (surface hits of guided passes) */
+    double ms_guide_train;        /* guide training between passes (profile = 1) */
 } tr_render_stats;
 
 /* ---- context ---------------------------------------------------------- */
@@ -232,7 +238,7 @@ tr_status tr_bsdf_sample_batch(tr_ctx* ctx, const tr_material* mats, int32_t num
  * MLP: 96 input features (81 used: position in the scene box, its sin/cos at
  * 12 octaves per axis, shading normal, outgoing direction) -> 4 hidden ReLU
  * layers of `hidden` units -> 48 outputs (40 used: per lobe k of 8, logit,
- * mean (3, normalised), log concentration clamped to [-4, 8]).  bf16 weights
+ * mean (3, normalised), log concentration clamped to [-4, 10]).  bf16 weights
  * and activations on MFMA with FP32 accumulation; FP32 master weights and
  * Adam.  Parameter layout (floats): for each layer l = 0..4 its weights
  * W_l[in][out] row-major (in = 96 for l = 0, else hidden; out = 48 for l = 4,

@@ -22,7 +22,7 @@ from __future__ import annotations
 import numpy as np
 
 N_IN, N_IN_USED, N_OUT, N_LOBES, N_LAYERS = 96, 81, 48, 8, 5
-LOG_KAPPA_MIN, LOG_KAPPA_MAX = -4.0, 8.0
+LOG_KAPPA_MIN, LOG_KAPPA_MAX = -4.0, 10.0
 Q_EPS = 1e-3
 
 

@@ -132,7 +132,7 @@ def test_device_mixture_matches_oracle(gpu_ctx):
     rng = np.random.default_rng(40)
     n = 20000
     raw = rng.normal(0, 1, (n, 48)).astype(np.float32)
-    raw[:, 4:40:5] = rng.uniform(-4.5, 8.5, (n, 8))
+    raw[:, 4:40:5] = rng.uniform(-4.5, 10.5, (n, 8))
     raw[:, 40:] = 0
     dirs = _unit(rng, n)
     u = rng.random((n, 3)).astype(np.float32)

@@ -60,7 +60,7 @@ def test_box_clamps_outside_points():
     assert F[g3[0]] == 0.0 and F[g3[1]] == 1.0 and F[g3[2]] == pytest.approx(0.5, abs=1e-5)
 
 
-@pytest.mark.parametrize("kappa", [np.exp(-4.0), 0.5, 1.0, 10.0, 300.0, np.exp(8.0)])
+@pytest.mark.parametrize("kappa", [np.exp(-4.0), 0.5, 1.0, 10.0, 300.0, np.exp(8.0), np.exp(10.0)])
 def test_vmf_normalisation(kappa):
     # the density depends on t = mu . w only: integral = 2 pi int_{-1}^{1} C e^{kappa (t - 1)} dt
     # with t = 1 - u / kappa (u in [0, 2 kappa]) the integrand is C e^{-u} / kappa at any kappa;
@@ -158,7 +158,7 @@ def test_loss_gradient_finite_differences():
 def test_loss_gradient_zero_outside_kappa_clamp():
     rng = np.random.default_rng(5)
     raw = _rand_raw(rng, 2)
-    raw[0, 4] = 9.0    # above the clamp
+    raw[0, 4] = 11.0   # above the clamp
     raw[1, 9] = -5.0   # below
     g = G.loss_grad(raw, _unit(rng, 2), np.ones(2))
     assert g[0, 4] == 0 and g[1, 9] == 0

@@ -18,6 +18,14 @@
 using namespace trhip;
 using namespace trcapi;
 
+// The ctypes bindings (tracer_amd/abi.py) assume these layouts.
+static_assert(sizeof(tr_material) == 72, "tr_material layout");
+static_assert(sizeof(tr_medium) == 32, "tr_medium layout");
+static_assert(sizeof(tr_camera) == 52, "tr_camera layout");
+static_assert(sizeof(tr_render_params) == 88, "tr_render_params layout");
+static_assert(sizeof(tr_render_stats) == 144, "tr_render_stats layout");
+static_assert(sizeof(tr_guide_config) == 48, "tr_guide_config layout");
+
 namespace {
 
 // DisneyBSDF constructor (src/material/disney.cpp:347-399) evaluated once per material.
@@ -54,6 +62,10 @@ DevMaterial make_material(const tr_material& m) {
         d.metallic = 0.0f;
         d.transmission = 0.0f;
     }
+    // Guided sampling only where the BSDF lobe is broad: the guide network does
+    // not see material parameters, so at specular-transmissive or glossy metal
+    // vertices its samples would mostly land where f vanishes.
+    d.guide_ok = d.type == MAT_LAMBERT || (m.transmission <= 0.0f && (m.metallic < 0.5f || d.ax >= 0.25f));
     return d;
 }
 
@@ -586,13 +598,15 @@ tr_status tr_render(tr_ctx* c, const tr_render_params* prm, double* film_sums, i
         rc.shadow_tr = P.shadow_transmittance;
         rc.nls = 1;
         rc.seed = P.seed;
-        rc.guide_frac = guided ? (P.guide_fraction > 0.0f ? P.guide_fraction : 0.5f) : 0.0f;
+        const float guide_frac = guided ? (P.guide_fraction > 0.0f ? P.guide_fraction : 0.5f) : 0.0f;
+        if (guided && !(P.guide_train_fraction >= 0.0f && P.guide_train_fraction < 1.0f))
+            throw InvalidArg("guide_train_fraction must lie in [0, 1)");
         rc.guide_max_depth = P.guide_max_depth;
         const float guide_lr = P.guide_lr > 0.0f ? P.guide_lr : c->guide.cfg.lr;
         uint64_t trained = 0;
         size_t evi = 0;
         const bool prof = P.profile != 0;
-        double ms_ext = 0, ms_sh = 0, ms_shd = 0, ms_oth = 0, ms_gd = 0;
+        double ms_ext = 0, ms_sh = 0, ms_shd = 0, ms_oth = 0, ms_gd = 0, ms_gt = 0;
         std::vector<std::pair<size_t, int>> marks;   // (event index, kind)
         hipEvent_t e0 = ev(c, evi++), e1 = ev(c, evi++);
         hip_check(hipEventRecord(e0, st), "event");
@@ -613,6 +627,7 @@ tr_status tr_render(tr_ctx* c, const tr_render_params* prm, double* film_sums, i
             const int num_paths = int(npix * k);
             const bool training = guided && pass < P.guide_train_passes;
             rc.guide_record = training ? 1 : 0;
+            rc.guide_frac = training ? P.guide_train_fraction : guide_frac;
             if (training) {
                 rc.rec_threshold = guide_record_threshold(c, num_paths);
                 rc.rec_seed = uint32_t(mix64(P.seed ^ (0x7261696eull + uint64_t(rc.sample_base))));
@@ -622,7 +637,7 @@ tr_status tr_render(tr_ctx* c, const tr_render_params* prm, double* film_sums, i
             timed(3, [&] { launch_generate(c->ds, p, rc, num_paths, cur, st); });
             for (int b = 0; b <= P.max_depth; ++b) {
                 timed(0, [&] { launch_extend(c->ds.tr, p, cur, num_paths, P.profile == 2, st); });
-                if (guided) timed(4, [&] { guide_infer(c, p, cur, num_paths); });
+                if (guided && rc.guide_frac > 0.0f) timed(4, [&] { guide_infer(c, p, cur, num_paths); });
                 timed(1, [&] { launch_shade(c->ds, p, rc, cur, num_paths, guided, st); });
                 timed(2, [&] { launch_shadow(c->ds.tr, p, num_paths, 0, rc.guide_record, st); });
                 ++launches;
@@ -659,7 +674,8 @@ tr_status tr_render(tr_ctx* c, const tr_render_params* prm, double* film_sums, i
                 if (m.second == 0) ms_ext += t;
                 else if (m.second == 1) ms_sh += t;
                 else if (m.second == 2) ms_shd += t;
-                else if (m.second >= 4) ms_gd += t;
+                else if (m.second == 4) ms_gd += t;
+                else if (m.second == 5) ms_gt += t;
                 else ms_oth += t;
             }
             stats->ms_extend = ms_ext;
@@ -673,6 +689,8 @@ tr_status tr_render(tr_ctx* c, const tr_render_params* prm, double* film_sums, i
             stats->guide_fallbacks = hs[5];
             stats->guide_vertices = hs[6];
             stats->guide_train_samples = trained;
+            stats->guide_inferences = hs[7];
+            stats->ms_guide_train = ms_gt;
         }
     });
 }

@@ -215,6 +215,13 @@ __global__ __launch_bounds__(256) void k_guide_infer(GuideNet net, DevScene s, D
         bf16x8 hh[4][H / 32][NT];
         f32x4 out[3][NT];
         forward_all<H, NT>(net.box, src, x, nn, wo, xin, hh, out, lane);
+        {
+            // evaluations of this wave's tile, counted once: lanes 0..15 hold its sample columns
+            int tot = 0;
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) tot += __popcll(__ballot(lane < 16 && slot[nt] >= 0));
+            if (lane == 0 && tot > 0) atomicAdd(&p.stats[7], (unsigned long long)tot);
+        }
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) {
             if (slot[nt] < 0) continue;

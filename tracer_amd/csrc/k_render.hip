@@ -259,14 +259,15 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene s, DevPaths p, R
                         const f3 lwo = to_local(wo), lng = to_local(ng);
                         Mixture mix;
                         bool use_guide = false;
-                        if (GUIDE && c.guide_frac > 0.0f && (c.guide_max_depth <= 0 || depth < c.guide_max_depth)) {
+                        const bool guide_here = GUIDE && mat.guide_ok;
+                        if (guide_here && c.guide_frac > 0.0f && (c.guide_max_depth <= 0 || depth < c.guide_max_depth)) {
                             use_guide = load_mixture(p, slot, mix);
                             g_fallback = !use_guide;
                             g_used = use_guide;
                         }
                         const float frac = c.guide_frac;
-                        const bool rec_path =
-                            GUIDE && c.guide_record && record_hash(uint32_t(slot), c.rec_seed) < c.rec_threshold;
+                        const bool rec_path = guide_here && c.guide_record &&
+                                              record_hash(uint32_t(slot), c.rec_seed) < c.rec_threshold;
                         const float su = rng.uniform(), u1 = rng.uniform(), u2 = rng.uniform();
                         f3 wi, fs;
                         float pdf;

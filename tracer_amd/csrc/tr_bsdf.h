@@ -37,7 +37,8 @@ struct DevMaterial {
     f3 spec_scale; float sheen_tint;
     float spec_tint, clearcoat, ax, ay;
     float tax, tay, cc_rough, w_diffuse;
-    float w_trans, w_spec, w_cc, pad;
+    float w_trans, w_spec, w_cc;
+    int guide_ok;   // broad enough for the guide mixture (set by the host; see make_material)
 };
 
 struct BsdfSample {

@@ -12,7 +12,7 @@
 // i.e. per axis the 12 octaves of the frequency encoding plus the position,
 // normal and direction components: 3 + 72 + 3 + 3 used inputs.
 // Output, per lobe k = 0..7: logit a_k, mean m_k (3, normalised), log-concentration
-// s_k: mixture weights softmax(a), kappa_k = exp(clamp(s_k, -4, 8)).
+// s_k: mixture weights softmax(a), kappa_k = exp(clamp(s_k, -4, 10)).
 //
 // MFMA layout (v_mfma_f32_16x16x32_bf16, one wavefront):
 //   A (16 x 32): lane l holds row l % 16, eight K values given by slot j
@@ -114,7 +114,7 @@ TR_HD bool guide_input_used(int in) {
 }
 
 // ---- vMF mixture -----------------------------------------------------------
-constexpr float kLogKappaMin = -4.0f, kLogKappaMax = 8.0f;
+constexpr float kLogKappaMin = -4.0f, kLogKappaMax = 10.0f;
 
 struct Mixture {
     float w[kGuideLobes];      // softmax weights

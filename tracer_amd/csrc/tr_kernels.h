@@ -44,7 +44,8 @@ struct DevPaths {
     float* ovf_t;
     int ovf_threads;  // threads the spill area is sized for
     // path guiding (null without the guide); stats[5] vertices that fell back
-    // to BSDF sampling, stats[6] vertices sampled with the guide mixture
+    // to BSDF sampling, stats[6] vertices sampled with the guide mixture,
+    // stats[7] network evaluations
     const float* graw;        // [slot][48] head outputs of the guide MLP
     int* grec;                // training record waiting for the radiance of its continuation, -1 = none
     float4* rec;              // training records [cap][4]: (x, n.x), (n.yz, wo.xy), (wo.z, wi), (-, slot, pdf, -)

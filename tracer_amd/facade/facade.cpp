@@ -469,6 +469,8 @@ public:
         rp.guide_train_passes = gpu.guide_train_passes;
         rp.guide_fraction = gpu.guide_fraction;
         rp.guide_lr = gpu.guide_lr;
+        rp.guide_train_fraction = gpu.guide_train_fraction;
+        rp.guide_max_depth = gpu.guide_max_depth;
         RenderTarget rt;
         rt.color = Image2D<Spectrum>(rp.width, rp.height);
         static_assert(sizeof(Spectrum) == 12, "Spectrum is three floats");
