@@ -102,27 +102,42 @@ struct TraceStack {
     float tn[LDS_STACK][BLOCK];
 };
 
-// Traverse one ray.  ANY: stop at the first hit in [tmin, tmax].
-// ovf_node/ovf_t: this thread's spill slots, `ovf_stride` apart.
-// COUNT: accumulate node visits and triangle tests.
-template <bool ANY, bool COUNT, int BLOCK, int LDS_STACK>
-TR_D bool trace(const TraceScene& sc, f3 o, float tmin, f3 d, float tmax, TraceStack<BLOCK, LDS_STACK>& st, int tid,
-                int* ovf_node, float* ovf_t, int ovf_stride, TraceHit& hit, uint32_t* nvis = nullptr,
-                uint32_t* ntri = nullptr) {
-    const RayWT wr = wt_setup(o, d);
-    const f3 inv = {safe_rcp(d.x), safe_rcp(d.y), safe_rcp(d.z)};
-    const f3 oinv = {o.x, o.y, o.z};
-    float best_t = tmax;
-    int best_prim = 0x7fffffff;
-    int best_tri = -1;
-    float bb1 = 0.0f, bb2 = 0.0f;
-    // Interval widening: Ize's 2*gamma(3) plus margin for the triangle's own
-    // t rounding.
-    const float lo_scale = 1.0f - 2e-6f, hi_scale = 1.0f + 2e-6f;
-    int sp = 0;
-    int cur = 0;
-    bool have = true;
-    while (true) {
+// Traversal state of one ray.  step() visits one node: it tests the node's
+// four child boxes, intersects the triangles of hit leaf children (leaves
+// first: they can only shrink best_t) and pushes the hit inner children,
+// nearest last.  Kernels either run step() to completion per ray (trace()) or
+// interleave steps with fetching new rays for lanes that finished.
+struct TraceRay {
+    RayWT wr;
+    f3 inv;
+    float tmin;
+    float best_t;
+    int best_prim, best_tri;
+    float b1, b2;
+    int sp, cur;
+    bool have;
+
+    TR_D void init(f3 o, float tmin_, f3 d, float tmax) {
+        wr = wt_setup(o, d);
+        inv = mk(safe_rcp(d.x), safe_rcp(d.y), safe_rcp(d.z));
+        tmin = tmin_;
+        best_t = tmax;
+        best_prim = 0x7fffffff;
+        best_tri = -1;
+        b1 = b2 = 0.0f;
+        sp = 0;
+        cur = 0;
+        have = true;
+    }
+
+    // One node visit.  Returns false once traversal is over: the stack is
+    // empty, or (ANY) a hit in [tmin, tmax] was found.
+    template <bool ANY, bool COUNT, int BLOCK, int LDS_STACK>
+    TR_D bool step(const TraceScene& sc, TraceStack<BLOCK, LDS_STACK>& st, int tid, int* ovf_node, float* ovf_t,
+                   int ovf_stride, uint32_t* nvis, uint32_t* ntri) {
+        // Interval widening: Ize's 2*gamma(3) plus margin for the triangle's own
+        // t rounding.
+        const float lo_scale = 1.0f - 2e-6f, hi_scale = 1.0f + 2e-6f;
         if (!have) {
             // pop, skipping entries farther than the current closest hit
             bool found = false;
@@ -134,9 +149,10 @@ TR_D bool trace(const TraceScene& sc, f3 o, float tmin, f3 d, float tmax, TraceS
                 else { n = ovf_node[(sp - LDS_STACK) * ovf_stride]; t = ovf_t[(sp - LDS_STACK) * ovf_stride]; }
                 if (t <= best_t) { cur = n; found = true; break; }
             }
-            if (!found) break;
+            if (!found) return false;
         }
         if (COUNT) ++*nvis;
+        const f3 o = wr.o;
         const float4* np = sc.nodes + size_t(cur) * 8;
         const float4 lox = np[0], hix = np[1], loy = np[2], hiy = np[3], loz = np[4], hiz = np[5];
         const int4 ch = *reinterpret_cast<const int4*>(np + 6);
@@ -148,9 +164,9 @@ TR_D bool trace(const TraceScene& sc, f3 o, float tmin, f3 d, float tmax, TraceS
             const float lz[4] = {loz.x, loz.y, loz.z, loz.w}, hz[4] = {hiz.x, hiz.y, hiz.z, hiz.w};
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
-                const float x0 = (lx[c] - oinv.x) * inv.x, x1 = (hx[c] - oinv.x) * inv.x;
-                const float y0 = (ly[c] - oinv.y) * inv.y, y1 = (hy[c] - oinv.y) * inv.y;
-                const float z0 = (lz[c] - oinv.z) * inv.z, z1 = (hz[c] - oinv.z) * inv.z;
+                const float x0 = (lx[c] - o.x) * inv.x, x1 = (hx[c] - o.x) * inv.x;
+                const float y0 = (ly[c] - o.y) * inv.y, y1 = (hy[c] - o.y) * inv.y;
+                const float z0 = (lz[c] - o.z) * inv.z, z1 = (hz[c] - o.z) * inv.z;
                 const float tnear = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin)) * lo_scale;
                 const float tfar = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1)) * hi_scale;
                 hm[c] = tnear <= tfar && tnear <= best_t && tfar >= tmin;
@@ -158,7 +174,6 @@ TR_D bool trace(const TraceScene& sc, f3 o, float tmin, f3 d, float tmax, TraceS
             }
         }
         const int chv[4] = {ch.x, ch.y, ch.z, ch.w};
-        // leaves first: they can only shrink best_t
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             if (!hm[c] || chv[c] >= 0) continue;
@@ -173,11 +188,11 @@ TR_D bool trace(const TraceScene& sc, f3 o, float tmin, f3 d, float tmax, TraceS
                 if (!wt_test(wr, tmin, best_t, xyz(a), xyz(b), xyz(c2), t, u1, u2)) continue;
                 const int prim = __float_as_int(a.w);
                 if (ANY) {
-                    hit.t = t; hit.tri = ti; hit.prim = prim; hit.b1 = u1; hit.b2 = u2;
-                    return true;
+                    best_t = t; best_tri = ti; best_prim = prim; b1 = u1; b2 = u2;
+                    return false;
                 }
                 if (t < best_t || prim < best_prim) {
-                    best_t = t; best_prim = prim; best_tri = ti; bb1 = u1; bb2 = u2;
+                    best_t = t; best_prim = prim; best_tri = ti; b1 = u1; b2 = u2;
                 }
             }
         }
@@ -189,7 +204,7 @@ TR_D bool trace(const TraceScene& sc, f3 o, float tmin, f3 d, float tmax, TraceS
         for (int c = 0; c < 4; ++c) {
             if (hm[c] && chv[c] >= 0 && tn[c] <= best_t) { ids[m] = chv[c]; ts[m] = tn[c]; ++m; }
         }
-        if (m == 0) { have = false; continue; }
+        if (m == 0) { have = false; return true; }
         // insertion sort, descending (far first) so the nearest ends last
         for (int i = 1; i < m; ++i) {
             const int ki = ids[i];
@@ -210,13 +225,31 @@ TR_D bool trace(const TraceScene& sc, f3 o, float tmin, f3 d, float tmax, TraceS
         if (sp > sc.max_stack) sp = sc.max_stack;   // unreachable by construction (3 * depth)
         cur = ids[m - 1];
         have = true;
+        return true;
     }
-    if (best_tri < 0) {
-        hit.t = 0.0f; hit.tri = -1; hit.prim = -1; hit.b1 = hit.b2 = 0.0f;
-        return false;
+
+    TR_D bool result(TraceHit& hit) const {
+        if (best_tri < 0) {
+            hit.t = 0.0f; hit.tri = -1; hit.prim = -1; hit.b1 = hit.b2 = 0.0f;
+            return false;
+        }
+        hit.t = best_t; hit.tri = best_tri; hit.prim = best_prim; hit.b1 = b1; hit.b2 = b2;
+        return true;
     }
-    hit.t = best_t; hit.tri = best_tri; hit.prim = best_prim; hit.b1 = bb1; hit.b2 = bb2;
-    return true;
+};
+
+// Traverse one ray to completion.  ANY: stop at the first hit in [tmin, tmax].
+// ovf_node/ovf_t: this thread's spill slots, `ovf_stride` apart.
+// COUNT: accumulate node visits and triangle tests.
+template <bool ANY, bool COUNT, int BLOCK, int LDS_STACK>
+TR_D bool trace(const TraceScene& sc, f3 o, float tmin, f3 d, float tmax, TraceStack<BLOCK, LDS_STACK>& st, int tid,
+                int* ovf_node, float* ovf_t, int ovf_stride, TraceHit& hit, uint32_t* nvis = nullptr,
+                uint32_t* ntri = nullptr) {
+    TraceRay r;
+    r.init(o, tmin, d, tmax);
+    while (r.step<ANY, COUNT, BLOCK, LDS_STACK>(sc, st, tid, ovf_node, ovf_t, ovf_stride, nvis, ntri)) {
+    }
+    return r.result(hit);
 }
 
 }  // namespace trhip
