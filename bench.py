@@ -24,7 +24,10 @@ Also reported on the same JSON line:
                  MFMA peak.
   cpu_baseline   the CPU restatement of the reference renderer (all lights
                  per vertex, binary SAH BVH, std::thread over 16x16 tiles) on
-                 configuration 2 at a reduced film, timed on this host.
+                 configuration 2 at a reduced film, timed on this host, with
+                 the GPU's time-to-equal-relMSE speedup on the same film.
+  other_configs  configuration 3 (fog) full frame, configuration 5 (18.9M
+                 triangles) at 16 spp.
 """
 from __future__ import annotations
 
@@ -57,28 +60,87 @@ def cpu_model() -> str:
 
 
 def cpu_baseline(scene, threads: int, target_s: float = 15.0) -> dict:
-    """The oracle's reference-mode renderer on a reduced film of the same scene."""
+    """The oracle's CPU renderer (fixed mode: the reference's algorithm with its
+    estimator defects corrected) on a reduced film of the same scene, and the
+    time-to-equal-relMSE ratio of the plain GPU path on that film."""
     from oracle import refcpu
+    from tracer_amd import native
     t0 = time.time()
     ref = refcpu.RefScene(scene, 3)
     build_s = time.time() - t0
     W, H = 256, 256
     probe = scene.render_params(width=W, height=H, spp=1, seed=99)
-    _, ms, _ = ref.render(probe, quirks=True, nee=0, threads=threads)
+    _, ms, _ = ref.render(probe, quirks=False, nee=0, threads=threads)
     spp = max(1, min(64, int(target_s * 1000.0 / max(ms, 1.0))))
     p = scene.render_params(width=W, height=H, spp=spp, seed=100)
-    _, ms, rays = ref.render(p, quirks=True, nee=0, threads=threads)
+    cpu_sums, ms, rays = ref.render(p, quirks=False, nee=0, threads=threads)
     samples = W * H * spp
+    # GPU on the same film: a 16384-spp reference, then the plain path at 64 spp
+    ctx = native.Context(0)
+    ctx.load_scene(scene)
+    R = None
+    for c in range(4):
+        R, _, _ = ctx.render(scene.render_params(width=W, height=H, spp=4096, seed=7, sample_offset=4096 * c), film=R,
+                             want_image=False)
+    Rimg = R[..., :3] / R[..., 3:]
+
+    def relmse(sums):
+        img = sums[..., :3] / sums[..., 3:]
+        return float(np.mean((img - Rimg) ** 2 / (Rimg ** 2 + 1e-2)))
+
+    gpu_spp = 64
+    g_err, g_ms = [], []
+    for k in range(4):
+        sums, _, st = ctx.render(scene.render_params(width=W, height=H, spp=gpu_spp, seed=300 + k), want_image=False)
+        g_err.append(relmse(sums))
+        g_ms.append(st.ms_total)
+    ctx.close()
+    cpu_err = relmse(cpu_sums)
+    # relMSE falls as 1/time for both: the ratio of (relMSE x time) is the time-to-equal-relMSE speedup
+    speedup = (cpu_err * ms) / (float(np.mean(g_err)) * float(np.mean(g_ms)))
     return {
         "value": samples / (ms / 1e3),
         "unit": "samples/s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"reference-mode CPU restatement (all-lights NEE + MIS BSDF ray, binary SAH BVH leaf 3, "
-                   f"16x16 tiles) on configuration 2, {W}x{H} film at {spp} spp ({samples} samples, "
+        "sample": (f"CPU restatement of the reference renderer in fixed mode (all-lights NEE + MIS BSDF ray, binary SAH "
+                   f"BVH leaf 3, 16x16 tiles) on configuration 2, {W}x{H} film at {spp} spp ({samples} samples, "
                    f"{ms / 1e3:.1f} s, {(rays[0] + rays[1]) / (ms / 1e3) / 1e6:.1f} Mrays/s); BVH build "
                    f"{build_s:.1f} s not timed; CPU: {cpu_model()}"),
+        "equal_relmse": {
+            "film": f"{W}x{H}", "reference": "plain GPU path, 16384 spp, FP64 film",
+            "cpu_spp": spp, "cpu_ms": ms, "cpu_relmse": cpu_err,
+            "gpu_spp": gpu_spp, "gpu_ms": float(np.mean(g_ms)), "gpu_relmse": float(np.mean(g_err)),
+            "gpu_speedup_at_equal_relmse": speedup,
+        },
     }
+
+
+def other_configs() -> dict:
+    """Full frames of configuration 3 (fog, delta tracking) and a reduced-spp
+    frame of configuration 5 (18.9M triangles)."""
+    from tracer_amd import native
+    out = {}
+    ctx = native.Context(0)
+    for cfg, spp in ((3, None), (5, 16)):
+        scene = native.config_scene(cfg)
+        t0 = time.time()
+        ctx.load_scene(scene)
+        bvh_s = time.time() - t0
+        p = scene.render_params(seed=0) if spp is None else scene.render_params(seed=0, spp=spp)
+        ctx.render(scene.render_params(seed=9, spp=1, width=p.width, height=p.height), want_image=False)
+        t0 = time.perf_counter()
+        _, _, st = ctx.render(p, want_image=False)
+        el = time.perf_counter() - t0
+        out[f"config{cfg}"] = {
+            "workload": (f"{scene.num_triangles} triangles, {p.width}x{p.height}, {p.spp} spp, max depth {p.max_depth}"),
+            "samples_per_s": p.width * p.height * p.spp / el, "mrays_per_s": (st.extension_rays + st.shadow_rays) / el / 1e6,
+            "s_per_frame": el, "bvh_build_s": bvh_s,
+        }
+        print(f"[bench] configuration {cfg}: {out[f'config{cfg}']['samples_per_s'] / 1e6:.1f} M samples/s",
+              file=sys.stderr, flush=True)
+    ctx.close()
+    return out
 
 
 def read_traffic(path: Path):
@@ -206,6 +268,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-relmse", action="store_true")
+    ap.add_argument("--no-other", action="store_true", help="skip the configuration 3 and 5 frames")
     ap.add_argument("--relmse-res", type=int, default=256)
     ap.add_argument("--ref-spp", type=int, default=65536)
     ap.add_argument("--relmse-spp", type=int, default=256)
@@ -295,6 +358,8 @@ def main() -> int:
             "ms_per_launch": ms_per_launch,
         },
     }
+    if not args.no_other:
+        out["other_configs"] = other_configs()
     if not args.no_relmse:
         print("[bench] equal-time relMSE on configuration 4 ...", file=sys.stderr, flush=True)
         out.update(equal_time(args.relmse_res, args.ref_spp, args.relmse_spp, args.relmse_seeds, 64))
@@ -305,6 +370,8 @@ def main() -> int:
         print(f"[bench] CPU baseline on {threads} threads ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(scene, threads)
         out["gpu_over_cpu_samples_per_s"] = value / out["cpu_baseline"]["value"]
+        print(f"[bench] GPU/CPU at equal relMSE: {out['cpu_baseline']['equal_relmse']['gpu_speedup_at_equal_relmse']:.0f}x",
+              file=sys.stderr, flush=True)
     print(json.dumps(out))
     return 0
 

@@ -7,7 +7,7 @@
 namespace trhip {
 
 constexpr int kTraceBlock = 256;
-constexpr int kLdsStack = 16;
+constexpr int kLdsStack = 8;
 constexpr int kShadeBlock = 256;
 
 // Wave-aggregated append: one atomic per wavefront, lanes get consecutive
