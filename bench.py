@@ -180,7 +180,7 @@ def throughput(ctx, scene, steps: int, warmup: int) -> dict:
     return {"W": W, "H": H, "spp": spp, "max_depth": base.max_depth, "elapsed": elapsed, **tot}
 
 
-def equal_time(res: int, ref_spp: int, spp: int, seeds: int, hidden: int) -> dict:
+def equal_time(res: int, ref_spp: int, spp: int, seeds: int, hidden: int, train_pp: int = 32, steps: int = 128) -> dict:
     """Configuration 4 at res x res: guided (training included) vs plain at the
     same device time, both against a ref_spp plain reference."""
     from tracer_amd import native
@@ -201,10 +201,10 @@ def equal_time(res: int, ref_spp: int, spp: int, seeds: int, hidden: int) -> dic
         img = sums[..., :3] / sums[..., 3:]
         return float(np.mean((img - R) ** 2 / (R ** 2 + 1e-2)))
 
-    train_spp, train_pp = spp // 2, 8   # training phase: passes of 8 spp, Adam steps after each
+    train_spp = spp // 2   # training phase: passes of train_pp spp, `steps` Adam steps after each
     g_err, g_ms, g_train_ms, g_inf_ms, g_inf, g_smp = [], [], [], [], 0, 0
     for k in range(seeds):
-        ctx.guide_init(hidden=hidden, seed=11 + k, batch=1 << 16, steps_per_pass=32)
+        ctx.guide_init(hidden=hidden, seed=11 + k, batch=1 << 16, steps_per_pass=steps)
         s1, _, a = ctx.render(scene.render_params(width=res, height=res, spp=train_spp, seed=200 + k, guide=1,
                                                   guide_train_passes=train_spp // train_pp, samples_per_pass=train_pp,
                                                   profile=1), want_image=False)
@@ -240,7 +240,7 @@ def equal_time(res: int, ref_spp: int, spp: int, seeds: int, hidden: int) -> dic
             "guided": {"spp": spp, "ms": float(np.mean(g_ms)), "relmse": float(np.mean(g_err)),
                        "relmse_median": float(np.median(g_err)),
                        "training": f"{train_spp} spp in passes of {train_pp} spp sampled without the guide "
-                                   f"(recording only), 32 Adam steps of 65536 samples after each (lr 1e-3), then "
+                                   f"(recording only), {steps} Adam steps of 65536 samples after each (lr 1e-3), then "
                                    f"{spp - train_spp} spp guided (one-sample MIS 0.5 guide / 0.5 BSDF)",
                        "ms_training": float(np.mean(g_train_ms)), "ms_inference": float(np.mean(g_inf_ms))},
             "plain_equal_time": {"spp": spp_eq, "ms": float(np.mean(p_ms)) * spp_eq / spp,
@@ -273,6 +273,8 @@ def main() -> int:
     ap.add_argument("--ref-spp", type=int, default=65536)
     ap.add_argument("--relmse-spp", type=int, default=256)
     ap.add_argument("--relmse-seeds", type=int, default=6)
+    ap.add_argument("--train-pass-spp", type=int, default=32, help="spp per training pass of the guided render")
+    ap.add_argument("--train-steps", type=int, default=128, help="Adam steps after each training pass")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one 1 GiB device-to-device copy (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -362,7 +364,8 @@ def main() -> int:
         out["other_configs"] = other_configs()
     if not args.no_relmse:
         print("[bench] equal-time relMSE on configuration 4 ...", file=sys.stderr, flush=True)
-        out.update(equal_time(args.relmse_res, args.ref_spp, args.relmse_spp, args.relmse_seeds, 64))
+        out.update(equal_time(args.relmse_res, args.ref_spp, args.relmse_spp, args.relmse_seeds, 64,
+                              args.train_pass_spp, args.train_steps))
         print(f"[bench] equal-time ratio {out['equal_time']['ratio_plain_over_guided']:.2f}", file=sys.stderr,
               flush=True)
     if not args.no_cpu:
