@@ -224,7 +224,7 @@ void build_device_scene(tr_ctx* c, const tr_bvh_opts* opts) {
     hip_check(hipStreamSynchronize(st), "scene upload");
 
     // traversal spill area: (max_stack - LDS entries) per thread of the largest grid
-    const int threads = trace_grid_threads(int64_t(1) << 40);
+    const int threads = trace_grid_threads();
     const int extra = std::max(1, c->max_stack - kLdsStack);
     c->ovf_node.reserve(size_t(extra) * threads * sizeof(int));
     c->ovf_t.reserve(size_t(extra) * threads * sizeof(float));
@@ -275,6 +275,22 @@ void ensure_paths(tr_ctx* c, int64_t P) {
     c->q0.reserve(n * 4); c->q1.reserve(n * 4);
     c->sh_o.reserve(n * 16); c->sh_d.reserve(n * 16); c->sh_c.reserve(n * 16);
     c->path_capacity = P;
+}
+
+// Samples per pixel in flight when the caller leaves it to the library. The
+// late bounces of a pass trace only the few paths still alive, and a launch of
+// a few thousand rays costs about as much as one of a hundred thousand, so the
+// pass holds as many paths as memory allows up to 2^26 (a 1024x1024 frame of
+// 64 spp in one pass). Path slots may use a quarter of the free device memory
+// plus what this context already holds.
+int auto_samples_per_pass(tr_ctx* c, int64_t npix, bool guided) {
+    const int64_t slot_bytes = 5 * 16 + 2 * 8 + 2 * 4 + 3 * 16 + (guided ? kGuideOut * 4 + 2 * 4 : 0);
+    size_t free_b = 0, total_b = 0;
+    hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+    const int64_t held = guided ? std::min(c->path_capacity, c->guide.path_capacity) : c->path_capacity;
+    const int64_t fit = held + int64_t(free_b / 4) / slot_bytes;
+    const int64_t target = std::min<int64_t>(int64_t(1) << 26, fit);
+    return int(std::max<int64_t>(1, target / npix));
 }
 
 DevPaths make_paths(tr_ctx* c) {
@@ -574,7 +590,8 @@ tr_status tr_render(tr_ctx* c, const tr_render_params* prm, double* film_sums, i
         if (P.guide && P.guide_train_passes < 0) throw InvalidArg("guide_train_passes must be >= 0");
         const int64_t npix = int64_t(P.width) * P.height;
         if (npix > (int64_t(1) << 28)) throw InvalidArg("image too large");
-        int spp_pp = P.samples_per_pass > 0 ? P.samples_per_pass : int(std::max<int64_t>(1, (int64_t(1) << 21) / npix));
+        int spp_pp = P.samples_per_pass;
+        if (spp_pp <= 0) spp_pp = auto_samples_per_pass(c, npix, P.guide != 0);
         spp_pp = std::min(spp_pp, P.spp);
         const int64_t maxP = npix * spp_pp;
         if (maxP > (int64_t(1) << 30)) throw InvalidArg("too many paths per pass");

@@ -1,19 +1,36 @@
 // Traversal kernels.  Built with -ffp-contract=off: the watertight triangle
 // test must round exactly like the CPU restatement it is checked against.
+#include <algorithm>
+#include <atomic>
+#include <stdexcept>
+
 #include "tr_kernels.h"
 
 namespace trhip {
 
-constexpr int kMaxTraceBlocks = 2048;
+namespace {
 
-int trace_grid_threads(int64_t n) {
-    int64_t blocks = (n + kTraceBlock - 1) / kTraceBlock;
-    if (blocks < 1) blocks = 1;
-    if (blocks > kMaxTraceBlocks) blocks = kMaxTraceBlocks;
-    return int(blocks) * kTraceBlock;
+enum TraceKernel { TK_CLOSEST, TK_ANY, TK_COUNTED, TK_EXTEND, TK_EXTEND_COUNT, TK_SHADOW, TK_NUM };
+
+// The trace kernels loop over their rays with a grid stride, so every wave
+// gets an equal share.  A grid that is not a whole number of the kernel's
+// resident waves leaves its last round part-empty: on configuration 2
+// (64 spp per pass) a 2048-block k_extend (1.6 rounds at 5 waves per SIMD)
+// took 48.5 ms per frame against 36.6 ms at three full rounds.
+constexpr int kTraceRounds = 3;
+constexpr int kMaxDevices = 64;
+std::atomic<int> g_grid[kMaxDevices][TK_NUM];
+
+template <typename K>
+int resident_rounds(K kernel) {
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kTraceBlock, 0) != hipSuccess)
+        throw std::runtime_error("trace kernel occupancy query failed");
+    return std::max(1, per_cu) * std::max(1, cus) * kTraceRounds;
 }
 
-static int trace_blocks(int64_t n) { return trace_grid_threads(n) / kTraceBlock; }
+}  // namespace
 
 __global__ __launch_bounds__(kTraceBlock) void k_trace_closest(TraceScene sc, const float4* __restrict__ rays,
                                                                 float4* __restrict__ hits, int64_t n, int* ovf_node,
@@ -70,8 +87,9 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_counted(TraceScene sc, co
 // one ray per lane at a time in queue order: a wave's lanes traverse rays of
 // neighbouring queue entries (nearby pixels) together.  Letting each lane run
 // ahead through its own sequence of rays measured slower on the MI355X
-// (0.32 vs 0.30 ms per launch on configuration 2), and fetching rays from a
-// global work counter much slower (0.49 ms: the atomics contend).
+// (0.32 vs 0.30 ms per launch on configuration 2 at 2^21 paths per pass), and
+// fetching rays from a global work counter much slower (0.49 ms: the atomics
+// contend).
 // COUNT: also accumulate node visits and triangle tests into stats[3..4].
 template <bool COUNT>
 __global__ __launch_bounds__(kTraceBlock) void k_extend(TraceScene sc, DevPaths p, int cur) {
@@ -136,35 +154,69 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(TraceScene sc, DevPaths 
     }
 }
 
+namespace {
+
+// Blocks of a full-size launch of kernel k on the current device.
+int grid_blocks(TraceKernel k) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) throw std::runtime_error("no current device");
+    int b = g_grid[dev][k].load(std::memory_order_relaxed);
+    if (b > 0) return b;
+    switch (k) {
+        case TK_CLOSEST: b = resident_rounds(k_trace_closest); break;
+        case TK_ANY: b = resident_rounds(k_trace_any); break;
+        case TK_COUNTED: b = resident_rounds(k_trace_counted); break;
+        case TK_EXTEND: b = resident_rounds(k_extend<false>); break;
+        case TK_EXTEND_COUNT: b = resident_rounds(k_extend<true>); break;
+        default: b = resident_rounds(k_shadow); break;
+    }
+    g_grid[dev][k].store(b, std::memory_order_relaxed);
+    return b;
+}
+
+// Blocks for n rays, at most the full-size grid and at most what the spill
+// area (one column per thread) covers.
+int trace_blocks(TraceKernel k, int64_t n, int ovf_threads) {
+    int64_t blocks = (n + kTraceBlock - 1) / kTraceBlock;
+    blocks = std::min<int64_t>(blocks, grid_blocks(k));
+    blocks = std::min<int64_t>(blocks, ovf_threads / kTraceBlock);
+    if (blocks < 1) throw std::runtime_error("traversal spill area smaller than one block");
+    return int(blocks);
+}
+
+}  // namespace
+
+int trace_grid_threads() {
+    int b = 0;
+    for (int k = 0; k < TK_NUM; ++k) b = std::max(b, grid_blocks(TraceKernel(k)));
+    return b * kTraceBlock;
+}
+
 void launch_trace_closest(const TraceScene& sc, const float4* rays, float4* hits, int64_t n, int* ovf_node, float* ovf_t,
                           int ovf_threads, hipStream_t st) {
     if (n <= 0) return;
-    const int blocks = trace_blocks(n);
-    (void)ovf_threads;
-    hipLaunchKernelGGL(k_trace_closest, dim3(blocks), dim3(kTraceBlock), 0, st, sc, rays, hits, n, ovf_node, ovf_t);
+    hipLaunchKernelGGL(k_trace_closest, dim3(trace_blocks(TK_CLOSEST, n, ovf_threads)), dim3(kTraceBlock), 0, st, sc, rays, hits, n, ovf_node, ovf_t);
 }
 void launch_trace_any(const TraceScene& sc, const float4* rays, int* occ, int64_t n, int* ovf_node, float* ovf_t,
                       int ovf_threads, hipStream_t st) {
     if (n <= 0) return;
-    (void)ovf_threads;
-    hipLaunchKernelGGL(k_trace_any, dim3(trace_blocks(n)), dim3(kTraceBlock), 0, st, sc, rays, occ, n, ovf_node, ovf_t);
+    hipLaunchKernelGGL(k_trace_any, dim3(trace_blocks(TK_ANY, n, ovf_threads)), dim3(kTraceBlock), 0, st, sc, rays, occ, n, ovf_node, ovf_t);
 }
 void launch_trace_counted(const TraceScene& sc, const float4* rays, float4* hits, uint32_t* nvis, uint32_t* ntri, int64_t n,
                           int* ovf_node, float* ovf_t, int ovf_threads, hipStream_t st) {
     if (n <= 0) return;
-    (void)ovf_threads;
-    hipLaunchKernelGGL(k_trace_counted, dim3(trace_blocks(n)), dim3(kTraceBlock), 0, st, sc, rays, hits, nvis, ntri, n,
+    hipLaunchKernelGGL(k_trace_counted, dim3(trace_blocks(TK_COUNTED, n, ovf_threads)), dim3(kTraceBlock), 0, st, sc, rays, hits, nvis, ntri, n,
                        ovf_node, ovf_t);
 }
 void launch_extend(const TraceScene& sc, const DevPaths& p, int cur, int max_paths, bool count, hipStream_t st) {
     if (count)
-        hipLaunchKernelGGL(k_extend<true>, dim3(trace_blocks(max_paths)), dim3(kTraceBlock), 0, st, sc, p, cur);
+        hipLaunchKernelGGL(k_extend<true>, dim3(trace_blocks(TK_EXTEND_COUNT, max_paths, p.ovf_threads)), dim3(kTraceBlock), 0, st, sc, p, cur);
     else
-        hipLaunchKernelGGL(k_extend<false>, dim3(trace_blocks(max_paths)), dim3(kTraceBlock), 0, st, sc, p, cur);
+        hipLaunchKernelGGL(k_extend<false>, dim3(trace_blocks(TK_EXTEND, max_paths, p.ovf_threads)), dim3(kTraceBlock), 0, st, sc, p, cur);
 }
 void launch_shadow(const TraceScene& sc, const DevPaths& p, int max_shadow, int accumulate_atomic, int record,
                    hipStream_t st) {
-    hipLaunchKernelGGL(k_shadow, dim3(trace_blocks(max_shadow)), dim3(kTraceBlock), 0, st, sc, p, accumulate_atomic,
+    hipLaunchKernelGGL(k_shadow, dim3(trace_blocks(TK_SHADOW, max_shadow, p.ovf_threads)), dim3(kTraceBlock), 0, st, sc, p, accumulate_atomic,
                        record);
 }
 

@@ -86,8 +86,9 @@ void launch_extend(const TraceScene& sc, const DevPaths& p, int cur, int max_pat
 // record != 0: validate the light-sample training records of unoccluded shadow rays
 void launch_shadow(const TraceScene& sc, const DevPaths& p, int max_shadow, int accumulate_atomic, int record,
                    hipStream_t st);
-// Grid size the spill area must cover for launches of up to n rays.
-int trace_grid_threads(int64_t n);
+// Threads of the largest trace-kernel grid on the current device: the spill
+// area holds one column per thread.
+int trace_grid_threads();
 
 // k_render.hip
 void launch_generate(const DevScene& s, const DevPaths& p, const RenderConsts& c, int num_paths, int cur, hipStream_t st);
