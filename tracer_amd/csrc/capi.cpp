@@ -129,6 +129,25 @@ void validate_scene(const tr_scene_desc* d) {
     }
 }
 
+// Cutpoint table over the light CDF (Chen and Asau 1974): entry j holds the
+// first index whose CDF value exceeds j / K.  K is a power of two, so u * K and
+// j / K are exact and a scan from entry floor(u * K) ends at the index the
+// binary search of Distribution1D::sample_discrete finds
+// (src/utility/distribution.hpp:49-67), after one or two CDF reads.
+std::vector<int> light_cutpoints(const std::vector<float>& cdf) {
+    const int n = int(cdf.size());
+    int K = 1;
+    while (K < n && K < (1 << 20)) K <<= 1;
+    std::vector<int> g(static_cast<size_t>(K));
+    int i = 0;
+    for (int j = 0; j < K; ++j) {
+        const float x = float(j) / float(K);
+        while (i < n - 1 && !(cdf[size_t(i)] > x)) ++i;
+        g[size_t(j)] = i;
+    }
+    return g;
+}
+
 void build_device_scene(tr_ctx* c, const tr_bvh_opts* opts) {
     HostScene& h = c->hs;
     const int64_t nt = int64_t(h.mat.size());
@@ -189,8 +208,12 @@ void build_device_scene(tr_ctx* c, const tr_bvh_opts* opts) {
         if (li >= 0) {
             const f3 L = mk(h.emission[3 * prim], h.emission[3 * prim + 1], h.emission[3 * prim + 2]);
             const double area = 0.5 * std::sqrt(double(cr.x) * cr.x + double(cr.y) * cr.y + double(cr.z) * cr.z);
+            // (L, area), (A, pmf), (B, triangle), (C, 0): the vertices travel with
+            // the light so that sampling it needs no second dependent load
             lights.push_back(f4(L, float(area)));
-            lights.push_back(make_float4(__builtin_bit_cast(float, int32_t(i)), 0.0f, 0.0f, 0.0f));
+            lights.push_back(f4(A, 0.0f));
+            lights.push_back(f4(B, __builtin_bit_cast(float, int32_t(i))));
+            lights.push_back(f4(C, 0.0f));
             power[size_t(li)] = double(kPi) * lum(L) * area;   // DiffuseLight::power, diffuse_light.cpp:17-20
         }
     }
@@ -205,7 +228,8 @@ void build_device_scene(tr_ctx* c, const tr_bvh_opts* opts) {
         acc += tot > 0.0 ? power[i] / tot : 0.0;
         cdf[i] = float(acc);
     }
-    for (int i = 0; i < nA; ++i) lights[2 * size_t(i) + 1].y = pmf[size_t(i)];
+    for (int i = 0; i < nA; ++i) lights[4 * size_t(i) + 1].w = pmf[size_t(i)];
+    const std::vector<int> guide = light_cutpoints(cdf);
 
     std::vector<DevMaterial> dm(h.mats.size());
     for (size_t i = 0; i < h.mats.size(); ++i) dm[i] = make_material(h.mats[i]);
@@ -221,6 +245,7 @@ void build_device_scene(tr_ctx* c, const tr_bvh_opts* opts) {
     upload(c->lights, lights.data(), lights.size() * sizeof(float4), st);
     upload(c->light_cdf, cdf.data(), cdf.size() * sizeof(float), st);
     upload(c->light_pmf, pmf.data(), pmf.size() * sizeof(float), st);
+    upload(c->light_guide, guide.data(), guide.size() * sizeof(int), st);
     hip_check(hipStreamSynchronize(st), "scene upload");
 
     // traversal spill area: (max_stack - LDS entries) per thread of the largest grid
@@ -240,6 +265,8 @@ void build_device_scene(tr_ctx* c, const tr_bvh_opts* opts) {
     s.lights = c->lights.as<float4>();
     s.light_cdf = c->light_cdf.as<float>();
     s.light_pmf = c->light_pmf.as<float>();
+    s.light_guide = c->light_guide.as<int>();
+    s.light_guide_size = int(guide.size());
     s.num_area_lights = nA;
     s.num_light_entries = tot > 0.0 ? nA + 2 : 0;
     s.has_env = h.has_env;

@@ -108,7 +108,7 @@ struct tr_ctx {
     trcapi::HostScene hs;
     bool have_scene = false, have_bvh = false;
     // scene on device
-    trcapi::DevBuf nodes, tris, tri_shade, mats, media, lights, light_cdf, light_pmf;
+    trcapi::DevBuf nodes, tris, tri_shade, mats, media, lights, light_cdf, light_pmf, light_guide;
     int64_t num_nodes = 0;
     int max_stack = 0;
     int bvh_depth = 0;

@@ -237,10 +237,10 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene s, DevPaths p, R
                     // ---- surface vertex ----
                     if (sr.light >= 0 && dn < 0.0f) {
                         // emission with MIS against light sampling (direct_illumination.cpp:176-197)
-                        const float4 l0 = s.lights[2 * sr.light], l1 = s.lights[2 * sr.light + 1];
+                        const float4 l0 = s.lights[4 * sr.light];
                         float w = 1.0f;
                         if (prev_pdf > 0.0f) {
-                            const float lp = l1.y * t * t / (l0.w * (-dn));
+                            const float lp = s.lights[4 * sr.light + 1].w * t * t / (l0.w * (-dn));
                             w = power_heuristic(prev_pdf, lp);
                         }
                         L += T * xyz(l0) * w;

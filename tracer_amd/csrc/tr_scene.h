@@ -32,9 +32,11 @@ struct DevScene {
     const float4* tri_shade;   // 3 float4 per leaf-ordered triangle: (n0, material), (n1, light), (n2, media)
     const DevMaterial* mats;
     const DevMedium* media;
-    const float4* lights;      // 2 float4 per area light: (L, area), (tri, pmf, -, -)
+    const float4* lights;      // 4 float4 per area light: (L, area), (A, pmf), (B, tri), (C, 0)
     const float* light_cdf;    // over [area lights..., env, sun]
     const float* light_pmf;
+    const int* light_guide;    // cutpoints: first CDF index above j / light_guide_size
+    int light_guide_size;      // a power of two
     int num_area_lights;
     int num_light_entries;
     int has_env, has_sun;
@@ -65,14 +67,15 @@ TR_D TriShade load_shade(const DevScene& s, int tri) {
 
 // Light selection by power (compute_light_power_distribution,
 // src/renderer/direct_illumination.cpp:257-264; Distribution1D::sample_discrete,
-// src/utility/distribution.hpp:49-67): binary search in the CDF.
+// src/utility/distribution.hpp:49-67): the first CDF entry above u, or the
+// last entry.  The scan starts at the cutpoint of u's bucket, which lies at or
+// before that entry, so it returns what a binary search returns.
 TR_D int pick_light(const DevScene& s, float u, float& pmf) {
-    int lo = 0, hi = s.num_light_entries - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (s.light_cdf[mid] > u) hi = mid;
-        else lo = mid + 1;
-    }
+    const int last = s.num_light_entries - 1;
+    const int K = s.light_guide_size;
+    const int j = min(int(u * float(K)), K - 1);
+    int lo = s.light_guide[max(j, 0)];
+    while (lo < last && !(s.light_cdf[lo] > u)) ++lo;
     while (lo > 0 && s.light_pmf[lo] == 0.0f) --lo;
     pmf = s.light_pmf[lo];
     return lo;
@@ -96,10 +99,9 @@ TR_D LightSampleD sample_light(const DevScene& s, int entry, f3 ref, float u, fl
     r.pdf = 0.0f;
     r.area = false;
     if (entry < s.num_area_lights) {
-        const float4 l0 = s.lights[2 * entry], l1 = s.lights[2 * entry + 1];
-        const int tri = __float_as_int(l1.x);
-        const float4* tp = s.tr.tris + size_t(tri) * 3;
-        const f3 A = xyz(tp[0]), B = xyz(tp[1]), C = xyz(tp[2]);
+        const float4* lr = s.lights + 4 * size_t(entry);
+        const float4 l0 = lr[0];
+        const f3 A = xyz(lr[1]), B = xyz(lr[2]), C = xyz(lr[3]);
         const float su = sqrtf(u);
         const float bx = 1.0f - su, by = v * su;
         const f3 pos = A * bx + B * by + C * (1.0f - bx - by);
