@@ -174,18 +174,27 @@ struct TraceRay {
             }
         }
         const int chv[4] = {ch.x, ch.y, ch.z, ch.w};
+        // Triangles of every hit leaf child in one loop (child order, then
+        // triangle order), so that one copy of the test serves all four
+        // children and a wave iterates as often as its busiest lane needs.
+        unsigned leaves = 0;
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            if (!hm[c] || chv[c] >= 0) continue;
-            const int code = ~chv[c];
-            const int first = code >> 4, cnt = code & 15;
-            for (int k = 0; k < cnt; ++k) {
-                const int ti = first + k;
-                const float4* tp = sc.tris + size_t(ti) * 3;
-                const float4 a = tp[0], b = tp[1], c2 = tp[2];
-                if (COUNT) ++*ntri;
-                float t, u1, u2;
-                if (!wt_test(wr, tmin, best_t, xyz(a), xyz(b), xyz(c2), t, u1, u2)) continue;
+        for (int c = 0; c < 4; ++c) leaves |= (hm[c] && chv[c] < 0) ? (1u << c) : 0u;
+        int ti = 0, tend = 0;
+        while (true) {
+            while (ti == tend && leaves != 0) {
+                const int c = __builtin_ctz(leaves);
+                leaves &= leaves - 1;
+                const int code = ~(c == 0 ? chv[0] : c == 1 ? chv[1] : c == 2 ? chv[2] : chv[3]);
+                ti = code >> 4;
+                tend = ti + (code & 15);
+            }
+            if (ti == tend) break;
+            const float4* tp = sc.tris + size_t(ti) * 3;
+            const float4 a = tp[0], b = tp[1], c2 = tp[2];
+            if (COUNT) ++*ntri;
+            float t, u1, u2;
+            if (wt_test(wr, tmin, best_t, xyz(a), xyz(b), xyz(c2), t, u1, u2)) {
                 const int prim = __float_as_int(a.w);
                 if (ANY) {
                     best_t = t; best_tri = ti; best_prim = prim; b1 = u1; b2 = u2;
@@ -195,26 +204,33 @@ struct TraceRay {
                     best_t = t; best_prim = prim; best_tri = ti; b1 = u1; b2 = u2;
                 }
             }
+            ++ti;
         }
-        // inner children, nearest first
-        int ids[4];
+        // Inner children still in front of the closest hit, farthest first
+        // (sorting network on (t, id); misses sort last with t = -inf), so
+        // the nearest is visited next and the others are pushed.
         float ts[4];
+        int ids[4];
         int m = 0;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            if (hm[c] && chv[c] >= 0 && tn[c] <= best_t) { ids[m] = chv[c]; ts[m] = tn[c]; ++m; }
+            const bool in = hm[c] && chv[c] >= 0 && tn[c] <= best_t;
+            ts[c] = in ? tn[c] : -INFINITY;
+            ids[c] = chv[c];
+            m += in ? 1 : 0;
         }
         if (m == 0) { have = false; return true; }
-        // insertion sort, descending (far first) so the nearest ends last
-        for (int i = 1; i < m; ++i) {
-            const int ki = ids[i];
-            const float kt = ts[i];
-            int j = i - 1;
-            while (j >= 0 && ts[j] < kt) { ids[j + 1] = ids[j]; ts[j + 1] = ts[j]; --j; }
-            ids[j + 1] = ki;
-            ts[j + 1] = kt;
-        }
-        for (int i = 0; i < m - 1; ++i) {
+        auto cx = [&](int i, int j) {
+            const bool sw = ts[i] < ts[j];
+            const float ti_ = ts[i], tj_ = ts[j];
+            const int ii = ids[i], ij = ids[j];
+            ts[i] = sw ? tj_ : ti_; ts[j] = sw ? ti_ : tj_;
+            ids[i] = sw ? ij : ii; ids[j] = sw ? ii : ij;
+        };
+        cx(0, 1); cx(2, 3); cx(0, 2); cx(1, 3); cx(1, 2);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            if (i >= m - 1) break;
             if (sp < LDS_STACK) { st.node[sp][tid] = ids[i]; st.tn[sp][tid] = ts[i]; }
             else if (sp < sc.max_stack) {
                 ovf_node[(sp - LDS_STACK) * ovf_stride] = ids[i];
@@ -222,6 +238,7 @@ struct TraceRay {
             }
             ++sp;
         }
+        cur = m == 1 ? ids[0] : m == 2 ? ids[1] : m == 3 ? ids[2] : ids[3];
         if (sp > sc.max_stack) sp = sc.max_stack;   // unreachable by construction (3 * depth)
         cur = ids[m - 1];
         have = true;
