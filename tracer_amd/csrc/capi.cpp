@@ -387,7 +387,7 @@ tr_status tr_create(tr_ctx** out, int32_t device) {
     c->device = device;
     tr_status s = guarded(c, [&] {
         hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
-        c->counters.reserve(64);
+        c->counters.reserve(size_t(kNumCounters) * kCounterStride * sizeof(unsigned int));
         c->stats.reserve(64);
     });
     if (s != TR_OK) {

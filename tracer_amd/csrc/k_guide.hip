@@ -182,7 +182,7 @@ __global__ __launch_bounds__(256) void k_guide_infer(GuideNet net, DevScene s, D
         __syncthreads();
     }
     const WeightSrc<WLDS> src = forward_src<WLDS>(net, H, lds_w);
-    const unsigned int n = p.counters[cur];
+    const unsigned int n = *queue_counter(p, cur);
     const int* q = p.queue[cur];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4;
     const unsigned int per_block = 4 * 16 * NT;

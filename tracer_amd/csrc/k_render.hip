@@ -41,9 +41,9 @@ TR_D void store_rng(const DevPaths& p, int slot, const Pcg& r) {
 __global__ __launch_bounds__(256) void k_generate(DevScene s, DevPaths p, RenderConsts c, int num_paths, int cur) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) {
-        p.counters[cur] = (unsigned int)num_paths;
-        p.counters[1 - cur] = 0u;
-        p.counters[2] = 0u;
+        *queue_counter(p, cur) = (unsigned int)num_paths;
+        *queue_counter(p, 1 - cur) = 0u;
+        *queue_counter(p, kShadowCounter) = 0u;
     }
     if (i >= num_paths) return;
     const int npix = c.width * c.height;
@@ -122,11 +122,14 @@ TR_D bool load_mixture(const DevPaths& p, int slot, Mixture& mix) {
 // the same random numbers as the plain kernel.
 template <bool GUIDE>
 __global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene s, DevPaths p, RenderConsts c, int cur) {
-    const unsigned int n = p.counters[cur];
+    __shared__ BlockAppendLds<kShadeBlock> app;
+    const unsigned int n = *queue_counter(p, cur);
     const int* q = p.queue[cur];
     int* qn = p.queue[1 - cur];
     const unsigned int stride = gridDim.x * blockDim.x;
-    // Loop bound is uniform across the wavefront so wave_append stays convergent.
+    unsigned int n_guided = 0, n_fallback = 0;
+    // The loop bound is uniform across the block: every thread reaches the
+    // block-wide appends.
     const unsigned int base0 = blockIdx.x * blockDim.x;
     for (unsigned int base = base0; base < n; base += stride) {
         const unsigned int i = base + threadIdx.x;
@@ -397,14 +400,13 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene s, DevPaths p, R
                 store_rng(p, slot, rng);
             }
         }
-        if (GUIDE) {
-            const unsigned long long mu = __ballot(g_used), mf = __ballot(g_fallback);
-            if (__lane_id() == 0) {
-                if (mu) atomicAdd(&p.stats[6], (unsigned long long)__popcll(mu));
-                if (mf) atomicAdd(&p.stats[5], (unsigned long long)__popcll(mf));
-            }
-            if (c.guide_record) {
-                const unsigned int ri = wave_append(p.rec_count, want_rec);
+        n_guided += g_used ? 1u : 0u;
+        n_fallback += g_fallback ? 1u : 0u;
+        int nee_ri = -1;
+        if (GUIDE && c.guide_record) {
+            unsigned int ri, nri;
+            block_append2(app, p.rec_count, want_rec, p.rec_count, want_nee_rec, ri, nri);
+            {
                 const bool stored = want_rec && ri < (unsigned int)p.rec_cap;
                 if (stored) {
                     p.rec[4 * ri] = rec0;
@@ -416,28 +418,35 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene s, DevPaths p, R
                 }
                 if (cont) p.grec[slot] = stored ? int(ri) : -1;
             }
-        }
-        int nee_ri = -1;
-        if (GUIDE && c.guide_record) {
-            const unsigned int ri = wave_append(p.rec_count, want_nee_rec);
-            if (want_nee_rec && ri < (unsigned int)p.rec_cap) {
-                nee_ri = int(ri);
-                p.rec[4 * ri] = nrec0;
-                p.rec[4 * ri + 1] = nrec1;
-                p.rec[4 * ri + 2] = nrec2;
-                p.rec[4 * ri + 3] = nrec3;
-                p.rec_aux[2 * ri] = make_float4(nee_weight, 0.0f, 0.0f, 0.0f);
-                p.rec_aux[2 * ri + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (want_nee_rec && nri < (unsigned int)p.rec_cap) {
+                nee_ri = int(nri);
+                p.rec[4 * nri] = nrec0;
+                p.rec[4 * nri + 1] = nrec1;
+                p.rec[4 * nri + 2] = nrec2;
+                p.rec[4 * nri + 3] = nrec3;
+                p.rec_aux[2 * nri] = make_float4(nee_weight, 0.0f, 0.0f, 0.0f);
+                p.rec_aux[2 * nri + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             }
         }
-        const unsigned int qi = wave_append(&p.counters[1 - cur], cont);
+        unsigned int qi, si;
+        block_append2(app, queue_counter(p, 1 - cur), cont, queue_counter(p, kShadowCounter), sh.emit, qi, si);
         if (cont) qn[qi] = slot;
-        const unsigned int si = wave_append(&p.counters[2], sh.emit);
         if (sh.emit) {
             p.sh_o[si] = f4(sh.o, sh.tmin);
             p.sh_d[si] = f4(sh.d, sh.tmax);
             p.sh_c[si] = f4(sh.contrib, __int_as_float(slot));
             if (GUIDE && c.guide_record) p.sh_rec[si] = nee_ri;
+        }
+    }
+    if (GUIDE) {
+        // guided and fallback vertex counts: one atomic per wave and launch
+        for (int off = 32; off > 0; off >>= 1) {
+            n_guided += __shfl_xor(n_guided, off);
+            n_fallback += __shfl_xor(n_fallback, off);
+        }
+        if (__lane_id() == 0) {
+            if (n_guided) atomicAdd(&p.stats[6], (unsigned long long)n_guided);
+            if (n_fallback) atomicAdd(&p.stats[5], (unsigned long long)n_fallback);
         }
     }
 }

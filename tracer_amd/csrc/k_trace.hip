@@ -97,10 +97,10 @@ __global__ __launch_bounds__(kTraceBlock) void k_extend(TraceScene sc, DevPaths 
     const int tid = threadIdx.x;
     const int gt = blockIdx.x * kTraceBlock + tid;
     const int stride = gridDim.x * kTraceBlock;
-    const unsigned int n = p.counters[cur];
+    const unsigned int n = *queue_counter(p, cur);
     if (gt == 0) {
-        p.counters[1 - cur] = 0u;   // next queue and shadow queue are filled by the shade kernel
-        p.counters[2] = 0u;
+        *queue_counter(p, 1 - cur) = 0u;   // next queue and shadow queue are filled by the shade kernel
+        *queue_counter(p, kShadowCounter) = 0u;
         atomicAdd(&p.stats[0], (unsigned long long)n);
     }
     const int* q = p.queue[cur];
@@ -127,7 +127,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(TraceScene sc, DevPaths 
     const int tid = threadIdx.x;
     const int gt = blockIdx.x * kTraceBlock + tid;
     const int stride = gridDim.x * kTraceBlock;
-    const unsigned int n = p.counters[2];
+    const unsigned int n = *queue_counter(p, kShadowCounter);
     if (gt == 0) atomicAdd(&p.stats[1], (unsigned long long)n);
     for (unsigned int i = gt; i < n; i += stride) {
         const float4 a = p.sh_o[i], b = p.sh_d[i];

@@ -37,7 +37,7 @@ struct DevPaths {
     float4* sh_o;     // shadow rays: (origin, tmin)
     float4* sh_d;     //              (direction, tmax)
     float4* sh_c;     //              (contribution rgb, path slot)
-    unsigned int* counters;   // [0..1] queue lengths, [2] shadow length
+    unsigned int* counters;   // queue lengths 0, 1 and the shadow queue length 2, kCounterStride apart
     unsigned long long* stats;  // [0] extension rays, [1] shadow rays, [2] non-finite samples,
                                 // [3] node visits, [4] triangle tests
     int* ovf_node;    // traversal stack spill area
@@ -56,6 +56,51 @@ struct DevPaths {
     int rec_cap;
     int* sh_rec;              // light-sample record of each shadow ray, -1 = none
 };
+
+// The queue counters sit 1 KB apart: appends to different queues then go to
+// different cache lines and do not serialise behind each other.
+constexpr int kCounterStride = 256;
+constexpr int kNumCounters = 3;
+constexpr int kShadowCounter = 2;
+TR_D unsigned int* queue_counter(const DevPaths& p, int k) { return p.counters + k * kCounterStride; }
+
+// Block-aggregated append to two counters: one atomic per counter and block
+// (the counters are device-wide and every atomic on them serialises, so a
+// wave-level append made the shade kernel wait on them).  Every thread of the
+// block must call it: it synchronises the block.  Slots are consecutive in
+// thread order.  c0 and c1 may be the same counter.
+template <int BLOCK>
+struct BlockAppendLds {
+    unsigned int cnt[2][BLOCK / 64];
+    unsigned int base[2];
+};
+
+template <int BLOCK>
+TR_D void block_append2(BlockAppendLds<BLOCK>& sh, unsigned int* c0, bool p0, unsigned int* c1, bool p1,
+                        unsigned int& i0, unsigned int& i1) {
+    const int lane = __lane_id(), wave = int(threadIdx.x) >> 6;
+    const unsigned long long m0 = __ballot(p0), m1 = __ballot(p1);
+    if (lane == 0) {
+        sh.cnt[0][wave] = (unsigned int)__popcll(m0);
+        sh.cnt[1][wave] = (unsigned int)__popcll(m1);
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        unsigned int tot = 0;
+        for (int w = 0; w < BLOCK / 64; ++w) tot += sh.cnt[threadIdx.x][w];
+        sh.base[threadIdx.x] = tot ? atomicAdd(threadIdx.x == 0 ? c0 : c1, tot) : 0u;
+    }
+    __syncthreads();
+    unsigned int b0 = sh.base[0], b1 = sh.base[1];
+    for (int w = 0; w < wave; ++w) {
+        b0 += sh.cnt[0][w];
+        b1 += sh.cnt[1][w];
+    }
+    const unsigned long long below = lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
+    i0 = b0 + (unsigned int)__popcll(m0 & below);
+    i1 = b1 + (unsigned int)__popcll(m1 & below);
+    __syncthreads();   // sh is reused by the next call
+}
 
 struct RenderConsts {
     int width, height;
