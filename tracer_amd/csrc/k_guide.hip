@@ -186,6 +186,7 @@ __global__ __launch_bounds__(256) void k_guide_infer(GuideNet net, DevScene s, D
     const int* q = p.queue[cur];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4;
     const unsigned int per_block = 4 * 16 * NT;
+    unsigned long long evals = 0;   // network evaluations of this wave, added to stats[7] once
     for (unsigned int bbase = blockIdx.x * per_block; bbase < n; bbase += gridDim.x * per_block) {
         keep_loads_in_loop();
         const unsigned int base = bbase + wave * 16 * NT;
@@ -215,13 +216,9 @@ __global__ __launch_bounds__(256) void k_guide_infer(GuideNet net, DevScene s, D
         bf16x8 hh[4][H / 32][NT];
         f32x4 out[3][NT];
         forward_all<H, NT>(net.box, src, x, nn, wo, xin, hh, out, lane);
-        {
-            // evaluations of this wave's tile, counted once: lanes 0..15 hold its sample columns
-            int tot = 0;
+        // evaluations of this wave's tile, counted once: lanes 0..15 hold its sample columns
 #pragma unroll
-            for (int nt = 0; nt < NT; ++nt) tot += __popcll(__ballot(lane < 16 && slot[nt] >= 0));
-            if (lane == 0 && tot > 0) atomicAdd(&p.stats[7], (unsigned long long)tot);
-        }
+        for (int nt = 0; nt < NT; ++nt) evals += __popcll(__ballot(lane < 16 && slot[nt] >= 0));
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) {
             if (slot[nt] < 0) continue;
@@ -232,6 +229,7 @@ __global__ __launch_bounds__(256) void k_guide_infer(GuideNet net, DevScene s, D
                     make_float4(out[mt][nt][0], out[mt][nt][1], out[mt][nt][2], out[mt][nt][3]);
         }
     }
+    if (lane == 0 && evals > 0) atomicAdd(&p.stats[7], evals);
 }
 
 // ---- inference on explicit points (x, n, wo: 9 floats each) -------------------
@@ -557,8 +555,12 @@ __global__ void k_guide_pack(GuideDims d, const float* master, unsigned short* f
 // of unity of the MIS weights the two kinds together estimate the gradient of
 // the full product target.  Records with a non-positive or non-finite weight
 // carry no gradient and are dropped.
-__global__ __launch_bounds__(256) void k_guide_finalize(DevPaths p, int num_records, float4* train,
-                                                        unsigned int* train_count, float* wsum) {
+constexpr int kFinalizeBlock = 1024;
+
+__global__ __launch_bounds__(kFinalizeBlock) void k_guide_finalize(DevPaths p, int num_records, float4* train,
+                                                                   unsigned int* train_count, float* wsum) {
+    __shared__ BlockAppendLds<kFinalizeBlock> app;
+    __shared__ float wpart[kFinalizeBlock / 64];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     bool keep = false;
     float w = 0.0f;
@@ -582,7 +584,9 @@ __global__ __launch_bounds__(256) void k_guide_finalize(DevPaths p, int num_reco
             }
         }
     }
-    const unsigned int ti = wave_append(train_count, keep);
+    // one atomic per block for the sample count and one for the weight sum
+    unsigned int ti, unused;
+    block_append2(app, train_count, keep, train_count, false, ti, unused);
     if (keep) {
         train[4 * ti] = r0;
         train[4 * ti + 1] = r1;
@@ -592,7 +596,13 @@ __global__ __launch_bounds__(256) void k_guide_finalize(DevPaths p, int num_reco
     float ws = keep ? w : 0.0f;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) ws += __shfl_down(ws, off);
-    if ((threadIdx.x & 63) == 0 && ws != 0.0f) atomicAdd(wsum, ws);
+    if ((threadIdx.x & 63) == 0) wpart[threadIdx.x >> 6] = ws;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float t = 0.0f;
+        for (int w = 0; w < kFinalizeBlock / 64; ++w) t += wpart[w];
+        if (t != 0.0f) atomicAdd(wsum, t);
+    }
 }
 
 // MFMA lane-layout check: D = A B for one 16 x 32 by 32 x 16 product, A and B
@@ -689,7 +699,7 @@ void launch_guide_pack(const GuideDims& d, const float* master, unsigned short* 
 void launch_guide_finalize(const DevPaths& p, int num_records, float4* train, unsigned int* train_count, float* wsum,
                            hipStream_t st) {
     if (num_records <= 0) return;
-    hipLaunchKernelGGL(k_guide_finalize, dim3((num_records + 255) / 256), dim3(256), 0, st, p, num_records, train,
+    hipLaunchKernelGGL(k_guide_finalize, dim3((num_records + kFinalizeBlock - 1) / kFinalizeBlock), dim3(kFinalizeBlock), 0, st, p, num_records, train,
                        train_count, wsum);
 }
 void launch_guide_mixture(const float* raw, const float* dirs, const float* u, int n, float* samples,

@@ -10,20 +10,6 @@ constexpr int kTraceBlock = 256;
 constexpr int kLdsStack = 8;
 constexpr int kShadeBlock = 256;
 
-// Wave-aggregated append: one atomic per wavefront, lanes get consecutive
-// slots in lane order.
-TR_D unsigned int wave_append(unsigned int* counter, bool pred) {
-    const unsigned long long mask = __ballot(pred);
-    if (mask == 0ull) return 0u;
-    const int lane = __lane_id();
-    const int leader = __ffsll((long long)mask) - 1;
-    unsigned int base = 0u;
-    if (lane == leader) base = atomicAdd(counter, (unsigned int)__popcll(mask));
-    base = __shfl(base, leader);
-    const unsigned long long below = lane == 0 ? 0ull : (mask & ((~0ull) >> (64 - lane)));
-    return base + (unsigned int)__popcll(below);
-}
-
 // Path state of the wavefront (one slot per camera sample in flight).
 struct DevPaths {
     float4* ray_o;    // (origin, tmin)
