@@ -62,10 +62,21 @@ DevMaterial make_material(const tr_material& m) {
         d.metallic = 0.0f;
         d.transmission = 0.0f;
     }
-    // Guided sampling only where the BSDF lobe is broad: the guide network does
-    // not see material parameters, so at specular-transmissive or glossy metal
-    // vertices its samples would mostly land where f vanishes.
-    d.guide_ok = d.type == MAT_LAMBERT || (m.transmission <= 0.0f && (m.metallic < 0.5f || d.ax >= 0.25f));
+    // Guided sampling only where the BSDF is broad: the guide network does not
+    // see material parameters, so at transmissive or glossy metal vertices its
+    // samples would mostly land where f vanishes.  Elsewhere the guide takes
+    // the broad part of the BSDF's lobe-selection weight (diffuse, and
+    // specular once alpha_x >= 0.25) and leaves the sharp lobes (smooth
+    // specular, clearcoat) to the BSDF sampler: the guide fraction at a vertex
+    // is guide_fraction * guide_share.
+    const bool guided = d.type == MAT_LAMBERT || (m.transmission <= 0.0f && (m.metallic < 0.5f || d.ax >= 0.25f));
+    float share = 1.0f;
+    if (d.type != MAT_LAMBERT) {
+        const float tot = d.w_diffuse + d.w_trans + d.w_spec + d.w_cc;
+        const float broad = d.w_diffuse + (d.ax >= 0.25f ? d.w_spec : 0.0f);
+        share = tot > 0.0f ? broad / tot : 0.0f;
+    }
+    d.guide_share = guided ? share : 0.0f;
     return d;
 }
 

@@ -116,8 +116,9 @@ TR_D bool load_mixture(const DevPaths& p, int slot, Mixture& mix) {
 }
 
 // GUIDE: surface vertices sample the continuation direction by one-sample MIS
-// between the guide mixture (probability c.guide_frac) and the BSDF, and
-// weight it, light samples and emitter hits with the combined pdf.  A vertex
+// between the guide mixture (probability c.guide_frac times the material's
+// guide_share) and the BSDF, and weight it, light samples and emitter hits
+// with the combined pdf.  A vertex
 // whose head outputs are not finite uses the plain BSDF estimator, drawing
 // the same random numbers as the plain kernel.
 template <bool GUIDE>
@@ -262,13 +263,13 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene s, DevPaths p, R
                         const f3 lwo = to_local(wo), lng = to_local(ng);
                         Mixture mix;
                         bool use_guide = false;
-                        const bool guide_here = GUIDE && mat.guide_ok;
+                        const bool guide_here = GUIDE && mat.guide_share > 0.0f;
                         if (guide_here && c.guide_frac > 0.0f && (c.guide_max_depth <= 0 || depth < c.guide_max_depth)) {
                             use_guide = load_mixture(p, slot, mix);
                             g_fallback = !use_guide;
                             g_used = use_guide;
                         }
-                        const float frac = c.guide_frac;
+                        const float frac = c.guide_frac * mat.guide_share;
                         const bool rec_path = guide_here && c.guide_record &&
                                               record_hash(uint32_t(slot), c.rec_seed) < c.rec_threshold;
                         const float su = rng.uniform(), u1 = rng.uniform(), u2 = rng.uniform();

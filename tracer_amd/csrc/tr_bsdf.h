@@ -38,7 +38,7 @@ struct DevMaterial {
     float spec_tint, clearcoat, ax, ay;
     float tax, tay, cc_rough, w_diffuse;
     float w_trans, w_spec, w_cc;
-    int guide_ok;   // broad enough for the guide mixture (set by the host; see make_material)
+    float guide_share;   // guide mixture probability at this material, as a share of the guide fraction (host, make_material)
 };
 
 struct BsdfSample {
