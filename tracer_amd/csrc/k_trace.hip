@@ -83,13 +83,46 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_counted(TraceScene sc, co
     }
 }
 
-// Wavefront extension: closest hit for every path slot in the current queue,
-// one ray per lane at a time in queue order: a wave's lanes traverse rays of
-// neighbouring queue entries (nearby pixels) together.  Letting each lane run
-// ahead through its own sequence of rays measured slower on the MI355X
-// (0.32 vs 0.30 ms per launch on configuration 2 at 2^21 paths per pass), and
-// fetching rays from a global work counter much slower (0.49 ms: the atomics
-// contend).
+// Work distribution of the wavefront trace kernels.  Each wave owns a
+// contiguous share of the queue (neighbouring entries: nearby pixels) and its
+// lanes traverse their own rays.  When fewer than kRefillLanes of the 64 lanes
+// are still traversing, the idle lanes take the next entries of the share.
+// Measured on configuration 2 (per frame): waiting for every lane (one ray
+// per lane at a time) 25.6 ms of k_extend and 8.9 ms of k_shadow; refilling
+// below 48 busy lanes 22.2 and 7.7 ms; below 32, 22.1 and 8.0 ms; refilling
+// any idle lane at once, 24.6 and 7.8 ms (the wave's rays drift apart).
+// Fetching single rays from a global work counter was much slower (the
+// atomics contend).
+constexpr int kRefillLanes = 48;
+
+struct WaveQueue {
+    unsigned int next, end;   // wave-uniform
+
+    TR_D explicit WaveQueue(unsigned int n) {
+        const unsigned int waves = gridDim.x * (kTraceBlock / 64);
+        const unsigned int w = blockIdx.x * (kTraceBlock / 64) + (threadIdx.x >> 6);
+        const unsigned int per = (n + waves - 1) / waves;
+        next = __builtin_amdgcn_readfirstlane(min(n, w * per));
+        end = __builtin_amdgcn_readfirstlane(min(n, next + per));
+    }
+
+    // Called by every lane of the wave: idle lanes that receive a queue entry
+    // call fetch(entry) and become busy.
+    template <typename F>
+    TR_D void refill(bool& busy, F&& fetch) {
+        if (__popcll(__ballot(busy)) < kRefillLanes && next < end) {
+            const unsigned long long idle = __ballot(!busy);
+            const unsigned int idx = next + (unsigned int)__popcll(idle & ((1ull << __lane_id()) - 1ull));
+            if (!busy && idx < end) {
+                fetch(idx);
+                busy = true;
+            }
+            next = __builtin_amdgcn_readfirstlane(min(end, next + (unsigned int)__popcll(idle)));
+        }
+    }
+};
+
+// Wavefront extension: closest hit for every path slot in the current queue.
 // COUNT: also accumulate node visits and triangle tests into stats[3..4].
 template <bool COUNT>
 __global__ __launch_bounds__(kTraceBlock) void k_extend(TraceScene sc, DevPaths p, int cur) {
@@ -105,13 +138,24 @@ __global__ __launch_bounds__(kTraceBlock) void k_extend(TraceScene sc, DevPaths 
     }
     const int* q = p.queue[cur];
     uint32_t cv = 0, ct = 0;
-    for (unsigned int i = gt; i < n; i += stride) {
-        const int slot = q[i];
-        const float4 a = p.ray_o[slot], b = p.ray_d[slot];
-        TraceHit h;
-        trace<false, COUNT, kTraceBlock, kLdsStack>(sc, xyz(a), a.w, xyz(b), b.w, st, tid, p.ovf_node + gt,
-                                                    p.ovf_t + gt, stride, h, &cv, &ct);
-        p.hit[slot] = make_float4(h.t, __int_as_float(h.tri), h.b1, h.b2);
+    WaveQueue wq(n);
+    TraceRay r;
+    bool busy = false;
+    int slot = 0;
+    while (true) {
+        wq.refill(busy, [&](unsigned int idx) {
+            slot = q[idx];
+            const float4 a = p.ray_o[slot], b = p.ray_d[slot];
+            r.init(xyz(a), a.w, xyz(b), b.w);
+        });
+        if (__ballot(busy) == 0ull) break;
+        if (busy && !r.step<false, COUNT, kTraceBlock, kLdsStack>(sc, st, tid, p.ovf_node + gt, p.ovf_t + gt, stride,
+                                                                  &cv, &ct)) {
+            TraceHit h;
+            r.result(h);
+            p.hit[slot] = make_float4(h.t, __int_as_float(h.tri), h.b1, h.b2);
+            busy = false;
+        }
     }
     if (COUNT) {
         atomicAdd(&p.stats[3], (unsigned long long)cv);
@@ -129,12 +173,23 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(TraceScene sc, DevPaths 
     const int stride = gridDim.x * kTraceBlock;
     const unsigned int n = *queue_counter(p, kShadowCounter);
     if (gt == 0) atomicAdd(&p.stats[1], (unsigned long long)n);
-    for (unsigned int i = gt; i < n; i += stride) {
-        const float4 a = p.sh_o[i], b = p.sh_d[i];
+    WaveQueue wq(n);
+    TraceRay r;
+    bool busy = false;
+    unsigned int i = 0;
+    while (true) {
+        wq.refill(busy, [&](unsigned int idx) {
+            i = idx;
+            const float4 a = p.sh_o[i], b = p.sh_d[i];
+            r.init(xyz(a), a.w, xyz(b), b.w);
+        });
+        if (__ballot(busy) == 0ull) break;
+        if (!busy || r.step<true, false, kTraceBlock, kLdsStack>(sc, st, tid, p.ovf_node + gt, p.ovf_t + gt, stride,
+                                                                 nullptr, nullptr))
+            continue;
+        busy = false;
         TraceHit h;
-        const bool occ = trace<true, false, kTraceBlock, kLdsStack>(sc, xyz(a), a.w, xyz(b), b.w, st, tid,
-                                                                    p.ovf_node + gt, p.ovf_t + gt, stride, h);
-        if (!occ) {
+        if (!r.result(h)) {
             if (record) {
                 const int ri = p.sh_rec[i];
                 if (ri >= 0) p.rec_aux[2 * ri + 1] = make_float4(0.0f, 0.0f, 0.0f, 2.0f);
@@ -146,9 +201,9 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(TraceScene sc, DevPaths 
                 atomicAdd(&p.rad[slot].y, c.y);
                 atomicAdd(&p.rad[slot].z, c.z);
             } else {
-                float4 r = p.rad[slot];
-                r.x += c.x; r.y += c.y; r.z += c.z;
-                p.rad[slot] = r;
+                float4 rad = p.rad[slot];
+                rad.x += c.x; rad.y += c.y; rad.z += c.z;
+                p.rad[slot] = rad;
             }
         }
     }
