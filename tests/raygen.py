@@ -50,6 +50,45 @@ def interior_rays(scene, n, seed, margin=1e-3):
     return r, t
 
 
+def edge_case_rays(scene, n, seed):
+    """Rays where a conservative box test matters: aimed exactly at triangle
+    vertices and edge midpoints (shared by neighbouring triangles), axis-aligned
+    directions (zero direction components), origins 1000 scene units away and
+    origins on a vertex.  n rays, in five equal groups."""
+    rng = np.random.default_rng(seed)
+    P = scene.positions.reshape(-1, 3).astype(np.float64)
+    I = scene.indices.reshape(-1, 3)
+    lo, hi = scene_bounds(scene)
+    k = n // 5
+    o = rng.uniform(lo, hi, (n, 3))
+    tri = rng.integers(0, I.shape[0], n)
+    corner = rng.integers(0, 3, n)
+    vert = P[I[tri, corner]]
+    mid = 0.5 * (P[I[tri, corner]] + P[I[tri, (corner + 1) % 3]])
+    w = rng.dirichlet((1, 1, 1), n)
+    interior = P[I[tri, 0]] * w[:, :1] + P[I[tri, 1]] * w[:, 1:2] + P[I[tri, 2]] * w[:, 2:3]
+    d = np.empty((n, 3))
+    d[:k] = vert[:k] - o[:k]
+    d[k:2 * k] = mid[k:2 * k] - o[k:2 * k]
+    a0 = rng.integers(0, 3, k)
+    a1 = (a0 + rng.integers(1, 3, k)) % 3   # a different axis
+    axes = np.eye(3)[a0] * rng.choice([-1.0, 1.0], (k, 1))
+    two = rng.random(k) < 0.5   # half of them with two non-zero components
+    axes = np.where(two[:, None], axes + np.eye(3)[a1] * rng.choice([-1.0, 1.0], (k, 1)), axes)
+    d[2 * k:3 * k] = axes
+    far = rng.normal(size=(k, 3))
+    far /= np.linalg.norm(far, axis=1, keepdims=True)
+    o[3 * k:4 * k] = 0.5 * (lo + hi) + 1000.0 * far
+    d[3 * k:4 * k] = interior[3 * k:4 * k] - o[3 * k:4 * k]
+    o[4 * k:] = vert[4 * k:]
+    d[4 * k:] = rng.normal(size=(n - 4 * k, 3))
+    r = np.zeros((n, 8), np.float32)
+    r[:, 0:3] = o
+    r[:, 4:7] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    r[:, 7] = BIG
+    return r
+
+
 def hit_prims(h):
     return h[:, 1].copy().view(np.int32)
 

@@ -270,6 +270,26 @@ def test_nan_guide_falls_back_to_plain(gpu_ctx):
     assert st.extension_rays == sp.extension_rays
 
 
+def test_training_is_reproducible(gpu_ctx):
+    """Two trainings with the same seed give the same samples and weights: the
+    samples are sorted by (path slot, depth, kind) and their weight sum is a
+    fixed-order reduction, so neither depends on the order in which
+    workgroups appended them."""
+    from tracer_amd import native
+    s = native.config_scene(4, detail=1 / 16)
+    gpu_ctx.load_scene(s)
+    runs = []
+    for _ in range(2):
+        gpu_ctx.guide_init(hidden=64, seed=3, batch=1 << 14, steps_per_pass=8)
+        gpu_ctx.render(s.render_params(width=32, height=32, spp=32, seed=4, guide=1, guide_train_passes=4,
+                                       samples_per_pass=8), want_image=False)
+        runs.append((gpu_ctx.guide_last_samples(), gpu_ctx.guide_params()))
+    (sa, pa), (sb, pb) = runs
+    assert len(sa) > 0
+    assert np.array_equal(sa, sb)
+    assert np.array_equal(pa, pb)
+
+
 def test_trained_guide_lowers_error_on_occluded_light(gpu_ctx):
     """Configuration 4 (the only light shines through a small aperture) at a
     reduced film: after training, the guided estimator's relMSE against a

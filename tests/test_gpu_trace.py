@@ -7,7 +7,7 @@ rays aimed at triangle interiors."""
 import numpy as np
 import pytest
 
-from tests.raygen import compare_with_mt, hit_prims, interior_rays, random_rays
+from tests.raygen import compare_with_mt, edge_case_rays, hit_prims, interior_rays, random_rays
 
 pytestmark = pytest.mark.gpu
 
@@ -53,6 +53,20 @@ def test_c1_any_hit(gpu_ctx, c1):
     rays = random_rays(s, 1 << 15, seed=3)
     rays[:, 7] = np.random.default_rng(4).uniform(0.01, 3.0, rays.shape[0])
     assert np.array_equal(gpu_ctx.trace_any(rays), ref.any_wt(rays))
+
+
+def test_c1_edge_case_rays_bit_exact(gpu_ctx, c1):
+    """Rays at shared vertices and edges, axis-aligned rays, distant origins and
+    origins on the surface: the widened box test must cull no box holding the
+    closest hit, so hits equal the brute-force loop exactly."""
+    s, ref = c1
+    gpu_ctx.load_scene(s)
+    rays = edge_case_rays(s, 5 * 4096, seed=7)
+    assert np.any(rays[:, 4:7] == 0.0), "expected axis-aligned rays"
+    _check_exact(gpu_ctx.trace_closest(rays), ref.closest_brute(rays))
+    short = rays.copy()
+    short[:, 7] = np.random.default_rng(8).uniform(0.01, 3.0, rays.shape[0])
+    assert np.array_equal(gpu_ctx.trace_any(short), ref.any_wt(short))
 
 
 @pytest.mark.parametrize("detail", [0.125])

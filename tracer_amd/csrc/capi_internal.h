@@ -79,13 +79,17 @@ struct GuideState {
     DevBuf raw;                 // head outputs per path slot
     DevBuf grec, rec, rec_aux;  // vertex records of training passes
     DevBuf sh_rec;              // light-sample record of each shadow ray
-    DevBuf train;               // weighted training samples
+    DevBuf train;               // weighted training samples, in key order
+    DevBuf train_raw;           // the same as the finalize kernel appended them
+    DevBuf keys, vals;          // sort keys and sample indices, two halves each (in, sorted)
+    DevBuf sort_tmp, wpart;     // radix-sort scratch, per-workgroup weight sums
+    size_t sort_tmp_bytes = 0;
     DevBuf acts, deltas, pw, pb;
     DevBuf misc;                // [0] record count, [1] sample count (uint32), [2] weight sum, [3] loss (float)
     DevBuf grad;
     int64_t path_capacity = 0;
     int64_t record_capacity = 0;
-    float avg_records_per_path = 4.0f;
+    float avg_records_per_path = 0.0f;   // records per recording path; 0 = not measured yet
     double last_recording_paths = 0.0;
     int64_t last_samples = 0;   // weighted samples of the latest training pass (in `train`)
     trhip::GuideDims dims() const { return trhip::GuideDims{cfg.hidden}; }
@@ -166,7 +170,7 @@ void guide_infer(tr_ctx* c, const trhip::DevPaths& p, int cur, int num_paths);
 // Fields of DevPaths for guided rendering.
 void guide_fill_paths(tr_ctx* c, trhip::DevPaths& p, bool training);
 // Record threshold for a training pass of num_paths paths.
-unsigned int guide_record_threshold(tr_ctx* c, int64_t num_paths);
+unsigned int guide_record_threshold(tr_ctx* c, int64_t num_paths, int max_depth);
 // Clears the record counter before a training pass.
 void guide_begin_pass(tr_ctx* c);
 // Turns the pass's records into weighted samples and takes the configured

@@ -113,7 +113,7 @@ void init_guide(tr_ctx* c, const tr_guide_config& cfg) {
     const std::vector<float> p = initial_params(cfg);
     upload_params(c, p.data());
     hip_check(hipStreamSynchronize(c->stream), "guide init sync");
-    g.avg_records_per_path = 4.0f;
+    g.avg_records_per_path = 0.0f;   // not measured yet
     g.last_samples = 0;
     g.box_set = false;   // back to the scene bounds until tr_guide_set_box
     g.ready = true;
@@ -209,6 +209,13 @@ void guide_prepare(tr_ctx* c, int64_t P, bool training) {
         g.rec.reserve(cap * 64);
         g.rec_aux.reserve(cap * 32);
         g.train.reserve(cap * 64);
+        g.train_raw.reserve(cap * 64);
+        g.keys.reserve(cap * 16);
+        g.vals.reserve(cap * 8);
+        g.wpart.reserve((cap / 256 + 1) * 4);
+        g.sort_tmp_bytes = guide_order_temp_bytes(int(cap));
+        if (g.sort_tmp_bytes == 0) throw HipError("radix sort scratch query failed");
+        g.sort_tmp.reserve(g.sort_tmp_bytes);
         g.record_capacity = g.cfg.record_capacity;
     }
 }
@@ -231,9 +238,15 @@ void guide_infer(tr_ctx* c, const DevPaths& p, int cur, int num_paths) {
     launch_guide_infer(g.net(), g.cfg.hidden, c->ds, p, cur, num_paths, g.raw.as<float>(), c->stream);
 }
 
-unsigned int guide_record_threshold(tr_ctx* c, int64_t num_paths) {
+// Share of the paths that record.  Records that do not fit are dropped in
+// whichever order the workgroups append them, so the pass aims at 90% of the
+// capacity; before the first pass measures the records per path, it assumes
+// the most a path can make (a continuation and a light-sample record per
+// surface vertex).
+unsigned int guide_record_threshold(tr_ctx* c, int64_t num_paths, int max_depth) {
     GuideState& g = c->guide;
-    const double want = double(g.record_capacity) / (double(num_paths) * double(g.avg_records_per_path));
+    const double per_path = g.avg_records_per_path > 0.0f ? double(g.avg_records_per_path) : 2.0 * (max_depth + 1);
+    const double want = 0.9 * double(g.record_capacity) / (double(num_paths) * per_path);
     const unsigned int thr = want >= 1.0 ? 0xFFFFFFFFu : unsigned(want * 4294967296.0);
     g.last_recording_paths = double(num_paths) * (double(thr) / 4294967296.0);
     return thr;
@@ -250,9 +263,18 @@ int64_t guide_train_pass(tr_ctx* c, const DevPaths& p, uint64_t pass_seed, float
     hip_check(hipMemcpyAsync(counts, g.misc.ptr, 4, hipMemcpyDeviceToHost, st), "D2H record count");
     hip_check(hipStreamSynchronize(st), "guide sync");
     const int nrec = int(std::min<int64_t>(counts[0], g.record_capacity));
-    launch_guide_finalize(p, nrec, g.train.as<float4>(), g.misc.as<unsigned int>() + 1, g.misc.as<float>() + 2, st);
+    unsigned long long* keys = g.keys.as<unsigned long long>();
+    unsigned int* vals = g.vals.as<unsigned int>();
+    const size_t cap = size_t(g.record_capacity);
+    launch_guide_finalize(p, nrec, g.train_raw.as<float4>(), keys, vals, g.misc.as<unsigned int>() + 1, st);
     hip_check(hipMemcpyAsync(counts, g.misc.ptr, 8, hipMemcpyDeviceToHost, st), "D2H sample count");
     hip_check(hipStreamSynchronize(st), "guide sync");
+    // samples in key order and their weight sum, independent of the order in
+    // which workgroups appended them: training is reproducible
+    hip_check(launch_guide_order(int(counts[1]), keys, keys + cap, vals, vals + cap, g.sort_tmp.ptr, g.sort_tmp_bytes,
+                                 g.train_raw.as<float4>(), g.train.as<float4>(), g.wpart.as<float>(), g.misc.as<float>() + 2,
+                                 st),
+              "guide sample sort");
     // records per recording path (the counter keeps counting past the
     // capacity), for the next pass's threshold
     if (counts[0] > 0 && g.last_recording_paths > 0.0)

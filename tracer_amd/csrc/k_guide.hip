@@ -3,6 +3,8 @@
 // loss + backward for training, weight / bias gradient reductions, Adam, and
 // the conversion of a training pass's path records into weighted samples.
 // Fragment layouts and the transposed-layer scheme are described in tr_guide.h.
+#include <hipcub/hipcub.hpp>
+
 #include "tr_guide_kernels.h"
 
 namespace trhip {
@@ -554,13 +556,16 @@ __global__ void k_guide_pack(GuideDims d, const float* master, unsigned short* f
 // MIS-weighted f |cos| L / p_light of its light direction.  By the partition
 // of unity of the MIS weights the two kinds together estimate the gradient of
 // the full product target.  Records with a non-positive or non-finite weight
-// carry no gradient and are dropped.
+// carry no gradient and are dropped.  Kept samples are appended in arrival
+// order with a key (path slot, vertex depth, kind) that identifies them; the
+// order kernels below sort them by it, so that training does not depend on
+// the order in which workgroups ran.
 constexpr int kFinalizeBlock = 1024;
 
 __global__ __launch_bounds__(kFinalizeBlock) void k_guide_finalize(DevPaths p, int num_records, float4* train,
-                                                                   unsigned int* train_count, float* wsum) {
+                                                                   unsigned long long* keys, unsigned int* vals,
+                                                                   unsigned int* train_count) {
     __shared__ BlockAppendLds<kFinalizeBlock> app;
-    __shared__ float wpart[kFinalizeBlock / 64];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     bool keep = false;
     float w = 0.0f;
@@ -584,7 +589,7 @@ __global__ __launch_bounds__(kFinalizeBlock) void k_guide_finalize(DevPaths p, i
             }
         }
     }
-    // one atomic per block for the sample count and one for the weight sum
+    // one atomic per block for the sample count
     unsigned int ti, unused;
     block_append2(app, train_count, keep, train_count, false, ti, unused);
     if (keep) {
@@ -592,17 +597,50 @@ __global__ __launch_bounds__(kFinalizeBlock) void k_guide_finalize(DevPaths p, i
         train[4 * ti + 1] = r1;
         train[4 * ti + 2] = r2;
         train[4 * ti + 3] = make_float4(w, 0.0f, r3.z, 0.0f);
+        const unsigned int slot = unsigned(__float_as_int(r3.y)), depth = unsigned(r3.x);
+        keys[ti] = ((unsigned long long)slot << 9) | (depth << 1) | (p.rec_aux[2 * i + 1].w == 2.0f ? 1u : 0u);
+        vals[ti] = ti;
     }
-    float ws = keep ? w : 0.0f;
+}
+
+// Samples in key order, and per-workgroup sums of their weights: one wave
+// shuffle tree and the waves in order, so the sums do not depend on timing.
+constexpr int kGatherBlock = 256;
+
+__global__ __launch_bounds__(kGatherBlock) void k_guide_gather(const float4* raw, const unsigned int* order, int n,
+                                                               float4* train, float* partial) {
+    __shared__ float wpart[kGatherBlock / 64];
+    const int i = blockIdx.x * kGatherBlock + threadIdx.x;
+    float w = 0.0f;
+    if (i < n) {
+        const unsigned int j = order[i];
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) ws += __shfl_down(ws, off);
-    if ((threadIdx.x & 63) == 0) wpart[threadIdx.x >> 6] = ws;
+        for (int k = 0; k < 4; ++k) train[4 * i + k] = raw[4 * size_t(j) + k];
+        w = raw[4 * size_t(j) + 3].x;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) w += __shfl_down(w, off);
+    if ((threadIdx.x & 63) == 0) wpart[threadIdx.x >> 6] = w;
     __syncthreads();
     if (threadIdx.x == 0) {
         float t = 0.0f;
-        for (int w = 0; w < kFinalizeBlock / 64; ++w) t += wpart[w];
-        if (t != 0.0f) atomicAdd(wsum, t);
+        for (int k = 0; k < kGatherBlock / 64; ++k) t += wpart[k];
+        partial[blockIdx.x] = t;
     }
+}
+
+// Sum of the workgroup partials in a fixed order (one workgroup).
+__global__ __launch_bounds__(256) void k_guide_sum(const float* partial, int nb, float* out) {
+    __shared__ float s[256];
+    float t = 0.0f;
+    for (int k = threadIdx.x; k < nb; k += 256) t += partial[k];
+    s[threadIdx.x] = t;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (int(threadIdx.x) < h) s[threadIdx.x] += s[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = s[0];
 }
 
 // MFMA lane-layout check: D = A B for one 16 x 32 by 32 x 16 product, A and B
@@ -696,11 +734,35 @@ void launch_guide_adam(const GuideDims& d, const GuideScratch& s, int n_pad, flo
 void launch_guide_pack(const GuideDims& d, const float* master, unsigned short* fwd, unsigned short* bwd, hipStream_t st) {
     hipLaunchKernelGGL(k_guide_pack, dim3((d.num_weights() + 255) / 256), dim3(256), 0, st, d, master, fwd, bwd);
 }
-void launch_guide_finalize(const DevPaths& p, int num_records, float4* train, unsigned int* train_count, float* wsum,
-                           hipStream_t st) {
+void launch_guide_finalize(const DevPaths& p, int num_records, float4* raw, unsigned long long* keys, unsigned int* vals,
+                           unsigned int* train_count, hipStream_t st) {
     if (num_records <= 0) return;
-    hipLaunchKernelGGL(k_guide_finalize, dim3((num_records + kFinalizeBlock - 1) / kFinalizeBlock), dim3(kFinalizeBlock), 0, st, p, num_records, train,
-                       train_count, wsum);
+    hipLaunchKernelGGL(k_guide_finalize, dim3((num_records + kFinalizeBlock - 1) / kFinalizeBlock), dim3(kFinalizeBlock), 0,
+                       st, p, num_records, raw, keys, vals, train_count);
+}
+// Keys are (slot << 9) | (depth << 1) | kind with slot < 2^30 and depth < 256.
+constexpr int kSampleKeyBits = 39;
+
+size_t guide_order_temp_bytes(int n) {
+    size_t bytes = 0;
+    if (hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, static_cast<const unsigned long long*>(nullptr),
+                                           static_cast<unsigned long long*>(nullptr), static_cast<const unsigned int*>(nullptr),
+                                           static_cast<unsigned int*>(nullptr), n, 0, kSampleKeyBits) != hipSuccess)
+        return 0;
+    return bytes;
+}
+hipError_t launch_guide_order(int n, unsigned long long* keys, unsigned long long* keys_sorted, unsigned int* vals,
+                              unsigned int* vals_sorted, void* temp, size_t temp_bytes, const float4* raw, float4* train,
+                              float* partial, float* wsum, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    size_t bytes = temp_bytes;
+    const hipError_t e = hipcub::DeviceRadixSort::SortPairs(temp, bytes, keys, keys_sorted, vals, vals_sorted, n, 0,
+                                                            kSampleKeyBits, st);
+    if (e != hipSuccess) return e;
+    const int nb = (n + kGatherBlock - 1) / kGatherBlock;
+    hipLaunchKernelGGL(k_guide_gather, dim3(nb), dim3(kGatherBlock), 0, st, raw, vals_sorted, n, train, partial);
+    hipLaunchKernelGGL(k_guide_sum, dim3(1), dim3(256), 0, st, partial, nb, wsum);
+    return hipGetLastError();
 }
 void launch_guide_mixture(const float* raw, const float* dirs, const float* u, int n, float* samples,
                           float* pdf_samples, float* pdf_dirs, hipStream_t st) {

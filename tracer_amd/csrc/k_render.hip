@@ -341,7 +341,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene s, DevPaths p, R
                                                     nrec0 = make_float4(pos.x, pos.y, pos.z, ns.x);
                                                     nrec1 = make_float4(ns.y, ns.z, wo.x, wo.y);
                                                     nrec2 = make_float4(wo.z, ls.wi.x, ls.wi.y, ls.wi.z);
-                                                    nrec3 = make_float4(0.0f, __int_as_float(slot), lp, 0.0f);
+                                                    nrec3 = make_float4(float(depth), __int_as_float(slot), lp, 0.0f);
                                                 }
                                             }
                                         }
@@ -368,7 +368,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene s, DevPaths p, R
                                 rec0 = make_float4(pos.x, pos.y, pos.z, ns.x);
                                 rec1 = make_float4(ns.y, ns.z, wo.x, wo.y);
                                 rec2 = make_float4(wo.z, wi.x, wi.y, wi.z);
-                                rec3 = make_float4(0.0f, __int_as_float(slot), pdf, 0.0f);
+                                rec3 = make_float4(float(depth), __int_as_float(slot), pdf, 0.0f);
                                 aux0 = make_float4(Tv.x > 0.0f ? 1.0f / Tv.x : 0.0f, Tv.y > 0.0f ? 1.0f / Tv.y : 0.0f,
                                                    Tv.z > 0.0f ? 1.0f / Tv.z : 0.0f, 0.0f);
                             }

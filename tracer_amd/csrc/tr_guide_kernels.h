@@ -64,8 +64,16 @@ void launch_guide_adam(const GuideDims& d, const GuideScratch& s, int n_pad, flo
                        int update, float* grad_out, hipStream_t st);
 void launch_guide_pack(const GuideDims& d, const float* master, unsigned short* fwd, unsigned short* bwd, hipStream_t st);
 // Records of a training pass -> compacted training batch with weights.
-void launch_guide_finalize(const DevPaths& p, int num_records, float4* train, unsigned int* train_count, float* wsum,
-                           hipStream_t st);
+void launch_guide_finalize(const DevPaths& p, int num_records, float4* raw, unsigned long long* keys, unsigned int* vals,
+                           unsigned int* train_count, hipStream_t st);
+// Sorts the finalized samples by key, writes them in that order to train and
+// their weight sum to wsum, both independent of the order in which
+// workgroups appended them.  temp: guide_order_temp_bytes(n) bytes; partial:
+// one float per 256 samples.
+size_t guide_order_temp_bytes(int n);
+hipError_t launch_guide_order(int n, unsigned long long* keys, unsigned long long* keys_sorted, unsigned int* vals,
+                              unsigned int* vals_sorted, void* temp, size_t temp_bytes, const float4* raw, float4* train,
+                              float* partial, float* wsum, hipStream_t st);
 void launch_guide_mixture(const float* raw, const float* dirs, const float* u, int n, float* samples,
                           float* pdf_samples, float* pdf_dirs, hipStream_t st);
 void launch_mfma_selftest(const float* A, const float* B, float* D, hipStream_t st);
