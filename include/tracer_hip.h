@@ -247,7 +247,7 @@ typedef struct tr_guide_config {
     int32_t hidden;            /* 64 or 128 */
     int32_t batch;             /* training samples per Adam step (1 << 18) */
     int32_t steps_per_pass;    /* Adam steps after each training pass (16) */
-    int32_t record_capacity;   /* path vertices recorded per training pass (1 << 20) */
+    int32_t record_capacity;   /* records aimed at per training pass (1 << 20); buffers grow so none is dropped */
     float lr, beta1, beta2, eps;   /* Adam (1e-3, 0.9, 0.99, 1e-8) */
     float weight_clamp;        /* training weights are clamped at this multiple of their mean (1000) */
     int32_t reserved0;

@@ -685,6 +685,7 @@ tr_status tr_render(tr_ctx* c, const tr_render_params* prm, double* film_sums, i
             rc.guide_frac = training ? P.guide_train_fraction : guide_frac;
             if (training) {
                 rc.rec_threshold = guide_record_threshold(c, num_paths, P.max_depth);
+                guide_fill_paths(c, p, true);   // the record buffers may have grown
                 rc.rec_seed = uint32_t(mix64(P.seed ^ (0x7261696eull + uint64_t(rc.sample_base))));
                 guide_begin_pass(c);
             }

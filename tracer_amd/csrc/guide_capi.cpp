@@ -190,6 +190,25 @@ void train_step(tr_ctx* c, const float4* rec, int n_total, int n, unsigned long 
 
 namespace trcapi {
 
+// Record buffers (vertex records, samples, sort keys and scratch) for cap
+// records; reserve only grows.
+void ensure_records(tr_ctx* c, int64_t cap_records) {
+    GuideState& g = c->guide;
+    if (cap_records <= g.record_capacity) return;
+    const size_t cap = size_t(cap_records);
+    g.rec.reserve(cap * 64);
+    g.rec_aux.reserve(cap * 32);
+    g.train.reserve(cap * 64);
+    g.train_raw.reserve(cap * 64);
+    g.keys.reserve(cap * 16);
+    g.vals.reserve(cap * 8);
+    g.wpart.reserve((cap / 256 + 1) * 4);
+    g.sort_tmp_bytes = guide_order_temp_bytes(int(cap));
+    if (g.sort_tmp_bytes == 0) throw HipError("radix sort scratch query failed");
+    g.sort_tmp.reserve(g.sort_tmp_bytes);
+    g.record_capacity = cap_records;
+}
+
 void guide_prepare(tr_ctx* c, int64_t P, bool training) {
     GuideState& g = c->guide;
     if (!g.ready) {
@@ -204,20 +223,7 @@ void guide_prepare(tr_ctx* c, int64_t P, bool training) {
         g.sh_rec.reserve(size_t(P) * 4);   // one shadow ray per path and bounce at most
         g.path_capacity = P;
     }
-    if (training && g.record_capacity < g.cfg.record_capacity) {
-        const size_t cap = size_t(g.cfg.record_capacity);
-        g.rec.reserve(cap * 64);
-        g.rec_aux.reserve(cap * 32);
-        g.train.reserve(cap * 64);
-        g.train_raw.reserve(cap * 64);
-        g.keys.reserve(cap * 16);
-        g.vals.reserve(cap * 8);
-        g.wpart.reserve((cap / 256 + 1) * 4);
-        g.sort_tmp_bytes = guide_order_temp_bytes(int(cap));
-        if (g.sort_tmp_bytes == 0) throw HipError("radix sort scratch query failed");
-        g.sort_tmp.reserve(g.sort_tmp_bytes);
-        g.record_capacity = g.cfg.record_capacity;
-    }
+    if (training) ensure_records(c, g.cfg.record_capacity);
 }
 
 void guide_fill_paths(tr_ctx* c, DevPaths& p, bool training) {
@@ -238,17 +244,21 @@ void guide_infer(tr_ctx* c, const DevPaths& p, int cur, int num_paths) {
     launch_guide_infer(g.net(), g.cfg.hidden, c->ds, p, cur, num_paths, g.raw.as<float>(), c->stream);
 }
 
-// Share of the paths that record.  Records that do not fit are dropped in
-// whichever order the workgroups append them, so the pass aims at 90% of the
-// capacity; before the first pass measures the records per path, it assumes
-// the most a path can make (a continuation and a light-sample record per
-// surface vertex).
+// Share of the paths that record: about record_capacity (the config's
+// records per pass) at the records per path the previous pass measured.  The
+// buffers then grow to hold every record the recording paths can make (a
+// continuation and a light-sample record per surface vertex), so none is
+// dropped: which records a full buffer drops would depend on the order in
+// which workgroups append them, and training would not be reproducible.
 unsigned int guide_record_threshold(tr_ctx* c, int64_t num_paths, int max_depth) {
     GuideState& g = c->guide;
-    const double per_path = g.avg_records_per_path > 0.0f ? double(g.avg_records_per_path) : 2.0 * (max_depth + 1);
-    const double want = 0.9 * double(g.record_capacity) / (double(num_paths) * per_path);
+    const double per_path = g.avg_records_per_path > 0.0f ? double(g.avg_records_per_path) : 12.0;
+    const double want = double(g.cfg.record_capacity) / (double(num_paths) * per_path);
     const unsigned int thr = want >= 1.0 ? 0xFFFFFFFFu : unsigned(want * 4294967296.0);
     g.last_recording_paths = double(num_paths) * (double(thr) / 4294967296.0);
+    // the slot hash picks close to this many paths: 5% and 65536 paths of margin
+    const double most = (g.last_recording_paths * 1.05 + 65536.0) * 2.0 * double(max_depth + 1);
+    ensure_records(c, int64_t(std::min(most, double(int64_t(1) << 26))));
     return thr;
 }
 
