@@ -162,12 +162,13 @@ def guide_flops(H: int) -> tuple[float, float]:
 def throughput(ctx, scene, steps: int, warmup: int) -> dict:
     base = scene.render_params(seed=0)
     W, H, spp = base.width, base.height, base.spp
+    film = ctx.host_array((H, W, 4))   # FP64 film sums of each frame, copied to page-locked host memory
     for i in range(warmup):
-        ctx.render(scene.render_params(seed=1000 + i), want_image=False)
+        ctx.render(scene.render_params(seed=1000 + i), want_image=False, out=film)
     tot = dict(ext=0, shadow=0, ms_extend=0.0, ms_shade=0.0, ms_shadow=0.0, ms_other=0.0, ms_dev=0.0, launches=0)
     t0 = time.perf_counter()
     for k in range(steps):
-        _, _, st = ctx.render(scene.render_params(seed=k, profile=1), want_image=False)
+        _, _, st = ctx.render(scene.render_params(seed=k, profile=1), want_image=False, out=film)
         tot["ext"] += st.extension_rays
         tot["shadow"] += st.shadow_rays
         tot["ms_extend"] += st.ms_extend

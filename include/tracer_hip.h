@@ -174,6 +174,13 @@ void tr_destroy(tr_ctx* ctx);
 const char* tr_last_error(const tr_ctx* ctx);
 /* Returns a static string describing the library build (arch, version). */
 const char* tr_version(void);
+/* Page-locked host memory, e.g. for film buffers that tr_render fills every
+ * frame: device-to-host copies into it run by DMA at full PCIe bandwidth
+ * instead of through a staging buffer.  No counterpart in the reference (its
+ * film lives in host memory).  Free with tr_host_free, whose ctx may be NULL
+ * (for buffers that outlive their context). */
+tr_status tr_host_alloc(tr_ctx* ctx, int64_t bytes, void** ptr);
+tr_status tr_host_free(tr_ctx* ctx, void* ptr);
 
 /* ---- scene (replaces Aggregate::build + create_general_scene,
  *      src/core/aggregate.hpp:19, src/factory/scene.hpp:14) ---------------- */

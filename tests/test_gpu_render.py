@@ -62,6 +62,14 @@ def test_render_is_deterministic(gpu_ctx, c1_scene):
     a, _, _ = gpu_ctx.render(p)
     b, _, _ = gpu_ctx.render(p)
     assert np.array_equal(a, b)
+    # a caller-provided output array receives the same sums (every element is overwritten)
+    out = np.full((64, 64, 4), np.nan)
+    c, _, _ = gpu_ctx.render(p, want_image=False, out=out)
+    assert c is out and np.array_equal(a, out)
+    # and so does one in page-locked host memory
+    pinned = gpu_ctx.host_array((64, 64, 4))
+    gpu_ctx.render(p, want_image=False, out=pinned)
+    assert np.array_equal(a, pinned)
 
 
 def test_progressive_equals_single(gpu_ctx, c1_scene):

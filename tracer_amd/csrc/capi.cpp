@@ -420,6 +420,22 @@ void tr_destroy(tr_ctx* c) {
 
 const char* tr_last_error(const tr_ctx* c) { return c ? c->err.c_str() : "null context"; }
 
+tr_status tr_host_alloc(tr_ctx* c, int64_t bytes, void** ptr) {
+    return guarded(c, [&] {
+        if (!ptr) throw InvalidArg("ptr is NULL");
+        if (bytes <= 0) throw InvalidArg("bytes must be positive");
+        *ptr = nullptr;
+        hip_check(hipHostMalloc(ptr, size_t(bytes), hipHostMallocDefault), "hipHostMalloc");
+    });
+}
+
+tr_status tr_host_free(tr_ctx* c, void* ptr) {
+    if (!c) return !ptr || hipHostFree(ptr) == hipSuccess ? TR_OK : TR_ERR_HIP;
+    return guarded(c, [&] {
+        if (ptr) hip_check(hipHostFree(ptr), "hipHostFree");
+    });
+}
+
 tr_status tr_scene_upload(tr_ctx* c, const tr_scene_desc* d) {
     return guarded(c, [&] {
         validate_scene(d);

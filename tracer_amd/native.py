@@ -6,6 +6,7 @@ the library is missing, loading raises NativeLibraryMissing.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 from dataclasses import dataclass, field
 from pathlib import Path
 
@@ -42,6 +43,10 @@ def hip_lib() -> C.CDLL:
         lib.tr_create.argtypes = [P(VP), I32]
         lib.tr_destroy.argtypes = [VP]
         lib.tr_destroy.restype = None
+        lib.tr_host_alloc.argtypes = [VP, C.c_int64, C.POINTER(C.c_void_p)]
+        lib.tr_host_alloc.restype = C.c_int32
+        lib.tr_host_free.argtypes = [VP, VP]
+        lib.tr_host_free.restype = C.c_int32
         lib.tr_last_error.argtypes = [VP]
         lib.tr_last_error.restype = C.c_char_p
         lib.tr_version.restype = C.c_char_p
@@ -253,10 +258,36 @@ class Context:
         self._check(self.lib.tr_bvh_info(self.h, C.byref(n), C.byref(nb), C.byref(tb)))
         return {"nodes": n.value, "node_bytes": nb.value, "tri_bytes": tb.value}
 
-    def render(self, params: TrRenderParams, film: np.ndarray | None = None, want_image: bool = True):
+    def host_array(self, shape, dtype=np.float64) -> np.ndarray:
+        """A zero-filled numpy array in page-locked host memory (tr_host_alloc),
+        freed when the array is garbage-collected."""
+        dt = np.dtype(dtype)
+        nbytes = int(np.prod(shape)) * dt.itemsize
+        ptr = C.c_void_p()
+        self._check(self.lib.tr_host_alloc(self.h, nbytes, C.byref(ptr)))
+        buf = (C.c_char * nbytes).from_address(ptr.value)
+        arr = np.frombuffer(buf, dtype=dt).reshape(shape)
+        arr[...] = 0
+        lib, p = self.lib, ptr.value
+        # freed without the context, which may be closed before the array goes
+        weakref.finalize(buf, lambda: lib.tr_host_free(None, C.c_void_p(p)))
+        return arr
+
+    def render(self, params: TrRenderParams, film: np.ndarray | None = None, want_image: bool = True,
+               out: np.ndarray | None = None):
+        """film: FP64 sums to accumulate onto (updated in place).  out: an
+        (H, W, 4) float64 array to receive the sums of a fresh render, reused
+        across calls instead of a new array per call."""
         W, H = params.width, params.height
         acc = film is not None
-        sums = film if acc else np.zeros((H, W, 4), np.float64)
+        if acc:
+            sums = film
+        elif out is not None:
+            if out.shape != (H, W, 4) or out.dtype != np.float64 or not out.flags.c_contiguous:
+                raise ValueError("out must be a C-contiguous (height, width, 4) float64 array")
+            sums = out
+        else:
+            sums = np.zeros((H, W, 4), np.float64)
         img = np.zeros((H, W, 3), np.float32) if want_image else None
         st = TrRenderStats()
         self._check(self.lib.tr_render(self.h, C.byref(params), dptr(sums), 1 if acc else 0,
