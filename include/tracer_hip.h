@@ -144,7 +144,8 @@ typedef struct tr_render_params {
     float guide_train_fraction;        /* MIS probability of the guide during training passes (0: they
                                           sample the BSDF alone and only record, so their image samples
                                           have the plain estimator's variance) */
-    int32_t reserved;
+    int32_t tail_paths;                /* plain passes: once at most this many paths are alive, one
+                                          kernel carries them to their ends (0 = 131072, < 0 = never) */
 } tr_render_params;
 
 typedef struct tr_render_stats {
@@ -166,6 +167,8 @@ typedef struct tr_render_stats {
     uint64_t guide_vertices;      /* surface vertices sampled with the guide mixture */
     uint64_t guide_inferences;    /* network evaluations (surface hits of guided passes) */
     double ms_guide_train;        /* guide training between passes (profile = 1) */
+    uint64_t tail_extension_rays; /* extension rays traced by the tail kernel (included above) */
+    double ms_tail;               /* tail kernel (profile = 1) */
 } tr_render_stats;
 
 /* ---- context ---------------------------------------------------------- */

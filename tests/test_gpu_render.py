@@ -157,3 +157,24 @@ def test_shadow_ratio_tracking_matches_closed_form(gpu_ctx):
     z = (a.mean(0) - b.mean(0)) / np.maximum(se, 1e-6)
     print(f"ratio vs closed form: max |z| {np.abs(z).max():.2f}")
     assert np.abs(z).max() < 5.0
+
+
+@pytest.mark.parametrize("config,detail", [(1, 1.0), (2, 1 / 16), (3, 1 / 12)])
+def test_tail_kernel_matches_per_bounce_kernels(gpu_ctx, config, detail):
+    """Finishing the last bounces of a plain pass in one kernel (tail_paths)
+    gives the film of the per-bounce kernels bit for bit: each path draws the
+    same random numbers and adds the same contributions in the same order.
+    A threshold as large as the pass starts the tail at the first bounce."""
+    from tracer_amd import native
+    s = native.config_scene(config, detail=detail)
+    gpu_ctx.load_scene(s)
+    films, stats = [], []
+    for tail in (-1, 48 * 48 * 8, 0, 2000):
+        sums, _, st = gpu_ctx.render(s.render_params(width=48, height=48, spp=8, seed=21, tail_paths=tail),
+                                     want_image=False)
+        films.append(sums)
+        stats.append(st)
+    assert stats[0].tail_extension_rays == 0 and stats[1].tail_extension_rays > 0
+    for f, st in zip(films[1:], stats[1:]):
+        assert np.array_equal(f, films[0])
+        assert st.extension_rays == stats[0].extension_rays and st.shadow_rays == stats[0].shadow_rays

@@ -36,7 +36,7 @@ def test_abi_struct_sizes():
     assert C.sizeof(abi.TrMaterial) == 72
     assert C.sizeof(abi.TrMedium) == 32
     assert C.sizeof(abi.TrCamera) == 52
-    assert C.sizeof(abi.TrRenderStats) == 144
+    assert C.sizeof(abi.TrRenderStats) == 160
     assert C.sizeof(abi.TrRenderParams) == 88
     assert C.sizeof(abi.TrGuideConfig) == 48
 

@@ -78,7 +78,7 @@ class TrRenderParams(C.Structure):
         ("guide", C.c_int32), ("guide_train_passes", C.c_int32),
         ("guide_fraction", C.c_float), ("guide_lr", C.c_float),
         ("profile", C.c_int32), ("guide_max_depth", C.c_int32),
-        ("guide_train_fraction", C.c_float), ("reserved", C.c_int32),
+        ("guide_train_fraction", C.c_float), ("tail_paths", C.c_int32),
     ]
 
 
@@ -90,6 +90,7 @@ class TrRenderStats(C.Structure):
         ("ms_other", C.c_double), ("ms_guide", C.c_double),
         ("extend_launches", C.c_uint64), ("node_visits", C.c_uint64), ("tri_tests", C.c_uint64),
         ("guide_vertices", C.c_uint64), ("guide_inferences", C.c_uint64), ("ms_guide_train", C.c_double),
+        ("tail_extension_rays", C.c_uint64), ("ms_tail", C.c_double),
     ]
 
     def as_dict(self) -> dict:

@@ -23,7 +23,7 @@ static_assert(sizeof(tr_material) == 72, "tr_material layout");
 static_assert(sizeof(tr_medium) == 32, "tr_medium layout");
 static_assert(sizeof(tr_camera) == 52, "tr_camera layout");
 static_assert(sizeof(tr_render_params) == 88, "tr_render_params layout");
-static_assert(sizeof(tr_render_stats) == 144, "tr_render_stats layout");
+static_assert(sizeof(tr_render_stats) == 160, "tr_render_stats layout");
 static_assert(sizeof(tr_guide_config) == 48, "tr_guide_config layout");
 
 namespace {
@@ -399,7 +399,7 @@ tr_status tr_create(tr_ctx** out, int32_t device) {
     tr_status s = guarded(c, [&] {
         hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
         c->counters.reserve(size_t(kNumCounters) * kCounterStride * sizeof(unsigned int));
-        c->stats.reserve(64);
+        c->stats.reserve(128);
     });
     if (s != TR_OK) {
         delete c;
@@ -656,7 +656,7 @@ tr_status tr_render(tr_ctx* c, const tr_render_params* prm, double* film_sums, i
             hip_check(hipMemcpyAsync(c->film.ptr, film_sums, size_t(npix) * 32, hipMemcpyHostToDevice, st), "H2D film");
         else
             hip_check(hipMemsetAsync(c->film.ptr, 0, size_t(npix) * 32, st), "film clear");
-        hip_check(hipMemsetAsync(c->stats.ptr, 0, 64, st), "stats clear");
+        hip_check(hipMemsetAsync(c->stats.ptr, 0, 128, st), "stats clear");
         const bool guided = P.guide != 0;
         const bool any_training = guided && P.guide_train_passes > 0;
         if (guided) guide_prepare(c, maxP, any_training);
@@ -677,7 +677,11 @@ tr_status tr_render(tr_ctx* c, const tr_render_params* prm, double* film_sums, i
         uint64_t trained = 0;
         size_t evi = 0;
         const bool prof = P.profile != 0;
-        double ms_ext = 0, ms_sh = 0, ms_shd = 0, ms_oth = 0, ms_gd = 0, ms_gt = 0;
+        double ms_ext = 0, ms_sh = 0, ms_shd = 0, ms_oth = 0, ms_gd = 0, ms_gt = 0, ms_tl = 0;
+        // plain passes finish their last bounces in one kernel once at most
+        // this many paths are alive
+        const int tail = P.tail_paths == 0 ? 131072 : P.tail_paths;
+        const bool use_tail = tail > 0 && !guided && P.profile != 2;
         std::vector<std::pair<size_t, int>> marks;   // (event index, kind)
         hipEvent_t e0 = ev(c, evi++), e1 = ev(c, evi++);
         hip_check(hipEventRecord(e0, st), "event");
@@ -708,6 +712,7 @@ tr_status tr_render(tr_ctx* c, const tr_render_params* prm, double* film_sums, i
             int cur = 0;
             timed(3, [&] { launch_generate(c->ds, p, rc, num_paths, cur, st); });
             for (int b = 0; b <= P.max_depth; ++b) {
+                if (use_tail) timed(6, [&] { launch_tail(c->ds, p, rc, cur, std::min(tail, num_paths), st); });
                 timed(0, [&] { launch_extend(c->ds.tr, p, cur, num_paths, P.profile == 2, st); });
                 if (guided && rc.guide_frac > 0.0f) timed(4, [&] { guide_infer(c, p, cur, num_paths); });
                 timed(1, [&] { launch_shade(c->ds, p, rc, cur, num_paths, guided, st); });
@@ -728,7 +733,7 @@ tr_status tr_render(tr_ctx* c, const tr_render_params* prm, double* film_sums, i
             launch_resolve(c->film.as<double4>(), c->s_out.as<float>(), int(npix), st);
             hip_check(hipMemcpyAsync(image, c->s_out.ptr, size_t(npix) * 12, hipMemcpyDeviceToHost, st), "D2H image");
         }
-        unsigned long long hs[8] = {0};
+        unsigned long long hs[16] = {0};
         hip_check(hipMemcpyAsync(hs, c->stats.ptr, sizeof hs, hipMemcpyDeviceToHost, st), "D2H stats");
         hip_check(hipStreamSynchronize(st), "render sync");
         if (stats) {
@@ -748,6 +753,7 @@ tr_status tr_render(tr_ctx* c, const tr_render_params* prm, double* film_sums, i
                 else if (m.second == 2) ms_shd += t;
                 else if (m.second == 4) ms_gd += t;
                 else if (m.second == 5) ms_gt += t;
+                else if (m.second == 6) ms_tl += t;
                 else ms_oth += t;
             }
             stats->ms_extend = ms_ext;
@@ -763,6 +769,8 @@ tr_status tr_render(tr_ctx* c, const tr_render_params* prm, double* film_sums, i
             stats->guide_train_samples = trained;
             stats->guide_inferences = hs[7];
             stats->ms_guide_train = ms_gt;
+            stats->tail_extension_rays = hs[8];
+            stats->ms_tail = ms_tl;
         }
     });
 }

@@ -19,6 +19,8 @@
 //     src/core/intersection.hpp:36-41);
 //   * the reference defects Q1, Q4, Q5, Q6 and Q23 (direct emission added
 //     before the medium sample) are not reproduced.
+#include <algorithm>
+
 #include "tr_guide.h"
 #include "tr_kernels.h"
 
@@ -44,6 +46,7 @@ __global__ __launch_bounds__(256) void k_generate(DevScene s, DevPaths p, Render
         *queue_counter(p, cur) = (unsigned int)num_paths;
         *queue_counter(p, 1 - cur) = 0u;
         *queue_counter(p, kShadowCounter) = 0u;
+        *queue_counter(p, kTailBlocksCounter) = 0u;
     }
     if (i >= num_paths) return;
     const int npix = c.width * c.height;
@@ -115,6 +118,289 @@ TR_D bool load_mixture(const DevPaths& p, int slot, Mixture& mix) {
     return true;
 }
 
+// Outputs of shade_path for the appends of the wavefront kernel.
+struct ShadeOut {
+    bool cont;                   // the path continues: its next ray, throughput and RNG are stored
+    ShadowOut sh;                // light sample to test with a shadow ray
+    bool g_used, g_fallback;     // guided vertex / fallback to the BSDF (non-finite head outputs)
+    bool want_rec, want_nee_rec; // training records of this vertex
+    float4 rec0, rec1, rec2, rec3, aux0;
+    float4 nrec0, nrec1, nrec2, nrec3;
+    float nee_weight;
+};
+
+// One vertex of the path in `slot`: reads its state and hit, adds emission,
+// samples the continuation and a light, and stores the updated state.
+template <bool GUIDE>
+TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c, int slot, ShadeOut& out) {
+    bool& cont = out.cont;
+    ShadowOut& sh = out.sh;
+    bool &g_used = out.g_used, &g_fallback = out.g_fallback, &want_rec = out.want_rec, &want_nee_rec = out.want_nee_rec;
+    float4 &rec0 = out.rec0, &rec1 = out.rec1, &rec2 = out.rec2, &rec3 = out.rec3, &aux0 = out.aux0;
+    float4 &nrec0 = out.nrec0, &nrec1 = out.nrec1, &nrec2 = out.nrec2, &nrec3 = out.nrec3;
+    float& nee_weight = out.nee_weight;
+    cont = false;
+    sh.emit = false;
+    g_used = g_fallback = want_rec = want_nee_rec = false;
+    rec0 = rec1 = rec2 = rec3 = aux0 = nrec0 = nrec1 = nrec2 = nrec3 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    nee_weight = 0.0f;
+    float4 next_o = make_float4(0, 0, 0, 0), next_d = make_float4(0, 0, 0, 0);
+    const float4 ro = p.ray_o[slot], rd = p.ray_d[slot], h = p.hit[slot];
+    float4 tp = p.thr[slot];
+    float4 rp = p.rad[slot];
+    const uint2 info = p.info[slot];
+    const int depth = int(info.y & 0xffu);
+    int scatter = int(info.y >> 8);
+    Pcg rng = load_rng(p, slot, info.x);
+    f3 T = xyz(tp);
+    const float prev_pdf = tp.w;
+    f3 L = xyz(rp);
+    const f3 o = xyz(ro), d = xyz(rd);
+    const int tri = __float_as_int(h.y);
+    bool alive = true;
+    if (tri < 0) {
+        // miss: constant environment (environment_light.cpp:179-188)
+        if (s.has_env) {
+            float w = 1.0f;
+            if (prev_pdf > 0.0f) w = power_heuristic(prev_pdf, s.light_pmf[s.num_area_lights] * (1.0f / (4.0f * kPi)));
+            L += T * s.env_L * w;
+        }
+        alive = false;
+    } else {
+        const float t = h.x;
+        const float4* tv = s.tr.tris + size_t(tri) * 3;
+        const f3 v0 = xyz(tv[0]), v1 = xyz(tv[1]), v2 = xyz(tv[2]);
+        const f3 ng = normalize(cross(v1 - v0, v2 - v0));
+        const TriShade sr = load_shade(s, tri);
+        const float dn = dot(d, ng);
+        // segment medium: the side of the hit surface the ray arrives from
+        const int mid = dn <= 0.0f ? sr.med_out : sr.med_in;
+        bool scattered = false;
+        f3 sp_pos;
+        if (mid >= 0) {
+            const DevMedium m = s.media[mid];
+            if (depth >= c.max_depth || scatter >= m.max_scatter) {
+                T *= mk(expf(-m.sigma_t.x * t), expf(-m.sigma_t.y * t), expf(-m.sigma_t.z * t));
+                if (scatter >= m.max_scatter) scatter = 0;
+            } else {
+                // delta / spectral tracking with the majorant
+                float tt = 0.0f;
+                for (int k = 0; k < 100000; ++k) {
+                    tt -= logf(1.0f - rng.uniform()) / m.majorant;
+                    if (tt >= t) break;
+                    const float xi = rng.uniform();
+                    if (xi < m.p_absorb) { alive = false; break; }
+                    if (xi < m.p_absorb + m.p_scatter) {
+                        T *= m.sigma_s * (1.0f / (m.majorant * m.p_scatter));
+                        scattered = true;
+                        sp_pos = o + d * tt;
+                        break;
+                    }
+                    T *= (f3s(m.majorant) - m.sigma_t) * (1.0f / (m.majorant * m.p_null));
+                }
+            }
+        }
+        if (!alive) {
+        } else if (scattered) {
+            // ---- medium vertex (path_trace_renderer.cpp:94-115) ----
+            const DevMedium m = s.media[mid];
+            ++scatter;
+            const f3 wo = -d;
+            if (s.num_light_entries > 0) {
+                float pmf;
+                const int e = pick_light(s, rng.uniform(), pmf);
+                const float lu = rng.uniform(), lv = rng.uniform();
+                const LightSampleD ls = sample_light(s, e, sp_pos, lu, lv);
+                if (ls.pdf != 0.0f && pmf > 0.0f && !is_black(ls.L)) {
+                    const float ph = hg_phase(m.g, -dot(ls.wi, wo));
+                    float w = 1.0f;
+                    float lp = pmf;
+                    if (ls.pdf > 0.0f) { lp *= ls.pdf; w = power_heuristic(lp, ph); }
+                    f3 trs = f3s(1.0f);
+                    if (ls.area) trs = shadow_tr(m, ls.dist, c.shadow_tr, rng);
+                    const f3 contrib = T * ls.L * trs * (ph * w / lp);
+                    if (max3(contrib) > 0.0f && finite3(contrib)) {
+                        sh.emit = true;
+                        sh.o = sp_pos;
+                        sh.tmin = kEps;
+                        sh.d = ls.wi;
+                        sh.tmax = ls.area ? ls.dist - kEps : ls.dist;
+                        sh.contrib = contrib;
+                    }
+                }
+            }
+            float ppdf;
+            const f3 wi = hg_sample(m.g, wo, rng.uniform(), rng.uniform(), ppdf);
+            // f / pdf = 1 for the phase function
+            next_o = f4(sp_pos, 0.0f);
+            next_d = f4(wi, 3.0e38f);
+            tp = f4(T, ppdf);
+            cont = true;
+        } else {
+            // ---- surface vertex ----
+            if (sr.light >= 0 && dn < 0.0f) {
+                // emission with MIS against light sampling (direct_illumination.cpp:176-197)
+                const float4 l0 = s.lights[4 * sr.light];
+                float w = 1.0f;
+                if (prev_pdf > 0.0f) {
+                    const float lp = s.lights[4 * sr.light + 1].w * t * t / (l0.w * (-dn));
+                    w = power_heuristic(prev_pdf, lp);
+                }
+                L += T * xyz(l0) * w;
+            }
+            if (depth >= c.max_depth) {
+                alive = false;
+            } else {
+                const float b1 = h.z, b2 = h.w;
+                const f3 pos = o + d * t;
+                const f3 ns = normalize(sr.n0 * (1.0f - b1 - b2) + sr.n1 * b1 + sr.n2 * b2);
+                f3 tx, ty;
+                onb(ns, tx, ty);
+                auto to_local = [&](f3 v) { return mk(dot(v, tx), dot(v, ty), dot(v, ns)); };
+                const DevMaterial mat = s.mats[sr.material];
+                const f3 wo = -d;
+                const f3 lwo = to_local(wo), lng = to_local(ng);
+                Mixture mix;
+                bool use_guide = false;
+                const bool guide_here = GUIDE && mat.guide_share > 0.0f;
+                if (guide_here && c.guide_frac > 0.0f && (c.guide_max_depth <= 0 || depth < c.guide_max_depth)) {
+                    use_guide = load_mixture(p, slot, mix);
+                    g_fallback = !use_guide;
+                    g_used = use_guide;
+                }
+                const float frac = c.guide_frac * mat.guide_share;
+                const bool rec_path = guide_here && c.guide_record &&
+                                      record_hash(uint32_t(slot), c.rec_seed) < c.rec_threshold;
+                const float su = rng.uniform(), u1 = rng.uniform(), u2 = rng.uniform();
+                f3 wi, fs;
+                float pdf;
+                if (use_guide) {
+                    float pdf_b;
+                    if (rng.uniform() < frac) {
+                        wi = mixture_sample(mix, su, u1, u2);
+                        const BsdfEval be = bsdf_eval(mat, to_local(wi), lwo, lng);
+                        fs = be.f;
+                        pdf_b = be.pdf;
+                    } else {
+                        const BsdfSample bs = bsdf_sample(mat, lwo, lng, su, u1, u2);
+                        wi = normalize(tx * bs.wi.x + ty * bs.wi.y + ns * bs.wi.z);
+                        fs = bs.f;
+                        pdf_b = bs.pdf;
+                    }
+                    pdf = is_black(fs) ? 0.0f : frac * mixture_pdf(mix, wi) + (1.0f - frac) * pdf_b;
+                } else {
+                    const BsdfSample bs = bsdf_sample(mat, lwo, lng, su, u1, u2);
+                    wi = normalize(tx * bs.wi.x + ty * bs.wi.y + ns * bs.wi.z);
+                    fs = bs.f;
+                    pdf = bs.pdf;
+                }
+                // Next-event estimation does not depend on the continuation sample: the
+                // reference estimates direct light before it samples the continuation
+                // and ends the path only afterwards (path_trace_renderer.cpp:141-163).
+                {
+                    // next-event estimation (direct_illumination.cpp:35-109)
+                    if (s.num_light_entries > 0) {
+                        float pmf;
+                        const int e = pick_light(s, rng.uniform(), pmf);
+                        const float lu = rng.uniform(), lv = rng.uniform();
+                        const LightSampleD ls = sample_light(s, e, pos, lu, lv);
+                        if (ls.pdf != 0.0f && pmf > 0.0f && !is_black(ls.L)) {
+                            const float dist = ls.area ? ls.dist - kEps : ls.dist;
+                            if (dist > kEps) {
+                                const BsdfEval be = bsdf_eval(mat, to_local(ls.wi), lwo, lng);
+                                if (!is_black(be.f)) {
+                                    float w = 1.0f;
+                                    float lp = pmf;
+                                    if (ls.pdf > 0.0f) {
+                                        lp *= ls.pdf;
+                                        const float sp = use_guide ? frac * mixture_pdf(mix, ls.wi) +
+                                                                         (1.0f - frac) * be.pdf
+                                                                   : be.pdf;
+                                        w = power_heuristic(lp, sp);
+                                    }
+                                    f3 trs = f3s(1.0f);
+                                    if (ls.area) {
+                                        const int lm = dot(ls.wi, ng) >= 0.0f ? sr.med_out : sr.med_in;
+                                        if (lm >= 0) trs = shadow_tr(s.media[lm], ls.dist, c.shadow_tr, rng);
+                                    }
+                                    const float cosg = fabsf(dot(ls.wi, ng));
+                                    const f3 contrib = T * ls.L * be.f * trs * (cosg * w / lp);
+                                    if (max3(contrib) > 0.0f && finite3(contrib)) {
+                                        sh.emit = true;
+                                        sh.o = pos;
+                                        sh.tmin = kEps;
+                                        sh.d = ls.wi;
+                                        sh.tmax = dist;
+                                        sh.contrib = contrib;
+                                        if (rec_path) {
+                                            // light-sample record: MIS-weighted f |cos| L / p_light,
+                                            // kept if the shadow ray reaches the light
+                                            nee_weight = lum(ls.L * be.f * trs) * (cosg * w / lp);
+                                            want_nee_rec = nee_weight > 0.0f && isfinite(nee_weight);
+                                            nrec0 = make_float4(pos.x, pos.y, pos.z, ns.x);
+                                            nrec1 = make_float4(ns.y, ns.z, wo.x, wo.y);
+                                            nrec2 = make_float4(wo.z, ls.wi.x, ls.wi.y, ls.wi.z);
+                                            nrec3 = make_float4(float(depth), __int_as_float(slot), lp, 0.0f);
+                                        }
+                                    }
+                                }
+                            }
+                        }
+                    }
+                }
+                if (is_black(fs) || !(pdf >= kEps)) {
+                    alive = false;
+                } else {
+                    // continuation (path_trace_renderer.cpp:155-163)
+                    const f3 Tv = T;
+                    T *= fs * (fabsf(dot(ng, wi)) / pdf);
+                    const f3 org = dot(ng, wi) > 0.0f ? pos + ng * kEps : pos - ng * kEps;
+                    next_o = f4(org, 0.0f);
+                    next_d = f4(wi, 3.0e38f);
+                    tp = f4(T, pdf);
+                    cont = finite3(T);
+                    if (rec_path && cont) {
+                        // record (x, n, wo, wi); its weight lum((L_end - L_next) * rr / Tv)
+                        // estimates f |cos| L_i / pdf, with emission MIS-weighted, once the
+                        // path has ended
+                        want_rec = true;
+                        rec0 = make_float4(pos.x, pos.y, pos.z, ns.x);
+                        rec1 = make_float4(ns.y, ns.z, wo.x, wo.y);
+                        rec2 = make_float4(wo.z, wi.x, wi.y, wi.z);
+                        rec3 = make_float4(float(depth), __int_as_float(slot), pdf, 0.0f);
+                        aux0 = make_float4(Tv.x > 0.0f ? 1.0f / Tv.x : 0.0f, Tv.y > 0.0f ? 1.0f / Tv.y : 0.0f,
+                                           Tv.z > 0.0f ? 1.0f / Tv.z : 0.0f, 0.0f);
+                    }
+                }
+            }
+        }
+    }
+    if (cont) {
+        // Russian roulette before the next segment (path_trace_renderer.cpp:54-58)
+        if (depth + 1 > c.min_depth) {
+            if (rng.uniform() > 0.9f) cont = false;
+            else {
+                tp.x /= 0.9f; tp.y /= 0.9f; tp.z /= 0.9f;
+                aux0.x *= 0.9f; aux0.y *= 0.9f; aux0.z *= 0.9f;
+            }
+        }
+    }
+    want_rec = want_rec && cont;
+    if (GUIDE && c.guide_record) {
+        // radiance gathered before this vertex closes the previous vertex's record
+        const int pr = p.grec[slot];
+        if (pr >= 0) p.rec_aux[2 * pr + 1] = make_float4(rp.x, rp.y, rp.z, 1.0f);
+    }
+    p.rad[slot] = f4(L, rp.w);
+    if (cont) {
+        p.ray_o[slot] = next_o;
+        p.ray_d[slot] = next_d;
+        p.thr[slot] = tp;
+        p.info[slot] = make_uint2(info.x, pack_info(depth + 1, scatter));
+        store_rng(p, slot, rng);
+    }
+}
+
 // GUIDE: surface vertices sample the continuation direction by one-sample MIS
 // between the guide mixture (probability c.guide_frac times the material's
 // guide_share) and the BSDF, and weight it, light samples and emitter hits
@@ -135,272 +421,21 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene s, DevPaths p, R
     for (unsigned int base = base0; base < n; base += stride) {
         const unsigned int i = base + threadIdx.x;
         const bool active = i < n;
-        bool cont = false;
-        ShadowOut sh;
-        sh.emit = false;
+        ShadeOut o;
+        o.cont = false;
+        o.sh.emit = false;
+        o.g_used = o.g_fallback = o.want_rec = o.want_nee_rec = false;
         int slot = 0;
-        float4 next_o = make_float4(0, 0, 0, 0), next_d = make_float4(0, 0, 0, 0);
-        bool g_used = false, g_fallback = false, want_rec = false, want_nee_rec = false;
-        float4 rec0{}, rec1{}, rec2{}, rec3{}, aux0{};
-        float4 nrec0{}, nrec1{}, nrec2{}, nrec3{};
-        float nee_weight = 0.0f;
         if (active) {
             slot = q[i];
-            const float4 ro = p.ray_o[slot], rd = p.ray_d[slot], h = p.hit[slot];
-            float4 tp = p.thr[slot];
-            float4 rp = p.rad[slot];
-            const uint2 info = p.info[slot];
-            const int depth = int(info.y & 0xffu);
-            int scatter = int(info.y >> 8);
-            Pcg rng = load_rng(p, slot, info.x);
-            f3 T = xyz(tp);
-            const float prev_pdf = tp.w;
-            f3 L = xyz(rp);
-            const f3 o = xyz(ro), d = xyz(rd);
-            const int tri = __float_as_int(h.y);
-            bool alive = true;
-            if (tri < 0) {
-                // miss: constant environment (environment_light.cpp:179-188)
-                if (s.has_env) {
-                    float w = 1.0f;
-                    if (prev_pdf > 0.0f) w = power_heuristic(prev_pdf, s.light_pmf[s.num_area_lights] * (1.0f / (4.0f * kPi)));
-                    L += T * s.env_L * w;
-                }
-                alive = false;
-            } else {
-                const float t = h.x;
-                const float4* tv = s.tr.tris + size_t(tri) * 3;
-                const f3 v0 = xyz(tv[0]), v1 = xyz(tv[1]), v2 = xyz(tv[2]);
-                const f3 ng = normalize(cross(v1 - v0, v2 - v0));
-                const TriShade sr = load_shade(s, tri);
-                const float dn = dot(d, ng);
-                // segment medium: the side of the hit surface the ray arrives from
-                const int mid = dn <= 0.0f ? sr.med_out : sr.med_in;
-                bool scattered = false;
-                f3 sp_pos;
-                if (mid >= 0) {
-                    const DevMedium m = s.media[mid];
-                    if (depth >= c.max_depth || scatter >= m.max_scatter) {
-                        T *= mk(expf(-m.sigma_t.x * t), expf(-m.sigma_t.y * t), expf(-m.sigma_t.z * t));
-                        if (scatter >= m.max_scatter) scatter = 0;
-                    } else {
-                        // delta / spectral tracking with the majorant
-                        float tt = 0.0f;
-                        for (int k = 0; k < 100000; ++k) {
-                            tt -= logf(1.0f - rng.uniform()) / m.majorant;
-                            if (tt >= t) break;
-                            const float xi = rng.uniform();
-                            if (xi < m.p_absorb) { alive = false; break; }
-                            if (xi < m.p_absorb + m.p_scatter) {
-                                T *= m.sigma_s * (1.0f / (m.majorant * m.p_scatter));
-                                scattered = true;
-                                sp_pos = o + d * tt;
-                                break;
-                            }
-                            T *= (f3s(m.majorant) - m.sigma_t) * (1.0f / (m.majorant * m.p_null));
-                        }
-                    }
-                }
-                if (!alive) {
-                } else if (scattered) {
-                    // ---- medium vertex (path_trace_renderer.cpp:94-115) ----
-                    const DevMedium m = s.media[mid];
-                    ++scatter;
-                    const f3 wo = -d;
-                    if (s.num_light_entries > 0) {
-                        float pmf;
-                        const int e = pick_light(s, rng.uniform(), pmf);
-                        const float lu = rng.uniform(), lv = rng.uniform();
-                        const LightSampleD ls = sample_light(s, e, sp_pos, lu, lv);
-                        if (ls.pdf != 0.0f && pmf > 0.0f && !is_black(ls.L)) {
-                            const float ph = hg_phase(m.g, -dot(ls.wi, wo));
-                            float w = 1.0f;
-                            float lp = pmf;
-                            if (ls.pdf > 0.0f) { lp *= ls.pdf; w = power_heuristic(lp, ph); }
-                            f3 trs = f3s(1.0f);
-                            if (ls.area) trs = shadow_tr(m, ls.dist, c.shadow_tr, rng);
-                            const f3 contrib = T * ls.L * trs * (ph * w / lp);
-                            if (max3(contrib) > 0.0f && finite3(contrib)) {
-                                sh.emit = true;
-                                sh.o = sp_pos;
-                                sh.tmin = kEps;
-                                sh.d = ls.wi;
-                                sh.tmax = ls.area ? ls.dist - kEps : ls.dist;
-                                sh.contrib = contrib;
-                            }
-                        }
-                    }
-                    float ppdf;
-                    const f3 wi = hg_sample(m.g, wo, rng.uniform(), rng.uniform(), ppdf);
-                    // f / pdf = 1 for the phase function
-                    next_o = f4(sp_pos, 0.0f);
-                    next_d = f4(wi, 3.0e38f);
-                    tp = f4(T, ppdf);
-                    cont = true;
-                } else {
-                    // ---- surface vertex ----
-                    if (sr.light >= 0 && dn < 0.0f) {
-                        // emission with MIS against light sampling (direct_illumination.cpp:176-197)
-                        const float4 l0 = s.lights[4 * sr.light];
-                        float w = 1.0f;
-                        if (prev_pdf > 0.0f) {
-                            const float lp = s.lights[4 * sr.light + 1].w * t * t / (l0.w * (-dn));
-                            w = power_heuristic(prev_pdf, lp);
-                        }
-                        L += T * xyz(l0) * w;
-                    }
-                    if (depth >= c.max_depth) {
-                        alive = false;
-                    } else {
-                        const float b1 = h.z, b2 = h.w;
-                        const f3 pos = o + d * t;
-                        const f3 ns = normalize(sr.n0 * (1.0f - b1 - b2) + sr.n1 * b1 + sr.n2 * b2);
-                        f3 tx, ty;
-                        onb(ns, tx, ty);
-                        auto to_local = [&](f3 v) { return mk(dot(v, tx), dot(v, ty), dot(v, ns)); };
-                        const DevMaterial mat = s.mats[sr.material];
-                        const f3 wo = -d;
-                        const f3 lwo = to_local(wo), lng = to_local(ng);
-                        Mixture mix;
-                        bool use_guide = false;
-                        const bool guide_here = GUIDE && mat.guide_share > 0.0f;
-                        if (guide_here && c.guide_frac > 0.0f && (c.guide_max_depth <= 0 || depth < c.guide_max_depth)) {
-                            use_guide = load_mixture(p, slot, mix);
-                            g_fallback = !use_guide;
-                            g_used = use_guide;
-                        }
-                        const float frac = c.guide_frac * mat.guide_share;
-                        const bool rec_path = guide_here && c.guide_record &&
-                                              record_hash(uint32_t(slot), c.rec_seed) < c.rec_threshold;
-                        const float su = rng.uniform(), u1 = rng.uniform(), u2 = rng.uniform();
-                        f3 wi, fs;
-                        float pdf;
-                        if (use_guide) {
-                            float pdf_b;
-                            if (rng.uniform() < frac) {
-                                wi = mixture_sample(mix, su, u1, u2);
-                                const BsdfEval be = bsdf_eval(mat, to_local(wi), lwo, lng);
-                                fs = be.f;
-                                pdf_b = be.pdf;
-                            } else {
-                                const BsdfSample bs = bsdf_sample(mat, lwo, lng, su, u1, u2);
-                                wi = normalize(tx * bs.wi.x + ty * bs.wi.y + ns * bs.wi.z);
-                                fs = bs.f;
-                                pdf_b = bs.pdf;
-                            }
-                            pdf = is_black(fs) ? 0.0f : frac * mixture_pdf(mix, wi) + (1.0f - frac) * pdf_b;
-                        } else {
-                            const BsdfSample bs = bsdf_sample(mat, lwo, lng, su, u1, u2);
-                            wi = normalize(tx * bs.wi.x + ty * bs.wi.y + ns * bs.wi.z);
-                            fs = bs.f;
-                            pdf = bs.pdf;
-                        }
-                        // Next-event estimation does not depend on the continuation sample: the
-                        // reference estimates direct light before it samples the continuation
-                        // and ends the path only afterwards (path_trace_renderer.cpp:141-163).
-                        {
-                            // next-event estimation (direct_illumination.cpp:35-109)
-                            if (s.num_light_entries > 0) {
-                                float pmf;
-                                const int e = pick_light(s, rng.uniform(), pmf);
-                                const float lu = rng.uniform(), lv = rng.uniform();
-                                const LightSampleD ls = sample_light(s, e, pos, lu, lv);
-                                if (ls.pdf != 0.0f && pmf > 0.0f && !is_black(ls.L)) {
-                                    const float dist = ls.area ? ls.dist - kEps : ls.dist;
-                                    if (dist > kEps) {
-                                        const BsdfEval be = bsdf_eval(mat, to_local(ls.wi), lwo, lng);
-                                        if (!is_black(be.f)) {
-                                            float w = 1.0f;
-                                            float lp = pmf;
-                                            if (ls.pdf > 0.0f) {
-                                                lp *= ls.pdf;
-                                                const float sp = use_guide ? frac * mixture_pdf(mix, ls.wi) +
-                                                                                 (1.0f - frac) * be.pdf
-                                                                           : be.pdf;
-                                                w = power_heuristic(lp, sp);
-                                            }
-                                            f3 trs = f3s(1.0f);
-                                            if (ls.area) {
-                                                const int lm = dot(ls.wi, ng) >= 0.0f ? sr.med_out : sr.med_in;
-                                                if (lm >= 0) trs = shadow_tr(s.media[lm], ls.dist, c.shadow_tr, rng);
-                                            }
-                                            const float cosg = fabsf(dot(ls.wi, ng));
-                                            const f3 contrib = T * ls.L * be.f * trs * (cosg * w / lp);
-                                            if (max3(contrib) > 0.0f && finite3(contrib)) {
-                                                sh.emit = true;
-                                                sh.o = pos;
-                                                sh.tmin = kEps;
-                                                sh.d = ls.wi;
-                                                sh.tmax = dist;
-                                                sh.contrib = contrib;
-                                                if (rec_path) {
-                                                    // light-sample record: MIS-weighted f |cos| L / p_light,
-                                                    // kept if the shadow ray reaches the light
-                                                    nee_weight = lum(ls.L * be.f * trs) * (cosg * w / lp);
-                                                    want_nee_rec = nee_weight > 0.0f && isfinite(nee_weight);
-                                                    nrec0 = make_float4(pos.x, pos.y, pos.z, ns.x);
-                                                    nrec1 = make_float4(ns.y, ns.z, wo.x, wo.y);
-                                                    nrec2 = make_float4(wo.z, ls.wi.x, ls.wi.y, ls.wi.z);
-                                                    nrec3 = make_float4(float(depth), __int_as_float(slot), lp, 0.0f);
-                                                }
-                                            }
-                                        }
-                                    }
-                                }
-                            }
-                        }
-                        if (is_black(fs) || !(pdf >= kEps)) {
-                            alive = false;
-                        } else {
-                            // continuation (path_trace_renderer.cpp:155-163)
-                            const f3 Tv = T;
-                            T *= fs * (fabsf(dot(ng, wi)) / pdf);
-                            const f3 org = dot(ng, wi) > 0.0f ? pos + ng * kEps : pos - ng * kEps;
-                            next_o = f4(org, 0.0f);
-                            next_d = f4(wi, 3.0e38f);
-                            tp = f4(T, pdf);
-                            cont = finite3(T);
-                            if (rec_path && cont) {
-                                // record (x, n, wo, wi); its weight lum((L_end - L_next) * rr / Tv)
-                                // estimates f |cos| L_i / pdf, with emission MIS-weighted, once the
-                                // path has ended
-                                want_rec = true;
-                                rec0 = make_float4(pos.x, pos.y, pos.z, ns.x);
-                                rec1 = make_float4(ns.y, ns.z, wo.x, wo.y);
-                                rec2 = make_float4(wo.z, wi.x, wi.y, wi.z);
-                                rec3 = make_float4(float(depth), __int_as_float(slot), pdf, 0.0f);
-                                aux0 = make_float4(Tv.x > 0.0f ? 1.0f / Tv.x : 0.0f, Tv.y > 0.0f ? 1.0f / Tv.y : 0.0f,
-                                                   Tv.z > 0.0f ? 1.0f / Tv.z : 0.0f, 0.0f);
-                            }
-                        }
-                    }
-                }
-            }
-            if (cont) {
-                // Russian roulette before the next segment (path_trace_renderer.cpp:54-58)
-                if (depth + 1 > c.min_depth) {
-                    if (rng.uniform() > 0.9f) cont = false;
-                    else {
-                        tp.x /= 0.9f; tp.y /= 0.9f; tp.z /= 0.9f;
-                        aux0.x *= 0.9f; aux0.y *= 0.9f; aux0.z *= 0.9f;
-                    }
-                }
-            }
-            want_rec = want_rec && cont;
-            if (GUIDE && c.guide_record) {
-                // radiance gathered before this vertex closes the previous vertex's record
-                const int pr = p.grec[slot];
-                if (pr >= 0) p.rec_aux[2 * pr + 1] = make_float4(rp.x, rp.y, rp.z, 1.0f);
-            }
-            p.rad[slot] = f4(L, rp.w);
-            if (cont) {
-                p.ray_o[slot] = next_o;
-                p.ray_d[slot] = next_d;
-                p.thr[slot] = tp;
-                p.info[slot] = make_uint2(info.x, pack_info(depth + 1, scatter));
-                store_rng(p, slot, rng);
-            }
+            shade_path<GUIDE>(s, p, c, slot, o);
         }
+        const bool cont = o.cont, g_used = o.g_used, g_fallback = o.g_fallback;
+        const bool want_rec = o.want_rec, want_nee_rec = o.want_nee_rec;
+        const ShadowOut& sh = o.sh;
+        const float4 &rec0 = o.rec0, &rec1 = o.rec1, &rec2 = o.rec2, &rec3 = o.rec3, &aux0 = o.aux0;
+        const float4 &nrec0 = o.nrec0, &nrec1 = o.nrec1, &nrec2 = o.nrec2, &nrec3 = o.nrec3;
+        const float nee_weight = o.nee_weight;
         n_guided += g_used ? 1u : 0u;
         n_fallback += g_fallback ? 1u : 0u;
         int nee_ri = -1;
@@ -448,6 +483,74 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene s, DevPaths p, R
         if (__lane_id() == 0) {
             if (n_guided) atomicAdd(&p.stats[6], (unsigned long long)n_guided);
             if (n_fallback) atomicAdd(&p.stats[5], (unsigned long long)n_fallback);
+        }
+    }
+}
+
+// The last bounces of a plain pass in one launch: once the queue holds at
+// most max_tail paths, each thread takes one and carries it to its end
+// (closest hit, shading, shadow ray, next bounce).  Late bounces hold few
+// paths, and launched one by one each costs as much as its slowest ray; here
+// a path's bounces follow each other directly.  A path draws the same random
+// numbers, makes the same decisions and adds the same contributions in the
+// same order as in the per-bounce kernels, so the film is identical.  The
+// last workgroup to finish empties the queue, so the per-bounce kernels that
+// follow find no work; with more than max_tail paths nothing happens.
+__global__ __launch_bounds__(kShadeBlock) void k_tail(DevScene s, DevPaths p, RenderConsts c, int cur,
+                                                      unsigned int max_tail) {
+    __shared__ TraceStack<kShadeBlock, kLdsStack> st;
+    const int tid = threadIdx.x;
+    const int gt = blockIdx.x * blockDim.x + tid;
+    const int stride = gridDim.x * blockDim.x;
+    const unsigned int queued = *queue_counter(p, cur);
+    const bool run = queued <= max_tail;
+    const unsigned int n = run ? queued : 0u;
+    const int* q = p.queue[cur];
+    unsigned long long n_ext = 0, n_sh = 0;
+    for (unsigned int i = gt; i < n; i += stride) {
+        const int slot = q[i];
+        bool cont = true;
+        while (cont) {
+            const float4 a = p.ray_o[slot], b = p.ray_d[slot];
+            TraceHit h;
+            trace<false, false, kShadeBlock, kLdsStack>(s.tr, xyz(a), a.w, xyz(b), b.w, st, tid, p.ovf_node + gt,
+                                                        p.ovf_t + gt, stride, h);
+            p.hit[slot] = make_float4(h.t, __int_as_float(h.tri), h.b1, h.b2);
+            ++n_ext;
+            ShadeOut o;
+            shade_path<false>(s, p, c, slot, o);
+            if (o.sh.emit) {
+                ++n_sh;
+                TraceHit hs;
+                if (!trace<true, false, kShadeBlock, kLdsStack>(s.tr, o.sh.o, o.sh.tmin, o.sh.d, o.sh.tmax, st, tid,
+                                                                p.ovf_node + gt, p.ovf_t + gt, stride, hs)) {
+                    float4 r = p.rad[slot];
+                    r.x += o.sh.contrib.x; r.y += o.sh.contrib.y; r.z += o.sh.contrib.z;
+                    p.rad[slot] = r;
+                }
+            }
+            cont = o.cont;
+        }
+    }
+    // ray counts: one atomic per wave and counter
+    for (int off = 32; off > 0; off >>= 1) {
+        n_ext += __shfl_xor(n_ext, off);
+        n_sh += __shfl_xor(n_sh, off);
+    }
+    if (__lane_id() == 0) {
+        if (n_ext) {
+            atomicAdd(&p.stats[0], n_ext);
+            atomicAdd(&p.stats[8], n_ext);
+        }
+        if (n_sh) atomicAdd(&p.stats[1], n_sh);
+    }
+    // every workgroup has read the queue length before it counts itself done
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned int* done = queue_counter(p, kTailBlocksCounter);
+        if (atomicAdd(done, 1u) == gridDim.x - 1) {
+            *done = 0u;
+            if (run) *queue_counter(p, cur) = 0u;
         }
     }
 }
@@ -554,6 +657,12 @@ void launch_shade(const DevScene& s, const DevPaths& p, const RenderConsts& c, i
         hipLaunchKernelGGL(k_shade<true>, dim3(shade_blocks(max_paths)), dim3(kShadeBlock), 0, st, s, p, c, cur);
     else
         hipLaunchKernelGGL(k_shade<false>, dim3(shade_blocks(max_paths)), dim3(kShadeBlock), 0, st, s, p, c, cur);
+}
+void launch_tail(const DevScene& s, const DevPaths& p, const RenderConsts& c, int cur, int max_tail, hipStream_t st) {
+    // one thread per path, and no more threads than the traversal spill area has columns
+    int blocks = std::min((max_tail + kShadeBlock - 1) / kShadeBlock, p.ovf_threads / kShadeBlock);
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_tail, dim3(blocks), dim3(kShadeBlock), 0, st, s, p, c, cur, (unsigned int)max_tail);
 }
 void launch_accumulate(const DevPaths& p, const RenderConsts& c, double4* film, hipStream_t st) {
     const int n = c.width * c.height;

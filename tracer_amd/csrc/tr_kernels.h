@@ -46,8 +46,9 @@ struct DevPaths {
 // The queue counters sit 1 KB apart: appends to different queues then go to
 // different cache lines and do not serialise behind each other.
 constexpr int kCounterStride = 256;
-constexpr int kNumCounters = 3;
+constexpr int kNumCounters = 4;
 constexpr int kShadowCounter = 2;
+constexpr int kTailBlocksCounter = 3;   // workgroups of the current k_tail launch that have finished
 TR_D unsigned int* queue_counter(const DevPaths& p, int k) { return p.counters + k * kCounterStride; }
 
 // Block-aggregated append to two counters: one atomic per counter and block
@@ -125,6 +126,9 @@ int trace_grid_threads();
 void launch_generate(const DevScene& s, const DevPaths& p, const RenderConsts& c, int num_paths, int cur, hipStream_t st);
 void launch_shade(const DevScene& s, const DevPaths& p, const RenderConsts& c, int cur, int max_paths, bool guide,
                   hipStream_t st);
+// Plain passes: when queue cur holds at most max_tail paths, every one is
+// carried to its end and the queue emptied; otherwise nothing happens.
+void launch_tail(const DevScene& s, const DevPaths& p, const RenderConsts& c, int cur, int max_tail, hipStream_t st);
 void launch_accumulate(const DevPaths& p, const RenderConsts& c, double4* film, hipStream_t st);
 void launch_resolve(const double4* film, float* image, int num_pixels, hipStream_t st);
 void launch_bsdf_eval(const DevMaterial* mats, const int* ids, const float* wi, const float* wo, const float* ng, int64_t n,
