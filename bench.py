@@ -12,9 +12,12 @@ Also reported on the same JSON line:
   roofline       k_extend (closest-hit traversal, the dominant kernel):
                  algorithmic bytes per extension ray = 32 (ray) + 16 (hit) +
                  128 * nodes visited + 48 * triangles tested, visit counts from
-                 an instrumented replay of the same frame; divided by the
-                 k_extend time measured with HIP events on the library's
-                 stream inside the timed region.  traffic: HBM bytes per launch
+                 an instrumented replay of the same frame (which runs every
+                 bounce through k_extend), times the rays of the k_extend
+                 launches (the tail kernel traces the last bounces' rays
+                 itself); divided by the k_extend time measured with HIP
+                 events on the library's stream inside the timed region.
+                 traffic: HBM bytes per launch
                  from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary.
   equal_time     configuration 4 (the only light shines through a 0.5 x 0.5
                  aperture) at a reduced film: relMSE against a 65536-spp plain
@@ -165,12 +168,15 @@ def throughput(ctx, scene, steps: int, warmup: int) -> dict:
     film = ctx.host_array((H, W, 4))   # FP64 film sums of each frame, copied to page-locked host memory
     for i in range(warmup):
         ctx.render(scene.render_params(seed=1000 + i), want_image=False, out=film)
-    tot = dict(ext=0, shadow=0, ms_extend=0.0, ms_shade=0.0, ms_shadow=0.0, ms_other=0.0, ms_dev=0.0, launches=0)
+    tot = dict(ext=0, shadow=0, tail_ext=0, ms_extend=0.0, ms_shade=0.0, ms_shadow=0.0, ms_tail=0.0, ms_other=0.0, ms_dev=0.0,
+               launches=0)
     t0 = time.perf_counter()
     for k in range(steps):
         _, _, st = ctx.render(scene.render_params(seed=k, profile=1), want_image=False, out=film)
         tot["ext"] += st.extension_rays
         tot["shadow"] += st.shadow_rays
+        tot["tail_ext"] += st.tail_extension_rays
+        tot["ms_tail"] += st.ms_tail
         tot["ms_extend"] += st.ms_extend
         tot["ms_shade"] += st.ms_shade
         tot["ms_shadow"] += st.ms_shadow
@@ -316,7 +322,8 @@ def main() -> int:
     tris_per_ray = cst.tri_tests / max(1, cst.extension_rays)
     bytes_per_ray = RAY_BYTES + HIT_BYTES + NODE_BYTES * nodes_per_ray + TRI_BYTES * tris_per_ray
     launches = max(1, tp["launches"])
-    bytes_per_launch = bytes_per_ray * tp["ext"] / launches
+    # rays of the k_extend launches: the tail kernel traces the last bounces' rays itself
+    bytes_per_launch = bytes_per_ray * (tp["ext"] - tp["tail_ext"]) / launches
     ms_per_launch = tp["ms_extend"] / launches
     achieved = bytes_per_launch / (ms_per_launch / 1e3) / 1e9 if ms_per_launch > 0 else 0.0
     traffic = read_traffic(ROOT / "profiles" / f"k_extend_c{args.config}_traffic.json")
@@ -345,7 +352,8 @@ def main() -> int:
         "mrays_per_s": mrays,
         "samples_per_s": value,
         "rays": {"extension": tp["ext"], "shadow": tp["shadow"]},
-        "kernel_ms": {k: round(tp[k], 3) for k in ("ms_extend", "ms_shade", "ms_shadow", "ms_other")},
+        "kernel_ms": {k: round(tp[k], 3) for k in ("ms_extend", "ms_shade", "ms_shadow", "ms_tail", "ms_other")},
+        "tail_extension_rays": tp["tail_ext"],
         "device_ms_per_step": tp["ms_dev"] / args.steps,
         "bvh": {"build_s": round(bvh_s, 3), **info, "nodes_per_ray": nodes_per_ray, "tris_per_ray": tris_per_ray},
         "roofline": {
