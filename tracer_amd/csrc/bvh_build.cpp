@@ -14,7 +14,7 @@ namespace trhip {
 namespace {
 
 constexpr float kInf = std::numeric_limits<float>::infinity();
-constexpr int kBins = 16;
+constexpr int kBins = 64;   // 16 bins: 7.11 node visits per ray on configuration 2, 64: 6.45
 
 struct Box {
     float lo[3] = {kInf, kInf, kInf};
