@@ -187,7 +187,8 @@ def throughput(ctx, scene, steps: int, warmup: int) -> dict:
     return {"W": W, "H": H, "spp": spp, "max_depth": base.max_depth, "elapsed": elapsed, **tot}
 
 
-def equal_time(res: int, ref_spp: int, spp: int, seeds: int, hidden: int, train_pp: int = 32, steps: int = 128) -> dict:
+def equal_time(res: int, ref_spp: int, spp: int, seeds: int, hidden: int, train_pp: int = 32, steps: int = 128,
+               train_spp: int = 0) -> dict:
     """Configuration 4 at res x res: guided (training included) vs plain at the
     same device time, both against a ref_spp plain reference."""
     from tracer_amd import native
@@ -208,7 +209,7 @@ def equal_time(res: int, ref_spp: int, spp: int, seeds: int, hidden: int, train_
         img = sums[..., :3] / sums[..., 3:]
         return float(np.mean((img - R) ** 2 / (R ** 2 + 1e-2)))
 
-    train_spp = spp // 2   # training phase: passes of train_pp spp, `steps` Adam steps after each
+    train_spp = train_spp or spp // 2   # training phase: passes of train_pp spp, `steps` Adam steps after each
     g_err, g_ms, g_train_ms, g_inf_ms, g_inf, g_smp = [], [], [], [], 0, 0
     for k in range(seeds):
         ctx.guide_init(hidden=hidden, seed=11 + k, batch=1 << 16, steps_per_pass=steps)
@@ -279,9 +280,10 @@ def main() -> int:
     ap.add_argument("--relmse-res", type=int, default=256)
     ap.add_argument("--ref-spp", type=int, default=65536)
     ap.add_argument("--relmse-spp", type=int, default=256)
-    ap.add_argument("--relmse-seeds", type=int, default=6)
+    ap.add_argument("--relmse-seeds", type=int, default=24)
     ap.add_argument("--train-pass-spp", type=int, default=32, help="spp per training pass of the guided render")
-    ap.add_argument("--train-steps", type=int, default=128, help="Adam steps after each training pass")
+    ap.add_argument("--train-steps", type=int, default=16, help="Adam steps after each training pass")
+    ap.add_argument("--train-spp", type=int, default=32, help="spp of the guided render's training phase (0: half)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one 1 GiB device-to-device copy (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -374,7 +376,7 @@ def main() -> int:
     if not args.no_relmse:
         print("[bench] equal-time relMSE on configuration 4 ...", file=sys.stderr, flush=True)
         out.update(equal_time(args.relmse_res, args.ref_spp, args.relmse_spp, args.relmse_seeds, 64,
-                              args.train_pass_spp, args.train_steps))
+                              args.train_pass_spp, args.train_steps, args.train_spp))
         print(f"[bench] equal-time ratio {out['equal_time']['ratio_plain_over_guided']:.2f}", file=sys.stderr,
               flush=True)
     if not args.no_cpu:
