@@ -293,6 +293,7 @@ struct GpuRenderOptions {
     float guide_lr = 1e-3f;
     float guide_train_fraction = 0.0f;      // guide probability during training passes
     int guide_max_depth = 0;                // 0 = guide every depth
+    int tail_paths = 0;                     // plain passes: one kernel finishes the last paths (0 = 131072, < 0 = off)
     tr_render_stats* stats_out = nullptr;   // optional
 };
 RC<Renderer> create_pt_renderer(const PTRendererParams& params, const GpuRenderOptions& gpu);

@@ -471,6 +471,7 @@ public:
         rp.guide_lr = gpu.guide_lr;
         rp.guide_train_fraction = gpu.guide_train_fraction;
         rp.guide_max_depth = gpu.guide_max_depth;
+        rp.tail_paths = gpu.tail_paths;
         RenderTarget rt;
         rt.color = Image2D<Spectrum>(rp.width, rp.height);
         static_assert(sizeof(Spectrum) == 12, "Spectrum is three floats");
