@@ -1,6 +1,7 @@
 /*
- * scenes.h — synthetic benchmark scenes (configurations 1-5) built through the
- * tracer:: factory API, and a C ABI to fetch them as tr_scene_desc.
+ * scenes.h — synthetic benchmark scenes (configurations 1-5) and OBJ scenes
+ * built through the tracer:: factory API, and a C ABI to fetch them as
+ * tr_scene_desc.
  *
  * There is no reference counterpart: the reference hard-codes its scenes in
  * src/main.cpp run*() (src/main.cpp:4-313) and loads external OBJ assets.
@@ -25,6 +26,15 @@ typedef struct trs_scene trs_scene;
  * terrain resolution) for quick tests.  Returns NULL on failure and sets
  * *err (static storage) when err is not NULL. */
 trs_scene* trs_create(int32_t config, uint64_t seed, double detail, const char** err);
+/* A scene read from a Wavefront OBJ file (and its MTL libraries) through
+ * load_model_from_file and create_texture_from_file, wired as the
+ * reference's run() does (src/main.cpp:212-282): Phong materials, emissive
+ * materials as area lights, MediumInterface{vacuum inside, fog outside} (the
+ * fog of src/main.cpp:242 when outside_fog != 0, else vacuum), Gaussian
+ * filter (0.5, 0.6), 3 primitives per BVH leaf.  Returns NULL and sets *err
+ * on a read or parse error, a triangle without a material, an image texture
+ * or a glass material (Ni > 1). */
+trs_scene* trs_load_obj(const char* obj_path, const tr_camera* camera, int32_t outside_fog, const char** err);
 const tr_scene_desc* trs_desc(const trs_scene* s);
 /* Film, spp and depths of the configuration (PTRendererParams + film). */
 void trs_render_defaults(const trs_scene* s, tr_render_params* out);

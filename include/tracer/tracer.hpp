@@ -9,7 +9,10 @@
 //   create_triangle_mesh        src/factory/shape.hpp:17
 //   create_geometric_primitive  src/factory/primivite.hpp:12-14
 //   create_disney               src/factory/material.hpp:37-52
-//   create_phong_material       src/factory/material.hpp:19-24 (Lambert part: ks = 0)
+//   create_phong_material       src/factory/material.hpp:19-24
+//   create_glass                src/factory/material.hpp:13-17 (declared; the GPU path rejects it)
+//   load_model_from_file        src/utility/mesh_load.hpp:93, mesh_load.cpp:25-113
+//   create_texture_from_file    src/utility/image_file.hpp:23-45, image_file.cpp:139-166 (constant colours)
 //   create_constant_texture2d   src/factory/texture.hpp:12
 //   create_ibl_light            src/factory/light.hpp:12-15 (constant texture)
 //   create_homogeneous_medium   src/factory/medium.hpp:21-26
@@ -130,6 +133,33 @@ struct mesh_t {
     std::vector<int> indices;
     std::vector<int> materials;
 };
+// mesh_load.hpp:28-79: one MTL material as the OBJ reader returns it.
+struct material_t {
+    std::string name;
+    real ambient[3] = {0, 0, 0};
+    real diffuse[3] = {0, 0, 0};
+    real specular[3] = {0, 0, 0};
+    real transmittance[3] = {0, 0, 0};
+    real emission[3] = {0, 0, 0};
+    real shininess = 1;
+    real ior = 1;        // index of refraction
+    real dissolve = 1;   // 1 == opaque; 0 == fully transparent
+    int illum = 0;
+    std::string map_ka, map_kd, map_ks, map_ns, map_bump, disp, map_d, refl;
+    real roughness = 0, metallic = 0, sheen = 0, clearcoat_thickness = 0, clearcoat_roughness = 0;
+    std::string map_pr, map_pm, map_ps, map_ke, norm;
+};
+// mesh_load.hpp:81-86
+struct model_t {
+    std::string name;
+    std::vector<mesh_t> mesh;
+    std::vector<material_t> material;
+};
+// Reads a Wavefront OBJ file and the MTL libraries it names (mesh_load.cpp:25-113):
+// one mesh per object/group with vertices deduplicated on (position, normal, uv),
+// faces triangulated, one material index per triangle (-1 = none).  Throws
+// std::runtime_error when the file cannot be read or parsed.
+model_t load_model_from_file(const std::string& path);
 
 template <class T>
 class Image2D {
@@ -181,6 +211,23 @@ RC<Material> create_disney(RC<const Texture2D> base_color, RC<const Texture2D> m
                            Box<const NormalMapper> normal_mapper, RC<const BSSRDFSurface> bssrdf);
 RC<Material> create_phong_material(RC<const Texture2D> map_ka, RC<const Texture2D> map_kd, RC<const Texture2D> map_ks,
                                    RC<const Texture2D> map_ns);
+// Specular glass; constructed so that scene code written for the reference
+// compiles, but the GPU renderer throws for a scene that uses it.
+RC<Material> create_glass(RC<const Texture2D> map_kr, RC<const Texture2D> map_kt, RC<const Texture2D> map_ior);
+
+// image_file.hpp:23-43: the textures of one MTL material.
+struct MaterialTexture {
+    RC<Texture2D> map_ka, map_kd, map_ks, map_kt, ior, map_ns, map_bump, disp, map_d, refl;
+    RC<Texture2D> map_pr, map_pm, map_ps, map_ke, norm;
+    bool has_emission = false;
+    bool as_glass = false;
+};
+// image_file.cpp:139-166 for materials given by constant colours: Ka, Kd, Ks,
+// Ns and Ke become constant textures, has_emission is set when Ke > 0, and
+// ior > 1 makes a glass (Ks, or white when Ks is black, as reflectance and
+// transmittance).  A material naming an image file throws
+// std::runtime_error: image textures are not supported by the GPU path.
+MaterialTexture create_texture_from_file(const material_t& material);
 
 class Shape {
 public:

@@ -40,10 +40,15 @@ enum {
 typedef struct tr_ctx tr_ctx;
 
 /* Material kinds.  TR_MAT_LAMBERT is the reference's Phong material with
- * ks = 0 (src/material/phong.cpp:29-70): base_color is kd.  TR_MAT_DISNEY is
- * create_disney (src/factory/material.hpp:37-52, src/material/disney.cpp)
- * with every texture constant. */
-enum { TR_MAT_LAMBERT = 0, TR_MAT_DISNEY = 1 };
+ * ks = 0 (src/material/phong.cpp:29-70): base_color is kd, already divided by
+ * max(1, max_i kd_i).  TR_MAT_DISNEY is create_disney
+ * (src/factory/material.hpp:37-52, src/material/disney.cpp) with every
+ * texture constant.  TR_MAT_PHONG is create_phong_material(ka, kd, ks, ns)
+ * (src/factory/material.hpp:19-24) with constant textures: base_color = kd,
+ * specular_scale = ks, shininess = ns as given; the library applies the
+ * energy normalisation and lobe weights of Phong::shading
+ * (src/material/phong.cpp:36-53). */
+enum { TR_MAT_LAMBERT = 0, TR_MAT_DISNEY = 1, TR_MAT_PHONG = 2 };
 
 typedef struct tr_material {
     int32_t type;
@@ -51,6 +56,7 @@ typedef struct tr_material {
     float metallic, roughness, transmission, transmission_roughness, ior;
     float specular_scale[3];
     float specular_tint, anisotropic, sheen, sheen_tint, clearcoat, clearcoat_gloss;
+    float shininess;   /* Phong exponent ns (TR_MAT_PHONG) */
 } tr_material;
 
 /* create_homogeneous_medium(sigma_a, sigma_s, g, max_scattering_count),

@@ -22,6 +22,13 @@
 //  * LambertRef<T>: Phong with ks = 0 (src/material/phong.cpp:29-70) is an
 //    AggregateBSDF (src/material/aggregate_bsdf.hpp:36-118) holding one
 //    DiffuseBXDF (src/material/bxdf/diffuse_bxdf.cpp:14-38) of weight 1.
+//  * PhongRef<T>: Phong with any kd, ks, ns (src/material/phong.cpp:29-70):
+//    the same AggregateBSDF with a DiffuseBXDF and a PhongSpecularBXDF
+//    (src/material/bxdf/phong_specular_bxdf.cpp:7-79), weighted by luminance.
+//      Q24 sample() picks the lobe with (u - 0.001) * sum(w) <= w_d
+//          (aggregate_bsdf.hpp:62-76), so the diffuse lobe is chosen with
+//          probability w_d + 0.001 while the returned pdf weighs it by w_d.
+//          Fixed: u * sum(w) <= w_d.  Q22 applies as for LambertRef.
 //  * HGRef<T>: Henyey-Greenstein phase function,
 //    src/medium/phase_function.hpp:12-77.
 #pragma once
@@ -496,6 +503,126 @@ public:
         }
         r.f = (albedo / kPi<T>) * normal_correct_factor(geo.z, shd.z, wi);
         r.pdf = pd;
+        return r;
+    }
+};
+
+// Phong(kd, ks, ns) through AggregateBSDF<2> (aggregate_bsdf.hpp:36-118):
+// the constructor restates Phong::shading (phong.cpp:29-60).
+template <class T>
+class PhongRef {
+public:
+    Coord<T> geo, shd;
+    RGB<T> kd, ks, albedo;
+    T ns = 1, wd = T(0.5), ws = T(0.5);
+    bool fixed = false;
+    PhongRef(const Coord<T>& g, const Coord<T>& s, const RGB<T>& kd_in, const RGB<T>& ks_in, T ns_in,
+             bool fixed_mode = false)
+        : geo(g), shd(s), kd(kd_in), ks(ks_in), ns(ns_in), fixed(fixed_mode) {
+        T dem = 1;   // phong.cpp:36-41
+        for (int i = 0; i < 3; ++i) dem = std::max(dem, kd[i] + ks[i]);
+        kd = kd / dem;
+        ks = ks / dem;
+        const T ld = kd.lum(), ls = ks.lum();   // phong.cpp:43-53
+        wd = ld + ls > T(kEps) ? ld / (ld + ls) : T(0.5);
+        ws = 1 - wd;
+        albedo = kd + ks;   // phong.cpp:55-56
+    }
+    bool fringe(const V3<T>& w) const { return geo.positive_z(w) ^ shd.positive_z(w); }
+    // phong_specular_bxdf.cpp:19-21: (e + 1) / (2 pi) cos^e
+    static T pow_cos_pdf(T e, T c) { return (e + 1) * (T(0.5) * kInvPi<T>) * std::pow(c, e); }
+    // phong_specular_bxdf.cpp:29-51 (value / ks and pdf are the same lobe)
+    T spec_lobe(const V3<T>& lwi, const V3<T>& lwo) const {
+        if (lwi.z <= 0 || lwo.z <= 0) return 0;
+        const V3<T> ideal(-lwo.x, -lwo.y, lwo.z);   // reflect(lwo, z), reflection.hpp:12-14
+        const T c = dot(ideal, lwi) / (lwi.length() * ideal.length());
+        return c <= 0 ? T(0) : pow_cos_pdf(ns, c);
+    }
+    RGB<T> f_local(const V3<T>& lwi, const V3<T>& lwo) const {   // sum of the two evaluates
+        if (lwi.z <= 0 || lwo.z <= 0) return RGB<T>(0);
+        return kd * kInvPi<T> + ks * spec_lobe(lwi, lwo);
+    }
+    T pdf_local(const V3<T>& lwi, const V3<T>& lwo) const {   // weighted, before / sum(w)
+        if (lwi.z <= 0 || lwo.z <= 0) return 0;
+        return wd * lwi.z * kInvPi<T> + ws * spec_lobe(lwi, lwo);
+    }
+    RGB<T> eval(const V3<T>& wi, const V3<T>& wo) const {
+        if (fringe(wi) | fringe(wo)) {   // bsdf.hpp:114-127
+            if (geo.positive_z(wi) & geo.positive_z(wo)) return albedo * kInvPi<T>;
+            return RGB<T>(0);
+        }
+        const V3<T> lwi = shd.to_local(wi).normalize(), lwo = shd.to_local(wo).normalize();
+        if (!lwi.z || !lwo.z) return RGB<T>(0);
+        if (fixed) return f_local(lwi, lwo) * normal_correct_factor(geo.z, shd.z, wi);
+        return f_local(lwi, lwo);
+    }
+    T pdf(const V3<T>& wi, const V3<T>& wo) const {
+        if (fringe(wi) | fringe(wo)) {
+            if (geo.positive_z(wi) ^ geo.positive_z(wo)) return 0;
+            return geo.to_local(wi).normalize().z * kInvPi<T>;
+        }
+        const V3<T> lwi = shd.to_local(wi).normalize(), lwo = shd.to_local(wo).normalize();
+        if (!lwi.z || !lwo.z) return 0;
+        const T sw = wd + ws;
+        return sw > 0 ? pdf_local(lwi, lwo) / sw : T(0);
+    }
+    BSDFSample<T> sample(const V3<T>& wo, T su, T u, T v) const {
+        BSDFSample<T> r;
+        if (fringe(wo)) {   // bsdf.hpp:141-153
+            if (!geo.positive_z(wo)) return r;
+            V3<T> l = cosine_sample_hemisphere(u, v);
+            T pd = l.z * kInvPi<T>;
+            if (pd < T(kEps)) return r;
+            V3<T> wi = geo.to_global(l).normalize();
+            r.wi = wi;
+            r.f = albedo * (kInvPi<T> * normal_correct_factor(geo.z, shd.z, wi));
+            r.pdf = pd;
+            return r;
+        }
+        const V3<T> lwo = shd.to_local(wo).normalize();
+        if (!lwo.z) return r;
+        const T sum = wd + ws;
+        const bool diffuse = fixed ? su * sum <= wd : (su - T(0.001)) * sum <= wd;
+        if (lwo.z <= 0) return r;   // both BXDF samplers reject wo below the surface
+        V3<T> lwi;
+        RGB<T> f_own;
+        T pdf_own;
+        if (diffuse) {   // diffuse_bxdf.cpp:20-31
+            lwi = cosine_sample_hemisphere(u, v);
+            pdf_own = lwi.z * kInvPi<T>;
+            if (pdf_own < T(0.001f)) return r;
+            f_own = kd * kInvPi<T>;
+        } else {         // phong_specular_bxdf.cpp:7-17, :55-69
+            const V3<T> ideal = V3<T>(-lwo.x, -lwo.y, lwo.z).normalize();
+            const T ct = std::pow(u, 1 / (ns + 1));
+            const T st = std::sqrt(std::max(T(0), 1 - ct * ct));
+            const T phi = 2 * kPi<T> * v;
+            const V3<T> l = V3<T>(st * std::cos(phi), st * std::sin(phi), ct).normalize();
+            lwi = Coord<T>::from_z(ideal).to_global(l).normalize();
+            if (lwi.z <= 0) return r;
+            pdf_own = pow_cos_pdf(ns, l.z);   // the sampler's own pdf, phong_specular_bxdf.cpp:67
+            f_own = ks * spec_lobe(lwi, lwo);
+        }
+        if (!(pdf_own > 0)) return r;   // BXDFSampleResult::is_valid, bxdf.hpp:16-18
+        lwi = lwi.normalize();
+        const V3<T> wi = shd.to_global(lwi);
+        r.wi = wi;
+        if (fixed) {
+            r.f = eval(wi, wo);
+            r.pdf = pdf(wi, wo);
+            return r;
+        }
+        // aggregate_bsdf.hpp:84-98: the chosen lobe's pdf times its weight
+        // plus the other lobe's pdf times its weight (not divided by sum(w)),
+        // f of both lobes, times the normal correction
+        if (diffuse) {
+            r.f = f_own + ks * spec_lobe(lwi, lwo);
+            r.pdf = pdf_own * wd + spec_lobe(lwi, lwo) * ws;
+        } else {
+            r.f = f_own + ((lwi.z <= 0 || lwo.z <= 0) ? RGB<T>(0) : kd * kInvPi<T>);
+            r.pdf = pdf_own * ws + ((lwi.z <= 0 || lwo.z <= 0) ? T(0) : lwi.z * kInvPi<T>) * wd;
+        }
+        r.f = r.f * normal_correct_factor(geo.z, shd.z, wi);
         return r;
     }
 };

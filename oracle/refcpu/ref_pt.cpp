@@ -236,9 +236,18 @@ struct SurfBSDF {
     int type = 0;
     std::unique_ptr<DisneyRef<float>> dis;
     std::unique_ptr<LambertRef<float>> lam;
-    RGB<float> eval(const V3f& wi, const V3f& wo) const { return type ? dis->eval(wi, wo) : lam->eval(wi, wo); }
-    float pdf(const V3f& wi, const V3f& wo) const { return type ? dis->pdf(wi, wo) : lam->pdf(wi, wo); }
-    BSDFSample<float> sample(const V3f& wo, float a, float b, float c) const { return type ? dis->sample(wo, a, b, c) : lam->sample(wo, a, b, c); }
+    std::unique_ptr<PhongRef<float>> pho;
+    RGB<float> eval(const V3f& wi, const V3f& wo) const {
+        return type == TR_MAT_DISNEY ? dis->eval(wi, wo) : type == TR_MAT_PHONG ? pho->eval(wi, wo) : lam->eval(wi, wo);
+    }
+    float pdf(const V3f& wi, const V3f& wo) const {
+        return type == TR_MAT_DISNEY ? dis->pdf(wi, wo) : type == TR_MAT_PHONG ? pho->pdf(wi, wo) : lam->pdf(wi, wo);
+    }
+    BSDFSample<float> sample(const V3f& wo, float a, float b, float c) const {
+        return type == TR_MAT_DISNEY ? dis->sample(wo, a, b, c)
+             : type == TR_MAT_PHONG  ? pho->sample(wo, a, b, c)
+                                     : lam->sample(wo, a, b, c);
+    }
 };
 static DisneyParams to_params(const tr_material& m) {
     DisneyParams p;
@@ -249,10 +258,13 @@ static DisneyParams to_params(const tr_material& m) {
     p.clearcoat_gloss = m.clearcoat_gloss;
     return p;
 }
+template <class T> static RGB<T> phong_kd(const tr_material& m) { return RGB<T>(m.base_color[0], m.base_color[1], m.base_color[2]); }
+template <class T> static RGB<T> phong_ks(const tr_material& m) { return RGB<T>(m.specular_scale[0], m.specular_scale[1], m.specular_scale[2]); }
 static void make_bsdf(const Scene& s, const Isect& is, bool quirks, SurfBSDF& b) {
     const tr_material& m = s.mats[s.mat[is.prim]];
     b.type = m.type;
     if (m.type == TR_MAT_DISNEY) b.dis.reset(new DisneyRef<float>(is.geo, is.shd, to_params(m), !quirks));
+    else if (m.type == TR_MAT_PHONG) b.pho.reset(new PhongRef<float>(is.geo, is.shd, phong_kd<float>(m), phong_ks<float>(m), m.shininess, !quirks));
     else b.lam.reset(new LambertRef<float>(is.geo, is.shd, RGB<float>(m.base_color[0], m.base_color[1], m.base_color[2]), !quirks));
 }
 
@@ -873,6 +885,9 @@ void ref_bsdf_eval(const tr_material* mats, const int32_t* ids, const float* wi,
             if (m.type == TR_MAT_DISNEY) {
                 DisneyRef<T> b(geo, shd, to_params(m), fixed != 0);
                 fv = b.eval(vi, vo); pv = b.pdf(vi, vo);
+            } else if (m.type == TR_MAT_PHONG) {
+                PhongRef<T> b(geo, shd, phong_kd<T>(m), phong_ks<T>(m), T(m.shininess), fixed != 0);
+                fv = b.eval(vi, vo); pv = b.pdf(vi, vo);
             } else {
                 LambertRef<T> b(geo, shd, RGB<T>(m.base_color[0], m.base_color[1], m.base_color[2]), fixed != 0);
                 fv = b.eval(vi, vo); pv = b.pdf(vi, vo);
@@ -895,6 +910,9 @@ void ref_bsdf_sample(const tr_material* mats, const int32_t* ids, const float* w
             BSDFSample<T> s;
             if (m.type == TR_MAT_DISNEY) {
                 DisneyRef<T> b(geo, shd, to_params(m), fixed != 0);
+                s = b.sample(vo, T(u[3 * i]), T(u[3 * i + 1]), T(u[3 * i + 2]));
+            } else if (m.type == TR_MAT_PHONG) {
+                PhongRef<T> b(geo, shd, phong_kd<T>(m), phong_ks<T>(m), T(m.shininess), fixed != 0);
                 s = b.sample(vo, T(u[3 * i]), T(u[3 * i + 1]), T(u[3 * i + 2]));
             } else {
                 LambertRef<T> b(geo, shd, RGB<T>(m.base_color[0], m.base_color[1], m.base_color[2]), fixed != 0);

@@ -1,0 +1,91 @@
+"""Small Wavefront OBJ / MTL scenes written into a directory for the ingest
+tests (the reference ships no OBJ assets)."""
+from __future__ import annotations
+
+from pathlib import Path
+
+from tracer_amd import abi
+
+
+def write(dirpath: Path, name: str, obj: str, mtl: str | None = None, mtl_name: str = "scene.mtl") -> Path:
+    dirpath.mkdir(parents=True, exist_ok=True)
+    if mtl is not None:
+        (dirpath / mtl_name).write_text(mtl)
+    p = dirpath / name
+    p.write_text(obj)
+    return p
+
+
+def camera(pos, target, up=(0.0, 1.0, 0.0), fov_deg=40.0, aspect=1.0) -> abi.TrCamera:
+    import math
+    c = abi.TrCamera()
+    c.pos = abi.f3(*pos)
+    c.target = abi.f3(*target)
+    c.up = abi.f3(*up)
+    c.fov_y = math.radians(fov_deg)
+    c.lens_radius = 0.0
+    c.focal_distance = 1.0
+    c.aspect = aspect
+    return c
+
+
+def box_room_obj(light_ke=(12.0, 12.0, 12.0), block_ks=(0.35, 0.35, 0.35), block_ns=25.0) -> tuple[str, str]:
+    """A closed-on-five-sides room [-1, 1]^3 open towards +z, a ceiling quad
+    light facing down, a tall block with a Phong specular lobe and a short
+    diffuse block; faces are quads (split by the reader), normals given per
+    face with v//vn corners."""
+    mtl = f"""# materials
+newmtl white
+Kd 0.73 0.73 0.73
+newmtl red
+Kd 0.65 0.05 0.05
+newmtl green
+Kd 0.12 0.45 0.15
+newmtl glossy
+Kd 0.45 0.40 0.30
+Ks {block_ks[0]} {block_ks[1]} {block_ks[2]}
+Ns {block_ns}
+newmtl light
+Kd 0.78 0.78 0.78
+Ke {light_ke[0]} {light_ke[1]} {light_ke[2]}
+"""
+    lines = ["mtllib scene.mtl"]
+    v, n = [], []
+
+    def quad(c0, c1, c2, c3, normal, mat, group=None):
+        if group:
+            lines.append(f"o {group}")
+        lines.append(f"usemtl {mat}")
+        base = len(v)
+        v.extend([c0, c1, c2, c3])
+        n.append(normal)
+        ni = len(n)
+        lines.append("f " + " ".join(f"{base + k + 1}//{ni}" for k in range(4)))
+
+    # room: floor, ceiling, back wall, left (red), right (green); normals point inwards
+    quad((-1, -1, 1), (1, -1, 1), (1, -1, -1), (-1, -1, -1), (0, 1, 0), "white", "room")
+    quad((-1, 1, -1), (1, 1, -1), (1, 1, 1), (-1, 1, 1), (0, -1, 0), "white")
+    quad((-1, -1, -1), (1, -1, -1), (1, 1, -1), (-1, 1, -1), (0, 0, 1), "white")
+    quad((-1, -1, 1), (-1, -1, -1), (-1, 1, -1), (-1, 1, 1), (1, 0, 0), "red")
+    quad((1, -1, -1), (1, -1, 1), (1, 1, 1), (1, 1, -1), (-1, 0, 0), "green")
+    # light: a quad just below the ceiling facing down
+    y = 0.998
+    quad((-0.25, y, -0.25), (0.25, y, -0.25), (0.25, y, 0.25), (-0.25, y, 0.25), (0, -1, 0), "light", "light")
+
+    def block(x0, x1, z0, z1, h, mat, name):
+        y0 = -0.998
+        first = True
+        for c, nn in (
+            (((x0, h, z1), (x1, h, z1), (x1, h, z0), (x0, h, z0)), (0, 1, 0)),
+            (((x0, y0, z1), (x1, y0, z1), (x1, h, z1), (x0, h, z1)), (0, 0, 1)),
+            (((x1, y0, z0), (x0, y0, z0), (x0, h, z0), (x1, h, z0)), (0, 0, -1)),
+            (((x0, y0, z0), (x0, y0, z1), (x0, h, z1), (x0, h, z0)), (-1, 0, 0)),
+            (((x1, y0, z1), (x1, y0, z0), (x1, h, z0), (x1, h, z1)), (1, 0, 0)),
+        ):
+            quad(*c, nn, mat, name if first else None)
+            first = False
+
+    block(-0.65, -0.05, -0.7, -0.1, 0.25, "glossy", "tall")
+    block(0.1, 0.6, 0.05, 0.55, -0.4, "white", "short")
+    text = "\n".join([f"v {a} {b} {c}" for a, b, c in v] + [f"vn {a} {b} {c}" for a, b, c in n] + lines) + "\n"
+    return text, mtl

@@ -1,4 +1,4 @@
-"""GPU Disney / Lambert BSDF (tr_bsdf_eval_batch, tr_bsdf_sample_batch)
+"""GPU Disney / Lambert / Phong BSDF (tr_bsdf_eval_batch, tr_bsdf_sample_batch)
 against the FP64 twin of the reference formulas (src/material/disney.cpp,
 oracle/refcpu/ref_bsdf.hpp, defect corrections Q2/Q3/Q21/Q22 on).
 
@@ -81,3 +81,65 @@ def test_sample_directions_match_fp64(gpu_ctx):
     print(f"direction abs diff: median {np.median(ang):.3g}, p99 {np.quantile(ang, 0.99):.3g}, max {ang.max():.3g}")
     assert np.quantile(ang, 0.99) < 1e-5
     assert np.quantile(ang, 0.9999) < 1e-3
+
+
+def _phong_materials():
+    from tracer_amd.abi import TR_MAT_PHONG, make_material
+    rng = np.random.default_rng(31)
+    mats = []
+    for ns in (0.0, 1.0, 4.0, 12.0, 30.0, 80.0):
+        for _ in range(3):
+            mats.append(make_material(type=TR_MAT_PHONG, base_color=tuple(rng.uniform(0, 0.9, 3)),
+                                      specular_scale=tuple(rng.uniform(0, 0.9, 3)), shininess=ns))
+    mats.append(make_material(type=TR_MAT_PHONG, base_color=(0.6, 0.5, 0.4), specular_scale=0.0, shininess=10.0))
+    mats.append(make_material(type=TR_MAT_PHONG, base_color=0.0, specular_scale=0.0, shininess=5.0))
+    return mats
+
+
+def test_phong_eval_pdf_matches_fp64(gpu_ctx):
+    """Phong (phong.cpp, phong_specular_bxdf.cpp) against the FP64 oracle.
+    Tolerance: 1e-5 relative plus the FP32 error of the lobe cosine c
+    (<= 4e-7 absolute) carried through c^ns, i.e. ns * 4e-7 / c relative, on
+    the domain where wi.z, wo.z >= 2e-2."""
+    from oracle import refcpu
+    rng = np.random.default_rng(32)
+    mats = _phong_materials()
+    n = 1 << 16
+    ids = rng.integers(0, len(mats), n).astype(np.int32)
+    wi, wo = sphere_dirs(rng, n), sphere_dirs(rng, n)
+    f, pdf = gpu_ctx.bsdf_eval(mats, ids, wi, wo)
+    fr, pr = refcpu.bsdf_eval(mats, ids, wi, wo, precision=1, fixed=1)
+    wi64, wo64 = wi.astype(np.float64), wo.astype(np.float64)
+    c = wi64[:, 2] * wo64[:, 2] - wi64[:, 0] * wo64[:, 0] - wi64[:, 1] * wo64[:, 1]
+    ns = np.array([mats[i].shininess for i in ids])
+    tol = REL_TOL + np.where(c > 0, ns * 4e-7 / np.maximum(c, 1e-30), 0.0)
+    dom = (np.abs(wi[:, 2]) >= 2e-2) & (np.abs(wo[:, 2]) >= 2e-2)
+    assert np.all(f[fr == 0] == 0) and np.all(pdf[pr == 0] == 0)
+    fm = dom[:, None] & (fr >= 1e-6)
+    pm = dom & (pr >= 1e-6)
+    ef = (_rel(f.astype(np.float64), fr) / tol[:, None])[fm]
+    ep = (_rel(pdf.astype(np.float64), pr) / tol)[pm]
+    print(f"Phong: f n={ef.size} max err/tol {ef.max():.3g}; pdf n={ep.size} max err/tol {ep.max():.3g}")
+    assert ef.max() <= 1.0 and ep.max() <= 1.0
+
+
+def test_phong_sample_directions_match_fp64(gpu_ctx):
+    """Same uniforms -> same lobe and direction (the lobe is built in the
+    frame of Coord::create_from_z on both sides)."""
+    from oracle import refcpu
+    rng = np.random.default_rng(33)
+    mats = _phong_materials()
+    n = 1 << 15
+    ids = rng.integers(0, len(mats), n).astype(np.int32)
+    wo = sphere_dirs(rng, n)
+    u = rng.uniform(0, 1, (n, 3)).astype(np.float32)
+    wi, f, pdf = gpu_ctx.bsdf_sample(mats, ids, wo, u)
+    wir, fr, pr = refcpu.bsdf_sample(mats, ids, wo, u, precision=1, fixed=1)
+    agree = (pdf > 0) == (pr > 0)
+    both = (pdf > 0) & (pr > 0) & (np.abs(wo[:, 2]) >= 2e-2)
+    print(f"Phong sample: valid agreement {agree.mean():.6f}, both valid {both.sum()}, valid fraction {(pr > 0).mean():.3f}")
+    assert agree.mean() > 0.999
+    d = np.linalg.norm(wi[both] - wir[both], axis=1)
+    print(f"direction abs diff: median {np.median(d):.3g}, p99 {np.quantile(d, 0.99):.3g}, max {d.max():.3g}")
+    assert np.quantile(d, 0.99) < 1e-5
+    assert np.quantile(d, 0.9999) < 1e-3

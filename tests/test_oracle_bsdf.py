@@ -227,3 +227,81 @@ def test_float_restatement_close_to_fp64():
     rel = np.abs(f32 - f64)[m] / f64[m]
     print(f"reference-precision Disney f: median rel err {np.median(rel):.2e}, max {rel.max():.2e}")
     assert np.median(rel) < 1e-5
+
+
+# ---- Phong (src/material/phong.cpp, phong_specular_bxdf.cpp) -------------------
+PHONG_CASES = [
+    dict(base_color=(0.5, 0.3, 0.2), specular_scale=(0.6, 0.2, 0.1), shininess=1.0),    # kd + ks > 1: normalised by 1.1
+    dict(base_color=(0.2, 0.4, 0.3), specular_scale=(0.3, 0.3, 0.3), shininess=12.0),
+    dict(base_color=(0.05, 0.05, 0.05), specular_scale=(0.8, 0.7, 0.6), shininess=60.0),
+]
+
+
+def _phong(**kw):
+    from tracer_amd.abi import TR_MAT_PHONG
+    return make_material(type=TR_MAT_PHONG, **kw)
+
+
+def test_phong_with_black_specular_is_lambert():
+    """ks = 0 makes the aggregate a single diffuse lobe of weight 1: the Phong
+    restatement then equals the Lambert one value for value and, in reference
+    mode, sample for sample (fixed mode re-evaluates the sampled direction,
+    so it agrees to rounding)."""
+    from oracle import refcpu
+    rng = np.random.default_rng(20)
+    kd = (0.3, 0.6, 0.9)
+    mats = [_phong(base_color=kd, specular_scale=0.0, shininess=7.0), make_material(type=TR_MAT_LAMBERT, base_color=kd)]
+    n = 4096
+    wi = rng.normal(size=(n, 3)).astype(np.float32)
+    wo = rng.normal(size=(n, 3)).astype(np.float32)
+    u = rng.uniform(0, 1, (n, 3)).astype(np.float32)
+    for fixed in (0, 1):
+        a = refcpu.bsdf_eval(mats, np.zeros(n, np.int32), wi, wo, fixed=fixed)
+        b = refcpu.bsdf_eval(mats, np.ones(n, np.int32), wi, wo, fixed=fixed)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+        sa = refcpu.bsdf_sample(mats, np.zeros(n, np.int32), wo, u, fixed=fixed)
+        sb = refcpu.bsdf_sample(mats, np.ones(n, np.int32), wo, u, fixed=fixed)
+        for x, y in zip(sa, sb):
+            assert np.array_equal(x, y) if fixed == 0 else np.allclose(x, y, rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("case", range(len(PHONG_CASES)))
+def test_phong_albedo_at_normal_incidence(case):
+    """For wo = n the specular lobe is centred on n, so the directional albedo
+    has the closed form kd' + ks' (ns + 1) / (ns + 2), with kd', ks' divided by
+    max(1, max_i(kd_i + ks_i)) (phong.cpp:36-41); the pdf integrates to
+    w_d + w_s = 1 over the hemisphere."""
+    from oracle import refcpu
+    c = PHONG_CASES[case]
+    kd, ks, ns = np.array(c["base_color"]), np.array(c["specular_scale"]), c["shininess"]
+    dem = max(1.0, (kd + ks).max())
+    want = kd / dem + ks / dem * (ns + 1) / (ns + 2)
+    h, dw = _hemisphere_grid(1500, 32, power=3)
+    wo = np.tile(np.float32([0, 0, 1]), (len(h), 1))
+    f, pdf = refcpu.bsdf_eval([_phong(**c)], np.zeros(len(h), np.int32), h.astype(np.float32), wo)
+    assert np.allclose((f * h[:, 2:3] * dw[:, None]).sum(0), want, rtol=1e-3)
+    assert abs((pdf * dw).sum() - 1.0) < 1e-3
+
+
+def test_phong_lobe_weights_follow_luminance():
+    """Away from the specular lobe the pdf is w_d cos / pi with
+    w_d = lum(kd) / (lum(kd) + lum(ks)), lum = 0.2126 R + 0.7152 G + 0.0722 B
+    (phong.cpp:43-53, spectrum.hpp:94-96); both black gives 1/2."""
+    from oracle import refcpu
+    wo = np.float32([[0.0, 0.0, 1.0]])
+    wi = np.float32([[0.96, 0.0, 0.28]])   # 73.7 degrees from the lobe axis: cos^200 underflows to ~1e-112
+    for kd, ks, wd in (((0.5, 0.3, 0.2), (0.1, 0.6, 0.3), None), ((0, 0, 0), (0, 0, 0), 0.5)):
+        if wd is None:
+            lum = lambda c: 0.2126 * c[0] + 0.7152 * c[1] + 0.0722 * c[2]
+            wd = lum(kd) / (lum(kd) + lum(ks))
+        _, pdf = refcpu.bsdf_eval([_phong(base_color=kd, specular_scale=ks, shininess=200.0)], np.zeros(1, np.int32),
+                                  wi, wo)
+        assert abs(pdf[0] / (0.28 / np.pi) - wd) < 1e-6   # float32 inputs
+
+
+@pytest.mark.parametrize("case", range(len(PHONG_CASES)))
+@pytest.mark.parametrize("wo", [(0.3, 0.2, 0.93), (0.8, -0.1, 0.59)])
+def test_phong_fixed_sampling_matches_pdf(case, wo):
+    wo = np.array(wo) / np.linalg.norm(wo)
+    p, lost = _chi2_sample_vs_pdf(_phong(**PHONG_CASES[case]), wo, fixed=1)
+    assert p > 1e-4, p

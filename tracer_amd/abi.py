@@ -10,7 +10,7 @@ import ctypes as C
 import numpy as np
 
 TR_OK, TR_ERR_INVALID, TR_ERR_HIP, TR_ERR_OOM, TR_ERR_STATE = range(5)
-TR_MAT_LAMBERT, TR_MAT_DISNEY = 0, 1
+TR_MAT_LAMBERT, TR_MAT_DISNEY, TR_MAT_PHONG = 0, 1, 2
 TR_FILTER_BOX, TR_FILTER_GAUSSIAN = 0, 1
 TR_TR_ANALYTIC, TR_TR_RATIO = 0, 1
 
@@ -26,6 +26,7 @@ class TrMaterial(C.Structure):
         ("specular_scale", f3),
         ("specular_tint", C.c_float), ("anisotropic", C.c_float), ("sheen", C.c_float),
         ("sheen_tint", C.c_float), ("clearcoat", C.c_float), ("clearcoat_gloss", C.c_float),
+        ("shininess", C.c_float),
     ]
 
 
@@ -130,7 +131,7 @@ def dptr(a: np.ndarray | None):
 
 MATERIAL_FIELDS = ["type", "base_color", "metallic", "roughness", "transmission", "transmission_roughness", "ior",
                    "specular_scale", "specular_tint", "anisotropic", "sheen", "sheen_tint", "clearcoat",
-                   "clearcoat_gloss"]
+                   "clearcoat_gloss", "shininess"]
 
 
 def make_material(**kw) -> TrMaterial:
@@ -142,7 +143,8 @@ def make_material(**kw) -> TrMaterial:
             v = (v, v, v)
         setattr(m, k, f3(*map(float, v)))
     defaults = dict(metallic=0.0, roughness=0.5, transmission=0.0, transmission_roughness=0.5, ior=1.5,
-                    specular_tint=0.0, anisotropic=0.0, sheen=0.0, sheen_tint=0.5, clearcoat=0.0, clearcoat_gloss=1.0)
+                    specular_tint=0.0, anisotropic=0.0, sheen=0.0, sheen_tint=0.5, clearcoat=0.0, clearcoat_gloss=1.0,
+                    shininess=1.0)
     for k, v in defaults.items():
         setattr(m, k, float(kw.get(k, v)))
     return m

@@ -38,7 +38,7 @@ HIP_SOURCES = {
     "guide_capi.cpp": ["-x", "hip"],
 }
 CXX_SOURCES = {"bvh_build.cpp": []}
-FACADE_SOURCES = ["facade.cpp", "scenes.cpp"]
+FACADE_SOURCES = ["facade.cpp", "scenes.cpp", "obj_load.cpp"]
 
 
 def _run(cmd: list[str]) -> None:

@@ -19,7 +19,7 @@ using namespace trhip;
 using namespace trcapi;
 
 // The ctypes bindings (tracer_amd/abi.py) assume these layouts.
-static_assert(sizeof(tr_material) == 72, "tr_material layout");
+static_assert(sizeof(tr_material) == 76, "tr_material layout");
 static_assert(sizeof(tr_medium) == 32, "tr_medium layout");
 static_assert(sizeof(tr_camera) == 52, "tr_camera layout");
 static_assert(sizeof(tr_render_params) == 88, "tr_render_params layout");
@@ -29,7 +29,41 @@ static_assert(sizeof(tr_guide_config) == 48, "tr_guide_config layout");
 namespace {
 
 // DisneyBSDF constructor (src/material/disney.cpp:347-399) evaluated once per material.
+void check_material(const tr_material& m) {
+    if (m.type != TR_MAT_LAMBERT && m.type != TR_MAT_DISNEY && m.type != TR_MAT_PHONG)
+        throw InvalidArg("unknown material type");
+    if (m.type == TR_MAT_PHONG) {
+        if (!(m.shininess >= 0.0f && m.shininess < 1e30f)) throw InvalidArg("Phong shininess must be finite and >= 0");
+        for (int k = 0; k < 3; ++k)
+            if (!(m.base_color[k] >= 0.0f && m.specular_scale[k] >= 0.0f)) throw InvalidArg("Phong kd and ks must be >= 0");
+    }
+}
+
+// Phong::shading (src/material/phong.cpp:29-60) with constant textures: kd
+// and ks divided by max(1, max_i(kd_i + ks_i)), lobe weights by luminance
+// (0.5 each when both are black).  The aggregate's albedo kd + ks is what the
+// black-fringe fallback uses (src/material/aggregate_bsdf.hpp:19-23).
+static DevMaterial make_phong(const tr_material& m) {
+    DevMaterial d{};
+    d.type = MAT_PHONG;
+    f3 kd = mk(m.base_color[0], m.base_color[1], m.base_color[2]);
+    f3 ks = mk(m.specular_scale[0], m.specular_scale[1], m.specular_scale[2]);
+    const float dem = std::max(1.0f, max3(kd + ks));
+    kd = kd * (1.0f / dem);
+    ks = ks * (1.0f / dem);
+    const float ld = lum(kd), ls = lum(ks);
+    d.w_diffuse = ld + ls > kEps ? ld / (ld + ls) : 0.5f;
+    d.w_spec = 1.0f - d.w_diffuse;
+    d.C = kd + ks;
+    d.Ctint = kd;
+    d.spec_scale = ks;
+    d.ns = m.shininess;
+    d.guide_share = d.w_diffuse;   // the guide shares the broad (diffuse) lobe's selection weight
+    return d;
+}
+
 DevMaterial make_material(const tr_material& m) {
+    if (m.type == TR_MAT_PHONG) return make_phong(m);
     DevMaterial d{};
     d.type = m.type == TR_MAT_DISNEY ? MAT_DISNEY : MAT_LAMBERT;
     d.C = mk(m.base_color[0], m.base_color[1], m.base_color[2]);
@@ -128,10 +162,7 @@ void validate_scene(const tr_scene_desc* d) {
             if (mo < -1 || mo >= d->num_media) throw InvalidArg("invalid medium_outside id");
         }
     }
-    for (int i = 0; i < d->num_materials; ++i) {
-        const int ty = d->materials[i].type;
-        if (ty != TR_MAT_LAMBERT && ty != TR_MAT_DISNEY) throw InvalidArg("unknown material type");
-    }
+    for (int i = 0; i < d->num_materials; ++i) check_material(d->materials[i]);
     for (int i = 0; i < d->num_media; ++i) {
         const tr_medium& m = d->media[i];
         if (!(m.g > -1.0f && m.g < 1.0f)) throw InvalidArg("medium g must lie in (-1, 1)");  // homogeneous.cpp:16
@@ -568,7 +599,7 @@ static void bsdf_common(tr_ctx* c, const tr_material* mats, int32_t num_mats, co
     if (!mats || num_mats <= 0 || !ids || n < 0) throw InvalidArg("bad BSDF batch");
     std::vector<DevMaterial> dm(static_cast<size_t>(num_mats));
     for (int i = 0; i < num_mats; ++i) {
-        if (mats[i].type != TR_MAT_LAMBERT && mats[i].type != TR_MAT_DISNEY) throw InvalidArg("unknown material type");
+        check_material(mats[i]);
         dm[size_t(i)] = make_material(mats[i]);
     }
     for (int64_t i = 0; i < n; ++i)

@@ -1,11 +1,13 @@
-// Materials on the device: Disney BSDF and Lambert, evaluated in the shading
-// frame (shading normal = +z) with the geometric normal given in that frame.
+// Materials on the device: Disney BSDF, Lambert and Phong, evaluated in the
+// shading frame (shading normal = +z) with the geometric normal given in that
+// frame.
 //
 // Replaces DisneyBSDF (src/material/disney.cpp:14-548) with its helpers
-// (src/material/utility/microfacet.cpp, src/utility/reflection.hpp) and the
-// Phong(ks = 0) Lambert AggregateBSDF (src/material/phong.cpp:29-70,
-// src/material/aggregate_bsdf.hpp, src/material/bxdf/diffuse_bxdf.cpp), plus
-// the black-fringe handling of LocalBSDF (src/core/bsdf.hpp:114-165).
+// (src/material/utility/microfacet.cpp, src/utility/reflection.hpp), the
+// Phong AggregateBSDF (src/material/phong.cpp:29-70,
+// src/material/aggregate_bsdf.hpp, src/material/bxdf/diffuse_bxdf.cpp,
+// src/material/bxdf/phong_specular_bxdf.cpp) — Lambert is its ks = 0 case —
+// plus the black-fringe handling of LocalBSDF (src/core/bsdf.hpp:114-165).
 //
 // The formulas are the reference's, rewritten for FP32 accuracy and without
 // transcendental round trips:
@@ -25,10 +27,13 @@
 
 namespace trhip {
 
-enum : int { MAT_LAMBERT = 0, MAT_DISNEY = 1 };
+enum : int { MAT_LAMBERT = 0, MAT_DISNEY = 1, MAT_PHONG = 2 };
 
 // Per-material constants, precomputed on the host from the create_disney
 // parameters exactly as the DisneyBSDF constructor does (disney.cpp:347-399).
+// Phong uses C = kd + ks (the aggregate albedo of the black-fringe fallback),
+// Ctint = kd, spec_scale = ks, ns, w_diffuse and w_spec, all after the
+// energy normalisation of Phong::shading.
 struct DevMaterial {
     int type;
     float metallic, roughness, transmission;
@@ -39,6 +44,7 @@ struct DevMaterial {
     float tax, tay, cc_rough, w_diffuse;
     float w_trans, w_spec, w_cc;
     float guide_share;   // guide mixture probability at this material, as a share of the guide fraction (host, make_material)
+    float ns;            // Phong exponent
 };
 
 struct BsdfSample {
@@ -317,6 +323,15 @@ struct BsdfEval {
     float pdf;
 };
 
+// PhongSpecularBXDF (phong_specular_bxdf.cpp:7-40): (ns + 1) / (2 pi) cos^ns(a)
+// with a the angle between wi and the mirror direction of wo about the
+// shading normal, reflect(wo, z) = (-wo.x, -wo.y, wo.z); zero when
+// cos(a) <= 0.  Both the value (times ks) and the pdf of the lobe.
+TR_HD float phong_lobe(f3 lwi, f3 lwo, float ns) {
+    const float c = lwi.z * lwo.z - lwi.x * lwo.x - lwi.y * lwo.y;
+    return c > 0.0f ? (ns + 1.0f) * (0.5f * kInvPi) * powf(c, ns) : 0.0f;
+}
+
 // eval + pdf for directions wi, wo (shading frame), geometric normal ng.
 TR_HD BsdfEval bsdf_eval(const DevMaterial& m, f3 wi, f3 wo, f3 ng) {
     BsdfEval r{f3s(0.0f), 0.0f};
@@ -331,6 +346,14 @@ TR_HD BsdfEval bsdf_eval(const DevMaterial& m, f3 wi, f3 wo, f3 ng) {
         if (lwi.z <= 0.0f || lwo.z <= 0.0f) return r;
         r.f = m.C * (kInvPi * normal_correct(ng, wi));
         r.pdf = lwi.z * kInvPi;
+        return r;
+    }
+    if (m.type == MAT_PHONG) {
+        // both lobes vanish unless wi and wo lie above the shading surface
+        if (lwi.z <= 0.0f || lwo.z <= 0.0f) return r;
+        const float ps = phong_lobe(lwi, lwo, m.ns);
+        r.f = (m.Ctint * kInvPi + m.spec_scale * ps) * normal_correct(ng, wi);
+        r.pdf = m.w_diffuse * lwi.z * kInvPi + m.w_spec * ps;
         return r;
     }
     if (fabsf(lwi.z) < kEps || fabsf(lwo.z) < kEps) return r;
@@ -375,6 +398,38 @@ TR_HD BsdfSample bsdf_sample(const DevMaterial& m, f3 wo, f3 ng, float su, float
         s.wi = wi;
         s.f = m.C * (kInvPi * normal_correct(ng, wi));
         s.pdf = pd;
+        return s;
+    }
+    if (m.type == MAT_PHONG) {
+        if (lwo.z <= 0.0f) return s;
+        if (su < m.w_diffuse) {
+            // DiffuseBXDF::sample (diffuse_bxdf.cpp:20-31)
+            wi = cosine_hemisphere(u, v);
+            if (wi.z * kInvPi < 0.001f) return s;
+            wi = normalize(wi);
+        } else {
+            // PowCosineSampleHemisphere about the mirror direction
+            // (phong_specular_bxdf.cpp:7-17, :55-69) in the frame that
+            // Coord::create_from_z builds (coordinate.hpp:84-93), so a
+            // uniform triple maps to the reference's direction; below the
+            // surface: no sample
+            const f3 z = normalize(mk(-lwo.x, -lwo.y, lwo.z));
+            const f3 y = fabsf(z.x) > fabsf(z.y) ? normalize(mk(-z.z, 0.0f, z.x)) : normalize(mk(0.0f, z.z, -z.y));
+            const f3 x = cross(y, z);
+            const float ct = powf(u, 1.0f / (m.ns + 1.0f));
+            const float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+            float sp, cp;
+            sincosf(2.0f * kPi * v, &sp, &cp);
+            wi = normalize(x * (st * cp) + y * (st * sp) + z * ct);
+            if (wi.z <= 0.0f) return s;
+        }
+        // the aggregate's value and pdf at wi: both lobes, weighted
+        // (aggregate_bsdf.hpp:84-98); a wi below the geometric surface takes
+        // the black-fringe path of eval
+        const BsdfEval e = bsdf_eval(m, wi, wo, ng);
+        s.wi = wi;
+        s.f = e.f;
+        s.pdf = e.pdf;
         return s;
     }
     if (fabsf(lwo.z) < kEps) return s;

@@ -157,6 +157,13 @@ RC<Material> create_phong_material(RC<const Texture2D> ka, RC<const Texture2D> k
     return m;
 }
 
+RC<Material> create_glass(RC<const Texture2D> kr, RC<const Texture2D> kt, RC<const Texture2D> ior) {
+    if (!kr || !kt || !ior) throw std::runtime_error("create_glass: texture must not be null");
+    auto m = newRC<GlassMaterial>();
+    m->kr = kr; m->kt = kt; m->ior = ior;
+    return m;
+}
+
 // triangle.cpp:216-236
 std::vector<RC<Shape>> create_triangle_mesh(const mesh_t& mesh, const Transform& t) {
     if (mesh.indices.size() % 3 != 0) throw std::runtime_error("create_triangle_mesh: index count not a multiple of 3");
@@ -305,15 +312,25 @@ static tr_material convert_material(const Material* m) {
         return r;
     }
     if (auto p = dynamic_cast<const PhongMaterial*>(m)) {
-        if (!p->kd->is_constant() || !p->ks->is_constant()) throw std::runtime_error("Phong: only constant textures are supported");
+        if (!p->kd->is_constant() || !p->ks->is_constant() || !p->ns->is_constant())
+            throw std::runtime_error("Phong: only constant textures are supported");
         const Spectrum kd = p->kd->evaluate(Point2f(0, 0)), ks = p->ks->evaluate(Point2f(0, 0));
-        if (!ks.is_back()) throw std::runtime_error("Phong: the GPU path supports the Lambert case (ks = 0) only");
-        // phong.cpp:36-41: energy conservation divides by max(1, max_i(kd_i + ks_i))
+        if (!ks.is_back()) {
+            // diffuse + Phong specular lobe; the library normalises and weighs them
+            r.type = TR_MAT_PHONG;
+            r.base_color[0] = kd.r; r.base_color[1] = kd.g; r.base_color[2] = kd.b;
+            r.specular_scale[0] = ks.r; r.specular_scale[1] = ks.g; r.specular_scale[2] = ks.b;
+            r.shininess = tex_s(p->ns);
+            return r;
+        }
+        // ks = 0: a single diffuse lobe.  phong.cpp:36-41: energy conservation
+        // divides by max(1, max_i(kd_i + ks_i))
         const float dem = std::max({1.0f, kd.r + ks.r, kd.g + ks.g, kd.b + ks.b});
         r.type = TR_MAT_LAMBERT;
         r.base_color[0] = kd.r / dem; r.base_color[1] = kd.g / dem; r.base_color[2] = kd.b / dem;
         return r;
     }
+    if (dynamic_cast<const GlassMaterial*>(m)) throw std::runtime_error("glass (create_glass) is not supported by the GPU path");
     throw std::runtime_error("unsupported material");
 }
 

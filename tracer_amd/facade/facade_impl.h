@@ -27,6 +27,10 @@ class PhongMaterial : public Material {
 public:
     RC<const Texture2D> ka, kd, ks, ns;
 };
+class GlassMaterial : public Material {
+public:
+    RC<const Texture2D> kr, kt, ior;
+};
 
 // triangle.cpp:15-33 — world-space vertices of one mesh
 struct TriangleMesh {

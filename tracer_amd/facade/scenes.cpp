@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <stdexcept>
 #include <string>
 
@@ -490,6 +491,47 @@ static void config5(Config& c, uint64_t seed, double detail) {
     finish(c, cam, nullptr);
 }
 
+// A scene from an OBJ file, wired the way run() does it (src/main.cpp:212-282):
+// every MTL material becomes a Phong material (a glass when Ni > 1, which the
+// GPU path rejects), emissive materials make their triangles area lights, all
+// triangles get MediumInterface{inside = vacuum, outside = fog} with the fog
+// of main.cpp:242 (or vacuum outside when fog is false), and the BVH has 3
+// primitives per leaf.
+static void obj_scene(Config& c, const std::string& path, const tr_camera& tc, bool fog) {
+    const model_t model = load_model_from_file(path);
+    std::vector<MaterialTexture> res;
+    std::vector<RC<Material>> mats;
+    for (const material_t& m : model.material) {
+        res.push_back(create_texture_from_file(m));
+        const MaterialTexture& t = res.back();
+        mats.push_back(t.as_glass ? create_glass(t.map_ks, t.map_kt, t.ior)
+                                  : create_phong_material(t.map_ka, t.map_kd, t.map_ks, t.map_ns));
+    }
+    Builder& b = c.builder;
+    b.mi.inside = create_vacuum();
+    b.mi.outside = fog ? create_homogeneous_medium(Spectrum(0.03f), Spectrum(0.3f), 0.8f, 25) : create_vacuum();
+    for (const mesh_t& mesh : model.mesh) {
+        auto tris = create_triangle_mesh(mesh, Transform());
+        for (size_t i = 0; i < tris.size(); ++i) {
+            const int id = mesh.materials[i];
+            // main.cpp:263 asserts a valid material per triangle
+            if (id < 0 || size_t(id) >= mats.size())
+                throw std::runtime_error("OBJ scene: triangle " + std::to_string(i) + " of mesh '" + mesh.name + "' has no material");
+            const Spectrum e = res[size_t(id)].has_emission ? res[size_t(id)].map_ke->evaluate(Point2f()) : Spectrum(0);
+            auto p = create_geometric_primitive(tris[i], mats[size_t(id)], b.mi, e);
+            if (auto l = p->as_area_light()) b.lights.push_back(l);
+            b.prims.push_back(std::move(p));
+        }
+    }
+    c.max_leaf = 3;
+    c.filter = create_gaussin_filter(0.5f, 0.6f);
+    auto cam = create_thin_lens_camera(tc.aspect, Point3f(tc.pos[0], tc.pos[1], tc.pos[2]),
+                                       Point3f(tc.target[0], tc.target[1], tc.target[2]),
+                                       Vector3f(tc.up[0], tc.up[1], tc.up[2]), tc.fov_y, tc.lens_radius,
+                                       tc.focal_distance);
+    finish(c, cam, nullptr);
+}
+
 }  // namespace scenes
 }  // namespace tracer
 
@@ -499,6 +541,8 @@ struct trs_scene {
     tr_render_params defaults{};
     int max_leaf = 3;
 };
+
+static trs_scene* make_trs(tracer::scenes::Config& c, uint64_t seed);
 
 extern "C" {
 
@@ -515,29 +559,21 @@ trs_scene* trs_create(int32_t config, uint64_t seed, double detail, const char**
             case 5: tracer::scenes::config5(c, seed, detail); break;
             default: throw std::runtime_error("config must be 1..5");
         }
-        auto* s = new trs_scene();
-        s->flat = tracer::detail::flatten(*c.scene);
-        s->desc = s->flat.desc();
-        s->max_leaf = c.max_leaf;
-        tr_render_params& p = s->defaults;
-        p.width = c.width;
-        p.height = c.height;
-        p.spp = c.spp;
-        p.min_depth = c.min_depth;
-        p.max_depth = c.max_depth;
-        p.direct_light_sample_num = 1;
-        if (auto* g = dynamic_cast<const tracer::detail::GaussianFilter*>(c.filter.get())) {
-            p.filter_type = TR_FILTER_GAUSSIAN;
-            p.filter_radius = g->r;
-            p.filter_alpha = g->alpha;
-        } else {
-            p.filter_type = TR_FILTER_BOX;
-            p.filter_radius = c.filter->radius();
-        }
-        p.seed = seed;
-        p.guide_fraction = 0.5f;
-        p.guide_lr = 1e-3f;
-        return s;
+        return make_trs(c, seed);
+    } catch (const std::exception& e) {
+        msg = e.what();
+        if (err) *err = msg.c_str();
+        return nullptr;
+    }
+}
+
+trs_scene* trs_load_obj(const char* path, const tr_camera* camera, int32_t outside_fog, const char** err) {
+    static thread_local std::string msg;
+    try {
+        if (!path || !camera) throw std::runtime_error("trs_load_obj: path and camera must not be NULL");
+        tracer::scenes::Config c;
+        tracer::scenes::obj_scene(c, path, *camera, outside_fog != 0);
+        return make_trs(c, 0);
     } catch (const std::exception& e) {
         msg = e.what();
         if (err) *err = msg.c_str();
@@ -552,3 +588,31 @@ int32_t trs_max_leaf(const trs_scene* s) { return s ? s->max_leaf : 3; }
 void trs_destroy(trs_scene* s) { delete s; }
 
 }  // extern "C"
+
+// Flattens a built scene and records its render defaults; throws on
+// content the GPU path does not support.
+static trs_scene* make_trs(tracer::scenes::Config& c, uint64_t seed) {
+    auto s = std::make_unique<trs_scene>();
+    s->flat = tracer::detail::flatten(*c.scene);
+    s->desc = s->flat.desc();
+    s->max_leaf = c.max_leaf;
+    tr_render_params& p = s->defaults;
+    p.width = c.width;
+    p.height = c.height;
+    p.spp = c.spp;
+    p.min_depth = c.min_depth;
+    p.max_depth = c.max_depth;
+    p.direct_light_sample_num = 1;
+    if (auto* g = dynamic_cast<const tracer::detail::GaussianFilter*>(c.filter.get())) {
+        p.filter_type = TR_FILTER_GAUSSIAN;
+        p.filter_radius = g->r;
+        p.filter_alpha = g->alpha;
+    } else {
+        p.filter_type = TR_FILTER_BOX;
+        p.filter_radius = c.filter->radius();
+    }
+    p.seed = seed;
+    p.guide_fraction = 0.5f;
+    p.guide_lr = 1e-3f;
+    return s.release();
+}

@@ -92,6 +92,8 @@ def scenes_lib() -> C.CDLL:
         lib = C.CDLL(str(p))
         lib.trs_create.argtypes = [C.c_int32, C.c_uint64, C.c_double, C.POINTER(C.c_char_p)]
         lib.trs_create.restype = C.c_void_p
+        lib.trs_load_obj.argtypes = [C.c_char_p, C.POINTER(abi.TrCamera), C.c_int32, C.POINTER(C.c_char_p)]
+        lib.trs_load_obj.restype = C.c_void_p
         lib.trs_desc.argtypes = [C.c_void_p]
         lib.trs_desc.restype = C.POINTER(TrSceneDesc)
         lib.trs_render_defaults.argtypes = [C.c_void_p, C.POINTER(TrRenderParams)]
@@ -195,6 +197,21 @@ def config_scene(config: int, seed: int | None = None, detail: float = 1.0) -> S
     h = lib.trs_create(config, config - 1 if seed is None else seed, detail, C.byref(err))
     if not h:
         raise TracerError(abi.TR_ERR_INVALID, (err.value or b"").decode())
+    return _scene_from_handle(lib, h)
+
+
+def obj_scene(path, camera: abi.TrCamera, outside_fog: bool = True) -> Scene:
+    """A Wavefront OBJ scene read by the C++ facade (load_model_from_file,
+    create_texture_from_file) and wired as the reference's run() wires one."""
+    lib = scenes_lib()
+    err = C.c_char_p()
+    h = lib.trs_load_obj(str(path).encode(), C.byref(camera), 1 if outside_fog else 0, C.byref(err))
+    if not h:
+        raise TracerError(abi.TR_ERR_INVALID, (err.value or b"").decode())
+    return _scene_from_handle(lib, h)
+
+
+def _scene_from_handle(lib, h) -> Scene:
     try:
         d = lib.trs_desc(h).contents
         nv, nt = d.num_vertices, d.num_triangles
