@@ -19,7 +19,11 @@
 //   quirks = 1 (Reference): the reference's estimator as written, including
 //     Q1 (medium segment sampled from the camera-ray origin), Q4 (scatter
 //     weight 1), Q5 (MIS-BSDF light hit attenuated by Tr(ray.o, isect.pos)),
-//     Q6 (no environment NEE at medium vertices) and the Disney Q2/Q3/Q21.
+//     Q6 (no environment NEE at medium vertices), Q23 (emission of the
+//     surface a segment reaches added before the segment's medium sample),
+//     Q26 (no direct light at a
+//     vertex whose continuation sample fails), the Disney Q2/Q3/Q21 and the
+//     Phong/Lambert Q22/Q24.
 //   quirks = 0 (Fixed): the same loop with those defects corrected.
 //   nee = 0: every light sampled at every vertex (reference); nee = 1: one
 //     light picked by power (compute_light_power_distribution,
@@ -477,14 +481,21 @@ struct Renderer {
             if (!coef.is_finite()) break;
             Isect is;
             bool found = intersect(o, d, 0, std::numeric_limits<float>::max(), &is);
+            // Q23: the reference adds the emission of the surface it reached
+            // before it samples the medium of the segment (path_trace_renderer.cpp:66-80
+            // before :84-122), so an emitter seen through a medium is not
+            // attenuated.  Fixed: the emission is added after the medium
+            // decision, only when the segment reaches the surface.
+            RGB<float> Le_hit(0);
             if (depth == 0 || specular) {
                 if (found) {
                     RGB<float> Le = s.tri_emission(is.prim);
-                    if (!Le.is_black() && dot(is.geo.z, -d) > 0) L += coef * Le;
+                    if (!Le.is_black() && dot(is.geo.z, -d) > 0) Le_hit = Le;
                 } else if (s.has_env) {
                     L += coef * s.env;
                 }
             }
+            if (c.quirks) L += coef * Le_hit;
             if (!found) break;
             const int mid = wo_medium_id(is);
             const tr_medium* m = med(mid);
@@ -518,12 +529,18 @@ struct Renderer {
                 coef *= medium_tr(m, o, is.pos);
                 scattering = 0;
             }
+            if (!c.quirks) L += coef * Le_hit;
             SurfBSDF bsdf;
             make_bsdf(s, is, c.quirks, bsdf);
             float a = sp.get(), b = sp.get(), cc = sp.get();
             BSDFSample<float> bs = bsdf.sample(is.wo, a, b, cc);
-            if (bs.f.is_black() || bs.pdf < kEps) break;
-            specular = false;   // Disney/Lambert are never delta
+            // Q26: the reference ends the path before its direct-light estimate
+            // when the continuation sample fails (path_trace_renderer.cpp:128-131
+            // before :141-153), losing that vertex's direct light.  Fixed: the
+            // direct light is estimated first and the path ends after it.
+            const bool cont_ok = !(bs.f.is_black() || bs.pdf < kEps);
+            if (!cont_ok && c.quirks) break;
+            specular = false;   // Disney/Lambert/Phong are never delta
             auto evalf = [&](const V3f& wi) { return bsdf.eval(wi, is.wo); };
             auto pdff = [&](const V3f& wi) { return bsdf.pdf(wi, is.wo); };
             auto samplef = [&](float x, float y, float z) { return bsdf.sample(is.wo, x, y, z); };
@@ -533,6 +550,7 @@ struct Renderer {
                 di += coef * sample_bsdf_mis(is.pos, is.geo.z, true, &is, nullptr, samplef, sp);
             }
             L += di * (1.0f / c.nls);
+            if (!cont_ok) break;
             coef *= bs.f * (std::abs(cosv(is.geo.z, bs.wi)) / bs.pdf);
             if (!coef.is_finite()) break;
             V3f nd = bs.wi.normalize();

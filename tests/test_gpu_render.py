@@ -182,12 +182,14 @@ def test_tail_kernel_matches_per_bounce_kernels(gpu_ctx, config, detail):
 
 def test_c1_phong_matches_fixed_estimator(gpu_ctx):
     """Configuration 1 with every face a Phong material with a specular lobe
-    (phong.cpp with ks = 0.3, ns = 20), against the oracle in fixed mode."""
+    (phong.cpp with ks = 0.3, ns = 20), against the oracle in fixed mode.  The
+    glossy lobes make per-seed tile means heavy-tailed at low spp, so each
+    seed renders 256 spp on the CPU."""
     from tracer_amd import abi, native
     s = native.config_scene(1)
     s.materials = [abi.make_material(type=abi.TR_MAT_PHONG, base_color=tuple(m.base_color), specular_scale=0.3,
                                      shininess=20.0) for m in s.materials]
-    _compare_estimators(gpu_ctx, s, 32, 32, 512, 64, 8, 8, quirks=False, nee=0)
+    _compare_estimators(gpu_ctx, s, 32, 32, 1024, 256, 8, 8, quirks=False, nee=0)
 
 
 @pytest.mark.parametrize("fog", [False, True])
@@ -200,5 +202,5 @@ def test_obj_room_matches_fixed_estimator(gpu_ctx, tmp_path, fog):
     obj, mtl = objfiles.box_room_obj()
     path = objfiles.write(tmp_path, "room.obj", obj, mtl)
     s = native.obj_scene(path, objfiles.camera((0.0, 0.0, 3.6), (0.0, 0.0, 0.0)), outside_fog=fog)
-    _compare_estimators(gpu_ctx, s, 32, 32, 256 if fog else 512, 32 if fog else 64, 8, 8, quirks=False, nee=0,
+    _compare_estimators(gpu_ctx, s, 32, 32, 512 if fog else 1024, 128 if fog else 256, 8, 8, quirks=False, nee=0,
                         max_depth=8)

@@ -294,9 +294,11 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
                     fs = bs.f;
                     pdf = bs.pdf;
                 }
-                // Next-event estimation does not depend on the continuation sample: the
-                // reference estimates direct light before it samples the continuation
-                // and ends the path only afterwards (path_trace_renderer.cpp:141-163).
+                // Next-event estimation does not depend on the continuation sample.  The
+                // reference ends the path before estimating direct light when the
+                // continuation sample fails (path_trace_renderer.cpp:128-131 before
+                // :141-153), which loses that vertex's direct light; here the direct
+                // light is estimated at every vertex and the path ends afterwards.
                 {
                     // next-event estimation (direct_illumination.cpp:35-109)
                     if (s.num_light_entries > 0) {
