@@ -31,9 +31,9 @@ trs_scene* trs_create(int32_t config, uint64_t seed, double detail, const char**
  * reference's run() does (src/main.cpp:212-282): Phong materials, emissive
  * materials as area lights, MediumInterface{vacuum inside, fog outside} (the
  * fog of src/main.cpp:242 when outside_fog != 0, else vacuum), Gaussian
- * filter (0.5, 0.6), 3 primitives per BVH leaf.  Returns NULL and sets *err
- * on a read or parse error, a triangle without a material, an image texture
- * or a glass material (Ni > 1). */
+ * filter (0.5, 0.6), 3 primitives per BVH leaf; a material with Ni > 1 is a
+ * glass.  Returns NULL and sets *err on a read or parse error, a triangle
+ * without a material or an image texture. */
 trs_scene* trs_load_obj(const char* obj_path, const tr_camera* camera, int32_t outside_fog, const char** err);
 const tr_scene_desc* trs_desc(const trs_scene* s);
 /* Film, spp and depths of the configuration (PTRendererParams + film). */

@@ -10,7 +10,7 @@
 //   create_geometric_primitive  src/factory/primivite.hpp:12-14
 //   create_disney               src/factory/material.hpp:37-52
 //   create_phong_material       src/factory/material.hpp:19-24
-//   create_glass                src/factory/material.hpp:13-17 (declared; the GPU path rejects it)
+//   create_glass                src/factory/material.hpp:13-17
 //   load_model_from_file        src/utility/mesh_load.hpp:93, mesh_load.cpp:25-113
 //   create_texture_from_file    src/utility/image_file.hpp:23-45, image_file.cpp:139-166 (constant colours)
 //   create_constant_texture2d   src/factory/texture.hpp:12
@@ -211,8 +211,8 @@ RC<Material> create_disney(RC<const Texture2D> base_color, RC<const Texture2D> m
                            Box<const NormalMapper> normal_mapper, RC<const BSSRDFSurface> bssrdf);
 RC<Material> create_phong_material(RC<const Texture2D> map_ka, RC<const Texture2D> map_kd, RC<const Texture2D> map_ks,
                                    RC<const Texture2D> map_ns);
-// Specular glass; constructed so that scene code written for the reference
-// compiles, but the GPU renderer throws for a scene that uses it.
+// Specular dielectric (src/material/glass.cpp): reflection colour map_kr,
+// refraction colour sqrt(map_kt), index of refraction map_ior.
 RC<Material> create_glass(RC<const Texture2D> map_kr, RC<const Texture2D> map_kt, RC<const Texture2D> map_ior);
 
 // image_file.hpp:23-43: the textures of one MTL material.

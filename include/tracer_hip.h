@@ -47,8 +47,12 @@ typedef struct tr_ctx tr_ctx;
  * (src/factory/material.hpp:19-24) with constant textures: base_color = kd,
  * specular_scale = ks, shininess = ns as given; the library applies the
  * energy normalisation and lobe weights of Phong::shading
- * (src/material/phong.cpp:36-53). */
-enum { TR_MAT_LAMBERT = 0, TR_MAT_DISNEY = 1, TR_MAT_PHONG = 2 };
+ * (src/material/phong.cpp:36-53).  TR_MAT_GLASS is create_glass(kr, kt, ior)
+ * (src/factory/material.hpp:13-17, src/material/glass.cpp) with constant
+ * textures: base_color = kr, specular_scale = kt, ior > 0; the refracted
+ * colour is sqrt(kt) as Glass::shading computes it (pass kt = kr for the
+ * reference's map_kr == map_kt case). */
+enum { TR_MAT_LAMBERT = 0, TR_MAT_DISNEY = 1, TR_MAT_PHONG = 2, TR_MAT_GLASS = 3 };
 
 typedef struct tr_material {
     int32_t type;

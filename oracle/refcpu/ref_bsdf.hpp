@@ -29,6 +29,8 @@
 //          (aggregate_bsdf.hpp:62-76), so the diffuse lobe is chosen with
 //          probability w_d + 0.001 while the returned pdf weighs it by w_d.
 //          Fixed: u * sum(w) <= w_d.  Q22 applies as for LambertRef.
+//  * GlassRef<T>: Glass / GlassBSDF (src/material/glass.cpp:7-146), a pair
+//    of delta lobes: reflection with probability F, refraction otherwise.
 //  * HGRef<T>: Henyey-Greenstein phase function,
 //    src/medium/phase_function.hpp:12-77.
 #pragma once
@@ -41,6 +43,7 @@ struct BSDFSample {
     V3<T> wi;
     RGB<T> f;
     T pdf = 0;
+    bool delta = false;   // BSDFSampleResult::is_delta (bsdf.hpp:17-25)
     bool valid() const { return pdf > 0 && !f.is_black(); }  // bsdf.hpp:90-92
 };
 
@@ -623,6 +626,51 @@ public:
             r.pdf = pdf_own * ws + ((lwi.z <= 0 || lwo.z <= 0) ? T(0) : lwi.z * kInvPi<T>) * wd;
         }
         r.f = r.f * normal_correct_factor(geo.z, shd.z, wi);
+        return r;
+    }
+};
+
+// Glass(kr, kt, ior): GlassBSDF with color_reflect = kr and color_refract =
+// sqrt(kt) (glass.cpp:96-112; kt = kr when map_kr == map_kt), Fresnel of
+// DielectricFresnelPoint(ior, 1) (fresnel_point.hpp).  eval and pdf are 0.
+template <class T>
+class GlassRef {
+public:
+    Coord<T> geo, shd;
+    RGB<T> kr, kt;
+    T ior;
+    GlassRef(const Coord<T>& g, const Coord<T>& s, const RGB<T>& kr_in, const RGB<T>& kt_in, T ior_in)
+        : geo(g), shd(s), kr(kr_in), kt(sqrt(kt_in)), ior(ior_in) {}
+    RGB<T> eval(const V3<T>&, const V3<T>&) const { return RGB<T>(0); }
+    T pdf(const V3<T>&, const V3<T>&) const { return 0; }
+    // glass.cpp:24-67
+    BSDFSample<T> sample(const V3<T>& wo, T su, T, T) const {
+        BSDFSample<T> r;
+        r.delta = true;
+        const V3<T> lwo = shd.to_local(wo).normalize();
+        const V3<T> nor = lwo.z > 0 ? V3<T>(0, 0, 1) : V3<T>(0, 0, -1);
+        const T fr = dielectric_fresnel(ior, T(1), lwo.z);
+        V3<T> lwi;
+        RGB<T> col;
+        T w;
+        if (su < fr) {
+            lwi = V3<T>(-lwo.x, -lwo.y, lwo.z);
+            col = kr;
+            w = fr;
+        } else {
+            const T eta_i = lwo.z > 0 ? T(1) : ior;
+            const T eta_t = lwo.z > 0 ? ior : T(1);
+            const T eta = eta_i / eta_t;
+            V3<T> t;
+            if (!refract_dir(lwo, nor, eta, false, t)) return r;
+            lwi = t.normalize();
+            col = kt * (eta * eta);   // radiance transport
+            w = 1 - fr;
+        }
+        const V3<T> wi = shd.to_global(lwi);
+        r.wi = wi;
+        r.f = col * (w / std::abs(lwi.z)) * normal_correct_factor(geo.z, shd.z, wi);
+        r.pdf = w;
         return r;
     }
 };

@@ -21,7 +21,7 @@
 //     weight 1), Q5 (MIS-BSDF light hit attenuated by Tr(ray.o, isect.pos)),
 //     Q6 (no environment NEE at medium vertices), Q23 (emission of the
 //     surface a segment reaches added before the segment's medium sample),
-//     Q26 (no direct light at a
+//     Q27 (specular flag kept across a medium scatter), Q26 (no direct light at a
 //     vertex whose continuation sample fails), the Disney Q2/Q3/Q21 and the
 //     Phong/Lambert Q22/Q24.
 //   quirks = 0 (Fixed): the same loop with those defects corrected.
@@ -241,15 +241,23 @@ struct SurfBSDF {
     std::unique_ptr<DisneyRef<float>> dis;
     std::unique_ptr<LambertRef<float>> lam;
     std::unique_ptr<PhongRef<float>> pho;
+    std::unique_ptr<GlassRef<float>> gla;
     RGB<float> eval(const V3f& wi, const V3f& wo) const {
-        return type == TR_MAT_DISNEY ? dis->eval(wi, wo) : type == TR_MAT_PHONG ? pho->eval(wi, wo) : lam->eval(wi, wo);
+        return type == TR_MAT_DISNEY ? dis->eval(wi, wo)
+             : type == TR_MAT_PHONG  ? pho->eval(wi, wo)
+             : type == TR_MAT_GLASS  ? gla->eval(wi, wo)
+                                     : lam->eval(wi, wo);
     }
     float pdf(const V3f& wi, const V3f& wo) const {
-        return type == TR_MAT_DISNEY ? dis->pdf(wi, wo) : type == TR_MAT_PHONG ? pho->pdf(wi, wo) : lam->pdf(wi, wo);
+        return type == TR_MAT_DISNEY ? dis->pdf(wi, wo)
+             : type == TR_MAT_PHONG  ? pho->pdf(wi, wo)
+             : type == TR_MAT_GLASS  ? gla->pdf(wi, wo)
+                                     : lam->pdf(wi, wo);
     }
     BSDFSample<float> sample(const V3f& wo, float a, float b, float c) const {
         return type == TR_MAT_DISNEY ? dis->sample(wo, a, b, c)
              : type == TR_MAT_PHONG  ? pho->sample(wo, a, b, c)
+             : type == TR_MAT_GLASS  ? gla->sample(wo, a, b, c)
                                      : lam->sample(wo, a, b, c);
     }
 };
@@ -269,6 +277,7 @@ static void make_bsdf(const Scene& s, const Isect& is, bool quirks, SurfBSDF& b)
     b.type = m.type;
     if (m.type == TR_MAT_DISNEY) b.dis.reset(new DisneyRef<float>(is.geo, is.shd, to_params(m), !quirks));
     else if (m.type == TR_MAT_PHONG) b.pho.reset(new PhongRef<float>(is.geo, is.shd, phong_kd<float>(m), phong_ks<float>(m), m.shininess, !quirks));
+    else if (m.type == TR_MAT_GLASS) b.gla.reset(new GlassRef<float>(is.geo, is.shd, phong_kd<float>(m), phong_ks<float>(m), m.ior));
     else b.lam.reset(new LambertRef<float>(is.geo, is.shd, RGB<float>(m.base_color[0], m.base_color[1], m.base_color[2]), !quirks));
 }
 
@@ -472,7 +481,7 @@ struct Renderer {
         const V3f cam_o = ro;
         V3f o = ro, d = rd.normalize();
         bool specular = false;
-        int scattering = 0;
+        int scattering = 0, s_depth = 0;
         for (int depth = 0; depth < c.max_depth; ++depth) {
             if (depth > c.min_depth) {
                 if (sp.get() > 0.9f) break;
@@ -522,7 +531,13 @@ struct Renderer {
                     o = ms.pos;
                     d = ps.wi.normalize();
                     coef *= ps.f / ps.pdf;
-                    specular = false;
+                    // Q27: the reference keeps the specular flag of the last surface
+                    // vertex across a medium scatter (path_trace_renderer.cpp:94-117
+                    // continue without resetting it), so after glass -> scatter the
+                    // next hit's emission is added unweighted on top of the scatter
+                    // vertex's light sample and MIS ray.  Fixed: a scatter vertex is
+                    // not specular.
+                    if (!c.quirks) specular = false;
                     continue;
                 }
             } else {
@@ -540,14 +555,22 @@ struct Renderer {
             // direct light is estimated first and the path ends after it.
             const bool cont_ok = !(bs.f.is_black() || bs.pdf < kEps);
             if (!cont_ok && c.quirks) break;
-            specular = false;   // Disney/Lambert/Phong are never delta
+            // path_trace_renderer.cpp:133-138: a specular (glass) bounce does not count
+            // towards the depth, up to max_specular_depth = 20 of them
+            specular = cont_ok && bs.delta;
+            if (specular && s_depth < 20) {
+                ++s_depth;
+                --depth;
+            }
             auto evalf = [&](const V3f& wi) { return bsdf.eval(wi, is.wo); };
             auto pdff = [&](const V3f& wi) { return bsdf.pdf(wi, is.wo); };
             auto samplef = [&](float x, float y, float z) { return bsdf.sample(is.wo, x, y, z); };
             RGB<float> di;
-            for (int i = 0; i < c.nls; ++i) {
-                di += coef * direct(is.pos, is.geo.z, true, &is, nullptr, evalf, pdff, sp);
-                di += coef * sample_bsdf_mis(is.pos, is.geo.z, true, &is, nullptr, samplef, sp);
+            if (!bs.delta) {   // path_trace_renderer.cpp:141: no light sample at a specular vertex
+                for (int i = 0; i < c.nls; ++i) {
+                    di += coef * direct(is.pos, is.geo.z, true, &is, nullptr, evalf, pdff, sp);
+                    di += coef * sample_bsdf_mis(is.pos, is.geo.z, true, &is, nullptr, samplef, sp);
+                }
             }
             L += di * (1.0f / c.nls);
             if (!cont_ok) break;
@@ -906,6 +929,8 @@ void ref_bsdf_eval(const tr_material* mats, const int32_t* ids, const float* wi,
             } else if (m.type == TR_MAT_PHONG) {
                 PhongRef<T> b(geo, shd, phong_kd<T>(m), phong_ks<T>(m), T(m.shininess), fixed != 0);
                 fv = b.eval(vi, vo); pv = b.pdf(vi, vo);
+            } else if (m.type == TR_MAT_GLASS) {
+                fv = RGB<T>(0); pv = 0;
             } else {
                 LambertRef<T> b(geo, shd, RGB<T>(m.base_color[0], m.base_color[1], m.base_color[2]), fixed != 0);
                 fv = b.eval(vi, vo); pv = b.pdf(vi, vo);
@@ -931,6 +956,9 @@ void ref_bsdf_sample(const tr_material* mats, const int32_t* ids, const float* w
                 s = b.sample(vo, T(u[3 * i]), T(u[3 * i + 1]), T(u[3 * i + 2]));
             } else if (m.type == TR_MAT_PHONG) {
                 PhongRef<T> b(geo, shd, phong_kd<T>(m), phong_ks<T>(m), T(m.shininess), fixed != 0);
+                s = b.sample(vo, T(u[3 * i]), T(u[3 * i + 1]), T(u[3 * i + 2]));
+            } else if (m.type == TR_MAT_GLASS) {
+                GlassRef<T> b(geo, shd, phong_kd<T>(m), phong_ks<T>(m), T(m.ior));
                 s = b.sample(vo, T(u[3 * i]), T(u[3 * i + 1]), T(u[3 * i + 2]));
             } else {
                 LambertRef<T> b(geo, shd, RGB<T>(m.base_color[0], m.base_color[1], m.base_color[2]), fixed != 0);

@@ -29,11 +29,12 @@ def camera(pos, target, up=(0.0, 1.0, 0.0), fov_deg=40.0, aspect=1.0) -> abi.TrC
     return c
 
 
-def box_room_obj(light_ke=(12.0, 12.0, 12.0), block_ks=(0.35, 0.35, 0.35), block_ns=25.0) -> tuple[str, str]:
+def box_room_obj(light_ke=(12.0, 12.0, 12.0), block_ks=(0.35, 0.35, 0.35), block_ns=25.0,
+                 glass_block: bool = False) -> tuple[str, str]:
     """A closed-on-five-sides room [-1, 1]^3 open towards +z, a ceiling quad
-    light facing down, a tall block with a Phong specular lobe and a short
-    diffuse block; faces are quads (split by the reader), normals given per
-    face with v//vn corners."""
+    light facing down, a tall block with a Phong specular lobe (or of glass,
+    Ni 1.5) and a short diffuse block; faces are quads (split by the reader),
+    normals given per face with v//vn corners."""
     mtl = f"""# materials
 newmtl white
 Kd 0.73 0.73 0.73
@@ -45,6 +46,9 @@ newmtl glossy
 Kd 0.45 0.40 0.30
 Ks {block_ks[0]} {block_ks[1]} {block_ks[2]}
 Ns {block_ns}
+newmtl glass
+Ks 0.95 0.95 0.95
+Ni 1.5
 newmtl light
 Kd 0.78 0.78 0.78
 Ke {light_ke[0]} {light_ke[1]} {light_ke[2]}
@@ -85,7 +89,7 @@ Ke {light_ke[0]} {light_ke[1]} {light_ke[2]}
             quad(*c, nn, mat, name if first else None)
             first = False
 
-    block(-0.65, -0.05, -0.7, -0.1, 0.25, "glossy", "tall")
+    block(-0.65, -0.05, -0.7, -0.1, 0.25, "glass" if glass_block else "glossy", "tall")
     block(0.1, 0.6, 0.05, 0.55, -0.4, "white", "short")
     text = "\n".join([f"v {a} {b} {c}" for a, b, c in v] + [f"vn {a} {b} {c}" for a, b, c in n] + lines) + "\n"
     return text, mtl

@@ -192,14 +192,17 @@ def test_c1_phong_matches_fixed_estimator(gpu_ctx):
     _compare_estimators(gpu_ctx, s, 32, 32, 1024, 256, 8, 8, quirks=False, nee=0)
 
 
-@pytest.mark.parametrize("fog", [False, True])
-def test_obj_room_matches_fixed_estimator(gpu_ctx, tmp_path, fog):
+@pytest.mark.parametrize("fog,glass", [(False, False), (True, False), (False, True), (True, True)])
+def test_obj_room_matches_fixed_estimator(gpu_ctx, tmp_path, fog, glass):
     """A room read from an OBJ / MTL pair through load_model_from_file and the
-    run()-style wiring (quad faces, a Phong block, an emissive quad; fog
-    outside every surface when fog is set), against the oracle in fixed mode."""
+    run()-style wiring (quad faces, a Phong or glass block, an emissive quad;
+    fog outside every surface when fog is set), against the oracle in fixed
+    mode.  Glass exercises the specular path: no light sample at the glass
+    vertex, unweighted emission at the next hit, specular bounces outside the
+    depth count."""
     from tests import objfiles
     from tracer_amd import native
-    obj, mtl = objfiles.box_room_obj()
+    obj, mtl = objfiles.box_room_obj(glass_block=glass)
     path = objfiles.write(tmp_path, "room.obj", obj, mtl)
     s = native.obj_scene(path, objfiles.camera((0.0, 0.0, 3.6), (0.0, 0.0, 0.0)), outside_fog=fog)
     _compare_estimators(gpu_ctx, s, 32, 32, 512 if fog else 1024, 128 if fog else 256, 8, 8, quirks=False, nee=0,

@@ -305,3 +305,53 @@ def test_phong_fixed_sampling_matches_pdf(case, wo):
     wo = np.array(wo) / np.linalg.norm(wo)
     p, lost = _chi2_sample_vs_pdf(_phong(**PHONG_CASES[case]), wo, fixed=1)
     assert p > 1e-4, p
+
+
+# ---- Glass (src/material/glass.cpp) -----------------------------------------------
+def _glass(kr=(1.0, 1.0, 1.0), kt=(1.0, 1.0, 1.0), ior=1.5):
+    from tracer_amd.abi import TR_MAT_GLASS
+    return make_material(type=TR_MAT_GLASS, base_color=kr, specular_scale=kt, ior=ior)
+
+
+@pytest.mark.parametrize("theta_deg,inside", [(0.0, False), (35.0, False), (70.0, False), (20.0, True), (50.0, True)])
+def test_glass_sampling_fresnel_snell(theta_deg, inside):
+    """The reflected fraction is the textbook unpolarised Fresnel reflectance,
+    reflected directions are mirror images, refracted ones obey
+    n_i sin(theta_i) = n_t sin(theta_t) in the plane of incidence, and beyond
+    the critical angle (from inside, 41.8 degrees for n = 1.5) every sample
+    reflects.  eval and pdf vanish for every pair of directions."""
+    from oracle import refcpu
+    th = np.radians(theta_deg)
+    wo = np.array([np.sin(th) * np.cos(0.3), np.sin(th) * np.sin(0.3), np.cos(th) * (-1 if inside else 1)])
+    n = 40000
+    rng = np.random.default_rng(41)
+    u = rng.uniform(0, 1, (n, 3)).astype(np.float32)
+    wos = np.tile(np.float32(wo), (n, 1))
+    mat = _glass(kr=(0.9, 0.8, 0.7), kt=(0.64, 0.81, 1.0))
+    wi, f, pdf = refcpu.bsdf_sample([mat], np.zeros(n, np.int32), wos, u, precision=1, fixed=1)
+    n1, n2 = (1.5, 1.0) if inside else (1.0, 1.5)
+    F = fresnel_textbook(n1, n2, np.cos(th))
+    refl = wi[:, 2] * wo[2] > 0
+    k = refl.sum()
+    assert abs(k / n - F) < 5 * np.sqrt(F * (1 - F) / n) + 1e-12
+    mirror = np.array([-wo[0], -wo[1], wo[2]])
+    assert np.allclose(wi[refl], mirror, atol=1e-6)
+    assert np.allclose(pdf[refl], F, rtol=1e-6)
+    # reflected weight f |cos| / pdf = kr
+    assert np.allclose(f[refl] * np.abs(wi[refl, 2:3]) / pdf[refl, None], [0.9, 0.8, 0.7], rtol=1e-6)
+    if (~refl).any():
+        t = wi[~refl]
+        sin_i = np.sin(th)
+        sin_t = np.linalg.norm(t[:, :2], axis=1)
+        assert np.allclose(n1 * sin_i, n2 * sin_t, atol=1e-6)
+        if theta_deg > 0:   # in the plane of incidence, on the other side of the normal
+            az = np.arctan2(t[:, 1], t[:, 0])
+            assert np.abs(np.angle(np.exp(1j * (az - 0.3 - np.pi)))).max() < 1e-5
+        # refracted weight: (n_i / n_t)^2 sqrt(kt)
+        wgt = f[~refl] * np.abs(t[:, 2:3]) / pdf[~refl, None]
+        assert np.allclose(wgt, (n1 / n2) ** 2 * np.sqrt([0.64, 0.81, 1.0]), rtol=1e-6)
+    crit = np.degrees(np.arcsin(1 / 1.5))
+    if inside and theta_deg > crit:
+        assert k == n
+    fe, pe = refcpu.bsdf_eval([mat], np.zeros(n, np.int32), wi.astype(np.float32), wos)
+    assert np.all(fe == 0) and np.all(pe == 0)

@@ -10,7 +10,7 @@ import ctypes as C
 import numpy as np
 
 TR_OK, TR_ERR_INVALID, TR_ERR_HIP, TR_ERR_OOM, TR_ERR_STATE = range(5)
-TR_MAT_LAMBERT, TR_MAT_DISNEY, TR_MAT_PHONG = 0, 1, 2
+TR_MAT_LAMBERT, TR_MAT_DISNEY, TR_MAT_PHONG, TR_MAT_GLASS = 0, 1, 2, 3
 TR_FILTER_BOX, TR_FILTER_GAUSSIAN = 0, 1
 TR_TR_ANALYTIC, TR_TR_RATIO = 0, 1
 

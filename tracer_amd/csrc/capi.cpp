@@ -30,8 +30,13 @@ namespace {
 
 // DisneyBSDF constructor (src/material/disney.cpp:347-399) evaluated once per material.
 void check_material(const tr_material& m) {
-    if (m.type != TR_MAT_LAMBERT && m.type != TR_MAT_DISNEY && m.type != TR_MAT_PHONG)
+    if (m.type != TR_MAT_LAMBERT && m.type != TR_MAT_DISNEY && m.type != TR_MAT_PHONG && m.type != TR_MAT_GLASS)
         throw InvalidArg("unknown material type");
+    if (m.type == TR_MAT_GLASS) {
+        if (!(m.ior > 0.0f && m.ior < 1e30f)) throw InvalidArg("glass ior must be finite and > 0");
+        for (int k = 0; k < 3; ++k)
+            if (!(m.base_color[k] >= 0.0f && m.specular_scale[k] >= 0.0f)) throw InvalidArg("glass kr and kt must be >= 0");
+    }
     if (m.type == TR_MAT_PHONG) {
         if (!(m.shininess >= 0.0f && m.shininess < 1e30f)) throw InvalidArg("Phong shininess must be finite and >= 0");
         for (int k = 0; k < 3; ++k)
@@ -62,8 +67,21 @@ static DevMaterial make_phong(const tr_material& m) {
     return d;
 }
 
+// Glass::shading (src/material/glass.cpp:96-112): reflection colour kr,
+// refraction colour sqrt(kt), the ior as given.  Never guided.
+static DevMaterial make_glass(const tr_material& m) {
+    DevMaterial d{};
+    d.type = MAT_GLASS;
+    d.C = mk(m.base_color[0], m.base_color[1], m.base_color[2]);
+    d.spec_scale = mk(std::sqrt(m.specular_scale[0]), std::sqrt(m.specular_scale[1]), std::sqrt(m.specular_scale[2]));
+    d.ior = m.ior;
+    d.guide_share = 0.0f;
+    return d;
+}
+
 DevMaterial make_material(const tr_material& m) {
     if (m.type == TR_MAT_PHONG) return make_phong(m);
+    if (m.type == TR_MAT_GLASS) return make_glass(m);
     DevMaterial d{};
     d.type = m.type == TR_MAT_DISNEY ? MAT_DISNEY : MAT_LAMBERT;
     d.C = mk(m.base_color[0], m.base_color[1], m.base_color[2]);
@@ -274,7 +292,11 @@ void build_device_scene(tr_ctx* c, const tr_bvh_opts* opts) {
     const std::vector<int> guide = light_cutpoints(cdf);
 
     std::vector<DevMaterial> dm(h.mats.size());
-    for (size_t i = 0; i < h.mats.size(); ++i) dm[i] = make_material(h.mats[i]);
+    c->has_delta = false;
+    for (size_t i = 0; i < h.mats.size(); ++i) {
+        dm[i] = make_material(h.mats[i]);
+        c->has_delta = c->has_delta || dm[i].type == MAT_GLASS;
+    }
     std::vector<DevMedium> dmed(h.media.size());
     for (size_t i = 0; i < h.media.size(); ++i) dmed[i] = make_medium(h.media[i]);
 
@@ -742,7 +764,10 @@ tr_status tr_render(tr_ctx* c, const tr_render_params* prm, double* film_sums, i
             }
             int cur = 0;
             timed(3, [&] { launch_generate(c->ds, p, rc, num_paths, cur, st); });
-            for (int b = 0; b <= P.max_depth; ++b) {
+            // specular bounces do not count towards the depth: a scene with
+            // glass needs up to kMaxSpecularDepth more rounds
+            const int rounds = P.max_depth + 1 + (c->has_delta ? kMaxSpecularDepth : 0);
+            for (int b = 0; b < rounds; ++b) {
                 if (use_tail) timed(6, [&] { launch_tail(c->ds, p, rc, cur, std::min(tail, num_paths), st); });
                 timed(0, [&] { launch_extend(c->ds.tr, p, cur, num_paths, P.profile == 2, st); });
                 if (guided && rc.guide_frac > 0.0f) timed(4, [&] { guide_infer(c, p, cur, num_paths); });

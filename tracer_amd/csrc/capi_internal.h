@@ -111,6 +111,7 @@ struct tr_ctx {
     std::string err;
     trcapi::HostScene hs;
     bool have_scene = false, have_bvh = false;
+    bool has_delta = false;   // a material samples specular (delta) lobes: paths may take extra bounces
     // scene on device
     trcapi::DevBuf nodes, tris, tri_shade, mats, media, lights, light_cdf, light_pmf, light_guide;
     int64_t num_nodes = 0;

@@ -26,8 +26,13 @@
 
 namespace trhip {
 
-// info.y layout: depth (8 bits) | scatter count (24 bits) << 8
-TR_D uint32_t pack_info(int depth, int scatter) { return uint32_t(depth) | (uint32_t(scatter) << 8); }
+// info.y layout: depth (8 bits) | specular bounces not counted in depth (8 bits) << 8 |
+// medium scatter count (16 bits) << 16.  Each scatter event takes one depth, so the
+// count stays below 256.
+TR_D uint32_t pack_info(int depth, int sdepth, int scatter) {
+    return uint32_t(depth) | (uint32_t(sdepth) << 8) | (uint32_t(scatter) << 16);
+}
+
 
 TR_D Pcg load_rng(const DevPaths& p, int slot, uint32_t pixel) {
     const uint2 s = p.rng[slot];
@@ -63,7 +68,7 @@ __global__ __launch_bounds__(256) void k_generate(DevScene s, DevPaths p, Render
     p.ray_d[i] = f4(d, 3.0e38f);
     p.thr[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
     p.rad[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    p.info[i] = make_uint2(pixel, pack_info(0, 0));
+    p.info[i] = make_uint2(pixel, pack_info(0, 0, 0));
     store_rng(p, i, rng);
     p.queue[cur][i] = i;
     if (p.grec) p.grec[i] = -1;
@@ -150,7 +155,9 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
     float4 rp = p.rad[slot];
     const uint2 info = p.info[slot];
     const int depth = int(info.y & 0xffu);
-    int scatter = int(info.y >> 8);
+    int sdepth = int((info.y >> 8) & 0xffu);
+    int scatter = int(info.y >> 16);
+    int next_depth = depth + 1;
     Pcg rng = load_rng(p, slot, info.x);
     f3 T = xyz(tp);
     const float prev_pdf = tp.w;
@@ -238,7 +245,9 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
             cont = true;
         } else {
             // ---- surface vertex ----
-            if (sr.light >= 0 && dn < 0.0f) {
+            // At the maximum depth the hit only completes the previous vertex's BSDF
+            // (MIS) sample; after a specular vertex the reference has no such sample.
+            if (sr.light >= 0 && dn < 0.0f && (depth < c.max_depth || prev_pdf > 0.0f)) {
                 // emission with MIS against light sampling (direct_illumination.cpp:176-197)
                 const float4 l0 = s.lights[4 * sr.light];
                 float w = 1.0f;
@@ -274,6 +283,7 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
                 const float su = rng.uniform(), u1 = rng.uniform(), u2 = rng.uniform();
                 f3 wi, fs;
                 float pdf;
+                bool delta = false;
                 if (use_guide) {
                     float pdf_b;
                     if (rng.uniform() < frac) {
@@ -293,6 +303,7 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
                     wi = normalize(tx * bs.wi.x + ty * bs.wi.y + ns * bs.wi.z);
                     fs = bs.f;
                     pdf = bs.pdf;
+                    delta = bs.delta;
                 }
                 // Next-event estimation does not depend on the continuation sample.  The
                 // reference ends the path before estimating direct light when the
@@ -300,8 +311,9 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
                 // :141-153), which loses that vertex's direct light; here the direct
                 // light is estimated at every vertex and the path ends afterwards.
                 {
-                    // next-event estimation (direct_illumination.cpp:35-109)
-                    if (s.num_light_entries > 0) {
+                    // next-event estimation (direct_illumination.cpp:35-109); none at a
+                    // specular vertex (path_trace_renderer.cpp:141)
+                    if (!delta && s.num_light_entries > 0) {
                         float pmf;
                         const int e = pick_light(s, rng.uniform(), pmf);
                         const float lu = rng.uniform(), lv = rng.uniform();
@@ -359,7 +371,13 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
                     const f3 org = dot(ng, wi) > 0.0f ? pos + ng * kEps : pos - ng * kEps;
                     next_o = f4(org, 0.0f);
                     next_d = f4(wi, 3.0e38f);
-                    tp = f4(T, pdf);
+                    // a specular vertex passes pdf 0: the next hit's emission is not
+                    // weighted against light sampling
+                    tp = f4(T, delta ? 0.0f : pdf);
+                    if (delta && sdepth < kMaxSpecularDepth) {
+                        ++sdepth;
+                        next_depth = depth;
+                    }
                     cont = finite3(T);
                     if (rec_path && cont) {
                         // record (x, n, wo, wi); its weight lum((L_end - L_next) * rr / Tv)
@@ -379,7 +397,7 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
     }
     if (cont) {
         // Russian roulette before the next segment (path_trace_renderer.cpp:54-58)
-        if (depth + 1 > c.min_depth) {
+        if (next_depth > c.min_depth) {
             if (rng.uniform() > 0.9f) cont = false;
             else {
                 tp.x /= 0.9f; tp.y /= 0.9f; tp.z /= 0.9f;
@@ -398,7 +416,7 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
         p.ray_o[slot] = next_o;
         p.ray_d[slot] = next_d;
         p.thr[slot] = tp;
-        p.info[slot] = make_uint2(info.x, pack_info(depth + 1, scatter));
+        p.info[slot] = make_uint2(info.x, pack_info(next_depth, sdepth, scatter));
         store_rng(p, slot, rng);
     }
 }

@@ -27,13 +27,14 @@
 
 namespace trhip {
 
-enum : int { MAT_LAMBERT = 0, MAT_DISNEY = 1, MAT_PHONG = 2 };
+enum : int { MAT_LAMBERT = 0, MAT_DISNEY = 1, MAT_PHONG = 2, MAT_GLASS = 3 };
 
 // Per-material constants, precomputed on the host from the create_disney
 // parameters exactly as the DisneyBSDF constructor does (disney.cpp:347-399).
 // Phong uses C = kd + ks (the aggregate albedo of the black-fringe fallback),
 // Ctint = kd, spec_scale = ks, ns, w_diffuse and w_spec, all after the
-// energy normalisation of Phong::shading.
+// energy normalisation of Phong::shading.  Glass uses C = the reflection
+// colour, spec_scale = the refraction colour and ior.
 struct DevMaterial {
     int type;
     float metallic, roughness, transmission;
@@ -51,6 +52,7 @@ struct BsdfSample {
     f3 wi;
     f3 f;
     float pdf;
+    bool delta;   // a specular (Dirac) sample: f and pdf are the weights of a delta lobe
 };
 
 TR_HD float pow5(float x) { const float t2 = x * x; return t2 * t2 * x; }
@@ -333,8 +335,10 @@ TR_HD float phong_lobe(f3 lwi, f3 lwo, float ns) {
 }
 
 // eval + pdf for directions wi, wo (shading frame), geometric normal ng.
+// Glass is a pair of delta lobes: zero for every pair of directions.
 TR_HD BsdfEval bsdf_eval(const DevMaterial& m, f3 wi, f3 wo, f3 ng) {
     BsdfEval r{f3s(0.0f), 0.0f};
+    if (m.type == MAT_GLASS) return r;
     if (fringe(wi, ng) || fringe(wo, ng)) {
         const bool gi = dot(wi, ng) > 0.0f, go = dot(wo, ng) > 0.0f;
         if (gi && go) r.f = m.C * kInvPi;
@@ -364,8 +368,43 @@ TR_HD BsdfEval bsdf_eval(const DevMaterial& m, f3 wi, f3 wo, f3 ng) {
 
 // Sample wi given wo; returns pdf = 0 when no valid direction was produced.
 // (gt, gb): a tangent frame around ng for the fringe fallback.
+// GlassBSDF::sample (src/material/glass.cpp:24-67): reflection with
+// probability F = dielectric Fresnel at lwo.z (ior inside, 1 outside), else
+// refraction about the shading normal on wo's side; f = colour * weight /
+// |lwi.z| times the shading-normal correction, radiance transport scales the
+// refracted weight by (eta_i / eta_t)^2.  Total internal reflection has F = 1.
+TR_HD BsdfSample glass_sample(const DevMaterial& m, f3 wo, f3 ng, float su) {
+    BsdfSample s{f3s(0.0f), f3s(0.0f), 0.0f, true};
+    const f3 lwo = normalize(wo);
+    const float F = fresnel_dielectric(m.ior, 1.0f, lwo.z);
+    f3 wi;
+    f3 col;
+    float w;
+    if (su < F) {
+        wi = mk(-lwo.x, -lwo.y, lwo.z);
+        col = m.C;
+        w = F;
+    } else {
+        const bool outside = lwo.z > 0.0f;
+        const float eta = outside ? 1.0f / m.ior : m.ior;   // eta_i / eta_t
+        const float ci = fabsf(lwo.z);
+        const float st2 = eta * eta * fmaxf(0.0f, 1.0f - ci * ci);
+        if (st2 >= 1.0f) return s;                          // refract_dir, reflection.hpp:43-51
+        const float ct = sqrtf(1.0f - st2);
+        const float nz = outside ? 1.0f : -1.0f;
+        wi = normalize(mk(-eta * lwo.x, -eta * lwo.y, (eta * ci - ct) * nz - eta * lwo.z));
+        col = m.spec_scale * (eta * eta);
+        w = fresnel_transmittance(m.ior, 1.0f, lwo.z, ct);   // 1 - F without cancellation
+    }
+    s.wi = wi;
+    s.f = col * (w / fabsf(wi.z) * normal_correct(ng, wi));
+    s.pdf = w;
+    return s;
+}
+
 TR_HD BsdfSample bsdf_sample(const DevMaterial& m, f3 wo, f3 ng, float su, float u, float v) {
-    BsdfSample s{f3s(0.0f), f3s(0.0f), 0.0f};
+    if (m.type == MAT_GLASS) return glass_sample(m, wo, ng, su);
+    BsdfSample s{f3s(0.0f), f3s(0.0f), 0.0f, false};
     if (fringe(wo, ng)) {
         // bsdf.hpp:141-153: cosine sampling about the geometric normal
         if (!(dot(wo, ng) > 0.0f)) return s;

@@ -9,6 +9,8 @@ namespace trhip {
 constexpr int kTraceBlock = 256;
 constexpr int kLdsStack = 8;
 constexpr int kShadeBlock = 256;
+// Specular bounces that do not count towards the depth (path_trace_renderer.cpp:20, :133-138).
+constexpr int kMaxSpecularDepth = 20;
 
 // Path state of the wavefront (one slot per camera sample in flight).
 struct DevPaths {

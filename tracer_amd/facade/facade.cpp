@@ -330,7 +330,15 @@ static tr_material convert_material(const Material* m) {
         r.base_color[0] = kd.r / dem; r.base_color[1] = kd.g / dem; r.base_color[2] = kd.b / dem;
         return r;
     }
-    if (dynamic_cast<const GlassMaterial*>(m)) throw std::runtime_error("glass (create_glass) is not supported by the GPU path");
+    if (auto g = dynamic_cast<const GlassMaterial*>(m)) {
+        if (!g->kr->is_constant() || !g->kt->is_constant() || !g->ior->is_constant())
+            throw std::runtime_error("Glass: only constant textures are supported");
+        r.type = TR_MAT_GLASS;
+        tex_rgb(g->kr, r.base_color);
+        tex_rgb(g->kt, r.specular_scale);
+        r.ior = tex_s(g->ior);
+        return r;
+    }
     throw std::runtime_error("unsupported material");
 }
 

@@ -492,8 +492,8 @@ static void config5(Config& c, uint64_t seed, double detail) {
 }
 
 // A scene from an OBJ file, wired the way run() does it (src/main.cpp:212-282):
-// every MTL material becomes a Phong material (a glass when Ni > 1, which the
-// GPU path rejects), emissive materials make their triangles area lights, all
+// every MTL material becomes a Phong material (a glass when Ni > 1),
+// emissive materials make their triangles area lights, all
 // triangles get MediumInterface{inside = vacuum, outside = fog} with the fog
 // of main.cpp:242 (or vacuum outside when fog is false), and the BVH has 3
 // primitives per leaf.
