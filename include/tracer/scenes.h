@@ -35,6 +35,10 @@ trs_scene* trs_create(int32_t config, uint64_t seed, double detail, const char**
  * glass.  Returns NULL and sets *err on a read or parse error, a triangle
  * without a material or an image texture. */
 trs_scene* trs_load_obj(const char* obj_path, const tr_camera* camera, int32_t outside_fog, const char** err);
+/* Reads a Radiance .hdr image (the loader behind create_texture2d_from_file):
+ * *width, *height, and when rgb is not NULL its height rows of width linear
+ * RGB texels (cap floats at least).  Returns 0, or 1 and *err on failure. */
+int32_t trs_read_hdr(const char* path, int32_t* width, int32_t* height, float* rgb, int64_t cap, const char** err);
 const tr_scene_desc* trs_desc(const trs_scene* s);
 /* Film, spp and depths of the configuration (PTRendererParams + film). */
 void trs_render_defaults(const trs_scene* s, tr_render_params* out);

@@ -14,7 +14,8 @@
 //   load_model_from_file        src/utility/mesh_load.hpp:93, mesh_load.cpp:25-113
 //   create_texture_from_file    src/utility/image_file.hpp:23-45, image_file.cpp:139-166 (constant colours)
 //   create_constant_texture2d   src/factory/texture.hpp:12
-//   create_ibl_light            src/factory/light.hpp:12-15 (constant texture)
+//   create_ibl_light            src/factory/light.hpp:12-15 (constant texture or .hdr image)
+//   create_texture2d_from_file  src/utility/image_file.hpp:21 (Radiance .hdr)
 //   create_homogeneous_medium   src/factory/medium.hpp:21-26
 //   create_vacuum               src/factory/medium.hpp:34
 //   create_thin_lens_camera     src/factory/camera.hpp:9-11
@@ -185,6 +186,10 @@ public:
     virtual bool is_constant() const { return false; }
 };
 RC<Texture2D> create_constant_texture2d(const Spectrum& constant);
+// image_file.hpp:21 for Radiance .hdr files (linear RGB): the texture for an
+// image environment (create_ibl_light).  Other formats throw
+// std::runtime_error: the GPU path has no 8-bit image textures.
+RC<const Texture2D> create_texture2d_from_file(const std::string& filename);
 
 class NormalMapper {
 public:

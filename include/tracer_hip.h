@@ -107,7 +107,10 @@ typedef struct tr_scene_desc {
     int32_t num_media;
     const tr_material* materials;
     const tr_medium* media;
-    /* Constant environment light: create_ibl_light(create_constant_texture2d(L)). */
+    /* Environment light.  has_env with env_width == 0: create_ibl_light(
+     * create_constant_texture2d(env_radiance)); with env_width > 0: an image
+     * environment (create_ibl_light over an HDR texture,
+     * src/light/environment_light.cpp:20-196), see env_pixels below. */
     int32_t has_env;
     float env_radiance[3];
     /* Directional light (no reference counterpart): direction points from the
@@ -116,6 +119,14 @@ typedef struct tr_scene_desc {
     float sun_direction[3];
     float sun_irradiance[3];
     tr_camera camera;
+    /* Image environment: env_height rows of env_width linear RGB texels
+     * (3 floats each), row 0 at v = 0 (theta = 0), sampled bilinearly
+     * (src/utility/sampler.hpp:12-25).  A world direction d maps to light
+     * space as env_world_to_light * d (row-major 3x3 rotation), then to
+     * u = phi / 2pi, v = theta / pi of the light-space direction. */
+    int32_t env_width, env_height;
+    const float* env_pixels;
+    float env_world_to_light[9];
 } tr_scene_desc;
 
 /* BVH options.  max_leaf_primitives mirrors create_bvh_accel(int)
@@ -253,6 +264,14 @@ tr_status tr_bsdf_eval_batch(tr_ctx* ctx, const tr_material* mats, int32_t num_m
 tr_status tr_bsdf_sample_batch(tr_ctx* ctx, const tr_material* mats, int32_t num_mats,
                                const int32_t* mat_ids, const float* wo, const float* ng,
                                const float* u, int64_t n, float* wi, float* f, float* pdf);
+
+/* Environment light of the uploaded scene (after tr_bvh_build), for tests:
+ * IBL::sample_li with uniforms u (n*2: u picks phi, v theta) -> unit
+ * directions wi (n*3), solid-angle pdf (0 = no sample) and radiance L (n*3)
+ * (src/light/environment_light.cpp:92-119); and IBL::light_emit / pdf at
+ * unit directions (n*3) (:179-196).  TR_ERR_STATE without an environment. */
+tr_status tr_env_sample_batch(tr_ctx* ctx, const float* u, int64_t n, float* wi, float* pdf, float* L);
+tr_status tr_env_eval_batch(tr_ctx* ctx, const float* dirs, int64_t n, float* L, float* pdf);
 
 /* ---- learned path guiding (no reference counterpart) ---------------------
  * MLP: 96 input features (81 used: position in the scene box, its sin/cos at

@@ -58,8 +58,54 @@ def lib() -> C.CDLL:
         L.ref_camera_rays.restype = None
         L.ref_render.argtypes = [VP, P(TrRenderParams), I32, I32, I32, P(C.c_double), P(C.c_uint64)]
         L.ref_render.restype = C.c_double
+        L.ref_env_create.argtypes = [C.c_int, C.c_int, P(C.c_float), P(C.c_float)]
+        L.ref_env_create.restype = VP
+        L.ref_env_destroy.argtypes = [VP]
+        L.ref_env_destroy.restype = None
+        L.ref_env_sample.argtypes = [VP, P(C.c_float), I64, P(C.c_float), P(C.c_float), P(C.c_float)]
+        L.ref_env_sample.restype = None
+        L.ref_env_eval.argtypes = [VP, P(C.c_float), I64, I32, P(C.c_float), P(C.c_float)]
+        L.ref_env_eval.restype = None
+        L.ref_env_radiance_integral.argtypes = [VP, P(C.c_float)]
+        L.ref_env_radiance_integral.restype = None
         _lib = L
     return _lib
+
+
+class RefEnv:
+    """The oracle's image environment light (IBL over an HDR texture)."""
+
+    def __init__(self, rgb: np.ndarray, world_to_light=None):
+        self.rgb = np.ascontiguousarray(rgb, np.float32)
+        h, w, _ = self.rgb.shape
+        rot = np.eye(3, dtype=np.float32) if world_to_light is None else np.asarray(world_to_light, np.float32)
+        self.rot = np.ascontiguousarray(rot.reshape(9))
+        self.h = lib().ref_env_create(w, h, fptr(self.rgb), fptr(self.rot))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().ref_env_destroy(self.h)
+            self.h = None
+
+    def sample(self, u: np.ndarray):
+        """u: (n, 2) -> (wi (n, 3), pdf (n,), L (n, 3)); pdf 0 = no sample."""
+        u = np.ascontiguousarray(u, np.float32)
+        n = u.shape[0]
+        wi, pdf, L = np.zeros((n, 3), np.float32), np.zeros(n, np.float32), np.zeros((n, 3), np.float32)
+        lib().ref_env_sample(self.h, fptr(u), n, fptr(wi), fptr(pdf), fptr(L))
+        return wi, pdf, L
+
+    def eval(self, dirs: np.ndarray, quirks: bool = False):
+        d = np.ascontiguousarray(dirs, np.float32)
+        n = d.shape[0]
+        L, pdf = np.zeros((n, 3), np.float32), np.zeros(n, np.float32)
+        lib().ref_env_eval(self.h, fptr(d), n, 1 if quirks else 0, fptr(L), fptr(pdf))
+        return L, pdf
+
+    def radiance_integral(self) -> np.ndarray:
+        out = np.zeros(3, np.float32)
+        lib().ref_env_radiance_integral(self.h, fptr(out))
+        return out
 
 
 class RefScene:

@@ -9,7 +9,7 @@
 //   NEE / MIS helpers             src/renderer/direct_illumination.cpp:11-264
 //   Triangle surface frames       src/shape/triangle.cpp:178-213, :61-97
 //   DiffuseLight                  src/light/diffuse_light.cpp:17-60
-//   IBL with a constant texture   src/light/environment_light.cpp:20-196
+//   IBL (constant or image)        src/light/environment_light.cpp:20-196 (ref_env.hpp)
 //   HomogeneousMedium / vacuum    src/medium/homogeneous.cpp:12-66, src/medium/vacuum.cpp:8-29
 //   ThinLensCamera::generate_ray  src/camera/thin_lens_camera.cpp:23-59, src/utility/transform.cpp:96-124
 //   GaussianFilter / Film tiles   src/filter/gaussian_filter.cpp:9-35, src/core/render.hpp:37-115
@@ -35,6 +35,7 @@
 #include "ref_triangle.hpp"
 #include "ref_bvh.hpp"
 #include "ref_bsdf.hpp"
+#include "ref_env.hpp"
 #include "tracer_hip.h"
 
 #include <atomic>
@@ -74,6 +75,8 @@ struct Scene {
     std::vector<int> lights;          // emissive triangles, one DiffuseLight each
     bool has_env = false;
     RGB<float> env;
+    std::shared_ptr<RefEnvMap> envmap;   // image environment, else the constant env
+    RGB<float> env_radiance(const V3f& dir) const { return envmap ? envmap->light_emit(dir) : env; }
     bool has_sun = false;
     V3f sun_dir;
     RGB<float> sun_E;
@@ -185,6 +188,13 @@ static float pdf_area(const Scene& s, int tri, const V3f& ref, const V3f& pos) {
 // distribution is uniform, theta = pi v, phi = 2 pi u, pdf = 1/(2 pi^2 sin).
 static LightSample sample_env(const Scene& s, const V3f& ref, float u, float v) {
     LightSample ls;
+    if (s.envmap) {
+        V3f wi;
+        if (!s.envmap->sample(u, v, wi, ls.pdf, ls.L)) return ls;
+        ls.pos = ref + wi * (2 * s.wradius);
+        ls.valid = true;
+        return ls;
+    }
     float theta = v * kPi<float>, phi = u * 2 * kPi<float>;
     float st = std::sin(theta), ct = std::cos(theta);
     V3f wi = V3f(st * std::cos(phi), st * std::sin(phi), ct).normalize();
@@ -195,7 +205,8 @@ static LightSample sample_env(const Scene& s, const V3f& ref, float u, float v) 
     ls.valid = true;
     return ls;
 }
-static float pdf_env(const V3f& dir) {
+static float pdf_env(const Scene& s, const V3f& dir, bool quirks) {
+    if (s.envmap) return s.envmap->pdf(dir, quirks);
     float theta = std::acos(std::clamp<float>(dir.z / float(dir.length()), -1, 1));
     float st = std::sin(theta);
     if (st == 0) return 0;
@@ -433,7 +444,7 @@ struct Renderer {
         return p;
     }
     float env_light_pdf(const V3f& dir) const {
-        float p = pdf_env(dir);
+        float p = pdf_env(s, dir, c.quirks);
         if (c.nee == 1) p *= s.lpmf[s.lights.size()];
         return p;
     }
@@ -453,7 +464,7 @@ struct Renderer {
         bool hit = intersect(o, wi, 0, std::numeric_limits<float>::max(), &t);
         if (!hit) {
             if (!s.has_env) return RGB<float>(0);
-            RGB<float> Le = s.env;
+            RGB<float> Le = s.env_radiance(wi);
             RGB<float> f = Le * bs.f;
             if (surface) f *= std::abs(cosv(ng, wi));
             float lp = env_light_pdf(wi);
@@ -501,7 +512,7 @@ struct Renderer {
                     RGB<float> Le = s.tri_emission(is.prim);
                     if (!Le.is_black() && dot(is.geo.z, -d) > 0) Le_hit = Le;
                 } else if (s.has_env) {
-                    L += coef * s.env;
+                    L += coef * s.env_radiance(d);
                 }
             }
             if (c.quirks) L += coef * Le_hit;
@@ -701,6 +712,8 @@ static Scene* build_scene(const tr_scene_desc* d, int max_leaf) {
     s->bvh.build(pb, max_leaf);
     s->has_env = d->has_env != 0;
     s->env = RGB<float>(d->env_radiance[0], d->env_radiance[1], d->env_radiance[2]);
+    if (d->has_env && d->env_width > 0)
+        s->envmap = std::make_shared<RefEnvMap>(d->env_width, d->env_height, d->env_pixels, d->env_world_to_light);
     s->has_sun = d->has_sun != 0;
     s->sun_dir = V3f(d->sun_direction[0], d->sun_direction[1], d->sun_direction[2]);
     s->sun_E = RGB<float>(d->sun_irradiance[0], d->sun_irradiance[1], d->sun_irradiance[2]);
@@ -712,7 +725,8 @@ static Scene* build_scene(const tr_scene_desc* d, int max_leaf) {
     const int nA = int(s->lights.size());
     s->lpmf.assign(nA + 2, 0.0f);
     for (int i = 0; i < nA; ++i) s->lpmf[i] = kPi<float> * s->tri_emission(s->lights[i]).lum() * s->tri_area(s->lights[i]);
-    if (s->has_env) s->lpmf[nA] = s->env.lum() * 4 * kPi<float> * s->wradius * s->wradius;
+    if (s->has_env && s->envmap) s->lpmf[nA] = s->envmap->avg_radiance.lum() * s->wradius * s->wradius;   // IBL::power
+    else if (s->has_env) s->lpmf[nA] = s->env.lum() * 4 * kPi<float> * s->wradius * s->wradius;
     if (s->has_sun) s->lpmf[nA + 1] = s->sun_E.lum() * kPi<float> * s->wradius * s->wradius;
     double tot = 0;
     for (float p : s->lpmf) tot += p;
@@ -972,6 +986,36 @@ void ref_bsdf_sample(const tr_material* mats, const int32_t* ids, const float* w
     }
 }
 // HG phase: value for cos between wi and wo, and sampling.
+// Image environment on its own: IBL sample_li / light_emit / pdf.
+void* ref_env_create(int w, int h, const float* texels, const float* world_to_light) {
+    return new RefEnvMap(w, h, texels, world_to_light);
+}
+void ref_env_destroy(void* e) { delete static_cast<RefEnvMap*>(e); }
+void ref_env_sample(void* e, const float* u, int64_t n, float* wi, float* pdf, float* L) {
+    const RefEnvMap* m = static_cast<RefEnvMap*>(e);
+    for (int64_t i = 0; i < n; ++i) {
+        V3f d;
+        float p = 0;
+        RGB<float> l;
+        if (!m->sample(u[2 * i], u[2 * i + 1], d, p, l)) { p = 0; d = V3f(0, 0, 0); l = RGB<float>(0); }
+        wi[3 * i] = d.x; wi[3 * i + 1] = d.y; wi[3 * i + 2] = d.z;
+        L[3 * i] = l.r; L[3 * i + 1] = l.g; L[3 * i + 2] = l.b;
+        pdf[i] = p;
+    }
+}
+void ref_env_eval(void* e, const float* dirs, int64_t n, int quirks, float* L, float* pdf) {
+    const RefEnvMap* m = static_cast<RefEnvMap*>(e);
+    for (int64_t i = 0; i < n; ++i) {
+        const V3f d(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]);
+        const RGB<float> l = m->light_emit(d);
+        L[3 * i] = l.r; L[3 * i + 1] = l.g; L[3 * i + 2] = l.b;
+        pdf[i] = m->pdf(d, quirks != 0);
+    }
+}
+void ref_env_radiance_integral(void* e, float* out3) {
+    const RefEnvMap* m = static_cast<RefEnvMap*>(e);
+    out3[0] = m->avg_radiance.r; out3[1] = m->avg_radiance.g; out3[2] = m->avg_radiance.b;
+}
 double ref_hg_eval(double g, double cos_wi_wo) { HGRef<double> h{g}; return h.phase(-cos_wi_wo); }
 void ref_hg_sample(float g, const float* wo, const float* u, int64_t n, float* wi, float* pdf) {
     HGRef<float> h{g};

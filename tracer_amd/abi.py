@@ -61,6 +61,9 @@ class TrSceneDesc(C.Structure):
         ("sun_direction", f3),
         ("sun_irradiance", f3),
         ("camera", TrCamera),
+        ("env_width", C.c_int32), ("env_height", C.c_int32),
+        ("env_pixels", C.POINTER(C.c_float)),
+        ("env_world_to_light", C.c_float * 9),
     ]
 
 

@@ -37,8 +37,8 @@ HIP_SOURCES = {
     "capi.cpp": ["-x", "hip"],
     "guide_capi.cpp": ["-x", "hip"],
 }
-CXX_SOURCES = {"bvh_build.cpp": []}
-FACADE_SOURCES = ["facade.cpp", "scenes.cpp", "obj_load.cpp"]
+CXX_SOURCES = {"bvh_build.cpp": [], "envmap.cpp": ["-ffp-contract=off"]}
+FACADE_SOURCES = ["facade.cpp", "scenes.cpp", "obj_load.cpp", "image_load.cpp"]
 
 
 def _run(cmd: list[str]) -> None:

@@ -164,6 +164,22 @@ void validate_scene(const tr_scene_desc* d) {
     if (d->num_triangles <= 0 || !d->indices || !d->material_ids) throw InvalidArg("scene has no triangles");
     if (d->num_materials <= 0 || !d->materials) throw InvalidArg("scene has no materials");
     if (d->num_media < 0 || (d->num_media > 0 && !d->media)) throw InvalidArg("media array missing");
+    if (d->has_env && d->env_width != 0) {
+        if (d->env_width < 0 || d->env_height <= 0 || !d->env_pixels) throw InvalidArg("environment image: bad size or NULL texels");
+        if (int64_t(d->env_width) * d->env_height > (int64_t(1) << 28)) throw InvalidArg("environment image too large");
+        const size_t n = size_t(d->env_width) * size_t(d->env_height) * 3;
+        for (size_t i = 0; i < n; ++i)
+            if (!(d->env_pixels[i] >= 0.0f && d->env_pixels[i] < 3.0e38f))
+                throw InvalidArg("environment texels must be finite and >= 0");
+        // a rotation: R R^T = I (the light -> world map is its transpose)
+        const float* r = d->env_world_to_light;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                const float dotij = r[3 * i] * r[3 * j] + r[3 * i + 1] * r[3 * j + 1] + r[3 * i + 2] * r[3 * j + 2];
+                if (!(std::fabs(dotij - (i == j ? 1.0f : 0.0f)) < 1e-4f))
+                    throw InvalidArg("env_world_to_light must be a rotation");
+            }
+    }
     for (int64_t t = 0; t < d->num_triangles; ++t) {
         for (int k = 0; k < 3; ++k) {
             const int32_t v = d->indices[3 * t + k];
@@ -277,7 +293,13 @@ void build_device_scene(tr_ctx* c, const tr_bvh_opts* opts) {
             power[size_t(li)] = double(kPi) * lum(L) * area;   // DiffuseLight::power, diffuse_light.cpp:17-20
         }
     }
-    if (h.has_env) power[size_t(nA)] = double(lum(mk(h.env[0], h.env[1], h.env[2]))) * 4.0 * kPi * wr * wr;
+    // IBL::power (environment_light.cpp:81-83): the radiance integral times R^2
+    if (h.has_env && h.envmap.w > 0) {
+        const double* ri = h.envmap.radiance_integral;
+        power[size_t(nA)] = (0.2126 * ri[0] + 0.7152 * ri[1] + 0.0722 * ri[2]) * double(wr) * double(wr);
+    } else if (h.has_env) {
+        power[size_t(nA)] = double(lum(mk(h.env[0], h.env[1], h.env[2]))) * 4.0 * kPi * wr * wr;
+    }
     if (h.has_sun) power[size_t(nA) + 1] = double(lum(mk(h.sun_E[0], h.sun_E[1], h.sun_E[2]))) * kPi * wr * wr;
     double tot = 0.0;
     for (double p : power) tot += p;
@@ -310,6 +332,30 @@ void build_device_scene(tr_ctx* c, const tr_bvh_opts* opts) {
     upload(c->light_cdf, cdf.data(), cdf.size() * sizeof(float), st);
     upload(c->light_pmf, pmf.data(), pmf.size() * sizeof(float), st);
     upload(c->light_guide, guide.data(), guide.size() * sizeof(int), st);
+    const EnvMapHost& em = h.envmap;
+    DevEnvMap env{};
+    if (h.has_env && em.w > 0) {
+        std::vector<float4> tex(size_t(em.w) * size_t(em.h));
+        for (size_t i = 0; i < tex.size(); ++i) tex[i] = make_float4(em.rgb[3 * i], em.rgb[3 * i + 1], em.rgb[3 * i + 2], 0.0f);
+        upload(c->env_tex, tex.data(), tex.size() * sizeof(float4), st);
+        upload(c->env_cond_cdf, em.cond_cdf.data(), em.cond_cdf.size() * sizeof(float), st);
+        upload(c->env_cond_func, em.cond_func.data(), em.cond_func.size() * sizeof(float), st);
+        upload(c->env_marg_cdf, em.marg_cdf.data(), em.marg_cdf.size() * sizeof(float), st);
+        upload(c->env_marg_func, em.marg_func.data(), em.marg_func.size() * sizeof(float), st);
+        env.tex = c->env_tex.as<float4>();
+        env.cond_cdf = c->env_cond_cdf.as<float>();
+        env.cond_func = c->env_cond_func.as<float>();
+        env.marg_cdf = c->env_marg_cdf.as<float>();
+        env.marg_func = c->env_marg_func.as<float>();
+        env.w = em.w;
+        env.h = em.h;
+        env.nu = em.nu;
+        env.nv = em.nv;
+        env.marg_int = em.marg_int;
+        env.r0 = mk(em.rot[0], em.rot[1], em.rot[2]);
+        env.r1 = mk(em.rot[3], em.rot[4], em.rot[5]);
+        env.r2 = mk(em.rot[6], em.rot[7], em.rot[8]);
+    }
     hip_check(hipStreamSynchronize(st), "scene upload");
 
     // traversal spill area: (max_stack - LDS entries) per thread of the largest grid
@@ -336,6 +382,7 @@ void build_device_scene(tr_ctx* c, const tr_bvh_opts* opts) {
     s.has_env = h.has_env;
     s.has_sun = h.has_sun;
     s.env_L = mk(h.env[0], h.env[1], h.env[2]);
+    s.env = env;
     const f3 sd = mk(h.sun_dir[0], h.sun_dir[1], h.sun_dir[2]);
     s.sun_dir = length(sd) > 0.0f ? normalize(sd) : mk(0.0f, 0.0f, 1.0f);
     s.sun_E = mk(h.sun_E[0], h.sun_E[1], h.sun_E[2]);
@@ -506,6 +553,15 @@ tr_status tr_scene_upload(tr_ctx* c, const tr_scene_desc* d) {
         h.has_env = d->has_env;
         h.has_sun = d->has_sun;
         std::memcpy(h.env, d->env_radiance, sizeof h.env);
+        h.envmap = EnvMapHost{};
+        if (d->has_env && d->env_width > 0) {
+            EnvMapHost& e = h.envmap;
+            e.w = d->env_width;
+            e.h = d->env_height;
+            e.rgb.assign(d->env_pixels, d->env_pixels + size_t(e.w) * size_t(e.h) * 3);
+            std::memcpy(e.rot, d->env_world_to_light, sizeof e.rot);
+            build_envmap(e);
+        }
         std::memcpy(h.sun_dir, d->sun_direction, sizeof h.sun_dir);
         std::memcpy(h.sun_E, d->sun_irradiance, sizeof h.sun_E);
         h.cam = d->camera;
@@ -649,6 +705,49 @@ tr_status tr_bsdf_eval_batch(tr_ctx* c, const tr_material* mats, int32_t num_mat
                          reinterpret_cast<float*>(out), reinterpret_cast<float*>(out + v3), c->stream);
         hip_check(hipGetLastError(), "k_bsdf_eval launch");
         hip_check(hipMemcpyAsync(f, out, v3, hipMemcpyDeviceToHost, c->stream), "D2H");
+        hip_check(hipMemcpyAsync(pdf, out + v3, size_t(n) * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+    });
+}
+
+static void env_common(tr_ctx* c, const void* a, const void* b, const void* d, int64_t n) {
+    if (!c->have_bvh) throw StateError("environment batch before tr_bvh_build");
+    if (!c->hs.has_env) throw StateError("the scene has no environment light");
+    if (n < 0 || n > (int64_t(1) << 26)) throw InvalidArg("n must lie in [0, 2^26]");
+    if (!a || !b || !d) throw InvalidArg("NULL array");
+}
+
+tr_status tr_env_sample_batch(tr_ctx* c, const float* u, int64_t n, float* wi, float* pdf, float* L) {
+    return guarded(c, [&] {
+        env_common(c, u, wi, pdf, n);
+        if (!L) throw InvalidArg("NULL array");
+        if (n == 0) return;
+        const size_t v3 = size_t(n) * 12;
+        upload(c->s_in, u, size_t(n) * 8, c->stream);
+        c->s_out.reserve(2 * v3 + size_t(n) * 4);
+        char* out = c->s_out.as<char>();
+        launch_env_sample(c->ds, c->s_in.as<float>(), n, reinterpret_cast<float*>(out),
+                          reinterpret_cast<float*>(out + 2 * v3), reinterpret_cast<float*>(out + v3), c->stream);
+        hip_check(hipGetLastError(), "k_env_sample launch");
+        hip_check(hipMemcpyAsync(wi, out, v3, hipMemcpyDeviceToHost, c->stream), "D2H");
+        hip_check(hipMemcpyAsync(L, out + v3, v3, hipMemcpyDeviceToHost, c->stream), "D2H");
+        hip_check(hipMemcpyAsync(pdf, out + 2 * v3, size_t(n) * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+    });
+}
+
+tr_status tr_env_eval_batch(tr_ctx* c, const float* dirs, int64_t n, float* L, float* pdf) {
+    return guarded(c, [&] {
+        env_common(c, dirs, L, pdf, n);
+        if (n == 0) return;
+        const size_t v3 = size_t(n) * 12;
+        upload(c->s_in, dirs, v3, c->stream);
+        c->s_out.reserve(v3 + size_t(n) * 4);
+        char* out = c->s_out.as<char>();
+        launch_env_eval(c->ds, c->s_in.as<float>(), n, reinterpret_cast<float*>(out), reinterpret_cast<float*>(out + v3),
+                        c->stream);
+        hip_check(hipGetLastError(), "k_env_eval launch");
+        hip_check(hipMemcpyAsync(L, out, v3, hipMemcpyDeviceToHost, c->stream), "D2H");
         hip_check(hipMemcpyAsync(pdf, out + v3, size_t(n) * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
         hip_check(hipStreamSynchronize(c->stream), "sync");
     });

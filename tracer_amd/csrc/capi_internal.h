@@ -8,6 +8,7 @@
 #include <string>
 #include <vector>
 
+#include "envmap.h"
 #include "tr_guide_kernels.h"
 #include "tr_kernels.h"
 
@@ -66,6 +67,7 @@ struct HostScene {
     float env[3] = {0, 0, 0}, sun_dir[3] = {0, 0, 1}, sun_E[3] = {0, 0, 0};
     tr_camera cam{};
     float lo[3], hi[3];
+    EnvMapHost envmap;   // w > 0: image environment
 };
 
 // Guide network, its optimiser state and the buffers of guided rendering.
@@ -114,6 +116,7 @@ struct tr_ctx {
     bool has_delta = false;   // a material samples specular (delta) lobes: paths may take extra bounces
     // scene on device
     trcapi::DevBuf nodes, tris, tri_shade, mats, media, lights, light_cdf, light_pmf, light_guide;
+    trcapi::DevBuf env_tex, env_cond_cdf, env_cond_func, env_marg_cdf, env_marg_func;
     int64_t num_nodes = 0;
     int max_stack = 0;
     int bvh_depth = 0;

@@ -166,11 +166,13 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
     const int tri = __float_as_int(h.y);
     bool alive = true;
     if (tri < 0) {
-        // miss: constant environment (environment_light.cpp:179-188)
+        // miss: the environment (environment_light.cpp:179-196)
         if (s.has_env) {
+            float epdf;
+            const f3 Le = env_eval(s, d, epdf);
             float w = 1.0f;
-            if (prev_pdf > 0.0f) w = power_heuristic(prev_pdf, s.light_pmf[s.num_area_lights] * (1.0f / (4.0f * kPi)));
-            L += T * s.env_L * w;
+            if (prev_pdf > 0.0f) w = power_heuristic(prev_pdf, s.light_pmf[s.num_area_lights] * epdf);
+            L += T * Le * w;
         }
         alive = false;
     } else {
@@ -700,6 +702,33 @@ void launch_bsdf_sample(const DevMaterial* mats, const int* ids, const float* wo
                         float* wi, float* f, float* pdf, hipStream_t st) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_bsdf_sample, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, mats, ids, wo, ng, u, n, wi, f, pdf);
+}
+
+// Environment light of the uploaded scene: samples (u: n*2) and lookups
+// (dirs: n*3) through the renderer's own functions.
+__global__ void k_env_sample(DevScene s, const float* u, int64_t n, float* wi, float* pdf, float* L) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const LightSampleD r = sample_light(s, s.num_area_lights, mk(0.0f, 0.0f, 0.0f), u[2 * i], u[2 * i + 1]);
+    wi[3 * i] = r.wi.x; wi[3 * i + 1] = r.wi.y; wi[3 * i + 2] = r.wi.z;
+    L[3 * i] = r.L.x; L[3 * i + 1] = r.L.y; L[3 * i + 2] = r.L.z;
+    pdf[i] = r.pdf;
+}
+__global__ void k_env_eval(DevScene s, const float* d, int64_t n, float* L, float* pdf) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float p;
+    const f3 e = env_eval(s, normalize(mk(d[3 * i], d[3 * i + 1], d[3 * i + 2])), p);
+    L[3 * i] = e.x; L[3 * i + 1] = e.y; L[3 * i + 2] = e.z;
+    pdf[i] = p;
+}
+void launch_env_sample(const DevScene& s, const float* u, int64_t n, float* wi, float* pdf, float* L, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_env_sample, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, s, u, n, wi, pdf, L);
+}
+void launch_env_eval(const DevScene& s, const float* d, int64_t n, float* L, float* pdf, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_env_eval, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, s, d, n, L, pdf);
 }
 
 }  // namespace trhip

@@ -27,6 +27,20 @@ struct DevCamera {
     float film_w, film_h, focal, lens_radius;
 };
 
+// Image environment (IBL, src/light/environment_light.cpp:20-196): texels
+// and the Distribution2D built on the host (distribution.hpp:80-110).  w = 0
+// means a constant environment of radiance env_L.
+struct DevEnvMap {
+    const float4* tex;          // h rows of w texels (RGB), row 0 at v = 0
+    const float* cond_cdf;      // nv rows of nu + 1
+    const float* cond_func;     // nv rows of nu
+    const float* marg_cdf;      // nv + 1
+    const float* marg_func;     // nv
+    int w, h, nu, nv;
+    float marg_int;
+    f3 r0, r1, r2;              // rows of the world -> light rotation
+};
+
 struct DevScene {
     TraceScene tr;
     const float4* tri_shade;   // 3 float4 per leaf-ordered triangle: (n0, material), (n1, light), (n2, media)
@@ -41,6 +55,7 @@ struct DevScene {
     int num_light_entries;
     int has_env, has_sun;
     f3 env_L;
+    DevEnvMap env;
     f3 sun_dir;
     f3 sun_E;
     f3 wcenter;
@@ -81,6 +96,69 @@ TR_D int pick_light(const DevScene& s, float u, float& pmf) {
     return lo;
 }
 
+// Texture2D::evaluate (core/texture.hpp:57-83: wrap_repeat) and
+// LinearSampler::Sample2D (utility/sampler.hpp:12-25) on the environment.
+TR_D f3 env_texel(const DevEnvMap& e, float u, float v) {
+    u = fminf(fmaxf(u - floorf(u), 0.0f), 1.0f) * float(e.w - 1);
+    v = fminf(fmaxf(v - floorf(v), 0.0f), 1.0f) * float(e.h - 1);
+    const int u0 = min(int(u), e.w - 1), u1 = min(u0 + 1, e.w - 1);
+    const int v0 = min(int(v), e.h - 1), v1 = min(v0 + 1, e.h - 1);
+    const float du = u - float(u0), dv = v - float(v0);
+    const f3 a = xyz(e.tex[size_t(v0) * e.w + u0]), b = xyz(e.tex[size_t(v0) * e.w + u1]);
+    const f3 c = xyz(e.tex[size_t(v1) * e.w + u0]), d = xyz(e.tex[size_t(v1) * e.w + u1]);
+    return (a * (1.0f - du) + b * du) * (1.0f - dv) + (c * (1.0f - du) + d * du) * dv;
+}
+
+// Distribution1D::sample_continuous (distribution.hpp:33-47): the last cdf
+// entry not above u, the offset within it, the cell's pdf.
+TR_D float sample_cont(const float* cdf, const float* func, float func_int, int n, float u, float& pdf, int& off) {
+    int lo = 0, hi = n - 1;   // largest i in [0, n - 1] with cdf[i] <= u
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (cdf[mid] <= u) lo = mid;
+        else hi = mid - 1;
+    }
+    off = lo;
+    float du = u - cdf[lo];
+    const float w = cdf[lo + 1] - cdf[lo];
+    if (w > 0.0f) du /= w;
+    pdf = func_int > 0.0f ? func[lo] / func_int : 0.0f;
+    return (float(lo) + du) / float(n);
+}
+
+// Light-space direction of a world direction and its (u, v) = (phi / 2pi, theta / pi).
+TR_D void env_uv(const DevEnvMap& e, f3 d, float& u, float& v, float& sin_theta) {
+    const f3 l = normalize(mk(dot(e.r0, d), dot(e.r1, d), dot(e.r2, d)));
+    float phi = (l.x == 0.0f && l.y == 0.0f) ? 0.0f : atan2f(l.y, l.x);
+    if (phi < 0.0f) phi += 2.0f * kPi;
+    const float theta = acosf(fminf(fmaxf(l.z, -1.0f), 1.0f));
+    u = fminf(fmaxf(phi * (0.5f / kPi), 0.0f), 1.0f);
+    v = fminf(fmaxf(theta * (1.0f / kPi), 0.0f), 1.0f);
+    sin_theta = sinf(theta);
+}
+
+// IBL::light_emit and IBL::pdf (environment_light.cpp:179-196) for a world
+// direction.  The pdf is the density sample_env draws:
+// conditional func / marginal integral / (2 pi^2 sin theta) (the reference's
+// pdf() multiplies by the marginal integral instead, defect Q28).
+TR_D f3 env_eval(const DevScene& s, f3 d, float& pdf) {
+    const DevEnvMap& e = s.env;
+    if (e.w == 0) {
+        pdf = 1.0f / (4.0f * kPi);
+        return s.env_L;
+    }
+    float u, v, st;
+    env_uv(e, d, u, v, st);
+    if (st == 0.0f || !(e.marg_int > 0.0f)) {
+        pdf = 0.0f;
+    } else {
+        const int iu = min(max(int(u * float(e.nu)), 0), e.nu - 1);
+        const int iv = min(max(int(v * float(e.nv)), 0), e.nv - 1);
+        pdf = e.cond_func[size_t(iv) * e.nu + iu] / e.marg_int / (2.0f * kPi * kPi * st);
+    }
+    return env_texel(e, u, v);
+}
+
 struct LightSampleD {
     f3 pos;     // point on the light (area) or a far point (env / sun)
     f3 wi;      // unit direction from the reference point
@@ -92,8 +170,8 @@ struct LightSampleD {
 
 // DiffuseLight::sample_li (src/light/diffuse_light.cpp:44-60) over a triangle
 // (Triangle::sample, src/shape/triangle.cpp:61-97: b.x weights A, b.y weights B);
-// the environment is sampled uniformly over the sphere and the sun is a
-// delta direction.
+// a constant environment is sampled uniformly over the sphere, an image one
+// by its 2D distribution, and the sun is a delta direction.
 TR_D LightSampleD sample_light(const DevScene& s, int entry, f3 ref, float u, float v) {
     LightSampleD r;
     r.pdf = 0.0f;
@@ -117,6 +195,32 @@ TR_D LightSampleD sample_light(const DevScene& s, int entry, f3 ref, float u, fl
         r.L = xyz(l0);
         r.pdf = d2 / (l0.w * cosl);
         r.area = true;
+        return r;
+    }
+    if (entry == s.num_area_lights && s.env.w > 0) {
+        // IBL::sample_li (environment_light.cpp:92-119): v picks theta, u phi
+        const DevEnvMap& e = s.env;
+        float pv, pu;
+        int iv, iu;
+        const float d1 = sample_cont(e.marg_cdf, e.marg_func, e.marg_int, e.nv, v, pv, iv);
+        const float* ccdf = e.cond_cdf + size_t(iv) * (e.nu + 1);
+        const float* cfun = e.cond_func + size_t(iv) * e.nu;
+        const float d0 = sample_cont(ccdf, cfun, e.marg_func[iv], e.nu, u, pu, iu);
+        const float map_pdf = pu * pv;
+        if (!(map_pdf > 0.0f)) return r;
+        const float theta = d1 * kPi, phi = d0 * 2.0f * kPi;
+        float st, ct, sp, cp;
+        sincosf(theta, &st, &ct);
+        sincosf(phi, &sp, &cp);
+        if (st == 0.0f) return r;
+        const f3 l = mk(st * cp, st * sp, ct);
+        // light -> world is the transpose of the world -> light rotation
+        r.wi = normalize(mk(e.r0.x * l.x + e.r1.x * l.y + e.r2.x * l.z, e.r0.y * l.x + e.r1.y * l.y + e.r2.y * l.z,
+                            e.r0.z * l.x + e.r1.z * l.y + e.r2.z * l.z));
+        r.dist = 2.0f * s.wradius;
+        r.pos = ref + r.wi * r.dist;
+        r.L = env_texel(e, d0, d1);
+        r.pdf = map_pdf / (2.0f * kPi * kPi * st);
         return r;
     }
     if (entry == s.num_area_lights) {

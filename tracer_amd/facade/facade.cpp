@@ -198,11 +198,16 @@ RC<Medium> create_vacuum() {   // vacuum.cpp:26-29: one shared instance
     return v;
 }
 
-RC<EnvironmentLight> create_ibl_light(RC<const Texture2D> tex, const Transform&) {
+// environment_light.cpp:198-203: a constant texture or an image from
+// create_texture2d_from_file; world_to_light is the rotation part of t.
+RC<EnvironmentLight> create_ibl_light(RC<const Texture2D> tex, const Transform& t) {
     if (!tex) throw std::runtime_error("create_ibl_light: texture must not be null");
-    if (!tex->is_constant()) throw std::runtime_error("create_ibl_light: only constant textures are supported by the GPU path");
+    if (!tex->is_constant() && !dynamic_cast<const ImageTexture2D*>(tex.get()))
+        throw std::runtime_error("create_ibl_light: the GPU path takes a constant texture or an .hdr image");
     auto l = newRC<IBLConstant>();
     l->tex = tex;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) l->world_to_light[i][j] = t.m[i][j];
     return l;
 }
 RC<Light> create_directional_light(const Vector3f& dir, const Spectrum& E) {
@@ -278,6 +283,10 @@ tr_scene_desc FlatScene::desc() const {
     std::memcpy(d.sun_direction, sun_dir, sizeof sun_dir);
     std::memcpy(d.sun_irradiance, sun_E, sizeof sun_E);
     d.camera = cam;
+    d.env_width = env_w;
+    d.env_height = env_h;
+    d.env_pixels = env_pix.empty() ? nullptr : env_pix.data();
+    std::memcpy(d.env_world_to_light, env_rot, sizeof env_rot);
     return d;
 }
 
@@ -418,8 +427,16 @@ FlatScene flatten(const Scene& scene) {
     if (gs->environment_light) {
         const auto* ibl = dynamic_cast<const IBLConstant*>(gs->environment_light.get());
         if (!ibl) throw std::runtime_error("render: unsupported environment light");
-        tex_rgb(ibl->tex, f.env);
         f.has_env = 1;
+        if (const auto* img = dynamic_cast<const ImageTexture2D*>(ibl->tex.get())) {
+            f.env_w = img->w;
+            f.env_h = img->h;
+            f.env_pix = img->rgb;
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) f.env_rot[3 * i + j] = ibl->world_to_light[i][j];
+        } else {
+            tex_rgb(ibl->tex, f.env);
+        }
     }
     for (const Light* l : gs->lights) {
         if (const auto* dl = dynamic_cast<const DirectionalLight*>(l)) {

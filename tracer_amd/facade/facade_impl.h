@@ -18,6 +18,17 @@ public:
     Spectrum c;
 };
 
+// An image loaded by create_texture2d_from_file: h rows of w RGB texels,
+// sampled bilinearly with wrap_repeat.
+class ImageTexture2D : public Texture2D {
+public:
+    Spectrum evaluate(const Point2f& uv) const override;
+    int w = 0, h = 0;
+    std::vector<float> rgb;
+};
+// Radiance RGBE (.hdr) reader; throws std::runtime_error on a read or format error.
+void read_hdr(const std::string& path, int& width, int& height, std::vector<float>& rgb);
+
 class DisneyMaterial : public Material {
 public:
     RC<const Texture2D> base_color, metallic, roughness, transmission, transmission_roughness, ior, specular_scale,
@@ -61,6 +72,7 @@ public:
 class IBLConstant : public EnvironmentLight {
 public:
     RC<const Texture2D> tex;
+    real world_to_light[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
 };
 class DirectionalLight : public Light {
 public:
@@ -123,6 +135,9 @@ struct FlatScene {
     std::vector<tr_medium> media;
     int has_env = 0, has_sun = 0;
     float env[3] = {0, 0, 0}, sun_dir[3] = {0, 0, 1}, sun_E[3] = {0, 0, 0};
+    int env_w = 0, env_h = 0;             // image environment
+    std::vector<float> env_pix;
+    float env_rot[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
     tr_camera cam{};
     int max_leaf = 3;
     tr_scene_desc desc() const;

@@ -65,6 +65,8 @@ def hip_lib() -> C.CDLL:
                                            P(C.c_float), I64, P(C.c_float), P(C.c_float)]
         lib.tr_bsdf_sample_batch.argtypes = [VP, P(TrMaterial), I32, P(C.c_int32), P(C.c_float), P(C.c_float),
                                              P(C.c_float), I64, P(C.c_float), P(C.c_float), P(C.c_float)]
+        lib.tr_env_sample_batch.argtypes = [VP, P(C.c_float), I64, P(C.c_float), P(C.c_float), P(C.c_float)]
+        lib.tr_env_eval_batch.argtypes = [VP, P(C.c_float), I64, P(C.c_float), P(C.c_float)]
         F = P(C.c_float)
         lib.tr_guide_default_config.argtypes = [P(abi.TrGuideConfig)]
         lib.tr_guide_default_config.restype = None
@@ -94,6 +96,9 @@ def scenes_lib() -> C.CDLL:
         lib.trs_create.restype = C.c_void_p
         lib.trs_load_obj.argtypes = [C.c_char_p, C.POINTER(abi.TrCamera), C.c_int32, C.POINTER(C.c_char_p)]
         lib.trs_load_obj.restype = C.c_void_p
+        lib.trs_read_hdr.argtypes = [C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_float),
+                                     C.c_int64, C.POINTER(C.c_char_p)]
+        lib.trs_read_hdr.restype = C.c_int32
         lib.trs_desc.argtypes = [C.c_void_p]
         lib.trs_desc.restype = C.POINTER(TrSceneDesc)
         lib.trs_render_defaults.argtypes = [C.c_void_p, C.POINTER(TrRenderParams)]
@@ -123,6 +128,8 @@ class Scene:
     medium_outside: np.ndarray | None = None
     media: list = field(default_factory=list)
     env: tuple | None = None
+    env_image: np.ndarray | None = None   # (h, w, 3) linear RGB: an image environment (needs env set)
+    env_world_to_light: np.ndarray | None = None   # 3x3 rotation
     sun: tuple | None = None          # (direction, irradiance)
     camera: abi.TrCamera | None = None
     defaults: TrRenderParams | None = None
@@ -169,6 +176,12 @@ class Scene:
         if self.env is not None:
             d.has_env = 1
             d.env_radiance = abi.f3(*self.env)
+            if self.env_image is not None:
+                self.env_image = np.ascontiguousarray(self.env_image, np.float32)
+                d.env_height, d.env_width = self.env_image.shape[:2]
+                d.env_pixels = fptr(self.env_image)
+                rot = np.eye(3) if self.env_world_to_light is None else np.asarray(self.env_world_to_light)
+                d.env_world_to_light = (C.c_float * 9)(*map(float, np.asarray(rot, np.float32).reshape(9)))
         if self.sun is not None:
             d.has_sun = 1
             d.sun_direction = abi.f3(*self.sun[0])
@@ -209,6 +222,21 @@ def obj_scene(path, camera: abi.TrCamera, outside_fog: bool = True) -> Scene:
     if not h:
         raise TracerError(abi.TR_ERR_INVALID, (err.value or b"").decode())
     return _scene_from_handle(lib, h)
+
+
+def read_hdr(path) -> np.ndarray:
+    """A Radiance .hdr image as (height, width, 3) linear RGB, read by the
+    C++ facade (the loader behind create_texture2d_from_file)."""
+    lib = scenes_lib()
+    err = C.c_char_p()
+    w, h = C.c_int32(), C.c_int32()
+    p = str(path).encode()
+    if lib.trs_read_hdr(p, C.byref(w), C.byref(h), None, 0, C.byref(err)) != 0:
+        raise TracerError(abi.TR_ERR_INVALID, (err.value or b"").decode())
+    out = np.zeros((h.value, w.value, 3), np.float32)
+    if lib.trs_read_hdr(p, C.byref(w), C.byref(h), fptr(out), out.size, C.byref(err)) != 0:
+        raise TracerError(abi.TR_ERR_INVALID, (err.value or b"").decode())
+    return out
 
 
 def _scene_from_handle(lib, h) -> Scene:
@@ -348,6 +376,22 @@ class Context:
         self._check(self.lib.tr_bsdf_eval_batch(self.h, arr, len(mats), iptr(ids), fptr(wi), fptr(wo), fptr(ngc), n,
                                                 fptr(f), fptr(pdf)))
         return f, pdf
+
+    def env_sample(self, u: np.ndarray):
+        """The loaded scene's environment light: u (n, 2) -> (wi, pdf, L)."""
+        u = np.ascontiguousarray(u, np.float32)
+        n = u.shape[0]
+        wi, pdf, L = np.zeros((n, 3), np.float32), np.zeros(n, np.float32), np.zeros((n, 3), np.float32)
+        self._check(self.lib.tr_env_sample_batch(self.h, fptr(u), n, fptr(wi), fptr(pdf), fptr(L)))
+        return wi, pdf, L
+
+    def env_eval(self, dirs: np.ndarray):
+        """Radiance and light pdf of the environment at unit directions (n, 3)."""
+        d = np.ascontiguousarray(dirs, np.float32)
+        n = d.shape[0]
+        L, pdf = np.zeros((n, 3), np.float32), np.zeros(n, np.float32)
+        self._check(self.lib.tr_env_eval_batch(self.h, fptr(d), n, fptr(L), fptr(pdf)))
+        return L, pdf
 
     def bsdf_sample(self, mats: list, ids: np.ndarray, wo: np.ndarray, u: np.ndarray, ng: np.ndarray | None = None):
         n = len(ids)
