@@ -30,7 +30,10 @@ Also reported on the same JSON line:
                  configuration 2 at a reduced film, timed on this host, with
                  the GPU's time-to-equal-relMSE speedup on the same film.
   other_configs  configuration 3 (fog) full frame, configuration 5 (18.9M
-                 triangles) at 16 spp.
+                 triangles) at 16 spp, configuration 2 under a 2048 x 1024
+                 image environment instead of its constant one, and OBJ
+                 ingest: a 328K-triangle OBJ/MTL pair read and flattened by
+                 the facade (MB/s, triangles/s) and rendered.
 """
 from __future__ import annotations
 
@@ -142,6 +145,62 @@ def other_configs() -> dict:
         }
         print(f"[bench] configuration {cfg}: {out[f'config{cfg}']['samples_per_s'] / 1e6:.1f} M samples/s",
               file=sys.stderr, flush=True)
+
+    def frame(scene, p):
+        ctx.load_scene(scene)
+        ctx.render(scene.render_params(seed=9, spp=1, width=p.width, height=p.height), want_image=False)
+        t0 = time.perf_counter()
+        _, _, st = ctx.render(p, want_image=False)
+        el = time.perf_counter() - t0
+        return {"samples_per_s": p.width * p.height * p.spp / el,
+                "mrays_per_s": (st.extension_rays + st.shadow_rays) / el / 1e6, "s_per_frame": el}
+
+    # configuration 2 under an image environment: a sky gradient with a sun disc
+    scene = native.config_scene(2)
+    H, W = 1024, 2048
+    v = (np.arange(H) + 0.5) / H
+    u = (np.arange(W) + 0.5) / W
+    U, V = np.meshgrid(u, v)
+    sky = (0.2 + 0.6 * np.clip(1 - V, 0, 1))[..., None] * np.array([0.6, 0.75, 1.0])
+    sky = sky + 200.0 * (((U - 0.3) ** 2 + (V - 0.25) ** 2) < 2e-4)[..., None]
+    scene.env_image = sky.astype(np.float32)
+    p = scene.render_params(seed=0)
+    out["config2_env_image"] = {"workload": f"configuration 2 with a {W}x{H} image environment, {p.width}x{p.height}, "
+                                            f"{p.spp} spp, max depth {p.max_depth}", **frame(scene, p)}
+    print(f"[bench] configuration 2, image environment: {out['config2_env_image']['samples_per_s'] / 1e6:.1f} M samples/s",
+          file=sys.stderr, flush=True)
+
+    # OBJ ingest: configuration 2's spheres at a quarter of the size, written as OBJ + MTL
+    import tempfile
+    from tracer_amd import abi
+    src = native.config_scene(2, detail=0.25)
+    with tempfile.TemporaryDirectory() as tmp:
+        P = src.positions.reshape(-1, 3)
+        N = src.normals.reshape(-1, 3)
+        I = src.indices.reshape(-1, 3) + 1
+        lines = ["mtllib m.mtl", "usemtl a"]
+        body = "\n".join(lines) + "\n"
+        body += "".join(f"v {x:.7g} {y:.7g} {z:.7g}\n" for x, y, z in P)
+        body += "".join(f"vn {x:.7g} {y:.7g} {z:.7g}\n" for x, y, z in N)
+        body += "".join(f"f {a}//{a} {b}//{b} {c}//{c}\n" for a, b, c in I)
+        path = Path(tmp) / "m.obj"
+        path.write_text(body)
+        (Path(tmp) / "m.mtl").write_text("newmtl a\nKd 0.6 0.5 0.4\nKs 0.2 0.2 0.2\nNs 30\n")
+        cam = abi.TrCamera.from_buffer_copy(src.camera)   # a copy the loader may keep
+        t0 = time.perf_counter()
+        obj = native.obj_scene(path, cam, outside_fog=False)
+        parse_s = time.perf_counter() - t0
+        size_mb = path.stat().st_size / 1e6
+    obj.env = (0.5, 0.5, 0.5)
+    p = obj.render_params(width=1024, height=1024, spp=16, max_depth=16, seed=0)
+    out["obj_ingest"] = {
+        "workload": f"{obj.num_triangles} triangles in a {size_mb:.1f} MB OBJ (v//vn corners, one Phong material), "
+                    f"rendered at {p.width}x{p.height}, {p.spp} spp under a constant environment",
+        "ingest_s": parse_s, "ingest_mb_per_s": size_mb / parse_s, "ingest_triangles_per_s": obj.num_triangles / parse_s,
+        "ingest_covers": "load_model_from_file, run()-style wiring, flattening to tr_scene_desc, copy to Python",
+        **frame(obj, p)}
+    print(f"[bench] OBJ ingest: {size_mb / parse_s:.1f} MB/s, {obj.num_triangles / parse_s / 1e6:.2f} M triangles/s; "
+          f"render {out['obj_ingest']['samples_per_s'] / 1e6:.1f} M samples/s", file=sys.stderr, flush=True)
     ctx.close()
     return out
 

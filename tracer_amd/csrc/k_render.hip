@@ -136,7 +136,8 @@ struct ShadeOut {
 
 // One vertex of the path in `slot`: reads its state and hit, adds emission,
 // samples the continuation and a light, and stores the updated state.
-template <bool GUIDE>
+// IMG: the scene has an image environment (its sampling code is compiled in).
+template <bool GUIDE, bool IMG>
 TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c, int slot, ShadeOut& out) {
     bool& cont = out.cont;
     ShadowOut& sh = out.sh;
@@ -169,7 +170,7 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
         // miss: the environment (environment_light.cpp:179-196)
         if (s.has_env) {
             float epdf;
-            const f3 Le = env_eval(s, d, epdf);
+            const f3 Le = env_eval<IMG>(s, d, epdf);
             float w = 1.0f;
             if (prev_pdf > 0.0f) w = power_heuristic(prev_pdf, s.light_pmf[s.num_area_lights] * epdf);
             L += T * Le * w;
@@ -219,7 +220,7 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
                 float pmf;
                 const int e = pick_light(s, rng.uniform(), pmf);
                 const float lu = rng.uniform(), lv = rng.uniform();
-                const LightSampleD ls = sample_light(s, e, sp_pos, lu, lv);
+                const LightSampleD ls = sample_light<IMG>(s, e, sp_pos, lu, lv);
                 if (ls.pdf != 0.0f && pmf > 0.0f && !is_black(ls.L)) {
                     const float ph = hg_phase(m.g, -dot(ls.wi, wo));
                     float w = 1.0f;
@@ -319,7 +320,7 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
                         float pmf;
                         const int e = pick_light(s, rng.uniform(), pmf);
                         const float lu = rng.uniform(), lv = rng.uniform();
-                        const LightSampleD ls = sample_light(s, e, pos, lu, lv);
+                        const LightSampleD ls = sample_light<IMG>(s, e, pos, lu, lv);
                         if (ls.pdf != 0.0f && pmf > 0.0f && !is_black(ls.L)) {
                             const float dist = ls.area ? ls.dist - kEps : ls.dist;
                             if (dist > kEps) {
@@ -429,7 +430,7 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
 // with the combined pdf.  A vertex
 // whose head outputs are not finite uses the plain BSDF estimator, drawing
 // the same random numbers as the plain kernel.
-template <bool GUIDE>
+template <bool GUIDE, bool IMG>
 __global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene s, DevPaths p, RenderConsts c, int cur) {
     __shared__ BlockAppendLds<kShadeBlock> app;
     const unsigned int n = *queue_counter(p, cur);
@@ -450,7 +451,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene s, DevPaths p, R
         int slot = 0;
         if (active) {
             slot = q[i];
-            shade_path<GUIDE>(s, p, c, slot, o);
+            shade_path<GUIDE, IMG>(s, p, c, slot, o);
         }
         const bool cont = o.cont, g_used = o.g_used, g_fallback = o.g_fallback;
         const bool want_rec = o.want_rec, want_nee_rec = o.want_nee_rec;
@@ -518,6 +519,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene s, DevPaths p, R
 // same order as in the per-bounce kernels, so the film is identical.  The
 // last workgroup to finish empties the queue, so the per-bounce kernels that
 // follow find no work; with more than max_tail paths nothing happens.
+template <bool IMG>
 __global__ __launch_bounds__(kShadeBlock) void k_tail(DevScene s, DevPaths p, RenderConsts c, int cur,
                                                       unsigned int max_tail) {
     __shared__ TraceStack<kShadeBlock, kLdsStack> st;
@@ -540,7 +542,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_tail(DevScene s, DevPaths p, Re
             p.hit[slot] = make_float4(h.t, __int_as_float(h.tri), h.b1, h.b2);
             ++n_ext;
             ShadeOut o;
-            shade_path<false>(s, p, c, slot, o);
+            shade_path<false, IMG>(s, p, c, slot, o);
             if (o.sh.emit) {
                 ++n_sh;
                 TraceHit hs;
@@ -675,16 +677,21 @@ void launch_generate(const DevScene& s, const DevPaths& p, const RenderConsts& c
 }
 void launch_shade(const DevScene& s, const DevPaths& p, const RenderConsts& c, int cur, int max_paths, bool guide,
                   hipStream_t st) {
-    if (guide)
-        hipLaunchKernelGGL(k_shade<true>, dim3(shade_blocks(max_paths)), dim3(kShadeBlock), 0, st, s, p, c, cur);
-    else
-        hipLaunchKernelGGL(k_shade<false>, dim3(shade_blocks(max_paths)), dim3(kShadeBlock), 0, st, s, p, c, cur);
+    const dim3 g(shade_blocks(max_paths)), b(kShadeBlock);
+    const bool img = s.env.w > 0;
+    if (guide && img) hipLaunchKernelGGL((k_shade<true, true>), g, b, 0, st, s, p, c, cur);
+    else if (guide) hipLaunchKernelGGL((k_shade<true, false>), g, b, 0, st, s, p, c, cur);
+    else if (img) hipLaunchKernelGGL((k_shade<false, true>), g, b, 0, st, s, p, c, cur);
+    else hipLaunchKernelGGL((k_shade<false, false>), g, b, 0, st, s, p, c, cur);
 }
 void launch_tail(const DevScene& s, const DevPaths& p, const RenderConsts& c, int cur, int max_tail, hipStream_t st) {
     // one thread per path, and no more threads than the traversal spill area has columns
     int blocks = std::min((max_tail + kShadeBlock - 1) / kShadeBlock, p.ovf_threads / kShadeBlock);
     if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(k_tail, dim3(blocks), dim3(kShadeBlock), 0, st, s, p, c, cur, (unsigned int)max_tail);
+    if (s.env.w > 0)
+        hipLaunchKernelGGL(k_tail<true>, dim3(blocks), dim3(kShadeBlock), 0, st, s, p, c, cur, (unsigned int)max_tail);
+    else
+        hipLaunchKernelGGL(k_tail<false>, dim3(blocks), dim3(kShadeBlock), 0, st, s, p, c, cur, (unsigned int)max_tail);
 }
 void launch_accumulate(const DevPaths& p, const RenderConsts& c, double4* film, hipStream_t st) {
     const int n = c.width * c.height;
@@ -709,7 +716,7 @@ void launch_bsdf_sample(const DevMaterial* mats, const int* ids, const float* wo
 __global__ void k_env_sample(DevScene s, const float* u, int64_t n, float* wi, float* pdf, float* L) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const LightSampleD r = sample_light(s, s.num_area_lights, mk(0.0f, 0.0f, 0.0f), u[2 * i], u[2 * i + 1]);
+    const LightSampleD r = sample_light<true>(s, s.num_area_lights, mk(0.0f, 0.0f, 0.0f), u[2 * i], u[2 * i + 1]);
     wi[3 * i] = r.wi.x; wi[3 * i + 1] = r.wi.y; wi[3 * i + 2] = r.wi.z;
     L[3 * i] = r.L.x; L[3 * i + 1] = r.L.y; L[3 * i + 2] = r.L.z;
     pdf[i] = r.pdf;
@@ -718,7 +725,7 @@ __global__ void k_env_eval(DevScene s, const float* d, int64_t n, float* L, floa
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float p;
-    const f3 e = env_eval(s, normalize(mk(d[3 * i], d[3 * i + 1], d[3 * i + 2])), p);
+    const f3 e = env_eval<true>(s, normalize(mk(d[3 * i], d[3 * i + 1], d[3 * i + 2])), p);
     L[3 * i] = e.x; L[3 * i + 1] = e.y; L[3 * i + 2] = e.z;
     pdf[i] = p;
 }

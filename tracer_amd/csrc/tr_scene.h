@@ -141,9 +141,12 @@ TR_D void env_uv(const DevEnvMap& e, f3 d, float& u, float& v, float& sin_theta)
 // direction.  The pdf is the density sample_env draws:
 // conditional func / marginal integral / (2 pi^2 sin theta) (the reference's
 // pdf() multiplies by the marginal integral instead, defect Q28).
+// IMG = false compiles the constant environment only (kernels for scenes
+// without an image environment keep their register budget).
+template <bool IMG>
 TR_D f3 env_eval(const DevScene& s, f3 d, float& pdf) {
     const DevEnvMap& e = s.env;
-    if (e.w == 0) {
+    if (!IMG || e.w == 0) {
         pdf = 1.0f / (4.0f * kPi);
         return s.env_L;
     }
@@ -172,6 +175,7 @@ struct LightSampleD {
 // (Triangle::sample, src/shape/triangle.cpp:61-97: b.x weights A, b.y weights B);
 // a constant environment is sampled uniformly over the sphere, an image one
 // by its 2D distribution, and the sun is a delta direction.
+template <bool IMG>
 TR_D LightSampleD sample_light(const DevScene& s, int entry, f3 ref, float u, float v) {
     LightSampleD r;
     r.pdf = 0.0f;
@@ -197,7 +201,7 @@ TR_D LightSampleD sample_light(const DevScene& s, int entry, f3 ref, float u, fl
         r.area = true;
         return r;
     }
-    if (entry == s.num_area_lights && s.env.w > 0) {
+    if (IMG && entry == s.num_area_lights && s.env.w > 0) {
         // IBL::sample_li (environment_light.cpp:92-119): v picks theta, u phi
         const DevEnvMap& e = s.env;
         float pv, pu;
