@@ -39,6 +39,14 @@ trs_scene* trs_load_obj(const char* obj_path, const tr_camera* camera, int32_t o
  * *width, *height, and when rgb is not NULL its height rows of width linear
  * RGB texels (cap floats at least).  Returns 0, or 1 and *err on failure. */
 int32_t trs_read_hdr(const char* path, int32_t* width, int32_t* height, float* rgb, int64_t cap, const char** err);
+/* write_image_to_hdr / write_image_to_png of height rows of width RGB texels
+ * (floats / bytes).  Return 0, or 1 and *err on failure. */
+int32_t trs_write_hdr(const char* path, int32_t width, int32_t height, const float* rgb, const char** err);
+int32_t trs_write_png(const char* path, int32_t width, int32_t height, const uint8_t* rgb, const char** err);
+/* In place: create_aces_tone_mapper(aces_exposure) when aces_exposure >= 0,
+ * then create_gamma_corrector(gamma) when gamma > 0.  Returns 0, or 1 on bad
+ * arguments. */
+int32_t trs_postprocess(float* rgb, int32_t width, int32_t height, float aces_exposure, float gamma);
 const tr_scene_desc* trs_desc(const trs_scene* s);
 /* Film, spp and depths of the configuration (PTRendererParams + film). */
 void trs_render_defaults(const trs_scene* s, tr_render_params* out);

@@ -16,6 +16,8 @@
 //   create_constant_texture2d   src/factory/texture.hpp:12
 //   create_ibl_light            src/factory/light.hpp:12-15 (constant texture or .hdr image)
 //   create_texture2d_from_file  src/utility/image_file.hpp:21 (Radiance .hdr)
+//   write_image_to_hdr / _png   src/utility/image_file.hpp:12-15
+//   create_gamma_corrector, create_aces_tone_mapper  src/post_processor/*.cpp
 //   create_homogeneous_medium   src/factory/medium.hpp:21-26
 //   create_vacuum               src/factory/medium.hpp:34
 //   create_thin_lens_camera     src/factory/camera.hpp:9-11
@@ -305,6 +307,24 @@ RC<Scene> create_general_scene(const RC<Aggregate>& accel);
 struct RenderTarget {
     Image2D<Spectrum> color;
 };
+
+// Point3b / Color3b (src/utility/geometry.hpp:635): 8-bit RGB.
+struct Color3b {
+    uint8_t x = 0, y = 0, z = 0;
+};
+// Radiance RGBE file (run-length scanlines) and 8-bit RGB PNG; throw
+// std::runtime_error when the file cannot be written.
+void write_image_to_hdr(const Image2D<Spectrum>& image, const std::string& filename);
+void write_image_to_png(const Image2D<Color3b>& image, const std::string& filename);
+
+// src/core/post_processor.hpp:12-17
+class PostProcessor {
+public:
+    virtual ~PostProcessor() = default;
+    virtual void process(RenderTarget& render_target) = 0;
+};
+RC<PostProcessor> create_gamma_corrector(real gamma);        // every channel to the power gamma
+RC<PostProcessor> create_aces_tone_mapper(real exposure);    // fitted ACES curve of exposure * channel
 
 class Film {
 public:

@@ -265,6 +265,14 @@ tr_status tr_bsdf_sample_batch(tr_ctx* ctx, const tr_material* mats, int32_t num
                                const int32_t* mat_ids, const float* wo, const float* ng,
                                const float* u, int64_t n, float* wi, float* f, float* pdf);
 
+/* The film of the last tr_render (kept on the device) as 8-bit RGB, rows of
+ * width pixels (cap >= width * height * 3 bytes): colour = sums / weight,
+ * then create_aces_tone_mapper(aces_exposure) when aces_exposure >= 0
+ * (src/post_processor/aces.cpp), colour^inv_gamma when inv_gamma > 0
+ * (src/post_processor/gamma.cpp), then clamp(int(255 colour), 0, 255)
+ * (src/main.cpp:290-299). */
+tr_status tr_film_to_ldr(tr_ctx* ctx, float aces_exposure, float inv_gamma, uint8_t* rgb, int64_t cap);
+
 /* Environment light of the uploaded scene (after tr_bvh_build), for tests:
  * IBL::sample_li with uniforms u (n*2: u picks phi, v theta) -> unit
  * directions wi (n*3), solid-angle pdf (0 = no sample) and radiance L (n*3)

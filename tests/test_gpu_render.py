@@ -207,3 +207,27 @@ def test_obj_room_matches_fixed_estimator(gpu_ctx, tmp_path, fog, glass):
     s = native.obj_scene(path, objfiles.camera((0.0, 0.0, 3.6), (0.0, 0.0, 0.0)), outside_fog=fog)
     _compare_estimators(gpu_ctx, s, 32, 32, 512 if fog else 1024, 128 if fog else 256, 8, 8, quirks=False, nee=0,
                         max_depth=8)
+
+
+@pytest.mark.parametrize("exposure,inv_gamma", [(-1.0, 1 / 2.2), (1.0, 1 / 2.2), (0.5, 1.0)])
+def test_film_to_ldr_matches_host(gpu_ctx, c1_scene, exposure, inv_gamma):
+    """tr_film_to_ldr (resolve, ACES, gamma and 8-bit quantisation fused on
+    the device) against the same steps on the host from the returned film
+    sums.  Bytes agree exactly except where 255 * colour lies within 1e-3 of
+    an integer, where FP32 rounding may move the truncation by one."""
+    p = c1_scene.render_params(width=64, height=48, spp=16, seed=5)
+    sums, _, _ = gpu_ctx.render(p, want_image=False)
+    ldr = gpu_ctx.film_to_ldr(exposure, inv_gamma)
+    w = sums[..., 3:]
+    c = np.where(w > 0, sums[..., :3] / np.where(w > 0, w, 1), 0.0)
+    if exposure >= 0:
+        x = c * exposure
+        c = np.clip(x * (2.51 * x + 0.03) / (x * (2.43 * x + 0.59) + 0.14), 0, 1)
+    c = c ** inv_gamma
+    v = c * 255
+    want = np.clip(np.floor(v), 0, 255).astype(np.int64)
+    diff = np.abs(ldr.astype(np.int64) - want)
+    edge = np.abs(v - np.round(v)) < 1e-3
+    print(f"bytes differing {np.count_nonzero(diff)} of {diff.size}, at edges {np.count_nonzero(diff & edge)}")
+    assert ldr.shape == (48, 64, 3)
+    assert np.all(diff[~edge] == 0) and np.all(diff <= 1)

@@ -38,7 +38,7 @@ HIP_SOURCES = {
     "guide_capi.cpp": ["-x", "hip"],
 }
 CXX_SOURCES = {"bvh_build.cpp": [], "envmap.cpp": ["-ffp-contract=off"]}
-FACADE_SOURCES = ["facade.cpp", "scenes.cpp", "obj_load.cpp", "image_load.cpp"]
+FACADE_SOURCES = ["facade.cpp", "scenes.cpp", "obj_load.cpp", "image_load.cpp", "image_write.cpp"]
 
 
 def _run(cmd: list[str]) -> None:
@@ -77,7 +77,7 @@ def build_facade() -> Path | None:
         return None
     out = LIB / "libtracer.so"
     _run(["g++", *COMMON, "-march=x86-64-v3", "-pthread", "-shared", *map(str, srcs), "-o", str(out),
-          f"-L{LIB}", "-ltracer_hip", "-Wl,-rpath,$ORIGIN"])
+          f"-L{LIB}", "-ltracer_hip", "-lz", "-Wl,-rpath,$ORIGIN"])
     return out
 
 

@@ -710,6 +710,20 @@ tr_status tr_bsdf_eval_batch(tr_ctx* c, const tr_material* mats, int32_t num_mat
     });
 }
 
+tr_status tr_film_to_ldr(tr_ctx* c, float aces_exposure, float inv_gamma, uint8_t* rgb, int64_t cap) {
+    return guarded(c, [&] {
+        if (c->film_w <= 0) throw StateError("tr_film_to_ldr before tr_render");
+        const int64_t npix = int64_t(c->film_w) * c->film_h;
+        if (!rgb || cap < npix * 3) throw InvalidArg("rgb is NULL or smaller than width * height * 3 bytes");
+        c->s_out.reserve(size_t(npix) * 3);
+        launch_film_ldr(c->film.as<double4>(), c->s_out.as<unsigned char>(), int(npix), aces_exposure, inv_gamma,
+                        c->stream);
+        hip_check(hipGetLastError(), "k_film_ldr launch");
+        hip_check(hipMemcpyAsync(rgb, c->s_out.ptr, size_t(npix) * 3, hipMemcpyDeviceToHost, c->stream), "D2H");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+    });
+}
+
 static void env_common(tr_ctx* c, const void* a, const void* b, const void* d, int64_t n) {
     if (!c->have_bvh) throw StateError("environment batch before tr_bvh_build");
     if (!c->hs.has_env) throw StateError("the scene has no environment light");
@@ -803,6 +817,8 @@ tr_status tr_render(tr_ctx* c, const tr_render_params* prm, double* film_sums, i
         if (maxP > (int64_t(1) << 30)) throw InvalidArg("too many paths per pass");
         ensure_paths(c, maxP);
         c->film.reserve(size_t(npix) * sizeof(double4));
+        c->film_w = P.width;
+        c->film_h = P.height;
         hipStream_t st = c->stream;
         if (accumulate && film_sums)
             hip_check(hipMemcpyAsync(c->film.ptr, film_sums, size_t(npix) * 32, hipMemcpyHostToDevice, st), "H2D film");

@@ -640,6 +640,32 @@ __global__ void k_resolve(const double4* film, float* image, int n) {
     image[3 * i] = r; image[3 * i + 1] = g; image[3 * i + 2] = b;
 }
 
+// The film as 8-bit RGB in one pass: resolve (sums / weight), the ACES
+// curve of exposure * colour (src/post_processor/aces.cpp) when exposure >= 0,
+// colour^inv_gamma (src/post_processor/gamma.cpp) when inv_gamma > 0, then
+// clamp(int(255 c), 0, 255) as src/main.cpp:290-299 quantises; one pixel
+// (32 bytes in, 3 out) per thread.
+TR_D float aces_curve(float x) {
+    return fminf(fmaxf((x * (2.51f * x + 0.03f)) / (x * (2.43f * x + 0.59f) + 0.14f), 0.0f), 1.0f);
+}
+TR_D unsigned char to_byte(float c, float exposure, float inv_gamma) {
+    if (exposure >= 0.0f) c = aces_curve(c * exposure);
+    if (inv_gamma > 0.0f) c = powf(c, inv_gamma);
+    const float v = c * 255.0f;
+    if (!(v > 0.0f)) return 0;            // also NaN
+    return v >= 255.0f ? 255 : (unsigned char)int(v);
+}
+__global__ void k_film_ldr(const double4* film, unsigned char* rgb, int n, float exposure, float inv_gamma) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double4 f = film[i];
+    float r = 0.0f, g = 0.0f, b = 0.0f;
+    if (f.w > 0.0) { r = float(f.x / f.w); g = float(f.y / f.w); b = float(f.z / f.w); }
+    rgb[3 * size_t(i)] = to_byte(r, exposure, inv_gamma);
+    rgb[3 * size_t(i) + 1] = to_byte(g, exposure, inv_gamma);
+    rgb[3 * size_t(i) + 2] = to_byte(b, exposure, inv_gamma);
+}
+
 __global__ void k_bsdf_eval(const DevMaterial* mats, const int* ids, const float* wi, const float* wo, const float* ng,
                             int64_t n, float* f, float* pdf) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -699,6 +725,11 @@ void launch_accumulate(const DevPaths& p, const RenderConsts& c, double4* film, 
 }
 void launch_resolve(const double4* film, float* image, int num_pixels, hipStream_t st) {
     hipLaunchKernelGGL(k_resolve, dim3((num_pixels + 255) / 256), dim3(256), 0, st, film, image, num_pixels);
+}
+void launch_film_ldr(const double4* film, unsigned char* rgb, int num_pixels, float exposure, float inv_gamma,
+                     hipStream_t st) {
+    hipLaunchKernelGGL(k_film_ldr, dim3((num_pixels + 255) / 256), dim3(256), 0, st, film, rgb, num_pixels, exposure,
+                       inv_gamma);
 }
 void launch_bsdf_eval(const DevMaterial* mats, const int* ids, const float* wi, const float* wo, const float* ng, int64_t n,
                       float* f, float* pdf, hipStream_t st) {

@@ -135,6 +135,8 @@ void launch_accumulate(const DevPaths& p, const RenderConsts& c, double4* film, 
 void launch_resolve(const double4* film, float* image, int num_pixels, hipStream_t st);
 void launch_bsdf_eval(const DevMaterial* mats, const int* ids, const float* wi, const float* wo, const float* ng, int64_t n,
                       float* f, float* pdf, hipStream_t st);
+void launch_film_ldr(const double4* film, unsigned char* rgb, int num_pixels, float exposure, float inv_gamma,
+                     hipStream_t st);
 void launch_env_sample(const DevScene& s, const float* u, int64_t n, float* wi, float* pdf, float* L, hipStream_t st);
 void launch_env_eval(const DevScene& s, const float* d, int64_t n, float* L, float* pdf, hipStream_t st);
 void launch_bsdf_sample(const DevMaterial* mats, const int* ids, const float* wo, const float* ng, const float* u,

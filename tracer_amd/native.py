@@ -66,6 +66,7 @@ def hip_lib() -> C.CDLL:
         lib.tr_bsdf_sample_batch.argtypes = [VP, P(TrMaterial), I32, P(C.c_int32), P(C.c_float), P(C.c_float),
                                              P(C.c_float), I64, P(C.c_float), P(C.c_float), P(C.c_float)]
         lib.tr_env_sample_batch.argtypes = [VP, P(C.c_float), I64, P(C.c_float), P(C.c_float), P(C.c_float)]
+        lib.tr_film_to_ldr.argtypes = [VP, C.c_float, C.c_float, P(C.c_uint8), I64]
         lib.tr_env_eval_batch.argtypes = [VP, P(C.c_float), I64, P(C.c_float), P(C.c_float)]
         F = P(C.c_float)
         lib.tr_guide_default_config.argtypes = [P(abi.TrGuideConfig)]
@@ -99,6 +100,12 @@ def scenes_lib() -> C.CDLL:
         lib.trs_read_hdr.argtypes = [C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_float),
                                      C.c_int64, C.POINTER(C.c_char_p)]
         lib.trs_read_hdr.restype = C.c_int32
+        lib.trs_write_hdr.argtypes = [C.c_char_p, C.c_int32, C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_char_p)]
+        lib.trs_write_hdr.restype = C.c_int32
+        lib.trs_write_png.argtypes = [C.c_char_p, C.c_int32, C.c_int32, C.POINTER(C.c_uint8), C.POINTER(C.c_char_p)]
+        lib.trs_write_png.restype = C.c_int32
+        lib.trs_postprocess.argtypes = [C.POINTER(C.c_float), C.c_int32, C.c_int32, C.c_float, C.c_float]
+        lib.trs_postprocess.restype = C.c_int32
         lib.trs_desc.argtypes = [C.c_void_p]
         lib.trs_desc.restype = C.POINTER(TrSceneDesc)
         lib.trs_render_defaults.argtypes = [C.c_void_p, C.POINTER(TrRenderParams)]
@@ -239,6 +246,32 @@ def read_hdr(path) -> np.ndarray:
     return out
 
 
+def write_hdr(path, rgb: np.ndarray) -> None:
+    """write_image_to_hdr of a (height, width, 3) float image."""
+    a = np.ascontiguousarray(rgb, np.float32)
+    err = C.c_char_p()
+    if scenes_lib().trs_write_hdr(str(path).encode(), a.shape[1], a.shape[0], fptr(a), C.byref(err)) != 0:
+        raise TracerError(abi.TR_ERR_INVALID, (err.value or b"").decode())
+
+
+def write_png(path, rgb: np.ndarray) -> None:
+    """write_image_to_png of a (height, width, 3) uint8 image."""
+    a = np.ascontiguousarray(rgb, np.uint8)
+    err = C.c_char_p()
+    if scenes_lib().trs_write_png(str(path).encode(), a.shape[1], a.shape[0], a.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                  C.byref(err)) != 0:
+        raise TracerError(abi.TR_ERR_INVALID, (err.value or b"").decode())
+
+
+def postprocess(rgb: np.ndarray, aces_exposure: float = -1.0, gamma: float = 0.0) -> np.ndarray:
+    """create_aces_tone_mapper(aces_exposure) (when >= 0) then
+    create_gamma_corrector(gamma) (when > 0) on a copy of a float image."""
+    a = np.array(rgb, np.float32, order="C", copy=True)
+    if scenes_lib().trs_postprocess(fptr(a), a.shape[1], a.shape[0], aces_exposure, gamma) != 0:
+        raise TracerError(abi.TR_ERR_INVALID, "trs_postprocess: bad arguments")
+    return a
+
+
 def _scene_from_handle(lib, h) -> Scene:
     try:
         d = lib.trs_desc(h).contents
@@ -324,6 +357,7 @@ class Context:
         (H, W, 4) float64 array to receive the sums of a fresh render, reused
         across calls instead of a new array per call."""
         W, H = params.width, params.height
+        self._film_shape = (H, W)
         acc = film is not None
         if acc:
             sums = film
@@ -376,6 +410,16 @@ class Context:
         self._check(self.lib.tr_bsdf_eval_batch(self.h, arr, len(mats), iptr(ids), fptr(wi), fptr(wo), fptr(ngc), n,
                                                 fptr(f), fptr(pdf)))
         return f, pdf
+
+    def film_to_ldr(self, aces_exposure: float = -1.0, inv_gamma: float = 1.0 / 2.2) -> np.ndarray:
+        """The last render's film as (height, width, 3) uint8 (tr_film_to_ldr)."""
+        if not getattr(self, "_film_shape", None):
+            raise TracerError(abi.TR_ERR_STATE, "film_to_ldr before render")
+        h, w = self._film_shape
+        out = np.zeros((h, w, 3), np.uint8)
+        self._check(self.lib.tr_film_to_ldr(self.h, aces_exposure, inv_gamma, out.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                            out.size))
+        return out
 
     def env_sample(self, u: np.ndarray):
         """The loaded scene's environment light: u (n, 2) -> (wi, pdf, L)."""
