@@ -39,6 +39,10 @@ trs_scene* trs_load_obj(const char* obj_path, const tr_camera* camera, int32_t o
  * *width, *height, and when rgb is not NULL its height rows of width linear
  * RGB texels (cap floats at least).  Returns 0, or 1 and *err on failure. */
 int32_t trs_read_hdr(const char* path, int32_t* width, int32_t* height, float* rgb, int64_t cap, const char** err);
+/* Reads an 8-bit PNG (the loader behind create_texture2d_from_file): rows
+ * bottom to top, as the reference flips 8-bit images on load; grey and
+ * palette images expand to RGB, alpha is dropped. */
+int32_t trs_read_png(const char* path, int32_t* width, int32_t* height, uint8_t* rgb, int64_t cap, const char** err);
 /* write_image_to_hdr / write_image_to_png of height rows of width RGB texels
  * (floats / bytes).  Return 0, or 1 and *err on failure. */
 int32_t trs_write_hdr(const char* path, int32_t width, int32_t height, const float* rgb, const char** err);

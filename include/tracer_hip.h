@@ -61,7 +61,20 @@ typedef struct tr_material {
     float specular_scale[3];
     float specular_tint, anisotropic, sheen, sheen_tint, clearcoat, clearcoat_gloss;
     float shininess;   /* Phong exponent ns (TR_MAT_PHONG) */
+    /* 0: none; k > 0: textures[k - 1] of the scene gives base_color per hit
+     * (kd of TR_MAT_PHONG, base_color of TR_MAT_DISNEY) at the hit's
+     * interpolated uv (Texture2D::evaluate, src/core/texture.hpp:57-83). */
+    int32_t base_color_texture;
 } tr_material;
+
+/* An image texture (create_texture2d_from_file, an MTL map): height rows of
+ * width RGB texels, 8-bit (rgb8, read as value / 255) or float (rgbf); the
+ * other pointer is NULL.  Sampled bilinearly with wrap_repeat. */
+typedef struct tr_texture {
+    int32_t width, height;
+    const uint8_t* rgb8;
+    const float* rgbf;
+} tr_texture;
 
 /* create_homogeneous_medium(sigma_a, sigma_s, g, max_scattering_count),
  * src/factory/medium.hpp:21-26, src/medium/homogeneous.cpp:12-18. */
@@ -127,6 +140,9 @@ typedef struct tr_scene_desc {
     int32_t env_width, env_height;
     const float* env_pixels;
     float env_world_to_light[9];
+    /* textures that materials name by base_color_texture (uvs required) */
+    int32_t num_textures;
+    const tr_texture* textures;
 } tr_scene_desc;
 
 /* BVH options.  max_leaf_primitives mirrors create_bvh_accel(int)

@@ -68,8 +68,20 @@ def lib() -> C.CDLL:
         L.ref_env_eval.restype = None
         L.ref_env_radiance_integral.argtypes = [VP, P(C.c_float)]
         L.ref_env_radiance_integral.restype = None
+        L.ref_tex_eval8.argtypes = [C.c_int, C.c_int, P(C.c_uint8), P(C.c_float), I64, I32, P(C.c_float)]
+        L.ref_tex_eval8.restype = None
         _lib = L
     return _lib
+
+
+def tex_eval8(rgb8: np.ndarray, uv: np.ndarray, quirks: bool) -> np.ndarray:
+    """Lookups of an 8-bit texture (h, w, 3) at uv (n, 2)."""
+    t = np.ascontiguousarray(rgb8, np.uint8)
+    uv = np.ascontiguousarray(uv, np.float32)
+    out = np.zeros((uv.shape[0], 3), np.float32)
+    lib().ref_tex_eval8(t.shape[1], t.shape[0], t.ctypes.data_as(C.POINTER(C.c_uint8)), fptr(uv), uv.shape[0],
+                        1 if quirks else 0, fptr(out))
+    return out
 
 
 class RefEnv:

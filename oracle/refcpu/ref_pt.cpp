@@ -23,7 +23,7 @@
 //     surface a segment reaches added before the segment's medium sample),
 //     Q27 (specular flag kept across a medium scatter), Q26 (no direct light at a
 //     vertex whose continuation sample fails), the Disney Q2/Q3/Q21 and the
-//     Phong/Lambert Q22/Q24.
+//     Phong/Lambert Q22/Q24 and the 8-bit texture interpolation Q29.
 //   quirks = 0 (Fixed): the same loop with those defects corrected.
 //   nee = 0: every light sampled at every vertex (reference); nee = 1: one
 //     light picked by power (compute_light_power_distribution,
@@ -58,14 +58,51 @@ struct Isect {
     Coord<float> geo, shd;
     int prim = -1;
     float t = 0;
+    float u = 0, v = 0;   // interpolated uv (alpha weights B, beta C)
     V3f eps_offset(const V3f& dir) const {   // intersection.hpp:19-23
         if (dot(geo.z, dir) > 0) return pos + geo.z * kEps;
         return pos - geo.z * kEps;
     }
 };
 
+// An image texture of the scene: 8-bit or float RGB texels.
+struct RefTex {
+    int w = 0, h = 0;
+    std::vector<uint8_t> rgb8;
+    std::vector<float> rgbf;
+    // Texture2D::evaluate (texture.hpp:57-83, wrap_repeat) with
+    // LinearSampler::Sample2D (sampler.hpp:12-25).
+    //   Q29 on 8-bit texels the reference interpolates Point3<uint8_t>
+    //       values: every weighted term and sum is truncated to an integer
+    //       before the final / 255 (image_texture2d.cpp:24-28).  Fixed mode
+    //       interpolates texel / 255 in floating point.
+    RGB<float> eval(double u, double v, bool quirks) const {
+        u = std::clamp(u - std::floor(u), 0.0, 1.0) * (w - 1);
+        v = std::clamp(v - std::floor(v), 0.0, 1.0) * (h - 1);
+        const int u0 = std::clamp(int(u), 0, w - 1), u1 = std::clamp(u0 + 1, 0, w - 1);
+        const int v0 = std::clamp(int(v), 0, h - 1), v1 = std::clamp(v0 + 1, 0, h - 1);
+        const double du = u - u0, dv = v - v0;
+        RGB<float> out;
+        for (int k = 0; k < 3; ++k) {
+            auto at = [&](int x, int y) { return (size_t(y) * size_t(w) + size_t(x)) * 3 + size_t(k); };
+            if (!rgb8.empty() && quirks) {
+                auto q = [](double x) { return int(uint8_t(x)); };
+                const int top = q(q(rgb8[at(u0, v0)] * (1.0 - du)) + q(rgb8[at(u1, v0)] * du));
+                const int bot = q(q(rgb8[at(u0, v1)] * (1.0 - du)) + q(rgb8[at(u1, v1)] * du));
+                out[k] = float(q(q(top * (1.0 - dv)) + q(bot * dv))) / 255.0f;
+            } else {
+                auto tx = [&](int x, int y) { return rgb8.empty() ? double(rgbf[at(x, y)]) : double(rgb8[at(x, y)]) / 255.0; };
+                out[k] = float((tx(u0, v0) * (1.0 - du) + tx(u1, v0) * du) * (1.0 - dv) +
+                               (tx(u0, v1) * (1.0 - du) + tx(u1, v1) * du) * dv);
+            }
+        }
+        return out;
+    }
+};
+
 struct Scene {
     std::vector<V3f> P, N;
+    std::vector<RefTex> tex;
     std::vector<float> UV;
     std::vector<int> idx, mat, med_in, med_out;
     std::vector<float> emis;
@@ -117,6 +154,8 @@ static void fill_isect(const Scene& s, int prim, const V3f& o, const V3f& d, flo
     is.pos = o + d * t;
     is.prim = prim;
     is.t = t;
+    is.u = ua + (ub - ua) * alpha + (uc - ua) * beta;
+    is.v = va + (vb - va) * alpha + (vc - va) * beta;
     V3f dpdu, dpdv;
     compute_ss_ts(AB, AC, ub - ua, vb - va, uc - ua, vc - va, gn, dpdu, dpdv);
     dpdu = cross(dpdv, gn);
@@ -284,7 +323,15 @@ static DisneyParams to_params(const tr_material& m) {
 template <class T> static RGB<T> phong_kd(const tr_material& m) { return RGB<T>(m.base_color[0], m.base_color[1], m.base_color[2]); }
 template <class T> static RGB<T> phong_ks(const tr_material& m) { return RGB<T>(m.specular_scale[0], m.specular_scale[1], m.specular_scale[2]); }
 static void make_bsdf(const Scene& s, const Isect& is, bool quirks, SurfBSDF& b) {
-    const tr_material& m = s.mats[s.mat[is.prim]];
+    tr_material m = s.mats[s.mat[is.prim]];
+    if (m.base_color_texture > 0) {   // kd / base_color from the texture at the hit's uv
+        const RGB<float> c = s.tex[size_t(m.base_color_texture) - 1].eval(is.u, is.v, quirks);
+        m.base_color[0] = c.r; m.base_color[1] = c.g; m.base_color[2] = c.b;
+        if (m.type == TR_MAT_LAMBERT) {   // Phong(ks = 0): kd / max(1, max_i kd_i)
+            const float dem = std::max(1.0f, std::max(c.r, std::max(c.g, c.b)));
+            for (int k = 0; k < 3; ++k) m.base_color[k] /= dem;
+        }
+    }
     b.type = m.type;
     if (m.type == TR_MAT_DISNEY) b.dis.reset(new DisneyRef<float>(is.geo, is.shd, to_params(m), !quirks));
     else if (m.type == TR_MAT_PHONG) b.pho.reset(new PhongRef<float>(is.geo, is.shd, phong_kd<float>(m), phong_ks<float>(m), m.shininess, !quirks));
@@ -692,6 +739,16 @@ static Scene* build_scene(const tr_scene_desc* d, int max_leaf) {
         for (int64_t i = 0; i < nv; ++i) s->N[i] = V3f(d->normals[3 * i], d->normals[3 * i + 1], d->normals[3 * i + 2]);
     }
     if (d->uvs) s->UV.assign(d->uvs, d->uvs + 2 * nv);
+    for (int i = 0; i < d->num_textures; ++i) {
+        const tr_texture& t = d->textures[i];
+        RefTex rt;
+        rt.w = t.width;
+        rt.h = t.height;
+        const size_t n = size_t(t.width) * size_t(t.height) * 3;
+        if (t.rgb8) rt.rgb8.assign(t.rgb8, t.rgb8 + n);
+        else rt.rgbf.assign(t.rgbf, t.rgbf + n);
+        s->tex.push_back(std::move(rt));
+    }
     s->idx.assign(d->indices, d->indices + 3 * nt);
     s->mat.assign(d->material_ids, d->material_ids + nt);
     if (d->emission) s->emis.assign(d->emission, d->emission + 3 * nt);
@@ -986,6 +1043,18 @@ void ref_bsdf_sample(const tr_material* mats, const int32_t* ids, const float* w
     }
 }
 // HG phase: value for cos between wi and wo, and sampling.
+// One 8-bit texture on its own: lookups at (u, v) pairs (RGB out).
+void ref_tex_eval8(int w, int h, const uint8_t* rgb8, const float* uv, int64_t n, int quirks, float* out) {
+    RefTex t;
+    t.w = w;
+    t.h = h;
+    t.rgb8.assign(rgb8, rgb8 + size_t(w) * size_t(h) * 3);
+    for (int64_t i = 0; i < n; ++i) {
+        const RGB<float> c = t.eval(uv[2 * i], uv[2 * i + 1], quirks != 0);
+        out[3 * i] = c.r; out[3 * i + 1] = c.g; out[3 * i + 2] = c.b;
+    }
+}
+
 // Image environment on its own: IBL sample_li / light_emit / pdf.
 void* ref_env_create(int w, int h, const float* texels, const float* world_to_light) {
     return new RefEnvMap(w, h, texels, world_to_light);

@@ -30,11 +30,14 @@ def camera(pos, target, up=(0.0, 1.0, 0.0), fov_deg=40.0, aspect=1.0) -> abi.TrC
 
 
 def box_room_obj(light_ke=(12.0, 12.0, 12.0), block_ks=(0.35, 0.35, 0.35), block_ns=25.0,
-                 glass_block: bool = False) -> tuple[str, str]:
+                 glass_block: bool = False, texture: str | None = None) -> tuple[str, str]:
     """A closed-on-five-sides room [-1, 1]^3 open towards +z, a ceiling quad
     light facing down, a tall block with a Phong specular lobe (or of glass,
     Ni 1.5) and a short diffuse block; faces are quads (split by the reader),
-    normals given per face with v//vn corners."""
+    normals given per face with v//vn corners.  With `texture` (an image file
+    name next to the MTL) the floor is a diffuse and the back wall a glossy
+    material whose map_Kd is that image, with v/vt/vn corners whose texture
+    coordinates run past 1 (the lookup wraps)."""
     mtl = f"""# materials
 newmtl white
 Kd 0.73 0.73 0.73
@@ -53,10 +56,20 @@ newmtl light
 Kd 0.78 0.78 0.78
 Ke {light_ke[0]} {light_ke[1]} {light_ke[2]}
 """
+    if texture:
+        mtl += f"""newmtl tex_floor
+Kd 0.9 0.9 0.9
+map_Kd {texture}
+newmtl tex_wall
+Kd 0.8 0.7 0.6
+Ks 0.3 0.3 0.3
+Ns 15
+map_Kd {texture}
+"""
     lines = ["mtllib scene.mtl"]
-    v, n = [], []
+    v, n, vt = [], [], []
 
-    def quad(c0, c1, c2, c3, normal, mat, group=None):
+    def quad(c0, c1, c2, c3, normal, mat, group=None, uv=None):
         if group:
             lines.append(f"o {group}")
         lines.append(f"usemtl {mat}")
@@ -64,12 +77,21 @@ Ke {light_ke[0]} {light_ke[1]} {light_ke[2]}
         v.extend([c0, c1, c2, c3])
         n.append(normal)
         ni = len(n)
-        lines.append("f " + " ".join(f"{base + k + 1}//{ni}" for k in range(4)))
+        if uv is None:
+            lines.append("f " + " ".join(f"{base + k + 1}//{ni}" for k in range(4)))
+        else:
+            tb = len(vt)
+            vt.extend(uv)
+            lines.append("f " + " ".join(f"{base + k + 1}/{tb + k + 1}/{ni}" for k in range(4)))
 
     # room: floor, ceiling, back wall, left (red), right (green); normals point inwards
-    quad((-1, -1, 1), (1, -1, 1), (1, -1, -1), (-1, -1, -1), (0, 1, 0), "white", "room")
+    floor_uv = ((0.0, 0.0), (2.5, 0.0), (2.5, 2.5), (0.0, 2.5)) if texture else None
+    wall_uv = ((-0.25, 0.0), (1.5, 0.0), (1.5, 1.0), (-0.25, 1.0)) if texture else None
+    quad((-1, -1, 1), (1, -1, 1), (1, -1, -1), (-1, -1, -1), (0, 1, 0), "tex_floor" if texture else "white", "room",
+         floor_uv)
     quad((-1, 1, -1), (1, 1, -1), (1, 1, 1), (-1, 1, 1), (0, -1, 0), "white")
-    quad((-1, -1, -1), (1, -1, -1), (1, 1, -1), (-1, 1, -1), (0, 0, 1), "white")
+    quad((-1, -1, -1), (1, -1, -1), (1, 1, -1), (-1, 1, -1), (0, 0, 1), "tex_wall" if texture else "white", None,
+         wall_uv)
     quad((-1, -1, 1), (-1, -1, -1), (-1, 1, -1), (-1, 1, 1), (1, 0, 0), "red")
     quad((1, -1, -1), (1, -1, 1), (1, 1, 1), (1, 1, -1), (-1, 0, 0), "green")
     # light: a quad just below the ceiling facing down
@@ -91,5 +113,6 @@ Ke {light_ke[0]} {light_ke[1]} {light_ke[2]}
 
     block(-0.65, -0.05, -0.7, -0.1, 0.25, "glass" if glass_block else "glossy", "tall")
     block(0.1, 0.6, 0.05, 0.55, -0.4, "white", "short")
-    text = "\n".join([f"v {a} {b} {c}" for a, b, c in v] + [f"vn {a} {b} {c}" for a, b, c in n] + lines) + "\n"
+    text = "\n".join([f"v {a} {b} {c}" for a, b, c in v] + [f"vn {a} {b} {c}" for a, b, c in n] +
+                     [f"vt {a} {b}" for a, b in vt] + lines) + "\n"
     return text, mtl

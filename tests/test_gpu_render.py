@@ -231,3 +231,44 @@ def test_film_to_ldr_matches_host(gpu_ctx, c1_scene, exposure, inv_gamma):
     print(f"bytes differing {np.count_nonzero(diff)} of {diff.size}, at edges {np.count_nonzero(diff & edge)}")
     assert ldr.shape == (48, 64, 3)
     assert np.all(diff[~edge] == 0) and np.all(diff <= 1)
+
+
+def test_obj_room_textured_matches_fixed_estimator(gpu_ctx, tmp_path):
+    """The OBJ room with an 8-bit PNG map_Kd on a diffuse floor and a Phong
+    back wall (texture coordinates past 1, wrapped), against the oracle in
+    fixed mode (Q29 off: texels interpolated in floating point)."""
+    from tests import objfiles
+    from tracer_amd import native
+    rng = np.random.default_rng(13)
+    checker = np.kron((np.indices((4, 4)).sum(0) % 2)[..., None] * 200 + 30, np.ones((4, 4, 3)))
+    checker = (checker * rng.uniform(0.6, 1.0, (16, 16, 3))).astype(np.uint8)
+    native.write_png(tmp_path / "checker.png", checker)
+    obj, mtl = objfiles.box_room_obj(texture="checker.png")
+    s = native.obj_scene(objfiles.write(tmp_path, "room.obj", obj, mtl),
+                         objfiles.camera((0.0, 0.0, 3.6), (0.0, 0.0, 0.0)), outside_fog=False)
+    assert len(s.textures) == 2
+    _compare_estimators(gpu_ctx, s, 32, 32, 1024, 256, 8, 8, quirks=False, nee=0, max_depth=8)
+
+
+def test_c2_disney_float_texture_matches_fixed_estimator(gpu_ctx):
+    """Configuration 2 with the Disney base colour read from a float texture
+    through spherical texture coordinates of the mesh vertices, against the
+    oracle in fixed mode."""
+    from tracer_amd import abi, native
+    s = native.config_scene(2, detail=1 / 64)
+    s.camera.pos[:] = (0.0, 2.0, 14.0)
+    p = s.positions.reshape(-1, 3).astype(np.float64)
+    d = p - p.mean(0)
+    r = np.linalg.norm(d, axis=1) + 1e-12
+    u = 0.5 + np.arctan2(d[:, 2], d[:, 0]) / (2 * np.pi)
+    v = np.arccos(np.clip(d[:, 1] / r, -1, 1)) / np.pi
+    s.uvs = np.stack([3 * u, 2 * v], 1).astype(np.float32)
+    rng = np.random.default_rng(14)
+    s.textures = [rng.uniform(0.05, 0.95, (8, 12, 3)).astype(np.float32)]
+    n = 0
+    for m in s.materials:
+        if m.type == abi.TR_MAT_DISNEY:
+            m.base_color_texture = 1
+            n += 1
+    assert n > 0
+    _compare_estimators(gpu_ctx, s, 32, 32, 512, 64, 8, 8, quirks=False, nee=0, max_depth=6)

@@ -71,3 +71,106 @@ def test_aces_and_gamma_closed_forms():
     assert np.allclose(g, x.astype(np.float64) ** (1 / 2.2), rtol=1e-6)
     # gamma <= 0 and exposure < 0 leave the image unchanged
     assert np.array_equal(native.postprocess(x), x)
+
+
+def _png_bytes(img: np.ndarray, filters, ctype=2, palette=None) -> bytes:
+    """A PNG built here: rows filtered with the given filter types (0..4)."""
+    h, w = img.shape[:2]
+    ch = img.shape[2] if img.ndim == 3 else 1
+    rows = img.reshape(h, w * ch).astype(np.int64)
+    out = bytearray()
+    prev = np.zeros(w * ch, np.int64)
+    for y in range(h):
+        cur = rows[y]
+        a = np.concatenate([np.zeros(ch, np.int64), cur[:-ch]])
+        c = np.concatenate([np.zeros(ch, np.int64), prev[:-ch]])
+        f = filters[y % len(filters)]
+        if f == 0: pred = np.zeros_like(cur)
+        elif f == 1: pred = a
+        elif f == 2: pred = prev
+        elif f == 3: pred = (a + prev) // 2
+        else:
+            p = a + prev - c
+            pa, pb, pc = np.abs(p - a), np.abs(p - prev), np.abs(p - c)
+            pred = np.where((pa <= pb) & (pa <= pc), a, np.where(pb <= pc, prev, c))
+        out.append(f)
+        out.extend(((cur - pred) & 255).astype(np.uint8).tobytes())
+        prev = cur
+
+    def chunk(kind, body):
+        return struct.pack(">I", len(body)) + kind + body + struct.pack(">I", zlib.crc32(kind + body) & 0xFFFFFFFF)
+    data = bytes([137, 80, 78, 71, 13, 10, 26, 10]) + chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, 8, ctype, 0, 0, 0))
+    if palette is not None:
+        data += chunk(b"PLTE", palette.astype(np.uint8).tobytes())
+    return data + chunk(b"IDAT", zlib.compress(bytes(out))) + chunk(b"IEND", b"")
+
+
+@pytest.mark.parametrize("filters", [[0], [1], [2], [3], [4], [0, 1, 2, 3, 4]])
+def test_png_reader_filters(tmp_path, filters):
+    """Each filter type decodes to the original bytes; rows come back bottom
+    to top (the reference flips 8-bit images on load)."""
+    rng = np.random.default_rng(len(filters) + filters[0])
+    img = rng.integers(0, 256, (6, 9, 3), dtype=np.uint8)
+    p = tmp_path / "f.png"
+    p.write_bytes(_png_bytes(img, filters))
+    assert np.array_equal(native.read_png(p), img[::-1])
+
+
+def test_png_reader_grey_palette_alpha(tmp_path):
+    rng = np.random.default_rng(9)
+    grey = rng.integers(0, 256, (4, 5), dtype=np.uint8)
+    p = tmp_path / "g.png"
+    p.write_bytes(_png_bytes(grey, [4], ctype=0))
+    assert np.array_equal(native.read_png(p), np.repeat(grey[::-1, :, None], 3, axis=2))
+    pal = rng.integers(0, 256, (16, 3))
+    idx = rng.integers(0, 16, (4, 5), dtype=np.uint8)
+    p.write_bytes(_png_bytes(idx, [0], ctype=3, palette=pal))
+    assert np.array_equal(native.read_png(p), pal[idx][::-1].astype(np.uint8))
+    rgba = rng.integers(0, 256, (3, 4, 4), dtype=np.uint8)
+    p.write_bytes(_png_bytes(rgba, [1], ctype=6))
+    assert np.array_equal(native.read_png(p), rgba[::-1, :, :3])
+    ga = rng.integers(0, 256, (3, 4, 2), dtype=np.uint8)
+    p.write_bytes(_png_bytes(ga, [0], ctype=4))
+    with pytest.raises(native.TracerError):
+        native.read_png(p)
+
+
+def test_png_writer_reader_round_trip(tmp_path):
+    rng = np.random.default_rng(10)
+    img = rng.integers(0, 256, (5, 7, 3), dtype=np.uint8)
+    p = tmp_path / "w.png"
+    native.write_png(p, img)
+    assert np.array_equal(native.read_png(p), img[::-1])
+
+
+def test_texture_interpolation_q29():
+    """Fixed mode interpolates texel / 255 in floating point; the reference
+    (Q29) truncates every weighted 8-bit term: between texels 0 and 255 at the
+    midpoint it gives 127 / 255 where the exact value is 1/2."""
+    from oracle import refcpu
+    tex = np.zeros((1, 2, 3), np.uint8)
+    tex[0, 1] = 255
+    uv = np.float32([[0.5, 0.0], [0.25, 0.0], [1.0 / 3.0, 0.0]])
+    fixed = refcpu.tex_eval8(tex, uv, quirks=False)
+    ref = refcpu.tex_eval8(tex, uv, quirks=True)
+    assert np.allclose(fixed[:, 0], [0.5, 0.25, 1 / 3], atol=1e-6)
+    assert np.array_equal(ref[:, 0] * 255, np.float32([127, 63, 85]))
+
+
+def test_obj_texture_is_decoded_and_scaled(tmp_path):
+    """An MTL map_Kd is read next to the MTL file, decoded to linear as
+    uint8(255 (c / 255)^2.2) and scaled by Kd as uint8(c * Kd)
+    (image_file.cpp:99-130)."""
+    from tests import objfiles
+    rng = np.random.default_rng(11)
+    img = rng.integers(0, 256, (4, 6, 3), dtype=np.uint8)
+    native.write_png(tmp_path / "t.png", img)
+    obj = "mtllib scene.mtl\nv 0 0 0\nv 1 0 0\nv 0 1 0\nvt 0 0\nvt 1 0\nvt 0 1\nusemtl t\nf 1/1 2/2 3/3\n"
+    s = native.obj_scene(objfiles.write(tmp_path, "m.obj", obj, "newmtl t\nKd 0.8 0.5 1.0\nmap_Kd t.png\n"),
+                         objfiles.camera((0, 0, 3), (0, 0, 0)), outside_fog=False)
+    assert len(s.textures) == 1 and s.materials[0].base_color_texture == 1
+    lin = (255 * (img[::-1].astype(np.float32) / 255) ** np.float32(2.2)).astype(np.uint8)
+    want = (lin.astype(np.float32) * np.float32([0.8, 0.5, 1.0])).astype(np.uint8)
+    got = s.textures[0]
+    assert np.max(np.abs(got.astype(int) - want.astype(int))) <= 1   # float pow rounding at integer edges
+    assert np.mean(got == want) > 0.99

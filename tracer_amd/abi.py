@@ -27,7 +27,13 @@ class TrMaterial(C.Structure):
         ("specular_tint", C.c_float), ("anisotropic", C.c_float), ("sheen", C.c_float),
         ("sheen_tint", C.c_float), ("clearcoat", C.c_float), ("clearcoat_gloss", C.c_float),
         ("shininess", C.c_float),
+        ("base_color_texture", C.c_int32),
     ]
+
+
+class TrTexture(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("rgb8", C.POINTER(C.c_uint8)),
+                ("rgbf", C.POINTER(C.c_float))]
 
 
 class TrMedium(C.Structure):
@@ -64,6 +70,8 @@ class TrSceneDesc(C.Structure):
         ("env_width", C.c_int32), ("env_height", C.c_int32),
         ("env_pixels", C.POINTER(C.c_float)),
         ("env_world_to_light", C.c_float * 9),
+        ("num_textures", C.c_int32),
+        ("textures", C.POINTER(TrTexture)),
     ]
 
 
@@ -134,7 +142,7 @@ def dptr(a: np.ndarray | None):
 
 MATERIAL_FIELDS = ["type", "base_color", "metallic", "roughness", "transmission", "transmission_roughness", "ior",
                    "specular_scale", "specular_tint", "anisotropic", "sheen", "sheen_tint", "clearcoat",
-                   "clearcoat_gloss", "shininess"]
+                   "clearcoat_gloss", "shininess", "base_color_texture"]
 
 
 def make_material(**kw) -> TrMaterial:
@@ -150,6 +158,7 @@ def make_material(**kw) -> TrMaterial:
                     shininess=1.0)
     for k, v in defaults.items():
         setattr(m, k, float(kw.get(k, v)))
+    m.base_color_texture = int(kw.get("base_color_texture", 0))
     return m
 
 

@@ -19,7 +19,7 @@ using namespace trhip;
 using namespace trcapi;
 
 // The ctypes bindings (tracer_amd/abi.py) assume these layouts.
-static_assert(sizeof(tr_material) == 76, "tr_material layout");
+static_assert(sizeof(tr_material) == 80, "tr_material layout");
 static_assert(sizeof(tr_medium) == 32, "tr_medium layout");
 static_assert(sizeof(tr_camera) == 52, "tr_camera layout");
 static_assert(sizeof(tr_render_params) == 88, "tr_render_params layout");
@@ -29,6 +29,8 @@ static_assert(sizeof(tr_guide_config) == 48, "tr_guide_config layout");
 namespace {
 
 // DisneyBSDF constructor (src/material/disney.cpp:347-399) evaluated once per material.
+DevMaterial make_material_untextured(const tr_material& m);
+
 void check_material(const tr_material& m) {
     if (m.type != TR_MAT_LAMBERT && m.type != TR_MAT_DISNEY && m.type != TR_MAT_PHONG && m.type != TR_MAT_GLASS)
         throw InvalidArg("unknown material type");
@@ -80,6 +82,14 @@ static DevMaterial make_glass(const tr_material& m) {
 }
 
 DevMaterial make_material(const tr_material& m) {
+    DevMaterial d = make_material_untextured(m);
+    d.tex = m.base_color_texture;
+    if (d.tex > 0 && d.type == MAT_PHONG)   // kept raw: normalised with kd per hit
+        d.spec_scale = mk(m.specular_scale[0], m.specular_scale[1], m.specular_scale[2]);
+    return d;
+}
+
+DevMaterial make_material_untextured(const tr_material& m) {
     if (m.type == TR_MAT_PHONG) return make_phong(m);
     if (m.type == TR_MAT_GLASS) return make_glass(m);
     DevMaterial d{};
@@ -197,6 +207,24 @@ void validate_scene(const tr_scene_desc* d) {
         }
     }
     for (int i = 0; i < d->num_materials; ++i) check_material(d->materials[i]);
+    if (d->num_textures < 0 || (d->num_textures > 0 && !d->textures)) throw InvalidArg("texture array missing");
+    for (int i = 0; i < d->num_textures; ++i) {
+        const tr_texture& t = d->textures[i];
+        if (t.width <= 0 || t.height <= 0 || int64_t(t.width) * t.height > (int64_t(1) << 28))
+            throw InvalidArg("texture " + std::to_string(i) + ": bad size");
+        if ((t.rgb8 == nullptr) == (t.rgbf == nullptr))
+            throw InvalidArg("texture " + std::to_string(i) + ": give exactly one of rgb8 and rgbf");
+        if (t.rgbf)
+            for (size_t k = 0; k < size_t(t.width) * size_t(t.height) * 3; ++k)
+                if (!(t.rgbf[k] >= 0.0f && t.rgbf[k] < 3.0e38f))
+                    throw InvalidArg("texture " + std::to_string(i) + ": texels must be finite and >= 0");
+    }
+    for (int i = 0; i < d->num_materials; ++i) {
+        const int t = d->materials[i].base_color_texture;
+        if (t < 0 || t > d->num_textures) throw InvalidArg("material " + std::to_string(i) + ": texture id out of range");
+        if (t > 0 && !d->uvs) throw InvalidArg("a textured material needs vertex uvs");
+        if (t > 0 && d->materials[i].type == TR_MAT_GLASS) throw InvalidArg("glass takes no texture");
+    }
     for (int i = 0; i < d->num_media; ++i) {
         const tr_medium& m = d->media[i];
         if (!(m.g > -1.0f && m.g < 1.0f)) throw InvalidArg("medium g must lie in (-1, 1)");  // homogeneous.cpp:16
@@ -332,6 +360,42 @@ void build_device_scene(tr_ctx* c, const tr_bvh_opts* opts) {
     upload(c->light_cdf, cdf.data(), cdf.size() * sizeof(float), st);
     upload(c->light_pmf, pmf.data(), pmf.size() * sizeof(float), st);
     upload(c->light_guide, guide.data(), guide.size() * sizeof(int), st);
+    // material textures: per-triangle uvs in leaf order, texel pools, a table
+    bool any_tex = false;
+    for (const tr_material& m : h.mats) any_tex = any_tex || m.base_color_texture > 0;
+    if (any_tex) {
+        std::vector<float4> uv(size_t(nt) * 2);
+        for (int64_t i = 0; i < nt; ++i) {
+            const int prim = r.tris[size_t(i)].prim;
+            float q[6];
+            for (int k = 0; k < 3; ++k) {
+                const int v = h.idx[3 * size_t(prim) + k];
+                q[2 * k] = h.UV[2 * size_t(v)];
+                q[2 * k + 1] = h.UV[2 * size_t(v) + 1];
+            }
+            uv[2 * size_t(i)] = make_float4(q[0], q[1], q[2], q[3]);
+            uv[2 * size_t(i) + 1] = make_float4(q[4], q[5], 0.0f, 0.0f);
+        }
+        std::vector<int4> info;
+        std::vector<uchar4> p8;
+        std::vector<float4> pf;
+        for (const HostTexture& t : h.tex) {
+            const size_t n = size_t(t.w) * size_t(t.h);
+            const bool is8 = !t.rgb8.empty();
+            if ((is8 ? p8.size() : pf.size()) + n > size_t(INT32_MAX)) throw InvalidArg("textures too large");
+            info.push_back(make_int4(t.w, t.h, is8 ? 1 : 0, int(is8 ? p8.size() : pf.size())));
+            for (size_t k = 0; k < n; ++k) {
+                if (is8) p8.push_back(make_uchar4(t.rgb8[3 * k], t.rgb8[3 * k + 1], t.rgb8[3 * k + 2], 0));
+                else pf.push_back(make_float4(t.rgbf[3 * k], t.rgbf[3 * k + 1], t.rgbf[3 * k + 2], 0.0f));
+            }
+        }
+        if (p8.empty()) p8.push_back(make_uchar4(0, 0, 0, 0));
+        if (pf.empty()) pf.push_back(make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+        upload(c->tri_uv, uv.data(), uv.size() * sizeof(float4), st);
+        upload(c->tex_info, info.data(), info.size() * sizeof(int4), st);
+        upload(c->tex8, p8.data(), p8.size() * sizeof(uchar4), st);
+        upload(c->texf, pf.data(), pf.size() * sizeof(float4), st);
+    }
     const EnvMapHost& em = h.envmap;
     DevEnvMap env{};
     if (h.has_env && em.w > 0) {
@@ -383,6 +447,11 @@ void build_device_scene(tr_ctx* c, const tr_bvh_opts* opts) {
     s.has_sun = h.has_sun;
     s.env_L = mk(h.env[0], h.env[1], h.env[2]);
     s.env = env;
+    s.tri_uv = any_tex ? c->tri_uv.as<float4>() : nullptr;
+    s.tex_info = any_tex ? c->tex_info.as<int4>() : nullptr;
+    s.tex8 = any_tex ? c->tex8.as<uchar4>() : nullptr;
+    s.texf = any_tex ? c->texf.as<float4>() : nullptr;
+    s.has_tex = any_tex ? 1 : 0;
     const f3 sd = mk(h.sun_dir[0], h.sun_dir[1], h.sun_dir[2]);
     s.sun_dir = length(sd) > 0.0f ? normalize(sd) : mk(0.0f, 0.0f, 1.0f);
     s.sun_E = mk(h.sun_E[0], h.sun_E[1], h.sun_E[2]);
@@ -543,6 +612,18 @@ tr_status tr_scene_upload(tr_ctx* c, const tr_scene_desc* d) {
         const size_t nv = size_t(d->num_vertices), nt = size_t(d->num_triangles);
         h.P.assign(d->positions, d->positions + 3 * nv);
         if (d->normals) h.N.assign(d->normals, d->normals + 3 * nv); else h.N.clear();
+        if (d->uvs) h.UV.assign(d->uvs, d->uvs + 2 * nv); else h.UV.clear();
+        h.tex.clear();
+        for (int i = 0; i < d->num_textures; ++i) {
+            const tr_texture& t = d->textures[i];
+            HostTexture ht;
+            ht.w = t.width;
+            ht.h = t.height;
+            const size_t n = size_t(t.width) * size_t(t.height) * 3;
+            if (t.rgb8) ht.rgb8.assign(t.rgb8, t.rgb8 + n);
+            else ht.rgbf.assign(t.rgbf, t.rgbf + n);
+            h.tex.push_back(std::move(ht));
+        }
         h.idx.assign(d->indices, d->indices + 3 * nt);
         h.mat.assign(d->material_ids, d->material_ids + nt);
         if (d->emission) h.emission.assign(d->emission, d->emission + 3 * nt); else h.emission.clear();

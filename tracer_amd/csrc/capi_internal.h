@@ -57,8 +57,15 @@ struct DevBuf {
     ~DevBuf() { release(); }
 };
 
+struct HostTexture {
+    int w = 0, h = 0;
+    std::vector<uint8_t> rgb8;   // 8-bit texels, or
+    std::vector<float> rgbf;     // float texels
+};
+
 struct HostScene {
-    std::vector<float> P, N;
+    std::vector<float> P, N, UV;
+    std::vector<HostTexture> tex;
     std::vector<int32_t> idx, mat, med_in, med_out;
     std::vector<float> emission;
     std::vector<tr_material> mats;
@@ -118,6 +125,7 @@ struct tr_ctx {
     // scene on device
     trcapi::DevBuf nodes, tris, tri_shade, mats, media, lights, light_cdf, light_pmf, light_guide;
     trcapi::DevBuf env_tex, env_cond_cdf, env_cond_func, env_marg_cdf, env_marg_func;
+    trcapi::DevBuf tri_uv, tex_info, tex8, texf;   // material textures
     int64_t num_nodes = 0;
     int max_stack = 0;
     int bvh_depth = 0;

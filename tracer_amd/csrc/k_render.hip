@@ -136,8 +136,9 @@ struct ShadeOut {
 
 // One vertex of the path in `slot`: reads its state and hit, adds emission,
 // samples the continuation and a light, and stores the updated state.
-// IMG: the scene has an image environment (its sampling code is compiled in).
-template <bool GUIDE, bool IMG>
+// EXT: the scene has an image environment or textured materials (their
+// code is compiled in only then).
+template <bool GUIDE, bool EXT>
 TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c, int slot, ShadeOut& out) {
     bool& cont = out.cont;
     ShadowOut& sh = out.sh;
@@ -170,7 +171,7 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
         // miss: the environment (environment_light.cpp:179-196)
         if (s.has_env) {
             float epdf;
-            const f3 Le = env_eval<IMG>(s, d, epdf);
+            const f3 Le = env_eval<EXT>(s, d, epdf);
             float w = 1.0f;
             if (prev_pdf > 0.0f) w = power_heuristic(prev_pdf, s.light_pmf[s.num_area_lights] * epdf);
             L += T * Le * w;
@@ -220,7 +221,7 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
                 float pmf;
                 const int e = pick_light(s, rng.uniform(), pmf);
                 const float lu = rng.uniform(), lv = rng.uniform();
-                const LightSampleD ls = sample_light<IMG>(s, e, sp_pos, lu, lv);
+                const LightSampleD ls = sample_light<EXT>(s, e, sp_pos, lu, lv);
                 if (ls.pdf != 0.0f && pmf > 0.0f && !is_black(ls.L)) {
                     const float ph = hg_phase(m.g, -dot(ls.wi, wo));
                     float w = 1.0f;
@@ -269,7 +270,8 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
                 f3 tx, ty;
                 onb(ns, tx, ty);
                 auto to_local = [&](f3 v) { return mk(dot(v, tx), dot(v, ty), dot(v, ns)); };
-                const DevMaterial mat = s.mats[sr.material];
+                DevMaterial mat = s.mats[sr.material];
+                if (EXT && mat.tex > 0) apply_texture(s, mat, tri, b1, b2);
                 const f3 wo = -d;
                 const f3 lwo = to_local(wo), lng = to_local(ng);
                 Mixture mix;
@@ -320,7 +322,7 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
                         float pmf;
                         const int e = pick_light(s, rng.uniform(), pmf);
                         const float lu = rng.uniform(), lv = rng.uniform();
-                        const LightSampleD ls = sample_light<IMG>(s, e, pos, lu, lv);
+                        const LightSampleD ls = sample_light<EXT>(s, e, pos, lu, lv);
                         if (ls.pdf != 0.0f && pmf > 0.0f && !is_black(ls.L)) {
                             const float dist = ls.area ? ls.dist - kEps : ls.dist;
                             if (dist > kEps) {
@@ -430,7 +432,7 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
 // with the combined pdf.  A vertex
 // whose head outputs are not finite uses the plain BSDF estimator, drawing
 // the same random numbers as the plain kernel.
-template <bool GUIDE, bool IMG>
+template <bool GUIDE, bool EXT>
 __global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene s, DevPaths p, RenderConsts c, int cur) {
     __shared__ BlockAppendLds<kShadeBlock> app;
     const unsigned int n = *queue_counter(p, cur);
@@ -451,7 +453,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene s, DevPaths p, R
         int slot = 0;
         if (active) {
             slot = q[i];
-            shade_path<GUIDE, IMG>(s, p, c, slot, o);
+            shade_path<GUIDE, EXT>(s, p, c, slot, o);
         }
         const bool cont = o.cont, g_used = o.g_used, g_fallback = o.g_fallback;
         const bool want_rec = o.want_rec, want_nee_rec = o.want_nee_rec;
@@ -519,7 +521,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene s, DevPaths p, R
 // same order as in the per-bounce kernels, so the film is identical.  The
 // last workgroup to finish empties the queue, so the per-bounce kernels that
 // follow find no work; with more than max_tail paths nothing happens.
-template <bool IMG>
+template <bool EXT>
 __global__ __launch_bounds__(kShadeBlock) void k_tail(DevScene s, DevPaths p, RenderConsts c, int cur,
                                                       unsigned int max_tail) {
     __shared__ TraceStack<kShadeBlock, kLdsStack> st;
@@ -542,7 +544,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_tail(DevScene s, DevPaths p, Re
             p.hit[slot] = make_float4(h.t, __int_as_float(h.tri), h.b1, h.b2);
             ++n_ext;
             ShadeOut o;
-            shade_path<false, IMG>(s, p, c, slot, o);
+            shade_path<false, EXT>(s, p, c, slot, o);
             if (o.sh.emit) {
                 ++n_sh;
                 TraceHit hs;
@@ -704,7 +706,7 @@ void launch_generate(const DevScene& s, const DevPaths& p, const RenderConsts& c
 void launch_shade(const DevScene& s, const DevPaths& p, const RenderConsts& c, int cur, int max_paths, bool guide,
                   hipStream_t st) {
     const dim3 g(shade_blocks(max_paths)), b(kShadeBlock);
-    const bool img = s.env.w > 0;
+    const bool img = s.env.w > 0 || s.has_tex;
     if (guide && img) hipLaunchKernelGGL((k_shade<true, true>), g, b, 0, st, s, p, c, cur);
     else if (guide) hipLaunchKernelGGL((k_shade<true, false>), g, b, 0, st, s, p, c, cur);
     else if (img) hipLaunchKernelGGL((k_shade<false, true>), g, b, 0, st, s, p, c, cur);
@@ -714,7 +716,7 @@ void launch_tail(const DevScene& s, const DevPaths& p, const RenderConsts& c, in
     // one thread per path, and no more threads than the traversal spill area has columns
     int blocks = std::min((max_tail + kShadeBlock - 1) / kShadeBlock, p.ovf_threads / kShadeBlock);
     if (blocks < 1) blocks = 1;
-    if (s.env.w > 0)
+    if (s.env.w > 0 || s.has_tex)
         hipLaunchKernelGGL(k_tail<true>, dim3(blocks), dim3(kShadeBlock), 0, st, s, p, c, cur, (unsigned int)max_tail);
     else
         hipLaunchKernelGGL(k_tail<false>, dim3(blocks), dim3(kShadeBlock), 0, st, s, p, c, cur, (unsigned int)max_tail);

@@ -46,6 +46,7 @@ struct DevMaterial {
     float w_trans, w_spec, w_cc;
     float guide_share;   // guide mixture probability at this material, as a share of the guide fraction (host, make_material)
     float ns;            // Phong exponent
+    int tex;             // 0 or 1 + index of the texture giving kd / base_color per hit
 };
 
 struct BsdfSample {

@@ -56,6 +56,13 @@ struct DevScene {
     int has_env, has_sun;
     f3 env_L;
     DevEnvMap env;
+    // material textures (has_tex): per-triangle uvs (2 float4 in leaf order),
+    // per texture (w, h, 8-bit, texel offset), 8-bit and float texel pools
+    const float4* tri_uv;
+    const int4* tex_info;
+    const uchar4* tex8;
+    const float4* texf;
+    int has_tex;
     f3 sun_dir;
     f3 sun_E;
     f3 wcenter;
@@ -160,6 +167,71 @@ TR_D f3 env_eval(const DevScene& s, f3 d, float& pdf) {
         pdf = e.cond_func[size_t(iv) * e.nu + iu] / e.marg_int / (2.0f * kPi * kPi * st);
     }
     return env_texel(e, u, v);
+}
+
+// Texture2D::evaluate (wrap_repeat) with LinearSampler::Sample2D in floating
+// point; 8-bit texels read as value / 255.
+TR_D f3 tex_lookup(const DevScene& s, int t, float u, float v) {
+    const int4 ti = s.tex_info[t];
+    const int w = ti.x, h = ti.y;
+    u = fminf(fmaxf(u - floorf(u), 0.0f), 1.0f) * float(w - 1);
+    v = fminf(fmaxf(v - floorf(v), 0.0f), 1.0f) * float(h - 1);
+    const int u0 = min(int(u), w - 1), u1 = min(u0 + 1, w - 1);
+    const int v0 = min(int(v), h - 1), v1 = min(v0 + 1, h - 1);
+    const float du = u - float(u0), dv = v - float(v0);
+    const size_t o = size_t(ti.w);
+    f3 a, b, c, d;
+    if (ti.z) {
+        auto ld = [&](int x, int y) {
+            const uchar4 q = s.tex8[o + size_t(y) * w + x];
+            return mk(float(q.x), float(q.y), float(q.z)) * (1.0f / 255.0f);
+        };
+        a = ld(u0, v0); b = ld(u1, v0); c = ld(u0, v1); d = ld(u1, v1);
+    } else {
+        a = xyz(s.texf[o + size_t(v0) * w + u0]); b = xyz(s.texf[o + size_t(v0) * w + u1]);
+        c = xyz(s.texf[o + size_t(v1) * w + u0]); d = xyz(s.texf[o + size_t(v1) * w + u1]);
+    }
+    return (a * (1.0f - du) + b * du) * (1.0f - dv) + (c * (1.0f - du) + d * du) * dv;
+}
+
+// The material at a textured hit: kd (Phong) or base_color (Disney) from the
+// texture at the interpolated uv (alpha weights B, beta C, triangle.cpp:178-190),
+// and the constants that depend on it recomputed as Phong::shading and the
+// DisneyBSDF constructor compute them (phong.cpp:36-53, disney.cpp:347-399).
+TR_D void apply_texture(const DevScene& s, DevMaterial& m, int tri, float b1, float b2) {
+    const float4 p = s.tri_uv[2 * size_t(tri)], q = s.tri_uv[2 * size_t(tri) + 1];
+    const float b0 = 1.0f - b1 - b2;
+    const float u = p.x * b0 + p.z * b1 + q.x * b2, v = p.y * b0 + p.w * b1 + q.y * b2;
+    const f3 c = tex_lookup(s, m.tex - 1, u, v);
+    if (m.type == MAT_PHONG) {
+        const f3 ks_raw = m.spec_scale;
+        const float dem = fmaxf(1.0f, max3(c + ks_raw));
+        const f3 kd = c * (1.0f / dem), ks = ks_raw * (1.0f / dem);
+        const float ld = lum(kd), ls = lum(ks);
+        m.w_diffuse = ld + ls > kEps ? ld / (ld + ls) : 0.5f;
+        m.w_spec = 1.0f - m.w_diffuse;
+        m.C = kd + ks;
+        m.Ctint = kd;
+        m.spec_scale = ks;
+        if (m.guide_share > 0.0f) m.guide_share = m.w_diffuse;
+    } else if (m.type == MAT_LAMBERT) {
+        m.C = c * (1.0f / fmaxf(1.0f, max3(c)));
+    } else if (m.type == MAT_DISNEY) {
+        m.C = c;
+        const float l = lum(c);
+        m.Ctint = l > 0.0f ? c * (1.0f / l) : f3s(1.0f);
+        const float A = fminf(fmaxf(l * (1.0f - m.metallic), 0.3f), 0.7f);
+        const float B = 1.0f - A;
+        m.w_diffuse = A * (1.0f - m.transmission);
+        m.w_trans = A * m.transmission;
+        m.w_spec = B * 2.0f / (2.0f + m.clearcoat);
+        m.w_cc = B * m.clearcoat / (2.0f + m.clearcoat);
+        if (m.guide_share > 0.0f) {   // the broad share of the lobe weights, as make_material sets it
+            const float tot = m.w_diffuse + m.w_trans + m.w_spec + m.w_cc;
+            const float broad = m.w_diffuse + (m.ax >= 0.25f ? m.w_spec : 0.0f);
+            m.guide_share = tot > 0.0f ? broad / tot : 0.0f;
+        }
+    }
 }
 
 struct LightSampleD {

@@ -287,6 +287,17 @@ tr_scene_desc FlatScene::desc() const {
     d.env_height = env_h;
     d.env_pixels = env_pix.empty() ? nullptr : env_pix.data();
     std::memcpy(d.env_world_to_light, env_rot, sizeof env_rot);
+    tex_desc.clear();
+    for (const ImageTexture2D& t : tex) {
+        tr_texture td{};
+        td.width = t.w;
+        td.height = t.h;
+        td.rgb8 = t.is8() ? t.rgb8.data() : nullptr;
+        td.rgbf = t.is8() ? nullptr : t.rgb.data();
+        tex_desc.push_back(td);
+    }
+    d.num_textures = int32_t(tex_desc.size());
+    d.textures = tex_desc.empty() ? nullptr : tex_desc.data();
     return d;
 }
 
@@ -296,16 +307,31 @@ static void tex_rgb(const RC<const Texture2D>& t, float out[3]) {
     out[0] = s.r; out[1] = s.g; out[2] = s.b;
 }
 
-static tr_material convert_material(const Material* m) {
+// An image texture's id in the flattened scene (1 + index), registering it once.
+static int32_t texture_id(const Texture2D* t, FlatScene& f, std::unordered_map<const Texture2D*, int32_t>& ids) {
+    const auto* img = dynamic_cast<const ImageTexture2D*>(t);
+    if (!img) throw std::runtime_error("render: unsupported texture type");
+    auto it = ids.find(t);
+    if (it != ids.end()) return it->second;
+    f.tex.push_back(*img);
+    const int32_t id = int32_t(f.tex.size());
+    ids[t] = id;
+    return id;
+}
+
+static tr_material convert_material(const Material* m, FlatScene& f, std::unordered_map<const Texture2D*, int32_t>& tex_ids) {
     tr_material r{};
     if (auto d = dynamic_cast<const DisneyMaterial*>(m)) {
-        const RC<const Texture2D>* all[] = {&d->base_color, &d->metallic, &d->roughness, &d->transmission,
+        // base_color may be an image; the other parameters are constants
+        const RC<const Texture2D>* all[] = {&d->metallic, &d->roughness, &d->transmission,
                                             &d->transmission_roughness, &d->ior, &d->specular_scale, &d->specular_tint,
                                             &d->anisotropic, &d->sheen, &d->sheen_tint, &d->clearcoat, &d->clearcoat_gloss};
         for (auto* t : all)
-            if (!(*t)->is_constant()) throw std::runtime_error("Disney: only constant textures are supported by the GPU path");
+            if (!(*t)->is_constant())
+                throw std::runtime_error("Disney: only base_color may be an image texture on the GPU path");
         r.type = TR_MAT_DISNEY;
-        tex_rgb(d->base_color, r.base_color);
+        if (d->base_color->is_constant()) tex_rgb(d->base_color, r.base_color);
+        else r.base_color_texture = texture_id(d->base_color.get(), f, tex_ids);
         r.metallic = tex_s(d->metallic);
         r.roughness = tex_s(d->roughness);
         r.transmission = tex_s(d->transmission);
@@ -321,8 +347,16 @@ static tr_material convert_material(const Material* m) {
         return r;
     }
     if (auto p = dynamic_cast<const PhongMaterial*>(m)) {
-        if (!p->kd->is_constant() || !p->ks->is_constant() || !p->ns->is_constant())
-            throw std::runtime_error("Phong: only constant textures are supported");
+        if (!p->ks->is_constant() || !p->ns->is_constant())
+            throw std::runtime_error("Phong: only kd may be an image texture on the GPU path");
+        if (!p->kd->is_constant()) {   // kd per hit from the image; the library normalises per hit
+            const Spectrum ks = p->ks->evaluate(Point2f(0, 0));
+            r.type = TR_MAT_PHONG;
+            r.base_color_texture = texture_id(p->kd.get(), f, tex_ids);
+            r.specular_scale[0] = ks.r; r.specular_scale[1] = ks.g; r.specular_scale[2] = ks.b;
+            r.shininess = tex_s(p->ns);
+            return r;
+        }
         const Spectrum kd = p->kd->evaluate(Point2f(0, 0)), ks = p->ks->evaluate(Point2f(0, 0));
         if (!ks.is_back()) {
             // diffuse + Phong specular lobe; the library normalises and weighs them
@@ -362,6 +396,7 @@ FlatScene flatten(const Scene& scene) {
     f.max_leaf = bvh->max_leaf;
     std::unordered_map<const TriangleMesh*, int32_t> mesh_base;
     std::unordered_map<const Material*, int32_t> mat_id;
+    std::unordered_map<const Texture2D*, int32_t> tex_ids;
     std::unordered_map<const Medium*, int32_t> med_id;
     auto medium_index = [&](const RC<const Medium>& m) -> int32_t {
         if (dynamic_cast<const VacuumMedium*>(m.get())) return -1;
@@ -409,7 +444,7 @@ FlatScene flatten(const Scene& scene) {
         int32_t mid;
         if (mit == mat_id.end()) {
             mid = int32_t(f.mats.size());
-            f.mats.push_back(convert_material(gp->material.get()));
+            f.mats.push_back(convert_material(gp->material.get(), f, tex_ids));
             mat_id[gp->material.get()] = mid;
         } else {
             mid = mit->second;

@@ -18,16 +18,25 @@ public:
     Spectrum c;
 };
 
-// An image loaded by create_texture2d_from_file: h rows of w RGB texels,
+// An image loaded by create_texture2d_from_file / create_texture_from_file:
+// h rows of w RGB texels, float (rgb) or 8-bit (rgb8, read as value / 255),
 // sampled bilinearly with wrap_repeat.
 class ImageTexture2D : public Texture2D {
 public:
     Spectrum evaluate(const Point2f& uv) const override;
     int w = 0, h = 0;
     std::vector<float> rgb;
+    std::vector<uint8_t> rgb8;
+    bool is8() const { return !rgb8.empty(); }
 };
 // Radiance RGBE (.hdr) reader; throws std::runtime_error on a read or format error.
 void read_hdr(const std::string& path, int& width, int& height, std::vector<float>& rgb);
+// 8-bit PNG reader (grey, RGB, palette, RGBA; grey + alpha is refused as the
+// reference refuses 2-channel images), rows bottom to top as the reference's
+// stbi_set_flip_vertically_on_load(true) returns them.
+void read_png(const std::string& path, int& width, int& height, std::vector<uint8_t>& rgb);
+// A texture named in an MTL material, scaled by its colour (image_file.cpp:100-127).
+RC<Texture2D> texture_from_file_scaled(const std::string& name, const real* scale);
 
 class DisneyMaterial : public Material {
 public:
@@ -138,6 +147,8 @@ struct FlatScene {
     int env_w = 0, env_h = 0;             // image environment
     std::vector<float> env_pix;
     float env_rot[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    std::vector<ImageTexture2D> tex;   // copies of the material textures
+    mutable std::vector<tr_texture> tex_desc;
     tr_camera cam{};
     int max_leaf = 3;
     tr_scene_desc desc() const;

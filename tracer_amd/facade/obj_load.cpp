@@ -156,6 +156,16 @@ void load_mtl(const std::string& path, std::vector<material_t>& mats, std::unord
         if (read_floats(rest, v, 3) < 1) parse_error(path, ln, "expected a colour");
         for (int k = 0; k < 3; ++k) dst[k] = v[k];
     };
+    // a texture found next to the MTL file is read from there, else the name is
+    // used as given (the reference opens it relative to the working directory)
+    auto tn = [&]() {
+        const std::string n = texture_name(rest);
+        if (!n.empty() && n[0] != '/') {
+            const std::string j = dir_of(path) + n;
+            if (std::ifstream(j).good()) return j;
+        }
+        return n;
+    };
     auto scalar = [&]() {
         float v = 0;
         if (read_floats(rest, &v, 1) != 1) parse_error(path, ln, "expected a number");
@@ -191,19 +201,19 @@ void load_mtl(const std::string& path, std::vector<material_t>& mats, std::unord
         else if (key == "Ps") cur->sheen = scalar();
         else if (key == "Pc") cur->clearcoat_thickness = scalar();
         else if (key == "Pcr") cur->clearcoat_roughness = scalar();
-        else if (key == "map_Ka") cur->map_ka = texture_name(rest);
-        else if (key == "map_Kd") cur->map_kd = texture_name(rest);
-        else if (key == "map_Ks") cur->map_ks = texture_name(rest);
-        else if (key == "map_Ns") cur->map_ns = texture_name(rest);
-        else if (key == "map_bump" || key == "map_Bump" || key == "bump") cur->map_bump = texture_name(rest);
-        else if (key == "disp") cur->disp = texture_name(rest);
-        else if (key == "map_d") cur->map_d = texture_name(rest);
-        else if (key == "refl") cur->refl = texture_name(rest);
-        else if (key == "map_Pr") cur->map_pr = texture_name(rest);
-        else if (key == "map_Pm") cur->map_pm = texture_name(rest);
-        else if (key == "map_Ps") cur->map_ps = texture_name(rest);
-        else if (key == "map_Ke") cur->map_ke = texture_name(rest);
-        else if (key == "norm") cur->norm = texture_name(rest);
+        else if (key == "map_Ka") cur->map_ka = tn();
+        else if (key == "map_Kd") cur->map_kd = tn();
+        else if (key == "map_Ks") cur->map_ks = tn();
+        else if (key == "map_Ns") cur->map_ns = tn();
+        else if (key == "map_bump" || key == "map_Bump" || key == "bump") cur->map_bump = tn();
+        else if (key == "disp") cur->disp = tn();
+        else if (key == "map_d") cur->map_d = tn();
+        else if (key == "refl") cur->refl = tn();
+        else if (key == "map_Pr") cur->map_pr = tn();
+        else if (key == "map_Pm") cur->map_pm = tn();
+        else if (key == "map_Ps") cur->map_ps = tn();
+        else if (key == "map_Ke") cur->map_ke = tn();
+        else if (key == "norm") cur->norm = tn();
     }
 }
 
@@ -340,8 +350,7 @@ model_t load_model_from_file(const std::string& path) {
 
 namespace {
 RC<Texture2D> constant_or_file(const std::string& name, const real* rgb) {
-    if (!name.empty())
-        throw std::runtime_error("create_texture_from_file: image textures are not supported by the GPU path (" + name + ")");
+    if (!name.empty()) return detail::texture_from_file_scaled(name, rgb);
     return create_constant_texture2d(Spectrum(rgb[0], rgb[1], rgb[2]));
 }
 }  // namespace

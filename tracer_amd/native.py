@@ -100,6 +100,9 @@ def scenes_lib() -> C.CDLL:
         lib.trs_read_hdr.argtypes = [C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_float),
                                      C.c_int64, C.POINTER(C.c_char_p)]
         lib.trs_read_hdr.restype = C.c_int32
+        lib.trs_read_png.argtypes = [C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_uint8),
+                                     C.c_int64, C.POINTER(C.c_char_p)]
+        lib.trs_read_png.restype = C.c_int32
         lib.trs_write_hdr.argtypes = [C.c_char_p, C.c_int32, C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_char_p)]
         lib.trs_write_hdr.restype = C.c_int32
         lib.trs_write_png.argtypes = [C.c_char_p, C.c_int32, C.c_int32, C.POINTER(C.c_uint8), C.POINTER(C.c_char_p)]
@@ -136,6 +139,7 @@ class Scene:
     media: list = field(default_factory=list)
     env: tuple | None = None
     env_image: np.ndarray | None = None   # (h, w, 3) linear RGB: an image environment (needs env set)
+    textures: list = field(default_factory=list)   # (h, w, 3) uint8 or float32 arrays; materials name 1 + index
     env_world_to_light: np.ndarray | None = None   # 3x3 rotation
     sun: tuple | None = None          # (direction, irradiance)
     camera: abi.TrCamera | None = None
@@ -194,7 +198,20 @@ class Scene:
             d.sun_direction = abi.f3(*self.sun[0])
             d.sun_irradiance = abi.f3(*self.sun[1])
         d.camera = self.camera if self.camera is not None else abi.TrCamera()
-        self._keep = [mats, media]
+        texs = None
+        if self.textures:
+            self.textures = [np.ascontiguousarray(t, np.uint8 if t.dtype == np.uint8 else np.float32)
+                             for t in self.textures]
+            texs = (abi.TrTexture * len(self.textures))()
+            for i, t in enumerate(self.textures):
+                texs[i].height, texs[i].width = t.shape[:2]
+                if t.dtype == np.uint8:
+                    texs[i].rgb8 = t.ctypes.data_as(C.POINTER(C.c_uint8))
+                else:
+                    texs[i].rgbf = fptr(t)
+            d.num_textures = len(self.textures)
+            d.textures = C.cast(texs, C.POINTER(abi.TrTexture))
+        self._keep = [mats, media, texs]
         return d
 
     def render_params(self, **kw) -> TrRenderParams:
@@ -246,6 +263,21 @@ def read_hdr(path) -> np.ndarray:
     return out
 
 
+def read_png(path) -> np.ndarray:
+    """An 8-bit PNG as (height, width, 3) uint8, rows bottom to top."""
+    lib = scenes_lib()
+    err = C.c_char_p()
+    w, h = C.c_int32(), C.c_int32()
+    p = str(path).encode()
+    if lib.trs_read_png(p, C.byref(w), C.byref(h), None, 0, C.byref(err)) != 0:
+        raise TracerError(abi.TR_ERR_INVALID, (err.value or b"").decode())
+    out = np.zeros((h.value, w.value, 3), np.uint8)
+    if lib.trs_read_png(p, C.byref(w), C.byref(h), out.ctypes.data_as(C.POINTER(C.c_uint8)), out.size,
+                        C.byref(err)) != 0:
+        raise TracerError(abi.TR_ERR_INVALID, (err.value or b"").decode())
+    return out
+
+
 def write_hdr(path, rgb: np.ndarray) -> None:
     """write_image_to_hdr of a (height, width, 3) float image."""
     a = np.ascontiguousarray(rgb, np.float32)
@@ -286,6 +318,14 @@ def _scene_from_handle(lib, h) -> Scene:
         media = [TrMedium.from_buffer_copy(d.media[i]) for i in range(d.num_media)]
         defaults = TrRenderParams()
         lib.trs_render_defaults(h, C.byref(defaults))
+        textures = []
+        for i in range(d.num_textures):
+            t = d.textures[i]
+            n = t.height * t.width * 3
+            if t.rgb8:
+                textures.append(np.ctypeslib.as_array(t.rgb8, shape=(n,)).reshape(t.height, t.width, 3).copy())
+            else:
+                textures.append(np.ctypeslib.as_array(t.rgbf, shape=(n,)).reshape(t.height, t.width, 3).copy())
         s = Scene(
             positions=arr(d.positions, 3 * nv, np.float32), indices=arr(d.indices, 3 * nt, np.int32),
             material_ids=arr(d.material_ids, nt, np.int32), materials=mats,
@@ -294,7 +334,8 @@ def _scene_from_handle(lib, h) -> Scene:
             medium_outside=arr(d.medium_outside, nt, np.int32), media=media,
             env=tuple(d.env_radiance) if d.has_env else None,
             sun=(tuple(d.sun_direction), tuple(d.sun_irradiance)) if d.has_sun else None,
-            camera=abi.TrCamera.from_buffer_copy(d.camera), defaults=defaults, max_leaf=lib.trs_max_leaf(h))
+            camera=abi.TrCamera.from_buffer_copy(d.camera), defaults=defaults, max_leaf=lib.trs_max_leaf(h),
+            textures=textures)
         return s
     finally:
         lib.trs_destroy(h)
