@@ -31,7 +31,9 @@ Also reported on the same JSON line:
                  the GPU's time-to-equal-relMSE speedup on the same film.
   other_configs  configuration 3 (fog) full frame, configuration 5 (18.9M
                  triangles) at 16 spp, configuration 2 under a 2048 x 1024
-                 image environment instead of its constant one, and OBJ
+                 image environment instead of its constant one, configuration
+                 2 with a 1024 x 1024 8-bit base-colour texture, configuration
+                 3 with a 64^3 density grid in place of its fog, and OBJ
                  ingest: a 328K-triangle OBJ/MTL pair read and flattened by
                  the facade (MB/s, triangles/s) and rendered.
 """
@@ -170,9 +172,50 @@ def other_configs() -> dict:
     print(f"[bench] configuration 2, image environment: {out['config2_env_image']['samples_per_s'] / 1e6:.1f} M samples/s",
           file=sys.stderr, flush=True)
 
+    # configuration 2 with the Disney base colour from a 1024x1024 8-bit texture
+    # through spherical texture coordinates of the mesh vertices
+    scene = native.config_scene(2)
+    P = scene.positions.reshape(-1, 3).astype(np.float64)
+    D = P - P.mean(0)
+    R = np.linalg.norm(D, axis=1) + 1e-12
+    scene.uvs = np.stack([0.5 + np.arctan2(D[:, 2], D[:, 0]) / (2 * np.pi) * 8,
+                          np.arccos(np.clip(D[:, 1] / R, -1, 1)) / np.pi * 4], 1).astype(np.float32)
+    tex = np.random.default_rng(1).integers(0, 256, (1024, 1024, 3), dtype=np.uint8)
+    scene.textures = [tex]
+    for m in scene.materials:
+        m.base_color_texture = 1
+    p = scene.render_params(seed=0)
+    out["config2_textured"] = {"workload": f"configuration 2 with every material's base colour from a 1024x1024 8-bit "
+                                           f"texture, {p.width}x{p.height}, {p.spp} spp, max depth {p.max_depth}",
+                               **frame(scene, p)}
+    print(f"[bench] configuration 2, textured: {out['config2_textured']['samples_per_s'] / 1e6:.1f} M samples/s",
+          file=sys.stderr, flush=True)
+
+    # configuration 3 with its homogeneous fog replaced by a 64^3 density grid
+    # spanning the room (a heterogeneous medium: delta and ratio tracking)
+    from tracer_amd import abi
+    scene = native.config_scene(3)
+    P = scene.positions.reshape(-1, 3)
+    lo, hi = P.min(0), P.max(0)
+    n = 64
+    z, y, x = np.meshgrid(*(np.arange(n) / (n - 1),) * 3, indexing="ij")
+    grid = 0.02 + 0.4 * np.exp(-((x - 0.5) ** 2 + (y - 0.4) ** 2 + (z - 0.5) ** 2) / 0.05)
+    fog = scene.media[0]
+    l2w = np.zeros((3, 4))
+    l2w[np.arange(3), np.arange(3)] = hi - lo
+    l2w[:, 3] = lo
+    scene.media = [abi.make_grid_medium(grid, albedo=tuple(fog.sigma_s[k] / max(fog.sigma_s[k] + fog.sigma_a[k], 1e-12)
+                                                         for k in range(3)),
+                                        g=fog.g, local_to_world=l2w, max_scattering_count=fog.max_scattering_count)]
+    p = scene.render_params(seed=0)
+    out["config3_grid_medium"] = {"workload": f"configuration 3 with a {n}^3 density grid in place of its fog, "
+                                              f"{p.width}x{p.height}, {p.spp} spp, max depth {p.max_depth}",
+                                  **frame(scene, p)}
+    print(f"[bench] configuration 3, grid medium: {out['config3_grid_medium']['samples_per_s'] / 1e6:.1f} M samples/s",
+          file=sys.stderr, flush=True)
+
     # OBJ ingest: configuration 2's spheres at a quarter of the size, written as OBJ + MTL
     import tempfile
-    from tracer_amd import abi
     src = native.config_scene(2, detail=0.25)
     with tempfile.TemporaryDirectory() as tmp:
         P = src.positions.reshape(-1, 3)

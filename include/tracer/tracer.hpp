@@ -188,10 +188,41 @@ public:
     virtual bool is_constant() const { return false; }
 };
 RC<Texture2D> create_constant_texture2d(const Spectrum& constant);
-// image_file.hpp:21 for Radiance .hdr files (linear RGB): the texture for an
-// image environment (create_ibl_light).  Other formats throw
-// std::runtime_error: the GPU path has no 8-bit image textures.
+// image_file.hpp:21: a Radiance .hdr file (linear RGB, the texture for an
+// image environment or any colour texture) or a PNG file (8-bit, rows flipped
+// as the reference loads them).  Other formats throw std::runtime_error.
 RC<const Texture2D> create_texture2d_from_file(const std::string& filename);
+
+// Volumes (src/utility/image.hpp, src/core/texture.hpp:108-170,
+// src/factory/texture.hpp:18-20): x fastest, then y, then z.
+template <class T>
+class Image3D {
+public:
+    Image3D() = default;
+    Image3D(int width, int height, int depth)
+        : w(width), h(height), d(depth), data(size_t(width) * size_t(height) * size_t(depth)) {}
+    T& at(int x, int y, int z) { return data[(size_t(z) * size_t(h) + size_t(y)) * size_t(w) + size_t(x)]; }
+    const T& at(int x, int y, int z) const { return data[(size_t(z) * size_t(h) + size_t(y)) * size_t(w) + size_t(x)]; }
+    int width() const { return w; }
+    int height() const { return h; }
+    int depth() const { return d; }
+    T* get_raw_data() { return data.data(); }
+    const T* get_raw_data() const { return data.data(); }
+private:
+    int w = 0, h = 0, d = 0;
+    std::vector<T> data;
+};
+
+class Texture3D {
+public:
+    virtual ~Texture3D() = default;
+    virtual Spectrum evaluate(const Point3f& uvw) const = 0;   // wrap_repeat, trilinear
+    virtual real evaluate_s(const Point3f& uvw) const { return evaluate(uvw).r; }
+    virtual real max_real() const = 0;
+    virtual bool is_constant() const { return false; }
+};
+RC<Texture3D> create_constant_texture3d(const Spectrum& constant);
+RC<Texture3D> create_image_texture3d(const RC<Image3D<real>>& image);
 
 class NormalMapper {
 public:
@@ -248,6 +279,11 @@ public:
 };
 RC<Medium> create_homogeneous_medium(const Spectrum& sigma_a, const Spectrum& sigma_s, real g, int max_scattering_count);
 RC<Medium> create_vacuum();
+// heterogeneous.cpp:101-108.  The GPU path takes a density that is an image or
+// constant 3-D texture and a constant albedo and g; other textures throw.
+RC<Medium> create_heterogeneous_medium(const Transform& local_to_world, RC<const Texture3D> density,
+                                       RC<const Texture3D> albedo, RC<const Texture3D> g, int max_scattering_count,
+                                       bool white_for_indirect);
 struct MediumInterface {
     RC<const Medium> inside;
     RC<const Medium> outside;

@@ -77,12 +77,26 @@ typedef struct tr_texture {
 } tr_texture;
 
 /* create_homogeneous_medium(sigma_a, sigma_s, g, max_scattering_count),
- * src/factory/medium.hpp:21-26, src/medium/homogeneous.cpp:12-18. */
+ * src/factory/medium.hpp:21-26, src/medium/homogeneous.cpp:12-18.
+ *
+ * With density != NULL the medium is create_heterogeneous_medium(
+ * local_to_world, density, albedo, g, max_scattering_count,
+ * white_for_indirect), src/factory/medium.hpp:13-19,
+ * src/medium/heterogeneous.cpp:10-108: a density grid of nx * ny * nz floats
+ * (x fastest, then y, then z; sigma_t = density) looked up trilinearly at the
+ * local position world_to_local(p) with every coordinate wrapped into [0, 1]
+ * (Texture3D's wrap_repeat).  sigma_s then holds the constant albedo, g the
+ * constant asymmetry, and sigma_a is unused.  local_to_world is a row-major
+ * 3x4 affine matrix; its inverse maps world positions to grid coordinates. */
 typedef struct tr_medium {
     float sigma_a[3];
     float sigma_s[3];
     float g;
-    int32_t max_scattering_count;
+    int32_t max_scattering_count;  /* scatters in a run before transmittance only; 0: none */
+    const float* density;          /* NULL: homogeneous */
+    int32_t nx, ny, nz;
+    int32_t white_for_indirect;    /* albedo 1 after the first scatter of a run */
+    float local_to_world[12];
 } tr_medium;
 
 /* create_thin_lens_camera(aspect, pos, target, up, fov, lens_radius,

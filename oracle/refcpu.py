@@ -70,6 +70,10 @@ def lib() -> C.CDLL:
         L.ref_env_radiance_integral.restype = None
         L.ref_tex_eval8.argtypes = [C.c_int, C.c_int, P(C.c_uint8), P(C.c_float), I64, I32, P(C.c_float)]
         L.ref_tex_eval8.restype = None
+        L.ref_grid_density.argtypes = [C.c_void_p, P(C.c_float), I64, P(C.c_float)]
+        L.ref_grid_density.restype = None
+        L.ref_grid_stats.argtypes = [C.c_void_p, P(C.c_float), P(C.c_float), I64, C.c_uint64, I32, P(C.c_double)]
+        L.ref_grid_stats.restype = None
         _lib = L
     return _lib
 
@@ -82,6 +86,26 @@ def tex_eval8(rgb8: np.ndarray, uv: np.ndarray, quirks: bool) -> np.ndarray:
     lib().ref_tex_eval8(t.shape[1], t.shape[0], t.ctypes.data_as(C.POINTER(C.c_uint8)), fptr(uv), uv.shape[0],
                         1 if quirks else 0, fptr(out))
     return out
+
+
+def grid_density(medium, pts: np.ndarray) -> np.ndarray:
+    """Densities of a heterogeneous medium (a TrMedium with a grid) at world
+    points (n, 3)."""
+    pts = np.ascontiguousarray(pts, np.float32)
+    out = np.zeros(pts.shape[0], np.float32)
+    lib().ref_grid_density(C.addressof(medium), fptr(pts), pts.shape[0], fptr(out))
+    return out
+
+
+def grid_stats(medium, a, b, trials: int, seed: int = 1, indirect: bool = False):
+    """(mean ratio-tracking transmittance, scatter fraction of delta
+    tracking, mean scatter weight) over `trials` segments a -> b."""
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    out = np.zeros(3, np.float64)
+    lib().ref_grid_stats(C.addressof(medium), fptr(a), fptr(b), trials, seed, 1 if indirect else 0,
+                         out.ctypes.data_as(C.POINTER(C.c_double)))
+    return tuple(out)
 
 
 class RefEnv:

@@ -100,6 +100,93 @@ struct RefTex {
     }
 };
 
+struct MediumSample { bool scattered = false; V3f pos; RGB<float> w{1, 1, 1}; };
+
+// Heterogeneous medium (src/medium/heterogeneous.cpp:10-108) over a float
+// density grid, x fastest: world_to_local = inverse(local_to_world), every
+// coordinate wrapped into [0, 1] (texture.hpp:114-116), trilinear in double
+// (LinearSampler::Sample3D, sampler.hpp:47-67); majorant = the largest voxel,
+// at least eps (common.hpp:47).
+struct RefGrid {
+    std::vector<float> d;   // empty: homogeneous medium
+    int nx = 0, ny = 0, nz = 0;
+    float w2l[12] = {};
+    float inv_max = 0;
+    RGB<float> albedo;
+    bool white = false;
+    void init(const tr_medium& m) {
+        d.assign(m.density, m.density + size_t(m.nx) * size_t(m.ny) * size_t(m.nz));
+        nx = m.nx; ny = m.ny; nz = m.nz;
+        float mx = 0;
+        for (float v : d) mx = std::max(mx, v);
+        inv_max = 1 / std::max(mx, kEps);
+        albedo = RGB<float>(m.sigma_s[0], m.sigma_s[1], m.sigma_s[2]);
+        white = m.white_for_indirect != 0;
+        // inverse of the 3x4 affine local_to_world by cofactors
+        const float* a = m.local_to_world;
+        const double c00 = double(a[5]) * a[10] - double(a[6]) * a[9], c01 = double(a[2]) * a[9] - double(a[1]) * a[10];
+        const double c02 = double(a[1]) * a[6] - double(a[2]) * a[5], c10 = double(a[6]) * a[8] - double(a[4]) * a[10];
+        const double c11 = double(a[0]) * a[10] - double(a[2]) * a[8], c12 = double(a[2]) * a[4] - double(a[0]) * a[6];
+        const double c20 = double(a[4]) * a[9] - double(a[5]) * a[8], c21 = double(a[1]) * a[8] - double(a[0]) * a[9];
+        const double c22 = double(a[0]) * a[5] - double(a[1]) * a[4];
+        const double det = a[0] * c00 + a[1] * c10 + a[2] * c20;
+        const double r[9] = {c00 / det, c01 / det, c02 / det, c10 / det, c11 / det, c12 / det, c20 / det, c21 / det, c22 / det};
+        for (int i = 0; i < 3; ++i) {
+            for (int j = 0; j < 3; ++j) w2l[4 * i + j] = float(r[3 * i + j]);
+            w2l[4 * i + 3] = float(-(r[3 * i] * a[3] + r[3 * i + 1] * a[7] + r[3 * i + 2] * a[11]));
+        }
+    }
+    float density(const V3f& p) const {
+        auto wrap = [](double x) { return std::clamp(x - std::floor(x), 0.0, 1.0); };
+        const float lx = w2l[0] * p.x + w2l[1] * p.y + w2l[2] * p.z + w2l[3];
+        const float ly = w2l[4] * p.x + w2l[5] * p.y + w2l[6] * p.z + w2l[7];
+        const float lz = w2l[8] * p.x + w2l[9] * p.y + w2l[10] * p.z + w2l[11];
+        const double u = wrap(lx) * (nx - 1), v = wrap(ly) * (ny - 1), k = wrap(lz) * (nz - 1);
+        const int u0 = std::clamp(int(u), 0, nx - 1), u1 = std::clamp(u0 + 1, 0, nx - 1);
+        const int v0 = std::clamp(int(v), 0, ny - 1), v1 = std::clamp(v0 + 1, 0, ny - 1);
+        const int k0 = std::clamp(int(k), 0, nz - 1), k1 = std::clamp(k0 + 1, 0, nz - 1);
+        const double du = u - u0, dv = v - v0, dk = k - k0;
+        auto at = [&](int x, int y, int z) { return double(d[(size_t(z) * size_t(ny) + size_t(y)) * size_t(nx) + size_t(x)]); };
+        return float(((at(u0, v0, k0) * (1.0 - du) + at(u1, v0, k0) * du) * (1.0 - dv) +
+                      (at(u0, v1, k0) * (1.0 - du) + at(u1, v1, k0) * du) * dv) * (1.0 - dk) +
+                     ((at(u0, v0, k1) * (1.0 - du) + at(u1, v0, k1) * du) * (1.0 - dv) +
+                      (at(u0, v1, k1) * (1.0 - du) + at(u1, v1, k1) * du) * dv) * dk);
+    }
+    // HeterogeneousMedium::tr (heterogeneous.cpp:35-52): ratio tracking
+    float tr(const V3f& a, const V3f& b, Sampler& sp) const {
+        float res = 1, t = 0;
+        const float tmax = float((b - a).length());
+        const V3f dir = (b - a).normalize();
+        while (true) {
+            t += -std::log(1 - sp.get()) * inv_max;
+            if (t >= tmax) break;
+            res *= 1 - density(a + dir * t) * inv_max;
+        }
+        return res;
+    }
+    // HeterogeneousMedium::sample (heterogeneous.cpp:54-82): delta tracking,
+    // weight = albedo at a scatter (1 after the first scatter of a run when
+    // white_for_indirect), 1 when the segment is crossed
+    MediumSample sample(const V3f& a, const V3f& b, Sampler& sp, bool indirect) const {
+        MediumSample r;
+        const float tmax = float((a - b).length());
+        const V3f dir = (b - a).normalize();
+        float t = 0;
+        while (true) {
+            t += -std::log(1 - sp.get()) * inv_max;
+            if (t >= tmax) break;
+            const V3f pos = a + dir * t;
+            if (sp.get() < density(pos) * inv_max) {
+                r.scattered = true;
+                r.pos = pos;
+                r.w = indirect && white ? RGB<float>(1) : albedo;
+                return r;
+            }
+        }
+        return r;
+    }
+};
+
 struct Scene {
     std::vector<V3f> P, N;
     std::vector<RefTex> tex;
@@ -108,6 +195,7 @@ struct Scene {
     std::vector<float> emis;
     std::vector<tr_material> mats;
     std::vector<tr_medium> media;
+    std::vector<RefGrid> grids;       // per medium; empty d: homogeneous
     RefBVH bvh;
     std::vector<int> lights;          // emissive triangles, one DiffuseLight each
     bool has_env = false;
@@ -254,7 +342,6 @@ static float pdf_env(const Scene& s, const V3f& dir, bool quirks) {
 
 // ----------------------------------------------------------------------------
 // Medium: homogeneous.cpp:26-66 (quirks) / spectral-MIS free flight (fixed)
-struct MediumSample { bool scattered = false; V3f pos; RGB<float> w{1, 1, 1}; };
 static RGB<float> medium_tr(const tr_medium* m, const V3f& a, const V3f& b) {
     if (!m) return RGB<float>(1);
     float d = float((a - b).length());
@@ -356,6 +443,16 @@ struct Renderer {
     Renderer(const Scene& sc, const RenderCfg& cf) : s(sc), c(cf) {}
 
     const tr_medium* med(int id) const { return id < 0 ? nullptr : &s.media[id]; }
+    const RefGrid* grid_of(const tr_medium* m) const {
+        if (!m) return nullptr;
+        const RefGrid& g = s.grids[size_t(m - s.media.data())];
+        return g.d.empty() ? nullptr : &g;
+    }
+    // transmittance of a segment: closed form (homogeneous) or ratio tracking (grid)
+    RGB<float> seg_tr(const tr_medium* m, const V3f& a, const V3f& b, Sampler& sp) const {
+        const RefGrid* g = grid_of(m);
+        return g ? RGB<float>(g->tr(a, b, sp)) : medium_tr(m, a, b);
+    }
     int wo_medium_id(const Isect& is) const {   // intersection.hpp:36-41
         return dot(is.wo, is.geo.z) >= 0 ? s.med_out[is.prim] : s.med_in[is.prim];
     }
@@ -398,7 +495,7 @@ struct Renderer {
             if (f.is_black()) return RGB<float>(0);
             float bp = pdff(dir);
             const tr_medium* m = surface ? med(medium_towards_light_id_surface) : cur_med;
-            RGB<float> tr = medium_tr(m, ls.pos, p);
+            RGB<float> tr = seg_tr(m, ls.pos, p, sp);
             RGB<float> val = tr * ls.L * f;
             if (surface) val *= std::abs(cosv(dir, ng));
             float w = power_heuristic(ls.pdf, bp);
@@ -473,7 +570,7 @@ struct Renderer {
             RGB<float> f = evalf(dir);
             if (f.is_black()) return RGB<float>(0);
             float bp = pdff(dir);
-            RGB<float> tr = medium_tr(med(medium_id(*is, dir)), ls.pos, p);
+            RGB<float> tr = seg_tr(med(medium_id(*is, dir)), ls.pos, p, sp);
             RGB<float> val = tr * ls.L * f * std::abs(cosv(dir, ng));
             return val * (power_heuristic(ls.pdf, bp) / ls.pdf);
         }
@@ -523,9 +620,9 @@ struct Renderer {
         RGB<float> tr;
         if (surface) {
             const tr_medium* m = med(medium_id(*is, bs.wi));
-            tr = c.quirks ? medium_tr(m, o, is->pos) : medium_tr(m, o, t.pos);   // Q5
+            tr = c.quirks ? seg_tr(m, o, is->pos, sp) : seg_tr(m, o, t.pos, sp);   // Q5
         } else {
-            tr = medium_tr(cur_med, o, t.pos);
+            tr = seg_tr(cur_med, o, t.pos, sp);
         }
         RGB<float> f = tr * Le * bs.f;
         if (surface) f *= std::abs(cosv(ng, wi));
@@ -568,7 +665,8 @@ struct Renderer {
             const tr_medium* m = med(mid);
             const int max_sc = m ? m->max_scattering_count : std::numeric_limits<int>::max();
             if (scattering < max_sc) {
-                MediumSample ms = medium_sample(m, c.quirks ? cam_o : o, is.pos, sp, c.quirks);   // Q1
+                MediumSample ms = grid_of(m) ? grid_of(m)->sample(c.quirks ? cam_o : o, is.pos, sp, scattering > 0)
+                                              : medium_sample(m, c.quirks ? cam_o : o, is.pos, sp, c.quirks);   // Q1
                 coef *= ms.w;
                 if (ms.scattered) {
                     scattering++;
@@ -599,7 +697,7 @@ struct Renderer {
                     continue;
                 }
             } else {
-                coef *= medium_tr(m, o, is.pos);
+                coef *= seg_tr(m, o, is.pos, sp);
                 scattering = 0;
             }
             if (!c.quirks) L += coef * Le_hit;
@@ -758,6 +856,12 @@ static Scene* build_scene(const tr_scene_desc* d, int max_leaf) {
     if (d->medium_outside) s->med_out.assign(d->medium_outside, d->medium_outside + nt);
     s->mats.assign(d->materials, d->materials + d->num_materials);
     if (d->num_media) s->media.assign(d->media, d->media + d->num_media);
+    s->grids.assign(s->media.size(), RefGrid{});
+    for (size_t i = 0; i < s->media.size(); ++i)
+        if (s->media[i].density) {
+            s->grids[i].init(s->media[i]);
+            s->media[i].density = nullptr;   // the grid copy above is used from here on
+        }
     std::vector<Bounds3> pb(nt);
     for (int64_t t = 0; t < nt; ++t) {
         Bounds3 b;
@@ -1053,6 +1157,32 @@ void ref_tex_eval8(int w, int h, const uint8_t* rgb8, const float* uv, int64_t n
         const RGB<float> c = t.eval(uv[2 * i], uv[2 * i + 1], quirks != 0);
         out[3 * i] = c.r; out[3 * i + 1] = c.g; out[3 * i + 2] = c.b;
     }
+}
+
+// A heterogeneous medium on its own: densities at world points, and over
+// `trials` segments a -> b the mean ratio-tracking transmittance, the fraction
+// of delta-tracking samples that scatter and their mean weight (red channel).
+void ref_grid_density(const tr_medium* m, const float* pts, int64_t n, float* out) {
+    RefGrid g;
+    g.init(*m);
+    for (int64_t i = 0; i < n; ++i) out[i] = g.density(V3f(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]));
+}
+void ref_grid_stats(const tr_medium* m, const float* a, const float* b, int64_t trials, uint64_t seed, int indirect,
+                    double* out) {
+    RefGrid g;
+    g.init(*m);
+    Sampler sp;
+    sp.rng.seed(seed, 7);
+    const V3f pa(a[0], a[1], a[2]), pb(b[0], b[1], b[2]);
+    double tr = 0, sc = 0, w = 0;
+    for (int64_t i = 0; i < trials; ++i) {
+        tr += g.tr(pa, pb, sp);
+        const MediumSample ms = g.sample(pa, pb, sp, indirect != 0);
+        if (ms.scattered) { sc += 1; w += ms.w.r; }
+    }
+    out[0] = tr / double(trials);
+    out[1] = sc / double(trials);
+    out[2] = sc > 0 ? w / sc : 0.0;
 }
 
 // Image environment on its own: IBL sample_li / light_emit / pdf.

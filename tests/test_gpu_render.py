@@ -272,3 +272,29 @@ def test_c2_disney_float_texture_matches_fixed_estimator(gpu_ctx):
             n += 1
     assert n > 0
     _compare_estimators(gpu_ctx, s, 32, 32, 512, 64, 8, 8, quirks=False, nee=0, max_depth=6)
+
+
+def _smoke_grid():
+    """A 12 x 10 x 8 density grid: a dense blob over a thin haze."""
+    nz, ny, nx = 8, 10, 12
+    z, y, x = np.meshgrid(np.arange(nz) / (nz - 1), np.arange(ny) / (ny - 1), np.arange(nx) / (nx - 1), indexing="ij")
+    r2 = (x - 0.6) ** 2 + (y - 0.45) ** 2 + (z - 0.55) ** 2
+    return (0.15 + 2.0 * np.exp(-r2 / 0.03)).astype(np.float32)
+
+
+@pytest.mark.parametrize("max_sc", [0, 3, 2 ** 31 - 1])
+def test_obj_room_grid_medium_matches_fixed_estimator(gpu_ctx, tmp_path, max_sc):
+    """The OBJ room filled with a heterogeneous medium built by the facade
+    (create_image_texture3d + create_heterogeneous_medium; the grid spans the
+    room and repeats outside it): delta tracking and ratio-tracked shadow
+    rays on the GPU against the oracle in fixed mode.  A limit of 3 ends runs
+    of scatters with ratio-tracked transmittance (path_trace_renderer.cpp:88-122);
+    0 allows no scatter at all, as in the reference."""
+    from tests import objfiles
+    from tracer_amd import native
+    obj, mtl = objfiles.box_room_obj()
+    path = objfiles.write(tmp_path, "room.obj", obj, mtl)
+    room = np.array([[2.0, 0, 0, -1.0], [0, 2.0, 0, -1.0], [0, 0, 2.0, -1.0]])
+    s = native.obj_grid_scene(path, objfiles.camera((0.0, 0.0, 3.6), (0.0, 0.0, 0.0)), _smoke_grid(), room,
+                              albedo=(0.8, 0.7, 0.6), g=0.4, max_scattering_count=max_sc)
+    _compare_estimators(gpu_ctx, s, 32, 32, 1024, 256, 8, 8, quirks=False, nee=0, max_depth=8)

@@ -37,7 +37,32 @@ class TrTexture(C.Structure):
 
 
 class TrMedium(C.Structure):
-    _fields_ = [("sigma_a", f3), ("sigma_s", f3), ("g", C.c_float), ("max_scattering_count", C.c_int32)]
+    _fields_ = [("sigma_a", f3), ("sigma_s", f3), ("g", C.c_float), ("max_scattering_count", C.c_int32),
+                ("density", C.POINTER(C.c_float)), ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+                ("white_for_indirect", C.c_int32), ("local_to_world", C.c_float * 12)]
+
+
+def make_grid_medium(density, albedo=(1.0, 1.0, 1.0), g=0.0, local_to_world=None, max_scattering_count=2 ** 31 - 1,
+                     white_for_indirect=False) -> TrMedium:
+    """create_heterogeneous_medium (src/medium/heterogeneous.cpp:101-108) over
+    a float density grid indexed [z, y, x]; local_to_world (3x4 or 4x4) maps
+    the unit grid cube into the world (identity by default).  The returned
+    structure keeps the grid alive."""
+    import numpy as np
+    grid = np.ascontiguousarray(density, np.float32)
+    if grid.ndim != 3:
+        raise ValueError("density must be a 3-D array indexed [z, y, x]")
+    m = TrMedium()
+    m.sigma_s = f3(*albedo)
+    m.g = g
+    m.max_scattering_count = max_scattering_count
+    m.density = grid.ctypes.data_as(C.POINTER(C.c_float))
+    m.nz, m.ny, m.nx = grid.shape
+    m.white_for_indirect = 1 if white_for_indirect else 0
+    M = np.eye(4) if local_to_world is None else np.asarray(local_to_world, np.float64)
+    m.local_to_world[:] = [float(x) for x in M[:3, :4].reshape(-1)]
+    m._grid = grid
+    return m
 
 
 class TrCamera(C.Structure):

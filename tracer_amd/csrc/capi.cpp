@@ -20,7 +20,7 @@ using namespace trcapi;
 
 // The ctypes bindings (tracer_amd/abi.py) assume these layouts.
 static_assert(sizeof(tr_material) == 80, "tr_material layout");
-static_assert(sizeof(tr_medium) == 32, "tr_medium layout");
+static_assert(sizeof(tr_medium) == 104, "tr_medium layout");
 static_assert(sizeof(tr_camera) == 52, "tr_camera layout");
 static_assert(sizeof(tr_render_params) == 88, "tr_render_params layout");
 static_assert(sizeof(tr_render_stats) == 160, "tr_render_stats layout");
@@ -142,15 +142,56 @@ DevMaterial make_material_untextured(const tr_material& m) {
     return d;
 }
 
+// Inverse of a row-major 3x4 affine matrix (cofactors, double precision).
+bool invert_affine(const float* a, double* out) {
+    const double m00 = a[0], m01 = a[1], m02 = a[2], m10 = a[4], m11 = a[5], m12 = a[6];
+    const double m20 = a[8], m21 = a[9], m22 = a[10];
+    const double c00 = m11 * m22 - m12 * m21, c01 = m02 * m21 - m01 * m22, c02 = m01 * m12 - m02 * m11;
+    const double c10 = m12 * m20 - m10 * m22, c11 = m00 * m22 - m02 * m20, c12 = m02 * m10 - m00 * m12;
+    const double c20 = m10 * m21 - m11 * m20, c21 = m01 * m20 - m00 * m21, c22 = m00 * m11 - m01 * m10;
+    const double det = m00 * c00 + m01 * c10 + m02 * c20;
+    if (!(std::abs(det) > 1e-30) || !std::isfinite(det)) return false;
+    const double r[9] = {c00 / det, c01 / det, c02 / det, c10 / det, c11 / det, c12 / det, c20 / det, c21 / det, c22 / det};
+    for (int i = 0; i < 3; ++i) {
+        for (int j = 0; j < 3; ++j) out[4 * i + j] = r[3 * i + j];
+        out[4 * i + 3] = -(r[3 * i] * a[3] + r[3 * i + 1] * a[7] + r[3 * i + 2] * a[11]);
+    }
+    return true;
+}
+
+// A grid medium (HeterogeneousMedium, src/medium/heterogeneous.cpp:12-29):
+// the majorant is the largest voxel, at least eps = 3e-4 (src/common.hpp:47);
+// sigma_s carries the albedo.
+DevMedium make_grid_medium(const tr_medium& m, const std::vector<float>& grid, int offset) {
+    DevMedium d{};
+    d.sigma_s = mk(m.sigma_s[0], m.sigma_s[1], m.sigma_s[2]);
+    d.g = std::min(std::max(m.g, -1.0f + kEps), 1.0f - kEps);
+    float mx = 0.0f;
+    for (float v : grid) mx = std::max(mx, v);
+    d.majorant = std::max(mx, kEps);
+    d.sigma_t = f3s(d.majorant);
+    d.max_scatter = m.max_scattering_count;   // 0: no scattering, transmittance only (as the reference)
+    d.grid = offset;
+    d.nx = m.nx;
+    d.ny = m.ny;
+    d.nz = m.nz;
+    double inv[12];
+    invert_affine(m.local_to_world, inv);
+    for (int k = 0; k < 12; ++k) d.w2l[k] = float(inv[k]);
+    d.white_indirect = m.white_for_indirect ? 1 : 0;
+    return d;
+}
+
 DevMedium make_medium(const tr_medium& m) {
     DevMedium d{};
+    d.grid = -1;
     d.sigma_s = mk(m.sigma_s[0], m.sigma_s[1], m.sigma_s[2]);
     const f3 sa = mk(m.sigma_a[0], m.sigma_a[1], m.sigma_a[2]);
     d.sigma_t = sa + d.sigma_s;
     // HomogeneousMedium clamps g into (-1, 1) (src/medium/homogeneous.cpp:16-17)
     d.g = std::min(std::max(m.g, -1.0f + kEps), 1.0f - kEps);
     d.majorant = max3(d.sigma_t);
-    d.max_scatter = m.max_scattering_count > 0 ? m.max_scattering_count : 0x7fffffff;
+    d.max_scatter = m.max_scattering_count;   // 0: no scattering, transmittance only (as the reference)
     if (d.majorant > 0.0f) {
         d.p_absorb = (sa.x + sa.y + sa.z) / (3.0f * d.majorant);
         d.p_scatter = (d.sigma_s.x + d.sigma_s.y + d.sigma_s.z) / (3.0f * d.majorant);
@@ -228,6 +269,16 @@ void validate_scene(const tr_scene_desc* d) {
     for (int i = 0; i < d->num_media; ++i) {
         const tr_medium& m = d->media[i];
         if (!(m.g > -1.0f && m.g < 1.0f)) throw InvalidArg("medium g must lie in (-1, 1)");  // homogeneous.cpp:16
+        if (m.density) {
+            if (m.nx < 1 || m.ny < 1 || m.nz < 1 || int64_t(m.nx) * m.ny * m.nz > (int64_t(1) << 31) - 1)
+                throw InvalidArg("density grid dimensions must be positive and hold fewer than 2^31 voxels");
+            const size_t n = size_t(m.nx) * size_t(m.ny) * size_t(m.nz);
+            for (size_t k = 0; k < n; ++k)
+                if (!(m.density[k] >= 0.0f && std::isfinite(m.density[k])))
+                    throw InvalidArg("density grid values must be finite and >= 0");
+            double inv[12];
+            if (!invert_affine(m.local_to_world, inv)) throw InvalidArg("medium local_to_world is singular");
+        }
         for (int k = 0; k < 3; ++k)
             if (m.sigma_a[k] < 0 || m.sigma_s[k] < 0) throw InvalidArg("medium coefficients must be >= 0");
     }
@@ -348,7 +399,16 @@ void build_device_scene(tr_ctx* c, const tr_bvh_opts* opts) {
         c->has_delta = c->has_delta || dm[i].type == MAT_GLASS;
     }
     std::vector<DevMedium> dmed(h.media.size());
-    for (size_t i = 0; i < h.media.size(); ++i) dmed[i] = make_medium(h.media[i]);
+    std::vector<float> voxels;
+    for (size_t i = 0; i < h.media.size(); ++i) {
+        if (h.grids[i].empty()) {
+            dmed[i] = make_medium(h.media[i]);
+        } else {
+            dmed[i] = make_grid_medium(h.media[i], h.grids[i], int(voxels.size()));
+            voxels.insert(voxels.end(), h.grids[i].begin(), h.grids[i].end());
+        }
+    }
+    if (voxels.size() > size_t(INT32_MAX)) throw InvalidArg("density grids hold more than 2^31 voxels in total");
 
     hipStream_t st = c->stream;
     upload(c->nodes, r.nodes.data(), r.nodes.size() * sizeof(Node4), st);
@@ -356,6 +416,7 @@ void build_device_scene(tr_ctx* c, const tr_bvh_opts* opts) {
     upload(c->tri_shade, shade.data(), shade.size() * sizeof(float4), st);
     upload(c->mats, dm.data(), dm.size() * sizeof(DevMaterial), st);
     upload(c->media, dmed.data(), dmed.size() * sizeof(DevMedium), st);
+    if (!voxels.empty()) upload(c->density, voxels.data(), voxels.size() * sizeof(float), st);
     upload(c->lights, lights.data(), lights.size() * sizeof(float4), st);
     upload(c->light_cdf, cdf.data(), cdf.size() * sizeof(float), st);
     upload(c->light_pmf, pmf.data(), pmf.size() * sizeof(float), st);
@@ -452,6 +513,8 @@ void build_device_scene(tr_ctx* c, const tr_bvh_opts* opts) {
     s.tex8 = any_tex ? c->tex8.as<uchar4>() : nullptr;
     s.texf = any_tex ? c->texf.as<float4>() : nullptr;
     s.has_tex = any_tex ? 1 : 0;
+    s.density = voxels.empty() ? nullptr : c->density.as<float>();
+    s.has_grid = voxels.empty() ? 0 : 1;
     const f3 sd = mk(h.sun_dir[0], h.sun_dir[1], h.sun_dir[2]);
     s.sun_dir = length(sd) > 0.0f ? normalize(sd) : mk(0.0f, 0.0f, 1.0f);
     s.sun_E = mk(h.sun_E[0], h.sun_E[1], h.sun_E[2]);
@@ -631,6 +694,14 @@ tr_status tr_scene_upload(tr_ctx* c, const tr_scene_desc* d) {
         if (d->medium_outside) h.med_out.assign(d->medium_outside, d->medium_outside + nt); else h.med_out.clear();
         h.mats.assign(d->materials, d->materials + d->num_materials);
         if (d->num_media) h.media.assign(d->media, d->media + d->num_media); else h.media.clear();
+        h.grids.assign(h.media.size(), {});
+        for (size_t i = 0; i < h.media.size(); ++i) {
+            tr_medium& m = h.media[i];
+            if (m.density) {
+                h.grids[i].assign(m.density, m.density + size_t(m.nx) * size_t(m.ny) * size_t(m.nz));
+                m.density = nullptr;   // the copy above is the grid from here on
+            }
+        }
         h.has_env = d->has_env;
         h.has_sun = d->has_sun;
         std::memcpy(h.env, d->env_radiance, sizeof h.env);

@@ -70,6 +70,7 @@ struct HostScene {
     std::vector<float> emission;
     std::vector<tr_material> mats;
     std::vector<tr_medium> media;
+    std::vector<std::vector<float>> grids;   // density grid per medium (empty: homogeneous)
     int has_env = 0, has_sun = 0;
     float env[3] = {0, 0, 0}, sun_dir[3] = {0, 0, 1}, sun_E[3] = {0, 0, 0};
     tr_camera cam{};
@@ -126,6 +127,7 @@ struct tr_ctx {
     trcapi::DevBuf nodes, tris, tri_shade, mats, media, lights, light_cdf, light_pmf, light_guide;
     trcapi::DevBuf env_tex, env_cond_cdf, env_cond_func, env_marg_cdf, env_marg_func;
     trcapi::DevBuf tri_uv, tex_info, tex8, texf;   // material textures
+    trcapi::DevBuf density;                        // voxels of every grid medium
     int64_t num_nodes = 0;
     int max_stack = 0;
     int bvh_depth = 0;

@@ -90,6 +90,15 @@ TR_D f3 shadow_tr(const DevMedium& m, float dist, int mode, Pcg& rng) {
     return T;
 }
 
+// Transmittance of a shadow segment from o along d: ratio tracking through a
+// grid medium (in the kernels specialised for extended scenes), else the
+// homogeneous estimate above.
+template <bool EXT>
+TR_D f3 segment_tr(const DevScene& s, const DevMedium& m, f3 o, f3 d, float dist, int mode, Pcg& rng) {
+    if (EXT && m.grid >= 0) return f3s(grid_tr(s, m, o, d, dist, rng));
+    return shadow_tr(m, dist, mode, rng);
+}
+
 struct ShadowOut {
     bool emit;
     f3 o;
@@ -188,7 +197,31 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
         const int mid = dn <= 0.0f ? sr.med_out : sr.med_in;
         bool scattered = false;
         f3 sp_pos;
-        if (mid >= 0) {
+        if (EXT && mid >= 0 && s.media[mid].grid >= 0) {
+            // heterogeneous medium: delta tracking against the grid maximum
+            // (HeterogeneousMedium::sample, src/medium/heterogeneous.cpp:54-82),
+            // ratio tracking when the run of scatters is at its limit
+            const DevMedium m = s.media[mid];
+            if (depth >= c.max_depth || scatter >= m.max_scatter) {
+                T *= grid_tr(s, m, o, d, t, rng);
+                if (scatter >= m.max_scatter) scatter = 0;
+            } else {
+                const float inv = 1.0f / m.majorant;
+                float tt = 0.0f;
+                for (int k = 0; k < (1 << 20); ++k) {
+                    tt -= logf(1.0f - rng.uniform()) * inv;
+                    if (tt >= t) break;
+                    const f3 x = o + d * tt;
+                    const float dens = grid_density(s, m, x);
+                    if (rng.uniform() < dens * inv) {
+                        if (!(m.white_indirect && scatter > 0)) T *= m.sigma_s;
+                        scattered = true;
+                        sp_pos = x;
+                        break;
+                    }
+                }
+            }
+        } else if (mid >= 0) {
             const DevMedium m = s.media[mid];
             if (depth >= c.max_depth || scatter >= m.max_scatter) {
                 T *= mk(expf(-m.sigma_t.x * t), expf(-m.sigma_t.y * t), expf(-m.sigma_t.z * t));
@@ -228,7 +261,7 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
                     float lp = pmf;
                     if (ls.pdf > 0.0f) { lp *= ls.pdf; w = power_heuristic(lp, ph); }
                     f3 trs = f3s(1.0f);
-                    if (ls.area) trs = shadow_tr(m, ls.dist, c.shadow_tr, rng);
+                    if (ls.area) trs = segment_tr<EXT>(s, m, sp_pos, ls.wi, ls.dist, c.shadow_tr, rng);
                     const f3 contrib = T * ls.L * trs * (ph * w / lp);
                     if (max3(contrib) > 0.0f && finite3(contrib)) {
                         sh.emit = true;
@@ -340,7 +373,7 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
                                     f3 trs = f3s(1.0f);
                                     if (ls.area) {
                                         const int lm = dot(ls.wi, ng) >= 0.0f ? sr.med_out : sr.med_in;
-                                        if (lm >= 0) trs = shadow_tr(s.media[lm], ls.dist, c.shadow_tr, rng);
+                                        if (lm >= 0) trs = segment_tr<EXT>(s, s.media[lm], pos, ls.wi, ls.dist, c.shadow_tr, rng);
                                     }
                                     const float cosg = fabsf(dot(ls.wi, ng));
                                     const f3 contrib = T * ls.L * be.f * trs * (cosg * w / lp);
@@ -706,7 +739,7 @@ void launch_generate(const DevScene& s, const DevPaths& p, const RenderConsts& c
 void launch_shade(const DevScene& s, const DevPaths& p, const RenderConsts& c, int cur, int max_paths, bool guide,
                   hipStream_t st) {
     const dim3 g(shade_blocks(max_paths)), b(kShadeBlock);
-    const bool img = s.env.w > 0 || s.has_tex;
+    const bool img = s.env.w > 0 || s.has_tex || s.has_grid;
     if (guide && img) hipLaunchKernelGGL((k_shade<true, true>), g, b, 0, st, s, p, c, cur);
     else if (guide) hipLaunchKernelGGL((k_shade<true, false>), g, b, 0, st, s, p, c, cur);
     else if (img) hipLaunchKernelGGL((k_shade<false, true>), g, b, 0, st, s, p, c, cur);
@@ -716,7 +749,7 @@ void launch_tail(const DevScene& s, const DevPaths& p, const RenderConsts& c, in
     // one thread per path, and no more threads than the traversal spill area has columns
     int blocks = std::min((max_tail + kShadeBlock - 1) / kShadeBlock, p.ovf_threads / kShadeBlock);
     if (blocks < 1) blocks = 1;
-    if (s.env.w > 0 || s.has_tex)
+    if (s.env.w > 0 || s.has_tex || s.has_grid)
         hipLaunchKernelGGL(k_tail<true>, dim3(blocks), dim3(kShadeBlock), 0, st, s, p, c, cur, (unsigned int)max_tail);
     else
         hipLaunchKernelGGL(k_tail<false>, dim3(blocks), dim3(kShadeBlock), 0, st, s, p, c, cur, (unsigned int)max_tail);

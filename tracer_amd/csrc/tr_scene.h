@@ -11,6 +11,8 @@ namespace trhip {
 // tracking: majorant = max channel sigma_t, collision-type probabilities from
 // the channel-averaged coefficients (spectral tracking; for RGB-equal media
 // this is plain analogue free-flight sampling).
+// A homogeneous medium, or (grid >= 0) a heterogeneous one whose sigma_t is a
+// density grid: sigma_s then holds the albedo, majorant the grid maximum.
 struct DevMedium {
     f3 sigma_s;
     float g;
@@ -18,6 +20,11 @@ struct DevMedium {
     float majorant;
     float p_scatter, p_absorb, p_null;
     int max_scatter;
+    int grid;              // first voxel in DevScene::density, -1 if homogeneous
+    int nx, ny, nz;
+    float w2l[12];         // world to grid coordinates, row-major 3x4
+    int white_indirect;    // albedo 1 after the first scatter of a run
+    int pad[3];
 };
 
 // ThinLensCamera (src/camera/thin_lens_camera.cpp:23-36) with look_at's
@@ -63,6 +70,9 @@ struct DevScene {
     const uchar4* tex8;
     const float4* texf;
     int has_tex;
+    // heterogeneous media: voxel pool of every density grid
+    const float* density;
+    int has_grid;
     f3 sun_dir;
     f3 sun_E;
     f3 wcenter;
@@ -171,6 +181,44 @@ TR_D f3 env_eval(const DevScene& s, f3 d, float& pdf) {
 
 // Texture2D::evaluate (wrap_repeat) with LinearSampler::Sample2D in floating
 // point; 8-bit texels read as value / 255.
+// Density of a grid medium at a world position: world to grid coordinates,
+// each wrapped into [0, 1] (Texture3D::wrap_repeat, src/core/texture.hpp:114-116),
+// then trilinear over the voxels (LinearSampler::Sample3D,
+// src/utility/sampler.hpp:47-67).
+TR_D float grid_density(const DevScene& s, const DevMedium& m, f3 p) {
+    const float* w = m.w2l;
+    const float lx = w[0] * p.x + w[1] * p.y + w[2] * p.z + w[3];
+    const float ly = w[4] * p.x + w[5] * p.y + w[6] * p.z + w[7];
+    const float lz = w[8] * p.x + w[9] * p.y + w[10] * p.z + w[11];
+    auto wrap = [](float x) { return fminf(fmaxf(x - floorf(x), 0.0f), 1.0f); };
+    const float u = wrap(lx) * float(m.nx - 1), v = wrap(ly) * float(m.ny - 1), k = wrap(lz) * float(m.nz - 1);
+    const int u0 = min(int(u), m.nx - 1), v0 = min(int(v), m.ny - 1), k0 = min(int(k), m.nz - 1);
+    const int u1 = min(u0 + 1, m.nx - 1), v1 = min(v0 + 1, m.ny - 1), k1 = min(k0 + 1, m.nz - 1);
+    const float du = u - float(u0), dv = v - float(v0), dk = k - float(k0);
+    const float* g = s.density + m.grid;
+    const size_t sy = size_t(m.nx), sz = size_t(m.nx) * size_t(m.ny);
+    auto at = [&](int x, int y, int z) { return __ldg(g + size_t(z) * sz + size_t(y) * sy + size_t(x)); };
+    const float c00 = at(u0, v0, k0) * (1.0f - du) + at(u1, v0, k0) * du;
+    const float c10 = at(u0, v1, k0) * (1.0f - du) + at(u1, v1, k0) * du;
+    const float c01 = at(u0, v0, k1) * (1.0f - du) + at(u1, v0, k1) * du;
+    const float c11 = at(u0, v1, k1) * (1.0f - du) + at(u1, v1, k1) * du;
+    return (c00 * (1.0f - dv) + c10 * dv) * (1.0f - dk) + (c01 * (1.0f - dv) + c11 * dv) * dk;
+}
+
+// Ratio-tracking transmittance of a grid medium over [0, dist) along d
+// (HeterogeneousMedium::tr, src/medium/heterogeneous.cpp:35-52).
+TR_D float grid_tr(const DevScene& s, const DevMedium& m, f3 o, f3 d, float dist, Pcg& rng) {
+    const float inv = 1.0f / m.majorant;
+    float T = 1.0f, t = 0.0f;
+    for (int k = 0; k < (1 << 16); ++k) {
+        t -= logf(1.0f - rng.uniform()) * inv;
+        if (t >= dist) break;
+        T *= 1.0f - grid_density(s, m, o + d * t) * inv;
+        if (T <= 0.0f) break;
+    }
+    return T;
+}
+
 TR_D f3 tex_lookup(const DevScene& s, int t, float u, float v) {
     const int4 ti = s.tex_info[t];
     const int w = ti.x, h = ti.y;

@@ -193,6 +193,57 @@ RC<Medium> create_homogeneous_medium(const Spectrum& sa, const Spectrum& ss, rea
     m->sigma_a = sa; m->sigma_s = ss; m->g = g; m->max_scattering_count = max_sc;
     return m;
 }
+// Texture3D::evaluate (texture.hpp:144-160): wrap_repeat on each coordinate,
+// then LinearSampler::Sample3D (sampler.hpp:47-67)
+Spectrum ImageTexture3D::evaluate(const Point3f& p) const {
+    const Image3D<real>& im = *image;
+    auto wrap = [](double x) { return std::clamp(x - std::floor(x), 0.0, 1.0); };
+    const int W = im.width(), H = im.height(), D = im.depth();
+    const double u = wrap(p.x) * (W - 1), v = wrap(p.y) * (H - 1), k = wrap(p.z) * (D - 1);
+    const int u0 = std::clamp(int(u), 0, W - 1), u1 = std::clamp(u0 + 1, 0, W - 1);
+    const int v0 = std::clamp(int(v), 0, H - 1), v1 = std::clamp(v0 + 1, 0, H - 1);
+    const int k0 = std::clamp(int(k), 0, D - 1), k1 = std::clamp(k0 + 1, 0, D - 1);
+    const double du = u - u0, dv = v - v0, dk = k - k0;
+    auto at = [&](int x, int y, int z) { return double(im.at(x, y, z)); };
+    const double r = ((at(u0, v0, k0) * (1 - du) + at(u1, v0, k0) * du) * (1 - dv) +
+                      (at(u0, v1, k0) * (1 - du) + at(u1, v1, k0) * du) * dv) * (1 - dk) +
+                     ((at(u0, v0, k1) * (1 - du) + at(u1, v0, k1) * du) * (1 - dv) +
+                      (at(u0, v1, k1) * (1 - du) + at(u1, v1, k1) * du) * dv) * dk;
+    return Spectrum(real(r));
+}
+RC<Texture3D> create_constant_texture3d(const Spectrum& c) {
+    auto t = newRC<ConstantTexture3D>();
+    t->value = c;
+    return t;
+}
+RC<Texture3D> create_image_texture3d(const RC<Image3D<real>>& image) {
+    if (!image || image->width() < 1 || image->height() < 1 || image->depth() < 1)
+        throw std::runtime_error("create_image_texture3d: empty image");
+    auto t = newRC<ImageTexture3D>();
+    t->image = image;
+    const size_t n = size_t(image->width()) * size_t(image->height()) * size_t(image->depth());
+    t->max_value = 0;
+    for (size_t i = 0; i < n; ++i) t->max_value = std::max(t->max_value, image->get_raw_data()[i]);
+    return t;
+}
+RC<Medium> create_heterogeneous_medium(const Transform& local_to_world, RC<const Texture3D> density,
+                                       RC<const Texture3D> albedo, RC<const Texture3D> g, int max_sc, bool white) {
+    if (!density || !albedo || !g) throw std::runtime_error("create_heterogeneous_medium: texture must not be null");
+    if (!albedo->is_constant() || !g->is_constant())
+        throw std::runtime_error("create_heterogeneous_medium: the GPU path takes a constant albedo and g");
+    if (!density->is_constant() && !dynamic_cast<const ImageTexture3D*>(density.get()))
+        throw std::runtime_error("create_heterogeneous_medium: density must come from create_image_texture3d");
+    const real gv = g->evaluate_s(Point3f(0, 0, 0));
+    if (!(gv > -1 && gv < 1)) throw std::runtime_error("create_heterogeneous_medium: g must lie in (-1, 1)");
+    auto m = newRC<HeterogeneousMedium>();
+    m->local_to_world = local_to_world;
+    m->density = std::move(density);
+    m->albedo = std::move(albedo);
+    m->g = std::move(g);
+    m->max_scattering_count = max_sc;
+    m->white_for_indirect = white;
+    return m;
+}
 RC<Medium> create_vacuum() {   // vacuum.cpp:26-29: one shared instance
     static RC<Medium> v = newRC<VacuumMedium>();
     return v;
@@ -402,9 +453,33 @@ FlatScene flatten(const Scene& scene) {
         if (dynamic_cast<const VacuumMedium*>(m.get())) return -1;
         auto it = med_id.find(m.get());
         if (it != med_id.end()) return it->second;
-        const auto* h = dynamic_cast<const HomogeneousMedium*>(m.get());
-        if (!h) throw std::runtime_error("render: only homogeneous media and vacuum are supported");
         tr_medium tm{};
+        if (const auto* hm = dynamic_cast<const HeterogeneousMedium*>(m.get())) {
+            const Spectrum a = hm->albedo->evaluate(Point3f(0, 0, 0));
+            tm.sigma_s[0] = a.r; tm.sigma_s[1] = a.g; tm.sigma_s[2] = a.b;
+            tm.g = hm->g->evaluate_s(Point3f(0, 0, 0));
+            tm.max_scattering_count = hm->max_scattering_count;
+            tm.white_for_indirect = hm->white_for_indirect ? 1 : 0;
+            // the flat scene keeps its own copy of the voxels (a constant density
+            // is a grid of one voxel)
+            if (const auto* it = dynamic_cast<const ImageTexture3D*>(hm->density.get())) {
+                const Image3D<real>& im = *it->image;
+                tm.nx = im.width(); tm.ny = im.height(); tm.nz = im.depth();
+                f.grids.emplace_back(im.get_raw_data(), im.get_raw_data() + size_t(tm.nx) * size_t(tm.ny) * size_t(tm.nz));
+            } else {
+                tm.nx = tm.ny = tm.nz = 1;
+                f.grids.emplace_back(1, hm->density->evaluate_s(Point3f(0, 0, 0)));
+            }
+            tm.density = f.grids.back().data();
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 4; ++j) tm.local_to_world[4 * i + j] = hm->local_to_world.m[i][j];
+            const int32_t id = int32_t(f.media.size());
+            f.media.push_back(tm);
+            med_id[m.get()] = id;
+            return id;
+        }
+        const auto* h = dynamic_cast<const HomogeneousMedium*>(m.get());
+        if (!h) throw std::runtime_error("render: only homogeneous and heterogeneous media and vacuum are supported");
         tm.sigma_a[0] = h->sigma_a.r; tm.sigma_a[1] = h->sigma_a.g; tm.sigma_a[2] = h->sigma_a.b;
         tm.sigma_s[0] = h->sigma_s.r; tm.sigma_s[1] = h->sigma_s.g; tm.sigma_s[2] = h->sigma_s.b;
         tm.g = h->g;

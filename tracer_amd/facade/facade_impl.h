@@ -1,6 +1,7 @@
 // Internal classes behind the tracer:: factory API and the scene flattener
 // that turns a tracer::Scene into a tr_scene_desc for libtracer_hip.so.
 #pragma once
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -73,6 +74,28 @@ public:
     int max_scattering_count = 0;
 };
 class VacuumMedium : public Medium {};
+class HeterogeneousMedium : public Medium {
+public:
+    Transform local_to_world;
+    RC<const Texture3D> density, albedo, g;
+    int max_scattering_count = 0;
+    bool white_for_indirect = false;
+};
+// Texture3D over a float volume (image_texture3d.cpp), or a constant
+class ImageTexture3D : public Texture3D {
+public:
+    RC<const Image3D<real>> image;
+    real max_value = 0;
+    Spectrum evaluate(const Point3f& uvw) const override;
+    real max_real() const override { return max_value; }
+};
+class ConstantTexture3D : public Texture3D {
+public:
+    Spectrum value;
+    Spectrum evaluate(const Point3f&) const override { return value; }
+    real max_real() const override { return value.r; }
+    bool is_constant() const override { return true; }
+};
 
 class DiffuseLight : public AreaLight {
 public:
@@ -142,6 +165,7 @@ struct FlatScene {
     std::vector<float> emission;
     std::vector<tr_material> mats;
     std::vector<tr_medium> media;
+    std::vector<std::vector<float>> grids;   // voxels of each heterogeneous medium, media point here
     int has_env = 0, has_sun = 0;
     float env[3] = {0, 0, 0}, sun_dir[3] = {0, 0, 1}, sun_E[3] = {0, 0, 0};
     int env_w = 0, env_h = 0;             // image environment

@@ -497,7 +497,16 @@ static void config5(Config& c, uint64_t seed, double detail) {
 // triangles get MediumInterface{inside = vacuum, outside = fog} with the fog
 // of main.cpp:242 (or vacuum outside when fog is false), and the BVH has 3
 // primitives per leaf.
-static void obj_scene(Config& c, const std::string& path, const tr_camera& tc, bool fog) {
+struct GridFog {   // a heterogeneous medium for obj_scene's outside
+    const float* density = nullptr;
+    int nx = 0, ny = 0, nz = 0;
+    float local_to_world[12];
+    float albedo[3];
+    float g = 0;
+    int max_scattering_count = 0;
+};
+
+static void obj_scene(Config& c, const std::string& path, const tr_camera& tc, bool fog, const GridFog* grid = nullptr) {
     const model_t model = load_model_from_file(path);
     std::vector<MaterialTexture> res;
     std::vector<RC<Material>> mats;
@@ -510,6 +519,19 @@ static void obj_scene(Config& c, const std::string& path, const tr_camera& tc, b
     Builder& b = c.builder;
     b.mi.inside = create_vacuum();
     b.mi.outside = fog ? create_homogeneous_medium(Spectrum(0.03f), Spectrum(0.3f), 0.8f, 25) : create_vacuum();
+    if (grid) {   // create_heterogeneous_medium over an Image3D<real> (heterogeneous.cpp:101-108)
+        auto im = newRC<Image3D<real>>(grid->nx, grid->ny, grid->nz);
+        std::copy(grid->density, grid->density + size_t(grid->nx) * size_t(grid->ny) * size_t(grid->nz),
+                  im->get_raw_data());
+        real m[4][4] = {};
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 4; ++j) m[i][j] = grid->local_to_world[4 * i + j];
+        m[3][3] = 1;
+        b.mi.outside = create_heterogeneous_medium(
+            Transform(m), create_image_texture3d(im),
+            create_constant_texture3d(Spectrum(grid->albedo[0], grid->albedo[1], grid->albedo[2])),
+            create_constant_texture3d(Spectrum(grid->g)), grid->max_scattering_count, false);
+    }
     for (const mesh_t& mesh : model.mesh) {
         auto tris = create_triangle_mesh(mesh, Transform());
         for (size_t i = 0; i < tris.size(); ++i) {
@@ -573,6 +595,30 @@ trs_scene* trs_load_obj(const char* path, const tr_camera* camera, int32_t outsi
         if (!path || !camera) throw std::runtime_error("trs_load_obj: path and camera must not be NULL");
         tracer::scenes::Config c;
         tracer::scenes::obj_scene(c, path, *camera, outside_fog != 0);
+        return make_trs(c, 0);
+    } catch (const std::exception& e) {
+        msg = e.what();
+        if (err) *err = msg.c_str();
+        return nullptr;
+    }
+}
+trs_scene* trs_load_obj_grid(const char* path, const tr_camera* camera, const float* density, int32_t nx, int32_t ny,
+                             int32_t nz, const float* local_to_world, const float* albedo, float g,
+                             int32_t max_scattering_count, const char** err) {
+    static thread_local std::string msg;
+    try {
+        if (!path || !camera || !density || !local_to_world || !albedo)
+            throw std::runtime_error("trs_load_obj_grid: pointers must not be NULL");
+        if (nx < 1 || ny < 1 || nz < 1) throw std::runtime_error("trs_load_obj_grid: grid dimensions must be positive");
+        tracer::scenes::GridFog gf;
+        gf.density = density;
+        gf.nx = nx; gf.ny = ny; gf.nz = nz;
+        std::copy(local_to_world, local_to_world + 12, gf.local_to_world);
+        std::copy(albedo, albedo + 3, gf.albedo);
+        gf.g = g;
+        gf.max_scattering_count = max_scattering_count;
+        tracer::scenes::Config c;
+        tracer::scenes::obj_scene(c, path, *camera, false, &gf);
         return make_trs(c, 0);
     } catch (const std::exception& e) {
         msg = e.what();

@@ -97,6 +97,10 @@ def scenes_lib() -> C.CDLL:
         lib.trs_create.restype = C.c_void_p
         lib.trs_load_obj.argtypes = [C.c_char_p, C.POINTER(abi.TrCamera), C.c_int32, C.POINTER(C.c_char_p)]
         lib.trs_load_obj.restype = C.c_void_p
+        lib.trs_load_obj_grid.argtypes = [C.c_char_p, C.POINTER(abi.TrCamera), C.POINTER(C.c_float), C.c_int32,
+                                          C.c_int32, C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                          C.c_float, C.c_int32, C.POINTER(C.c_char_p)]
+        lib.trs_load_obj_grid.restype = C.c_void_p
         lib.trs_read_hdr.argtypes = [C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_float),
                                      C.c_int64, C.POINTER(C.c_char_p)]
         lib.trs_read_hdr.restype = C.c_int32
@@ -248,6 +252,27 @@ def obj_scene(path, camera: abi.TrCamera, outside_fog: bool = True) -> Scene:
     return _scene_from_handle(lib, h)
 
 
+def obj_grid_scene(path, camera: abi.TrCamera, density, local_to_world, albedo=(0.9, 0.9, 0.9), g=0.0,
+                   max_scattering_count: int = 2 ** 31 - 1) -> Scene:
+    """obj_scene with a heterogeneous medium outside every surface, built by
+    the C++ facade through create_image_texture3d and
+    create_heterogeneous_medium; density is indexed [z, y, x] and
+    local_to_world (3x4 or 4x4) maps the unit grid cube into the world."""
+    lib = scenes_lib()
+    err = C.c_char_p()
+    grid = np.ascontiguousarray(density, np.float32)
+    if grid.ndim != 3:
+        raise ValueError("density must be a 3-D array indexed [z, y, x]")
+    l2w = np.ascontiguousarray(np.asarray(local_to_world, np.float64)[:3, :4], np.float32)
+    alb = np.ascontiguousarray(albedo, np.float32)
+    nz, ny, nx = grid.shape
+    h = lib.trs_load_obj_grid(str(path).encode(), C.byref(camera), fptr(grid), nx, ny, nz, fptr(l2w), fptr(alb),
+                              float(g), int(max_scattering_count), C.byref(err))
+    if not h:
+        raise TracerError(abi.TR_ERR_INVALID, (err.value or b"").decode())
+    return _scene_from_handle(lib, h)
+
+
 def read_hdr(path) -> np.ndarray:
     """A Radiance .hdr image as (height, width, 3) linear RGB, read by the
     C++ facade (the loader behind create_texture2d_from_file)."""
@@ -316,6 +341,11 @@ def _scene_from_handle(lib, h) -> Scene:
 
         mats = [TrMaterial.from_buffer_copy(d.materials[i]) for i in range(d.num_materials)]
         media = [TrMedium.from_buffer_copy(d.media[i]) for i in range(d.num_media)]
+        for m in media:   # copy each density grid out of the handle's memory
+            if m.density:
+                grid = np.ctypeslib.as_array(m.density, shape=(m.nz * m.ny * m.nx,)).reshape(m.nz, m.ny, m.nx).copy()
+                m.density = grid.ctypes.data_as(C.POINTER(C.c_float))
+                m._grid = grid
         defaults = TrRenderParams()
         lib.trs_render_defaults(h, C.byref(defaults))
         textures = []
