@@ -310,6 +310,16 @@ tr_status tr_film_to_ldr(tr_ctx* ctx, float aces_exposure, float inv_gamma, uint
  * unit directions (n*3) (:179-196).  TR_ERR_STATE without an environment. */
 tr_status tr_env_sample_batch(tr_ctx* ctx, const float* u, int64_t n, float* wi, float* pdf, float* L);
 tr_status tr_env_eval_batch(tr_ctx* ctx, const float* dirs, int64_t n, float* L, float* pdf);
+/* Lookups in the uploaded scene (after tr_bvh_build), for tests: the density
+ * of grid medium `medium` at world positions (n*3)
+ * (HeterogeneousMedium's density->evaluate_s(world_to_local(p)),
+ * src/medium/heterogeneous.cpp:45-47 with src/utility/sampler.hpp:47-67), and
+ * texture `texture` (1-based, as materials name it) at uv (n*2) -> rgb (n*3)
+ * (Texture2D::evaluate, src/core/texture.hpp:57-83, texels interpolated as
+ * floats).  TR_ERR_INVALID for a medium without a grid or a texture id out of
+ * range; TR_ERR_STATE when no material names a texture. */
+tr_status tr_medium_density_batch(tr_ctx* ctx, int32_t medium, const float* pos, int64_t n, float* density);
+tr_status tr_texture_eval_batch(tr_ctx* ctx, int32_t texture, const float* uv, int64_t n, float* rgb);
 
 /* ---- learned path guiding (no reference counterpart) ---------------------
  * MLP: 96 input features (81 used: position in the scene box, its sin/cos at

@@ -919,6 +919,42 @@ tr_status tr_env_eval_batch(tr_ctx* c, const float* dirs, int64_t n, float* L, f
     });
 }
 
+tr_status tr_medium_density_batch(tr_ctx* c, int32_t medium, const float* pos, int64_t n, float* density) {
+    return guarded(c, [&] {
+        if (!c->have_bvh) throw StateError("medium batch before tr_bvh_build");
+        if (medium < 0 || size_t(medium) >= c->hs.media.size() || c->hs.grids[size_t(medium)].empty())
+            throw InvalidArg("medium is not a density grid of the scene");
+        if (n < 0 || n > (int64_t(1) << 26)) throw InvalidArg("n must lie in [0, 2^26]");
+        if (!pos || !density) throw InvalidArg("NULL array");
+        if (n == 0) return;
+        upload(c->s_in, pos, size_t(n) * 12, c->stream);
+        c->s_out.reserve(size_t(n) * 4);
+        launch_grid_density(c->ds, medium, c->s_in.as<float>(), n, c->s_out.as<float>(), c->stream);
+        hip_check(hipGetLastError(), "k_grid_density launch");
+        hip_check(hipMemcpyAsync(density, c->s_out.as<float>(), size_t(n) * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+    });
+}
+
+tr_status tr_texture_eval_batch(tr_ctx* c, int32_t texture, const float* uv, int64_t n, float* rgb) {
+    return guarded(c, [&] {
+        if (!c->have_bvh) throw StateError("texture batch before tr_bvh_build");
+        if (texture < 1 || size_t(texture) > c->hs.tex.size()) throw InvalidArg("texture id out of range");
+        bool used = false;   // texture tables are uploaded only for scenes whose materials name one
+        for (const tr_material& m : c->hs.mats) used = used || m.base_color_texture > 0;
+        if (!used) throw StateError("no material of the scene names a texture");
+        if (n < 0 || n > (int64_t(1) << 26)) throw InvalidArg("n must lie in [0, 2^26]");
+        if (!uv || !rgb) throw InvalidArg("NULL array");
+        if (n == 0) return;
+        upload(c->s_in, uv, size_t(n) * 8, c->stream);
+        c->s_out.reserve(size_t(n) * 12);
+        launch_tex_eval(c->ds, texture, c->s_in.as<float>(), n, c->s_out.as<float>(), c->stream);
+        hip_check(hipGetLastError(), "k_tex_eval launch");
+        hip_check(hipMemcpyAsync(rgb, c->s_out.as<float>(), size_t(n) * 12, hipMemcpyDeviceToHost, c->stream), "D2H");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+    });
+}
+
 tr_status tr_bsdf_sample_batch(tr_ctx* c, const tr_material* mats, int32_t num_mats, const int32_t* ids, const float* wo,
                                const float* ng, const float* u, int64_t n, float* wi, float* f, float* pdf) {
     return guarded(c, [&] {

@@ -795,6 +795,25 @@ __global__ void k_env_eval(DevScene s, const float* d, int64_t n, float* L, floa
     L[3 * i] = e.x; L[3 * i + 1] = e.y; L[3 * i + 2] = e.z;
     pdf[i] = p;
 }
+__global__ void k_grid_density(DevScene s, int medium, const float* x, int64_t n, float* out) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = grid_density(s, s.media[medium], mk(x[3 * i], x[3 * i + 1], x[3 * i + 2]));
+}
+__global__ void k_tex_eval(DevScene s, int texture, const float* uv, int64_t n, float* out) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const f3 c = tex_lookup(s, texture - 1, uv[2 * i], uv[2 * i + 1]);
+    out[3 * i] = c.x; out[3 * i + 1] = c.y; out[3 * i + 2] = c.z;
+}
+void launch_grid_density(const DevScene& s, int medium, const float* x, int64_t n, float* out, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_grid_density, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, s, medium, x, n, out);
+}
+void launch_tex_eval(const DevScene& s, int texture, const float* uv, int64_t n, float* out, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_tex_eval, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, s, texture, uv, n, out);
+}
 void launch_env_sample(const DevScene& s, const float* u, int64_t n, float* wi, float* pdf, float* L, hipStream_t st) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_env_sample, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, s, u, n, wi, pdf, L);

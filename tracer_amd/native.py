@@ -68,6 +68,8 @@ def hip_lib() -> C.CDLL:
         lib.tr_env_sample_batch.argtypes = [VP, P(C.c_float), I64, P(C.c_float), P(C.c_float), P(C.c_float)]
         lib.tr_film_to_ldr.argtypes = [VP, C.c_float, C.c_float, P(C.c_uint8), I64]
         lib.tr_env_eval_batch.argtypes = [VP, P(C.c_float), I64, P(C.c_float), P(C.c_float)]
+        lib.tr_medium_density_batch.argtypes = [VP, C.c_int32, P(C.c_float), I64, P(C.c_float)]
+        lib.tr_texture_eval_batch.argtypes = [VP, C.c_int32, P(C.c_float), I64, P(C.c_float)]
         F = P(C.c_float)
         lib.tr_guide_default_config.argtypes = [P(abi.TrGuideConfig)]
         lib.tr_guide_default_config.restype = None
@@ -499,6 +501,20 @@ class Context:
         wi, pdf, L = np.zeros((n, 3), np.float32), np.zeros(n, np.float32), np.zeros((n, 3), np.float32)
         self._check(self.lib.tr_env_sample_batch(self.h, fptr(u), n, fptr(wi), fptr(pdf), fptr(L)))
         return wi, pdf, L
+
+    def medium_density(self, medium: int, pos: np.ndarray) -> np.ndarray:
+        """Density of grid medium `medium` of the loaded scene at world positions (n, 3)."""
+        x = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+        out = np.zeros(x.shape[0], np.float32)
+        self._check(self.lib.tr_medium_density_batch(self.h, int(medium), fptr(x), x.shape[0], fptr(out)))
+        return out
+
+    def texture_eval(self, texture: int, uv: np.ndarray) -> np.ndarray:
+        """Texture `texture` (1-based) of the loaded scene at uv (n, 2) -> (n, 3)."""
+        t = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+        out = np.zeros((t.shape[0], 3), np.float32)
+        self._check(self.lib.tr_texture_eval_batch(self.h, int(texture), fptr(t), t.shape[0], fptr(out)))
+        return out
 
     def env_eval(self, dirs: np.ndarray):
         """Radiance and light pdf of the environment at unit directions (n, 3)."""
