@@ -400,8 +400,9 @@ def main() -> int:
         print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4}))
         return 0
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (rendering shards by sample index "
-              "across GPUs, as DESIGN.md describes). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (rendering would shard by sample index "
+              "across GPUs, each rendering its own samples of every pixel into its own FP64 film). "
+              "Run with --gpus 1.", file=sys.stderr)
         return 2
 
     from tracer_amd import native
