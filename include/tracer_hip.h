@@ -179,7 +179,7 @@ typedef struct tr_render_params {
     int32_t max_depth;
     int32_t direct_light_sample_num;   /* light samples per vertex (reference default 1) */
     int32_t filter_type;               /* TR_FILTER_* (reference: Gaussian only) */
-    float filter_radius;               /* <= 1 */
+    float filter_radius;               /* pixels, in (0, 16] */
     float filter_alpha;
     int32_t shadow_transmittance;      /* TR_TR_* */
     uint64_t seed;
@@ -320,6 +320,43 @@ tr_status tr_env_eval_batch(tr_ctx* ctx, const float* dirs, int64_t n, float* L,
  * range; TR_ERR_STATE when no material names a texture. */
 tr_status tr_medium_density_batch(tr_ctx* ctx, int32_t medium, const float* pos, int64_t n, float* density);
 tr_status tr_texture_eval_batch(tr_ctx* ctx, int32_t texture, const float* uv, int64_t n, float* rgb);
+
+/* Test batches over the uploaded scene (after tr_bvh_build) through the
+ * renderer's own device code.
+ *
+ * tr_camera_ray_batch: ThinLensCamera::generate_ray
+ * (src/camera/thin_lens_camera.cpp:40-59) at film positions film_uv (n*2, in
+ * [0, 1]: pixel position / film size) with lens samples lens_uv (n*2) ->
+ * rays (n*8: origin, 0, unit direction, 3e38).
+ *
+ * tr_light_pmf: the light-selection pmf of next-event estimation
+ * (compute_light_power_distribution, src/renderer/direct_illumination.cpp:257-264,
+ * over DiffuseLight::power = pi L A and IBL::power = radiance integral * R^2,
+ * src/light/diffuse_light.cpp:17-20, src/light/environment_light.cpp:88-90):
+ * entries 0..num_area_lights-1 are the emissive triangles in scene order, then
+ * the environment, then the sun (0 when absent).  Call with pmf = NULL to get
+ * *n = num_area_lights + 2.
+ *
+ * tr_light_pick_batch: Distribution1D::sample_discrete
+ * (src/utility/distribution.hpp:49-67) of that pmf at uniforms u (n, in
+ * [0, 1)) -> light index (same numbering) and its pmf.
+ *
+ * tr_hg_sample_batch: HenyeyGreensteinPhaseFunction::sample
+ * (src/medium/phase_function.hpp:23-48) around wo (n*3) with uniforms u (n*2)
+ * -> wi (n*3) and pdf (n).  Needs no scene.
+ *
+ * tr_film_accumulate_batch: the film's Tile::add_sample + merge
+ * (src/core/render.hpp:37-57, 90-100) as tr_render applies it: radiance holds
+ * width * height * spp RGB samples, sample s of pixel (x, y) at index
+ * (s * height + y) * width + x, placed at the film position tr_render draws for
+ * sample sample_offset + s of that pixel (seed of params); params give the
+ * filter.  film_sums (W*H*4 doubles) is overwritten.  At most 2^26 samples. */
+tr_status tr_camera_ray_batch(tr_ctx* ctx, const float* film_uv, const float* lens_uv, int64_t n, float* rays);
+tr_status tr_light_pmf(tr_ctx* ctx, float* pmf, int64_t cap, int64_t* n);
+tr_status tr_light_pick_batch(tr_ctx* ctx, const float* u, int64_t n, int32_t* idx, float* pmf);
+tr_status tr_hg_sample_batch(tr_ctx* ctx, float g, const float* wo, const float* u, int64_t n, float* wi, float* pdf);
+tr_status tr_film_accumulate_batch(tr_ctx* ctx, const tr_render_params* params, const float* radiance,
+                                   double* film_sums);
 
 /* ---- learned path guiding (no reference counterpart) ---------------------
  * MLP: 96 input features (81 used: position in the scene box, its sin/cos at

@@ -36,6 +36,11 @@ def lib() -> C.CDLL:
         L.ref_bvh_export.restype = None
         L.ref_light_pmf.argtypes = [VP, P(C.c_float)]
         L.ref_light_pmf.restype = None
+        L.ref_light_pick.argtypes = [VP, P(C.c_float), I64, P(C.c_int32), P(C.c_float)]
+        L.ref_light_pick.restype = None
+        L.ref_film_splat.argtypes = [C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, P(C.c_float), P(C.c_float), I64,
+                                     P(C.c_double)]
+        L.ref_film_splat.restype = None
         for name in ("ref_trace_closest_mt", "ref_trace_closest_wt_brute", "ref_trace_closest_wt_bvh"):
             f = getattr(L, name)
             f.argtypes = [VP, P(C.c_float), P(C.c_float), I64, I32]
@@ -191,6 +196,14 @@ class RefScene:
         lib().ref_light_pmf(self.h, fptr(out))
         return out
 
+    def light_pick(self, u):
+        """Distribution1D::sample_discrete over the light powers: (index, pmf)."""
+        u = np.ascontiguousarray(u, np.float32).reshape(-1)
+        idx = np.zeros(u.size, np.int32)
+        pmf = np.zeros(u.size, np.float32)
+        lib().ref_light_pick(self.h, fptr(u), u.size, iptr(idx), fptr(pmf))
+        return idx, pmf
+
     def camera_rays(self, film_uv, lens_uv):
         film_uv = np.ascontiguousarray(film_uv, np.float32)
         lens_uv = np.ascontiguousarray(lens_uv, np.float32)
@@ -204,6 +217,16 @@ class RefScene:
         rays = (C.c_uint64 * 2)()
         ms = lib().ref_render(self.h, C.byref(params), int(quirks), int(nee), threads, dptr(sums), rays)
         return sums, ms, (rays[0], rays[1])
+
+
+def film_splat(width: int, height: int, filter_type: int, radius: float, alpha: float, pos, L) -> np.ndarray:
+    """Tile::add_sample of samples at film positions pos (n, 2, pixel units)
+    with radiance L (n, 3), in order, over a width x height film -> (H, W, 4)."""
+    pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 2)
+    L = np.ascontiguousarray(L, np.float32).reshape(-1, 3)
+    sums = np.zeros((height, width, 4), np.float64)
+    lib().ref_film_splat(width, height, filter_type, radius, alpha, fptr(pos), fptr(L), pos.shape[0], dptr(sums))
+    return sums
 
 
 def bsdf_eval(mats, ids, wi, wo, ng=None, precision=1, fixed=1):

@@ -427,6 +427,55 @@ static void make_bsdf(const Scene& s, const Isect& is, bool quirks, SurfBSDF& b)
 }
 
 // ----------------------------------------------------------------------------
+// Film filter and Tile::add_sample (src/core/render.hpp:37-57): a sample at
+// film position (x, y) in pixel units goes to every pixel (qx, qy) of the tile
+// with qx in [ceil(x - 0.5 - r), floor(x - 0.5 + r)] (same in y), weighted by
+// the filter at the offset from the pixel centre.  GaussianFilter::eval
+// (src/filter/gaussian_filter.cpp:20-30): prod over x and y of
+// max(0, exp(-alpha d^2) - exp(-alpha r^2)).  The box filter (no reference
+// counterpart) weights 1 inside |d| <= r; at r <= 0.5 it keeps a sample in its
+// own pixel.
+struct FilmFilter {
+    int type;
+    float r, alpha;
+    float eval(float dx, float dy) const {
+        if (type == TR_FILTER_BOX) return (std::abs(dx) <= r && std::abs(dy) <= r) ? 1.0f : 0.0f;
+        const float e = std::exp(-alpha * r * r);
+        auto g = [&](float x) { return std::max(0.0f, std::exp(-alpha * x * x) - e); };
+        return g(dx) * g(dy);
+    }
+    // Film::get_film_tile (render.hpp:79-88): pixel bounds [x0, x1) x [y0, y1)
+    // padded by the radius and clipped to the film.
+    void tile_bounds(int x0, int y0, int x1, int y1, int W, int H, int& px0, int& py0, int& px1, int& py1) const {
+        px0 = std::max(0, int(std::ceil(float(x0) - 0.5f - r)));
+        py0 = std::max(0, int(std::ceil(float(y0) - 0.5f - r)));
+        px1 = std::min(W, int(std::floor(float(x1) - 0.5f + r)) + 1);
+        py1 = std::min(H, int(std::floor(float(y1) - 0.5f + r)) + 1);
+    }
+    // Tile::add_sample into a tile buffer of (R, G, B, weight) doubles over
+    // [px0, px1) x [py0, py1).
+    void add_sample(float x, float y, float Lr, float Lg, float Lb, int px0, int py0, int px1, int py1,
+                    double* tile) const {
+        const int tw = px1 - px0;
+        if (type == TR_FILTER_BOX && r <= 0.5f) {
+            const int qx = int(std::floor(x)), qy = int(std::floor(y));
+            if (qx < px0 || qy < py0 || qx >= px1 || qy >= py1) return;
+            double* tp = &tile[(size_t(qy - py0) * tw + (qx - px0)) * 4];
+            tp[0] += Lr; tp[1] += Lg; tp[2] += Lb; tp[3] += 1.0;
+            return;
+        }
+        const float dxp = x - 0.5f, dyp = y - 0.5f;
+        const int qx0 = std::max(px0, int(std::ceil(dxp - r))), qx1 = std::min(px1, int(std::floor(dxp + r)) + 1);
+        const int qy0 = std::max(py0, int(std::ceil(dyp - r))), qy1 = std::min(py1, int(std::floor(dyp + r)) + 1);
+        for (int qy = qy0; qy < qy1; ++qy)
+            for (int qx = qx0; qx < qx1; ++qx) {
+                const float w = eval(x - (float(qx) + 0.5f), y - (float(qy) + 0.5f));
+                double* tp = &tile[(size_t(qy - py0) * tw + (qx - px0)) * 4];
+                tp[0] += double(Lr) * w; tp[1] += double(Lg) * w; tp[2] += double(Lb) * w; tp[3] += w;
+            }
+    }
+};
+
 struct RenderCfg {
     int W, H, spp, min_depth, max_depth, nls;
     int filter; float fr, fa;
@@ -758,13 +807,6 @@ struct Renderer {
         d = right * ld.x + nup * ld.y + dir * ld.z;
     }
 
-    float filter_w(float dx, float dy) const {
-        if (c.filter == TR_FILTER_BOX) return (std::abs(dx) <= c.fr && std::abs(dy) <= c.fr) ? 1.0f : 0.0f;
-        float e = std::exp(-c.fa * c.fr * c.fr);
-        auto g = [&](float x) { return std::max(0.0f, std::exp(-c.fa * x * x) - e); };
-        return g(dx) * g(dy);
-    }
-
     void render(double* sums, int threads) {
         const int W = c.W, H = c.H;
         const int TS = 16;
@@ -778,9 +820,10 @@ struct Renderer {
                 if (t >= tx * ty) break;
                 int x0 = (t % tx) * TS, y0 = (t / tx) * TS;
                 int x1 = std::min(W, x0 + TS), y1 = std::min(H, y0 + TS);
-                // tile padded by the filter radius (render.hpp:79-88)
-                int px0 = std::max(0, x0 - 1), py0 = std::max(0, y0 - 1);
-                int px1 = std::min(W, x1 + 1), py1 = std::min(H, y1 + 1);
+                // tile padded by the filter radius (Film::get_film_tile, render.hpp:79-88)
+                const FilmFilter flt{c.filter, c.fr, c.fa};
+                int px0 = 0, py0 = 0, px1 = 0, py1 = 0;
+                flt.tile_bounds(x0, y0, x1, y1, W, H, px0, py0, px1, py1);
                 int tw = px1 - px0;
                 tile.assign(size_t(tw) * (py1 - py0) * 4, 0.0);
                 for (int y = y0; y < y1; ++y)
@@ -796,20 +839,7 @@ struct Renderer {
                             camera_ray(pxf / W, pyf / H, lu, lv, o, d);
                             RGB<float> L = Li(o, d, sp);
                             if (!L.is_finite()) continue;
-                            if (c.filter == TR_FILTER_BOX && c.fr <= 0.5f) {
-                                double* tp = &tile[(size_t(y - py0) * tw + (x - px0)) * 4];
-                                tp[0] += L.r; tp[1] += L.g; tp[2] += L.b; tp[3] += 1.0;
-                                continue;
-                            }
-                            float dxp = pxf - 0.5f, dyp = pyf - 0.5f;
-                            int qx0 = std::max(px0, int(std::ceil(dxp - c.fr))), qx1 = std::min(px1, int(std::floor(dxp + c.fr)) + 1);
-                            int qy0 = std::max(py0, int(std::ceil(dyp - c.fr))), qy1 = std::min(py1, int(std::floor(dyp + c.fr)) + 1);
-                            for (int qy = qy0; qy < qy1; ++qy)
-                                for (int qx = qx0; qx < qx1; ++qx) {
-                                    float w = filter_w(pxf - (qx + 0.5f), pyf - (qy + 0.5f));
-                                    double* tp = &tile[(size_t(qy - py0) * tw + (qx - px0)) * 4];
-                                    tp[0] += double(L.r) * w; tp[1] += double(L.g) * w; tp[2] += double(L.b) * w; tp[3] += w;
-                                }
+                            flt.add_sample(pxf, pyf, L.r, L.g, L.b, px0, py0, px1, py1, tile.data());
                         }
                     }
                 std::lock_guard<std::mutex> lk(mtx);   // render.hpp:90-100
@@ -921,6 +951,36 @@ void ref_bvh_export(void* h, void* nodes_out, int32_t* prims_out) {
 void ref_light_pmf(void* h, float* pmf_out) {
     Scene* s = static_cast<Scene*>(h);
     std::memcpy(pmf_out, s->lpmf.data(), s->lpmf.size() * sizeof(float));
+}
+// Distribution1D (src/utility/distribution.hpp:12-29) over the light powers
+// and its sample_discrete (:49-67): the last entry of the running-sum table
+// cdf[0..n] (cdf[0] = 0, cdf[n] = 1) that is <= u, clamped to [0, n - 1];
+// pmf = func[i] / (func_int * n).  Light i < num_area is the scene's i-th
+// emissive triangle in triangle order, then the environment, then the sun.
+void ref_light_pick(void* h, const float* u, int64_t n, int32_t* idx, float* pmf) {
+    Scene* s = static_cast<Scene*>(h);
+    const int m = int(s->lpmf.size());
+    std::vector<float> func(s->lpmf), cdf(size_t(m) + 1, 0.0f);
+    for (int i = 0; i < m; ++i) cdf[size_t(i) + 1] = cdf[size_t(i)] + func[size_t(i)] / float(m);
+    const float func_int = cdf[size_t(m)];
+    for (int i = 1; i <= m; ++i) cdf[size_t(i)] = func_int > 0 ? cdf[size_t(i)] / func_int : float(i) / float(m);
+    for (int64_t k = 0; k < n; ++k) {
+        int off = m;   // last cdf entry <= u
+        while (off > 0 && !(cdf[size_t(off)] <= u[k])) --off;
+        off = std::min(std::max(off, 0), m - 1);
+        idx[k] = off;
+        pmf[k] = func_int > 0 ? func[size_t(off)] / (func_int * float(m)) : 0.0f;
+    }
+}
+
+// Tile::add_sample over a whole W x H film (one tile covering it, so only the
+// film bounds clip): samples at film positions pos (n*2, pixel units) with
+// radiance L (n*3), in order, accumulated onto sums (W*H*4 doubles).
+void ref_film_splat(int W, int H, int filter, float radius, float alpha, const float* pos, const float* L, int64_t n,
+                    double* sums) {
+    const FilmFilter f{filter, radius, alpha};
+    for (int64_t k = 0; k < n; ++k)
+        f.add_sample(pos[2 * k], pos[2 * k + 1], L[3 * k], L[3 * k + 1], L[3 * k + 2], 0, 0, W, H, sums);
 }
 
 // ---- ray queries --------------------------------------------------------

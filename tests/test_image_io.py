@@ -30,6 +30,44 @@ def test_hdr_round_trip(tmp_path, w, h):
     assert np.all(np.abs(back - img) <= step + 1e-12)
 
 
+def test_hdr_writer_clamps_what_rgbe_cannot_hold(tmp_path):
+    """Infinite, huge, NaN and negative channels: the largest RGBE value
+    (255/256 * 2^127, exponent byte 255) for the first two, 0 for the others."""
+    big = np.float32(np.ldexp(255.0 / 256.0, 127))
+    img = np.array([[[np.inf, 1.0, 0.0], [np.finfo(np.float32).max, 0.0, 0.0], [np.nan, 2.0, -1.0],
+                     [big, big, big]]], np.float32)
+    p = tmp_path / "c.hdr"
+    native.write_hdr(p, img)
+    back = native.read_hdr(p)
+    assert np.isfinite(back).all()
+    assert back[0, 0, 0] >= 0.99 * big and back[0, 1, 0] >= 0.99 * big and back[0, 3].min() >= 0.99 * big
+    assert back[0, 0, 1] <= 2.0 ** 120 and back[0, 2, 0] == 0.0 and back[0, 2, 2] == 0.0
+    assert abs(back[0, 2, 1] - 2.0) <= 2.0 / 128
+
+
+def test_scene_with_non_finite_vertices_is_rejected():
+    import ctypes as C
+    from tracer_amd import abi
+    s = native.config_scene(1)
+    lib = native.hip_lib()
+    lib.tr_bvh4_export.argtypes = [C.POINTER(abi.TrSceneDesc), C.POINTER(abi.TrBvhOpts), C.c_void_p, C.c_int64,
+                                   C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32)]
+    n, depth = C.c_int64(), C.c_int32()
+    opts = abi.TrBvhOpts(4, 1)
+    d = s.desc()
+    assert lib.tr_bvh4_export(C.byref(d), C.byref(opts), None, 0, None, C.byref(n), C.byref(depth)) == abi.TR_OK
+    for field, value in (("positions", np.nan), ("positions", np.inf), ("normals", np.nan)):
+        bad = native.config_scene(1)
+        arr = getattr(bad, field)
+        if arr is None:
+            bad.normals = np.zeros_like(bad.positions) + np.float32([0, 1, 0] * (bad.positions.size // 3))
+            arr = bad.normals
+        arr.reshape(-1)[7] = value
+        d = bad.desc()
+        assert lib.tr_bvh4_export(C.byref(d), C.byref(opts), None, 0, None, C.byref(n), C.byref(depth)) == \
+            abi.TR_ERR_INVALID, field
+
+
 def _png_chunks(data: bytes):
     assert data[:8] == bytes([137, 80, 78, 71, 13, 10, 26, 10])
     pos, out = 8, []

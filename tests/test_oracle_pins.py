@@ -1,0 +1,326 @@
+"""Pins of oracle functions that the BSDF, geometry and media tests leave
+open, each by mathematics that does not restate the oracle's code:
+
+* whole estimator (PathTraceRenderer::eval_pixel_li,
+  src/renderer/path_trace_renderer.cpp:27-210): a closed emitting Lambertian
+  box, whose image is Le * sum_{k<=D} rho^k, in both oracle modes and with both
+  light strategies; a depth off by one is rejected by the same statistic;
+* film filter (Tile::add_sample, src/core/render.hpp:37-57, GaussianFilter,
+  src/filter/gaussian_filter.cpp:9-35): one sample's footprint and weights
+  written out pixel by pixel, a constant image resolves to itself for every
+  radius, the box filter averages;
+* light-selection pmf (compute_light_power_distribution,
+  src/renderer/direct_illumination.cpp:257-264): pi L A per triangle and
+  L 4 pi R^2 for a constant environment, summed from the scene arrays in
+  float64, and the discrete sampler against numpy's searchsorted;
+* camera (ThinLensCamera, src/camera/thin_lens_camera.cpp:23-59): centre ray
+  along the view direction, edge rays at fov / 2 and atan(aspect tan(fov / 2)),
+  image top towards up and image left towards -right, thin-lens rays through
+  the focal plane point of their film position;
+* Henyey-Greenstein sampler (src/medium/phase_function.hpp:23-48): chi-square
+  of the scattering cosine against the closed-form CDF and of the azimuth
+  against uniform;
+* Disney lobe amplitudes (src/material/disney.cpp:146-209): the diffuse and
+  retro-reflection lobe equals Burley's 2012 form (a different expression of
+  the same model), the sheen lobe's Schlick weight and tint, and the
+  clearcoat lobe at normal incidence, 0.04 D_GTR1(1) / 4.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+from scipy import stats
+
+from tests.furnace import furnace_box, furnace_expected
+from tracer_amd import abi
+
+
+def _pixel_stats(sums):
+    img = sums[..., :3] / sums[..., 3:]
+    px = img.reshape(-1, 3)
+    return px.mean(0), px.std(0, ddof=1) / np.sqrt(px.shape[0])
+
+
+@pytest.mark.parametrize("quirks,nee", [(True, 0), (False, 0), (False, 1)])
+def test_furnace_closed_form(quirks, nee):
+    from oracle import refcpu
+    rho = (0.5, 0.6, 0.7)
+    s = furnace_box(rho=rho)
+    ref = refcpu.RefScene(s, 3)
+    for max_depth, min_depth in ((4, 10), (6, 1)):
+        p = s.render_params(width=32, height=32, spp=64, seed=3, max_depth=max_depth, min_depth=min_depth)
+        sums, _, _ = ref.render(p, quirks=quirks, nee=nee, threads=8)
+        m, se = _pixel_stats(sums)
+        z = (m - furnace_expected((1, 1, 1), rho, max_depth)) / se
+        assert np.abs(z).max() < 5.0, (max_depth, min_depth, m, z)
+        # the statistic has power: one bounce more or less is far outside it
+        for wrong in (max_depth - 1, max_depth + 1):
+            zw = (m - furnace_expected((1, 1, 1), rho, wrong)) / se
+            assert np.abs(zw).max() > 10.0
+
+
+def _gauss_w(d, r, a):
+    return max(0.0, np.exp(-a * d * d) - np.exp(-a * r * r))
+
+
+@pytest.mark.parametrize("r,alpha", [(0.5, 0.6), (1.0, 0.6), (1.5, 2.0), (2.5, 0.3)])
+def test_gaussian_splat_footprint(r, alpha):
+    from oracle import refcpu
+    W, H = 12, 10
+    x, y = 5.3, 6.85
+    L = np.array([[2.0, 3.0, 5.0]], np.float32)
+    sums = refcpu.film_splat(W, H, abi.TR_FILTER_GAUSSIAN, r, alpha, [[x, y]], L)
+    for qy in range(H):
+        for qx in range(W):
+            dx, dy = x - (qx + 0.5), y - (qy + 0.5)
+            inside = abs(dx) <= r and abs(dy) <= r
+            w = _gauss_w(dx, r, alpha) * _gauss_w(dy, r, alpha) if inside else 0.0
+            assert sums[qy, qx, 3] == pytest.approx(w, rel=1e-5, abs=1e-7)
+            assert np.allclose(sums[qy, qx, :3], L[0] * w, rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.parametrize("ftype,r", [(abi.TR_FILTER_GAUSSIAN, 0.5), (abi.TR_FILTER_GAUSSIAN, 1.7),
+                                     (abi.TR_FILTER_BOX, 0.5), (abi.TR_FILTER_BOX, 1.2)])
+def test_film_resolves_constant_radiance(ftype, r):
+    from oracle import refcpu
+    rng = np.random.default_rng(5)
+    W, H, spp = 9, 7, 16
+    pos = np.stack([np.repeat(np.arange(W * H) % W, spp) + rng.uniform(0, 1, W * H * spp),
+                    np.repeat(np.arange(W * H) // W, spp) + rng.uniform(0, 1, W * H * spp)], 1)
+    L = np.tile([[0.25, 1.5, 4.0]], (pos.shape[0], 1))
+    sums = refcpu.film_splat(W, H, ftype, r, 0.6, pos, L)
+    assert (sums[..., 3] > 0).all()
+    assert np.allclose(sums[..., :3] / sums[..., 3:], [0.25, 1.5, 4.0], rtol=1e-6)
+    if ftype == abi.TR_FILTER_BOX and r <= 0.5:
+        # every pixel holds exactly its own samples
+        assert np.array_equal(sums[..., 3], np.full((H, W), float(spp)))
+
+
+def _closed_form_pmf(scene):
+    P = scene.positions.reshape(-1, 3).astype(np.float64)
+    I = scene.indices.reshape(-1, 3)
+    E = scene.emission.reshape(-1, 3).astype(np.float64)
+    lum = lambda c: 0.2126 * c[..., 0] + 0.7152 * c[..., 1] + 0.0722 * c[..., 2]
+    tri = P[I]
+    area = 0.5 * np.linalg.norm(np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0]), axis=1)
+    em = (E != 0).any(1)
+    powers = list(np.pi * lum(E[em]) * area[em])
+    used = P[np.unique(I)]
+    lo, hi = used.min(0), used.max(0)
+    R = np.linalg.norm(hi - 0.5 * (lo + hi))
+    powers.append(lum(np.asarray(scene.env, np.float64)) * 4 * np.pi * R * R if scene.env is not None else 0.0)
+    powers.append(lum(np.asarray(scene.sun[1], np.float64)) * np.pi * R * R if scene.sun is not None else 0.0)
+    powers = np.array(powers)
+    return powers / powers.sum()
+
+
+@pytest.mark.parametrize("config,detail", [(1, 1.0), (2, 1 / 16), (5, 1 / 2048)])
+def test_light_pmf_closed_form(config, detail):
+    from oracle import refcpu
+    from tracer_amd import native
+    s = native.config_scene(config, detail=detail)
+    expect = _closed_form_pmf(s)
+    ref = refcpu.RefScene(s, 3)
+    pmf = ref.light_pmf(len(expect) - 2)
+    assert np.allclose(pmf, expect, rtol=2e-6, atol=1e-9)
+    # Distribution1D::sample_discrete = the first running sum above u
+    u = np.random.default_rng(2).uniform(0, 1, 20000).astype(np.float32)
+    cdf = np.cumsum(expect)
+    keep = np.min(np.abs(u[:, None] - cdf[None, :]), axis=1) > 1e-5
+    idx, p = ref.light_pick(u)
+    want = np.searchsorted(cdf, u.astype(np.float64), side="right")
+    assert np.array_equal(idx[keep], want[keep])
+    assert np.allclose(p, expect[idx], rtol=2e-6)
+
+
+def _camera_scene(pos, target, up, fov_deg, aspect, lens_radius=0.0, focal=1.0):
+    s = furnace_box(half=50.0)
+    cam = abi.TrCamera()
+    cam.pos[:], cam.target[:], cam.up[:] = pos, target, up
+    cam.fov_y = np.deg2rad(fov_deg)
+    cam.aspect = aspect
+    cam.lens_radius = lens_radius
+    cam.focal_distance = focal
+    s.camera = cam
+    return s
+
+
+def test_camera_closed_forms():
+    from oracle import refcpu
+    fov, aspect = 40.0, 2.0
+    ref = refcpu.RefScene(_camera_scene((0, 0, 5), (0, 0, 0), (0, 1, 0), fov, aspect), 3)
+    film = np.array([[0.5, 0.5], [0.5, 0.0], [0.5, 1.0], [0.0, 0.5], [1.0, 0.5], [0.0, 0.0]], np.float32)
+    rays = ref.camera_rays(film, np.full_like(film, 0.5))
+    d = rays[:, 4:7].astype(np.float64)
+    view = np.array([0.0, 0.0, -1.0])
+    assert np.allclose(rays[:, :3], [0, 0, 5], atol=1e-6)
+    assert np.allclose(d[0], view, atol=1e-6)
+    ang = np.degrees(np.arccos(np.clip(d @ view, -1, 1)))
+    half_h = np.degrees(np.arctan(aspect * np.tan(np.radians(fov / 2))))
+    assert ang[1] == pytest.approx(fov / 2, abs=1e-4) and ang[2] == pytest.approx(fov / 2, abs=1e-4)
+    assert ang[3] == pytest.approx(half_h, abs=1e-4) and ang[4] == pytest.approx(half_h, abs=1e-4)
+    # row 0 is the top of the image (towards up); column 0 looks to the viewer's left,
+    # which is world -x for a camera looking down -z with y up
+    assert d[1, 1] > 0 and d[2, 1] < 0 and d[3, 0] < 0 and d[4, 0] > 0
+    corner = np.array([-aspect * np.tan(np.radians(fov / 2)), np.tan(np.radians(fov / 2)), -1.0])
+    assert np.allclose(d[5], corner / np.linalg.norm(corner), atol=1e-6)
+
+
+def test_thin_lens_rays_meet_on_the_focal_plane():
+    from oracle import refcpu
+    focal, radius = 4.0, 0.3
+    ref = refcpu.RefScene(_camera_scene((1, 2, 6), (0, 0, 0), (0, 1, 0), 50.0, 1.0, radius, focal), 3)
+    rng = np.random.default_rng(1)
+    view = -np.array([1.0, 2.0, 6.0]) / np.linalg.norm([1, 2, 6])
+    for fx, fy in ((0.5, 0.5), (0.1, 0.8), (0.9, 0.3)):
+        lens = rng.uniform(0, 1, (64, 2)).astype(np.float32)
+        rays = ref.camera_rays(np.tile([[fx, fy]], (64, 1)), lens).astype(np.float64)
+        o, d = rays[:, :3], rays[:, 4:7]
+        spread = np.linalg.norm(o - [1, 2, 6], axis=1)
+        assert spread.max() <= radius * (1 + 1e-5) and spread.max() > 0.5 * radius
+        p = o + d * (focal / (d @ view))[:, None]
+        assert np.abs(p - p.mean(0)).max() < 2e-5 * focal
+
+
+def _hg_samples(g, n, seed):
+    from oracle import refcpu
+    L = refcpu.lib()
+    rng = np.random.default_rng(seed)
+    wo = rng.normal(size=(n, 3))
+    wo = (wo / np.linalg.norm(wo, axis=1, keepdims=True)).astype(np.float32)
+    u = rng.uniform(0, 1, (n, 2)).astype(np.float32)
+    wi = np.zeros_like(wo)
+    pdf = np.zeros(n, np.float32)
+    F = C.POINTER(C.c_float)
+    L.ref_hg_sample(float(g), wo.ctypes.data_as(F), u.ctypes.data_as(F), n, wi.ctypes.data_as(F),
+                    pdf.ctypes.data_as(F))
+    return wo, u, wi, pdf
+
+
+def hg_cdf(g, x):
+    """P(-cos(wi, wo) <= x) for the Henyey-Greenstein density in the scattering cosine."""
+    if abs(g) < 1e-6:
+        return 0.5 * (x + 1)
+    return (1 - g * g) / (2 * g) * (1 / np.sqrt(1 + g * g - 2 * g * x) - 1 / (1 + g))
+
+
+def hg_chi2(g, wo, wi, bins=40):
+    """p-values of the scattering cosine against hg_cdf and of the azimuth against uniform."""
+    wo, wi = wo.astype(np.float64), wi.astype(np.float64)
+    mu = -np.einsum("ij,ij->i", wi, wo)
+    edges = np.linspace(-1, 1, bins + 1)
+    expect = np.diff(hg_cdf(g, edges)) * len(mu)
+    obs, _ = np.histogram(mu, edges)
+    ok = expect > 5
+    p_mu = stats.chisquare(obs[ok], expect[ok] * obs[ok].sum() / expect[ok].sum()).pvalue
+    # azimuth around wo, measured from a fixed reference axis
+    a = np.where(np.abs(wo[:, 0:1]) < 0.9, [[1.0, 0, 0]], [[0, 1.0, 0]])
+    t = a - wo * np.einsum("ij,ij->i", a, wo)[:, None]
+    t /= np.linalg.norm(t, axis=1, keepdims=True)
+    b = np.cross(wo, t)
+    phi = np.arctan2(np.einsum("ij,ij->i", wi, b), np.einsum("ij,ij->i", wi, t))
+    obs_phi, _ = np.histogram(phi, np.linspace(-np.pi, np.pi, 33))
+    p_phi = stats.chisquare(obs_phi).pvalue
+    return p_mu, p_phi
+
+
+@pytest.mark.parametrize("g", [0.7, -0.4, 0.0, 0.95])
+def test_hg_sampler_chi_square(g):
+    from oracle import refcpu
+    wo, u, wi, pdf = _hg_samples(g, 200000, 7)
+    p_mu, p_phi = hg_chi2(g, wo, wi)
+    assert p_mu > 1e-4 and p_phi > 1e-4
+    cos_io = np.einsum("ij,ij->i", wi.astype(np.float64), wo)
+    want = np.array([refcpu.lib().ref_hg_eval(g, c) for c in cos_io[:500]])
+    # float32 pdf of the sampler against the float64 phase function; the denominator
+    # 1 + g^2 + 2 g cos cancels near the forward peak, which scales the rounding error
+    dem = 1 + g * g + 2 * g * cos_io[:500]
+    tol = 1e-6 + 1e-6 * (1 + g * g + 2 * abs(g)) / dem
+    assert (np.abs(pdf[:500] / want - 1) < tol).all()
+    # a sampler with the sign of the scattering cosine flipped fails the same test
+    if abs(g) > 0.3:
+        flipped = wi - 2 * cos_io[:, None] * wo
+        assert hg_chi2(g, wo, flipped)[0] < 1e-6
+
+
+def _disney(**kw):
+    base = dict(type=abi.TR_MAT_DISNEY, base_color=(0.0, 0.0, 0.0), metallic=0.0, roughness=0.5, transmission=0.0,
+                transmission_roughness=0.5, ior=1.5, specular_scale=0.0, specular_tint=0.0, anisotropic=0.0,
+                sheen=0.0, sheen_tint=0.0, clearcoat=0.0, clearcoat_gloss=0.0)
+    base.update(kw)
+    return abi.make_material(**base)
+
+
+def _dirs(theta_i, theta_o, phi):
+    wi = np.stack([np.sin(theta_i), np.zeros_like(theta_i), np.cos(theta_i)], 1)
+    wo = np.stack([np.sin(theta_o) * np.cos(phi), np.sin(theta_o) * np.sin(phi), np.cos(theta_o)], 1)
+    return wi.astype(np.float32), wo.astype(np.float32)
+
+
+def _eval(mat, wi, wo):
+    from oracle import refcpu
+    f, _ = refcpu.bsdf_eval([mat], np.zeros(len(wi), np.int32), wi, wo, precision=1, fixed=1)
+    return f
+
+
+def test_disney_diffuse_equals_burley_2012():
+    """Burley 2012: f_d = C / pi (1 + (F_D90 - 1)(1 - cos_l)^5)(1 + (F_D90 - 1)(1 - cos_v)^5),
+    F_D90 = 1/2 + 2 roughness cos^2(theta_d); the reference's lambert-and-retro split is the same model."""
+    rng = np.random.default_rng(3)
+    n = 4000
+    wi, wo = _dirs(rng.uniform(0.05, 1.5, n), rng.uniform(0.05, 1.5, n), rng.uniform(0, 2 * np.pi, n))
+    for rough in (0.1, 0.5, 1.0):
+        C3 = np.array([0.8, 0.4, 0.2])
+        f = _eval(_disney(base_color=tuple(C3), roughness=rough), wi, wo)
+        wi64, wo64 = wi.astype(np.float64), wo.astype(np.float64)
+        h = wi64 + wo64
+        h /= np.linalg.norm(h, axis=1, keepdims=True)
+        cd = np.einsum("ij,ij->i", wi64, h)
+        fd90 = 0.5 + 2 * rough * cd * cd
+        burley = (1 + (fd90 - 1) * (1 - wi64[:, 2]) ** 5) * (1 + (fd90 - 1) * (1 - wo64[:, 2]) ** 5) / np.pi
+        assert np.allclose(f, burley[:, None] * C3, rtol=1e-5, atol=1e-9)
+    # at normal incidence the lobe is Lambert's
+    f = _eval(_disney(base_color=(0.5, 0.5, 0.5), roughness=0.9), *(np.array([[0, 0, 1.0]], np.float32),) * 2)
+    assert np.allclose(f, 0.5 / np.pi, rtol=1e-6)
+
+
+def test_disney_sheen_lobe():
+    """Sheen: 4 sheen mix(1, C / lum(C), sheen_tint) (1 - cos theta_d)^5 (disney.cpp:156-159),
+    scaled by (1 - metallic)(1 - transmission): zero when wi = wo, largest towards grazing
+    opposite directions, tinted towards the base colour's hue."""
+    rng = np.random.default_rng(4)
+    n = 3000
+    wi, wo = _dirs(rng.uniform(0.05, 1.5, n), rng.uniform(0.05, 1.5, n), rng.uniform(0, 2 * np.pi, n))
+    C3 = np.array([0.6, 0.3, 0.1])
+    tint = C3 / (0.2126 * C3[0] + 0.7152 * C3[1] + 0.0722 * C3[2])
+    for sheen, st in ((0.5, 0.0), (0.3, 1.0), (0.4, 0.5)):
+        diff = _eval(_disney(base_color=tuple(C3), sheen=sheen, sheen_tint=st), wi, wo) - \
+            _eval(_disney(base_color=tuple(C3), sheen=0.0), wi, wo)
+        h = wi.astype(np.float64) + wo
+        h /= np.linalg.norm(h, axis=1, keepdims=True)
+        cd = np.einsum("ij,ij->i", wi.astype(np.float64), h)
+        want = 4 * sheen * ((1 - st) + st * tint)[None, :] * ((1 - cd) ** 5)[:, None]
+        assert np.allclose(diff, want, rtol=1e-4, atol=1e-7)
+    same = np.array([[0.3, 0.2, 0.93]], np.float32)
+    same /= np.linalg.norm(same)
+    d = _eval(_disney(base_color=tuple(C3), sheen=0.5), same, same) - _eval(_disney(base_color=tuple(C3)), same, same)
+    assert np.abs(d).max() < 1e-7
+    # metallic 0.75 keeps a quarter of it
+    a = _eval(_disney(base_color=tuple(C3), sheen=0.5, metallic=0.75), wi[:50], wo[:50]) - \
+        _eval(_disney(base_color=tuple(C3), metallic=0.75), wi[:50], wo[:50])
+    b = _eval(_disney(base_color=tuple(C3), sheen=0.5), wi[:50], wo[:50]) - _eval(_disney(base_color=tuple(C3)), wi[:50],
+                                                                                  wo[:50])
+    assert np.allclose(a, 0.25 * b, rtol=1e-5, atol=1e-9)
+
+
+@pytest.mark.parametrize("gloss", [0.0, 0.5, 1.0])
+def test_disney_clearcoat_normal_incidence(gloss):
+    """Clearcoat alone (black base, no specular) at wi = wo = n: Fresnel 0.04, Smith G = 1 and
+    f = clearcoat 0.04 D / 4 with Burley's GTR1 D(1) = (a^2 - 1) / (pi ln(a^2) a^2),
+    a = max(1e-4, ((1 - gloss) 0.1)^2) (disney.cpp:198-209, 387-389; microfacet.cpp:29-35)."""
+    cc = 0.8
+    z = np.array([[0, 0, 1.0]], np.float32)
+    f = _eval(_disney(clearcoat=cc, clearcoat_gloss=gloss), z, z)
+    a = max(1e-4, ((1 - gloss) * 0.1) ** 2)
+    D = (a * a - 1) / (np.pi * np.log(a * a) * a * a)
+    assert np.allclose(f, cc * 0.04 * D / 4, rtol=1e-5)

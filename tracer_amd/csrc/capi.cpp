@@ -231,6 +231,19 @@ void validate_scene(const tr_scene_desc* d) {
                     throw InvalidArg("env_world_to_light must be a rotation");
             }
     }
+    // non-finite vertex data would reach the BVH build (bin index of a NaN
+    // centroid) and the shading frames
+    for (int64_t i = 0; i < 3 * d->num_vertices; ++i)
+        if (!std::isfinite(d->positions[i])) throw InvalidArg("vertex positions must be finite");
+    if (d->normals)
+        for (int64_t i = 0; i < 3 * d->num_vertices; ++i)
+            if (!std::isfinite(d->normals[i])) throw InvalidArg("vertex normals must be finite");
+    if (d->uvs)
+        for (int64_t i = 0; i < 2 * d->num_vertices; ++i)
+            if (!std::isfinite(d->uvs[i])) throw InvalidArg("vertex uvs must be finite");
+    if (d->emission)
+        for (int64_t i = 0; i < 3 * d->num_triangles; ++i)
+            if (!(d->emission[i] >= 0.0f && d->emission[i] < 3.0e38f)) throw InvalidArg("emission must be finite and >= 0");
     for (int64_t t = 0; t < d->num_triangles; ++t) {
         for (int k = 0; k < 3; ++k) {
             const int32_t v = d->indices[3 * t + k];
@@ -317,21 +330,20 @@ void build_device_scene(tr_ctx* c, const tr_bvh_opts* opts) {
     c->bvh_depth = r.depth;
     c->max_stack = 3 * r.depth + 4;
 
-    // leaf-ordered shading records and light table
+    // leaf-ordered shading records; the light table lists the emissive
+    // triangles in scene order, as the reference's scene.lights does, so that
+    // the power distribution over it samples the same light for the same u
     std::vector<float4> shade(size_t(nt) * 3);
     std::vector<int> tri_to_light(size_t(nt), -1);
-    std::vector<int> light_tris;   // leaf-ordered index of each emissive triangle
-    std::vector<float4> lights;
-    for (int64_t i = 0; i < nt; ++i) {
-        const int prim = r.tris[size_t(i)].prim;
-        const bool em = !h.emission.empty() &&
-                        (h.emission[3 * prim] != 0.0f || h.emission[3 * prim + 1] != 0.0f || h.emission[3 * prim + 2] != 0.0f);
-        if (em) {
-            tri_to_light[size_t(i)] = int(light_tris.size());
-            light_tris.push_back(int(i));
-        }
+    int nA = 0;
+    {
+        std::vector<int> leaf_of(size_t(nt), 0);
+        for (int64_t i = 0; i < nt; ++i) leaf_of[size_t(r.tris[size_t(i)].prim)] = int(i);
+        for (int64_t prim = 0; prim < nt && !h.emission.empty(); ++prim)
+            if (h.emission[3 * prim] != 0.0f || h.emission[3 * prim + 1] != 0.0f || h.emission[3 * prim + 2] != 0.0f)
+                tri_to_light[size_t(leaf_of[size_t(prim)])] = nA++;
     }
-    const int nA = int(light_tris.size());
+    std::vector<float4> lights(size_t(nA) * 4);
     std::vector<double> power(size_t(nA) + 2, 0.0);
     // world bounding sphere (Bounds3::bounding_sphere, geometry.hpp:752-755)
     const f3 wc = mk(0.5f * (h.lo[0] + h.hi[0]), 0.5f * (h.lo[1] + h.hi[1]), 0.5f * (h.lo[2] + h.hi[2]));
@@ -365,10 +377,10 @@ void build_device_scene(tr_ctx* c, const tr_bvh_opts* opts) {
             const double area = 0.5 * std::sqrt(double(cr.x) * cr.x + double(cr.y) * cr.y + double(cr.z) * cr.z);
             // (L, area), (A, pmf), (B, triangle), (C, 0): the vertices travel with
             // the light so that sampling it needs no second dependent load
-            lights.push_back(f4(L, float(area)));
-            lights.push_back(f4(A, 0.0f));
-            lights.push_back(f4(B, __builtin_bit_cast(float, int32_t(i))));
-            lights.push_back(f4(C, 0.0f));
+            lights[4 * size_t(li)] = f4(L, float(area));
+            lights[4 * size_t(li) + 1] = f4(A, 0.0f);
+            lights[4 * size_t(li) + 2] = f4(B, __builtin_bit_cast(float, int32_t(i)));
+            lights[4 * size_t(li) + 3] = f4(C, 0.0f);
             power[size_t(li)] = double(kPi) * lum(L) * area;   // DiffuseLight::power, diffuse_light.cpp:17-20
         }
     }
@@ -575,6 +587,18 @@ DevPaths make_paths(tr_ctx* c) {
     p.ovf_t = c->ovf_t.as<float>();
     p.ovf_threads = c->ovf_threads;
     return p;
+}
+
+// Film filter of a render: box or GaussianFilter (src/filter/gaussian_filter.cpp)
+// of any radius up to kMaxFilterRadius pixels (the gather of k_accumulate
+// visits (2 floor(r + 1/2) + 1)^2 neighbours per sample).
+constexpr float kMaxFilterRadius = 16.0f;
+void check_filter(const tr_render_params& P) {
+    if (P.filter_type != TR_FILTER_BOX && P.filter_type != TR_FILTER_GAUSSIAN) throw InvalidArg("unknown filter");
+    if (!(P.filter_radius > 0.0f && P.filter_radius <= kMaxFilterRadius))
+        throw InvalidArg("filter radius must lie in (0, 16] pixels");
+    if (P.filter_type == TR_FILTER_GAUSSIAN && !(P.filter_alpha >= 0.0f && P.filter_alpha < 1e30f))
+        throw InvalidArg("Gaussian filter alpha must be finite and >= 0");
 }
 
 hipEvent_t ev(tr_ctx* c, size_t i) {
@@ -981,6 +1005,108 @@ tr_status tr_bsdf_sample_batch(tr_ctx* c, const tr_material* mats, int32_t num_m
     });
 }
 
+tr_status tr_camera_ray_batch(tr_ctx* c, const float* film_uv, const float* lens_uv, int64_t n, float* rays) {
+    return guarded(c, [&] {
+        if (!c->have_bvh) throw StateError("camera batch before tr_bvh_build");
+        if (n < 0 || n > (int64_t(1) << 26)) throw InvalidArg("n must lie in [0, 2^26]");
+        if (!film_uv || !lens_uv || !rays) throw InvalidArg("NULL array");
+        if (n == 0) return;
+        c->s_in.reserve(size_t(n) * 16);
+        float* in = c->s_in.as<float>();
+        hip_check(hipMemcpyAsync(in, film_uv, size_t(n) * 8, hipMemcpyHostToDevice, c->stream), "H2D");
+        hip_check(hipMemcpyAsync(in + 2 * n, lens_uv, size_t(n) * 8, hipMemcpyHostToDevice, c->stream), "H2D");
+        c->s_out.reserve(size_t(n) * 32);
+        launch_camera_rays(c->ds, in, in + 2 * n, n, c->s_out.as<float4>(), c->stream);
+        hip_check(hipGetLastError(), "k_camera_rays launch");
+        hip_check(hipMemcpyAsync(rays, c->s_out.ptr, size_t(n) * 32, hipMemcpyDeviceToHost, c->stream), "D2H");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+    });
+}
+
+tr_status tr_light_pmf(tr_ctx* c, float* pmf, int64_t cap, int64_t* n) {
+    return guarded(c, [&] {
+        if (!c->have_bvh) throw StateError("tr_light_pmf before tr_bvh_build");
+        if (!n) throw InvalidArg("n is NULL");
+        const int nA = c->ds.num_area_lights;
+        *n = nA + 2;
+        if (!pmf) return;
+        if (cap < nA + 2) throw InvalidArg("pmf holds fewer than num_area_lights + 2 entries");
+        hip_check(hipMemcpy(pmf, c->light_pmf.ptr, size_t(nA + 2) * sizeof(float), hipMemcpyDeviceToHost), "D2H");
+    });
+}
+
+tr_status tr_light_pick_batch(tr_ctx* c, const float* u, int64_t n, int32_t* idx, float* pmf) {
+    return guarded(c, [&] {
+        if (!c->have_bvh) throw StateError("light batch before tr_bvh_build");
+        if (n < 0 || n > (int64_t(1) << 26)) throw InvalidArg("n must lie in [0, 2^26]");
+        if (!u || !idx || !pmf) throw InvalidArg("NULL array");
+        if (c->ds.num_light_entries == 0) throw StateError("the scene has no light");
+        if (n == 0) return;
+        for (int64_t i = 0; i < n; ++i)
+            if (!(u[i] >= 0.0f && u[i] < 1.0f)) throw InvalidArg("uniforms must lie in [0, 1)");
+        upload(c->s_in, u, size_t(n) * 4, c->stream);
+        c->s_out.reserve(size_t(n) * 8);
+        int* di = c->s_out.as<int>();
+        float* dp = reinterpret_cast<float*>(di + n);
+        launch_light_pick(c->ds, c->s_in.as<float>(), n, di, dp, c->stream);
+        hip_check(hipGetLastError(), "k_light_pick launch");
+        hip_check(hipMemcpyAsync(idx, di, size_t(n) * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+        hip_check(hipMemcpyAsync(pmf, dp, size_t(n) * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+    });
+}
+
+tr_status tr_hg_sample_batch(tr_ctx* c, float g, const float* wo, const float* u, int64_t n, float* wi, float* pdf) {
+    return guarded(c, [&] {
+        if (!(g > -1.0f && g < 1.0f)) throw InvalidArg("g must lie in (-1, 1)");
+        if (n < 0 || n > (int64_t(1) << 26)) throw InvalidArg("n must lie in [0, 2^26]");
+        if (!wo || !u || !wi || !pdf) throw InvalidArg("NULL array");
+        if (n == 0) return;
+        c->s_in.reserve(size_t(n) * 20);
+        float* in = c->s_in.as<float>();
+        hip_check(hipMemcpyAsync(in, wo, size_t(n) * 12, hipMemcpyHostToDevice, c->stream), "H2D");
+        hip_check(hipMemcpyAsync(in + 3 * n, u, size_t(n) * 8, hipMemcpyHostToDevice, c->stream), "H2D");
+        c->s_out.reserve(size_t(n) * 16);
+        float* out = c->s_out.as<float>();
+        launch_hg_sample(g, in, in + 3 * n, n, out, out + 3 * n, c->stream);
+        hip_check(hipGetLastError(), "k_hg_sample launch");
+        hip_check(hipMemcpyAsync(wi, out, size_t(n) * 12, hipMemcpyDeviceToHost, c->stream), "D2H");
+        hip_check(hipMemcpyAsync(pdf, out + 3 * n, size_t(n) * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+    });
+}
+
+tr_status tr_film_accumulate_batch(tr_ctx* c, const tr_render_params* prm, const float* radiance, double* film_sums) {
+    return guarded(c, [&] {
+        if (!prm || !radiance || !film_sums) throw InvalidArg("NULL argument");
+        const tr_render_params& P = *prm;
+        if (P.width <= 0 || P.height <= 0 || P.spp <= 0) throw InvalidArg("width, height and spp must be positive");
+        check_filter(P);
+        const int64_t npix = int64_t(P.width) * P.height;
+        const int64_t n = npix * P.spp;
+        if (n > (int64_t(1) << 26)) throw InvalidArg("width * height * spp must be at most 2^26");
+        ensure_paths(c, n);
+        upload(c->s_in, radiance, size_t(n) * 12, c->stream);
+        DevPaths p = make_paths(c);
+        launch_unpack_radiance(c->s_in.as<float>(), n, p.rad, c->stream);
+        c->film.reserve(size_t(npix) * sizeof(double4));
+        hip_check(hipMemsetAsync(c->film.ptr, 0, size_t(npix) * 32, c->stream), "film clear");
+        hip_check(hipMemsetAsync(c->stats.ptr, 0, 128, c->stream), "stats clear");
+        RenderConsts rc{};
+        rc.width = P.width; rc.height = P.height;
+        rc.filter = P.filter_type; rc.filter_radius = P.filter_radius; rc.filter_alpha = P.filter_alpha;
+        rc.seed = P.seed;
+        rc.spp_in_pass = P.spp;
+        rc.sample_base = P.sample_offset;
+        launch_accumulate(p, rc, c->film.as<double4>(), c->stream);
+        hip_check(hipGetLastError(), "k_accumulate launch");
+        hip_check(hipMemcpyAsync(film_sums, c->film.ptr, size_t(npix) * 32, hipMemcpyDeviceToHost, c->stream), "D2H");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+        c->film_w = P.width;
+        c->film_h = P.height;
+    });
+}
+
 tr_status tr_render(tr_ctx* c, const tr_render_params* prm, double* film_sums, int32_t accumulate, float* image,
                     tr_render_stats* stats) {
     return guarded(c, [&] {
@@ -990,8 +1116,7 @@ tr_status tr_render(tr_ctx* c, const tr_render_params* prm, double* film_sums, i
         if (P.width <= 0 || P.height <= 0 || P.spp <= 0) throw InvalidArg("width, height and spp must be positive");
         if (P.max_depth < 0 || P.min_depth < 0) throw InvalidArg("depths must be >= 0");
         if (P.direct_light_sample_num > 1) throw InvalidArg("direct_light_sample_num > 1 is not supported (use 1)");
-        if (P.filter_type != TR_FILTER_BOX && P.filter_type != TR_FILTER_GAUSSIAN) throw InvalidArg("unknown filter");
-        if (!(P.filter_radius > 0.0f && P.filter_radius <= 1.0f)) throw InvalidArg("filter radius must lie in (0, 1]");
+        check_filter(P);
         if (P.guide != 0 && P.guide != 1) throw InvalidArg("guide must be 0 or 1");
         if (P.guide && !(P.guide_fraction >= 0.0f && P.guide_fraction < 1.0f))
             throw InvalidArg("guide_fraction must lie in [0, 1) (0 = the default 0.5)");

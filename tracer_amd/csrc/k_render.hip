@@ -617,8 +617,13 @@ __global__ __launch_bounds__(kShadeBlock) void k_tail(DevScene s, DevPaths p, Re
 // Film accumulation for one pass (Tile::add_sample + Film::merge_film_tile,
 // src/core/render.hpp:37-57, 90-100): every pixel gathers the samples of the
 // pass that its filter footprint covers, in a fixed order, into FP64 sums —
-// deterministic without atomics.  Sample positions are re-derived from the
-// sample's RNG stream (its first two numbers).
+// deterministic without atomics.  A sample at film position (qx + fu, qy + fv)
+// reaches pixel px when |qx + fu - px - 0.5| <= r, so only pixels with
+// |qx - px| <= floor(r + 0.5) can contribute.  Sample positions are re-derived
+// from the sample's RNG stream (its first two numbers).  A sample whose
+// radiance is not finite counts as black with its filter weight, as
+// eval_pixel_li returns black for an invalid L
+// (src/renderer/path_trace_renderer.cpp:199-203) and the film adds it.
 __global__ __launch_bounds__(256) void k_accumulate(DevPaths p, RenderConsts c, double4* film) {
     const int pix = blockIdx.x * blockDim.x + threadIdx.x;
     const int npix = c.width * c.height;
@@ -626,32 +631,36 @@ __global__ __launch_bounds__(256) void k_accumulate(DevPaths p, RenderConsts c, 
     const int px = pix % c.width, py = pix / c.width;
     double4 acc = film[pix];
     unsigned long long bad = 0;
-    const bool own_only = c.filter == 0 ? c.filter_radius <= 0.5f : c.filter_radius <= 0.5f;
+    // a box of radius <= 1/2 keeps every sample in its own pixel
+    const bool own_only = c.filter == 0 && c.filter_radius <= 0.5f;
+    const int R = own_only ? 0 : int(floorf(c.filter_radius + 0.5f));
     const float e = expf(-c.filter_alpha * c.filter_radius * c.filter_radius);
     for (int sidx = 0; sidx < c.spp_in_pass; ++sidx) {
-        for (int dy = -1; dy <= 1; ++dy) {
-            for (int dx = -1; dx <= 1; ++dx) {
-                if (own_only && (dx != 0 || dy != 0)) continue;
-                const int qx = px + dx, qy = py + dy;
-                if (qx < 0 || qy < 0 || qx >= c.width || qy >= c.height) continue;
+        for (int dy = -R; dy <= R; ++dy) {
+            const int qy = py + dy;
+            if (qy < 0 || qy >= c.height) continue;
+            for (int dx = -R; dx <= R; ++dx) {
+                const int qx = px + dx;
+                if (qx < 0 || qx >= c.width) continue;
                 const int qpix = qy * c.width + qx;
                 const int slot = sidx * npix + qpix;
-                const float4 L = p.rad[slot];
+                float4 L = p.rad[slot];
                 const bool fin = isfinite(L.x) && isfinite(L.y) && isfinite(L.z);
-                if (dx == 0 && dy == 0 && !fin) ++bad;
-                if (!fin) continue;
+                if (!fin) {
+                    if (dx == 0 && dy == 0) ++bad;
+                    L = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                }
                 float w = 1.0f;
-                if (!(c.filter == 0 && own_only)) {
+                if (!own_only) {
                     Pcg rng = pcg_for_sample(c.seed, uint32_t(qpix), uint32_t(c.sample_base + sidx));
                     const float fu = rng.uniform(), fv = rng.uniform();
                     const float ox = float(qx) + fu - (float(px) + 0.5f), oy = float(qy) + fv - (float(py) + 0.5f);
-                    if (c.filter == 0) {
-                        w = (fabsf(ox) <= c.filter_radius && fabsf(oy) <= c.filter_radius) ? 1.0f : 0.0f;
-                    } else {
+                    if (!(fabsf(ox) <= c.filter_radius && fabsf(oy) <= c.filter_radius)) continue;
+                    if (c.filter != 0) {
                         // GaussianFilter::eval (src/filter/gaussian_filter.cpp:20-30)
                         const float gx = fmaxf(0.0f, expf(-c.filter_alpha * ox * ox) - e);
                         const float gy = fmaxf(0.0f, expf(-c.filter_alpha * oy * oy) - e);
-                        w = (fabsf(ox) <= c.filter_radius && fabsf(oy) <= c.filter_radius) ? gx * gy : 0.0f;
+                        w = gx * gy;
                     }
                 }
                 if (w == 0.0f) continue;
@@ -805,6 +814,54 @@ __global__ void k_tex_eval(DevScene s, int texture, const float* uv, int64_t n, 
     if (i >= n) return;
     const f3 c = tex_lookup(s, texture - 1, uv[2 * i], uv[2 * i + 1]);
     out[3 * i] = c.x; out[3 * i + 1] = c.y; out[3 * i + 2] = c.z;
+}
+// Test batches through the renderer's own device functions: camera rays
+// (ThinLensCamera::generate_ray), light selection (pick_light), HG sampling,
+// and film samples with given radiance for k_accumulate.
+__global__ void k_camera_rays(DevScene s, const float* film_uv, const float* lens_uv, int64_t n, float4* rays) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    f3 o, d;
+    camera_ray(s.cam, film_uv[2 * i], film_uv[2 * i + 1], lens_uv[2 * i], lens_uv[2 * i + 1], o, d);
+    rays[2 * i] = f4(o, 0.0f);
+    rays[2 * i + 1] = f4(d, 3.0e38f);
+}
+__global__ void k_light_pick(DevScene s, const float* u, int64_t n, int* idx, float* pmf) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float pm;
+    idx[i] = pick_light(s, u[i], pm);
+    pmf[i] = pm;
+}
+__global__ void k_hg_sample(float g, const float* wo, const float* u, int64_t n, float* wi, float* pdf) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float pp;
+    const f3 w = hg_sample(g, mk(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]), u[2 * i], u[2 * i + 1], pp);
+    wi[3 * i] = w.x; wi[3 * i + 1] = w.y; wi[3 * i + 2] = w.z;
+    pdf[i] = pp;
+}
+__global__ void k_unpack_radiance(const float* L, int64_t n, float4* rad) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    rad[i] = make_float4(L[3 * i], L[3 * i + 1], L[3 * i + 2], 0.0f);
+}
+void launch_camera_rays(const DevScene& s, const float* film_uv, const float* lens_uv, int64_t n, float4* rays,
+                        hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_camera_rays, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, s, film_uv, lens_uv, n, rays);
+}
+void launch_light_pick(const DevScene& s, const float* u, int64_t n, int* idx, float* pmf, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_light_pick, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, s, u, n, idx, pmf);
+}
+void launch_hg_sample(float g, const float* wo, const float* u, int64_t n, float* wi, float* pdf, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_hg_sample, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, g, wo, u, n, wi, pdf);
+}
+void launch_unpack_radiance(const float* L, int64_t n, float4* rad, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_unpack_radiance, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, L, n, rad);
 }
 void launch_grid_density(const DevScene& s, int medium, const float* x, int64_t n, float* out, hipStream_t st) {
     if (n <= 0) return;

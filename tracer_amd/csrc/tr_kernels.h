@@ -141,6 +141,11 @@ void launch_env_sample(const DevScene& s, const float* u, int64_t n, float* wi, 
 void launch_env_eval(const DevScene& s, const float* d, int64_t n, float* L, float* pdf, hipStream_t st);
 void launch_grid_density(const DevScene& s, int medium, const float* x, int64_t n, float* out, hipStream_t st);
 void launch_tex_eval(const DevScene& s, int texture, const float* uv, int64_t n, float* out, hipStream_t st);
+void launch_camera_rays(const DevScene& s, const float* film_uv, const float* lens_uv, int64_t n, float4* rays,
+                        hipStream_t st);
+void launch_light_pick(const DevScene& s, const float* u, int64_t n, int* idx, float* pmf, hipStream_t st);
+void launch_hg_sample(float g, const float* wo, const float* u, int64_t n, float* wi, float* pdf, hipStream_t st);
+void launch_unpack_radiance(const float* L, int64_t n, float4* rad, hipStream_t st);
 void launch_bsdf_sample(const DevMaterial* mats, const int* ids, const float* wo, const float* ng, const float* u,
                         int64_t n, float* wi, float* f, float* pdf, hipStream_t st);
 

@@ -28,7 +28,15 @@ namespace tracer {
 namespace {
 [[noreturn]] void fail(const std::string& what) { throw std::runtime_error(what); }
 
+// One RGBE pixel.  Channels are first clamped into what the format holds:
+// negative and NaN to 0, anything above 255/256 * 2^127 (infinity included)
+// to that largest value, whose exponent byte is 255.
 void put_rgbe(float r, float g, float b, unsigned char* o) {
+    const float kMax = std::ldexp(255.0f / 256.0f, 127);
+    auto clamp = [&](float c) { return c > 0.0f ? std::min(c, kMax) : 0.0f; };   // NaN > 0 is false
+    r = clamp(r);
+    g = clamp(g);
+    b = clamp(b);
     const float v = std::max(r, std::max(g, b));
     if (!(v >= 1e-32f)) {
         o[0] = o[1] = o[2] = o[3] = 0;

@@ -71,6 +71,11 @@ def hip_lib() -> C.CDLL:
         lib.tr_medium_density_batch.argtypes = [VP, C.c_int32, P(C.c_float), I64, P(C.c_float)]
         lib.tr_texture_eval_batch.argtypes = [VP, C.c_int32, P(C.c_float), I64, P(C.c_float)]
         F = P(C.c_float)
+        lib.tr_camera_ray_batch.argtypes = [VP, F, F, I64, F]
+        lib.tr_light_pmf.argtypes = [VP, F, I64, P(I64)]
+        lib.tr_light_pick_batch.argtypes = [VP, F, I64, P(C.c_int32), F]
+        lib.tr_hg_sample_batch.argtypes = [VP, C.c_float, F, F, I64, F, F]
+        lib.tr_film_accumulate_batch.argtypes = [VP, P(TrRenderParams), F, P(C.c_double)]
         lib.tr_guide_default_config.argtypes = [P(abi.TrGuideConfig)]
         lib.tr_guide_default_config.restype = None
         lib.tr_guide_init.argtypes = [VP, P(abi.TrGuideConfig)]
@@ -538,6 +543,49 @@ class Context:
                                                   fptr(wi), fptr(f), fptr(pdf)))
         return wi, f, pdf
 
+
+    # ---- test batches through the renderer's device code ----------------
+    def camera_rays(self, film_uv: np.ndarray, lens_uv: np.ndarray) -> np.ndarray:
+        """Camera rays of the loaded scene: film_uv, lens_uv (n, 2) -> (n, 8)."""
+        f = np.ascontiguousarray(film_uv, np.float32).reshape(-1, 2)
+        l = np.ascontiguousarray(lens_uv, np.float32).reshape(-1, 2)
+        out = np.zeros((f.shape[0], 8), np.float32)
+        self._check(self.lib.tr_camera_ray_batch(self.h, fptr(f), fptr(l), f.shape[0], fptr(out)))
+        return out
+
+    def light_pmf(self) -> np.ndarray:
+        """Light-selection pmf: emissive triangles in scene order, environment, sun."""
+        n = C.c_int64()
+        self._check(self.lib.tr_light_pmf(self.h, fptr(None), 0, C.byref(n)))
+        out = np.zeros(n.value, np.float32)
+        self._check(self.lib.tr_light_pmf(self.h, fptr(out), out.size, C.byref(n)))
+        return out
+
+    def light_pick(self, u: np.ndarray):
+        """Light index (numbered as light_pmf) and pmf picked by uniforms u (n,)."""
+        uu = np.ascontiguousarray(u, np.float32).reshape(-1)
+        idx = np.zeros(uu.size, np.int32)
+        pmf = np.zeros(uu.size, np.float32)
+        self._check(self.lib.tr_light_pick_batch(self.h, fptr(uu), uu.size, iptr(idx), fptr(pmf)))
+        return idx, pmf
+
+    def hg_sample(self, g: float, wo: np.ndarray, u: np.ndarray):
+        """Henyey-Greenstein samples around wo (n, 3) with uniforms u (n, 2) -> (wi, pdf)."""
+        w = np.ascontiguousarray(wo, np.float32).reshape(-1, 3)
+        uu = np.ascontiguousarray(u, np.float32).reshape(-1, 2)
+        wi = np.zeros_like(w)
+        pdf = np.zeros(w.shape[0], np.float32)
+        self._check(self.lib.tr_hg_sample_batch(self.h, float(g), fptr(w), fptr(uu), w.shape[0], fptr(wi), fptr(pdf)))
+        return wi, pdf
+
+    def film_accumulate(self, params: TrRenderParams, radiance: np.ndarray) -> np.ndarray:
+        """Film sums (H, W, 4) of given sample radiance (spp, H, W, 3) placed where tr_render places them."""
+        L = np.ascontiguousarray(radiance, np.float32)
+        if L.shape != (params.spp, params.height, params.width, 3):
+            raise ValueError("radiance must be (spp, height, width, 3)")
+        sums = np.zeros((params.height, params.width, 4), np.float64)
+        self._check(self.lib.tr_film_accumulate_batch(self.h, C.byref(params), fptr(L), dptr(sums)))
+        return sums
 
     # ---- learned path guiding --------------------------------------------
     def guide_config(self, **kw) -> abi.TrGuideConfig:
