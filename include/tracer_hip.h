@@ -275,6 +275,15 @@ tr_status tr_trace_any(tr_ctx* ctx, const float* rays, int32_t* occluded, int64_
 /* Device-pointer forms, enqueued on the context stream. */
 tr_status tr_trace_closest_device(tr_ctx* ctx, const float* d_rays, float* d_hits, int64_t n);
 tr_status tr_trace_any_device(tr_ctx* ctx, const float* d_rays, int32_t* d_occluded, int64_t n);
+/* tr_trace_closest_device without any device allocation, for callers that
+ * cannot allocate (code under HIP graph capture, serving loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference.  Same rays, hits
+ * and results as tr_trace_closest_device; the traversal stack spill area is a
+ * static 12.6 MB array per device instead of the context's allocation, so the
+ * scene's BVH may need at most 56 stack entries (4-wide depth 17) — deeper
+ * trees get TR_ERR_INVALID — and at most 32768 threads traverse at once.  Calls
+ * that overlap on one GPU (any contexts) share that array and race. */
+tr_status tr_trace_closest_device_noalloc(tr_ctx* ctx, const float* d_rays, float* d_hits, int64_t n);
 /* Closest-hit with per-ray visit counters (nodes, triangles) for the
  * algorithmic-bytes model; slower, never used by tr_render. */
 tr_status tr_trace_closest_counted(tr_ctx* ctx, const float* rays, float* hits,

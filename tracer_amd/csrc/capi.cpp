@@ -654,6 +654,7 @@ tr_status tr_create(tr_ctx** out, int32_t device) {
     c->device = device;
     tr_status s = guarded(c, [&] {
         hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+        hip_check(noalloc_scratch_init(), "hipGetSymbolAddress (allocation-free scratch)");
         c->counters.reserve(size_t(kNumCounters) * kCounterStride * sizeof(unsigned int));
         c->stats.reserve(128);
     });
@@ -785,6 +786,23 @@ tr_status tr_trace_closest_device(tr_ctx* c, const float* d_rays, float* d_hits,
         if (n < 0 || (n > 0 && (!d_rays || !d_hits))) throw InvalidArg("bad ray batch");
         launch_trace_closest(c->ds.tr, reinterpret_cast<const float4*>(d_rays), reinterpret_cast<float4*>(d_hits), n,
                              c->ovf_node.as<int>(), c->ovf_t.as<float>(), c->ovf_threads, c->stream);
+        hip_check(hipGetLastError(), "k_trace_closest launch");
+    });
+}
+
+tr_status tr_trace_closest_device_noalloc(tr_ctx* c, const float* d_rays, float* d_hits, int64_t n) {
+    return guarded(c, [&] {
+        if (!c->have_bvh) throw StateError("tr_trace_closest_device_noalloc before tr_bvh_build");
+        if (n < 0 || (n > 0 && (!d_rays || !d_hits))) throw InvalidArg("bad ray batch");
+        if (c->max_stack > noalloc_max_stack())
+            throw InvalidArg("the BVH needs " + std::to_string(c->max_stack) + " traversal stack entries; "
+                             "tr_trace_closest_device_noalloc holds at most " + std::to_string(noalloc_max_stack()));
+        try {
+            launch_trace_closest_noalloc(c->ds.tr, reinterpret_cast<const float4*>(d_rays),
+                                         reinterpret_cast<float4*>(d_hits), n, c->stream);
+        } catch (const std::invalid_argument& e) {
+            throw InvalidArg(e.what());
+        }
         hip_check(hipGetLastError(), "k_trace_closest launch");
     });
 }

@@ -2,7 +2,9 @@
 // test must round exactly like the CPU restatement it is checked against.
 #include <algorithm>
 #include <atomic>
+#include <mutex>
 #include <stdexcept>
+#include <string>
 
 #include "tr_kernels.h"
 
@@ -245,6 +247,61 @@ int trace_grid_threads() {
     int b = 0;
     for (int k = 0; k < TK_NUM; ++k) b = std::max(b, grid_blocks(TraceKernel(k)));
     return b * kTraceBlock;
+}
+
+// ---- allocation-free closest-hit queries ---------------------------------
+// tr_trace_closest_device_noalloc traverses with its spill area in this
+// file-scope array instead of the context's allocation: kNoAllocEntries stack
+// entries (node id, then entry distance as float bits) for each of
+// kNoAllocThreads threads, 12.6 MB per device, held until the process ends.
+// Every call on a device shares that device's copy: calls that overlap on one
+// GPU race.  The array's name is only a host-side stand-in; its device
+// address is looked up once per device (noalloc_scratch_init) and used from
+// then on.
+constexpr int kNoAllocBlocks = 128;
+constexpr int kNoAllocThreads = kNoAllocBlocks * kTraceBlock;
+constexpr int kNoAllocEntries = 48;
+static __device__ __align__(256) int scratch[2 * kNoAllocEntries * kNoAllocThreads];
+
+namespace {
+std::mutex g_scratch_mtx;
+void* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+hipError_t noalloc_scratch_init() {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mtx);
+    if (g_scratch_ptr[dev]) return hipSuccess;
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[dev] = ptr;
+    return hipSuccess;
+}
+
+int noalloc_max_stack() { return kNoAllocEntries + kLdsStack; }
+
+void launch_trace_closest_noalloc(const TraceScene& sc, const float4* rays, float4* hits, int64_t n, hipStream_t st) {
+    if (n <= 0) return;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) throw std::runtime_error("no current device");
+    void* ptr = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mtx);
+        ptr = g_scratch_ptr[dev];
+    }
+    if (!ptr) throw std::runtime_error("the allocation-free scratch of this device is not set up");
+    if (sc.max_stack - kLdsStack > kNoAllocEntries)
+        throw std::invalid_argument("the BVH is too deep for the allocation-free spill area (at most " +
+                                    std::to_string(kNoAllocEntries + kLdsStack) + " stack entries)");
+    const int64_t want = (n + kTraceBlock - 1) / kTraceBlock;
+    const int blocks = int(std::min<int64_t>(want, kNoAllocBlocks));
+    int* node = static_cast<int*>(ptr);
+    float* t = reinterpret_cast<float*>(node + size_t(kNoAllocEntries) * kNoAllocThreads);
+    hipLaunchKernelGGL(k_trace_closest, dim3(blocks), dim3(kTraceBlock), 0, st, sc, rays, hits, n, node, t);
 }
 
 void launch_trace_closest(const TraceScene& sc, const float4* rays, float4* hits, int64_t n, int* ovf_node, float* ovf_t,

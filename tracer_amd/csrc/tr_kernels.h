@@ -115,6 +115,13 @@ void launch_trace_any(const TraceScene& sc, const float4* rays, int* occ, int64_
                       int ovf_threads, hipStream_t st);
 void launch_trace_counted(const TraceScene& sc, const float4* rays, float4* hits, uint32_t* nvis, uint32_t* ntri,
                           int64_t n, int* ovf_node, float* ovf_t, int ovf_threads, hipStream_t st);
+// Closest hits with the traversal spill area in a file-scope device array
+// (no device allocation).  noalloc_scratch_init looks up its address on the
+// current device (once per device); noalloc_max_stack is the deepest traversal
+// stack it holds.
+hipError_t noalloc_scratch_init();
+int noalloc_max_stack();
+void launch_trace_closest_noalloc(const TraceScene& sc, const float4* rays, float4* hits, int64_t n, hipStream_t st);
 // wavefront: extend the current queue, trace the shadow queue
 void launch_extend(const TraceScene& sc, const DevPaths& p, int cur, int max_paths, bool count, hipStream_t st);
 // record != 0: validate the light-sample training records of unoccluded shadow rays
