@@ -20,9 +20,11 @@ Also reported on the same JSON line:
                  traffic: HBM bytes per launch
                  from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary.
   equal_time     configuration 4 (the only light shines through a 0.5 x 0.5
-                 aperture) at a reduced film: relMSE against a 65536-spp plain
-                 reference of the guided renderer (training included) and of
-                 the plain renderer given the same device time.
+                 aperture) at its 1024 x 1024 film, 256 spp: relMSE against a
+                 16384-spp plain reference of the guided renderer (learned
+                 light sampling, training included) and of the plain renderer
+                 given the same device time; the guide network alone and with
+                 the learned light sampling as ablations.
   guide_network  GFLOP/s of guide inference and training against the bf16
                  MFMA peak.
   cpu_baseline   the CPU restatement of the reference renderer (all lights
@@ -289,21 +291,33 @@ def throughput(ctx, scene, steps: int, warmup: int) -> dict:
     return {"W": W, "H": H, "spp": spp, "max_depth": base.max_depth, "elapsed": elapsed, **tot}
 
 
-def equal_time(res: int, ref_spp: int, spp: int, seeds: int, hidden: int, train_pp: int = 32, steps: int = 128,
-               train_spp: int = 0) -> dict:
-    """Configuration 4 at res x res: guided (training included) vs plain at the
-    same device time, both against a ref_spp plain reference."""
+def equal_time(res: int, ref_spp: int, spp: int, seeds: int, ablation_seeds: int, hidden: int, light_grid: int,
+               light_train_passes: int, light_train_pp: int, net_train_pp: int, net_steps: int) -> dict:
+    """Configuration 4 at res x res (its stated 1024 x 1024 by default): the
+    guided renderer against the plain one at the same device time, both
+    against a ref_spp plain reference.
+
+    Guided (headline): learned light sampling (guide = 2): light_train_passes
+    training passes of light_train_pp spp each (the first samples lights by
+    power, the later ones the distribution learned so far), then the rest of
+    the spp sampling the learned distribution; training included in the time.
+    Plain: a render of the same device time (its spp scaled by the time ratio
+    measured here), independent seeds.
+    Ablations (relMSE x time against the plain render's, fewer seeds): the
+    guide network alone (guide = 1) and the network with the learned light
+    sampling."""
     from tracer_amd import native
     scene = native.config_scene(4)
     ctx = native.Context(0)
     ctx.load_scene(scene)
     t0 = time.perf_counter()
-    chunks = 8   # progressive chunks of the reference, accumulated on one FP64 film
+    chunk = min(1024, ref_spp)
     ref = None
-    for c in range(chunks):
-        p = scene.render_params(width=res, height=res, spp=ref_spp // chunks, seed=1, sample_offset=c * (ref_spp // chunks))
+    for c in range(ref_spp // chunk):
+        p = scene.render_params(width=res, height=res, spp=chunk, seed=1, sample_offset=c * chunk)
         ref, _, _ = ctx.render(p, film=ref, want_image=False)
-        print(f"[bench] reference chunk {c + 1}/{chunks} ({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+        print(f"[bench] reference chunk {c + 1}/{ref_spp // chunk} ({time.perf_counter() - t0:.1f} s)", file=sys.stderr,
+              flush=True)
     ref_s = time.perf_counter() - t0
     R = ref[..., :3] / ref[..., 3:]
 
@@ -311,53 +325,82 @@ def equal_time(res: int, ref_spp: int, spp: int, seeds: int, hidden: int, train_
         img = sums[..., :3] / sums[..., 3:]
         return float(np.mean((img - R) ** 2 / (R ** 2 + 1e-2)))
 
-    train_spp = train_spp or spp // 2   # training phase: passes of train_pp spp, `steps` Adam steps after each
-    g_err, g_ms, g_train_ms, g_inf_ms, g_inf, g_smp = [], [], [], [], 0, 0
+    def light_render(k):
+        ts = light_train_passes * light_train_pp
+        s1, _, a = ctx.render(scene.render_params(width=res, height=res, spp=ts, seed=200 + k, guide=2,
+                                                  guide_train_passes=light_train_passes,
+                                                  samples_per_pass=light_train_pp, guide_light_grid=light_grid),
+                              want_image=False)
+        s2, _, b = ctx.render(scene.render_params(width=res, height=res, spp=spp - ts, seed=200 + k, sample_offset=ts,
+                                                  guide=2, guide_light_grid=light_grid), film=s1, want_image=False)
+        return relmse(s2), a.ms_total + b.ms_total
+
+    def net_render(k, grid):
+        ctx.guide_init(hidden=hidden, seed=11 + k, batch=1 << 16, steps_per_pass=net_steps)
+        s1, _, a = ctx.render(scene.render_params(width=res, height=res, spp=net_train_pp, seed=200 + k, guide=1,
+                                                  guide_train_passes=1, samples_per_pass=net_train_pp,
+                                                  guide_light_grid=grid, profile=1), want_image=False)
+        s2, _, b = ctx.render(scene.render_params(width=res, height=res, spp=spp - net_train_pp, seed=200 + k,
+                                                  sample_offset=net_train_pp, guide=1, guide_light_grid=grid,
+                                                  profile=1), film=s1, want_image=False)
+        return relmse(s2), a.ms_total + b.ms_total, a, b
+
+    ctx.render(scene.render_params(width=res, height=res, spp=1, seed=5), want_image=False)   # warm-up
+    p_ms, p_err_same = [], []
     for k in range(seeds):
-        ctx.guide_init(hidden=hidden, seed=11 + k, batch=1 << 16, steps_per_pass=steps)
-        s1, _, a = ctx.render(scene.render_params(width=res, height=res, spp=train_spp, seed=200 + k, guide=1,
-                                                  guide_train_passes=train_spp // train_pp, samples_per_pass=train_pp,
-                                                  profile=1), want_image=False)
-        s2, _, b = ctx.render(scene.render_params(width=res, height=res, spp=spp - train_spp, seed=200 + k,
-                                                  sample_offset=train_spp, guide=1, guide_train_passes=0, profile=1),
-                              film=s1, want_image=False)
-        g_err.append(relmse(s2))
-        g_ms.append(a.ms_total + b.ms_total)
-        g_train_ms.append(a.ms_guide_train)
-        g_inf_ms.append(a.ms_guide + b.ms_guide)
-        g_inf += a.guide_inferences + b.guide_inferences
-        g_smp += a.guide_train_samples
-    p_ms = []
-    for k in range(seeds):
-        _, _, st = ctx.render(scene.render_params(width=res, height=res, spp=spp, seed=300 + k), want_image=False)
+        sums, _, st = ctx.render(scene.render_params(width=res, height=res, spp=spp, seed=300 + k), want_image=False)
         p_ms.append(st.ms_total)
-    spp_eq = max(1, int(round(spp * np.mean(g_ms) / np.mean(p_ms))))
-    p_err, p_err_same = [], []
-    for k in range(seeds):
-        sums, _, _ = ctx.render(scene.render_params(width=res, height=res, spp=spp_eq, seed=400 + k), want_image=False)
-        p_err.append(relmse(sums))
-        sums, _, _ = ctx.render(scene.render_params(width=res, height=res, spp=spp, seed=500 + k), want_image=False)
         p_err_same.append(relmse(sums))
+    g = [light_render(k) for k in range(seeds)]
+    g_err, g_ms = [x[0] for x in g], [x[1] for x in g]
+    spp_eq = max(1, int(round(spp * np.mean(g_ms) / np.mean(p_ms))))
+    p_err, p_eq_ms = [], []
+    for k in range(seeds):
+        sums, _, st = ctx.render(scene.render_params(width=res, height=res, spp=spp_eq, seed=400 + k), want_image=False)
+        p_err.append(relmse(sums))
+        p_eq_ms.append(st.ms_total)
+    print(f"[bench] equal time: plain {np.mean(p_err):.5f} ({spp_eq} spp), learned light sampling {np.mean(g_err):.5f}",
+          file=sys.stderr, flush=True)
+    # ablations: relMSE x device time against the plain render at spp
+    plain_work = float(np.mean(p_err_same)) * float(np.mean(p_ms))
+    ablations = {}
+    g_inf, g_inf_ms, g_smp, g_train_ms = 0, 0.0, 0, 0.0
+    for name, grid in (("network", 0), ("network_and_light", light_grid)):
+        if ablation_seeds <= 0:
+            break
+        r = [net_render(k, grid) for k in range(ablation_seeds)]
+        e, ms = [x[0] for x in r], [x[1] for x in r]
+        ablations[name] = {"relmse": float(np.mean(e)), "ms": float(np.mean(ms)),
+                           "ratio_plain_over_guided_by_time_scaling": plain_work / (float(np.mean(e)) * float(np.mean(ms)))}
+        if grid == 0:
+            for _, _, a, b in r:
+                g_inf += a.guide_inferences + b.guide_inferences
+                g_inf_ms += a.ms_guide + b.ms_guide
+                g_smp += a.guide_train_samples
+                g_train_ms += a.ms_guide_train
     ctx.close()
     fwd, trn = guide_flops(hidden)
-    inf_tflops = g_inf * fwd / (np.sum(g_inf_ms) / 1e3) / 1e12 if np.sum(g_inf_ms) > 0 else 0.0
-    trn_tflops = g_smp * trn / (np.sum(g_train_ms) / 1e3) / 1e12 if np.sum(g_train_ms) > 0 else 0.0
+    inf_tflops = g_inf * fwd / (g_inf_ms / 1e3) / 1e12 if g_inf_ms > 0 else 0.0
+    trn_tflops = g_smp * trn / (g_train_ms / 1e3) / 1e12 if g_train_ms > 0 else 0.0
+    ref_floor = float(np.mean(p_err_same)) * spp / ref_spp
     return {
         "equal_time": {
-            "workload": (f"configuration 4 ({scene.num_triangles} triangles, light through a 0.5x0.5 aperture) at "
-                         f"{res}x{res}, max depth 16; reference: plain GPU path, {ref_spp} spp, FP64 film "
-                         f"({ref_s:.1f} s); {seeds} seeds each"),
-            "guided": {"spp": spp, "ms": float(np.mean(g_ms)), "relmse": float(np.mean(g_err)),
-                       "relmse_median": float(np.median(g_err)),
-                       "training": f"{train_spp} spp in passes of {train_pp} spp sampled without the guide "
-                                   f"(recording only), {steps} Adam steps of 65536 samples after each (lr 1e-3), then "
-                                   f"{spp - train_spp} spp guided (one-sample MIS 0.5 guide / 0.5 BSDF)",
-                       "ms_training": float(np.mean(g_train_ms)), "ms_inference": float(np.mean(g_inf_ms))},
-            "plain_equal_time": {"spp": spp_eq, "ms": float(np.mean(p_ms)) * spp_eq / spp,
-                                 "relmse": float(np.mean(p_err)), "relmse_median": float(np.median(p_err))},
-            "plain_equal_spp": {"spp": spp, "ms": float(np.mean(p_ms)), "relmse": float(np.mean(p_err_same))},
+            "workload": (f"configuration 4 ({scene.num_triangles} triangles, the only light behind a 0.5x0.5 aperture) at "
+                         f"{res}x{res}, {spp} spp, max depth 16; reference: plain GPU path, {ref_spp} spp, FP64 film "
+                         f"({ref_s:.0f} s; its own relMSE is about {ref_floor:.1e}, the plain 256-spp error scaled by "
+                         f"spp); {seeds} seeds each"),
+            "guided": {"mode": "learned light sampling (guide = 2)", "spp": spp, "ms": float(np.mean(g_ms)),
+                       "relmse": float(np.mean(g_err)), "relmse_median": float(np.median(g_err)),
+                       "training": f"{light_train_passes} passes of {light_train_pp} spp ({light_grid}^3 cells), "
+                                   f"the first by power, later ones sampling what was learned so far; time included"},
+            "plain_equal_time": {"spp": spp_eq, "ms": float(np.mean(p_eq_ms)), "relmse": float(np.mean(p_err)),
+                                 "relmse_median": float(np.median(p_err))},
+            "plain_equal_spp": {"spp": spp, "ms": float(np.mean(p_ms)), "relmse": float(np.mean(p_err_same)),
+                                "relmse_median": float(np.median(p_err_same))},
             "ratio_plain_over_guided": float(np.mean(p_err) / np.mean(g_err)),
             "ratio_median": float(np.median(p_err) / np.median(g_err)),
+            "ablations": {**ablations, "seeds": ablation_seeds,
+                          "network_training": f"one {net_train_pp}-spp pass, {net_steps} Adam steps of 65536 samples"},
         },
         "guide_network": {
             "hidden": hidden, "flops_per_inference": fwd, "flops_per_training_sample": trn,
@@ -379,13 +422,16 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-relmse", action="store_true")
     ap.add_argument("--no-other", action="store_true", help="skip the configuration 3 and 5 frames")
-    ap.add_argument("--relmse-res", type=int, default=256)
-    ap.add_argument("--ref-spp", type=int, default=65536)
+    ap.add_argument("--relmse-res", type=int, default=1024)
+    ap.add_argument("--ref-spp", type=int, default=16384)
     ap.add_argument("--relmse-spp", type=int, default=256)
-    ap.add_argument("--relmse-seeds", type=int, default=24)
-    ap.add_argument("--train-pass-spp", type=int, default=32, help="spp per training pass of the guided render")
-    ap.add_argument("--train-steps", type=int, default=16, help="Adam steps after each training pass")
-    ap.add_argument("--train-spp", type=int, default=32, help="spp of the guided render's training phase (0: half)")
+    ap.add_argument("--relmse-seeds", type=int, default=6)
+    ap.add_argument("--ablation-seeds", type=int, default=2)
+    ap.add_argument("--light-grid", type=int, default=20, help="cells per axis of the learned light sampling")
+    ap.add_argument("--light-train-passes", type=int, default=8)
+    ap.add_argument("--light-train-pass-spp", type=int, default=8)
+    ap.add_argument("--train-pass-spp", type=int, default=32, help="spp of the guide network's training pass")
+    ap.add_argument("--train-steps", type=int, default=16, help="Adam steps after the network's training pass")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one 1 GiB device-to-device copy (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -478,8 +524,9 @@ def main() -> int:
         out["other_configs"] = other_configs()
     if not args.no_relmse:
         print("[bench] equal-time relMSE on configuration 4 ...", file=sys.stderr, flush=True)
-        out.update(equal_time(args.relmse_res, args.ref_spp, args.relmse_spp, args.relmse_seeds, 64,
-                              args.train_pass_spp, args.train_steps, args.train_spp))
+        out.update(equal_time(args.relmse_res, args.ref_spp, args.relmse_spp, args.relmse_seeds, args.ablation_seeds, 64,
+                              args.light_grid, args.light_train_passes, args.light_train_pass_spp, args.train_pass_spp,
+                              args.train_steps))
         print(f"[bench] equal-time ratio {out['equal_time']['ratio_plain_over_guided']:.2f}", file=sys.stderr,
               flush=True)
     if not args.no_cpu:

@@ -185,7 +185,9 @@ typedef struct tr_render_params {
     uint64_t seed;
     int32_t sample_offset;             /* index of the first sample, for progressive rendering */
     int32_t samples_per_pass;          /* samples per pixel in flight at once, 0 = auto */
-    int32_t guide;                     /* 0 = plain path tracer, 1 = learned path guiding */
+    int32_t guide;                     /* 0 = plain path tracer, 1 = learned path guiding (the network, and
+                                          the learned light sampling of guide_light_grid when > 0),
+                                          2 = the learned light sampling alone (needs guide_light_grid > 0) */
     int32_t guide_train_passes;        /* passes that train the guide before it is used (guide=1) */
     float guide_fraction;              /* one-sample MIS probability of the guide (0.5) */
     float guide_lr;                    /* Adam learning rate (1e-3) */
@@ -197,6 +199,14 @@ typedef struct tr_render_params {
                                           have the plain estimator's variance) */
     int32_t tail_paths;                /* plain passes: once at most this many paths are alive, one
                                           kernel carries them to their ends (0 = 131072, < 0 = never) */
+    int32_t guide_light_grid;          /* guide = 1, 2: > 0 learns where light comes from: the scene box
+                                          is cut into guide_light_grid^3 cells (at most 64), training passes
+                                          learn per cell which lights and which parts of them deliver
+                                          unoccluded light, and later passes draw next-event light samples
+                                          from that (mixed with the power-based choice); 0 = off.  What is
+                                          learned carries over to later calls on the context (progressive
+                                          rendering) and starts again with a call that trains from sample 0
+                                          (sample_offset = 0), a new scene or tr_guide_init */
 } tr_render_params;
 
 typedef struct tr_render_stats {

@@ -116,6 +116,7 @@ class TrRenderParams(C.Structure):
         ("guide_fraction", C.c_float), ("guide_lr", C.c_float),
         ("profile", C.c_int32), ("guide_max_depth", C.c_int32),
         ("guide_train_fraction", C.c_float), ("tail_paths", C.c_int32),
+        ("guide_light_grid", C.c_int32),
     ]
 
 

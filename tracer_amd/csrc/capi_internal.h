@@ -113,6 +113,18 @@ struct GuideState {
     }
 };
 
+// Learned light-sample distribution (tr_render_params::guide_light_grid):
+// training sums, sampling tables and the training info of shadow rays.  The
+// sums accumulate over every training pass until the scene, the grid or the
+// guide is reset.
+struct LightCacheState {
+    int grid = 0, strata = 0, bins = 0, num_area = 0;
+    float lo[3] = {0, 0, 0}, scale[3] = {0, 0, 0};
+    DevBuf sum, hits, pmf, table, work, sh_g;
+    int64_t sh_cap = 0;
+    bool built = false;   // the tables hold a distribution from at least one training pass
+};
+
 }  // namespace trcapi
 
 struct tr_ctx {
@@ -142,6 +154,7 @@ struct tr_ctx {
     trcapi::DevBuf s_in, s_out, s_aux, s_aux2, s_aux3, s_aux4;
     std::vector<hipEvent_t> events;
     trcapi::GuideState guide;
+    trcapi::LightCacheState lcache;
 };
 
 namespace trcapi {

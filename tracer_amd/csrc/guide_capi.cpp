@@ -117,6 +117,8 @@ void init_guide(tr_ctx* c, const tr_guide_config& cfg) {
     g.last_samples = 0;
     g.box_set = false;   // back to the scene bounds until tr_guide_set_box
     g.ready = true;
+    c->lcache.grid = 0;  // the learned light distribution starts again with the network
+    c->lcache.built = false;
 }
 
 void ensure_box(tr_ctx* c) {

@@ -67,7 +67,7 @@ __global__ __launch_bounds__(256) void k_generate(DevScene s, DevPaths p, Render
     p.ray_o[i] = f4(o, 0.0f);
     p.ray_d[i] = f4(d, 3.0e38f);
     p.thr[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-    p.rad[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    p.rad[i] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-1));
     p.info[i] = make_uint2(pixel, pack_info(0, 0, 0));
     store_rng(p, i, rng);
     p.queue[cur][i] = i;
@@ -141,13 +141,62 @@ struct ShadeOut {
     float4 rec0, rec1, rec2, rec3, aux0;
     float4 nrec0, nrec1, nrec2, nrec3;
     float nee_weight;
+    int lc_cell, lc_bin;         // light-cache training: cell and bin of the light sample (-1: none)
+    float lc_val;                // its unoccluded contribution / sampling density (luminance)
 };
+
+// Light-cache mode of a shading kernel: 0 off, 1 sampling (the cache is
+// trained), 2 training (light samples by power note their cell and bin),
+// 3 decided per pass by DevLightCache::use / train (the guide kernels).
+template <int LCM> TR_D bool lc_use(const DevPaths& p) { return LCM == 1 || (LCM == 3 && p.lc.use); }
+template <int LCM> TR_D bool lc_train(const DevPaths& p) { return LCM == 2 || (LCM == 3 && p.lc.train); }
+
+// Next-event light choice: by power, or (guided passes with a trained light
+// cache) from the cache of the vertex's cell.  Draws the same three uniforms
+// either way.  Training passes note the cell and bin of the power-based choice.
+// here_cell: the cell of x when the cache is sampled.
+template <int LCM>
+TR_D int nee_pick(const DevScene& s, const DevPaths& p, f3 x, int here_cell, Pcg& rng, float& lu, float& lv, float& pmf,
+                  int& cell, int& bin) {
+    cell = -1;
+    bin = -1;
+    if (lc_use<LCM>(p)) {
+        const int b = lc_sample_bin(p.lc, here_cell, rng.uniform(), pmf);
+        lu = rng.uniform();
+        lv = rng.uniform();
+        if (lc_train<LCM>(p)) {
+            cell = here_cell;
+            bin = b;
+        }
+        return lc_bin_light(p.lc, b, lu, lv, pmf);
+    }
+    const int e = pick_light(s, rng.uniform(), pmf);
+    lu = rng.uniform();
+    lv = rng.uniform();
+    if (lc_train<LCM>(p)) {
+        cell = lc_cell(p.lc, x);
+        bin = lc_bin_of_sample(p.lc, e, lu, lv);
+    }
+    return e;
+}
+
+// Selection probability of light `light` hit at barycentrics (b1, b2) for the
+// vertex whose cell is prev_cell (-1: the power-based choice), per unit of the
+// light's area parameters; light == num_area_lights: the environment.
+template <int LCM>
+TR_D float nee_select_pdf(const DevScene& s, const DevPaths& p, int prev_cell, int light, float b1, float b2) {
+    if (lc_use<LCM>(p) && prev_cell >= 0) {
+        if (light >= s.num_area_lights) return lc_pmf(p.lc, prev_cell, lc_env_bin(p.lc));
+        return lc_pmf(p.lc, prev_cell, lc_bin_of_hit(p.lc, light, b1, b2)) * float(p.lc.strata * p.lc.strata);
+    }
+    return light >= s.num_area_lights ? s.light_pmf[s.num_area_lights] : s.lights[4 * light + 1].w;
+}
 
 // One vertex of the path in `slot`: reads its state and hit, adds emission,
 // samples the continuation and a light, and stores the updated state.
 // EXT: the scene has an image environment or textured materials (their
 // code is compiled in only then).
-template <bool GUIDE, bool EXT>
+template <bool GUIDE, bool EXT, int LCM>
 TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c, int slot, ShadeOut& out) {
     bool& cont = out.cont;
     ShadowOut& sh = out.sh;
@@ -160,6 +209,9 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
     g_used = g_fallback = want_rec = want_nee_rec = false;
     rec0 = rec1 = rec2 = rec3 = aux0 = nrec0 = nrec1 = nrec2 = nrec3 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     nee_weight = 0.0f;
+    out.lc_cell = out.lc_bin = -1;
+    out.lc_val = 0.0f;
+    int next_cell = -1;   // cell of this vertex for the next hit's MIS weight (guided passes with a light cache)
     float4 next_o = make_float4(0, 0, 0, 0), next_d = make_float4(0, 0, 0, 0);
     const float4 ro = p.ray_o[slot], rd = p.ray_d[slot], h = p.hit[slot];
     float4 tp = p.thr[slot];
@@ -174,6 +226,7 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
     const float prev_pdf = tp.w;
     f3 L = xyz(rp);
     const f3 o = xyz(ro), d = xyz(rd);
+    const int prev_cell = __float_as_int(rp.w);
     const int tri = __float_as_int(h.y);
     bool alive = true;
     if (tri < 0) {
@@ -182,7 +235,8 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
             float epdf;
             const f3 Le = env_eval<EXT>(s, d, epdf);
             float w = 1.0f;
-            if (prev_pdf > 0.0f) w = power_heuristic(prev_pdf, s.light_pmf[s.num_area_lights] * epdf);
+            if (prev_pdf > 0.0f)
+                w = power_heuristic(prev_pdf, nee_select_pdf<LCM>(s, p, prev_cell, s.num_area_lights, 0.0f, 0.0f) * epdf);
             L += T * Le * w;
         }
         alive = false;
@@ -250,10 +304,11 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
             const DevMedium m = s.media[mid];
             ++scatter;
             const f3 wo = -d;
+            if (lc_use<LCM>(p)) next_cell = lc_cell(p.lc, sp_pos);
             if (s.num_light_entries > 0) {
-                float pmf;
-                const int e = pick_light(s, rng.uniform(), pmf);
-                const float lu = rng.uniform(), lv = rng.uniform();
+                float pmf, lu, lv;
+                int lcell, lbin;
+                const int e = nee_pick<LCM>(s, p, sp_pos, next_cell, rng, lu, lv, pmf, lcell, lbin);
                 const LightSampleD ls = sample_light<EXT>(s, e, sp_pos, lu, lv);
                 if (ls.pdf != 0.0f && pmf > 0.0f && !is_black(ls.L)) {
                     const float ph = hg_phase(m.g, -dot(ls.wi, wo));
@@ -264,6 +319,11 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
                     if (ls.area) trs = segment_tr<EXT>(s, m, sp_pos, ls.wi, ls.dist, c.shadow_tr, rng);
                     const f3 contrib = T * ls.L * trs * (ph * w / lp);
                     if (max3(contrib) > 0.0f && finite3(contrib)) {
+                        if (LCM >= 2 && lbin >= 0) {
+                            out.lc_cell = lcell;
+                            out.lc_bin = lbin;
+                            out.lc_val = fminf(lum(ls.L * trs) / lp, kLcMaxValue);
+                        }
                         sh.emit = true;
                         sh.o = sp_pos;
                         sh.tmin = kEps;
@@ -289,7 +349,7 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
                 const float4 l0 = s.lights[4 * sr.light];
                 float w = 1.0f;
                 if (prev_pdf > 0.0f) {
-                    const float lp = s.lights[4 * sr.light + 1].w * t * t / (l0.w * (-dn));
+                    const float lp = nee_select_pdf<LCM>(s, p, prev_cell, sr.light, h.z, h.w) * t * t / (l0.w * (-dn));
                     w = power_heuristic(prev_pdf, lp);
                 }
                 L += T * xyz(l0) * w;
@@ -322,10 +382,12 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
                 f3 wi, fs;
                 float pdf;
                 bool delta = false;
+                // guide samples are folded into the hemisphere of wo about the shading normal
+                const f3 nsd = dot(wo, ns) >= 0.0f ? ns : -ns;
                 if (use_guide) {
                     float pdf_b;
                     if (rng.uniform() < frac) {
-                        wi = mixture_sample(mix, su, u1, u2);
+                        wi = fold_to_hemisphere(mixture_sample(mix, su, u1, u2), nsd);
                         const BsdfEval be = bsdf_eval(mat, to_local(wi), lwo, lng);
                         fs = be.f;
                         pdf_b = be.pdf;
@@ -335,7 +397,7 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
                         fs = bs.f;
                         pdf_b = bs.pdf;
                     }
-                    pdf = is_black(fs) ? 0.0f : frac * mixture_pdf(mix, wi) + (1.0f - frac) * pdf_b;
+                    pdf = is_black(fs) ? 0.0f : frac * mixture_pdf_folded(mix, wi, nsd) + (1.0f - frac) * pdf_b;
                 } else {
                     const BsdfSample bs = bsdf_sample(mat, lwo, lng, su, u1, u2);
                     wi = normalize(tx * bs.wi.x + ty * bs.wi.y + ns * bs.wi.z);
@@ -351,10 +413,11 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
                 {
                     // next-event estimation (direct_illumination.cpp:35-109); none at a
                     // specular vertex (path_trace_renderer.cpp:141)
+                    if (lc_use<LCM>(p)) next_cell = lc_cell(p.lc, pos);
                     if (!delta && s.num_light_entries > 0) {
-                        float pmf;
-                        const int e = pick_light(s, rng.uniform(), pmf);
-                        const float lu = rng.uniform(), lv = rng.uniform();
+                        float pmf, lu, lv;
+                        int lcell, lbin;
+                        const int e = nee_pick<LCM>(s, p, pos, next_cell, rng, lu, lv, pmf, lcell, lbin);
                         const LightSampleD ls = sample_light<EXT>(s, e, pos, lu, lv);
                         if (ls.pdf != 0.0f && pmf > 0.0f && !is_black(ls.L)) {
                             const float dist = ls.area ? ls.dist - kEps : ls.dist;
@@ -365,7 +428,7 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
                                     float lp = pmf;
                                     if (ls.pdf > 0.0f) {
                                         lp *= ls.pdf;
-                                        const float sp = use_guide ? frac * mixture_pdf(mix, ls.wi) +
+                                        const float sp = use_guide ? frac * mixture_pdf_folded(mix, ls.wi, nsd) +
                                                                          (1.0f - frac) * be.pdf
                                                                    : be.pdf;
                                         w = power_heuristic(lp, sp);
@@ -378,6 +441,11 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
                                     const float cosg = fabsf(dot(ls.wi, ng));
                                     const f3 contrib = T * ls.L * be.f * trs * (cosg * w / lp);
                                     if (max3(contrib) > 0.0f && finite3(contrib)) {
+                                        if (LCM >= 2 && lbin >= 0) {
+                                            out.lc_cell = lcell;
+                                            out.lc_bin = lbin;
+                                            out.lc_val = fminf(lum(ls.L * trs) / lp, kLcMaxValue);
+                                        }
                                         sh.emit = true;
                                         sh.o = pos;
                                         sh.tmin = kEps;
@@ -449,7 +517,9 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
         const int pr = p.grec[slot];
         if (pr >= 0) p.rec_aux[2 * pr + 1] = make_float4(rp.x, rp.y, rp.z, 1.0f);
     }
-    p.rad[slot] = f4(L, rp.w);
+    // rad.w: the cell of this vertex for the next hit's MIS weight (read only
+    // by the kernels that sample the light cache)
+    p.rad[slot] = f4(L, __int_as_float(next_cell));
     if (cont) {
         p.ray_o[slot] = next_o;
         p.ray_d[slot] = next_d;
@@ -465,8 +535,8 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
 // with the combined pdf.  A vertex
 // whose head outputs are not finite uses the plain BSDF estimator, drawing
 // the same random numbers as the plain kernel.
-template <bool GUIDE, bool EXT>
-__global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene s, DevPaths p, RenderConsts c, int cur) {
+template <bool GUIDE, bool EXT, int LCM>
+TR_D void shade_queue(const DevScene& s, const DevPaths& p, const RenderConsts& c, int cur) {
     __shared__ BlockAppendLds<kShadeBlock> app;
     const unsigned int n = *queue_counter(p, cur);
     const int* q = p.queue[cur];
@@ -486,7 +556,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene s, DevPaths p, R
         int slot = 0;
         if (active) {
             slot = q[i];
-            shade_path<GUIDE, EXT>(s, p, c, slot, o);
+            shade_path<GUIDE, EXT, LCM>(s, p, c, slot, o);
         }
         const bool cont = o.cont, g_used = o.g_used, g_fallback = o.g_fallback;
         const bool want_rec = o.want_rec, want_nee_rec = o.want_nee_rec;
@@ -530,6 +600,8 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene s, DevPaths p, R
             p.sh_d[si] = f4(sh.d, sh.tmax);
             p.sh_c[si] = f4(sh.contrib, __int_as_float(slot));
             if (GUIDE && c.guide_record) p.sh_rec[si] = nee_ri;
+            if (lc_train<LCM>(p))
+                p.sh_g[si] = make_float4(__int_as_float(o.lc_cell), __int_as_float(o.lc_bin), o.lc_val, 0.0f);
         }
     }
     if (GUIDE) {
@@ -545,6 +617,18 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene s, DevPaths p, R
     }
 }
 
+template <bool GUIDE, bool EXT, int LCM>
+__global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene s, DevPaths p, RenderConsts c, int cur) {
+    shade_queue<GUIDE, EXT, LCM>(s, p, c, cur);
+}
+// Plain scenes (no image environment, textures or grids) keep three waves
+// per SIMD, also when the light cache adds its few registers.
+template <int LCM>
+__global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_shade3(DevScene s, DevPaths p,
+                                                                                                 RenderConsts c, int cur) {
+    shade_queue<false, false, LCM>(s, p, c, cur);
+}
+
 // The last bounces of a plain pass in one launch: once the queue holds at
 // most max_tail paths, each thread takes one and carries it to its end
 // (closest hit, shading, shadow ray, next bounce).  Late bounces hold few
@@ -554,7 +638,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene s, DevPaths p, R
 // same order as in the per-bounce kernels, so the film is identical.  The
 // last workgroup to finish empties the queue, so the per-bounce kernels that
 // follow find no work; with more than max_tail paths nothing happens.
-template <bool EXT>
+template <bool EXT, int LCM>
 __global__ __launch_bounds__(kShadeBlock) void k_tail(DevScene s, DevPaths p, RenderConsts c, int cur,
                                                       unsigned int max_tail) {
     __shared__ TraceStack<kShadeBlock, kLdsStack> st;
@@ -577,7 +661,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_tail(DevScene s, DevPaths p, Re
             p.hit[slot] = make_float4(h.t, __int_as_float(h.tri), h.b1, h.b2);
             ++n_ext;
             ShadeOut o;
-            shade_path<false, EXT>(s, p, c, slot, o);
+            shade_path<false, EXT, LCM>(s, p, c, slot, o);
             if (o.sh.emit) {
                 ++n_sh;
                 TraceHit hs;
@@ -586,6 +670,8 @@ __global__ __launch_bounds__(kShadeBlock) void k_tail(DevScene s, DevPaths p, Re
                     float4 r = p.rad[slot];
                     r.x += o.sh.contrib.x; r.y += o.sh.contrib.y; r.z += o.sh.contrib.z;
                     p.rad[slot] = r;
+                    if (lc_train<LCM>(p) && o.lc_cell >= 0 && o.lc_val > 0.0f)
+                        lc_record(p.lc, o.lc_cell, o.lc_bin, o.lc_val);
                 }
             }
             cont = o.cont;
@@ -745,23 +831,54 @@ void launch_generate(const DevScene& s, const DevPaths& p, const RenderConsts& c
     const int blocks = (num_paths + 255) / 256;
     hipLaunchKernelGGL(k_generate, dim3(blocks > 0 ? blocks : 1), dim3(256), 0, st, s, p, c, num_paths, cur);
 }
+// Light-cache mode of the plain kernels for this pass (see lc_use / lc_train).
+static int plain_lc_mode(const DevPaths& p) {
+    if (p.lc.grid <= 0) return 0;
+    if (p.lc.train && p.lc.use) return 3;
+    return p.lc.train ? 2 : (p.lc.use ? 1 : 0);
+}
+template <bool EXT, int LCM>
+static void launch_shade_plain(const DevScene& s, const DevPaths& p, const RenderConsts& c, int cur, dim3 g, hipStream_t st) {
+    if constexpr (EXT) hipLaunchKernelGGL((k_shade<false, EXT, LCM>), g, dim3(kShadeBlock), 0, st, s, p, c, cur);
+    else hipLaunchKernelGGL((k_shade3<LCM>), g, dim3(kShadeBlock), 0, st, s, p, c, cur);
+}
 void launch_shade(const DevScene& s, const DevPaths& p, const RenderConsts& c, int cur, int max_paths, bool guide,
                   hipStream_t st) {
     const dim3 g(shade_blocks(max_paths)), b(kShadeBlock);
     const bool img = s.env.w > 0 || s.has_tex || s.has_grid;
-    if (guide && img) hipLaunchKernelGGL((k_shade<true, true>), g, b, 0, st, s, p, c, cur);
-    else if (guide) hipLaunchKernelGGL((k_shade<true, false>), g, b, 0, st, s, p, c, cur);
-    else if (img) hipLaunchKernelGGL((k_shade<false, true>), g, b, 0, st, s, p, c, cur);
-    else hipLaunchKernelGGL((k_shade<false, false>), g, b, 0, st, s, p, c, cur);
+    if (guide) {
+        if (img) hipLaunchKernelGGL((k_shade<true, true, 3>), g, b, 0, st, s, p, c, cur);
+        else hipLaunchKernelGGL((k_shade<true, false, 3>), g, b, 0, st, s, p, c, cur);
+        return;
+    }
+    switch (plain_lc_mode(p) + (img ? 4 : 0)) {
+        case 0: launch_shade_plain<false, 0>(s, p, c, cur, g, st); break;
+        case 1: launch_shade_plain<false, 1>(s, p, c, cur, g, st); break;
+        case 2: launch_shade_plain<false, 2>(s, p, c, cur, g, st); break;
+        case 3: launch_shade_plain<false, 3>(s, p, c, cur, g, st); break;
+        case 4: launch_shade_plain<true, 0>(s, p, c, cur, g, st); break;
+        case 5: launch_shade_plain<true, 1>(s, p, c, cur, g, st); break;
+        case 6: launch_shade_plain<true, 2>(s, p, c, cur, g, st); break;
+        default: launch_shade_plain<true, 3>(s, p, c, cur, g, st); break;
+    }
 }
 void launch_tail(const DevScene& s, const DevPaths& p, const RenderConsts& c, int cur, int max_tail, hipStream_t st) {
     // one thread per path, and no more threads than the traversal spill area has columns
     int blocks = std::min((max_tail + kShadeBlock - 1) / kShadeBlock, p.ovf_threads / kShadeBlock);
     if (blocks < 1) blocks = 1;
-    if (s.env.w > 0 || s.has_tex || s.has_grid)
-        hipLaunchKernelGGL(k_tail<true>, dim3(blocks), dim3(kShadeBlock), 0, st, s, p, c, cur, (unsigned int)max_tail);
-    else
-        hipLaunchKernelGGL(k_tail<false>, dim3(blocks), dim3(kShadeBlock), 0, st, s, p, c, cur, (unsigned int)max_tail);
+    const bool img = s.env.w > 0 || s.has_tex || s.has_grid;
+    const dim3 g(blocks), b(kShadeBlock);
+    const unsigned int mt = (unsigned int)max_tail;
+    switch (plain_lc_mode(p) + (img ? 4 : 0)) {
+        case 0: hipLaunchKernelGGL((k_tail<false, 0>), g, b, 0, st, s, p, c, cur, mt); break;
+        case 1: hipLaunchKernelGGL((k_tail<false, 1>), g, b, 0, st, s, p, c, cur, mt); break;
+        case 2: hipLaunchKernelGGL((k_tail<false, 2>), g, b, 0, st, s, p, c, cur, mt); break;
+        case 3: hipLaunchKernelGGL((k_tail<false, 3>), g, b, 0, st, s, p, c, cur, mt); break;
+        case 4: hipLaunchKernelGGL((k_tail<true, 0>), g, b, 0, st, s, p, c, cur, mt); break;
+        case 5: hipLaunchKernelGGL((k_tail<true, 1>), g, b, 0, st, s, p, c, cur, mt); break;
+        case 6: hipLaunchKernelGGL((k_tail<true, 2>), g, b, 0, st, s, p, c, cur, mt); break;
+        default: hipLaunchKernelGGL((k_tail<true, 3>), g, b, 0, st, s, p, c, cur, mt); break;
+    }
 }
 void launch_accumulate(const DevPaths& p, const RenderConsts& c, double4* film, hipStream_t st) {
     const int n = c.width * c.height;
@@ -815,6 +932,58 @@ __global__ void k_tex_eval(DevScene s, int texture, const float* uv, int64_t n, 
     const f3 c = tex_lookup(s, texture - 1, uv[2 * i], uv[2 * i + 1]);
     out[3 * i] = c.x; out[3 * i + 1] = c.y; out[3 * i + 2] = c.z;
 }
+// Light-cache distributions from the training sums, one thread per cell:
+// p(bin) = a sum(bin) / sum(cell) + (1 - a) default(bin), a = (1 - eps) h /
+// (h + h0) with h the cell's unoccluded training samples, where the default
+// is the power-based light choice spread evenly over a light's strata (cells
+// without an unoccluded sample keep the default alone), then Vose's alias
+// table over the bins; `work` holds one cell's small and large lists (small
+// from the front, large from the back).
+__global__ __launch_bounds__(256) void k_lc_build(DevLightCache lc, const float* light_pmf, float eps, int cells,
+                                                  float* pmf, float2* table, int* work) {
+    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= cells) return;
+    const int B = lc.bins, S2 = lc.strata * lc.strata, env = lc.num_area * S2;
+    const size_t base = size_t(cell) * B;
+    unsigned long long tot = 0;
+    for (int b = 0; b < B; ++b) tot += lc.sum[base + b];
+    const double inv = tot ? 1.0 / double(tot) : 0.0;
+    const float h = float(lc.hits[cell]);
+    const float a = tot ? (1.0f - eps) * h / (h + kLcHits0) : 0.0f;
+    for (int b = 0; b < B; ++b) {
+        const float def = b < env ? light_pmf[b / S2] / float(S2) : light_pmf[lc.num_area + (b - env)];
+        const float q = a * float(double(lc.sum[base + b]) * inv) + (1.0f - a) * def;
+        pmf[base + b] = q;
+    }
+    int* w = work + base;
+    int ns = 0, nl = 0;
+    for (int b = 0; b < B; ++b) {
+        const float q = pmf[base + b] * float(B);
+        table[base + b] = make_float2(q, __int_as_float(b));
+        if (q < 1.0f) w[ns++] = b;
+        else w[B - 1 - nl++] = b;
+    }
+    while (ns > 0 && nl > 0) {
+        const int sm = w[--ns];
+        const int lg = w[B - nl];
+        const float qs = table[base + sm].x;
+        table[base + sm] = make_float2(qs, __int_as_float(lg));
+        const float ql = (table[base + lg].x + qs) - 1.0f;
+        table[base + lg].x = ql;
+        if (ql < 1.0f) {
+            --nl;
+            w[ns++] = lg;
+        }
+    }
+    while (nl > 0) { const int lg = w[B - nl--]; table[base + lg] = make_float2(1.0f, __int_as_float(lg)); }
+    while (ns > 0) { const int sm = w[--ns]; table[base + sm] = make_float2(1.0f, __int_as_float(sm)); }
+}
+void launch_lc_build(const DevLightCache& lc, const float* light_pmf, float eps, int cells, float* pmf, float2* table,
+                     int* work, hipStream_t st) {
+    hipLaunchKernelGGL(k_lc_build, dim3((cells + 255) / 256), dim3(256), 0, st, lc, light_pmf, eps, cells, pmf, table,
+                       work);
+}
+
 // Test batches through the renderer's own device functions: camera rays
 // (ThinLensCamera::generate_ray), light selection (pick_light), HG sampling,
 // and film samples with given radiance for k_accumulate.

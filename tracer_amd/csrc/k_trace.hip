@@ -196,6 +196,12 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(TraceScene sc, DevPaths 
                 const int ri = p.sh_rec[i];
                 if (ri >= 0) p.rec_aux[2 * ri + 1] = make_float4(0.0f, 0.0f, 0.0f, 2.0f);
             }
+            if (p.lc.train) {
+                // light-cache training: the unoccluded sample's contribution / density
+                const float4 g = p.sh_g[i];
+                const int cell = __float_as_int(g.x);
+                if (cell >= 0 && g.z > 0.0f) lc_record(p.lc, cell, __float_as_int(g.y), g.z);
+            }
             const float4 c = p.sh_c[i];
             const int slot = __float_as_int(c.w);
             if (atomic_add) {

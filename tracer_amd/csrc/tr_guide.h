@@ -180,6 +180,20 @@ TR_HD f3 mixture_sample(const Mixture& m, float u0, float u1, float u2) {
     return normalize(t * (r * cp) + b * (r * sp) + m.mu[k] * wz);
 }
 
+// The mixture folded into the hemisphere around the unit vector n: a sample
+// below the plane of n is mirrored about it, so none is wasted where a
+// reflecting surface scatters nothing.  The folded density is q(w) + q(w'),
+// w' the mirror image of w, above the plane and 0 below.
+TR_HD f3 fold_to_hemisphere(f3 w, f3 n) {
+    const float c = dot(w, n);
+    return c < 0.0f ? w - n * (2.0f * c) : w;
+}
+TR_HD float mixture_pdf_folded(const Mixture& m, f3 w, f3 n) {
+    const float c = dot(w, n);
+    if (c < 0.0f) return 0.0f;
+    return mixture_pdf(m, w) + mixture_pdf(m, w - n * (2.0f * c));
+}
+
 // Loss -weight * log(q + q_eps) of one training record and its gradient with
 // respect to the head outputs.  Its expectation over directions sampled with
 // pdf p and weight F / p is the cross entropy of the mixture against the

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include "tr_lightcache.h"
 #include "tr_scene.h"
 
 namespace trhip {
@@ -18,7 +19,7 @@ struct DevPaths {
     float4* ray_d;    // (direction, tmax)
     float4* hit;      // (t, leaf-ordered triangle, b1, b2)
     float4* thr;      // (throughput rgb, pdf of the sampled direction; 0 = camera ray)
-    float4* rad;      // (radiance rgb, -)
+    float4* rad;      // (radiance rgb, cell of the vertex that sampled the ray as int bits: -1 = none)
     uint2* rng;       // PCG32 state
     uint2* info;      // (pixel, depth | scatter count << 8)
     int* queue[2];    // active path slots for the current / next bounce
@@ -43,6 +44,9 @@ struct DevPaths {
     unsigned int* rec_count;
     int rec_cap;
     int* sh_rec;              // light-sample record of each shadow ray, -1 = none
+    // learned light-sample distribution (guided passes; lc.grid == 0: off)
+    DevLightCache lc;
+    float4* sh_g;             // training passes: (cell, bin as int bits, contribution / density, -) per shadow ray
 };
 
 // The queue counters sit 1 KB apart: appends to different queues then go to
@@ -148,6 +152,9 @@ void launch_env_sample(const DevScene& s, const float* u, int64_t n, float* wi, 
 void launch_env_eval(const DevScene& s, const float* d, int64_t n, float* L, float* pdf, hipStream_t st);
 void launch_grid_density(const DevScene& s, int medium, const float* x, int64_t n, float* out, hipStream_t st);
 void launch_tex_eval(const DevScene& s, int texture, const float* uv, int64_t n, float* out, hipStream_t st);
+// Light-cache sampling tables (pmf, alias table) of every cell from the training sums.
+void launch_lc_build(const DevLightCache& lc, const float* light_pmf, float eps, int cells, float* pmf, float2* table,
+                     int* work, hipStream_t st);
 void launch_camera_rays(const DevScene& s, const float* film_uv, const float* lens_uv, int64_t n, float4* rays,
                         hipStream_t st);
 void launch_light_pick(const DevScene& s, const float* u, int64_t n, int* idx, float* pmf, hipStream_t st);
