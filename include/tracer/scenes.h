@@ -66,5 +66,21 @@ void trs_destroy(trs_scene* s);
 
 #ifdef __cplusplus
 }
+
+#include "tracer/tracer.hpp"
+
+namespace tracer {
+namespace scenes {
+/* Configuration 1..5 as the factory API builds it: the scene objects (with
+ * camera, lights and environment set) that trs_create flattens, the film
+ * filter and the render defaults. */
+struct BuiltScene {
+    RC<Scene> scene;
+    RC<Filter> filter;
+    int width = 0, height = 0, spp = 0, min_depth = 0, max_depth = 0;
+};
+BuiltScene build_config(int config, uint64_t seed, double detail);
+}  // namespace scenes
+}  // namespace tracer
 #endif
 #endif /* TRACER_SCENES_H */

@@ -260,11 +260,12 @@ struct MaterialTexture {
     bool has_emission = false;
     bool as_glass = false;
 };
-// image_file.cpp:139-166 for materials given by constant colours: Ka, Kd, Ks,
-// Ns and Ke become constant textures, has_emission is set when Ke > 0, and
-// ior > 1 makes a glass (Ks, or white when Ks is black, as reflectance and
-// transmittance).  A material naming an image file throws
-// std::runtime_error: image textures are not supported by the GPU path.
+// image_file.cpp:139-166: Ka, Kd, Ks, Ns and Ke become constant textures,
+// map_Kd an image texture (8-bit maps decoded and scaled by Kd as the
+// reference does), has_emission is set when Ke > 0, and ior > 1 makes a glass
+// (Ks, or white when Ks is black, as reflectance and transmittance).  Image
+// maps other than map_Kd throw std::runtime_error: the GPU path samples image
+// textures for the diffuse colour only.
 MaterialTexture create_texture_from_file(const material_t& material);
 
 class Shape {
@@ -395,13 +396,14 @@ struct GpuRenderOptions {
     uint64_t seed = 0;
     int samples_per_pass = 0;        // 0 = automatic
     int shadow_transmittance = 0;    // 0 = closed form, 1 = ratio tracking
-    int guide = 0;                   // 1 = learned path guiding
+    int guide = 0;                   // 1 = learned path guiding, 2 = learned light sampling alone
     int guide_train_passes = 8;
     float guide_fraction = 0.5f;
     float guide_lr = 1e-3f;
     float guide_train_fraction = 0.0f;      // guide probability during training passes
     int guide_max_depth = 0;                // 0 = guide every depth
     int tail_paths = 0;                     // plain passes: one kernel finishes the last paths (0 = 131072, < 0 = off)
+    int guide_light_grid = 0;               // learned light sampling: cells per axis (guide 1 or 2), 0 = off
     tr_render_stats* stats_out = nullptr;   // optional
 };
 RC<Renderer> create_pt_renderer(const PTRendererParams& params, const GpuRenderOptions& gpu);

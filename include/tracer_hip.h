@@ -177,7 +177,8 @@ typedef struct tr_render_params {
     int32_t spp;
     int32_t min_depth;                 /* Russian roulette (survival 0.9) when depth > min_depth */
     int32_t max_depth;
-    int32_t direct_light_sample_num;   /* light samples per vertex (reference default 1) */
+    int32_t direct_light_sample_num;   /* light samples per vertex, averaged (reference default 1; 0 = 1;
+                                          > 1 with guide = 0 only, at most 64) */
     int32_t filter_type;               /* TR_FILTER_* (reference: Gaussian only) */
     float filter_radius;               /* pixels, in (0, 16] */
     float filter_alpha;
@@ -252,6 +253,11 @@ tr_status tr_scene_upload(tr_ctx* ctx, const tr_scene_desc* scene);
 /* BVHAccel::build (src/accelerator/bvh.cpp:78-105): host SAH build, 4-wide
  * collapse, upload.  Requires tr_scene_upload first. */
 tr_status tr_bvh_build(tr_ctx* ctx, const tr_bvh_opts* opts);
+/* Scene::set_camera (src/core/scene.hpp:15-31) for an uploaded scene: the
+ * camera of later renders and camera batches, without rebuilding the BVH.
+ * TR_ERR_INVALID for a degenerate camera (position = target, up parallel to
+ * the view direction, fov outside (0, pi), non-finite values). */
+tr_status tr_set_camera(tr_ctx* ctx, const tr_camera* camera);
 /* Number of nodes / bytes of the device BVH and triangle arrays. */
 tr_status tr_bvh_info(tr_ctx* ctx, int64_t* num_nodes, int64_t* node_bytes, int64_t* tri_bytes);
 

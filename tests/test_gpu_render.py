@@ -93,11 +93,11 @@ def test_ray_counts(gpu_ctx, c1_scene):
 
 
 def _compare_estimators(gpu_ctx, scene, W, H, gpu_spp, cpu_spp, seeds, tile, quirks, nee, max_depth=None, z_max=5.0,
-                        rel_max=0.02):
+                        rel_max=0.02, **params):
     from oracle import refcpu
     gpu_ctx.load_scene(scene)
     ref = refcpu.RefScene(scene, 3)
-    extra = {} if max_depth is None else {"max_depth": max_depth}
+    extra = dict(params) if max_depth is None else {"max_depth": max_depth, **params}
 
     def gpu_render(seed):
         sums, _, st = gpu_ctx.render(scene.render_params(width=W, height=H, spp=gpu_spp, seed=seed, **extra),

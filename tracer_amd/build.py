@@ -81,6 +81,19 @@ def build_facade() -> Path | None:
     return out
 
 
+def build_harness() -> Path | None:
+    """The compiled drop-in harness of the facade tests (tests/cpp/facade_harness.cpp)."""
+    src = ROOT / "tests" / "cpp" / "facade_harness.cpp"
+    if not src.exists():
+        return None
+    out_dir = ROOT / "tests" / "_build"
+    out_dir.mkdir(parents=True, exist_ok=True)
+    out = out_dir / "facade_harness"
+    _run(["g++", "-O2", "-std=c++17", f"-I{ROOT / 'include'}", str(src), "-o", str(out), f"-L{LIB}", "-ltracer",
+          "-ltracer_hip", "-Wl,-rpath,$ORIGIN/../../tracer_amd/lib"])
+    return out
+
+
 def build_oracle() -> None:
     _run(["make", "-s", "-C", str(ROOT / "oracle"), "-j4"])
     if Path("/root/reference/dep/pcg_random.hpp").exists():
@@ -90,6 +103,7 @@ def build_oracle() -> None:
 def build_all() -> None:
     build_hip()
     build_facade()
+    build_harness()
     build_oracle()
 
 

@@ -316,6 +316,42 @@ std::vector<int> light_cutpoints(const std::vector<float>& cdf) {
     return g;
 }
 
+// ThinLensCamera (thin_lens_camera.cpp:23-36) with look_at's basis
+// (transform.cpp:96-124).
+DevCamera device_camera(const tr_camera& cm) {
+    DevCamera d{};
+    const f3 pos = mk(cm.pos[0], cm.pos[1], cm.pos[2]);
+    const f3 dir = normalize(mk(cm.target[0], cm.target[1], cm.target[2]) - pos);
+    const f3 right = normalize(cross(normalize(mk(cm.up[0], cm.up[1], cm.up[2])), dir));
+    d.pos = pos;
+    d.dir = dir;
+    d.right = right;
+    d.up = cross(dir, right);
+    d.focal = cm.focal_distance > 0.0f ? cm.focal_distance : 1.0f;
+    d.film_h = 2.0f * d.focal * std::tan(cm.fov_y * 0.5f);
+    d.film_w = (cm.aspect > 0.0f ? cm.aspect : 1.0f) * d.film_h;
+    d.lens_radius = cm.lens_radius;
+    return d;
+}
+
+void check_camera(const tr_camera& cm) {
+    const float* v[3] = {cm.pos, cm.target, cm.up};
+    for (const float* a : v)
+        for (int k = 0; k < 3; ++k)
+            if (!std::isfinite(a[k])) throw InvalidArg("camera vectors must be finite");
+    const double dx = double(cm.target[0]) - cm.pos[0], dy = double(cm.target[1]) - cm.pos[1],
+                 dz = double(cm.target[2]) - cm.pos[2];
+    const double dl = std::sqrt(dx * dx + dy * dy + dz * dz);
+    const double ul = std::sqrt(double(cm.up[0]) * cm.up[0] + double(cm.up[1]) * cm.up[1] + double(cm.up[2]) * cm.up[2]);
+    if (!(dl > 0.0) || !(ul > 0.0)) throw InvalidArg("camera target must differ from its position and up must not be zero");
+    const double cx = cm.up[1] * dz - cm.up[2] * dy, cy = cm.up[2] * dx - cm.up[0] * dz, cz = cm.up[0] * dy - cm.up[1] * dx;
+    if (!(std::sqrt(cx * cx + cy * cy + cz * cz) > 1e-6 * dl * ul)) throw InvalidArg("camera up is parallel to its view direction");
+    if (!(cm.fov_y > 0.0f && cm.fov_y < 3.14159f)) throw InvalidArg("camera fov_y must lie in (0, pi)");
+    if (!(cm.lens_radius >= 0.0f && std::isfinite(cm.lens_radius))) throw InvalidArg("camera lens radius must be >= 0");
+    if (!(cm.focal_distance >= 0.0f && std::isfinite(cm.focal_distance)) || !(cm.aspect >= 0.0f && std::isfinite(cm.aspect)))
+        throw InvalidArg("camera focal distance and aspect must be finite and >= 0");
+}
+
 void build_device_scene(tr_ctx* c, const tr_bvh_opts* opts) {
     HostScene& h = c->hs;
     const int64_t nt = int64_t(h.mat.size());
@@ -534,30 +570,19 @@ void build_device_scene(tr_ctx* c, const tr_bvh_opts* opts) {
     s.sun_E = mk(h.sun_E[0], h.sun_E[1], h.sun_E[2]);
     s.wcenter = wc;
     s.wradius = wr;
-    // camera (thin_lens_camera.cpp:23-36, transform.cpp:96-124)
-    const tr_camera& cm = h.cam;
-    const f3 pos = mk(cm.pos[0], cm.pos[1], cm.pos[2]);
-    const f3 dir = normalize(mk(cm.target[0], cm.target[1], cm.target[2]) - pos);
-    const f3 right = normalize(cross(normalize(mk(cm.up[0], cm.up[1], cm.up[2])), dir));
-    const f3 up = cross(dir, right);
-    s.cam.pos = pos;
-    s.cam.dir = dir;
-    s.cam.right = right;
-    s.cam.up = up;
-    s.cam.focal = cm.focal_distance > 0.0f ? cm.focal_distance : 1.0f;
-    s.cam.film_h = 2.0f * s.cam.focal * std::tan(cm.fov_y * 0.5f);
-    s.cam.film_w = (cm.aspect > 0.0f ? cm.aspect : 1.0f) * s.cam.film_h;
-    s.cam.lens_radius = cm.lens_radius;
+    s.cam = device_camera(h.cam);
     c->have_bvh = true;
 }
 
-void ensure_paths(tr_ctx* c, int64_t P) {
+// Path slots for P paths and shadow slots for nls light samples each.
+void ensure_paths(tr_ctx* c, int64_t P, int nls = 1) {
+    const size_t ns = size_t(P) * size_t(nls);
+    c->sh_o.reserve(ns * 16); c->sh_d.reserve(ns * 16); c->sh_c.reserve(ns * 16);
     if (P <= c->path_capacity) return;
     const size_t n = size_t(P);
     c->ray_o.reserve(n * 16); c->ray_d.reserve(n * 16); c->hit.reserve(n * 16);
     c->thr.reserve(n * 16); c->rad.reserve(n * 16); c->rng.reserve(n * 8); c->info.reserve(n * 8);
     c->q0.reserve(n * 4); c->q1.reserve(n * 4);
-    c->sh_o.reserve(n * 16); c->sh_d.reserve(n * 16); c->sh_c.reserve(n * 16);
     c->path_capacity = P;
 }
 
@@ -567,8 +592,8 @@ void ensure_paths(tr_ctx* c, int64_t P) {
 // pass holds as many paths as memory allows up to 2^26 (a 1024x1024 frame of
 // 64 spp in one pass). Path slots may use a quarter of the free device memory
 // plus what this context already holds.
-int auto_samples_per_pass(tr_ctx* c, int64_t npix, bool guided) {
-    const int64_t slot_bytes = 5 * 16 + 2 * 8 + 2 * 4 + 3 * 16 + (guided ? kGuideOut * 4 + 2 * 4 : 0);
+int auto_samples_per_pass(tr_ctx* c, int64_t npix, bool guided, int nls = 1) {
+    const int64_t slot_bytes = 5 * 16 + 2 * 8 + 2 * 4 + 3 * 16 * nls + (guided ? kGuideOut * 4 + 2 * 4 : 0);
     size_t free_b = 0, total_b = 0;
     hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
     const int64_t held = guided ? std::min(c->path_capacity, c->guide.path_capacity) : c->path_capacity;
@@ -822,6 +847,16 @@ tr_status tr_scene_upload(tr_ctx* c, const tr_scene_desc* d) {
             }
         c->have_scene = true;
         c->have_bvh = false;
+    });
+}
+
+tr_status tr_set_camera(tr_ctx* c, const tr_camera* camera) {
+    return guarded(c, [&] {
+        if (!camera) throw InvalidArg("camera is NULL");
+        if (!c->have_scene) throw StateError("tr_set_camera before tr_scene_upload");
+        check_camera(*camera);
+        c->hs.cam = *camera;
+        if (c->have_bvh) c->ds.cam = device_camera(*camera);
     });
 }
 
@@ -1205,7 +1240,11 @@ tr_status tr_render(tr_ctx* c, const tr_render_params* prm, double* film_sums, i
         const tr_render_params& P = *prm;
         if (P.width <= 0 || P.height <= 0 || P.spp <= 0) throw InvalidArg("width, height and spp must be positive");
         if (P.max_depth < 0 || P.min_depth < 0) throw InvalidArg("depths must be >= 0");
-        if (P.direct_light_sample_num > 1) throw InvalidArg("direct_light_sample_num > 1 is not supported (use 1)");
+        if (P.direct_light_sample_num < 0 || P.direct_light_sample_num > 64)
+            throw InvalidArg("direct_light_sample_num must lie in [0, 64] (0 = 1)");
+        const int nls = std::max(1, P.direct_light_sample_num);
+        if (nls > 1 && P.guide != 0)
+            throw InvalidArg("direct_light_sample_num > 1 is supported by the plain path tracer (guide = 0)");
         check_filter(P);
         if (P.guide < 0 || P.guide > 2) throw InvalidArg("guide must be 0, 1 or 2");
         if (P.guide == 2 && P.guide_light_grid <= 0)
@@ -1217,11 +1256,11 @@ tr_status tr_render(tr_ctx* c, const tr_render_params* prm, double* film_sums, i
         const int64_t npix = int64_t(P.width) * P.height;
         if (npix > (int64_t(1) << 28)) throw InvalidArg("image too large");
         int spp_pp = P.samples_per_pass;
-        if (spp_pp <= 0) spp_pp = auto_samples_per_pass(c, npix, P.guide == 1);
+        if (spp_pp <= 0) spp_pp = auto_samples_per_pass(c, npix, P.guide == 1, nls);
         spp_pp = std::min(spp_pp, P.spp);
         const int64_t maxP = npix * spp_pp;
         if (maxP > (int64_t(1) << 30)) throw InvalidArg("too many paths per pass");
-        ensure_paths(c, maxP);
+        ensure_paths(c, maxP, nls);
         c->film.reserve(size_t(npix) * sizeof(double4));
         c->film_w = P.width;
         c->film_h = P.height;
@@ -1245,7 +1284,7 @@ tr_status tr_render(tr_ctx* c, const tr_render_params* prm, double* film_sums, i
         rc.min_depth = P.min_depth; rc.max_depth = P.max_depth;
         rc.filter = P.filter_type; rc.filter_radius = P.filter_radius; rc.filter_alpha = P.filter_alpha;
         rc.shadow_tr = P.shadow_transmittance;
-        rc.nls = 1;
+        rc.nls = nls;
         rc.seed = P.seed;
         const float guide_frac = guided ? (P.guide_fraction > 0.0f ? P.guide_fraction : 0.5f) : 0.0f;
         if (guided && !(P.guide_train_fraction >= 0.0f && P.guide_train_fraction < 1.0f))
@@ -1259,7 +1298,7 @@ tr_status tr_render(tr_ctx* c, const tr_render_params* prm, double* film_sums, i
         // plain passes finish their last bounces in one kernel once at most
         // this many paths are alive
         const int tail = P.tail_paths == 0 ? 131072 : P.tail_paths;
-        const bool use_tail = tail > 0 && !guided && P.profile != 2;
+        const bool use_tail = tail > 0 && !guided && P.profile != 2 && nls == 1;
         std::vector<std::pair<size_t, int>> marks;   // (event index, kind)
         hipEvent_t e0 = ev(c, evi++), e1 = ev(c, evi++);
         hip_check(hipEventRecord(e0, st), "event");
@@ -1302,7 +1341,8 @@ tr_status tr_render(tr_ctx* c, const tr_render_params* prm, double* film_sums, i
                 timed(0, [&] { launch_extend(c->ds.tr, p, cur, num_paths, P.profile == 2, st); });
                 if (guided && rc.guide_frac > 0.0f) timed(4, [&] { guide_infer(c, p, cur, num_paths); });
                 timed(1, [&] { launch_shade(c->ds, p, rc, cur, num_paths, guided, st); });
-                timed(2, [&] { launch_shadow(c->ds.tr, p, num_paths, 0, rc.guide_record, st); });
+                if (nls > 1) timed(2, [&] { launch_shadow_multi(c->ds.tr, p, cur, num_paths, nls, st); });
+                else timed(2, [&] { launch_shadow(c->ds.tr, p, num_paths, 0, rc.guide_record, st); });
                 ++launches;
                 cur = 1 - cur;
             }

@@ -192,11 +192,20 @@ TR_D float nee_select_pdf(const DevScene& s, const DevPaths& p, int prev_cell, i
     return light >= s.num_area_lights ? s.light_pmf[s.num_area_lights] : s.lights[4 * light + 1].w;
 }
 
+// A light sample of a vertex with several (direct_light_sample_num > 1),
+// in its fixed shadow slot e; w of the contribution = 0 until the shadow
+// kernel finds the segment unoccluded.
+TR_D void put_shadow(const DevPaths& p, size_t e, const ShadowOut& sh) {
+    p.sh_o[e] = f4(sh.o, sh.tmin);
+    p.sh_d[e] = f4(sh.d, sh.tmax);
+    p.sh_c[e] = f4(sh.contrib, 0.0f);
+}
+
 // One vertex of the path in `slot`: reads its state and hit, adds emission,
 // samples the continuation and a light, and stores the updated state.
 // EXT: the scene has an image environment or textured materials (their
 // code is compiled in only then).
-template <bool GUIDE, bool EXT, int LCM>
+template <bool GUIDE, bool EXT, int LCM, bool MULTI = false>
 TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c, int slot, ShadeOut& out) {
     bool& cont = out.cont;
     ShadowOut& sh = out.sh;
@@ -212,6 +221,12 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
     out.lc_cell = out.lc_bin = -1;
     out.lc_val = 0.0f;
     int next_cell = -1;   // cell of this vertex for the next hit's MIS weight (guided passes with a light cache)
+    // direct_light_sample_num light samples per vertex (MULTI): sample k goes to
+    // shadow slot slot * nls + k, marked unused (tmax < 0) unless it carries light
+    const int nls = MULTI ? c.nls : 1;
+    const float fnls = float(nls);
+    if (MULTI)
+        for (int k = 0; k < nls; ++k) p.sh_d[size_t(slot) * nls + k] = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
     float4 next_o = make_float4(0, 0, 0, 0), next_d = make_float4(0, 0, 0, 0);
     const float4 ro = p.ray_o[slot], rd = p.ray_d[slot], h = p.hit[slot];
     float4 tp = p.thr[slot];
@@ -236,7 +251,7 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
             const f3 Le = env_eval<EXT>(s, d, epdf);
             float w = 1.0f;
             if (prev_pdf > 0.0f)
-                w = power_heuristic(prev_pdf, nee_select_pdf<LCM>(s, p, prev_cell, s.num_area_lights, 0.0f, 0.0f) * epdf);
+                w = power_heuristic(prev_pdf, nee_select_pdf<LCM>(s, p, prev_cell, s.num_area_lights, 0.0f, 0.0f) * epdf * fnls);
             L += T * Le * w;
         }
         alive = false;
@@ -305,7 +320,7 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
             ++scatter;
             const f3 wo = -d;
             if (lc_use<LCM>(p)) next_cell = lc_cell(p.lc, sp_pos);
-            if (s.num_light_entries > 0) {
+            auto nee_medium = [&](int k) {
                 float pmf, lu, lv;
                 int lcell, lbin;
                 const int e = nee_pick<LCM>(s, p, sp_pos, next_cell, rng, lu, lv, pmf, lcell, lbin);
@@ -314,23 +329,31 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
                     const float ph = hg_phase(m.g, -dot(ls.wi, wo));
                     float w = 1.0f;
                     float lp = pmf;
-                    if (ls.pdf > 0.0f) { lp *= ls.pdf; w = power_heuristic(lp, ph); }
+                    if (ls.pdf > 0.0f) { lp *= ls.pdf; w = power_heuristic(lp * fnls, ph); }
                     f3 trs = f3s(1.0f);
                     if (ls.area) trs = segment_tr<EXT>(s, m, sp_pos, ls.wi, ls.dist, c.shadow_tr, rng);
-                    const f3 contrib = T * ls.L * trs * (ph * w / lp);
+                    const f3 contrib = T * ls.L * trs * (ph * w / (lp * fnls));
                     if (max3(contrib) > 0.0f && finite3(contrib)) {
                         if (LCM >= 2 && lbin >= 0) {
                             out.lc_cell = lcell;
                             out.lc_bin = lbin;
                             out.lc_val = fminf(lum(ls.L * trs) / lp, kLcMaxValue);
                         }
-                        sh.emit = true;
+                        sh.emit = !MULTI;
                         sh.o = sp_pos;
                         sh.tmin = kEps;
                         sh.d = ls.wi;
                         sh.tmax = ls.area ? ls.dist - kEps : ls.dist;
                         sh.contrib = contrib;
+                        if (MULTI) put_shadow(p, size_t(slot) * nls + k, sh);
                     }
+                }
+            };
+            if (s.num_light_entries > 0) {
+                if constexpr (MULTI) {
+                    for (int k = 0; k < nls; ++k) nee_medium(k);
+                } else {
+                    nee_medium(0);
                 }
             }
             float ppdf;
@@ -349,7 +372,7 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
                 const float4 l0 = s.lights[4 * sr.light];
                 float w = 1.0f;
                 if (prev_pdf > 0.0f) {
-                    const float lp = nee_select_pdf<LCM>(s, p, prev_cell, sr.light, h.z, h.w) * t * t / (l0.w * (-dn));
+                    const float lp = nee_select_pdf<LCM>(s, p, prev_cell, sr.light, h.z, h.w) * t * t / (l0.w * (-dn)) * fnls;
                     w = power_heuristic(prev_pdf, lp);
                 }
                 L += T * xyz(l0) * w;
@@ -414,7 +437,7 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
                     // next-event estimation (direct_illumination.cpp:35-109); none at a
                     // specular vertex (path_trace_renderer.cpp:141)
                     if (lc_use<LCM>(p)) next_cell = lc_cell(p.lc, pos);
-                    if (!delta && s.num_light_entries > 0) {
+                    auto nee_surface = [&](int k) {
                         float pmf, lu, lv;
                         int lcell, lbin;
                         const int e = nee_pick<LCM>(s, p, pos, next_cell, rng, lu, lv, pmf, lcell, lbin);
@@ -431,7 +454,7 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
                                         const float sp = use_guide ? frac * mixture_pdf_folded(mix, ls.wi, nsd) +
                                                                          (1.0f - frac) * be.pdf
                                                                    : be.pdf;
-                                        w = power_heuristic(lp, sp);
+                                        w = power_heuristic(lp * fnls, sp);
                                     }
                                     f3 trs = f3s(1.0f);
                                     if (ls.area) {
@@ -439,19 +462,20 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
                                         if (lm >= 0) trs = segment_tr<EXT>(s, s.media[lm], pos, ls.wi, ls.dist, c.shadow_tr, rng);
                                     }
                                     const float cosg = fabsf(dot(ls.wi, ng));
-                                    const f3 contrib = T * ls.L * be.f * trs * (cosg * w / lp);
+                                    const f3 contrib = T * ls.L * be.f * trs * (cosg * w / (lp * fnls));
                                     if (max3(contrib) > 0.0f && finite3(contrib)) {
                                         if (LCM >= 2 && lbin >= 0) {
                                             out.lc_cell = lcell;
                                             out.lc_bin = lbin;
                                             out.lc_val = fminf(lum(ls.L * trs) / lp, kLcMaxValue);
                                         }
-                                        sh.emit = true;
+                                        sh.emit = !MULTI;
                                         sh.o = pos;
                                         sh.tmin = kEps;
                                         sh.d = ls.wi;
                                         sh.tmax = dist;
                                         sh.contrib = contrib;
+                                        if (MULTI) put_shadow(p, size_t(slot) * nls + k, sh);
                                         if (rec_path) {
                                             // light-sample record: MIS-weighted f |cos| L / p_light,
                                             // kept if the shadow ray reaches the light
@@ -465,6 +489,13 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
                                     }
                                 }
                             }
+                        }
+                    };
+                    if (!delta && s.num_light_entries > 0) {
+                        if constexpr (MULTI) {
+                            for (int k = 0; k < nls; ++k) nee_surface(k);
+                        } else {
+                            nee_surface(0);
                         }
                     }
                 }
@@ -535,7 +566,7 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
 // with the combined pdf.  A vertex
 // whose head outputs are not finite uses the plain BSDF estimator, drawing
 // the same random numbers as the plain kernel.
-template <bool GUIDE, bool EXT, int LCM>
+template <bool GUIDE, bool EXT, int LCM, bool MULTI = false>
 TR_D void shade_queue(const DevScene& s, const DevPaths& p, const RenderConsts& c, int cur) {
     __shared__ BlockAppendLds<kShadeBlock> app;
     const unsigned int n = *queue_counter(p, cur);
@@ -556,7 +587,7 @@ TR_D void shade_queue(const DevScene& s, const DevPaths& p, const RenderConsts& 
         int slot = 0;
         if (active) {
             slot = q[i];
-            shade_path<GUIDE, EXT, LCM>(s, p, c, slot, o);
+            shade_path<GUIDE, EXT, LCM, MULTI>(s, p, c, slot, o);
         }
         const bool cont = o.cont, g_used = o.g_used, g_fallback = o.g_fallback;
         const bool want_rec = o.want_rec, want_nee_rec = o.want_nee_rec;
@@ -621,6 +652,12 @@ template <bool GUIDE, bool EXT, int LCM>
 __global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene s, DevPaths p, RenderConsts c, int cur) {
     shade_queue<GUIDE, EXT, LCM>(s, p, c, cur);
 }
+// The plain renderer with direct_light_sample_num > 1 light samples per vertex.
+template <bool EXT>
+__global__ __launch_bounds__(kShadeBlock) void k_shade_multi(DevScene s, DevPaths p, RenderConsts c, int cur) {
+    shade_queue<false, EXT, 0, true>(s, p, c, cur);
+}
+
 // Plain scenes (no image environment, textures or grids) keep three waves
 // per SIMD, also when the light cache adds its few registers.
 template <int LCM>
@@ -846,6 +883,11 @@ void launch_shade(const DevScene& s, const DevPaths& p, const RenderConsts& c, i
                   hipStream_t st) {
     const dim3 g(shade_blocks(max_paths)), b(kShadeBlock);
     const bool img = s.env.w > 0 || s.has_tex || s.has_grid;
+    if (c.nls > 1) {
+        if (img) hipLaunchKernelGGL((k_shade_multi<true>), g, b, 0, st, s, p, c, cur);
+        else hipLaunchKernelGGL((k_shade_multi<false>), g, b, 0, st, s, p, c, cur);
+        return;
+    }
     if (guide) {
         if (img) hipLaunchKernelGGL((k_shade<true, true, 3>), g, b, 0, st, s, p, c, cur);
         else hipLaunchKernelGGL((k_shade<true, false, 3>), g, b, 0, st, s, p, c, cur);

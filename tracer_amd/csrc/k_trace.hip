@@ -12,7 +12,7 @@ namespace trhip {
 
 namespace {
 
-enum TraceKernel { TK_CLOSEST, TK_ANY, TK_COUNTED, TK_EXTEND, TK_EXTEND_COUNT, TK_SHADOW, TK_NUM };
+enum TraceKernel { TK_CLOSEST, TK_ANY, TK_COUNTED, TK_EXTEND, TK_EXTEND_COUNT, TK_SHADOW, TK_SHADOW_MULTI, TK_NUM };
 
 // The trace kernels loop over their rays with a grid stride, so every wave
 // gets an equal share.  A grid that is not a whole number of the kernel's
@@ -217,6 +217,61 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(TraceScene sc, DevPaths 
     }
 }
 
+// Shadow rays of vertices with several light samples (direct_light_sample_num
+// > 1): sample k of the path in queue entry i sits at slot * nls + k; unused
+// samples have tmax < 0.  An unoccluded sample sets w of its contribution to 1;
+// k_gather_multi then adds the path's samples in order, so the film stays
+// deterministic.
+__global__ __launch_bounds__(kTraceBlock) void k_shadow_multi(TraceScene sc, DevPaths p, int cur, int nls) {
+    __shared__ TraceStack<kTraceBlock, kLdsStack> st;
+    const int tid = threadIdx.x;
+    const int gt = blockIdx.x * kTraceBlock + tid;
+    const int stride = gridDim.x * kTraceBlock;
+    const unsigned int n = *queue_counter(p, cur) * (unsigned int)nls;
+    const int* q = p.queue[cur];
+    WaveQueue wq(n);
+    TraceRay r;
+    bool busy = false, valid = false;
+    size_t e = 0;
+    while (true) {
+        wq.refill(busy, [&](unsigned int idx) {
+            e = size_t(q[idx / (unsigned int)nls]) * nls + idx % (unsigned int)nls;
+            const float4 a = p.sh_o[e], b = p.sh_d[e];
+            valid = b.w >= 0.0f;
+            r.init(xyz(a), a.w, xyz(b), valid ? b.w : -1.0f);
+        });
+        if (__ballot(busy) == 0ull) break;
+        if (!busy || r.step<true, false, kTraceBlock, kLdsStack>(sc, st, tid, p.ovf_node + gt, p.ovf_t + gt, stride,
+                                                                 nullptr, nullptr))
+            continue;
+        busy = false;
+        TraceHit h;
+        if (valid && !r.result(h)) p.sh_c[e].w = 1.0f;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_gather_multi(DevPaths p, int cur, int nls) {
+    const unsigned int n = *queue_counter(p, cur);
+    const int* q = p.queue[cur];
+    unsigned long long rays = 0;
+    for (unsigned int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int slot = q[i];
+        float ax = 0.0f, ay = 0.0f, az = 0.0f;
+        for (int k = 0; k < nls; ++k) {
+            const size_t e = size_t(slot) * nls + k;
+            if (!(p.sh_d[e].w >= 0.0f)) continue;
+            ++rays;
+            const float4 c = p.sh_c[e];
+            if (c.w > 0.5f) { ax += c.x; ay += c.y; az += c.z; }
+        }
+        float4 rad = p.rad[slot];
+        rad.x += ax; rad.y += ay; rad.z += az;
+        p.rad[slot] = rad;
+    }
+    for (int off = 32; off > 0; off >>= 1) rays += __shfl_xor(rays, off);
+    if (__lane_id() == 0 && rays) atomicAdd(&p.stats[1], rays);
+}
+
 namespace {
 
 // Blocks of a full-size launch of kernel k on the current device.
@@ -231,6 +286,7 @@ int grid_blocks(TraceKernel k) {
         case TK_COUNTED: b = resident_rounds(k_trace_counted); break;
         case TK_EXTEND: b = resident_rounds(k_extend<false>); break;
         case TK_EXTEND_COUNT: b = resident_rounds(k_extend<true>); break;
+        case TK_SHADOW_MULTI: b = resident_rounds(k_shadow_multi); break;
         default: b = resident_rounds(k_shadow); break;
     }
     g_grid[dev][k].store(b, std::memory_order_relaxed);
@@ -331,6 +387,13 @@ void launch_extend(const TraceScene& sc, const DevPaths& p, int cur, int max_pat
         hipLaunchKernelGGL(k_extend<true>, dim3(trace_blocks(TK_EXTEND_COUNT, max_paths, p.ovf_threads)), dim3(kTraceBlock), 0, st, sc, p, cur);
     else
         hipLaunchKernelGGL(k_extend<false>, dim3(trace_blocks(TK_EXTEND, max_paths, p.ovf_threads)), dim3(kTraceBlock), 0, st, sc, p, cur);
+}
+void launch_shadow_multi(const TraceScene& sc, const DevPaths& p, int cur, int max_paths, int nls, hipStream_t st) {
+    const int64_t n = int64_t(max_paths) * nls;
+    hipLaunchKernelGGL(k_shadow_multi, dim3(trace_blocks(TK_SHADOW_MULTI, n, p.ovf_threads)), dim3(kTraceBlock), 0, st, sc, p,
+                       cur, nls);
+    const int blocks = int(std::min<int64_t>((max_paths + 255) / 256, 4096));
+    hipLaunchKernelGGL(k_gather_multi, dim3(blocks > 0 ? blocks : 1), dim3(256), 0, st, p, cur, nls);
 }
 void launch_shadow(const TraceScene& sc, const DevPaths& p, int max_shadow, int accumulate_atomic, int record,
                    hipStream_t st) {

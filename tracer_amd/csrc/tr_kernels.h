@@ -131,6 +131,10 @@ void launch_extend(const TraceScene& sc, const DevPaths& p, int cur, int max_pat
 // record != 0: validate the light-sample training records of unoccluded shadow rays
 void launch_shadow(const TraceScene& sc, const DevPaths& p, int max_shadow, int accumulate_atomic, int record,
                    hipStream_t st);
+// direct_light_sample_num > 1: the light samples of queue cur's paths
+// (nls per path, at slot * nls + k), then their unoccluded contributions added
+// to the paths' radiance in sample order.
+void launch_shadow_multi(const TraceScene& sc, const DevPaths& p, int cur, int max_paths, int nls, hipStream_t st);
 // Threads of the largest trace-kernel grid on the current device: the spill
 // area holds one column per thread.
 int trace_grid_threads();

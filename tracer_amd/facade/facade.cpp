@@ -1,6 +1,7 @@
 // Implementation of the tracer:: factory API (include/tracer/tracer.hpp) and
 // of the GPU-backed PathTraceRenderer.
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -304,6 +305,13 @@ RC<Filter> create_box_filter(real radius) {
     f->r = radius;
     return f;
 }
+namespace detail {
+uint64_t next_version() {
+    static std::atomic<uint64_t> counter{0};
+    return ++counter;
+}
+}  // namespace detail
+
 RC<Scene> create_general_scene(const RC<Aggregate>& accel) {
     if (!std::dynamic_pointer_cast<BVHAccel>(accel)) throw std::runtime_error("create_general_scene: aggregate must come from create_bvh_accel");
     return newRC<GeneralScene>(accel);
@@ -556,7 +564,14 @@ FlatScene flatten(const Scene& scene) {
             f.sun_E[0] = dl->irradiance.r; f.sun_E[1] = dl->irradiance.g; f.sun_E[2] = dl->irradiance.b;
         }
     }
-    tr_camera& c = f.cam;
+    f.cam = camera_desc(scene);
+    return f;
+}
+
+tr_camera camera_desc(const Scene& scene) {
+    const auto* cam = dynamic_cast<const ThinLensCamera*>(scene.get_camera().get());
+    if (!cam) throw std::runtime_error("render: scene has no camera");
+    tr_camera c{};
     c.pos[0] = cam->pos.x; c.pos[1] = cam->pos.y; c.pos[2] = cam->pos.z;
     c.target[0] = cam->target.x; c.target[1] = cam->target.y; c.target[2] = cam->target.z;
     c.up[0] = cam->up.x; c.up[1] = cam->up.y; c.up[2] = cam->up.z;
@@ -564,7 +579,7 @@ FlatScene flatten(const Scene& scene) {
     c.lens_radius = cam->lens_radius;
     c.focal_distance = cam->focal_dist;
     c.aspect = cam->aspect;
-    return f;
+    return c;
 }
 
 }  // namespace detail
@@ -586,7 +601,11 @@ public:
             if (s != TR_OK) throw std::runtime_error(std::string(what) + ": " + (ctx ? tr_last_error(ctx) : "no context"));
         };
         if (!ctx) check(tr_create(&ctx, gpu.device), "tr_create");
-        if (&scene != last_scene) {
+        // The geometry is uploaded again for another scene object or a rebuilt
+        // aggregate, or when the light list or environment changed; the camera
+        // is sent with every render, so set_camera between renders takes effect.
+        const Key key = scene_key(scene);
+        if (!(key == last_key)) {
             FlatScene f = flatten(scene);
             const tr_scene_desc d = f.desc();
             check(tr_scene_upload(ctx, &d), "tr_scene_upload");
@@ -594,8 +613,10 @@ public:
             bo.max_leaf_primitives = std::min(std::max(f.max_leaf, 1), 15);
             bo.num_threads = 0;
             check(tr_bvh_build(ctx, &bo), "tr_bvh_build");
-            last_scene = &scene;
+            last_key = key;
         }
+        const tr_camera cam = camera_desc(scene);
+        check(tr_set_camera(ctx, &cam), "tr_set_camera");
         tr_render_params rp{};
         rp.width = film.width();
         rp.height = film.height();
@@ -624,6 +645,7 @@ public:
         rp.guide_train_fraction = gpu.guide_train_fraction;
         rp.guide_max_depth = gpu.guide_max_depth;
         rp.tail_paths = gpu.tail_paths;
+        rp.guide_light_grid = gpu.guide_light_grid;
         RenderTarget rt;
         rt.color = Image2D<Spectrum>(rp.width, rp.height);
         static_assert(sizeof(Spectrum) == 12, "Spectrum is three floats");
@@ -634,10 +656,26 @@ public:
     }
 
 private:
+    struct Key {
+        uint64_t scene = 0, accel = 0;
+        size_t lights = 0;
+        const void* env = nullptr;
+        bool operator==(const Key& o) const { return scene == o.scene && accel == o.accel && lights == o.lights && env == o.env; }
+    };
+    static Key scene_key(const Scene& s) {
+        Key k;
+        if (const auto* gs = dynamic_cast<const GeneralScene*>(&s)) {
+            k.scene = gs->uid;
+            if (const auto* b = dynamic_cast<const BVHAccel*>(gs->accel.get())) k.accel = b->version;
+        }
+        k.lights = s.lights.size();
+        k.env = s.environment_light.get();
+        return k;
+    }
     PTRendererParams params;
     GpuRenderOptions gpu;
     tr_ctx* ctx = nullptr;
-    const Scene* last_scene = nullptr;
+    Key last_key;
     std::mutex mtx;
 };
 

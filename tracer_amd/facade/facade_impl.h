@@ -122,12 +122,19 @@ public:
     const AreaLight* as_area_light() const noexcept override { return light.get(); }
 };
 
+// Process-wide counter for the version stamps below.
+uint64_t next_version();
+
 class BVHAccel : public Aggregate {
 public:
     explicit BVHAccel(int max_leaf) : max_leaf(max_leaf) {}
-    void build(std::vector<RC<Primitive>> p) override { prims = std::move(p); }
+    void build(std::vector<RC<Primitive>> p) override {
+        prims = std::move(p);
+        version = next_version();
+    }
     int max_leaf;
     std::vector<RC<Primitive>> prims;
+    uint64_t version = 0;   // changes with every build: renderers re-upload the geometry
 };
 
 class ThinLensCamera : public Camera {
@@ -150,12 +157,13 @@ public:
 
 class GeneralScene : public Scene {
 public:
-    explicit GeneralScene(RC<Aggregate> a) : accel(std::move(a)) {}
+    explicit GeneralScene(RC<Aggregate> a) : accel(std::move(a)), uid(next_version()) {}
     void set_camera(RC<const Camera> c) override { camera = std::move(c); }
     RC<const Camera> get_camera() const noexcept override { return camera; }
     void prepare_to_render() override {}
     RC<Aggregate> accel;
     RC<const Camera> camera;
+    const uint64_t uid;   // distinct for every scene object, also one built at a freed one's address
 };
 
 // A scene flattened to plain arrays; desc() points into them.
@@ -180,6 +188,8 @@ struct FlatScene {
 
 // Throws std::runtime_error for scene content the GPU path does not support.
 FlatScene flatten(const Scene& scene);
+// The scene's camera alone (a ThinLensCamera, else std::runtime_error).
+tr_camera camera_desc(const Scene& scene);
 
 }  // namespace detail
 }  // namespace tracer

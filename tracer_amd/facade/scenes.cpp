@@ -554,6 +554,32 @@ static void obj_scene(Config& c, const std::string& path, const tr_camera& tc, b
     finish(c, cam, nullptr);
 }
 
+static void build(Config& c, int config, uint64_t seed, double detail) {
+    if (!(detail > 0.0 && detail <= 1.0)) throw std::runtime_error("detail must lie in (0, 1]");
+    switch (config) {
+        case 1: config1(c, seed); break;
+        case 2: config2(c, seed, detail); break;
+        case 3: config3(c, seed, detail); break;
+        case 4: config4(c, seed, detail); break;
+        case 5: config5(c, seed, detail); break;
+        default: throw std::runtime_error("config must be 1..5");
+    }
+}
+
+BuiltScene build_config(int config, uint64_t seed, double detail) {
+    Config c;
+    build(c, config, seed, detail);
+    BuiltScene b;
+    b.scene = c.scene;
+    b.filter = c.filter;
+    b.width = c.width;
+    b.height = c.height;
+    b.spp = c.spp;
+    b.min_depth = c.min_depth;
+    b.max_depth = c.max_depth;
+    return b;
+}
+
 }  // namespace scenes
 }  // namespace tracer
 
@@ -571,16 +597,8 @@ extern "C" {
 trs_scene* trs_create(int32_t config, uint64_t seed, double detail, const char** err) {
     static thread_local std::string msg;
     try {
-        if (!(detail > 0.0 && detail <= 1.0)) throw std::runtime_error("detail must lie in (0, 1]");
         tracer::scenes::Config c;
-        switch (config) {
-            case 1: tracer::scenes::config1(c, seed); break;
-            case 2: tracer::scenes::config2(c, seed, detail); break;
-            case 3: tracer::scenes::config3(c, seed, detail); break;
-            case 4: tracer::scenes::config4(c, seed, detail); break;
-            case 5: tracer::scenes::config5(c, seed, detail); break;
-            default: throw std::runtime_error("config must be 1..5");
-        }
+        tracer::scenes::build(c, config, seed, detail);
         return make_trs(c, seed);
     } catch (const std::exception& e) {
         msg = e.what();

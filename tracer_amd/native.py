@@ -72,6 +72,7 @@ def hip_lib() -> C.CDLL:
         lib.tr_texture_eval_batch.argtypes = [VP, C.c_int32, P(C.c_float), I64, P(C.c_float)]
         F = P(C.c_float)
         lib.tr_camera_ray_batch.argtypes = [VP, F, F, I64, F]
+        lib.tr_set_camera.argtypes = [VP, P(abi.TrCamera)]
         lib.tr_light_pmf.argtypes = [VP, F, I64, P(I64)]
         lib.tr_light_pick_batch.argtypes = [VP, F, I64, P(C.c_int32), F]
         lib.tr_hg_sample_batch.argtypes = [VP, C.c_float, F, F, I64, F, F]
@@ -545,6 +546,10 @@ class Context:
 
 
     # ---- test batches through the renderer's device code ----------------
+    def set_camera(self, camera: abi.TrCamera):
+        """Scene::set_camera for the uploaded scene (tr_set_camera)."""
+        self._check(self.lib.tr_set_camera(self.h, C.byref(camera)))
+
     def camera_rays(self, film_uv: np.ndarray, lens_uv: np.ndarray) -> np.ndarray:
         """Camera rays of the loaded scene: film_uv, lens_uv (n, 2) -> (n, 8)."""
         f = np.ascontiguousarray(film_uv, np.float32).reshape(-1, 2)
