@@ -69,10 +69,31 @@ def cpu_model() -> str:
     return "unknown"
 
 
-def cpu_baseline(scene, threads: int, target_s: float = 15.0) -> dict:
+def cpu_topology() -> dict:
+    """Logical CPUs this process may run on, physical cores among them, SMT."""
+    allowed = sorted(os.sched_getaffinity(0))
+    cores = set()
+    for cpu in allowed:
+        try:
+            base = f"/sys/devices/system/cpu/cpu{cpu}/topology/"
+            pkg = open(base + "physical_package_id").read().strip()
+            core = open(base + "core_id").read().strip()
+            cores.add((pkg, core))
+        except OSError:
+            cores.add(("?", str(cpu)))
+    try:
+        smt = open("/sys/devices/system/cpu/smt/active").read().strip() == "1"
+    except OSError:
+        smt = len(cores) < len(allowed)
+    return {"model": cpu_model(), "logical_cpus_allowed": len(allowed), "physical_cores_allowed": len(cores),
+            "smt_active": smt, "os_cpu_count": os.cpu_count()}
+
+
+def cpu_baseline(scene, threads: int, target_s: float = 8.0, runs: int = 3) -> dict:
     """The oracle's CPU renderer (fixed mode: the reference's algorithm with its
-    estimator defects corrected) on a reduced film of the same scene, and the
-    time-to-equal-relMSE ratio of the plain GPU path on that film."""
+    estimator defects corrected) on a reduced film of the same scene, timed
+    `runs` times (the median is reported), and the time-to-equal-relMSE ratio
+    of the plain GPU path on that film."""
     from oracle import refcpu
     from tracer_amd import native
     t0 = time.time()
@@ -82,8 +103,14 @@ def cpu_baseline(scene, threads: int, target_s: float = 15.0) -> dict:
     probe = scene.render_params(width=W, height=H, spp=1, seed=99)
     _, ms, _ = ref.render(probe, quirks=False, nee=0, threads=threads)
     spp = max(1, min(64, int(target_s * 1000.0 / max(ms, 1.0))))
-    p = scene.render_params(width=W, height=H, spp=spp, seed=100)
-    cpu_sums, ms, rays = ref.render(p, quirks=False, nee=0, threads=threads)
+    timings = []
+    for r in range(runs):
+        p = scene.render_params(width=W, height=H, spp=spp, seed=100 + r)
+        sums_r, ms_r, rays_r = ref.render(p, quirks=False, nee=0, threads=threads)
+        timings.append(ms_r)
+        if r == 0:
+            cpu_sums, rays = sums_r, rays_r
+    ms = float(np.median(timings))
     samples = W * H * spp
     # GPU on the same film: a 16384-spp reference, then the plain path at 64 spp
     ctx = native.Context(0)
@@ -113,10 +140,12 @@ def cpu_baseline(scene, threads: int, target_s: float = 15.0) -> dict:
         "unit": "samples/s",
         "cores": threads,
         "kind": "port",
+        "runs_ms": timings,
+        "host": cpu_topology(),
         "sample": (f"CPU restatement of the reference renderer in fixed mode (all-lights NEE + MIS BSDF ray, binary SAH "
                    f"BVH leaf 3, 16x16 tiles) on configuration 2, {W}x{H} film at {spp} spp ({samples} samples, "
-                   f"{ms / 1e3:.1f} s, {(rays[0] + rays[1]) / (ms / 1e3) / 1e6:.1f} Mrays/s); BVH build "
-                   f"{build_s:.1f} s not timed; CPU: {cpu_model()}"),
+                   f"median of {runs} runs {ms / 1e3:.1f} s, {(rays[0] + rays[1]) / (ms / 1e3) / 1e6:.1f} Mrays/s) on "
+                   f"{threads} threads; BVH build {build_s:.1f} s not timed; CPU: {cpu_model()}"),
         "equal_relmse": {
             "film": f"{W}x{H}", "reference": "plain GPU path, 16384 spp, FP64 film",
             "cpu_spp": spp, "cpu_ms": ms, "cpu_relmse": cpu_err,
@@ -252,7 +281,7 @@ def other_configs() -> dict:
 
 def read_traffic(path: Path):
     try:
-        return json.loads(path.read_text()).get("bytes_per_launch")
+        return json.loads(path.read_text())
     except (OSError, ValueError):
         return None
 
@@ -477,7 +506,8 @@ def main() -> int:
     bytes_per_launch = bytes_per_ray * (tp["ext"] - tp["tail_ext"]) / launches
     ms_per_launch = tp["ms_extend"] / launches
     achieved = bytes_per_launch / (ms_per_launch / 1e3) / 1e9 if ms_per_launch > 0 else 0.0
-    traffic = read_traffic(ROOT / "profiles" / f"k_extend_c{args.config}_traffic.json")
+    traffic_info = read_traffic(ROOT / "profiles" / f"k_extend_c{args.config}_traffic.json")
+    traffic = traffic_info.get("bytes_per_launch") if traffic_info else None
     ctx.close()
 
     out = {
@@ -518,6 +548,16 @@ def main() -> int:
             "bytes_per_ray": bytes_per_ray,
             "launches": launches,
             "ms_per_launch": ms_per_launch,
+            # what the counters show (profiles/k_extend_c2_kernel_stats.md): the bytes that miss L2, over
+            # this run's launch time, against the HBM peak; the L2 hit rate; lanes active per VALU instruction
+            "counter_achieved_gbs": (traffic / (ms_per_launch / 1e3) / 1e9) if traffic and ms_per_launch > 0 else None,
+            "counter_frac": (traffic / (ms_per_launch / 1e3) / 1e9 / HBM_PEAK_GBS) if traffic and ms_per_launch > 0
+            else None,
+            "l2_hit_rate": traffic_info.get("l2_hit_rate") if traffic_info else None,
+            "lanes_active_per_valu_instruction": traffic_info.get("lanes_active_per_valu_instruction")
+            if traffic_info else None,
+            "limiter": "instruction issue under divergence: the bytes that leave L2 use about a fifth of the HBM "
+                       "bandwidth, while the VALU is busy with about 19 of 64 lanes active per instruction",
         },
     }
     if not args.no_other:
@@ -530,7 +570,8 @@ def main() -> int:
         print(f"[bench] equal-time ratio {out['equal_time']['ratio_plain_over_guided']:.2f}", file=sys.stderr,
               flush=True)
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        # every logical CPU the job may use, up to the 16 that make up one GPU's share of the host
+        threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
         print(f"[bench] CPU baseline on {threads} threads ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(scene, threads)
         out["gpu_over_cpu_samples_per_s"] = value / out["cpu_baseline"]["value"]
