@@ -4,6 +4,7 @@
 // Names, signatures and argument meaning follow the reference headers so a
 // scene assembled the way src/main.cpp assembles one renders unchanged:
 //   create_pt_renderer          src/factory/renderer.hpp:12-21
+//   create_bdpt_renderer        src/factory/renderer.hpp:39-48
 //   create_general_scene        src/factory/scene.hpp:14
 //   create_bvh_accel            src/factory/accelerator.hpp:12
 //   create_triangle_mesh        src/factory/shape.hpp:17
@@ -407,5 +408,18 @@ struct GpuRenderOptions {
     tr_render_stats* stats_out = nullptr;   // optional
 };
 RC<Renderer> create_pt_renderer(const PTRendererParams& params, const GpuRenderOptions& gpu);
+
+// src/factory/renderer.hpp:39-48.  worker_count and task_tile_size are CPU
+// scheduling controls without meaning on the GPU.
+struct BDPTRendererParams {
+    int worker_count = 0;
+    int task_tile_size = 16;
+    int max_camera_vertex_count = 10;
+    int max_light_vertex_count = 10;
+    int spp = 1;
+};
+RC<Renderer> create_bdpt_renderer(const BDPTRendererParams& params);
+// GpuRenderOptions used: device, seed, samples_per_pass, stats_out.
+RC<Renderer> create_bdpt_renderer(const BDPTRendererParams& params, const GpuRenderOptions& gpu);
 
 }  // namespace tracer

@@ -280,6 +280,45 @@ void tr_rng_sequence(uint64_t seed, uint32_t pixel, uint32_t sample, int64_t n, 
 tr_status tr_render(tr_ctx* ctx, const tr_render_params* params, double* film_sums,
                     int32_t accumulate, float* image, tr_render_stats* stats);
 
+/* ---- bidirectional path tracing (replaces BDPTRenderer::render,
+ *      src/renderer/bdpt_renderer.cpp:1087-1185, the renderer behind
+ *      create_bdpt_renderer, src/factory/renderer.hpp:39-48) ------------- */
+typedef struct tr_bdpt_params {
+    int32_t width, height;
+    int32_t spp;
+    int32_t max_camera_vertex_count;   /* vertices of a camera subpath, the camera's included (reference
+                                          default 10), in [1, 32] */
+    int32_t max_light_vertex_count;    /* vertices of a light subpath, the light's included (reference
+                                          default 10), in [0, 32] */
+    int32_t filter_type;               /* TR_FILTER_*: the filter of the camera-side estimate; the light-
+                                          tracing (t = 1) splats go to the pixel they land in, weighted
+                                          1 / spp, as the reference adds its splat image */
+    float filter_radius;
+    float filter_alpha;
+    uint64_t seed;
+    int32_t sample_offset;             /* index of the first sample, for progressive rendering */
+    int32_t samples_per_pass;          /* samples per pixel in flight at once, 0 = auto */
+    int32_t only_s;                    /* testing: 0 = every strategy, MIS-weighted; k > 0 keeps only the
+                                          strategies with s = k - 1 light vertices, each with weight 1 */
+    int32_t only_t;                    /* testing: 0 = every strategy; k > 0 keeps only the strategies with
+                                          t = k camera vertices, each with weight 1 */
+    int32_t profile;                   /* 1 = time the kernels with HIP events */
+    int32_t reserved;
+} tr_bdpt_params;
+
+/* Renders with every connection strategy (s light vertices, t camera
+ * vertices) of the reference, combined by the balance heuristic; every light
+ * path of at most max_camera_vertex_count + max_light_vertex_count vertices
+ * gets weight 1 in total.  film_sums, accumulate, image and stats as for
+ * tr_render: the light-tracing splats of the call are folded into the
+ * call's film sums so that sums / weight resolves to the camera-side estimate
+ * plus splats / spp.  stats->extension_rays counts subpath rays,
+ * shadow_rays connection rays.  The directional sun light has no
+ * counterpart in the reference's BDPT: scenes with one are rejected
+ * (TR_ERR_INVALID). */
+tr_status tr_render_bdpt(tr_ctx* ctx, const tr_bdpt_params* params, double* film_sums, int32_t accumulate,
+                         float* image, tr_render_stats* stats);
+
 /* ---- ray queries (replace Aggregate::intersect_p / intersect,
  *      src/accelerator/bvh.cpp:112-185, with the watertight test) --------- */
 /* rays: n * 8 floats (ox, oy, oz, tmin, dx, dy, dz, tmax); directions need not

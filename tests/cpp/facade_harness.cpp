@@ -5,7 +5,9 @@
 //   a: the scene as built;
 //   b: the same renderer after scene->set_camera(second camera);
 //   c: a fresh renderer on the scene with the second camera;
-//   d: the first renderer on another scene object (configuration 1, seed 5).
+//   d: the first renderer on another scene object (configuration 1, seed 5);
+//   e: create_bdpt_renderer(BDPTRendererParams{.., 5, 5, spp 16})->render on the
+//      first scene with its second camera.
 #include <cstdio>
 #include <stdexcept>
 #include <string>
@@ -48,6 +50,12 @@ int main(int argc, char** argv) {
         save(r2->render(*b.scene, film), out + "_c.f32");
         auto b2 = scenes::build_config(1, 5, 1.0);
         save(r->render(*b2.scene, film), out + "_d.f32");
+        BDPTRendererParams bp;
+        bp.max_camera_vertex_count = 5;
+        bp.max_light_vertex_count = 5;
+        bp.spp = 16;
+        auto rb = create_bdpt_renderer(bp, opt);
+        save(rb->render(*b.scene, film), out + "_e.f32");
     } catch (const std::exception& e) {
         std::fprintf(stderr, "facade_harness: %s\n", e.what());
         return 1;

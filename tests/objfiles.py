@@ -30,8 +30,8 @@ def camera(pos, target, up=(0.0, 1.0, 0.0), fov_deg=40.0, aspect=1.0) -> abi.TrC
 
 
 def box_room_obj(light_ke=(12.0, 12.0, 12.0), block_ks=(0.35, 0.35, 0.35), block_ns=25.0,
-                 glass_block: bool = False, texture: str | None = None) -> tuple[str, str]:
-    """A closed-on-five-sides room [-1, 1]^3 open towards +z, a ceiling quad
+                 glass_block: bool = False, texture: str | None = None, closed: bool = False) -> tuple[str, str]:
+    """A closed-on-five-sides room [-1, 1]^3 open towards +z (closed: all six), a ceiling quad
     light facing down, a tall block with a Phong specular lobe (or of glass,
     Ni 1.5) and a short diffuse block; faces are quads (split by the reader),
     normals given per face with v//vn corners.  With `texture` (an image file
@@ -94,6 +94,8 @@ map_Kd {texture}
          wall_uv)
     quad((-1, -1, 1), (-1, -1, -1), (-1, 1, -1), (-1, 1, 1), (1, 0, 0), "red")
     quad((1, -1, -1), (1, -1, 1), (1, 1, 1), (1, 1, -1), (-1, 0, 0), "green")
+    if closed:   # the sixth wall, at z = +1 facing -z
+        quad((1, -1, 1), (-1, -1, 1), (-1, 1, 1), (1, 1, 1), (0, 0, -1), "white")
     # light: a quad just below the ceiling facing down
     y = 0.998
     quad((-0.25, y, -0.25), (0.25, y, -0.25), (0.25, y, 0.25), (-0.25, y, 0.25), (0, -1, 0), "light", "light")

@@ -5,7 +5,9 @@ Film(res, create_gaussin_filter(0.5, 0.6))) as a caller of the reference
 does (src/core/renderer.hpp:16, src/main.cpp).  Its images equal the C ABI's
 tr_render of the same flattened scene bit for bit; Scene::set_camera between
 two renders of one renderer takes effect (the second image equals a fresh
-renderer's), and a new scene object is uploaded anew."""
+renderer's), a new scene object is uploaded anew, and
+create_bdpt_renderer(params)->render (src/factory/renderer.hpp:39-48) equals
+tr_render_bdpt bit for bit."""
 import subprocess
 from pathlib import Path
 
@@ -39,7 +41,7 @@ def test_facade_render_equals_c_abi(gpu_ctx, tmp_path):
     prefix = tmp_path / "img"
     r = subprocess.run([str(HARNESS), str(prefix)], capture_output=True, text=True, timeout=240)
     assert r.returncode == 0, r.stderr
-    imgs = {k: np.fromfile(f"{prefix}_{k}.f32", np.float32).reshape(H, W, 3) for k in "abcd"}
+    imgs = {k: np.fromfile(f"{prefix}_{k}.f32", np.float32).reshape(H, W, 3) for k in "abcde"}
     assert all(np.isfinite(v).all() and v.mean() > 0 for v in imgs.values())
     assert np.array_equal(imgs["a"], _c_abi_image(gpu_ctx, 0))
     # set_camera between two renders of the same renderer
@@ -51,6 +53,15 @@ def test_facade_render_equals_c_abi(gpu_ctx, tmp_path):
     assert np.array_equal(imgs["b"], _c_abi_image(gpu_ctx, 0, cam))
     # another scene object on the first renderer
     assert np.array_equal(imgs["d"], _c_abi_image(gpu_ctx, 5))
+    # create_bdpt_renderer(...)->render equals tr_render_bdpt of the same scene and camera
+    from tracer_amd import native
+    s = native.config_scene(1, seed=0)
+    gpu_ctx.load_scene(s)
+    gpu_ctx.set_camera(cam)
+    p = s.bdpt_params(5, 5, width=W, height=H, spp=16, seed=7, filter_type=abi.TR_FILTER_GAUSSIAN, filter_radius=0.5,
+                      filter_alpha=0.6)
+    _, bimg, _ = gpu_ctx.render_bdpt(p)
+    assert np.array_equal(imgs["e"], bimg)
 
 
 def test_set_camera_rejects_degenerate_cameras(gpu_ctx):

@@ -38,6 +38,7 @@ def test_abi_struct_sizes():
     assert C.sizeof(abi.TrCamera) == 52
     assert C.sizeof(abi.TrRenderStats) == 160
     assert C.sizeof(abi.TrRenderParams) == 96
+    assert C.sizeof(abi.TrBdptParams) == 64
     assert C.sizeof(abi.TrGuideConfig) == 48
 
 

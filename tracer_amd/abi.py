@@ -120,6 +120,18 @@ class TrRenderParams(C.Structure):
     ]
 
 
+class TrBdptParams(C.Structure):
+    """tr_bdpt_params (include/tracer_hip.h): BDPTRendererParams plus the film."""
+    _fields_ = [
+        ("width", C.c_int32), ("height", C.c_int32), ("spp", C.c_int32),
+        ("max_camera_vertex_count", C.c_int32), ("max_light_vertex_count", C.c_int32),
+        ("filter_type", C.c_int32), ("filter_radius", C.c_float), ("filter_alpha", C.c_float),
+        ("seed", C.c_uint64),
+        ("sample_offset", C.c_int32), ("samples_per_pass", C.c_int32),
+        ("only_s", C.c_int32), ("only_t", C.c_int32), ("profile", C.c_int32), ("reserved", C.c_int32),
+    ]
+
+
 class TrRenderStats(C.Structure):
     _fields_ = [
         ("camera_paths", C.c_uint64), ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64),

@@ -34,6 +34,7 @@ HIP_SOURCES = {
     "k_trace.hip": ["-ffp-contract=off"],
     "k_render.hip": [],
     "k_guide.hip": [],
+    "k_bdpt.hip": [],
     "capi.cpp": ["-x", "hip"],
     "guide_capi.cpp": ["-x", "hip"],
 }

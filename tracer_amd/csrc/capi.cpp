@@ -24,6 +24,7 @@ static_assert(sizeof(tr_medium) == 104, "tr_medium layout");
 static_assert(sizeof(tr_camera) == 52, "tr_camera layout");
 static_assert(sizeof(tr_render_params) == 96, "tr_render_params layout");
 static_assert(sizeof(tr_render_stats) == 160, "tr_render_stats layout");
+static_assert(sizeof(tr_bdpt_params) == 64, "tr_bdpt_params layout");
 static_assert(sizeof(tr_guide_config) == 48, "tr_guide_config layout");
 
 namespace {
@@ -620,13 +621,13 @@ DevPaths make_paths(tr_ctx* c) {
 // of any radius up to kMaxFilterRadius pixels (the gather of k_accumulate
 // visits (2 floor(r + 1/2) + 1)^2 neighbours per sample).
 constexpr float kMaxFilterRadius = 16.0f;
-void check_filter(const tr_render_params& P) {
-    if (P.filter_type != TR_FILTER_BOX && P.filter_type != TR_FILTER_GAUSSIAN) throw InvalidArg("unknown filter");
-    if (!(P.filter_radius > 0.0f && P.filter_radius <= kMaxFilterRadius))
-        throw InvalidArg("filter radius must lie in (0, 16] pixels");
-    if (P.filter_type == TR_FILTER_GAUSSIAN && !(P.filter_alpha >= 0.0f && P.filter_alpha < 1e30f))
+void check_filter(int type, float radius, float alpha) {
+    if (type != TR_FILTER_BOX && type != TR_FILTER_GAUSSIAN) throw InvalidArg("unknown filter");
+    if (!(radius > 0.0f && radius <= kMaxFilterRadius)) throw InvalidArg("filter radius must lie in (0, 16] pixels");
+    if (type == TR_FILTER_GAUSSIAN && !(alpha >= 0.0f && alpha < 1e30f))
         throw InvalidArg("Gaussian filter alpha must be finite and >= 0");
 }
+void check_filter(const tr_render_params& P) { check_filter(P.filter_type, P.filter_radius, P.filter_alpha); }
 
 // Learned light-sample distribution (guide_light_grid): bins per cell aim
 // at about kLcBinsTarget (S^2 strata per emissive triangle, S in [1, 16], plus
@@ -696,6 +697,41 @@ void lc_build(tr_ctx* c) {
                     L.table.as<float2>(), L.work.as<int>(), c->stream);
     hip_check(hipGetLastError(), "k_lc_build launch");
     L.built = true;
+}
+
+// Bidirectional path tracing: per camera sample 80-byte vertices of both
+// subpaths, two subpath rays with their hits, weights and RNG states, and per
+// strategy a contribution and (at most) one connection ray.
+constexpr int kBdMaxVertices = 32;
+int64_t bdpt_slot_bytes(int nc, int nl) {
+    const int64_t K = int64_t(nc) * (nl + 1);
+    return int64_t(nc + nl) * int64_t(sizeof(BdVertex)) + 2 * 4 + 2 * (32 + 16 + 16 + 8 + 2 * 4) + 16 + K * (16 + 32 + 4);
+}
+void bdpt_ensure(tr_ctx* c, int64_t slots, int nc, int nl, int64_t npix) {
+    BdptState& B = c->bdpt;
+    const int64_t K = int64_t(nc) * (nl + 1);
+    const int64_t verts = slots * (nc + nl), entries = slots * K;
+    if (slots > B.slots) {
+        const size_t n = size_t(slots);
+        B.ncv.reserve(n * 4); B.nlv.reserve(n * 4); B.rad.reserve(n * 16);
+        B.gray.reserve(2 * n * 32); B.ghit.reserve(2 * n * 16); B.gbeta.reserve(2 * n * 16); B.grng.reserve(2 * n * 8);
+        B.gq0.reserve(2 * n * 4); B.gq1.reserve(2 * n * 4);
+        B.slots = slots;
+    }
+    B.cv.reserve(size_t(slots) * nc * sizeof(BdVertex));
+    B.lv.reserve(size_t(slots) * std::max(nl, 1) * sizeof(BdVertex));
+    B.vertices = std::max(B.vertices, verts);
+    if (entries > B.entries) {
+        B.con.reserve(size_t(entries) * 16);
+        B.sray.reserve(size_t(entries) * 32);
+        B.sidx.reserve(size_t(entries) * 4);
+        B.entries = entries;
+    }
+    if (npix > B.pixels) {
+        B.splat.reserve(size_t(npix) * 24);
+        B.w0.reserve(size_t(npix) * 8);
+        B.pixels = npix;
+    }
 }
 
 hipEvent_t ev(tr_ctx* c, size_t i) {
@@ -1400,6 +1436,154 @@ tr_status tr_render(tr_ctx* c, const tr_render_params* prm, double* film_sums, i
             stats->ms_guide_train = ms_gt;
             stats->tail_extension_rays = hs[8];
             stats->ms_tail = ms_tl;
+        }
+    });
+}
+
+tr_status tr_render_bdpt(tr_ctx* c, const tr_bdpt_params* prm, double* film_sums, int32_t accumulate, float* image,
+                         tr_render_stats* stats) {
+    return guarded(c, [&] {
+        if (!c->have_bvh) throw StateError("tr_render_bdpt before tr_bvh_build");
+        if (!prm) throw InvalidArg("params is NULL");
+        const tr_bdpt_params& P = *prm;
+        if (P.width <= 0 || P.height <= 0 || P.spp <= 0) throw InvalidArg("width, height and spp must be positive");
+        const int nc = P.max_camera_vertex_count, nl = P.max_light_vertex_count;
+        if (nc < 1 || nc > kBdMaxVertices) throw InvalidArg("max_camera_vertex_count must lie in [1, 32]");
+        if (nl < 0 || nl > kBdMaxVertices) throw InvalidArg("max_light_vertex_count must lie in [0, 32]");
+        check_filter(P.filter_type, P.filter_radius, P.filter_alpha);
+        if (P.only_s < 0 || P.only_s > nl + 1 || P.only_t < 0 || P.only_t > nc)
+            throw InvalidArg("only_s must lie in [0, max_light_vertex_count + 1] and only_t in [0, max_camera_vertex_count]");
+        if (P.samples_per_pass < 0 || P.sample_offset < 0) throw InvalidArg("samples_per_pass and sample_offset must be >= 0");
+        if (c->ds.has_sun) throw InvalidArg("bidirectional path tracing does not support the directional sun light");
+        const int64_t npix = int64_t(P.width) * P.height;
+        if (npix > (int64_t(1) << 28)) throw InvalidArg("image too large");
+        const int64_t K = int64_t(nc) * (nl + 1);
+        int spp_pp = P.samples_per_pass;
+        if (spp_pp <= 0) {
+            // samples in flight: a quarter of the free device memory plus what the
+            // context holds, at most 2^26 strategy entries
+            size_t free_b = 0, total_b = 0;
+            hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+            const int64_t fit = c->bdpt.slots + int64_t(free_b / 4) / bdpt_slot_bytes(nc, nl);
+            const int64_t target = std::min<int64_t>(fit, (int64_t(1) << 26) / K);
+            spp_pp = int(std::max<int64_t>(1, target / npix));
+        }
+        spp_pp = std::min(spp_pp, P.spp);
+        const int64_t slots = npix * spp_pp;
+        if (slots * K >= (int64_t(1) << 31) || slots > (int64_t(1) << 29))
+            throw InvalidArg("too many samples per pass for the strategy table (lower samples_per_pass)");
+        bdpt_ensure(c, slots, nc, nl, npix);
+        c->film.reserve(size_t(npix) * sizeof(double4));
+        c->film_w = P.width;
+        c->film_h = P.height;
+        hipStream_t st = c->stream;
+        BdptState& S = c->bdpt;
+        if (accumulate && film_sums)
+            hip_check(hipMemcpyAsync(c->film.ptr, film_sums, size_t(npix) * 32, hipMemcpyHostToDevice, st), "H2D film");
+        else
+            hip_check(hipMemsetAsync(c->film.ptr, 0, size_t(npix) * 32, st), "film clear");
+        hip_check(hipMemsetAsync(c->stats.ptr, 0, 128, st), "stats clear");
+        hip_check(hipMemsetAsync(S.splat.ptr, 0, size_t(npix) * 24, st), "splat clear");
+        launch_bd_snapshot(c->film.as<double4>(), S.w0.as<double>(), int(npix), st);
+
+        BdPaths b{};
+        b.cv = S.cv.as<BdVertex>(); b.lv = S.lv.as<BdVertex>();
+        b.ncv = S.ncv.as<int>(); b.nlv = S.nlv.as<int>();
+        b.gray = S.gray.as<float4>(); b.ghit = S.ghit.as<float4>(); b.gbeta = S.gbeta.as<float4>();
+        b.grng = S.grng.as<uint2>();
+        b.gq[0] = S.gq0.as<int>(); b.gq[1] = S.gq1.as<int>();
+        b.counters = c->counters.as<unsigned int>();
+        b.con = S.con.as<float4>(); b.sray = S.sray.as<float4>(); b.sidx = S.sidx.as<int>();
+        b.splat = S.splat.as<unsigned long long>();
+        b.rad = S.rad.as<float4>();
+        b.stats = c->stats.as<unsigned long long>();
+        b.ovf_node = c->ovf_node.as<int>(); b.ovf_t = c->ovf_t.as<float>(); b.ovf_threads = c->ovf_threads;
+        BdConsts k{};
+        k.width = P.width; k.height = P.height; k.npix = int(npix);
+        k.nc = nc; k.nl = nl; k.K = int(K);
+        k.only_s = P.only_s - 1;
+        k.only_t = P.only_t;
+        k.seed = P.seed;
+        const DevCamera& cam = c->ds.cam;
+        k.film_area = cam.film_w * cam.film_h;
+        k.lens_area = cam.lens_radius > 0.0f ? float(kPi) * cam.lens_radius * cam.lens_radius : 1.0f;
+        k.env_pdf_pos = 1.0f / (float(kPi) * c->ds.wradius * c->ds.wradius);
+        // k_accumulate reads the camera-side radiance of each slot
+        DevPaths p{};
+        p.rad = b.rad;
+        p.stats = b.stats;
+        RenderConsts rc{};
+        rc.width = P.width; rc.height = P.height;
+        rc.filter = P.filter_type; rc.filter_radius = P.filter_radius; rc.filter_alpha = P.filter_alpha;
+        rc.seed = P.seed;
+
+        size_t evi = 0;
+        const bool prof = P.profile != 0;
+        std::vector<std::pair<size_t, int>> marks;
+        hipEvent_t e0 = ev(c, evi++), e1 = ev(c, evi++);
+        hip_check(hipEventRecord(e0, st), "event");
+        auto timed = [&](int kind, auto&& launch) {
+            if (!prof) { launch(); return; }
+            const size_t a = evi++, bb = evi++;
+            hip_check(hipEventRecord(ev(c, a), st), "event");
+            launch();
+            hip_check(hipEventRecord(ev(c, bb), st), "event");
+            marks.push_back({a, kind});
+        };
+        const int steps = std::max(nc, nl) - 1;
+        for (int done = 0; done < P.spp; done += spp_pp) {
+            const int kk = std::min(spp_pp, P.spp - done);
+            k.slots = int(npix * kk);
+            k.sample_base = P.sample_offset + done;
+            rc.spp_in_pass = kk;
+            rc.sample_base = k.sample_base;
+            hip_check(hipMemsetAsync(c->counters.ptr, 0, size_t(3) * kCounterStride * sizeof(unsigned int), st), "counters");
+            timed(3, [&] { launch_bd_start(c->ds, b, k, st); });
+            int cur = 0;
+            for (int step = 0; step < steps; ++step) {
+                timed(0, [&] { launch_bd_trace(c->ds.tr, b, cur, 2 * k.slots, st); });
+                timed(1, [&] { launch_bd_extend(c->ds, b, k, cur, st); });
+                cur = 1 - cur;
+            }
+            timed(1, [&] { launch_bd_connect(c->ds, b, k, st); });
+            timed(2, [&] { launch_bd_occlude(c->ds.tr, b, int64_t(k.slots) * K, st); });
+            timed(3, [&] {
+                launch_bd_gather(b, k, st);
+                launch_accumulate(p, rc, c->film.as<double4>(), st);
+            });
+            hip_check(hipGetLastError(), "bdpt launch");
+        }
+        launch_bd_fold(b, c->film.as<double4>(), S.w0.as<double>(), int(npix), P.spp, st);
+        hip_check(hipEventRecord(e1, st), "event");
+        if (film_sums)
+            hip_check(hipMemcpyAsync(film_sums, c->film.ptr, size_t(npix) * 32, hipMemcpyDeviceToHost, st), "D2H film");
+        if (image) {
+            c->s_out.reserve(size_t(npix) * 12);
+            launch_resolve(c->film.as<double4>(), c->s_out.as<float>(), int(npix), st);
+            hip_check(hipMemcpyAsync(image, c->s_out.ptr, size_t(npix) * 12, hipMemcpyDeviceToHost, st), "D2H image");
+        }
+        unsigned long long hs[16] = {0};
+        hip_check(hipMemcpyAsync(hs, c->stats.ptr, sizeof hs, hipMemcpyDeviceToHost, st), "D2H stats");
+        hip_check(hipStreamSynchronize(st), "bdpt sync");
+        if (stats) {
+            std::memset(stats, 0, sizeof *stats);
+            stats->camera_paths = uint64_t(npix) * uint64_t(P.spp);
+            stats->extension_rays = hs[0];
+            stats->shadow_rays = hs[1];
+            stats->nonfinite_dropped = hs[2] + hs[9];
+            float ms = 0.0f;
+            hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+            stats->ms_total = ms;
+            double acc[4] = {0, 0, 0, 0};
+            for (auto& m : marks) {
+                float t = 0.0f;
+                hip_check(hipEventElapsedTime(&t, ev(c, m.first), ev(c, m.first + 1)), "elapsed");
+                acc[m.second] += t;
+            }
+            stats->ms_extend = acc[0];
+            stats->ms_shade = acc[1];
+            stats->ms_shadow = acc[2];
+            stats->ms_other = acc[3];
         }
     });
 }

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "envmap.h"
+#include "tr_bdpt.h"
 #include "tr_guide_kernels.h"
 #include "tr_kernels.h"
 
@@ -125,6 +126,13 @@ struct LightCacheState {
     bool built = false;   // the tables hold a distribution from at least one training pass
 };
 
+// Bidirectional path tracing buffers (tr_render_bdpt), sized for `slots`
+// camera samples with nc + nl vertices and K strategies each.
+struct BdptState {
+    DevBuf cv, lv, ncv, nlv, gray, ghit, gbeta, grng, gq0, gq1, con, sray, sidx, rad, splat, w0;
+    int64_t slots = 0, vertices = 0, entries = 0, pixels = 0;
+};
+
 }  // namespace trcapi
 
 struct tr_ctx {
@@ -155,6 +163,7 @@ struct tr_ctx {
     std::vector<hipEvent_t> events;
     trcapi::GuideState guide;
     trcapi::LightCacheState lcache;
+    trcapi::BdptState bdpt;
 };
 
 namespace trcapi {

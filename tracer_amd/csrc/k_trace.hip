@@ -6,13 +6,15 @@
 #include <stdexcept>
 #include <string>
 
+#include "tr_bdpt.h"
 #include "tr_kernels.h"
 
 namespace trhip {
 
 namespace {
 
-enum TraceKernel { TK_CLOSEST, TK_ANY, TK_COUNTED, TK_EXTEND, TK_EXTEND_COUNT, TK_SHADOW, TK_SHADOW_MULTI, TK_NUM };
+enum TraceKernel { TK_CLOSEST, TK_ANY, TK_COUNTED, TK_EXTEND, TK_EXTEND_COUNT, TK_SHADOW, TK_SHADOW_MULTI, TK_BD_TRACE,
+                   TK_BD_OCCLUDE, TK_NUM };
 
 // The trace kernels loop over their rays with a grid stride, so every wave
 // gets an equal share.  A grid that is not a whole number of the kernel's
@@ -272,6 +274,73 @@ __global__ __launch_bounds__(256) void k_gather_multi(DevPaths p, int cur, int n
     if (__lane_id() == 0 && rays) atomicAdd(&p.stats[1], rays);
 }
 
+// Bidirectional path tracing: closest hits of the subpath rays in queue cur.
+__global__ __launch_bounds__(kTraceBlock) void k_bd_trace(TraceScene sc, BdPaths b, int cur) {
+    __shared__ TraceStack<kTraceBlock, kLdsStack> st;
+    const int tid = threadIdx.x;
+    const int gt = blockIdx.x * kTraceBlock + tid;
+    const int stride = gridDim.x * kTraceBlock;
+    const unsigned int n = *bd_counter(b, cur);
+    if (gt == 0) {
+        *bd_counter(b, 1 - cur) = 0u;   // filled by k_bd_extend
+        atomicAdd(&b.stats[0], (unsigned long long)n);
+    }
+    const int* q = b.gq[cur];
+    WaveQueue wq(n);
+    TraceRay r;
+    bool busy = false;
+    int ray = 0;
+    while (true) {
+        wq.refill(busy, [&](unsigned int idx) {
+            ray = q[idx];
+            const float4 a = b.gray[2 * size_t(ray)], d = b.gray[2 * size_t(ray) + 1];
+            r.init(xyz(a), a.w, xyz(d), d.w);
+        });
+        if (__ballot(busy) == 0ull) break;
+        if (busy && !r.step<false, false, kTraceBlock, kLdsStack>(sc, st, tid, b.ovf_node + gt, b.ovf_t + gt, stride,
+                                                                  nullptr, nullptr)) {
+            TraceHit h;
+            r.result(h);
+            b.ghit[ray] = make_float4(h.t, __int_as_float(h.tri), h.b1, h.b2);
+            busy = false;
+        }
+    }
+}
+
+// Bidirectional path tracing: visibility of the compacted connection rays;
+// an occluded connection contributes nothing.
+__global__ __launch_bounds__(kTraceBlock) void k_bd_occlude(TraceScene sc, BdPaths b) {
+    __shared__ TraceStack<kTraceBlock, kLdsStack> st;
+    const int tid = threadIdx.x;
+    const int gt = blockIdx.x * kTraceBlock + tid;
+    const int stride = gridDim.x * kTraceBlock;
+    const unsigned int n = *bd_counter(b, 2);
+    if (gt == 0) atomicAdd(&b.stats[1], (unsigned long long)n);
+    WaveQueue wq(n);
+    TraceRay r;
+    bool busy = false;
+    unsigned int i = 0;
+    while (true) {
+        wq.refill(busy, [&](unsigned int idx) {
+            i = idx;
+            const float4 a = b.sray[2 * size_t(i)], d = b.sray[2 * size_t(i) + 1];
+            r.init(xyz(a), a.w, xyz(d), d.w);
+        });
+        if (__ballot(busy) == 0ull) break;
+        if (!busy || r.step<true, false, kTraceBlock, kLdsStack>(sc, st, tid, b.ovf_node + gt, b.ovf_t + gt, stride,
+                                                                 nullptr, nullptr))
+            continue;
+        busy = false;
+        TraceHit h;
+        if (r.result(h)) {
+            float4* c = b.con + b.sidx[i];
+            c->x = 0.0f;
+            c->y = 0.0f;
+            c->z = 0.0f;
+        }
+    }
+}
+
 namespace {
 
 // Blocks of a full-size launch of kernel k on the current device.
@@ -287,6 +356,8 @@ int grid_blocks(TraceKernel k) {
         case TK_EXTEND: b = resident_rounds(k_extend<false>); break;
         case TK_EXTEND_COUNT: b = resident_rounds(k_extend<true>); break;
         case TK_SHADOW_MULTI: b = resident_rounds(k_shadow_multi); break;
+        case TK_BD_TRACE: b = resident_rounds(k_bd_trace); break;
+        case TK_BD_OCCLUDE: b = resident_rounds(k_bd_occlude); break;
         default: b = resident_rounds(k_shadow); break;
     }
     g_grid[dev][k].store(b, std::memory_order_relaxed);
@@ -394,6 +465,14 @@ void launch_shadow_multi(const TraceScene& sc, const DevPaths& p, int cur, int m
                        cur, nls);
     const int blocks = int(std::min<int64_t>((max_paths + 255) / 256, 4096));
     hipLaunchKernelGGL(k_gather_multi, dim3(blocks > 0 ? blocks : 1), dim3(256), 0, st, p, cur, nls);
+}
+void launch_bd_trace(const TraceScene& sc, const BdPaths& b, int cur, int max_rays, hipStream_t st) {
+    hipLaunchKernelGGL(k_bd_trace, dim3(trace_blocks(TK_BD_TRACE, max_rays, b.ovf_threads)), dim3(kTraceBlock), 0, st, sc, b,
+                       cur);
+}
+void launch_bd_occlude(const TraceScene& sc, const BdPaths& b, int64_t max_rays, hipStream_t st) {
+    hipLaunchKernelGGL(k_bd_occlude, dim3(trace_blocks(TK_BD_OCCLUDE, max_rays, b.ovf_threads)), dim3(kTraceBlock), 0, st,
+                       sc, b);
 }
 void launch_shadow(const TraceScene& sc, const DevPaths& p, int max_shadow, int accumulate_atomic, int record,
                    hipStream_t st) {

@@ -587,23 +587,24 @@ tr_camera camera_desc(const Scene& scene) {
 // ---- the GPU path tracer behind create_pt_renderer ---------------------------
 namespace {
 
-class GpuPathTraceRenderer : public Renderer {
+// The device context and the uploaded scene of a GPU renderer.  The geometry
+// is uploaded again for another scene object or a rebuilt aggregate, or when
+// the light list or environment changed; the camera is sent with every
+// render, so set_camera between renders takes effect.
+class GpuRendererBase : public Renderer {
 public:
-    GpuPathTraceRenderer(const PTRendererParams& p, const GpuRenderOptions& g) : params(p), gpu(g) {
-        if (p.spp <= 0 || p.max_depth < 0 || p.min_depth < 0) throw std::runtime_error("create_pt_renderer: invalid params");
-    }
-    ~GpuPathTraceRenderer() override {
+    explicit GpuRendererBase(const GpuRenderOptions& g) : gpu(g) {}
+    ~GpuRendererBase() override {
         if (ctx) tr_destroy(ctx);
     }
-    RenderTarget render(const Scene& scene, Film film) override {
-        std::lock_guard<std::mutex> lk(mtx);
-        auto check = [&](tr_status s, const char* what) {
-            if (s != TR_OK) throw std::runtime_error(std::string(what) + ": " + (ctx ? tr_last_error(ctx) : "no context"));
-        };
+
+protected:
+    void check(tr_status s, const char* what) {
+        if (s != TR_OK) throw std::runtime_error(std::string(what) + ": " + (ctx ? tr_last_error(ctx) : "no context"));
+    }
+    // Call with mtx held.
+    void prepare(const Scene& scene) {
         if (!ctx) check(tr_create(&ctx, gpu.device), "tr_create");
-        // The geometry is uploaded again for another scene object or a rebuilt
-        // aggregate, or when the light list or environment changed; the camera
-        // is sent with every render, so set_camera between renders takes effect.
         const Key key = scene_key(scene);
         if (!(key == last_key)) {
             FlatScene f = flatten(scene);
@@ -617,6 +618,54 @@ public:
         }
         const tr_camera cam = camera_desc(scene);
         check(tr_set_camera(ctx, &cam), "tr_set_camera");
+    }
+    static void film_filter(const Film& film, int32_t& type, float& radius, float& alpha) {
+        const Filter* flt = film.get_filter().get();
+        if (const auto* gf = dynamic_cast<const GaussianFilter*>(flt)) {
+            type = TR_FILTER_GAUSSIAN;
+            radius = gf->r;
+            alpha = gf->alpha;
+        } else if (const auto* bf = dynamic_cast<const BoxFilter*>(flt)) {
+            type = TR_FILTER_BOX;
+            radius = bf->r;
+            alpha = 0.0f;
+        } else {
+            throw std::runtime_error("render: unsupported filter");
+        }
+    }
+
+    GpuRenderOptions gpu;
+    tr_ctx* ctx = nullptr;
+    std::mutex mtx;
+
+private:
+    struct Key {
+        uint64_t scene = 0, accel = 0;
+        size_t lights = 0;
+        const void* env = nullptr;
+        bool operator==(const Key& o) const { return scene == o.scene && accel == o.accel && lights == o.lights && env == o.env; }
+    };
+    static Key scene_key(const Scene& s) {
+        Key k;
+        if (const auto* gs = dynamic_cast<const GeneralScene*>(&s)) {
+            k.scene = gs->uid;
+            if (const auto* b = dynamic_cast<const BVHAccel*>(gs->accel.get())) k.accel = b->version;
+        }
+        k.lights = s.lights.size();
+        k.env = s.environment_light.get();
+        return k;
+    }
+    Key last_key;
+};
+
+class GpuPathTraceRenderer : public GpuRendererBase {
+public:
+    GpuPathTraceRenderer(const PTRendererParams& p, const GpuRenderOptions& g) : GpuRendererBase(g), params(p) {
+        if (p.spp <= 0 || p.max_depth < 0 || p.min_depth < 0) throw std::runtime_error("create_pt_renderer: invalid params");
+    }
+    RenderTarget render(const Scene& scene, Film film) override {
+        std::lock_guard<std::mutex> lk(mtx);
+        prepare(scene);
         tr_render_params rp{};
         rp.width = film.width();
         rp.height = film.height();
@@ -624,17 +673,7 @@ public:
         rp.min_depth = params.min_depth;
         rp.max_depth = params.max_depth;
         rp.direct_light_sample_num = params.direct_light_sample_num;
-        const Filter* flt = film.get_filter().get();
-        if (const auto* gf = dynamic_cast<const GaussianFilter*>(flt)) {
-            rp.filter_type = TR_FILTER_GAUSSIAN;
-            rp.filter_radius = gf->r;
-            rp.filter_alpha = gf->alpha;
-        } else if (const auto* bf = dynamic_cast<const BoxFilter*>(flt)) {
-            rp.filter_type = TR_FILTER_BOX;
-            rp.filter_radius = bf->r;
-        } else {
-            throw std::runtime_error("render: unsupported filter");
-        }
+        film_filter(film, rp.filter_type, rp.filter_radius, rp.filter_alpha);
         rp.shadow_transmittance = gpu.shadow_transmittance;
         rp.seed = gpu.seed;
         rp.samples_per_pass = gpu.samples_per_pass;
@@ -656,27 +695,39 @@ public:
     }
 
 private:
-    struct Key {
-        uint64_t scene = 0, accel = 0;
-        size_t lights = 0;
-        const void* env = nullptr;
-        bool operator==(const Key& o) const { return scene == o.scene && accel == o.accel && lights == o.lights && env == o.env; }
-    };
-    static Key scene_key(const Scene& s) {
-        Key k;
-        if (const auto* gs = dynamic_cast<const GeneralScene*>(&s)) {
-            k.scene = gs->uid;
-            if (const auto* b = dynamic_cast<const BVHAccel*>(gs->accel.get())) k.accel = b->version;
-        }
-        k.lights = s.lights.size();
-        k.env = s.environment_light.get();
-        return k;
-    }
     PTRendererParams params;
-    GpuRenderOptions gpu;
-    tr_ctx* ctx = nullptr;
-    Key last_key;
-    std::mutex mtx;
+};
+
+// BDPTRenderer (src/renderer/bdpt_renderer.cpp:1066-1185) on the GPU.
+class GpuBDPTRenderer : public GpuRendererBase {
+public:
+    GpuBDPTRenderer(const BDPTRendererParams& p, const GpuRenderOptions& g) : GpuRendererBase(g), params(p) {
+        if (p.spp <= 0 || p.max_camera_vertex_count < 1 || p.max_light_vertex_count < 0)
+            throw std::runtime_error("create_bdpt_renderer: invalid params");
+    }
+    RenderTarget render(const Scene& scene, Film film) override {
+        std::lock_guard<std::mutex> lk(mtx);
+        prepare(scene);
+        tr_bdpt_params bp{};
+        bp.width = film.width();
+        bp.height = film.height();
+        bp.spp = params.spp;
+        bp.max_camera_vertex_count = params.max_camera_vertex_count;
+        bp.max_light_vertex_count = params.max_light_vertex_count;
+        film_filter(film, bp.filter_type, bp.filter_radius, bp.filter_alpha);
+        bp.seed = gpu.seed;
+        bp.samples_per_pass = gpu.samples_per_pass;
+        RenderTarget rt;
+        rt.color = Image2D<Spectrum>(bp.width, bp.height);
+        tr_render_stats st{};
+        check(tr_render_bdpt(ctx, &bp, nullptr, 0, reinterpret_cast<float*>(rt.color.get_raw_data()), &st),
+              "tr_render_bdpt");
+        if (gpu.stats_out) *gpu.stats_out = st;
+        return rt;
+    }
+
+private:
+    BDPTRendererParams params;
 };
 
 }  // namespace
@@ -684,6 +735,12 @@ private:
 RC<Renderer> create_pt_renderer(const PTRendererParams& params) { return newRC<GpuPathTraceRenderer>(params, GpuRenderOptions{}); }
 RC<Renderer> create_pt_renderer(const PTRendererParams& params, const GpuRenderOptions& gpu) {
     return newRC<GpuPathTraceRenderer>(params, gpu);
+}
+RC<Renderer> create_bdpt_renderer(const BDPTRendererParams& params) {
+    return newRC<GpuBDPTRenderer>(params, GpuRenderOptions{});
+}
+RC<Renderer> create_bdpt_renderer(const BDPTRendererParams& params, const GpuRenderOptions& gpu) {
+    return newRC<GpuBDPTRenderer>(params, gpu);
 }
 
 }  // namespace tracer

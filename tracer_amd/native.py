@@ -54,6 +54,7 @@ def hip_lib() -> C.CDLL:
         lib.tr_bvh_build.argtypes = [VP, P(abi.TrBvhOpts)]
         lib.tr_bvh_info.argtypes = [VP, P(I64), P(I64), P(I64)]
         lib.tr_render.argtypes = [VP, P(TrRenderParams), P(C.c_double), I32, P(C.c_float), P(TrRenderStats)]
+        lib.tr_render_bdpt.argtypes = [VP, P(abi.TrBdptParams), P(C.c_double), I32, P(C.c_float), P(TrRenderStats)]
         lib.tr_trace_closest.argtypes = [VP, P(C.c_float), P(C.c_float), I64]
         lib.tr_trace_any.argtypes = [VP, P(C.c_float), P(C.c_int32), I64]
         lib.tr_trace_closest_device.argtypes = [VP, VP, VP, I64]
@@ -234,6 +235,20 @@ class Scene:
             p.width, p.height, p.spp, p.min_depth, p.max_depth = 64, 64, 1, 3, 8
             p.direct_light_sample_num, p.filter_type, p.filter_radius = 1, abi.TR_FILTER_BOX, 0.5
             p.guide_fraction, p.guide_lr = 0.5, 1e-3
+        for k, v in kw.items():
+            setattr(p, k, v)
+        return p
+
+
+    def bdpt_params(self, max_camera_vertex_count: int = 10, max_light_vertex_count: int = 10,
+                    **kw) -> abi.TrBdptParams:
+        """BDPTRendererParams (src/factory/renderer.hpp:39-45) with the scene's film,
+        spp, filter and seed."""
+        d = self.render_params()
+        p = abi.TrBdptParams()
+        p.width, p.height, p.spp, p.seed = d.width, d.height, d.spp, d.seed
+        p.filter_type, p.filter_radius, p.filter_alpha = d.filter_type, d.filter_radius, d.filter_alpha
+        p.max_camera_vertex_count, p.max_light_vertex_count = max_camera_vertex_count, max_light_vertex_count
         for k, v in kw.items():
             setattr(p, k, v)
         return p
@@ -450,6 +465,19 @@ class Context:
         st = TrRenderStats()
         self._check(self.lib.tr_render(self.h, C.byref(params), dptr(sums), 1 if acc else 0,
                                        fptr(img) if img is not None else fptr(None), C.byref(st)))
+        return sums, img, st
+
+    def render_bdpt(self, params: abi.TrBdptParams, film: np.ndarray | None = None, want_image: bool = True):
+        """Bidirectional path tracing (tr_render_bdpt); film: FP64 sums to
+        accumulate onto (updated in place)."""
+        W, H = params.width, params.height
+        self._film_shape = (H, W)
+        acc = film is not None
+        sums = film if acc else np.zeros((H, W, 4), np.float64)
+        img = np.zeros((H, W, 3), np.float32) if want_image else None
+        st = TrRenderStats()
+        self._check(self.lib.tr_render_bdpt(self.h, C.byref(params), dptr(sums), 1 if acc else 0,
+                                            fptr(img) if img is not None else fptr(None), C.byref(st)))
         return sums, img, st
 
     def trace_closest(self, rays: np.ndarray) -> np.ndarray:
