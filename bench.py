@@ -32,7 +32,10 @@ Also reported on the same JSON line:
                  configuration 2 at a reduced film, timed on this host, with
                  the GPU's time-to-equal-relMSE speedup on the same film.
   other_configs  configuration 3 (fog) full frame, configuration 5 (18.9M
-                 triangles) at 16 spp, configuration 2 under a 2048 x 1024
+                 triangles) full frame at 256 spp, plain and guided (network
+                 width 128 with learned light sampling; relMSE on a 480x270
+                 crop), the bidirectional path tracer on configuration 1,
+                 configuration 2 under a 2048 x 1024
                  image environment instead of its constant one, configuration
                  2 with a 1024 x 1024 8-bit base-colour texture, configuration
                  3 with a 64^3 density grid in place of its fog, and OBJ
@@ -155,13 +158,87 @@ def cpu_baseline(scene, threads: int, target_s: float = 8.0, runs: int = 3) -> d
     }
 
 
+def config5_guided(ctx, scene, hidden: int = 128, light_grid: int = 20, train_spp: int = 8, train_passes: int = 4,
+                   seeds: int = 2) -> dict:
+    """Configuration 5 at its full film and 256 spp with the guide network
+    widened to `hidden` (and learned light sampling on a light_grid^3 grid),
+    against the plain renderer: device time of the full frame, and relMSE at
+    equal spp and equal time on a 480x270 crop of the scene (the same camera
+    at a quarter of the resolution) against a 16384-spp plain reference."""
+    P = scene.render_params(seed=0)
+    W, H, spp = P.width, P.height, P.spp
+
+    def guided(w, h, seed, n_spp, profile=0):
+        ctx.guide_init(hidden=hidden, seed=11 + seed, batch=1 << 16, steps_per_pass=8)
+        ts = train_spp * train_passes
+        s1, _, a = ctx.render(scene.render_params(width=w, height=h, spp=ts, seed=seed, guide=1,
+                                                  guide_train_passes=train_passes, samples_per_pass=train_spp,
+                                                  guide_light_grid=light_grid, profile=profile), want_image=False)
+        s2, _, b = ctx.render(scene.render_params(width=w, height=h, spp=n_spp - ts, seed=seed, sample_offset=ts,
+                                                  guide=1, guide_light_grid=light_grid, profile=profile),
+                              film=s1, want_image=False)
+        return s2, a.ms_total + b.ms_total, a, b
+
+    _, g_ms, a, b = guided(W, H, 900, spp, profile=1)
+    _, _, pst = ctx.render(scene.render_params(seed=901), want_image=False)
+    full = {"film": f"{W}x{H}", "spp": spp, "plain_ms": pst.ms_total, "guided_ms": g_ms,
+            "guided_samples_per_s": W * H * spp / (g_ms / 1e3), "plain_samples_per_s": W * H * spp / (pst.ms_total / 1e3),
+            "guide_inference_ms": a.ms_guide + b.ms_guide, "guide_training_ms": a.ms_guide_train + b.ms_guide_train}
+    w, h = W // 4, H // 4
+    R = None
+    for c in range(32):
+        R, _, _ = ctx.render(scene.render_params(width=w, height=h, spp=512, seed=1, sample_offset=512 * c), film=R,
+                             want_image=False)
+    Rimg = R[..., :3] / R[..., 3:]
+
+    def relmse(sums):
+        img = sums[..., :3] / sums[..., 3:]
+        return float(np.mean((img - Rimg) ** 2 / (Rimg ** 2 + 1e-2)))
+
+    g_err, g_t, p_err, p_t = [], [], [], []
+    for k in range(seeds):
+        sg, ms, _, _ = guided(w, h, 500 + k, spp)
+        g_err.append(relmse(sg))
+        g_t.append(ms)
+        sp, _, st = ctx.render(scene.render_params(width=w, height=h, spp=spp, seed=600 + k), want_image=False)
+        p_err.append(relmse(sp))
+        p_t.append(st.ms_total)
+    pe, pt_, ge, gt = map(lambda v: float(np.mean(v)), (p_err, p_t, g_err, g_t))
+    return {"full_frame": full,
+            "crop": {"film": f"{w}x{h}", "reference": "plain, 16384 spp", "spp": spp, "plain_relmse": pe, "plain_ms": pt_,
+                     "guided_relmse": ge, "guided_ms": gt, "equal_spp_ratio": pe / ge,
+                     "equal_time_ratio_plain_over_guided": (pe * pt_) / (ge * gt)},
+            "guide": f"network width {hidden}, {train_passes} training passes of {train_spp} spp, learned light "
+                     f"sampling on a {light_grid}^3 grid"}
+
+
+def bdpt_config1(ctx) -> dict:
+    """Bidirectional path tracing (tr_render_bdpt) of configuration 1 at
+    1024x1024 with the reference's default vertex counts (10 camera and 10
+    light vertices): samples, subpath rays and connection rays per second."""
+    from tracer_amd import native
+    scene = native.config_scene(1)
+    ctx.load_scene(scene)
+    p = scene.bdpt_params(10, 10, width=1024, height=1024, spp=8, seed=1, profile=1)
+    ctx.render_bdpt(scene.bdpt_params(10, 10, width=1024, height=1024, spp=1, seed=2), want_image=False)
+    t0 = time.perf_counter()
+    _, _, st = ctx.render_bdpt(p, want_image=False)
+    el = time.perf_counter() - t0
+    return {"workload": "configuration 1, 1024x1024, 8 spp, max_camera_vertex_count 10, max_light_vertex_count 10",
+            "samples_per_s": 1024 * 1024 * 8 / el, "subpath_mrays_per_s": st.extension_rays / el / 1e6,
+            "connection_mrays_per_s": st.shadow_rays / el / 1e6, "s_per_frame": el,
+            "device_ms": {"subpath_traversal": st.ms_extend, "subpath_vertices": st.ms_shade,
+                          "connections": st.ms_tail, "visibility": st.ms_shadow, "start_gather_film": st.ms_other}}
+
+
 def other_configs() -> dict:
-    """Full frames of configuration 3 (fog, delta tracking) and a reduced-spp
-    frame of configuration 5 (18.9M triangles)."""
+    """Full frames of configuration 3 (fog, delta tracking) and configuration 5
+    (18.9M triangles, 256 spp), configuration 5 guided, and the bidirectional
+    path tracer."""
     from tracer_amd import native
     out = {}
     ctx = native.Context(0)
-    for cfg, spp in ((3, None), (5, 16)):
+    for cfg, spp in ((3, None), (5, None)):
         scene = native.config_scene(cfg)
         t0 = time.time()
         ctx.load_scene(scene)
@@ -178,6 +255,13 @@ def other_configs() -> dict:
         }
         print(f"[bench] configuration {cfg}: {out[f'config{cfg}']['samples_per_s'] / 1e6:.1f} M samples/s",
               file=sys.stderr, flush=True)
+        if cfg == 5:
+            out["config5_guided"] = config5_guided(ctx, scene)
+            print(f"[bench] configuration 5 guided: {json.dumps(out['config5_guided']['crop'])}", file=sys.stderr,
+                  flush=True)
+    out["bdpt_config1"] = bdpt_config1(ctx)
+    print(f"[bench] BDPT configuration 1: {out['bdpt_config1']['samples_per_s'] / 1e6:.1f} M samples/s",
+          file=sys.stderr, flush=True)
 
     def frame(scene, p):
         ctx.load_scene(scene)

@@ -313,7 +313,10 @@ typedef struct tr_bdpt_params {
  * tr_render: the light-tracing splats of the call are folded into the
  * call's film sums so that sums / weight resolves to the camera-side estimate
  * plus splats / spp.  stats->extension_rays counts subpath rays,
- * shadow_rays connection rays.  The directional sun light has no
+ * shadow_rays connection rays; with profile = 1, ms_extend is subpath
+ * traversal, ms_shade the subpath vertices, ms_tail the connection
+ * strategies, ms_shadow their visibility, ms_other subpath starts, gather
+ * and film.  The directional sun light has no
  * counterpart in the reference's BDPT: scenes with one are rejected
  * (TR_ERR_INVALID). */
 tr_status tr_render_bdpt(tr_ctx* ctx, const tr_bdpt_params* params, double* film_sums, int32_t accumulate,

@@ -1545,7 +1545,7 @@ tr_status tr_render_bdpt(tr_ctx* c, const tr_bdpt_params* prm, double* film_sums
                 timed(1, [&] { launch_bd_extend(c->ds, b, k, cur, st); });
                 cur = 1 - cur;
             }
-            timed(1, [&] { launch_bd_connect(c->ds, b, k, st); });
+            timed(4, [&] { launch_bd_connect(c->ds, b, k, st); });
             timed(2, [&] { launch_bd_occlude(c->ds.tr, b, int64_t(k.slots) * K, st); });
             timed(3, [&] {
                 launch_bd_gather(b, k, st);
@@ -1574,7 +1574,7 @@ tr_status tr_render_bdpt(tr_ctx* c, const tr_bdpt_params* prm, double* film_sums
             float ms = 0.0f;
             hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
             stats->ms_total = ms;
-            double acc[4] = {0, 0, 0, 0};
+            double acc[5] = {0, 0, 0, 0, 0};
             for (auto& m : marks) {
                 float t = 0.0f;
                 hip_check(hipEventElapsedTime(&t, ev(c, m.first), ev(c, m.first + 1)), "elapsed");
@@ -1584,6 +1584,7 @@ tr_status tr_render_bdpt(tr_ctx* c, const tr_bdpt_params* prm, double* film_sums
             stats->ms_shade = acc[1];
             stats->ms_shadow = acc[2];
             stats->ms_other = acc[3];
+            stats->ms_tail = acc[4];
         }
     });
 }

@@ -69,12 +69,14 @@ TR_D float pdf_to_area(float pdf, f3 from, const BdVertex& v) {
     return pdf * fabsf(dot(xyz(v.n), w)) / (sqrtf(d2) * d2);
 }
 
-// A scattering vertex with its shading frame and material.
+// A scattering vertex with its shading frame; its material is read from the
+// scene's table where it is evaluated (a copy is made only for a textured one).
 struct BdScatter {
     bool medium;
     f3 pos, wo, ng, ns, tx, ty;
-    DevMaterial mat;
-    float g;
+    int mat, tri;
+    bool glass, textured;
+    float b1, b2, g;
     int med_in, med_out;
 };
 
@@ -82,25 +84,40 @@ TR_D void load_scatter(const DevScene& s, const BdVertex& v, BdScatter& c) {
     c.pos = xyz(v.p);
     c.wo = xyz(v.wo);
     const int aux = vaux(v);
+    c.b1 = v.x.x;
+    c.b2 = v.x.y;
+    c.tri = aux;
     if (vtype(v) == BD_MEDIUM) {
         c.medium = true;
         c.g = s.media[aux].g;
         c.med_in = c.med_out = aux;
         c.ng = c.ns = c.tx = c.ty = f3s(0.0f);
-        c.mat = DevMaterial{};
+        c.mat = 0;
+        c.glass = c.textured = false;
         return;
     }
     c.medium = false;
     c.g = 0.0f;
     c.ng = xyz(v.n);
     const TriShade sr = load_shade(s, aux);
-    const float b1 = v.x.x, b2 = v.x.y;
-    c.ns = normalize(sr.n0 * (1.0f - b1 - b2) + sr.n1 * b1 + sr.n2 * b2);
+    c.ns = normalize(sr.n0 * (1.0f - c.b1 - c.b2) + sr.n1 * c.b1 + sr.n2 * c.b2);
     onb(c.ns, c.tx, c.ty);
-    c.mat = s.mats[sr.material];
-    if (c.mat.tex > 0) apply_texture(s, c.mat, aux, b1, b2);
+    c.mat = sr.material;
+    const DevMaterial& m = s.mats[sr.material];
+    c.glass = m.type == MAT_GLASS;
+    c.textured = m.tex > 0;
     c.med_in = sr.med_in;
     c.med_out = sr.med_out;
+}
+
+// bsdf_eval with the vertex's material (textured: kd / base colour at its uv).
+TR_D BsdfEval sc_bsdf(const DevScene& s, const BdScatter& c, f3 lwi, f3 lwo, f3 lng) {
+    if (c.textured) {
+        DevMaterial m = s.mats[c.mat];
+        apply_texture(s, m, c.tri, c.b1, c.b2);
+        return bsdf_eval(m, lwi, lwo, lng);
+    }
+    return bsdf_eval(s.mats[c.mat], lwi, lwo, lng);
 }
 
 TR_D f3 sc_local(const BdScatter& c, f3 w) { return mk(dot(w, c.tx), dot(w, c.ty), dot(w, c.ns)); }
@@ -116,20 +133,20 @@ TR_D float adjoint_scale(const DevMaterial& m, f3 lwi, f3 lwo) {
 
 // f(wi, wo) at the vertex (BSDF::eval / the phase function); importance:
 // light-subpath transport.
-TR_D f3 sc_f(const BdScatter& c, f3 wi, bool importance) {
+TR_D f3 sc_f(const DevScene& s, const BdScatter& c, f3 wi, bool importance) {
     if (c.medium) return f3s(hg_phase(c.g, -dot(wi, c.wo)));
     const f3 lwi = sc_local(c, wi), lwo = sc_local(c, c.wo);
-    f3 f = bsdf_eval(c.mat, lwi, lwo, sc_local(c, c.ng)).f;
-    if (importance) f *= adjoint_scale(c.mat, lwi, lwo);
+    f3 f = sc_bsdf(s, c, lwi, lwo, sc_local(c, c.ng)).f;
+    if (importance) f *= adjoint_scale(s.mats[c.mat], lwi, lwo);
     return f;
 }
 
 // Density of sampling w1 when w2 is the outgoing direction (BSDF::pdf(wi = w1,
 // wo = w2), src/core/bsdf.hpp:35); 0 at a specular vertex.
-TR_D float sc_pdf(const BdScatter& c, f3 w1, f3 w2) {
+TR_D float sc_pdf(const DevScene& s, const BdScatter& c, f3 w1, f3 w2) {
     if (c.medium) return hg_phase(c.g, -dot(w1, w2));
-    if (c.mat.type == MAT_GLASS) return 0.0f;
-    return bsdf_eval(c.mat, sc_local(c, w1), sc_local(c, w2), sc_local(c, c.ng)).pdf;
+    if (c.glass) return 0.0f;
+    return sc_bsdf(s, c, sc_local(c, w1), sc_local(c, w2), sc_local(c, c.ng)).pdf;
 }
 
 // Medium on the side of direction w (SurfaceIntersection::medium, src/core/intersection.hpp:36-41).
@@ -520,7 +537,7 @@ TR_D void connect_one(const DevScene& s, const BdPaths& b, const BdConsts& k, in
         const float cz2 = cz * cz;
         const float we = cam.focal * cam.focal / (k.film_area * k.lens_area * cz2 * cz2);
         const f3 l2c = -w;
-        const f3 f = sc_f(B, l2c, true);
+        const f3 f = sc_f(s, B, l2c, true);
         if (is_black(f)) return;
         float G = cz / d2;
         if (!B.medium) G *= fabsf(dot(B.ng, l2c));
@@ -529,7 +546,7 @@ TR_D void connect_one(const DevScene& s, const BdPaths& b, const BdConsts& k, in
         if (mis) {
             const float pdf_dir = cam.focal * cam.focal / (k.film_area * cz2 * cz);
             const float c_fwd = pdf_to_area(pdf_dir, cpos, bv);
-            const float d_fwd = pdf_to_area(sc_pdf(B, B.wo, l2c), B.pos, L[sl - 2]);
+            const float d_fwd = pdf_to_area(sc_pdf(s, B, B.wo, l2c), B.pos, L[sl - 2]);
             cn.c *= mis_weight(C, 1, L, sl, 0.0f, 0.0f, c_fwd, d_fwd, k);
         }
         cn.splat = min(int(px), k.width - 1) + k.width * min(int(py), k.height - 1);
@@ -554,7 +571,7 @@ TR_D void connect_one(const DevScene& s, const BdPaths& b, const BdConsts& k, in
             const f3 w = dv * (1.0f / dist);
             const float cosl = -dot(xyz(cv.n), w);
             if (!(cosl > 0.0f)) return;
-            f3 f = sc_f(B, w, false);
+            f3 f = sc_f(s, B, w, false);
             if (!B.medium) f *= fabsf(dot(B.ng, w));
             if (is_black(f)) return;
             const f3 Le = xyz(s.lights[4 * vaux(cv)]);
@@ -562,8 +579,8 @@ TR_D void connect_one(const DevScene& s, const BdPaths& b, const BdConsts& k, in
             cn.c = xyz(bv.beta) * f * Le * tr * (cosl / (d2 * cv.n.w));
             if (mis) {
                 const float b_bwd = pdf_to_area(cosl * kInvPi, cpos, bv);
-                const float a_bwd = pdf_to_area(sc_pdf(B, B.wo, w), B.pos, av);
-                const float c_fwd = pdf_to_area(sc_pdf(B, w, B.wo), B.pos, cv);
+                const float a_bwd = pdf_to_area(sc_pdf(s, B, B.wo, w), B.pos, av);
+                const float c_fwd = pdf_to_area(sc_pdf(s, B, w, B.wo), B.pos, cv);
                 cn.c *= mis_weight(C, t, L, 1, a_bwd, b_bwd, c_fwd, 0.0f, k);
             }
             cn.ray = true;
@@ -572,15 +589,15 @@ TR_D void connect_one(const DevScene& s, const BdPaths& b, const BdConsts& k, in
             cn.tmax = dist - kEps;
         } else {
             const f3 w = -xyz(cv.n);
-            f3 f = sc_f(B, w, false);
+            f3 f = sc_f(s, B, w, false);
             if (!B.medium) f *= fabsf(dot(B.ng, w));
             if (is_black(f)) return;
             float epdf;
             const f3 Le = env_eval<true>(s, w, epdf);
             cn.c = xyz(bv.beta) * f * Le * (1.0f / cv.n.w);
             if (mis) {
-                const float a_bwd = pdf_to_area(sc_pdf(B, B.wo, w), B.pos, av);
-                const float c_fwd = sc_pdf(B, w, B.wo);
+                const float a_bwd = pdf_to_area(sc_pdf(s, B, B.wo, w), B.pos, av);
+                const float c_fwd = sc_pdf(s, B, w, B.wo);
                 cn.c *= mis_weight(C, t, L, 1, a_bwd, k.env_pdf_pos, c_fwd, 0.0f, k);
             }
             cn.ray = true;
@@ -591,32 +608,39 @@ TR_D void connect_one(const DevScene& s, const BdPaths& b, const BdConsts& k, in
         return;
     }
 
-    // two subpaths joined (tx_sx_path_contrib :678-737, mis_weight_tx_sx :914-956)
+    // two subpaths joined (tx_sx_path_contrib :678-737, mis_weight_tx_sx :914-956);
+    // the camera-side vertex is finished before the light-side one is loaded,
+    // so only one shading frame and material are live at a time
     const BdVertex& cv = L[sl - 1];
     const BdVertex& dvx = L[sl - 2];
-    BdScatter Cc;
-    load_scatter(s, cv, Cc);
-    const f3 dv = Cc.pos - B.pos;
+    const f3 cpos = xyz(cv.p);
+    const f3 dv = cpos - B.pos;
     const float d2 = dot(dv, dv), dist = sqrtf(d2);
     const f3 w = dv * (1.0f / dist);
-    const f3 fb = sc_f(B, w, false);
-    if (is_black(fb)) return;
-    const f3 fc = sc_f(Cc, -w, true);
-    if (is_black(fc)) return;
+    f3 c = sc_f(s, B, w, false);
+    if (is_black(c)) return;
     float G = 1.0f / d2;
     if (!B.medium) G *= fabsf(dot(B.ng, w));
-    if (!Cc.medium) G *= fabsf(dot(Cc.ng, w));
-    const f3 tr = bd_tr(s, sc_medium(B, w), B.pos, w, dist, rng);
-    cn.c = xyz(bv.beta) * fb * tr * xyz(cv.beta) * fc * G;
+    c *= bd_tr(s, sc_medium(B, w), B.pos, w, dist, rng) * xyz(bv.beta);
+    float a_bwd = 0.0f, c_fwd = 0.0f;
     if (mis) {
-        const float a_bwd = pdf_to_area(sc_pdf(B, B.wo, w), B.pos, av);
-        const float b_bwd = pdf_to_area(sc_pdf(Cc, -w, Cc.wo), Cc.pos, bv);
-        const float c_fwd = pdf_to_area(sc_pdf(B, w, B.wo), B.pos, cv);
-        const float d_fwd = pdf_to_area(sc_pdf(Cc, Cc.wo, -w), Cc.pos, dvx);
+        a_bwd = pdf_to_area(sc_pdf(s, B, B.wo, w), B.pos, av);
+        c_fwd = pdf_to_area(sc_pdf(s, B, w, B.wo), B.pos, cv);
+    }
+    const f3 bpos = B.pos;
+    BdScatter Cc;
+    load_scatter(s, cv, Cc);
+    const f3 fc = sc_f(s, Cc, -w, true);
+    if (is_black(fc)) return;
+    if (!Cc.medium) G *= fabsf(dot(Cc.ng, w));
+    cn.c = c * xyz(cv.beta) * fc * G;
+    if (mis) {
+        const float b_bwd = pdf_to_area(sc_pdf(s, Cc, -w, Cc.wo), cpos, bv);
+        const float d_fwd = pdf_to_area(sc_pdf(s, Cc, Cc.wo, -w), cpos, dvx);
         cn.c *= mis_weight(C, t, L, sl, a_bwd, b_bwd, c_fwd, d_fwd, k);
     }
     cn.ray = true;
-    cn.o = B.pos;
+    cn.o = bpos;
     cn.d = w;
     cn.tmax = dist - kEps;
 }
