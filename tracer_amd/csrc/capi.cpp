@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -28,6 +29,24 @@ static_assert(sizeof(tr_bdpt_params) == 64, "tr_bdpt_params layout");
 static_assert(sizeof(tr_guide_config) == 48, "tr_guide_config layout");
 
 namespace {
+
+// Eighths of a wave's traversing lanes that must hold a leaf before the wave
+// tests triangles, for closest-hit and any-hit rays (TraceScene::leaf_share,
+// leaf_share_any).  TRACER_HIP_LEAF_SHARE / TRACER_HIP_LEAF_SHARE_ANY (0-8)
+// override them for tuning.
+int env_share(const char* name, int dflt) {
+    const char* e = std::getenv(name);
+    if (e && e[0] >= '0' && e[0] <= '8' && e[1] == '\0') return e[0] - '0';
+    return dflt;
+}
+int trace_leaf_share() {
+    static const int v = env_share("TRACER_HIP_LEAF_SHARE", 4);
+    return v;
+}
+int trace_leaf_share_any() {
+    static const int v = env_share("TRACER_HIP_LEAF_SHARE_ANY", 4);
+    return v;
+}
 
 // DisneyBSDF constructor (src/material/disney.cpp:347-399) evaluated once per material.
 DevMaterial make_material_untextured(const tr_material& m);
@@ -545,6 +564,8 @@ void build_device_scene(tr_ctx* c, const tr_bvh_opts* opts) {
     s.tr.nodes = c->nodes.as<float4>();
     s.tr.tris = c->tris.as<float4>();
     s.tr.max_stack = c->max_stack;
+    s.tr.leaf_share = trace_leaf_share();
+    s.tr.leaf_share_any = trace_leaf_share_any();
     s.tri_shade = c->tri_shade.as<float4>();
     s.mats = c->mats.as<DevMaterial>();
     s.media = c->media.as<DevMedium>();
@@ -747,7 +768,7 @@ hipEvent_t ev(tr_ctx* c, size_t i) {
 
 extern "C" {
 
-const char* tr_version(void) { return "tracer_hip 0.1 (gfx950, wavefront path tracer, 4-wide BVH)"; }
+const char* tr_version(void) { return "tracer_hip 0.2 (gfx950, wavefront path tracer, 4-wide BVH)"; }
 
 tr_status tr_bvh4_export(const tr_scene_desc* d, const tr_bvh_opts* opts, void* nodes_out, int64_t nodes_cap,
                          void* tris_out, int64_t* num_nodes, int32_t* depth) {

@@ -1,5 +1,6 @@
-// Ray traversal on gfx950: 4-wide BVH (bvh_build.h), one ray per lane,
-// near-first order, traversal stack in LDS with a global-memory spill area.
+// Ray traversal on gfx950: 4-wide BVH (Node4, bvh_build.h), one ray per
+// lane, near-first order, traversal stack in LDS with a global-memory spill
+// area, triangle tests gathered across the wave.
 //
 // Replaces BVHAccel::intersect_p / intersect (src/accelerator/bvh.cpp:112-185)
 // and Triangle::intersect_p / intersect (src/shape/triangle.cpp:117-175).
@@ -21,6 +22,8 @@ struct TraceScene {
     const float4* nodes;   // Node4 as 8 float4 per node
     const float4* tris;    // TriPacked as 3 float4 per triangle
     int max_stack;         // stack entries needed by the deepest path
+    int leaf_share;        // eighths of a wave's traversing lanes that must hold a leaf before it tests triangles
+    int leaf_share_any;    // the same for any-hit (shadow) rays
 };
 
 struct TraceHit {
@@ -102,11 +105,17 @@ struct TraceStack {
     float tn[LDS_STACK][BLOCK];
 };
 
-// Traversal state of one ray.  step() visits one node: it tests the node's
-// four child boxes, intersects the triangles of hit leaf children (leaves
-// first: they can only shrink best_t) and pushes the hit inner children,
-// nearest last.  Kernels either run step() to completion per ray (trace()) or
-// interleave steps with fetching new rays for lanes that finished.
+// Traversal state of one ray.  Each step() either visits one inner node
+// (tests its four child boxes, pushes the hit children that are still in
+// front of the closest hit, farthest first, and keeps the nearest; leaves and
+// inner nodes alike) or tests one triangle of the leaf the ray holds.  Which
+// of the two a wave does is decided by ballot over the lanes that called
+// step(): triangles once at least leaf_share eighths of them hold a leaf,
+// node visits otherwise, so lanes that reach a leaf wait for others to reach
+// theirs and the triangle test runs on many lanes at once instead of one
+// lane's whole leaf at a time.  Kernels either run step() to completion per
+// ray (trace()) or interleave steps with fetching new rays for lanes that
+// finished.
 struct TraceRay {
     RayWT wr;
     f3 inv;
@@ -114,8 +123,10 @@ struct TraceRay {
     float best_t;
     int best_prim, best_tri;
     float b1, b2;
-    int sp, cur;
-    bool have;
+    int sp;
+    int cur;        // >= 0: inner node to visit; < 0: leaf, triangles [ti, tend)
+    int ti, tend;
+    bool have;      // cur is valid
 
     TR_D void init(f3 o, float tmin_, f3 d, float tmax) {
         wr = wt_setup(o, d);
@@ -127,11 +138,12 @@ struct TraceRay {
         b1 = b2 = 0.0f;
         sp = 0;
         cur = 0;
+        ti = tend = 0;
         have = true;
     }
 
-    // One node visit.  Returns false once traversal is over: the stack is
-    // empty, or (ANY) a hit in [tmin, tmax] was found.
+    // One node visit or one triangle test.  Returns false once traversal is
+    // over: the stack is empty, or (ANY) a hit in [tmin, tmax] was found.
     template <bool ANY, bool COUNT, int BLOCK, int LDS_STACK>
     TR_D bool step(const TraceScene& sc, TraceStack<BLOCK, LDS_STACK>& st, int tid, int* ovf_node, float* ovf_t,
                    int ovf_stride, uint32_t* nvis, uint32_t* ntri) {
@@ -150,46 +162,18 @@ struct TraceRay {
                 if (t <= best_t) { cur = n; found = true; break; }
             }
             if (!found) return false;
-        }
-        if (COUNT) ++*nvis;
-        const f3 o = wr.o;
-        const float4* np = sc.nodes + size_t(cur) * 8;
-        const float4 lox = np[0], hix = np[1], loy = np[2], hiy = np[3], loz = np[4], hiz = np[5];
-        const int4 ch = *reinterpret_cast<const int4*>(np + 6);
-        float tn[4];
-        bool hm[4];
-        {
-            const float lx[4] = {lox.x, lox.y, lox.z, lox.w}, hx[4] = {hix.x, hix.y, hix.z, hix.w};
-            const float ly[4] = {loy.x, loy.y, loy.z, loy.w}, hy[4] = {hiy.x, hiy.y, hiy.z, hiy.w};
-            const float lz[4] = {loz.x, loz.y, loz.z, loz.w}, hz[4] = {hiz.x, hiz.y, hiz.z, hiz.w};
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                const float x0 = (lx[c] - o.x) * inv.x, x1 = (hx[c] - o.x) * inv.x;
-                const float y0 = (ly[c] - o.y) * inv.y, y1 = (hy[c] - o.y) * inv.y;
-                const float z0 = (lz[c] - o.z) * inv.z, z1 = (hz[c] - o.z) * inv.z;
-                const float tnear = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin)) * lo_scale;
-                const float tfar = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1)) * hi_scale;
-                hm[c] = tnear <= tfar && tnear <= best_t && tfar >= tmin;
-                tn[c] = tnear;
-            }
-        }
-        const int chv[4] = {ch.x, ch.y, ch.z, ch.w};
-        // Triangles of every hit leaf child in one loop (child order, then
-        // triangle order), so that one copy of the test serves all four
-        // children and a wave iterates as often as its busiest lane needs.
-        unsigned leaves = 0;
-#pragma unroll
-        for (int c = 0; c < 4; ++c) leaves |= (hm[c] && chv[c] < 0) ? (1u << c) : 0u;
-        int ti = 0, tend = 0;
-        while (true) {
-            while (ti == tend && leaves != 0) {
-                const int c = __builtin_ctz(leaves);
-                leaves &= leaves - 1;
-                const int code = ~(c == 0 ? chv[0] : c == 1 ? chv[1] : c == 2 ? chv[2] : chv[3]);
+            if (cur < 0) {
+                const int code = ~cur;
                 ti = code >> 4;
                 tend = ti + (code & 15);
             }
-            if (ti == tend) break;
+            have = true;
+        }
+        const bool leaf = cur < 0;
+        const int n_leaf = __popcll(__ballot(leaf));
+        const int n_all = __popcll(__ballot(1));
+        if (n_leaf > 0 && 8 * n_leaf >= (ANY ? sc.leaf_share_any : sc.leaf_share) * n_all) {
+            if (!leaf) return true;   // waits for the wave's next node round
             const float4* tp = sc.tris + size_t(ti) * 3;
             const float4 a = tp[0], b = tp[1], c2 = tp[2];
             if (COUNT) ++*ntri;
@@ -204,17 +188,41 @@ struct TraceRay {
                     best_t = t; best_prim = prim; best_tri = ti; b1 = u1; b2 = u2;
                 }
             }
-            ++ti;
+            if (++ti == tend) have = false;
+            return true;
         }
-        // Inner children still in front of the closest hit, farthest first
-        // (sorting network on (t, id); misses sort last with t = -inf), so
-        // the nearest is visited next and the others are pushed.
+        if (leaf) return true;   // holds its leaf until enough lanes have one
+        if (COUNT) ++*nvis;
+        const f3 o = wr.o;
+        // Node4: lox, hix, loy, hiy, loz, hiz (float4 each), child[4]
+        const float4* np = sc.nodes + size_t(cur) * 8;
+        const float4 lox = np[0], hix = np[1], loy = np[2], hiy = np[3], loz = np[4], hiz = np[5];
+        const int4 ch = *reinterpret_cast<const int4*>(np + 6);
+        const float lx[4] = {lox.x, lox.y, lox.z, lox.w}, hx[4] = {hix.x, hix.y, hix.z, hix.w};
+        const float ly[4] = {loy.x, loy.y, loy.z, loy.w}, hy[4] = {hiy.x, hiy.y, hiy.z, hiy.w};
+        const float lz[4] = {loz.x, loz.y, loz.z, loz.w}, hz[4] = {hiz.x, hiz.y, hiz.z, hiz.w};
+        float tn[4];
+        bool hm[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const float x0 = (lx[c] - o.x) * inv.x, x1 = (hx[c] - o.x) * inv.x;
+            const float y0 = (ly[c] - o.y) * inv.y, y1 = (hy[c] - o.y) * inv.y;
+            const float z0 = (lz[c] - o.z) * inv.z, z1 = (hz[c] - o.z) * inv.z;
+            const float tnear = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin)) * lo_scale;
+            const float tfar = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1)) * hi_scale;
+            hm[c] = tnear <= tfar && tnear <= best_t && tfar >= tmin;
+            tn[c] = tnear;
+        }
+        // Children to visit, farthest first (sorting network on t; misses and
+        // empty leaves sort last with t = -inf), so the nearest is kept and
+        // the others are pushed.
+        const int chv[4] = {ch.x, ch.y, ch.z, ch.w};
         float ts[4];
         int ids[4];
         int m = 0;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            const bool in = hm[c] && chv[c] >= 0 && tn[c] <= best_t;
+            const bool in = hm[c] && (chv[c] >= 0 || ((~chv[c]) & 15) != 0);
             ts[c] = in ? tn[c] : -INFINITY;
             ids[c] = chv[c];
             m += in ? 1 : 0;
@@ -238,9 +246,13 @@ struct TraceRay {
             }
             ++sp;
         }
-        cur = m == 1 ? ids[0] : m == 2 ? ids[1] : m == 3 ? ids[2] : ids[3];
         if (sp > sc.max_stack) sp = sc.max_stack;   // unreachable by construction (3 * depth)
         cur = ids[m - 1];
+        if (cur < 0) {
+            const int code = ~cur;
+            ti = code >> 4;
+            tend = ti + (code & 15);
+        }
         have = true;
         return true;
     }
