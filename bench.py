@@ -363,6 +363,16 @@ def other_configs() -> dict:
     return out
 
 
+def limiter_text(info, traffic, ms_per_launch):
+    """What the counter pass says bounds the traversal kernel."""
+    if not info or not traffic or ms_per_launch <= 0:
+        return None
+    frac = traffic / (ms_per_launch / 1e3) / 1e9 / HBM_PEAK_GBS
+    return (f"VALU issue under divergence and memory latency: the bytes that leave L2 use {frac:.2f} of the HBM "
+            f"bandwidth, while the VALU is {info.get('valu_busy', 0):.0%} busy with "
+            f"{info.get('lanes_active_per_valu_instruction', 0):.0f} of 64 lanes active per instruction")
+
+
 def read_traffic(path: Path):
     try:
         return json.loads(path.read_text())
@@ -640,8 +650,8 @@ def main() -> int:
             "l2_hit_rate": traffic_info.get("l2_hit_rate") if traffic_info else None,
             "lanes_active_per_valu_instruction": traffic_info.get("lanes_active_per_valu_instruction")
             if traffic_info else None,
-            "limiter": "instruction issue under divergence: the bytes that leave L2 use about a fifth of the HBM "
-                       "bandwidth, while the VALU is busy with about 19 of 64 lanes active per instruction",
+            "valu_busy": traffic_info.get("valu_busy") if traffic_info else None,
+            "limiter": limiter_text(traffic_info, traffic, ms_per_launch),
         },
     }
     if not args.no_other:
