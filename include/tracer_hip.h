@@ -417,11 +417,19 @@ tr_status tr_texture_eval_batch(tr_ctx* ctx, int32_t texture, const float* uv, i
  * width * height * spp RGB samples, sample s of pixel (x, y) at index
  * (s * height + y) * width + x, placed at the film position tr_render draws for
  * sample sample_offset + s of that pixel (seed of params); params give the
- * filter.  film_sums (W*H*4 doubles) is overwritten.  At most 2^26 samples. */
+ * filter.  film_sums (W*H*4 doubles) is overwritten.  At most 2^26 samples.
+ *
+ * tr_shading_frame_batch: the shading frame of SurfaceIntersection
+ * (src/shape/triangle.cpp:178-211: shading_coord from the normal derivatives
+ * of compute_ss_ts, src/utility/coordinate.hpp:54-68, or coordinate(sn),
+ * src/utility/geometry.hpp:887-894) as the renderer builds it at a hit of
+ * primitive prims[i] (scene order) with barycentrics bary (n*2: weights of
+ * vertex 1 and vertex 2) -> frames (n*9: x, y, z axes).  At most 2^26 hits. */
 tr_status tr_camera_ray_batch(tr_ctx* ctx, const float* film_uv, const float* lens_uv, int64_t n, float* rays);
 tr_status tr_light_pmf(tr_ctx* ctx, float* pmf, int64_t cap, int64_t* n);
 tr_status tr_light_pick_batch(tr_ctx* ctx, const float* u, int64_t n, int32_t* idx, float* pmf);
 tr_status tr_hg_sample_batch(tr_ctx* ctx, float g, const float* wo, const float* u, int64_t n, float* wi, float* pdf);
+tr_status tr_shading_frame_batch(tr_ctx* ctx, const int32_t* prims, const float* bary, int64_t n, float* frames);
 tr_status tr_film_accumulate_batch(tr_ctx* ctx, const tr_render_params* params, const float* radiance,
                                    double* film_sums);
 

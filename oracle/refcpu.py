@@ -61,6 +61,8 @@ def lib() -> C.CDLL:
         L.ref_hg_sample.restype = None
         L.ref_camera_rays.argtypes = [VP, P(C.c_float), P(C.c_float), I64, P(C.c_float)]
         L.ref_camera_rays.restype = None
+        L.ref_shading_frames.argtypes = [VP, P(C.c_int32), P(C.c_float), I64, P(C.c_float)]
+        L.ref_shading_frames.restype = None
         L.ref_render.argtypes = [VP, P(TrRenderParams), I32, I32, I32, P(C.c_double), P(C.c_uint64)]
         L.ref_render.restype = C.c_double
         L.ref_env_create.argtypes = [C.c_int, C.c_int, P(C.c_float), P(C.c_float)]
@@ -210,6 +212,15 @@ class RefScene:
         rays = np.zeros((film_uv.shape[0], 8), np.float32)
         lib().ref_camera_rays(self.h, fptr(film_uv), fptr(lens_uv), film_uv.shape[0], fptr(rays))
         return rays
+
+    def shading_frames(self, prims, bary):
+        """(n, 3, 3) shading frames (rows x, y, z) at hits of prims with barycentrics bary (n, 2)."""
+        prims = np.ascontiguousarray(prims, np.int32)
+        bary = np.ascontiguousarray(bary, np.float32)
+        out = np.zeros((prims.shape[0], 9), np.float32)
+        lib().ref_shading_frames(self.h, prims.ctypes.data_as(C.POINTER(C.c_int32)), fptr(bary), prims.shape[0],
+                                 fptr(out))
+        return out.reshape(-1, 3, 3)
 
     def render(self, params: TrRenderParams, quirks: bool, nee: int, threads: int = 0, sums=None):
         if sums is None:

@@ -1283,6 +1283,23 @@ void ref_hg_sample(float g, const float* wo, const float* u, int64_t n, float* w
         wi[3 * i] = s.wi.x; wi[3 * i + 1] = s.wi.y; wi[3 * i + 2] = s.wi.z; pdf[i] = s.pdf;
     }
 }
+// Shading frames of SurfaceIntersection (triangle.cpp:178-211, through
+// fill_isect) at hits of prims[i] with barycentrics (alpha, beta) = bary[2i..]:
+// x, y, z axes as nine floats per hit.
+void ref_shading_frames(void* hs, const int32_t* prims, const float* bary, int64_t n, float* out) {
+    Scene* s = static_cast<Scene*>(hs);
+    for (int64_t i = 0; i < n; ++i) {
+        Isect is;
+        fill_isect(*s, prims[i], V3f(0, 0, 0), V3f(0, 0, 1), 1.0f, bary[2 * i], bary[2 * i + 1], is);
+        const V3f ax[3] = {is.shd.x, is.shd.y, is.shd.z};
+        for (int k = 0; k < 3; ++k) {
+            out[9 * i + 3 * k] = ax[k].x;
+            out[9 * i + 3 * k + 1] = ax[k].y;
+            out[9 * i + 3 * k + 2] = ax[k].z;
+        }
+    }
+}
+
 void ref_camera_rays(void* hs, const float* film_uv, const float* lens_uv, int64_t n, float* rays) {
     Scene* s = static_cast<Scene*>(hs);
     RenderCfg c{};

@@ -8,6 +8,8 @@ code against the oracle, and of the whole estimator against closed forms.
   samples counted as black (path_trace_renderer.cpp:199-203);
 * camera: device rays against the oracle's ThinLensCamera for a pinhole and a
   thin lens;
+* shading frames: the device's tangent frame at surface hits against the
+  oracle's SurfaceIntersection frame, with and without texture coordinates;
 * light selection: the device pmf and pick_light against the oracle's power
   distribution and Distribution1D::sample_discrete;
 * Henyey-Greenstein: same scattering cosine and pdf as the oracle for the same
@@ -93,6 +95,32 @@ def test_camera_rays_match_oracle(gpu_ctx):
         assert np.allclose(g[:, :3], o[:, :3], atol=2e-6 * max(1.0, np.abs(o[:, :3]).max()))
         assert np.allclose(g[:, 4:7], o[:, 4:7], atol=2e-6)
         assert np.allclose(np.linalg.norm(g[:, 4:7], axis=1), 1.0, atol=1e-6)
+
+
+@pytest.mark.parametrize("scene", ["c1", "uv_sphere"])
+def test_shading_frames_match_oracle(gpu_ctx, scene):
+    """Device shading frames (tr_shading_frame_batch) against the oracle's
+    SurfaceIntersection frames (triangle.cpp:178-211): coordinate(sn) on
+    configuration 1 (no texture coordinates), the normal derivatives of
+    compute_ss_ts on a UV-mapped sphere with an anisotropic material."""
+    from oracle import refcpu
+    from tests.uvsphere import anisotropic_sphere_scene
+    from tracer_amd import native
+    if scene == "c1":
+        s = native.config_scene(1)
+    else:
+        s = anisotropic_sphere_scene(abi.make_material(type=abi.TR_MAT_DISNEY, metallic=1.0, roughness=0.3,
+                                                       anisotropic=0.9))
+    gpu_ctx.load_scene(s)
+    rng = np.random.default_rng(11)
+    n = 8192
+    prims = rng.integers(0, s.num_triangles, n).astype(np.int32)
+    b = rng.dirichlet((1, 1, 1), n)[:, 1:].astype(np.float32)
+    g = gpu_ctx.shading_frames(prims, b)
+    o = refcpu.RefScene(s, 3).shading_frames(prims, b)
+    err = np.abs(g - o).max(axis=(1, 2))
+    print(f"{scene}: max frame difference {err.max():.2e}")
+    assert err.max() < 2e-5
 
 
 @pytest.mark.parametrize("config,detail", [(1, 1.0), (2, 1 / 16), (5, 1 / 2048)])

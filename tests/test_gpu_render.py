@@ -130,6 +130,19 @@ def test_c2_disney_matches_fixed_estimator(gpu_ctx):
     _compare_estimators(gpu_ctx, s, 32, 32, 512, 64, 8, 8, quirks=False, nee=0, max_depth=6)
 
 
+@pytest.mark.parametrize("anisotropic", [0.9, 0.0])
+def test_anisotropic_sphere_matches_fixed_estimator(gpu_ctx, anisotropic):
+    """A UV-mapped metallic Disney sphere with a stretched highlight
+    (anisotropic 0.9; its tangent frame follows the texture coordinates) and
+    the isotropic one, over a ground quad under an emitter and the
+    environment, against the oracle in fixed mode."""
+    from tests.uvsphere import anisotropic_sphere_scene
+    from tracer_amd import abi
+    s = anisotropic_sphere_scene(abi.make_material(type=abi.TR_MAT_DISNEY, base_color=(0.9, 0.7, 0.5), metallic=1.0,
+                                                   roughness=0.25, anisotropic=anisotropic))
+    _compare_estimators(gpu_ctx, s, 32, 32, 512, 64, 8, 8, quirks=False, nee=0, max_depth=5)
+
+
 def test_c3_medium_matches_fixed_estimator(gpu_ctx):
     """The closed fog-filled room of configuration 3 (one sphere), delta
     tracking on the GPU against the oracle's free-flight sampling in fixed

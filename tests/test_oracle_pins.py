@@ -324,3 +324,45 @@ def test_disney_clearcoat_normal_incidence(gloss):
     a = max(1e-4, ((1 - gloss) * 0.1) ** 2)
     D = (a * a - 1) / (np.pi * np.log(a * a) * a * a)
     assert np.allclose(f, cc * 0.04 * D / 4, rtol=1e-5)
+
+
+def test_shading_frame_follows_the_uv_parameterisation():
+    """SurfaceIntersection's shading frame (triangle.cpp:200-211) on a
+    latitude-longitude sphere with radial vertex normals: the normal field is
+    the position, so its derivative along v (the polar angle) is the polar
+    tangent, and x = cross(dndv, sn) lies along the azimuthal tangent
+    (-sin phi, 0, cos phi) up to the tessellation's error; z is the
+    interpolated normal.  Without texture coordinates the frame is
+    coordinate(sn) (geometry.hpp:887-894): x has a zero y (|n_x| > |n_y|) or
+    zero x component."""
+    from oracle import refcpu
+    from tests.uvsphere import anisotropic_sphere_scene
+    s = anisotropic_sphere_scene(abi.make_material(type=abi.TR_MAT_DISNEY, metallic=1.0, roughness=0.3,
+                                                   anisotropic=0.9))
+    ref = refcpu.RefScene(s, 3)
+    rng = np.random.default_rng(7)
+    n_sphere = int(np.count_nonzero(s.material_ids == 0))
+    prims = rng.integers(0, n_sphere, 4000).astype(np.int32)
+    b = rng.dirichlet((1, 1, 1), 4000)[:, 1:].astype(np.float32)
+    F = ref.shading_frames(prims, b)
+    x, y, z = F[:, 0], F[:, 1], F[:, 2]
+    for a in (x, y, z):
+        assert np.allclose(np.linalg.norm(a, axis=1), 1.0, atol=1e-5)
+    assert np.abs(np.einsum('ij,ij->i', x, z)).max() < 1e-5 and np.abs(np.einsum('ij,ij->i', y, z)).max() < 1e-5
+    P = s.positions.reshape(-1, 3)
+    I = s.indices.reshape(-1, 3)[prims]
+    p = P[I[:, 0]] * (1 - b[:, :1] - b[:, 1:]) + P[I[:, 1]] * b[:, :1] + P[I[:, 2]] * b[:, 1:]
+    phi = np.arctan2(p[:, 2], p[:, 0])
+    e_phi = np.stack([-np.sin(phi), np.zeros_like(phi), np.cos(phi)], 1)
+    polar = np.abs(p[:, 1]) > 0.97          # rows next to the poles: the tessellation dominates
+    al = np.abs(np.einsum('ij,ij->i', x, e_phi))
+    assert np.all(al[~polar] > 0.99), al[~polar].min()
+    # a planted mistake (the derivative along u instead of v) turns x onto the polar tangent
+    assert np.median(np.abs(np.einsum('ij,ij->i', y, e_phi))[~polar]) < 0.05
+    # without texture coordinates: coordinate(sn)
+    s.uvs = None
+    F = refcpu.RefScene(s, 3).shading_frames(prims, b)
+    x, z = F[:, 0], F[:, 2]
+    big_x = np.abs(z[:, 0]) > np.abs(z[:, 1])
+    assert np.abs(x[big_x, 1]).max() < 1e-6 and np.abs(x[~big_x, 0]).max() < 1e-6
+    assert np.abs(np.einsum('ij,ij->i', x, z)).max() < 1e-5

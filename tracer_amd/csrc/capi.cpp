@@ -372,6 +372,22 @@ void check_camera(const tr_camera& cm) {
         throw InvalidArg("camera focal distance and aspect must be finite and >= 0");
 }
 
+// compute_ss_ts (src/utility/coordinate.hpp:54-68) over the vertex-normal
+// differences AB, AC and the uv differences (ab, ac): the derivative along v,
+// ts = (m10 AB - m00 AC) / det, with w = 1 where SurfaceIntersection keeps it
+// (det != 0 and both derivatives at least eps long, triangle.cpp:205-207);
+// w = 0 leaves the frame to coordinate(sn) per hit.
+float4 normal_dv(f3 AB, f3 AC, float abx, float aby, float acx, float acy) {
+    const float m00 = abx, m01 = aby, m10 = acx, m11 = acy;
+    const float det = m00 * m11 - m01 * m10;
+    if (det == 0.0f) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float inv_det = 1.0f / det;
+    const f3 ss = AB * (m11 * inv_det) - AC * (m01 * inv_det);
+    const f3 ts = AB * (m10 * inv_det) - AC * (m00 * inv_det);
+    if (dot(ss, ss) < kEps || dot(ts, ts) < kEps) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    return f4(ts, 1.0f);
+}
+
 void build_device_scene(tr_ctx* c, const tr_bvh_opts* opts) {
     HostScene& h = c->hs;
     const int64_t nt = int64_t(h.mat.size());
@@ -395,7 +411,8 @@ void build_device_scene(tr_ctx* c, const tr_bvh_opts* opts) {
     std::vector<int> tri_to_light(size_t(nt), -1);
     int nA = 0;
     {
-        std::vector<int> leaf_of(size_t(nt), 0);
+        std::vector<int>& leaf_of = c->leaf_of;
+        leaf_of.assign(size_t(nt), 0);
         for (int64_t i = 0; i < nt; ++i) leaf_of[size_t(r.tris[size_t(i)].prim)] = int(i);
         for (int64_t prim = 0; prim < nt && !h.emission.empty(); ++prim)
             if (h.emission[3 * prim] != 0.0f || h.emission[3 * prim + 1] != 0.0f || h.emission[3 * prim + 2] != 0.0f)
@@ -527,6 +544,27 @@ void build_device_scene(tr_ctx* c, const tr_bvh_opts* opts) {
         upload(c->tex8, p8.data(), p8.size() * sizeof(uchar4), st);
         upload(c->texf, pf.data(), pf.size() * sizeof(float4), st);
     }
+    // normal derivatives for the tangent frames of anisotropic materials
+    bool any_aniso = false;
+    for (const tr_material& m : h.mats) any_aniso = any_aniso || (m.type == TR_MAT_DISNEY && m.anisotropic != 0.0f);
+    const bool want_tan = any_aniso && !h.UV.empty() && !h.N.empty();
+    if (want_tan) {
+        std::vector<float4> dv(static_cast<size_t>(nt));
+        for (int64_t i = 0; i < nt; ++i) {
+            const int prim = r.tris[size_t(i)].prim;
+            // the shading record's vertex normals: normalised as Triangle::intersect does
+            f3 n[3];
+            float u[3], v[3];
+            for (int k = 0; k < 3; ++k) {
+                const int vi = h.idx[3 * size_t(prim) + k];
+                n[k] = xyz(shade[3 * size_t(i) + k]);
+                u[k] = h.UV[2 * size_t(vi)];
+                v[k] = h.UV[2 * size_t(vi) + 1];
+            }
+            dv[size_t(i)] = normal_dv(n[1] - n[0], n[2] - n[0], u[1] - u[0], v[1] - v[0], u[2] - u[0], v[2] - v[0]);
+        }
+        upload(c->tri_tan, dv.data(), dv.size() * sizeof(float4), st);
+    }
     const EnvMapHost& em = h.envmap;
     DevEnvMap env{};
     if (h.has_env && em.w > 0) {
@@ -581,6 +619,7 @@ void build_device_scene(tr_ctx* c, const tr_bvh_opts* opts) {
     s.env_L = mk(h.env[0], h.env[1], h.env[2]);
     s.env = env;
     s.tri_uv = any_tex ? c->tri_uv.as<float4>() : nullptr;
+    s.tri_tan = want_tan ? c->tri_tan.as<float4>() : nullptr;
     s.tex_info = any_tex ? c->tex_info.as<int4>() : nullptr;
     s.tex8 = any_tex ? c->tex8.as<uchar4>() : nullptr;
     s.texf = any_tex ? c->texf.as<float4>() : nullptr;
@@ -1201,6 +1240,31 @@ tr_status tr_camera_ray_batch(tr_ctx* c, const float* film_uv, const float* lens
         launch_camera_rays(c->ds, in, in + 2 * n, n, c->s_out.as<float4>(), c->stream);
         hip_check(hipGetLastError(), "k_camera_rays launch");
         hip_check(hipMemcpyAsync(rays, c->s_out.ptr, size_t(n) * 32, hipMemcpyDeviceToHost, c->stream), "D2H");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+    });
+}
+
+tr_status tr_shading_frame_batch(tr_ctx* c, const int32_t* prims, const float* bary, int64_t n, float* frames) {
+    return guarded(c, [&] {
+        if (!c->have_bvh) throw StateError("shading-frame batch before tr_bvh_build");
+        if (n < 0 || n > (int64_t(1) << 26)) throw InvalidArg("n must lie in [0, 2^26]");
+        if (!prims || !bary || !frames) throw InvalidArg("NULL array");
+        if (n == 0) return;
+        std::vector<int32_t> leaf(static_cast<size_t>(n));
+        const int64_t nt = int64_t(c->leaf_of.size());
+        for (int64_t i = 0; i < n; ++i) {
+            if (prims[i] < 0 || prims[i] >= nt) throw InvalidArg("primitive index out of range");
+            leaf[size_t(i)] = c->leaf_of[size_t(prims[i])];
+        }
+        c->s_in.reserve(size_t(n) * 12);
+        int* dtri = c->s_in.as<int>();
+        float* db = reinterpret_cast<float*>(dtri + n);
+        hip_check(hipMemcpyAsync(dtri, leaf.data(), size_t(n) * 4, hipMemcpyHostToDevice, c->stream), "H2D");
+        hip_check(hipMemcpyAsync(db, bary, size_t(n) * 8, hipMemcpyHostToDevice, c->stream), "H2D");
+        c->s_out.reserve(size_t(n) * 36);
+        launch_shading_frames(c->ds, dtri, db, n, c->s_out.as<float>(), c->stream);
+        hip_check(hipGetLastError(), "k_shading_frames launch");
+        hip_check(hipMemcpyAsync(frames, c->s_out.ptr, size_t(n) * 36, hipMemcpyDeviceToHost, c->stream), "D2H");
         hip_check(hipStreamSynchronize(c->stream), "sync");
     });
 }

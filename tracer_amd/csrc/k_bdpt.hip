@@ -101,7 +101,7 @@ TR_D void load_scatter(const DevScene& s, const BdVertex& v, BdScatter& c) {
     c.ng = xyz(v.n);
     const TriShade sr = load_shade(s, aux);
     c.ns = normalize(sr.n0 * (1.0f - c.b1 - c.b2) + sr.n1 * c.b1 + sr.n2 * c.b2);
-    onb(c.ns, c.tx, c.ty);
+    shading_frame(s, aux, c.ns, c.tx, c.ty);
     c.mat = sr.material;
     const DevMaterial& m = s.mats[sr.material];
     c.glass = m.type == MAT_GLASS;
@@ -379,7 +379,7 @@ TR_D bool extend_one(const DevScene& s, const BdPaths& b, const BdConsts& k, int
     if (cnt >= maxc) return false;
     const f3 ns = normalize(sr.n0 * (1.0f - b1 - b2) + sr.n1 * b1 + sr.n2 * b2);
     f3 tx, ty;
-    onb(ns, tx, ty);
+    shading_frame(s, tri, ns, tx, ty);
     const f3 lwo = mk(dot(wo, tx), dot(wo, ty), dot(wo, ns)), lng = mk(dot(ng, tx), dot(ng, ty), dot(ng, ns));
     const float su = rng.uniform(), u1 = rng.uniform(), u2 = rng.uniform();
     const BsdfSample bs = bsdf_sample(mat, lwo, lng, su, u1, u2);

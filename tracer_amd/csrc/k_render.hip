@@ -384,7 +384,7 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
                 const f3 pos = o + d * t;
                 const f3 ns = normalize(sr.n0 * (1.0f - b1 - b2) + sr.n1 * b1 + sr.n2 * b2);
                 f3 tx, ty;
-                onb(ns, tx, ty);
+                shading_frame(s, tri, ns, tx, ty);
                 auto to_local = [&](f3 v) { return mk(dot(v, tx), dot(v, ty), dot(v, ns)); };
                 DevMaterial mat = s.mats[sr.material];
                 if (EXT && mat.tex > 0) apply_texture(s, mat, tri, b1, b2);
@@ -1029,6 +1029,17 @@ void launch_lc_build(const DevLightCache& lc, const float* light_pmf, float eps,
 // Test batches through the renderer's own device functions: camera rays
 // (ThinLensCamera::generate_ray), light selection (pick_light), HG sampling,
 // and film samples with given radiance for k_accumulate.
+__global__ void k_shading_frames(DevScene s, const int* tri, const float* bary, int64_t n, float* out) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float b1 = bary[2 * i], b2 = bary[2 * i + 1];
+    const TriShade sr = load_shade(s, tri[i]);
+    const f3 ns = normalize(sr.n0 * (1.0f - b1 - b2) + sr.n1 * b1 + sr.n2 * b2);
+    f3 tx, ty;
+    shading_frame(s, tri[i], ns, tx, ty);
+    const float v[9] = {tx.x, tx.y, tx.z, ty.x, ty.y, ty.z, ns.x, ns.y, ns.z};
+    for (int k = 0; k < 9; ++k) out[9 * i + k] = v[k];
+}
 __global__ void k_camera_rays(DevScene s, const float* film_uv, const float* lens_uv, int64_t n, float4* rays) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -1056,6 +1067,11 @@ __global__ void k_unpack_radiance(const float* L, int64_t n, float4* rad) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     rad[i] = make_float4(L[3 * i], L[3 * i + 1], L[3 * i + 2], 0.0f);
+}
+void launch_shading_frames(const DevScene& s, const int* tri, const float* bary, int64_t n, float* frames,
+                           hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_shading_frames, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, s, tri, bary, n, frames);
 }
 void launch_camera_rays(const DevScene& s, const float* film_uv, const float* lens_uv, int64_t n, float4* rays,
                         hipStream_t st) {

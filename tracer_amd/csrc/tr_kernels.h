@@ -159,6 +159,8 @@ void launch_tex_eval(const DevScene& s, int texture, const float* uv, int64_t n,
 // Light-cache sampling tables (pmf, alias table) of every cell from the training sums.
 void launch_lc_build(const DevLightCache& lc, const float* light_pmf, float eps, int cells, float* pmf, float2* table,
                      int* work, hipStream_t st);
+void launch_shading_frames(const DevScene& s, const int* tri, const float* bary, int64_t n, float* frames,
+                           hipStream_t st);
 void launch_camera_rays(const DevScene& s, const float* film_uv, const float* lens_uv, int64_t n, float4* rays,
                         hipStream_t st);
 void launch_light_pick(const DevScene& s, const float* u, int64_t n, int* idx, float* pmf, hipStream_t st);

@@ -66,6 +66,10 @@ struct DevScene {
     // material textures (has_tex): per-triangle uvs (2 float4 in leaf order),
     // per texture (w, h, 8-bit, texel offset), 8-bit and float texel pools
     const float4* tri_uv;
+    // per leaf-ordered triangle: the normal derivative along v of
+    // compute_ss_ts (xyz) and 1 in w where it is used (scenes with texture
+    // coordinates and an anisotropic material; NULL otherwise)
+    const float4* tri_tan;
     const int4* tex_info;
     const uchar4* tex8;
     const float4* texf;
@@ -95,6 +99,26 @@ TR_D TriShade load_shade(const DevScene& s, int tri) {
     r.med_in = (med << 16) >> 16;   // sign-extended low half
     r.med_out = med >> 16;
     return r;
+}
+
+// Tangent axes of a surface hit as SurfaceIntersection::shading_coord
+// (src/shape/triangle.cpp:200-211) builds them around the interpolated
+// normal ns: x = cross(dndv, ns), y = cross(ns, x), normalised.  dndv is the
+// triangle's normal derivative along v (compute_ss_ts,
+// src/utility/coordinate.hpp:54-68) where the table holds one, else the
+// second axis of coordinate(ns) (src/utility/geometry.hpp:887-894), for which
+// x is coordinate's first axis.  Only anisotropic BSDFs see the difference
+// from any other frame around ns.
+TR_D void shading_frame(const DevScene& s, int tri, f3 ns, f3& tx, f3& ty) {
+    const float4 tv = s.tri_tan ? s.tri_tan[tri] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (tv.w != 0.0f) {
+        tx = normalize(cross(xyz(tv), ns));
+    } else if (fabsf(ns.x) > fabsf(ns.y)) {
+        tx = mk(-ns.z, 0.0f, ns.x) * (1.0f / sqrtf(ns.x * ns.x + ns.z * ns.z));
+    } else {
+        tx = mk(0.0f, ns.z, -ns.y) * (1.0f / sqrtf(ns.y * ns.y + ns.z * ns.z));
+    }
+    ty = cross(ns, tx);
 }
 
 // Light selection by power (compute_light_power_distribution,

@@ -73,6 +73,7 @@ def hip_lib() -> C.CDLL:
         lib.tr_texture_eval_batch.argtypes = [VP, C.c_int32, P(C.c_float), I64, P(C.c_float)]
         F = P(C.c_float)
         lib.tr_camera_ray_batch.argtypes = [VP, F, F, I64, F]
+        lib.tr_shading_frame_batch.argtypes = [VP, P(I32), F, I64, F]
         lib.tr_set_camera.argtypes = [VP, P(abi.TrCamera)]
         lib.tr_light_pmf.argtypes = [VP, F, I64, P(I64)]
         lib.tr_light_pick_batch.argtypes = [VP, F, I64, P(C.c_int32), F]
@@ -585,6 +586,15 @@ class Context:
         out = np.zeros((f.shape[0], 8), np.float32)
         self._check(self.lib.tr_camera_ray_batch(self.h, fptr(f), fptr(l), f.shape[0], fptr(out)))
         return out
+
+    def shading_frames(self, prims: np.ndarray, bary: np.ndarray) -> np.ndarray:
+        """Device shading frames at hits of prims (scene order) with barycentrics bary (n, 2) -> (n, 3, 3), rows x, y, z."""
+        t = np.ascontiguousarray(prims, np.int32).reshape(-1)
+        b = np.ascontiguousarray(bary, np.float32).reshape(-1, 2)
+        out = np.zeros((t.shape[0], 9), np.float32)
+        self._check(self.lib.tr_shading_frame_batch(self.h, t.ctypes.data_as(C.POINTER(C.c_int32)), fptr(b), t.shape[0],
+                                                    fptr(out)))
+        return out.reshape(-1, 3, 3)
 
     def light_pmf(self) -> np.ndarray:
         """Light-selection pmf: emissive triangles in scene order, environment, sun."""
