@@ -242,6 +242,10 @@ class Material {
 public:
     virtual ~Material() = default;
 };
+// Disney material (src/factory/material.hpp:37-52, src/material/disney.cpp):
+// every parameter a constant or an image texture read at each hit's uv
+// (scalars from the red channel), and the normal map of normal_mapper, if it
+// has one, turning the shading frame per hit.
 RC<Material> create_disney(RC<const Texture2D> base_color, RC<const Texture2D> metallic, RC<const Texture2D> roughness,
                            RC<const Texture2D> transmission, RC<const Texture2D> transmission_roughness,
                            RC<const Texture2D> ior, RC<const Texture2D> specular_scale, RC<const Texture2D> specular_tint,

@@ -65,7 +65,26 @@ typedef struct tr_material {
      * (kd of TR_MAT_PHONG, base_color of TR_MAT_DISNEY) at the hit's
      * interpolated uv (Texture2D::evaluate, src/core/texture.hpp:57-83). */
     int32_t base_color_texture;
+    /* TR_MAT_DISNEY only: 0: the constant above; k > 0: textures[k - 1] gives
+     * the parameter per hit at the hit's uv (Disney::shading evaluates every
+     * texture, src/material/disney.cpp:604-621; a scalar parameter takes the
+     * red channel, Texture2D::evaluate_s, src/core/texture.hpp:73-82), in the
+     * order metallic, roughness, transmission, transmission_roughness, ior,
+     * specular_scale (RGB), specular_tint, anisotropic, sheen, sheen_tint,
+     * clearcoat, clearcoat_gloss (TR_PARAM_*). */
+    int32_t param_texture[12];
+    /* TR_MAT_DISNEY only: 0: none; k > 0: textures[k - 1] is the normal map of
+     * NormalMapper::reorient (src/core/material.hpp:37-48): the texel c gives
+     * the tangent-space normal 2c - 1, and the shading frame is turned so
+     * that its z axis is that normal. */
+    int32_t normal_texture;
 } tr_material;
+
+enum {
+    TR_PARAM_METALLIC = 0, TR_PARAM_ROUGHNESS, TR_PARAM_TRANSMISSION, TR_PARAM_TRANSMISSION_ROUGHNESS, TR_PARAM_IOR,
+    TR_PARAM_SPECULAR_SCALE, TR_PARAM_SPECULAR_TINT, TR_PARAM_ANISOTROPIC, TR_PARAM_SHEEN, TR_PARAM_SHEEN_TINT,
+    TR_PARAM_CLEARCOAT, TR_PARAM_CLEARCOAT_GLOSS, TR_PARAM_COUNT
+};
 
 /* An image texture (create_texture2d_from_file, an MTL map): height rows of
  * width RGB texels, 8-bit (rgb8, read as value / 255) or float (rgbf); the

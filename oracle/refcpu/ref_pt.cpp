@@ -420,7 +420,40 @@ static void make_bsdf(const Scene& s, const Isect& is, bool quirks, SurfBSDF& b)
         }
     }
     b.type = m.type;
-    if (m.type == TR_MAT_DISNEY) b.dis.reset(new DisneyRef<float>(is.geo, is.shd, to_params(m), !quirks));
+    if (m.type == TR_MAT_DISNEY) {
+        // Disney::shading (disney.cpp:604-621): every parameter texture at the
+        // hit's uv, evaluate_s taking the red channel (texture.hpp:73-82), and
+        // the shading frame turned by NormalMapper::reorient (material.hpp:37-48)
+        DisneyParams p = to_params(m);
+        auto scal = [&](int k, float c) {
+            return m.param_texture[k] > 0 ? s.tex[size_t(m.param_texture[k]) - 1].eval(is.u, is.v, quirks).r : c;
+        };
+        p.metallic = scal(TR_PARAM_METALLIC, p.metallic);
+        p.roughness = scal(TR_PARAM_ROUGHNESS, p.roughness);
+        p.transmission = scal(TR_PARAM_TRANSMISSION, p.transmission);
+        p.transmission_roughness = scal(TR_PARAM_TRANSMISSION_ROUGHNESS, p.transmission_roughness);
+        p.ior = scal(TR_PARAM_IOR, p.ior);
+        if (m.param_texture[TR_PARAM_SPECULAR_SCALE] > 0) {
+            const RGB<float> c = s.tex[size_t(m.param_texture[TR_PARAM_SPECULAR_SCALE]) - 1].eval(is.u, is.v, quirks);
+            p.specular_scale[0] = c.r; p.specular_scale[1] = c.g; p.specular_scale[2] = c.b;
+        }
+        p.specular_tint = scal(TR_PARAM_SPECULAR_TINT, p.specular_tint);
+        p.anisotropic = scal(TR_PARAM_ANISOTROPIC, p.anisotropic);
+        p.sheen = scal(TR_PARAM_SHEEN, p.sheen);
+        p.sheen_tint = scal(TR_PARAM_SHEEN_TINT, p.sheen_tint);
+        p.clearcoat = scal(TR_PARAM_CLEARCOAT, p.clearcoat);
+        p.clearcoat_gloss = scal(TR_PARAM_CLEARCOAT_GLOSS, p.clearcoat_gloss);
+        Coord<float> shd = is.shd;
+        if (m.normal_texture > 0) {
+            const RGB<float> c = s.tex[size_t(m.normal_texture) - 1].eval(is.u, is.v, quirks);
+            const V3f ln = V3f(c.r * 2 - 1, c.g * 2 - 1, c.b * 2 - 1).normalize();
+            const V3f wn = shd.to_global(ln);
+            const V3f nx = cross(shd.y, wn);   // coordinate.hpp:107-113
+            const V3f ny = cross(wn, nx);
+            shd = Coord<float>(nx, ny, wn);
+        }
+        b.dis.reset(new DisneyRef<float>(is.geo, shd, p, !quirks));
+    }
     else if (m.type == TR_MAT_PHONG) b.pho.reset(new PhongRef<float>(is.geo, is.shd, phong_kd<float>(m), phong_ks<float>(m), m.shininess, !quirks));
     else if (m.type == TR_MAT_GLASS) b.gla.reset(new GlassRef<float>(is.geo, is.shd, phong_kd<float>(m), phong_ks<float>(m), m.ior));
     else b.lam.reset(new LambertRef<float>(is.geo, is.shd, RGB<float>(m.base_color[0], m.base_color[1], m.base_color[2]), !quirks));

@@ -234,6 +234,55 @@ def test_bdpt_obj_room_matches_path_tracer(gpu_ctx, tmp_path, fog, glass):
     _compare_bdpt(gpu_ctx, s, W, H, 256, 8, 8, nc, nl, pt)
 
 
+def _env_ground(half=4.0, rho=0.5, env=1.0):
+    """A Lambert quad (rho) at y = 0 under a constant environment (env),
+    seen from straight above by a narrow pinhole camera: every pixel sees the
+    quad, and the quad sees the whole upper hemisphere of sky."""
+    from tracer_amd import native
+    P = np.array([[-half, 0, -half], [half, 0, -half], [half, 0, half], [-half, 0, half]], np.float32)
+    cam = abi.TrCamera()
+    cam.pos[:] = (0.0, 2.0, 0.0)
+    cam.target[:] = (0.0, 0.0, 0.001)
+    cam.up[:] = (0.0, 0.0, 1.0)
+    cam.fov_y, cam.focal_distance, cam.aspect = 0.7, 1.0, 1.0
+    return native.Scene(positions=P, indices=np.array([[0, 2, 1], [0, 3, 2]], np.int32),
+                        material_ids=np.zeros(2, np.int32),
+                        materials=[abi.make_material(type=abi.TR_MAT_LAMBERT, base_color=(rho, rho, rho))],
+                        env=(env, env, env), camera=cam)
+
+
+@pytest.mark.parametrize("only_s", [0, 1, 2, 3])
+def test_bdpt_environment_ground_closed_form(gpu_ctx, only_s):
+    """Radiance rho L = 0.5 at every pixel, through the weighted
+    combination (only_s = 0) and through each strategy alone: camera
+    subpaths that escape to the environment (s = 0), light samples of the
+    environment (s = 1) and light subpaths that start on the environment's
+    disk and are traced to the camera (s = 2).  The last needs the disk on the
+    side the light comes from (the reference's IBL::sample_le sends its
+    subpaths away from the scene)."""
+    s = _env_ground()
+    m, se = _furnace_z(gpu_ctx, s, 3, 3, seeds=6, spp=64 if only_s != 3 else 256, only_s=only_s)
+    z = (m - 0.5) / np.maximum(se, 1e-9)
+    print(f"only_s={only_s}: mean {m}, se {se}, z {z}")
+    assert np.abs(z).max() < 5.0 and np.abs(m / 0.5 - 1).max() < 0.02
+
+
+def test_bdpt_textured_disney_matches_path_tracer(gpu_ctx):
+    """Per-hit Disney parameters on both subpaths, over the interpolated
+    normals of a UV sphere under an area light and the environment: the
+    textured sphere scene against the GPU path tracer."""
+    from tests.uvsphere import textured_disney_sphere_scene
+    s = textured_disney_sphere_scene(normal_map=False)
+    nc = nl = 5
+    W = H = 32
+
+    def pt(seed):
+        p = s.render_params(width=W, height=H, spp=1024, seed=9000 + seed, max_depth=nc + nl - 2, min_depth=64)
+        return gpu_ctx.render(p, want_image=False)[0]
+
+    _compare_bdpt(gpu_ctx, s, W, H, 256, 8, 8, nc, nl, pt)
+
+
 def test_bdpt_is_reproducible_and_progressive(gpu_ctx):
     from tracer_amd import native
     s = native.config_scene(1)

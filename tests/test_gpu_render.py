@@ -143,6 +143,18 @@ def test_anisotropic_sphere_matches_fixed_estimator(gpu_ctx, anisotropic):
     _compare_estimators(gpu_ctx, s, 32, 32, 512, 64, 8, 8, quirks=False, nee=0, max_depth=5)
 
 
+@pytest.mark.parametrize("normal_map", [False, True])
+def test_disney_parameter_textures_match_fixed_estimator(gpu_ctx, normal_map):
+    """Disney::shading with image textures (disney.cpp:604-621): base
+    colour, metallic, roughness, specular scale, clearcoat and sheen read per
+    hit from float textures, with and without a bump normal map
+    (NormalMapper::reorient, material.hpp:37-48), against the oracle in fixed
+    mode."""
+    from tests.uvsphere import textured_disney_sphere_scene
+    s = textured_disney_sphere_scene(normal_map=normal_map)
+    _compare_estimators(gpu_ctx, s, 32, 32, 512, 64, 8, 8, quirks=False, nee=0, max_depth=5)
+
+
 def test_c3_medium_matches_fixed_estimator(gpu_ctx):
     """The closed fog-filled room of configuration 3 (one sphere), delta
     tracking on the GPU against the oracle's free-flight sampling in fixed

@@ -366,3 +366,42 @@ def test_shading_frame_follows_the_uv_parameterisation():
     big_x = np.abs(z[:, 0]) > np.abs(z[:, 1])
     assert np.abs(x[big_x, 1]).max() < 1e-6 and np.abs(x[~big_x, 0]).max() < 1e-6
     assert np.abs(np.einsum('ij,ij->i', x, z)).max() < 1e-5
+
+
+def test_disney_parameter_textures_reproduce_constants():
+    """Textured Disney parameters and a flat normal map in the oracle
+    (Disney::shading, disney.cpp:604-621; NormalMapper::reorient,
+    material.hpp:37-48): every parameter read from a texture filled with its
+    constant, and a normal map of texels (0.5, 0.5, 1) (the unperturbed
+    tangent-space normal), render what the constant material renders with the
+    same random numbers, up to the rounding of the lookups and of the
+    reoriented frame.  A normal map tilting every normal changes the image."""
+    from oracle import refcpu
+    from tests.uvsphere import anisotropic_sphere_scene
+    const = dict(base_color=(0.7, 0.5, 0.3), metallic=0.4, roughness=0.35, transmission=0.0, ior=1.5,
+                 transmission_roughness=0.4,
+                 specular_scale=(0.9, 0.8, 0.7), specular_tint=0.2, anisotropic=0.5, sheen=0.3, sheen_tint=0.4,
+                 clearcoat=0.6, clearcoat_gloss=0.8)
+    plain = anisotropic_sphere_scene(abi.make_material(type=abi.TR_MAT_DISNEY, **const))
+    names = abi.DISNEY_PARAMS
+    texs = [np.full((2, 2, 3), const["base_color"], np.float32)]
+    for k in names:
+        v = const[k]
+        texs.append(np.full((2, 2, 3), v if np.ndim(v) else (v, v, v), np.float32))
+    texs.append(np.full((2, 2, 3), (0.5, 0.5, 1.0), np.float32))
+    zero = {k: (0.0 if k != "specular_scale" else (0.0, 0.0, 0.0)) for k in names}
+    mat = abi.make_material(type=abi.TR_MAT_DISNEY, base_color=(0.0, 0.0, 0.0), base_color_texture=1,
+                            param_textures={k: 2 + i for i, k in enumerate(names)}, normal_texture=2 + len(names),
+                            **zero)
+    textured = anisotropic_sphere_scene(mat)
+    textured.textures = texs
+    p = dict(width=24, height=24, spp=8, seed=3, max_depth=4)
+    a = refcpu.RefScene(plain, 3).render(plain.render_params(**p), quirks=False, nee=0, threads=4)[0]
+    b = refcpu.RefScene(textured, 3).render(textured.render_params(**p), quirks=False, nee=0, threads=4)[0]
+    ia, ib = a[..., :3] / a[..., 3:], b[..., :3] / b[..., 3:]
+    assert np.abs(ia - ib).max() <= 1e-3 * max(1.0, ia.max()), np.abs(ia - ib).max()
+    assert np.abs(ia - ib).mean() < 1e-5
+    tilted = anisotropic_sphere_scene(mat)
+    tilted.textures = texs[:-1] + [np.full((2, 2, 3), (0.85, 0.5, 0.85), np.float32)]
+    c = refcpu.RefScene(tilted, 3).render(tilted.render_params(**p), quirks=False, nee=0, threads=4)[0]
+    assert np.abs(ia - c[..., :3] / c[..., 3:]).mean() > 1e-2

@@ -65,3 +65,36 @@ def anisotropic_sphere_scene(material, light=(0.0, 2.5, 0.0), env=0.3):
     return Scene(positions=np.concatenate(pos), indices=np.concatenate(idx), material_ids=np.concatenate(mid),
                  materials=mats, normals=np.concatenate(nrm), uvs=np.concatenate(uvs), emission=np.concatenate(emis),
                  env=(env, env, env), camera=cam)
+
+
+def textured_disney_sphere_scene(normal_map=True, seed=0):
+    """The sphere scene with a Disney material whose base colour, metallic,
+    roughness, specular scale, clearcoat and sheen come from float textures
+    (stripes and checks in u and v) and, optionally, a bump normal map
+    (tangent-space normals tilted by up to ~30 degrees, encoded as (n + 1) / 2).
+    Returns the scene; textures are numbered from 1 in Scene.textures."""
+    rng = np.random.default_rng(seed)
+    h, w = 32, 64
+    v, u = np.meshgrid((np.arange(h) + 0.5) / h, (np.arange(w) + 0.5) / w, indexing="ij")
+    base = np.stack([0.2 + 0.7 * u, 0.3 + 0.5 * v, 0.8 - 0.6 * u], -1)
+    checks = ((np.floor(u * 8) + np.floor(v * 4)) % 2)
+    metallic = np.repeat(checks[..., None], 3, -1)
+    rough = np.repeat((0.1 + 0.8 * (0.5 + 0.5 * np.sin(2 * np.pi * 3 * u)))[..., None], 3, -1)
+    spec = np.stack([0.5 + 0.5 * v, 0.5 + 0.3 * u, np.full_like(u, 0.8)], -1)
+    coat = np.repeat((v > 0.5).astype(float)[..., None], 3, -1)
+    sheen = np.repeat((0.5 * u)[..., None], 3, -1)
+    texs = [base, metallic, rough, spec, coat, sheen]
+    if normal_map:
+        tx = 0.6 * np.sin(2 * np.pi * 6 * u)
+        ty = 0.6 * np.sin(2 * np.pi * 3 * v)
+        n = np.stack([tx, ty, np.ones_like(u)], -1)
+        n /= np.linalg.norm(n, axis=-1, keepdims=True)
+        texs.append((n + 1) / 2)
+    mat = abi.make_material(type=abi.TR_MAT_DISNEY, base_color_texture=1, roughness=0.5, transmission=0.0, ior=1.5,
+                            sheen_tint=0.5, clearcoat_gloss=0.7,
+                            param_textures={"metallic": 2, "roughness": 3, "specular_scale": 4, "clearcoat": 5,
+                                            "sheen": 6},
+                            normal_texture=7 if normal_map else 0)
+    s = anisotropic_sphere_scene(mat)
+    s.textures = [t.astype(np.float32) for t in texs]
+    return s

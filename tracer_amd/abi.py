@@ -28,6 +28,8 @@ class TrMaterial(C.Structure):
         ("sheen_tint", C.c_float), ("clearcoat", C.c_float), ("clearcoat_gloss", C.c_float),
         ("shininess", C.c_float),
         ("base_color_texture", C.c_int32),
+        ("param_texture", C.c_int32 * 12),
+        ("normal_texture", C.c_int32),
     ]
 
 
@@ -178,6 +180,10 @@ def dptr(a: np.ndarray | None):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
+# order of tr_material::param_texture (TR_PARAM_*)
+DISNEY_PARAMS = ["metallic", "roughness", "transmission", "transmission_roughness", "ior", "specular_scale",
+                 "specular_tint", "anisotropic", "sheen", "sheen_tint", "clearcoat", "clearcoat_gloss"]
+
 MATERIAL_FIELDS = ["type", "base_color", "metallic", "roughness", "transmission", "transmission_roughness", "ior",
                    "specular_scale", "specular_tint", "anisotropic", "sheen", "sheen_tint", "clearcoat",
                    "clearcoat_gloss", "shininess", "base_color_texture"]
@@ -197,6 +203,9 @@ def make_material(**kw) -> TrMaterial:
     for k, v in defaults.items():
         setattr(m, k, float(kw.get(k, v)))
     m.base_color_texture = int(kw.get("base_color_texture", 0))
+    for k, t in (kw.get("param_textures") or {}).items():   # {"roughness": 1-based texture id, ...}
+        m.param_texture[DISNEY_PARAMS.index(k)] = int(t)
+    m.normal_texture = int(kw.get("normal_texture", 0))
     return m
 
 

@@ -20,7 +20,7 @@ using namespace trhip;
 using namespace trcapi;
 
 // The ctypes bindings (tracer_amd/abi.py) assume these layouts.
-static_assert(sizeof(tr_material) == 80, "tr_material layout");
+static_assert(sizeof(tr_material) == 132, "tr_material layout");
 static_assert(sizeof(tr_medium) == 104, "tr_medium layout");
 static_assert(sizeof(tr_camera) == 52, "tr_camera layout");
 static_assert(sizeof(tr_render_params) == 96, "tr_render_params layout");
@@ -109,37 +109,40 @@ DevMaterial make_material(const tr_material& m) {
     return d;
 }
 
+// Whether a material reads a texture per hit (base colour / kd, a Disney
+// parameter or a normal map); has_param_textures: the last two.
+bool has_param_textures(const tr_material& m) {
+    bool any = m.normal_texture > 0;
+    for (int k = 0; k < TR_PARAM_COUNT; ++k) any = any || m.param_texture[k] > 0;
+    return any;
+}
+bool uses_texture(const tr_material& m) { return m.base_color_texture > 0 || has_param_textures(m); }
+
+// The constant parameters of a Disney (or Lambert) material.
+DisneyIn disney_in(const tr_material& m) {
+    DisneyIn p;
+    p.base_color = mk(m.base_color[0], m.base_color[1], m.base_color[2]);
+    p.metallic = m.metallic;
+    p.roughness = m.roughness;
+    p.transmission = m.transmission;
+    p.transmission_roughness = m.transmission_roughness;
+    p.ior = m.ior;
+    p.specular_scale = mk(m.specular_scale[0], m.specular_scale[1], m.specular_scale[2]);
+    p.specular_tint = m.specular_tint;
+    p.anisotropic = m.anisotropic;
+    p.sheen = m.sheen;
+    p.sheen_tint = m.sheen_tint;
+    p.clearcoat = m.clearcoat;
+    p.clearcoat_gloss = m.clearcoat_gloss;
+    return p;
+}
+
 DevMaterial make_material_untextured(const tr_material& m) {
     if (m.type == TR_MAT_PHONG) return make_phong(m);
     if (m.type == TR_MAT_GLASS) return make_glass(m);
     DevMaterial d{};
     d.type = m.type == TR_MAT_DISNEY ? MAT_DISNEY : MAT_LAMBERT;
-    d.C = mk(m.base_color[0], m.base_color[1], m.base_color[2]);
-    const float l = lum(d.C);
-    d.Ctint = l > 0.0f ? d.C * (1.0f / l) : f3s(1.0f);
-    d.metallic = m.metallic;
-    d.roughness = m.roughness;
-    d.transmission = m.transmission;
-    d.ior = std::max(1.01f, m.ior);
-    d.spec_scale = mk(m.specular_scale[0], m.specular_scale[1], m.specular_scale[2]);
-    d.spec_tint = m.specular_tint;
-    d.sheen = m.sheen;
-    d.sheen_tint = m.sheen_tint;
-    d.clearcoat = m.clearcoat;
-    const float aspect = m.anisotropic > 0.0f ? std::sqrt(1.0f - 0.9f * m.anisotropic) : 1.0f;
-    d.ax = std::max(0.001f, m.roughness * m.roughness / aspect);
-    d.ay = std::max(0.001f, m.roughness * m.roughness * aspect);
-    d.tax = std::max(0.001f, m.transmission_roughness * m.transmission_roughness / aspect);
-    d.tay = std::max(0.001f, m.transmission_roughness * m.transmission_roughness * aspect);
-    float cr = (1.0f - m.clearcoat_gloss) * 0.1f + m.clearcoat_gloss * 0.0f;
-    cr *= cr;
-    d.cc_rough = std::max(cr, 0.0001f);
-    const float A = std::min(std::max(l * (1.0f - m.metallic), 0.3f), 0.7f);
-    const float B = 1.0f - A;
-    d.w_diffuse = A * (1.0f - m.transmission);
-    d.w_trans = A * m.transmission;
-    d.w_spec = B * 2.0f / (2.0f + m.clearcoat);
-    d.w_cc = B * m.clearcoat / (2.0f + m.clearcoat);
+    disney_setup(disney_in(m), d);
     if (d.type == MAT_LAMBERT) {
         d.metallic = 0.0f;
         d.transmission = 0.0f;
@@ -294,10 +297,19 @@ void validate_scene(const tr_scene_desc* d) {
                     throw InvalidArg("texture " + std::to_string(i) + ": texels must be finite and >= 0");
     }
     for (int i = 0; i < d->num_materials; ++i) {
-        const int t = d->materials[i].base_color_texture;
-        if (t < 0 || t > d->num_textures) throw InvalidArg("material " + std::to_string(i) + ": texture id out of range");
-        if (t > 0 && !d->uvs) throw InvalidArg("a textured material needs vertex uvs");
-        if (t > 0 && d->materials[i].type == TR_MAT_GLASS) throw InvalidArg("glass takes no texture");
+        const tr_material& m = d->materials[i];
+        int ids[TR_PARAM_COUNT + 2];
+        ids[0] = m.base_color_texture;
+        for (int k = 0; k < TR_PARAM_COUNT; ++k) ids[1 + k] = m.param_texture[k];
+        ids[TR_PARAM_COUNT + 1] = m.normal_texture;
+        for (int k = 0; k < TR_PARAM_COUNT + 2; ++k) {
+            const int t = ids[k];
+            if (t < 0 || t > d->num_textures) throw InvalidArg("material " + std::to_string(i) + ": texture id out of range");
+            if (t > 0 && !d->uvs) throw InvalidArg("a textured material needs vertex uvs");
+            if (t > 0 && m.type == TR_MAT_GLASS) throw InvalidArg("glass takes no texture");
+            if (t > 0 && k > 0 && m.type != TR_MAT_DISNEY)
+                throw InvalidArg("only Disney materials take parameter textures and normal maps");
+        }
     }
     for (int i = 0; i < d->num_media; ++i) {
         const tr_medium& m = d->media[i];
@@ -481,9 +493,28 @@ void build_device_scene(tr_ctx* c, const tr_bvh_opts* opts) {
 
     std::vector<DevMaterial> dm(h.mats.size());
     c->has_delta = false;
+    // Disney materials with parameter textures or a normal map: one row each
+    // of texture ids (base colour, the twelve parameters, the normal map) and
+    // constant parameters, referenced by DevMaterial::texset
+    std::vector<int> mat_tex;
+    std::vector<float> mat_raw;
     for (size_t i = 0; i < h.mats.size(); ++i) {
-        dm[i] = make_material(h.mats[i]);
+        const tr_material& m = h.mats[i];
+        dm[i] = make_material(m);
         c->has_delta = c->has_delta || dm[i].type == MAT_GLASS;
+        if (m.type == TR_MAT_DISNEY && has_param_textures(m)) {
+            dm[i].texset = int(mat_tex.size() / kMatTexRow) + 1;
+            mat_tex.push_back(m.base_color_texture);
+            for (int k = 0; k < TR_PARAM_COUNT; ++k) mat_tex.push_back(m.param_texture[k]);
+            mat_tex.push_back(m.normal_texture);
+            mat_tex.resize(size_t(dm[i].texset) * kMatTexRow, 0);
+            const DisneyIn p = disney_in(m);
+            const float raw[kMatRawRow] = {p.base_color.x, p.base_color.y, p.base_color.z, p.metallic, p.roughness,
+                                           p.transmission, p.transmission_roughness, p.ior, p.specular_scale.x,
+                                           p.specular_scale.y, p.specular_scale.z, p.specular_tint, p.anisotropic,
+                                           p.sheen, p.sheen_tint, p.clearcoat, p.clearcoat_gloss, 0.0f, 0.0f, 0.0f};
+            mat_raw.insert(mat_raw.end(), raw, raw + kMatRawRow);
+        }
     }
     std::vector<DevMedium> dmed(h.media.size());
     std::vector<float> voxels;
@@ -510,7 +541,11 @@ void build_device_scene(tr_ctx* c, const tr_bvh_opts* opts) {
     upload(c->light_guide, guide.data(), guide.size() * sizeof(int), st);
     // material textures: per-triangle uvs in leaf order, texel pools, a table
     bool any_tex = false;
-    for (const tr_material& m : h.mats) any_tex = any_tex || m.base_color_texture > 0;
+    for (const tr_material& m : h.mats) any_tex = any_tex || uses_texture(m);
+    if (!mat_tex.empty()) {
+        upload(c->mat_tex, mat_tex.data(), mat_tex.size() * sizeof(int), st);
+        upload(c->mat_raw, mat_raw.data(), mat_raw.size() * sizeof(float), st);
+    }
     if (any_tex) {
         std::vector<float4> uv(size_t(nt) * 2);
         for (int64_t i = 0; i < nt; ++i) {
@@ -544,10 +579,14 @@ void build_device_scene(tr_ctx* c, const tr_bvh_opts* opts) {
         upload(c->tex8, p8.data(), p8.size() * sizeof(uchar4), st);
         upload(c->texf, pf.data(), pf.size() * sizeof(float4), st);
     }
-    // normal derivatives for the tangent frames of anisotropic materials
-    bool any_aniso = false;
-    for (const tr_material& m : h.mats) any_aniso = any_aniso || (m.type == TR_MAT_DISNEY && m.anisotropic != 0.0f);
-    const bool want_tan = any_aniso && !h.UV.empty() && !h.N.empty();
+    // normal derivatives for the tangent frames of anisotropic materials and
+    // normal maps (the only users of the frame's orientation about the normal)
+    bool any_oriented = false;
+    for (const tr_material& m : h.mats)
+        any_oriented = any_oriented || (m.type == TR_MAT_DISNEY &&
+                                        (m.anisotropic != 0.0f || m.param_texture[TR_PARAM_ANISOTROPIC] > 0 ||
+                                         m.normal_texture > 0));
+    const bool want_tan = any_oriented && !h.UV.empty() && !h.N.empty();
     if (want_tan) {
         std::vector<float4> dv(static_cast<size_t>(nt));
         for (int64_t i = 0; i < nt; ++i) {
@@ -619,6 +658,8 @@ void build_device_scene(tr_ctx* c, const tr_bvh_opts* opts) {
     s.env_L = mk(h.env[0], h.env[1], h.env[2]);
     s.env = env;
     s.tri_uv = any_tex ? c->tri_uv.as<float4>() : nullptr;
+    s.mat_tex = mat_tex.empty() ? nullptr : c->mat_tex.as<int>();
+    s.mat_raw = mat_raw.empty() ? nullptr : c->mat_raw.as<float>();
     s.tri_tan = want_tan ? c->tri_tan.as<float4>() : nullptr;
     s.tex_info = any_tex ? c->tex_info.as<int4>() : nullptr;
     s.tex8 = any_tex ? c->tex8.as<uchar4>() : nullptr;
@@ -1186,7 +1227,7 @@ tr_status tr_texture_eval_batch(tr_ctx* c, int32_t texture, const float* uv, int
         if (!c->have_bvh) throw StateError("texture batch before tr_bvh_build");
         if (texture < 1 || size_t(texture) > c->hs.tex.size()) throw InvalidArg("texture id out of range");
         bool used = false;   // texture tables are uploaded only for scenes whose materials name one
-        for (const tr_material& m : c->hs.mats) used = used || m.base_color_texture > 0;
+        for (const tr_material& m : c->hs.mats) used = used || uses_texture(m);
         if (!used) throw StateError("no material of the scene names a texture");
         if (n < 0 || n > (int64_t(1) << 26)) throw InvalidArg("n must lie in [0, 2^26]");
         if (!uv || !rgb) throw InvalidArg("NULL array");

@@ -148,6 +148,7 @@ struct tr_ctx {
     trcapi::DevBuf env_tex, env_cond_cdf, env_cond_func, env_marg_cdf, env_marg_func;
     trcapi::DevBuf tri_uv, tex_info, tex8, texf;   // material textures
     trcapi::DevBuf tri_tan;                        // normal derivatives (anisotropic materials)
+    trcapi::DevBuf mat_tex, mat_raw;               // per-hit Disney parameters: texture ids, constants
     std::vector<int32_t> leaf_of;                  // leaf-order index of each primitive
     trcapi::DevBuf density;                        // voxels of every grid medium
     int64_t num_nodes = 0;

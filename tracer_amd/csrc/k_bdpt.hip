@@ -6,7 +6,8 @@
 // (src/renderer/bdpt_renderer.cpp:289-1185).  The strategies, the vertex
 // densities and their reassignment per strategy follow the reference; three
 // of its weights are corrected, each of which leaves the reference's
-// combination of strategies biased:
+// combination of strategies biased (and so are two more of its choices, the
+// environment's emission disk and the shading-normal adjoint, below):
 //   * compute_mis_weight (bdpt_renderer.cpp:741-762) never counts the
 //     strategy with no light vertex: its light loop adds a ratio only for
 //     i >= 1.  A path that the camera subpath reaches on its own is then
@@ -23,6 +24,17 @@
 //   * mis_weight_tx_s0 (:816-830) evaluates the environment's emission
 //     density at light_to_ref, opposite to the direction it samples; here at
 //     the sampled direction (the same for a constant environment).
+//   * IBL::sample_le (environment_light.cpp:142-145) starts a light subpath
+//     on a disk at world_center - wi R and sends it along -wi, away from the
+//     scene: no such subpath ever hits it, while the weights still count the
+//     strategies that would use one.  Here the disk lies at world_center +
+//     wi R, on the side the light comes from.
+//   * Light subpaths evaluate the BSDF in its radiance form, whose normal
+//     correction |wi.ns| / |wi.ng| is not the adjoint's where shading and
+//     geometric normals differ (only refraction's eta^2 changes with the
+//     transport mode, disney.cpp:98, glass.cpp:65); here importance transport
+//     takes |wo.ns| / |wo.ng| (Veach 1997, sec. 5.3), so camera and light
+//     subpaths agree over interpolated normals and normal maps.
 //   * At a specular vertex the reference divides by 1 instead of the vertex's
 //     density (:748, :755) and keeps the sampled lobe weight as the density
 //     of the vertex after it (:369-378).  Two strategies then disagree on
@@ -105,18 +117,33 @@ TR_D void load_scatter(const DevScene& s, const BdVertex& v, BdScatter& c) {
     c.mat = sr.material;
     const DevMaterial& m = s.mats[sr.material];
     c.glass = m.type == MAT_GLASS;
-    c.textured = m.tex > 0;
+    c.textured = m.tex > 0 || m.texset > 0;
+    if (m.texset > 0) {   // the normal map turns the frame once; sc_bsdf reads the parameters
+        float u, v;
+        hit_uv(s, aux, c.b1, c.b2, u, v);
+        normal_map_frame(s, m, u, v, c.tx, c.ty, c.ns);
+    }
     c.med_in = sr.med_in;
     c.med_out = sr.med_out;
 }
 
-// bsdf_eval with the vertex's material (textured: kd / base colour at its uv).
-TR_D BsdfEval sc_bsdf(const DevScene& s, const BdScatter& c, f3 lwi, f3 lwo, f3 lng) {
+// The vertex's material (textured: kd / base colour or the Disney
+// parameters at its uv).
+TR_D DevMaterial sc_material(const DevScene& s, const BdScatter& c) {
+    DevMaterial m = s.mats[c.mat];
     if (c.textured) {
-        DevMaterial m = s.mats[c.mat];
-        apply_texture(s, m, c.tri, c.b1, c.b2);
-        return bsdf_eval(m, lwi, lwo, lng);
+        if (m.texset > 0) {
+            float u, v;
+            hit_uv(s, c.tri, c.b1, c.b2, u, v);
+            disney_params_at(s, m, u, v);
+        } else {
+            apply_texture(s, m, c.tri, c.b1, c.b2);
+        }
     }
+    return m;
+}
+TR_D BsdfEval sc_bsdf(const DevScene& s, const BdScatter& c, f3 lwi, f3 lwo, f3 lng) {
+    if (c.textured) return bsdf_eval(sc_material(s, c), lwi, lwo, lng);
     return bsdf_eval(s.mats[c.mat], lwi, lwo, lng);
 }
 
@@ -131,13 +158,27 @@ TR_D float adjoint_scale(const DevMaterial& m, f3 lwi, f3 lwo) {
     return lwo.z > 0.0f ? m.ior * m.ior : 1.0f / (m.ior * m.ior);
 }
 
+// Importance transport through a vertex whose shading normal differs from
+// its geometric one (Veach 1997, sec. 5.3): the BSDF's value carries
+// |wi.ns| / |wi.ng| for radiance (normal_correct_factor, coordinate.hpp:43-47),
+// and a light subpath needs |wo.ns| / |wo.ng| instead; the reference's BDPT
+// keeps the radiance form, so its light subpaths over shading normals
+// disagree with its camera subpaths.  Black-fringe directions (a Lambert lobe
+// about the geometric normal) and specular vertices keep their value.
+TR_D float shading_adjoint(const DevMaterial& m, f3 lwi, f3 lwo, f3 lng) {
+    if (m.type == MAT_GLASS || fringe(lwi, lng) || fringe(lwo, lng)) return 1.0f;
+    const float num = fabsf(lwo.z) * fabsf(dot(lwi, lng)), den = fabsf(dot(lwo, lng)) * fabsf(lwi.z);
+    return den > 0.0f ? num / den : 0.0f;
+}
+
 // f(wi, wo) at the vertex (BSDF::eval / the phase function); importance:
 // light-subpath transport.
 TR_D f3 sc_f(const DevScene& s, const BdScatter& c, f3 wi, bool importance) {
     if (c.medium) return f3s(hg_phase(c.g, -dot(wi, c.wo)));
-    const f3 lwi = sc_local(c, wi), lwo = sc_local(c, c.wo);
-    f3 f = sc_bsdf(s, c, lwi, lwo, sc_local(c, c.ng)).f;
-    if (importance) f *= adjoint_scale(s.mats[c.mat], lwi, lwo);
+    const f3 lwi = sc_local(c, wi), lwo = sc_local(c, c.wo), lng = sc_local(c, c.ng);
+    const DevMaterial m = sc_material(s, c);
+    f3 f = bsdf_eval(m, lwi, lwo, lng).f;
+    if (importance) f *= adjoint_scale(m, lwi, lwo) * shading_adjoint(m, lwi, lwo, lng);
     return f;
 }
 
@@ -279,11 +320,15 @@ __global__ __launch_bounds__(kBdBlock) void k_bd_start(DevScene s, BdPaths b, Bd
             } else if (pmf > 0.0f && e == s.num_area_lights) {
                 const LightSampleD ls = sample_light<true>(s, e, s.wcenter, u1, u2);
                 if (ls.pdf > 0.0f) {
+                    // the disk faces the scene from the light's side: the
+                    // reference places it at center - wi R and shoots along
+                    // -wi (environment_light.cpp:142-145), away from the
+                    // scene, so its light subpaths never hit anything
                     f3 tx, ty;
                     onb(ls.wi, tx, ty);
                     float dx, dy;
                     concentric_disk(u3, u4, dx, dy);
-                    const f3 pos = s.wcenter + (tx * dx + ty * dy - ls.wi) * s.wradius;
+                    const f3 pos = s.wcenter + (tx * dx + ty * dy + ls.wi) * s.wradius;
                     const f3 dir = -ls.wi;
                     const float init_pdf = ls.pdf * pmf;
                     put_vertex(l0, pos, 0.0f, dir, init_pdf, f3s(1.0f / init_pdf), BD_ENV, f3s(0.0f), e, 0.0f, 0.0f);
@@ -371,22 +416,22 @@ TR_D bool extend_one(const DevScene& s, const BdPaths& b, const BdConsts& k, int
     const float pa = pdf_sa * fabsf(dot(ng, wo)) / (t * t);
     const float b1 = h.z, b2 = h.w;
     DevMaterial mat = s.mats[sr.material];
-    if (mat.tex > 0) apply_texture(s, mat, tri, b1, b2);
+    f3 ns = normalize(sr.n0 * (1.0f - b1 - b2) + sr.n1 * b1 + sr.n2 * b2);
+    f3 tx, ty;
+    shading_frame(s, tri, ns, tx, ty);
+    apply_material_textures(s, mat, tri, b1, b2, tx, ty, ns);
     const bool delta = mat.type == MAT_GLASS;
     put_vertex(V[cnt], pos, light ? 0.0f : pa, ng, light ? (after_env ? k.env_pdf_pos : pa) : 0.0f, beta,
                BD_SURFACE | (delta ? kBdDelta : 0), wo, tri, b1, b2);
     *cntp = ++cnt;
     if (cnt >= maxc) return false;
-    const f3 ns = normalize(sr.n0 * (1.0f - b1 - b2) + sr.n1 * b1 + sr.n2 * b2);
-    f3 tx, ty;
-    shading_frame(s, tri, ns, tx, ty);
     const f3 lwo = mk(dot(wo, tx), dot(wo, ty), dot(wo, ns)), lng = mk(dot(ng, tx), dot(ng, ty), dot(ng, ns));
     const float su = rng.uniform(), u1 = rng.uniform(), u2 = rng.uniform();
     const BsdfSample bs = bsdf_sample(mat, lwo, lng, su, u1, u2);
     if (!(bs.pdf > 0.0f) || is_black(bs.f)) return false;
     const f3 wi = normalize(tx * bs.wi.x + ty * bs.wi.y + ns * bs.wi.z);
     f3 f = bs.f;
-    if (light) f *= adjoint_scale(mat, bs.wi, lwo);
+    if (light) f *= adjoint_scale(mat, bs.wi, lwo) * shading_adjoint(mat, bs.wi, lwo, lng);
     const float rpdf = delta ? 0.0f : bsdf_eval(mat, lwo, bs.wi, lng).pdf;
     const float rev = pdf_to_area(rpdf, pos, prev);
     if (light) prev.p.w = rev;

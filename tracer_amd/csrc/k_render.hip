@@ -382,12 +382,12 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
             } else {
                 const float b1 = h.z, b2 = h.w;
                 const f3 pos = o + d * t;
-                const f3 ns = normalize(sr.n0 * (1.0f - b1 - b2) + sr.n1 * b1 + sr.n2 * b2);
+                f3 ns = normalize(sr.n0 * (1.0f - b1 - b2) + sr.n1 * b1 + sr.n2 * b2);
                 f3 tx, ty;
                 shading_frame(s, tri, ns, tx, ty);
-                auto to_local = [&](f3 v) { return mk(dot(v, tx), dot(v, ty), dot(v, ns)); };
                 DevMaterial mat = s.mats[sr.material];
-                if (EXT && mat.tex > 0) apply_texture(s, mat, tri, b1, b2);
+                if (EXT) apply_material_textures(s, mat, tri, b1, b2, tx, ty, ns);
+                auto to_local = [&](f3 v) { return mk(dot(v, tx), dot(v, ty), dot(v, ns)); };
                 const f3 wo = -d;
                 const f3 lwo = to_local(wo), lng = to_local(ng);
                 Mixture mix;

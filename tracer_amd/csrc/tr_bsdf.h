@@ -47,7 +47,49 @@ struct DevMaterial {
     float guide_share;   // guide mixture probability at this material, as a share of the guide fraction (host, make_material)
     float ns;            // Phong exponent
     int tex;             // 0 or 1 + index of the texture giving kd / base_color per hit
+    int texset;          // 0 or 1 + row of DevScene::mat_tex / mat_raw: Disney parameters or a normal map per hit
 };
+
+// Parameters of create_disney (src/factory/material.hpp:37-52) at one surface
+// point: the values of its thirteen textures there.
+struct DisneyIn {
+    f3 base_color;
+    float metallic, roughness, transmission, transmission_roughness, ior;
+    f3 specular_scale;
+    float specular_tint, anisotropic, sheen, sheen_tint, clearcoat, clearcoat_gloss;
+};
+
+// DisneyBSDF constructor (src/material/disney.cpp:347-399): lobe constants and
+// lobe-selection weights from the parameters.  type, guide_share, ns, tex and
+// texset are left as they are.
+TR_HD void disney_setup(const DisneyIn& p, DevMaterial& d) {
+    d.C = p.base_color;
+    const float l = lum(d.C);
+    d.Ctint = l > 0.0f ? d.C * (1.0f / l) : f3s(1.0f);
+    d.metallic = p.metallic;
+    d.roughness = p.roughness;
+    d.transmission = p.transmission;
+    d.ior = fmaxf(1.01f, p.ior);
+    d.spec_scale = p.specular_scale;
+    d.spec_tint = p.specular_tint;
+    d.sheen = p.sheen;
+    d.sheen_tint = p.sheen_tint;
+    d.clearcoat = p.clearcoat;
+    const float aspect = p.anisotropic > 0.0f ? sqrtf(1.0f - 0.9f * p.anisotropic) : 1.0f;
+    d.ax = fmaxf(0.001f, p.roughness * p.roughness / aspect);
+    d.ay = fmaxf(0.001f, p.roughness * p.roughness * aspect);
+    d.tax = fmaxf(0.001f, p.transmission_roughness * p.transmission_roughness / aspect);
+    d.tay = fmaxf(0.001f, p.transmission_roughness * p.transmission_roughness * aspect);
+    float cr = (1.0f - p.clearcoat_gloss) * 0.1f + p.clearcoat_gloss * 0.0f;
+    cr *= cr;
+    d.cc_rough = fmaxf(cr, 0.0001f);
+    const float A = fminf(fmaxf(l * (1.0f - p.metallic), 0.3f), 0.7f);
+    const float B = 1.0f - A;
+    d.w_diffuse = A * (1.0f - p.transmission);
+    d.w_trans = A * p.transmission;
+    d.w_spec = B * 2.0f / (2.0f + p.clearcoat);
+    d.w_cc = B * p.clearcoat / (2.0f + p.clearcoat);
+}
 
 struct BsdfSample {
     f3 wi;

@@ -48,6 +48,9 @@ struct DevEnvMap {
     f3 r0, r1, r2;              // rows of the world -> light rotation
 };
 
+constexpr int kMatTexRow = 16;
+constexpr int kMatRawRow = 20;
+
 struct DevScene {
     TraceScene tr;
     const float4* tri_shade;   // 3 float4 per leaf-ordered triangle: (n0, material), (n1, light), (n2, media)
@@ -68,8 +71,14 @@ struct DevScene {
     const float4* tri_uv;
     // per leaf-ordered triangle: the normal derivative along v of
     // compute_ss_ts (xyz) and 1 in w where it is used (scenes with texture
-    // coordinates and an anisotropic material; NULL otherwise)
+    // coordinates and an anisotropic material or a normal map; NULL otherwise)
     const float4* tri_tan;
+    // Disney materials with per-hit parameters (DevMaterial::texset = 1 + row):
+    // kMatTexRow texture ids (base colour, the twelve parameters in
+    // tr_material::param_texture order, the normal map; 0 = constant) and
+    // kMatRawRow constants in DisneyIn order
+    const int* mat_tex;
+    const float* mat_raw;
     const int4* tex_info;
     const uchar4* tex8;
     const float4* texf;
@@ -107,8 +116,9 @@ TR_D TriShade load_shade(const DevScene& s, int tri) {
 // triangle's normal derivative along v (compute_ss_ts,
 // src/utility/coordinate.hpp:54-68) where the table holds one, else the
 // second axis of coordinate(ns) (src/utility/geometry.hpp:887-894), for which
-// x is coordinate's first axis.  Only anisotropic BSDFs see the difference
-// from any other frame around ns.
+// x is coordinate's first axis.  Only anisotropic BSDFs and normal maps (whose
+// texels are tangent-space normals) see the difference from any other frame
+// around ns.
 TR_D void shading_frame(const DevScene& s, int tri, f3 ns, f3& tx, f3& ty) {
     const float4 tv = s.tri_tan ? s.tri_tan[tri] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (tv.w != 0.0f) {
@@ -303,6 +313,70 @@ TR_D void apply_texture(const DevScene& s, DevMaterial& m, int tri, float b1, fl
             const float broad = m.w_diffuse + (m.ax >= 0.25f ? m.w_spec : 0.0f);
             m.guide_share = tot > 0.0f ? broad / tot : 0.0f;
         }
+    }
+}
+
+// Texture coordinates of a hit (alpha weights vertex 1, beta vertex 2).
+TR_D void hit_uv(const DevScene& s, int tri, float b1, float b2, float& u, float& v) {
+    const float4 p = s.tri_uv[2 * size_t(tri)], q = s.tri_uv[2 * size_t(tri) + 1];
+    const float b0 = 1.0f - b1 - b2;
+    u = p.x * b0 + p.z * b1 + q.x * b2;
+    v = p.y * b0 + p.w * b1 + q.y * b2;
+}
+
+// A Disney material with textured parameters (DevMaterial::texset > 0) at
+// uv: every parameter read from its texture (a scalar takes the red channel,
+// Texture2D::evaluate_s) or its constant, and the DisneyBSDF constructor rerun
+// (Disney::shading, src/material/disney.cpp:604-621).
+TR_D void disney_params_at(const DevScene& s, DevMaterial& m, float u, float v) {
+    const int* ids = s.mat_tex + size_t(m.texset - 1) * kMatTexRow;
+    const float* r = s.mat_raw + size_t(m.texset - 1) * kMatRawRow;
+    auto col = [&](int id, float a, float b, float c) { return id > 0 ? tex_lookup(s, id - 1, u, v) : mk(a, b, c); };
+    auto scal = [&](int k, float c) { return ids[1 + k] > 0 ? tex_lookup(s, ids[1 + k] - 1, u, v).x : c; };
+    DisneyIn in;
+    in.base_color = col(ids[0], r[0], r[1], r[2]);
+    in.metallic = scal(0, r[3]);
+    in.roughness = scal(1, r[4]);
+    in.transmission = scal(2, r[5]);
+    in.transmission_roughness = scal(3, r[6]);
+    in.ior = scal(4, r[7]);
+    in.specular_scale = col(ids[6], r[8], r[9], r[10]);
+    in.specular_tint = scal(6, r[11]);
+    in.anisotropic = scal(7, r[12]);
+    in.sheen = scal(8, r[13]);
+    in.sheen_tint = scal(9, r[14]);
+    in.clearcoat = scal(10, r[15]);
+    in.clearcoat_gloss = scal(11, r[16]);
+    disney_setup(in, m);
+}
+
+// NormalMapper::reorient (src/core/material.hpp:37-48) of the shading frame
+// (tx, ty, ns) at uv when the material has a normal map: z = the frame's
+// image of normalize(2c - 1) for the texel c, x = cross(y, z), y = cross(z, x)
+// (Coord::rotate_to_new_z, src/utility/coordinate.hpp:107-113), normalised.
+TR_D void normal_map_frame(const DevScene& s, const DevMaterial& m, float u, float v, f3& tx, f3& ty, f3& ns) {
+    const int id = s.mat_tex[size_t(m.texset - 1) * kMatTexRow + 13];
+    if (id <= 0) return;
+    const f3 c = tex_lookup(s, id - 1, u, v);
+    const f3 ln = normalize(mk(c.x * 2.0f - 1.0f, c.y * 2.0f - 1.0f, c.z * 2.0f - 1.0f));
+    const f3 z = normalize(tx * ln.x + ty * ln.y + ns * ln.z);
+    const f3 x = normalize(cross(ty, z));
+    ty = normalize(cross(z, x));
+    tx = x;
+    ns = z;
+}
+
+// Everything a material reads per hit: textured Disney parameters and the
+// normal map, or the kd / base-colour texture.
+TR_D void apply_material_textures(const DevScene& s, DevMaterial& m, int tri, float b1, float b2, f3& tx, f3& ty,
+                                  f3& ns) {
+    if (m.texset > 0) {
+        float u, v;
+        hit_uv(s, tri, b1, b2, u, v);
+        disney_params_at(s, m, u, v);
+        normal_map_frame(s, m, u, v, tx, ty, ns);
+    } else if (m.tex > 0) {
+        apply_texture(s, m, tri, b1, b2);
     }
 }
 

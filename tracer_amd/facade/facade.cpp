@@ -133,11 +133,11 @@ RC<Material> create_disney(RC<const Texture2D> base_color, RC<const Texture2D> m
                            RC<const Texture2D> anisotropic, RC<const Texture2D> sheen, RC<const Texture2D> sheen_tint,
                            RC<const Texture2D> clearcoat, RC<const Texture2D> clearcoat_gloss,
                            Box<const NormalMapper> normal_mapper, RC<const BSSRDFSurface> bssrdf) {
-    // disney.cpp:621,638 dereference both; a normal map would need textures on the GPU path.
+    // disney.cpp:621,638 dereference both
     if (!normal_mapper) throw std::runtime_error("create_disney: normal_mapper must not be null");
-    if (normal_mapper->map()) throw std::runtime_error("create_disney: normal maps are not supported by the GPU path");
     (void)bssrdf;
     auto m = newRC<DisneyMaterial>();
+    m->normal_map = normal_mapper->map();
     m->base_color = base_color; m->metallic = metallic; m->roughness = roughness; m->transmission = transmission;
     m->transmission_roughness = transmission_roughness; m->ior = ior; m->specular_scale = specular_scale;
     m->specular_tint = specular_tint; m->anisotropic = anisotropic; m->sheen = sheen; m->sheen_tint = sheen_tint;
@@ -381,28 +381,28 @@ static int32_t texture_id(const Texture2D* t, FlatScene& f, std::unordered_map<c
 static tr_material convert_material(const Material* m, FlatScene& f, std::unordered_map<const Texture2D*, int32_t>& tex_ids) {
     tr_material r{};
     if (auto d = dynamic_cast<const DisneyMaterial*>(m)) {
-        // base_color may be an image; the other parameters are constants
-        const RC<const Texture2D>* all[] = {&d->metallic, &d->roughness, &d->transmission,
-                                            &d->transmission_roughness, &d->ior, &d->specular_scale, &d->specular_tint,
-                                            &d->anisotropic, &d->sheen, &d->sheen_tint, &d->clearcoat, &d->clearcoat_gloss};
-        for (auto* t : all)
-            if (!(*t)->is_constant())
-                throw std::runtime_error("Disney: only base_color may be an image texture on the GPU path");
+        // every parameter a constant or an image read per hit (tr_material::param_texture
+        // order), and the normal map of the NormalMapper
         r.type = TR_MAT_DISNEY;
         if (d->base_color->is_constant()) tex_rgb(d->base_color, r.base_color);
         else r.base_color_texture = texture_id(d->base_color.get(), f, tex_ids);
-        r.metallic = tex_s(d->metallic);
-        r.roughness = tex_s(d->roughness);
-        r.transmission = tex_s(d->transmission);
-        r.transmission_roughness = tex_s(d->transmission_roughness);
-        r.ior = tex_s(d->ior);
-        tex_rgb(d->specular_scale, r.specular_scale);
-        r.specular_tint = tex_s(d->specular_tint);
-        r.anisotropic = tex_s(d->anisotropic);
-        r.sheen = tex_s(d->sheen);
-        r.sheen_tint = tex_s(d->sheen_tint);
-        r.clearcoat = tex_s(d->clearcoat);
-        r.clearcoat_gloss = tex_s(d->clearcoat_gloss);
+        const RC<const Texture2D>* params[TR_PARAM_COUNT] = {
+            &d->metallic, &d->roughness, &d->transmission, &d->transmission_roughness, &d->ior, &d->specular_scale,
+            &d->specular_tint, &d->anisotropic, &d->sheen, &d->sheen_tint, &d->clearcoat, &d->clearcoat_gloss};
+        float* consts[TR_PARAM_COUNT] = {&r.metallic, &r.roughness, &r.transmission, &r.transmission_roughness,
+                                         &r.ior, nullptr, &r.specular_tint, &r.anisotropic, &r.sheen,
+                                         &r.sheen_tint, &r.clearcoat, &r.clearcoat_gloss};
+        for (int k = 0; k < TR_PARAM_COUNT; ++k) {
+            const RC<const Texture2D>& t = *params[k];
+            if (!t->is_constant()) {
+                r.param_texture[k] = texture_id(t.get(), f, tex_ids);
+            } else if (k == TR_PARAM_SPECULAR_SCALE) {
+                tex_rgb(t, r.specular_scale);
+            } else {
+                *consts[k] = tex_s(t);
+            }
+        }
+        if (d->normal_map) r.normal_texture = texture_id(d->normal_map.get(), f, tex_ids);
         return r;
     }
     if (auto p = dynamic_cast<const PhongMaterial*>(m)) {

@@ -43,6 +43,7 @@ class DisneyMaterial : public Material {
 public:
     RC<const Texture2D> base_color, metallic, roughness, transmission, transmission_roughness, ior, specular_scale,
         specular_tint, anisotropic, sheen, sheen_tint, clearcoat, clearcoat_gloss;
+    RC<const Texture2D> normal_map;   // NormalMapper's map, or null
 };
 class PhongMaterial : public Material {
 public:
