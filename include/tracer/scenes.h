@@ -37,11 +37,15 @@ trs_scene* trs_create(int32_t config, uint64_t seed, double detail, const char**
 trs_scene* trs_load_obj(const char* obj_path, const tr_camera* camera, int32_t outside_fog, const char** err);
 /* trs_load_obj with a heterogeneous medium outside every surface:
  * create_heterogeneous_medium(local_to_world (row-major 3x4),
- * create_image_texture3d(nx x ny x nz floats, x fastest), constant albedo,
- * constant g, max_scattering_count, false), src/medium/heterogeneous.cpp:101-108. */
+ * create_image_texture3d(nx x ny x nz floats, x fastest), albedo, g,
+ * max_scattering_count, false), src/medium/heterogeneous.cpp:101-108: the
+ * albedo is the constant RGB albedo[3] or, when albedo_grid is not NULL, an
+ * image of nx x ny x nz grey values; g the constant or, when g_grid is not
+ * NULL, an image of nx x ny x nz values. */
 trs_scene* trs_load_obj_grid(const char* obj_path, const tr_camera* camera, const float* density, int32_t nx, int32_t ny,
                              int32_t nz, const float* local_to_world, const float* albedo, float g,
-                             int32_t max_scattering_count, const char** err);
+                             const float* albedo_grid, const float* g_grid, int32_t max_scattering_count,
+                             const char** err);
 /* Reads a Radiance .hdr image (the loader behind create_texture2d_from_file):
  * *width, *height, and when rgb is not NULL its height rows of width linear
  * RGB texels (cap floats at least).  Returns 0, or 1 and *err on failure. */

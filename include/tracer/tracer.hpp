@@ -285,8 +285,9 @@ public:
 };
 RC<Medium> create_homogeneous_medium(const Spectrum& sigma_a, const Spectrum& sigma_s, real g, int max_scattering_count);
 RC<Medium> create_vacuum();
-// heterogeneous.cpp:101-108.  The GPU path takes a density that is an image or
-// constant 3-D texture and a constant albedo and g; other textures throw.
+// heterogeneous.cpp:101-108.  Density, albedo and g are each an image or a
+// constant 3-D texture; an image albedo or g is looked up at every collision
+// (a grey image gives the same albedo in every channel).
 RC<Medium> create_heterogeneous_medium(const Transform& local_to_world, RC<const Texture3D> density,
                                        RC<const Texture3D> albedo, RC<const Texture3D> g, int max_scattering_count,
                                        bool white_for_indirect);

@@ -106,7 +106,13 @@ typedef struct tr_texture {
  * local position world_to_local(p) with every coordinate wrapped into [0, 1]
  * (Texture3D's wrap_repeat).  sigma_s then holds the constant albedo, g the
  * constant asymmetry, and sigma_a is unused.  local_to_world is a row-major
- * 3x4 affine matrix; its inverse maps world positions to grid coordinates. */
+ * 3x4 affine matrix; its inverse maps world positions to grid coordinates.
+ * albedo_grid (anx * any * anz RGB triples) and g_grid (gnx * gny * gnz
+ * values), when not NULL, replace the constant albedo and g: both are looked
+ * up trilinearly at the collision's local position, as HeterogeneousMedium::
+ * sample reads albedo->evaluate and g->evaluate_s there
+ * (src/medium/heterogeneous.cpp:66-78); g is kept inside (-1, 1) as the
+ * constant is. */
 typedef struct tr_medium {
     float sigma_a[3];
     float sigma_s[3];
@@ -116,6 +122,10 @@ typedef struct tr_medium {
     int32_t nx, ny, nz;
     int32_t white_for_indirect;    /* albedo 1 after the first scatter of a run */
     float local_to_world[12];
+    const float* albedo_grid;      /* NULL: sigma_s is the albedo */
+    int32_t anx, any, anz;
+    const float* g_grid;           /* NULL: g is the asymmetry */
+    int32_t gnx, gny, gnz;
 } tr_medium;
 
 /* create_thin_lens_camera(aspect, pos, target, up, fov, lens_radius,
@@ -405,6 +415,11 @@ tr_status tr_env_eval_batch(tr_ctx* ctx, const float* dirs, int64_t n, float* L,
  * floats).  TR_ERR_INVALID for a medium without a grid or a texture id out of
  * range; TR_ERR_STATE when no material names a texture. */
 tr_status tr_medium_density_batch(tr_ctx* ctx, int32_t medium, const float* pos, int64_t n, float* density);
+/* The albedo (n*3) and asymmetry (n) that delta tracking in grid medium
+ * `medium` uses at world points pos (n*3): albedo->evaluate and
+ * g->evaluate_s at the local position (src/medium/heterogeneous.cpp:66-78)
+ * from the medium's albedo and g grids, or its constants. */
+tr_status tr_medium_albedo_g_batch(tr_ctx* ctx, int32_t medium, const float* pos, int64_t n, float* albedo, float* g);
 tr_status tr_texture_eval_batch(tr_ctx* ctx, int32_t texture, const float* uv, int64_t n, float* rgb);
 
 /* Test batches over the uploaded scene (after tr_bvh_build) through the

@@ -79,6 +79,8 @@ def lib() -> C.CDLL:
         L.ref_tex_eval8.restype = None
         L.ref_grid_density.argtypes = [C.c_void_p, P(C.c_float), I64, P(C.c_float)]
         L.ref_grid_density.restype = None
+        L.ref_grid_albedo_g.argtypes = [C.c_void_p, P(C.c_float), I64, P(C.c_float), P(C.c_float)]
+        L.ref_grid_albedo_g.restype = None
         L.ref_grid_stats.argtypes = [C.c_void_p, P(C.c_float), P(C.c_float), I64, C.c_uint64, I32, P(C.c_double)]
         L.ref_grid_stats.restype = None
         _lib = L
@@ -102,6 +104,16 @@ def grid_density(medium, pts: np.ndarray) -> np.ndarray:
     out = np.zeros(pts.shape[0], np.float32)
     lib().ref_grid_density(C.addressof(medium), fptr(pts), pts.shape[0], fptr(out))
     return out
+
+
+def grid_albedo_g(medium, pts: np.ndarray):
+    """Albedo (n, 3) and asymmetry (n,) of a heterogeneous medium at world
+    points (n, 3): its grids, or its constants."""
+    pts = np.ascontiguousarray(pts, np.float32)
+    a = np.zeros((pts.shape[0], 3), np.float32)
+    g = np.zeros(pts.shape[0], np.float32)
+    lib().ref_grid_albedo_g(C.addressof(medium), fptr(pts), pts.shape[0], fptr(a), fptr(g))
+    return a, g
 
 
 def grid_stats(medium, a, b, trials: int, seed: int = 1, indirect: bool = False):

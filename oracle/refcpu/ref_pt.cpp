@@ -100,7 +100,7 @@ struct RefTex {
     }
 };
 
-struct MediumSample { bool scattered = false; V3f pos; RGB<float> w{1, 1, 1}; };
+struct MediumSample { bool scattered = false; V3f pos; RGB<float> w{1, 1, 1}; float g = 0; };
 
 // Heterogeneous medium (src/medium/heterogeneous.cpp:10-108) over a float
 // density grid, x fastest: world_to_local = inverse(local_to_world), every
@@ -113,10 +113,24 @@ struct RefGrid {
     float w2l[12] = {};
     float inv_max = 0;
     RGB<float> albedo;
+    float g = 0;
+    // albedo->evaluate and g->evaluate_s at the collision (heterogeneous.cpp:66-78)
+    // from their own grids when given: RGB triples, asymmetries
+    std::vector<float> ad, gd;
+    int anx = 0, any = 0, anz = 0, gnx = 0, gny = 0, gnz = 0;
     bool white = false;
     void init(const tr_medium& m) {
         d.assign(m.density, m.density + size_t(m.nx) * size_t(m.ny) * size_t(m.nz));
         nx = m.nx; ny = m.ny; nz = m.nz;
+        g = m.g;
+        if (m.albedo_grid) {
+            ad.assign(m.albedo_grid, m.albedo_grid + 3 * size_t(m.anx) * size_t(m.any) * size_t(m.anz));
+            anx = m.anx; any = m.any; anz = m.anz;
+        }
+        if (m.g_grid) {
+            gd.assign(m.g_grid, m.g_grid + size_t(m.gnx) * size_t(m.gny) * size_t(m.gnz));
+            gnx = m.gnx; gny = m.gny; gnz = m.gnz;
+        }
         float mx = 0;
         for (float v : d) mx = std::max(mx, v);
         inv_max = 1 / std::max(mx, kEps);
@@ -136,22 +150,32 @@ struct RefGrid {
             w2l[4 * i + 3] = float(-(r[3 * i] * a[3] + r[3 * i + 1] * a[7] + r[3 * i + 2] * a[11]));
         }
     }
-    float density(const V3f& p) const {
+    // channel c of a grid of ch values per voxel (x fastest) at world point p
+    float lookup(const std::vector<float>& grid, int gx, int gy, int gz, int ch, int c, const V3f& p) const {
         auto wrap = [](double x) { return std::clamp(x - std::floor(x), 0.0, 1.0); };
         const float lx = w2l[0] * p.x + w2l[1] * p.y + w2l[2] * p.z + w2l[3];
         const float ly = w2l[4] * p.x + w2l[5] * p.y + w2l[6] * p.z + w2l[7];
         const float lz = w2l[8] * p.x + w2l[9] * p.y + w2l[10] * p.z + w2l[11];
-        const double u = wrap(lx) * (nx - 1), v = wrap(ly) * (ny - 1), k = wrap(lz) * (nz - 1);
-        const int u0 = std::clamp(int(u), 0, nx - 1), u1 = std::clamp(u0 + 1, 0, nx - 1);
-        const int v0 = std::clamp(int(v), 0, ny - 1), v1 = std::clamp(v0 + 1, 0, ny - 1);
-        const int k0 = std::clamp(int(k), 0, nz - 1), k1 = std::clamp(k0 + 1, 0, nz - 1);
+        const double u = wrap(lx) * (gx - 1), v = wrap(ly) * (gy - 1), k = wrap(lz) * (gz - 1);
+        const int u0 = std::clamp(int(u), 0, gx - 1), u1 = std::clamp(u0 + 1, 0, gx - 1);
+        const int v0 = std::clamp(int(v), 0, gy - 1), v1 = std::clamp(v0 + 1, 0, gy - 1);
+        const int k0 = std::clamp(int(k), 0, gz - 1), k1 = std::clamp(k0 + 1, 0, gz - 1);
         const double du = u - u0, dv = v - v0, dk = k - k0;
-        auto at = [&](int x, int y, int z) { return double(d[(size_t(z) * size_t(ny) + size_t(y)) * size_t(nx) + size_t(x)]); };
+        auto at = [&](int x, int y, int z) {
+            return double(grid[((size_t(z) * size_t(gy) + size_t(y)) * size_t(gx) + size_t(x)) * size_t(ch) + size_t(c)]);
+        };
         return float(((at(u0, v0, k0) * (1.0 - du) + at(u1, v0, k0) * du) * (1.0 - dv) +
                       (at(u0, v1, k0) * (1.0 - du) + at(u1, v1, k0) * du) * dv) * (1.0 - dk) +
                      ((at(u0, v0, k1) * (1.0 - du) + at(u1, v0, k1) * du) * (1.0 - dv) +
                       (at(u0, v1, k1) * (1.0 - du) + at(u1, v1, k1) * du) * dv) * dk);
     }
+    float density(const V3f& p) const { return lookup(d, nx, ny, nz, 1, 0, p); }
+    RGB<float> albedo_at(const V3f& p) const {
+        if (ad.empty()) return albedo;
+        return RGB<float>(lookup(ad, anx, any, anz, 3, 0, p), lookup(ad, anx, any, anz, 3, 1, p),
+                          lookup(ad, anx, any, anz, 3, 2, p));
+    }
+    float g_at(const V3f& p) const { return gd.empty() ? g : lookup(gd, gnx, gny, gnz, 1, 0, p); }
     // HeterogeneousMedium::tr (heterogeneous.cpp:35-52): ratio tracking
     float tr(const V3f& a, const V3f& b, Sampler& sp) const {
         float res = 1, t = 0;
@@ -179,7 +203,8 @@ struct RefGrid {
             if (sp.get() < density(pos) * inv_max) {
                 r.scattered = true;
                 r.pos = pos;
-                r.w = indirect && white ? RGB<float>(1) : albedo;
+                r.w = indirect && white ? RGB<float>(1) : albedo_at(pos);
+                r.g = g_at(pos);
                 return r;
             }
         }
@@ -752,7 +777,7 @@ struct Renderer {
                 coef *= ms.w;
                 if (ms.scattered) {
                     scattering++;
-                    HGRef<float> hg{std::clamp<float>(m->g, -1 + kEps, 1 - kEps)};
+                    HGRef<float> hg{std::clamp<float>(grid_of(m) ? ms.g : m->g, -1 + kEps, 1 - kEps)};
                     const V3f wo = -((is.pos - (c.quirks ? cam_o : o)).normalize());
                     auto evalf = [&](const V3f& wi) { return RGB<float>(hg.eval(wi, wo)); };
                     auto pdff = [&](const V3f& wi) { return hg.eval(wi, wo); };
@@ -923,7 +948,9 @@ static Scene* build_scene(const tr_scene_desc* d, int max_leaf) {
     for (size_t i = 0; i < s->media.size(); ++i)
         if (s->media[i].density) {
             s->grids[i].init(s->media[i]);
-            s->media[i].density = nullptr;   // the grid copy above is used from here on
+            s->media[i].density = nullptr;   // the grid copies above are used from here on
+            s->media[i].albedo_grid = nullptr;
+            s->media[i].g_grid = nullptr;
         }
     std::vector<Bounds3> pb(nt);
     for (int64_t t = 0; t < nt; ++t) {
@@ -1259,6 +1286,17 @@ void ref_grid_density(const tr_medium* m, const float* pts, int64_t n, float* ou
     RefGrid g;
     g.init(*m);
     for (int64_t i = 0; i < n; ++i) out[i] = g.density(V3f(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]));
+}
+// Albedo (RGB) and asymmetry of the medium at world points: its grids or constants.
+void ref_grid_albedo_g(const tr_medium* m, const float* pts, int64_t n, float* albedo, float* g) {
+    RefGrid r;
+    r.init(*m);
+    for (int64_t i = 0; i < n; ++i) {
+        const V3f p(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
+        const RGB<float> a = r.albedo_at(p);
+        albedo[3 * i] = a.r; albedo[3 * i + 1] = a.g; albedo[3 * i + 2] = a.b;
+        g[i] = r.g_at(p);
+    }
 }
 void ref_grid_stats(const tr_medium* m, const float* a, const float* b, int64_t trials, uint64_t seed, int indirect,
                     double* out) {

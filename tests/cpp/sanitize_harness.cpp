@@ -38,6 +38,7 @@ void ref_bsdf_eval(const tr_material* mats, const int32_t* ids, const float* wi,
 void ref_bsdf_sample(const tr_material* mats, const int32_t* ids, const float* wo, const float* ng, const float* u,
                      int64_t n, int precision, int fixed, double* wi, double* f, double* pdf);
 void ref_grid_density(const tr_medium* m, const float* pts, int64_t n, float* out);
+void ref_grid_albedo_g(const tr_medium* m, const float* pts, int64_t n, float* albedo, float* g);
 void* ref_env_create(int w, int h, const float* texels, const float* world_to_light);
 void ref_env_destroy(void* e);
 void ref_env_sample(void* e, const float* u, int64_t n, float* wi, float* pdf, float* L);
@@ -229,14 +230,22 @@ void obj_files(const std::string& dir) {
     }
     std::vector<float> dens(4 * 5 * 6, 0.3f);
     float l2w[12] = {2, 0, 0, -1, 0, 2, 0, -1, 0, 0, 2, -1}, alb[3] = {0.8f, 0.8f, 0.8f};
-    trs_scene* g = trs_load_obj_grid((dir + "/room.obj").c_str(), &cam, dens.data(), 4, 5, 6, l2w, alb, 0.3f, 4, &err);
-    check(g != nullptr, "obj room with a density grid");
-    exercise_scene(*trs_desc(g), trs_max_leaf(g), "grid room");
-    const tr_medium& m = trs_desc(g)->media[trs_desc(g)->num_media - 1];
-    std::vector<float> pts(3 * 300), dout(300);
-    for (float& x : pts) x = uni(-3, 3);
-    ref_grid_density(&m, pts.data(), 300, dout.data());
-    trs_destroy(g);
+    std::vector<float> alb_grid(4 * 5 * 6), g_grid(4 * 5 * 6);
+    for (float& x : alb_grid) x = uni(0, 1);
+    for (float& x : g_grid) x = uni(-0.9f, 0.9f);
+    for (int with_grids = 0; with_grids < 2; ++with_grids) {
+        trs_scene* g = trs_load_obj_grid((dir + "/room.obj").c_str(), &cam, dens.data(), 4, 5, 6, l2w, alb, 0.3f,
+                                         with_grids ? alb_grid.data() : nullptr, with_grids ? g_grid.data() : nullptr,
+                                         4, &err);
+        check(g != nullptr, "obj room with a density grid");
+        exercise_scene(*trs_desc(g), trs_max_leaf(g), "grid room");
+        const tr_medium& m = trs_desc(g)->media[trs_desc(g)->num_media - 1];
+        std::vector<float> pts(3 * 300), dout(300), aout(900), gout(300);
+        for (float& x : pts) x = uni(-3, 3);
+        ref_grid_density(&m, pts.data(), 300, dout.data());
+        ref_grid_albedo_g(&m, pts.data(), 300, aout.data(), gout.data());
+        trs_destroy(g);
+    }
     // malformed inputs: an error, never a crash
     const char* broken[] = {
         "v 0 0 0\nv 1 0 0\nf 1 2 3\n",                       // index past the vertices

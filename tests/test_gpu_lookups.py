@@ -38,6 +38,30 @@ def test_grid_density_matches_oracle(gpu_ctx, tmp_path):
         gpu_ctx.medium_density(1, pts[:4])
 
 
+def test_grid_albedo_and_g_match_oracle(gpu_ctx, tmp_path):
+    """Device albedo and asymmetry of a grid medium (tr_medium_albedo_g_batch)
+    against the oracle's lookups, for albedo and g grids of their own sizes
+    under a rotated transform, and for the constants."""
+    from oracle import refcpu
+    from tracer_amd import abi, native
+    rng = np.random.default_rng(23)
+    path, cam = _room(tmp_path)
+    dens = rng.uniform(0.0, 2.0, (5, 6, 7)).astype(np.float32)
+    alb = rng.uniform(0.0, 1.0, (4, 9, 6, 3)).astype(np.float32)
+    g = rng.uniform(-0.95, 0.95, (7, 3, 5)).astype(np.float32)
+    rot = np.array([[0.0, -1.6, 0.0, 0.8], [2.0, 0.0, 0.0, -1.0], [0.0, 0.0, 2.5, -1.2]])
+    s = native.obj_grid_scene(path, cam, dens, rot)
+    for albedo, gg in ((alb, g), ((0.3, 0.5, 0.7), 0.35)):
+        s.media[0] = abi.make_grid_medium(dens, albedo=albedo, g=gg, local_to_world=rot)
+        gpu_ctx.load_scene(s)
+        pts = rng.uniform(-3.0, 3.0, (1 << 16, 3)).astype(np.float32)
+        ga, g_g = gpu_ctx.medium_albedo_g(0, pts)
+        oa, o_g = refcpu.grid_albedo_g(s.media[0], pts)
+        err = max(np.abs(ga - oa).max(), np.abs(g_g - np.clip(o_g, -1 + 3e-4, 1 - 3e-4)).max())
+        print(f"max |albedo, g difference| {err:.3g}")
+        assert err < 1e-5
+
+
 def test_texture_lookup_matches_oracle(gpu_ctx, tmp_path):
     from oracle import refcpu
     from tracer_amd import native

@@ -307,6 +307,27 @@ def _smoke_grid():
     return (0.15 + 2.0 * np.exp(-r2 / 0.03)).astype(np.float32)
 
 
+def test_obj_room_grid_albedo_and_g_match_fixed_estimator(gpu_ctx, tmp_path):
+    """The grid-medium room with the albedo and the asymmetry read per
+    collision from images of the density's size (create_heterogeneous_medium
+    with image textures, heterogeneous.cpp:66-78, built by the facade):
+    albedo 0.3 to 0.95 across the room, g from -0.5 to 0.8, against the
+    oracle in fixed mode."""
+    from tests import objfiles
+    from tracer_amd import native
+    obj, mtl = objfiles.box_room_obj()
+    path = objfiles.write(tmp_path, "room.obj", obj, mtl)
+    room = np.array([[2.0, 0, 0, -1.0], [0, 2.0, 0, -1.0], [0, 0, 2.0, -1.0]])
+    dens = _smoke_grid()
+    nz, ny, nx = dens.shape
+    z, y, x = np.meshgrid(np.arange(nz) / (nz - 1), np.arange(ny) / (ny - 1), np.arange(nx) / (nx - 1), indexing="ij")
+    albedo = (0.3 + 0.65 * x).astype(np.float32)
+    g = (-0.5 + 1.3 * y).astype(np.float32)
+    s = native.obj_grid_scene(path, objfiles.camera((0.0, 0.0, 3.6), (0.0, 0.0, 0.0)), dens, room, albedo=albedo, g=g)
+    assert s.media[0].albedo_grid and s.media[0].g_grid
+    _compare_estimators(gpu_ctx, s, 32, 32, 1024, 256, 8, 8, quirks=False, nee=0, max_depth=8)
+
+
 @pytest.mark.parametrize("max_sc", [0, 3, 2 ** 31 - 1])
 def test_obj_room_grid_medium_matches_fixed_estimator(gpu_ctx, tmp_path, max_sc):
     """The OBJ room filled with a heterogeneous medium built by the facade

@@ -34,7 +34,7 @@ def test_scene_library_exports_header():
 def test_abi_struct_sizes():
     from tracer_amd import abi
     assert C.sizeof(abi.TrMaterial) == 132
-    assert C.sizeof(abi.TrMedium) == 104
+    assert C.sizeof(abi.TrMedium) == 152
     assert C.sizeof(abi.TrCamera) == 52
     assert C.sizeof(abi.TrRenderStats) == 160
     assert C.sizeof(abi.TrRenderParams) == 96

@@ -100,3 +100,35 @@ def test_facade_grid_scene(tmp_path):
         native.obj_grid_scene(path, cam, grid[0], ROOM)
     with pytest.raises(native.TracerError):
         native.obj_grid_scene(path, cam, grid, np.zeros((3, 4)))
+
+
+def test_albedo_and_g_grids():
+    """HeterogeneousMedium::sample reads albedo->evaluate and g->evaluate_s at
+    the collision (heterogeneous.cpp:66-78): the oracle's lookups of albedo
+    and g grids reproduce affine fields exactly (trilinear interpolation of an
+    affine function), on grids of their own sizes; constant grids give the
+    constant's scatter weight."""
+    from oracle import refcpu
+    rng = np.random.default_rng(4)
+    dens = rng.uniform(0.5, 1.5, (3, 4, 5)).astype(np.float32)
+    alb = np.stack([_affine_grid((6, 5, 4), 0.2, (0.3, 0.1, 0.2)), _affine_grid((6, 5, 4), 0.5, (-0.2, 0.2, 0.1)),
+                    _affine_grid((6, 5, 4), 0.4, (0.1, -0.1, 0.3))], -1)
+    g = _affine_grid((4, 7, 3), -0.3, (0.4, 0.2, -0.1))
+    m = abi.make_grid_medium(dens, albedo=alb, g=g, local_to_world=ROOM)
+    p = rng.uniform(-0.999, 0.999, (2000, 3)).astype(np.float32)
+    local = (p + 1) / 2
+    a, gg = refcpu.grid_albedo_g(m, p)
+    want_a = np.stack([0.2 + local @ np.array([0.3, 0.1, 0.2]), 0.5 + local @ np.array([-0.2, 0.2, 0.1]),
+                       0.4 + local @ np.array([0.1, -0.1, 0.3])], -1)
+    assert np.max(np.abs(a - want_a)) < 2e-5
+    assert np.max(np.abs(gg - (-0.3 + local @ np.array([0.4, 0.2, -0.1])))) < 2e-5
+    # without grids: the constants
+    m2 = abi.make_grid_medium(dens, albedo=(0.7, 0.6, 0.5), g=0.25, local_to_world=ROOM)
+    a2, g2 = refcpu.grid_albedo_g(m2, p)
+    assert np.all(a2 == np.float32([0.7, 0.6, 0.5])) and np.all(g2 == np.float32(0.25))
+    # a constant albedo grid scatters with the constant's weight
+    a_const = np.full((2, 3, 2, 3), 0.6, np.float32)
+    m3 = abi.make_grid_medium(dens, albedo=a_const, local_to_world=ROOM)
+    pa, pb = np.float32([-0.9, 0.1, 0.2]), np.float32([0.9, -0.2, -0.1])
+    _, frac, w = refcpu.grid_stats(m3, pa, pb, 20000, seed=3)
+    assert frac > 0.5 and abs(w - 0.6) < 1e-6

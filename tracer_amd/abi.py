@@ -41,29 +41,48 @@ class TrTexture(C.Structure):
 class TrMedium(C.Structure):
     _fields_ = [("sigma_a", f3), ("sigma_s", f3), ("g", C.c_float), ("max_scattering_count", C.c_int32),
                 ("density", C.POINTER(C.c_float)), ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
-                ("white_for_indirect", C.c_int32), ("local_to_world", C.c_float * 12)]
+                ("white_for_indirect", C.c_int32), ("local_to_world", C.c_float * 12),
+                ("albedo_grid", C.POINTER(C.c_float)), ("anx", C.c_int32), ("any", C.c_int32), ("anz", C.c_int32),
+                ("g_grid", C.POINTER(C.c_float)), ("gnx", C.c_int32), ("gny", C.c_int32), ("gnz", C.c_int32)]
 
 
 def make_grid_medium(density, albedo=(1.0, 1.0, 1.0), g=0.0, local_to_world=None, max_scattering_count=2 ** 31 - 1,
                      white_for_indirect=False) -> TrMedium:
     """create_heterogeneous_medium (src/medium/heterogeneous.cpp:101-108) over
     a float density grid indexed [z, y, x]; local_to_world (3x4 or 4x4) maps
-    the unit grid cube into the world (identity by default).  The returned
-    structure keeps the grid alive."""
+    the unit grid cube into the world (identity by default).  albedo is an RGB
+    triple or an RGB grid indexed [z, y, x, c]; g a number or a grid indexed
+    [z, y, x] (their own sizes, the same unit cube).  The returned structure
+    keeps the grids alive."""
     import numpy as np
     grid = np.ascontiguousarray(density, np.float32)
     if grid.ndim != 3:
         raise ValueError("density must be a 3-D array indexed [z, y, x]")
     m = TrMedium()
-    m.sigma_s = f3(*albedo)
-    m.g = g
+    keep = [grid]
+    if np.ndim(albedo) == 4:
+        a = np.ascontiguousarray(albedo, np.float32)
+        m.albedo_grid = a.ctypes.data_as(C.POINTER(C.c_float))
+        m.anz, m.any, m.anx = a.shape[:3]
+        m.sigma_s = f3(1.0, 1.0, 1.0)
+        keep.append(a)
+    else:
+        m.sigma_s = f3(*albedo)
+    if np.ndim(g) == 3:
+        gg = np.ascontiguousarray(g, np.float32)
+        m.g_grid = gg.ctypes.data_as(C.POINTER(C.c_float))
+        m.gnz, m.gny, m.gnx = gg.shape
+        m.g = 0.0
+        keep.append(gg)
+    else:
+        m.g = g
     m.max_scattering_count = max_scattering_count
     m.density = grid.ctypes.data_as(C.POINTER(C.c_float))
     m.nz, m.ny, m.nx = grid.shape
     m.white_for_indirect = 1 if white_for_indirect else 0
     M = np.eye(4) if local_to_world is None else np.asarray(local_to_world, np.float64)
     m.local_to_world[:] = [float(x) for x in M[:3, :4].reshape(-1)]
-    m._grid = grid
+    m._grid = keep
     return m
 
 

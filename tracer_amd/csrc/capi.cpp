@@ -21,7 +21,7 @@ using namespace trcapi;
 
 // The ctypes bindings (tracer_amd/abi.py) assume these layouts.
 static_assert(sizeof(tr_material) == 132, "tr_material layout");
-static_assert(sizeof(tr_medium) == 104, "tr_medium layout");
+static_assert(sizeof(tr_medium) == 152, "tr_medium layout");
 static_assert(sizeof(tr_camera) == 52, "tr_camera layout");
 static_assert(sizeof(tr_render_params) == 96, "tr_render_params layout");
 static_assert(sizeof(tr_render_stats) == 160, "tr_render_stats layout");
@@ -202,12 +202,16 @@ DevMedium make_grid_medium(const tr_medium& m, const std::vector<float>& grid, i
     invert_affine(m.local_to_world, inv);
     for (int k = 0; k < 12; ++k) d.w2l[k] = float(inv[k]);
     d.white_indirect = m.white_for_indirect ? 1 : 0;
+    d.albedo_grid = -1;
+    d.g_grid = -1;
     return d;
 }
 
 DevMedium make_medium(const tr_medium& m) {
     DevMedium d{};
     d.grid = -1;
+    d.albedo_grid = -1;
+    d.g_grid = -1;
     d.sigma_s = mk(m.sigma_s[0], m.sigma_s[1], m.sigma_s[2]);
     const f3 sa = mk(m.sigma_a[0], m.sigma_a[1], m.sigma_a[2]);
     d.sigma_t = sa + d.sigma_s;
@@ -323,6 +327,25 @@ void validate_scene(const tr_scene_desc* d) {
                     throw InvalidArg("density grid values must be finite and >= 0");
             double inv[12];
             if (!invert_affine(m.local_to_world, inv)) throw InvalidArg("medium local_to_world is singular");
+            auto check_dims = [](int x, int y, int z, int ch, const char* what) {
+                if (x < 1 || y < 1 || z < 1 || int64_t(x) * y * z * ch > (int64_t(1) << 31) - 1)
+                    throw InvalidArg(std::string(what) + " grid dimensions must be positive and hold fewer than 2^31 values");
+                return size_t(x) * size_t(y) * size_t(z) * size_t(ch);
+            };
+            if (m.albedo_grid) {
+                const size_t n = check_dims(m.anx, m.any, m.anz, 3, "albedo");
+                for (size_t k = 0; k < n; ++k)
+                    if (!(m.albedo_grid[k] >= 0.0f && std::isfinite(m.albedo_grid[k])))
+                        throw InvalidArg("albedo grid values must be finite and >= 0");
+            }
+            if (m.g_grid) {
+                const size_t n = check_dims(m.gnx, m.gny, m.gnz, 1, "g");
+                for (size_t k = 0; k < n; ++k)
+                    if (!(m.g_grid[k] >= -1.0f && m.g_grid[k] <= 1.0f))
+                        throw InvalidArg("g grid values must lie in [-1, 1]");
+            }
+        } else if (m.albedo_grid || m.g_grid) {
+            throw InvalidArg("albedo and g grids need a density grid");
         }
         for (int k = 0; k < 3; ++k)
             if (m.sigma_a[k] < 0 || m.sigma_s[k] < 0) throw InvalidArg("medium coefficients must be >= 0");
@@ -524,9 +547,20 @@ void build_device_scene(tr_ctx* c, const tr_bvh_opts* opts) {
         } else {
             dmed[i] = make_grid_medium(h.media[i], h.grids[i], int(voxels.size()));
             voxels.insert(voxels.end(), h.grids[i].begin(), h.grids[i].end());
+            // albedo and g grids follow in the same pool
+            if (!h.albedo_grids[i].empty()) {
+                dmed[i].albedo_grid = int(voxels.size());
+                dmed[i].anx = h.media[i].anx; dmed[i].any = h.media[i].any; dmed[i].anz = h.media[i].anz;
+                voxels.insert(voxels.end(), h.albedo_grids[i].begin(), h.albedo_grids[i].end());
+            }
+            if (!h.g_grids[i].empty()) {
+                dmed[i].g_grid = int(voxels.size());
+                dmed[i].gnx = h.media[i].gnx; dmed[i].gny = h.media[i].gny; dmed[i].gnz = h.media[i].gnz;
+                voxels.insert(voxels.end(), h.g_grids[i].begin(), h.g_grids[i].end());
+            }
         }
     }
-    if (voxels.size() > size_t(INT32_MAX)) throw InvalidArg("density grids hold more than 2^31 voxels in total");
+    if (voxels.size() > size_t(INT32_MAX)) throw InvalidArg("medium grids hold more than 2^31 values in total");
 
     hipStream_t st = c->stream;
     upload(c->nodes, r.nodes.data(), r.nodes.size() * sizeof(Node4), st);
@@ -954,11 +988,21 @@ tr_status tr_scene_upload(tr_ctx* c, const tr_scene_desc* d) {
         h.mats.assign(d->materials, d->materials + d->num_materials);
         if (d->num_media) h.media.assign(d->media, d->media + d->num_media); else h.media.clear();
         h.grids.assign(h.media.size(), {});
+        h.albedo_grids.assign(h.media.size(), {});
+        h.g_grids.assign(h.media.size(), {});
         for (size_t i = 0; i < h.media.size(); ++i) {
             tr_medium& m = h.media[i];
             if (m.density) {
                 h.grids[i].assign(m.density, m.density + size_t(m.nx) * size_t(m.ny) * size_t(m.nz));
-                m.density = nullptr;   // the copy above is the grid from here on
+                m.density = nullptr;   // the copies here are the grids from here on
+            }
+            if (m.albedo_grid) {
+                h.albedo_grids[i].assign(m.albedo_grid, m.albedo_grid + 3 * size_t(m.anx) * size_t(m.any) * size_t(m.anz));
+                m.albedo_grid = nullptr;
+            }
+            if (m.g_grid) {
+                h.g_grids[i].assign(m.g_grid, m.g_grid + size_t(m.gnx) * size_t(m.gny) * size_t(m.gnz));
+                m.g_grid = nullptr;
             }
         }
         h.has_env = d->has_env;
@@ -1218,6 +1262,25 @@ tr_status tr_medium_density_batch(tr_ctx* c, int32_t medium, const float* pos, i
         launch_grid_density(c->ds, medium, c->s_in.as<float>(), n, c->s_out.as<float>(), c->stream);
         hip_check(hipGetLastError(), "k_grid_density launch");
         hip_check(hipMemcpyAsync(density, c->s_out.as<float>(), size_t(n) * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+    });
+}
+
+tr_status tr_medium_albedo_g_batch(tr_ctx* c, int32_t medium, const float* pos, int64_t n, float* albedo, float* g) {
+    return guarded(c, [&] {
+        if (!c->have_bvh) throw StateError("medium batch before tr_bvh_build");
+        if (medium < 0 || size_t(medium) >= c->hs.media.size() || c->hs.grids[size_t(medium)].empty())
+            throw InvalidArg("medium is not a density grid of the scene");
+        if (n < 0 || n > (int64_t(1) << 26)) throw InvalidArg("n must lie in [0, 2^26]");
+        if (!pos || !albedo || !g) throw InvalidArg("NULL array");
+        if (n == 0) return;
+        upload(c->s_in, pos, size_t(n) * 12, c->stream);
+        c->s_out.reserve(size_t(n) * 16);
+        float* out = c->s_out.as<float>();
+        launch_grid_albedo_g(c->ds, medium, c->s_in.as<float>(), n, out, out + 3 * n, c->stream);
+        hip_check(hipGetLastError(), "k_grid_albedo_g launch");
+        hip_check(hipMemcpyAsync(albedo, out, size_t(n) * 12, hipMemcpyDeviceToHost, c->stream), "D2H");
+        hip_check(hipMemcpyAsync(g, out + 3 * n, size_t(n) * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
         hip_check(hipStreamSynchronize(c->stream), "sync");
     });
 }

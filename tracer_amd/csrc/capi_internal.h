@@ -72,6 +72,7 @@ struct HostScene {
     std::vector<tr_material> mats;
     std::vector<tr_medium> media;
     std::vector<std::vector<float>> grids;   // density grid per medium (empty: homogeneous)
+    std::vector<std::vector<float>> albedo_grids, g_grids;   // per medium, empty: constant
     int has_env = 0, has_sun = 0;
     float env[3] = {0, 0, 0}, sun_dir[3] = {0, 0, 1}, sun_E[3] = {0, 0, 0};
     tr_camera cam{};

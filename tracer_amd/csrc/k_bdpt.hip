@@ -101,7 +101,7 @@ TR_D void load_scatter(const DevScene& s, const BdVertex& v, BdScatter& c) {
     c.tri = aux;
     if (vtype(v) == BD_MEDIUM) {
         c.medium = true;
-        c.g = s.media[aux].g;
+        c.g = medium_g(s, s.media[aux], c.pos);
         c.med_in = c.med_out = aux;
         c.ng = c.ns = c.tx = c.ty = f3s(0.0f);
         c.mat = 0;
@@ -219,7 +219,7 @@ TR_D int track_segment(const DevScene& s, const DevMedium& m, f3 o, f3 d, float 
             if (tt >= t) return 0;
             const f3 p = o + d * tt;
             if (rng.uniform() < grid_density(s, m, p) * inv) {
-                T *= m.sigma_s;   // the albedo of a grid medium
+                T *= medium_albedo(s, m, p);   // the albedo of a grid medium at the collision
                 x = p;
                 return 1;
             }
@@ -401,7 +401,7 @@ TR_D bool extend_one(const DevScene& s, const BdPaths& b, const BdConsts& k, int
         *cntp = ++cnt;
         if (cnt >= maxc) return false;
         float ppdf;
-        const f3 wi = hg_sample(s.media[mid].g, wo, rng.uniform(), rng.uniform(), ppdf);
+        const f3 wi = hg_sample(medium_g(s, s.media[mid], x), wo, rng.uniform(), rng.uniform(), ppdf);
         // the phase function is symmetric: the reverse density is ppdf
         const float rev = pdf_to_area(ppdf, x, prev);
         if (light) prev.p.w = rev;

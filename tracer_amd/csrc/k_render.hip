@@ -283,7 +283,7 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
                     const f3 x = o + d * tt;
                     const float dens = grid_density(s, m, x);
                     if (rng.uniform() < dens * inv) {
-                        if (!(m.white_indirect && scatter > 0)) T *= m.sigma_s;
+                        if (!(m.white_indirect && scatter > 0)) T *= medium_albedo(s, m, x);
                         scattered = true;
                         sp_pos = x;
                         break;
@@ -317,6 +317,7 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
         } else if (scattered) {
             // ---- medium vertex (path_trace_renderer.cpp:94-115) ----
             const DevMedium m = s.media[mid];
+            const float mg = EXT ? medium_g(s, m, sp_pos) : m.g;   // the asymmetry at the collision
             ++scatter;
             const f3 wo = -d;
             if (lc_use<LCM>(p)) next_cell = lc_cell(p.lc, sp_pos);
@@ -326,7 +327,7 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
                 const int e = nee_pick<LCM>(s, p, sp_pos, next_cell, rng, lu, lv, pmf, lcell, lbin);
                 const LightSampleD ls = sample_light<EXT>(s, e, sp_pos, lu, lv);
                 if (ls.pdf != 0.0f && pmf > 0.0f && !is_black(ls.L)) {
-                    const float ph = hg_phase(m.g, -dot(ls.wi, wo));
+                    const float ph = hg_phase(mg, -dot(ls.wi, wo));
                     float w = 1.0f;
                     float lp = pmf;
                     if (ls.pdf > 0.0f) { lp *= ls.pdf; w = power_heuristic(lp * fnls, ph); }
@@ -357,7 +358,7 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
                 }
             }
             float ppdf;
-            const f3 wi = hg_sample(m.g, wo, rng.uniform(), rng.uniform(), ppdf);
+            const f3 wi = hg_sample(mg, wo, rng.uniform(), rng.uniform(), ppdf);
             // f / pdf = 1 for the phase function
             next_o = f4(sp_pos, 0.0f);
             next_d = f4(wi, 3.0e38f);
@@ -963,6 +964,15 @@ __global__ void k_env_eval(DevScene s, const float* d, int64_t n, float* L, floa
     L[3 * i] = e.x; L[3 * i + 1] = e.y; L[3 * i + 2] = e.z;
     pdf[i] = p;
 }
+__global__ void k_grid_albedo_g(DevScene s, int medium, const float* x, int64_t n, float* albedo, float* g) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const DevMedium& m = s.media[medium];
+    const f3 p = mk(x[3 * i], x[3 * i + 1], x[3 * i + 2]);
+    const f3 a = medium_albedo(s, m, p);
+    albedo[3 * i] = a.x; albedo[3 * i + 1] = a.y; albedo[3 * i + 2] = a.z;
+    g[i] = medium_g(s, m, p);
+}
 __global__ void k_grid_density(DevScene s, int medium, const float* x, int64_t n, float* out) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -1089,6 +1099,11 @@ void launch_hg_sample(float g, const float* wo, const float* u, int64_t n, float
 void launch_unpack_radiance(const float* L, int64_t n, float4* rad, hipStream_t st) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_unpack_radiance, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, L, n, rad);
+}
+void launch_grid_albedo_g(const DevScene& s, int medium, const float* x, int64_t n, float* albedo, float* g,
+                          hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_grid_albedo_g, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, s, medium, x, n, albedo, g);
 }
 void launch_grid_density(const DevScene& s, int medium, const float* x, int64_t n, float* out, hipStream_t st) {
     if (n <= 0) return;

@@ -154,6 +154,8 @@ void launch_film_ldr(const double4* film, unsigned char* rgb, int num_pixels, fl
                      hipStream_t st);
 void launch_env_sample(const DevScene& s, const float* u, int64_t n, float* wi, float* pdf, float* L, hipStream_t st);
 void launch_env_eval(const DevScene& s, const float* d, int64_t n, float* L, float* pdf, hipStream_t st);
+void launch_grid_albedo_g(const DevScene& s, int medium, const float* x, int64_t n, float* albedo, float* g,
+                          hipStream_t st);
 void launch_grid_density(const DevScene& s, int medium, const float* x, int64_t n, float* out, hipStream_t st);
 void launch_tex_eval(const DevScene& s, int texture, const float* uv, int64_t n, float* out, hipStream_t st);
 // Light-cache sampling tables (pmf, alias table) of every cell from the training sums.

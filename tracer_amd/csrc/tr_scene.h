@@ -24,7 +24,11 @@ struct DevMedium {
     int nx, ny, nz;
     float w2l[12];         // world to grid coordinates, row-major 3x4
     int white_indirect;    // albedo 1 after the first scatter of a run
-    int pad[3];
+    int albedo_grid;       // first value in DevScene::density of an RGB albedo grid (anx * any * anz triples), -1: sigma_s
+    int g_grid;            // first value of an asymmetry grid (gnx * gny * gnz), -1: g
+    int pad0;
+    int anx, any, anz, gnx, gny, gnz;
+    int pad1[2];
 };
 
 // ThinLensCamera (src/camera/thin_lens_camera.cpp:23-36) with look_at's
@@ -219,24 +223,61 @@ TR_D f3 env_eval(const DevScene& s, f3 d, float& pdf) {
 // each wrapped into [0, 1] (Texture3D::wrap_repeat, src/core/texture.hpp:114-116),
 // then trilinear over the voxels (LinearSampler::Sample3D,
 // src/utility/sampler.hpp:47-67).
-TR_D float grid_density(const DevScene& s, const DevMedium& m, f3 p) {
+// Grid coordinates of world point p in medium m: world_to_local, every
+// coordinate wrapped into [0, 1] (Texture3D's wrap_repeat).
+TR_D f3 grid_local(const DevMedium& m, f3 p) {
     const float* w = m.w2l;
     const float lx = w[0] * p.x + w[1] * p.y + w[2] * p.z + w[3];
     const float ly = w[4] * p.x + w[5] * p.y + w[6] * p.z + w[7];
     const float lz = w[8] * p.x + w[9] * p.y + w[10] * p.z + w[11];
     auto wrap = [](float x) { return fminf(fmaxf(x - floorf(x), 0.0f), 1.0f); };
-    const float u = wrap(lx) * float(m.nx - 1), v = wrap(ly) * float(m.ny - 1), k = wrap(lz) * float(m.nz - 1);
-    const int u0 = min(int(u), m.nx - 1), v0 = min(int(v), m.ny - 1), k0 = min(int(k), m.nz - 1);
-    const int u1 = min(u0 + 1, m.nx - 1), v1 = min(v0 + 1, m.ny - 1), k1 = min(k0 + 1, m.nz - 1);
+    return mk(wrap(lx), wrap(ly), wrap(lz));
+}
+
+// Trilinear lookup (LinearSampler::Sample3D, src/utility/sampler.hpp:47-67)
+// of an nx * ny * nz grid of CH floats per voxel (x fastest) at wrapped local
+// coordinates l: CH values into out.
+template <int CH>
+TR_D void grid_sample(const float* g, int nx, int ny, int nz, f3 l, float* out) {
+    const float u = l.x * float(nx - 1), v = l.y * float(ny - 1), k = l.z * float(nz - 1);
+    const int u0 = min(int(u), nx - 1), v0 = min(int(v), ny - 1), k0 = min(int(k), nz - 1);
+    const int u1 = min(u0 + 1, nx - 1), v1 = min(v0 + 1, ny - 1), k1 = min(k0 + 1, nz - 1);
     const float du = u - float(u0), dv = v - float(v0), dk = k - float(k0);
-    const float* g = s.density + m.grid;
-    const size_t sy = size_t(m.nx), sz = size_t(m.nx) * size_t(m.ny);
-    auto at = [&](int x, int y, int z) { return __ldg(g + size_t(z) * sz + size_t(y) * sy + size_t(x)); };
-    const float c00 = at(u0, v0, k0) * (1.0f - du) + at(u1, v0, k0) * du;
-    const float c10 = at(u0, v1, k0) * (1.0f - du) + at(u1, v1, k0) * du;
-    const float c01 = at(u0, v0, k1) * (1.0f - du) + at(u1, v0, k1) * du;
-    const float c11 = at(u0, v1, k1) * (1.0f - du) + at(u1, v1, k1) * du;
-    return (c00 * (1.0f - dv) + c10 * dv) * (1.0f - dk) + (c01 * (1.0f - dv) + c11 * dv) * dk;
+    const size_t sy = size_t(nx), sz = size_t(nx) * size_t(ny);
+    auto at = [&](int x, int y, int z, int c) {
+        return __ldg(g + (size_t(z) * sz + size_t(y) * sy + size_t(x)) * CH + c);
+    };
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+        const float c00 = at(u0, v0, k0, c) * (1.0f - du) + at(u1, v0, k0, c) * du;
+        const float c10 = at(u0, v1, k0, c) * (1.0f - du) + at(u1, v1, k0, c) * du;
+        const float c01 = at(u0, v0, k1, c) * (1.0f - du) + at(u1, v0, k1, c) * du;
+        const float c11 = at(u0, v1, k1, c) * (1.0f - du) + at(u1, v1, k1, c) * du;
+        out[c] = (c00 * (1.0f - dv) + c10 * dv) * (1.0f - dk) + (c01 * (1.0f - dv) + c11 * dv) * dk;
+    }
+}
+
+TR_D float grid_density(const DevScene& s, const DevMedium& m, f3 p) {
+    float d;
+    grid_sample<1>(s.density + m.grid, m.nx, m.ny, m.nz, grid_local(m, p), &d);
+    return d;
+}
+
+// Albedo and asymmetry of a grid medium at p (HeterogeneousMedium::sample
+// reads albedo->evaluate and g->evaluate_s at the collision,
+// src/medium/heterogeneous.cpp:66-78): their grids, or the constants.  g is
+// kept inside (-1, 1) as the constant is.
+TR_D f3 medium_albedo(const DevScene& s, const DevMedium& m, f3 p) {
+    if (m.albedo_grid < 0) return m.sigma_s;
+    float a[3];
+    grid_sample<3>(s.density + m.albedo_grid, m.anx, m.any, m.anz, grid_local(m, p), a);
+    return mk(a[0], a[1], a[2]);
+}
+TR_D float medium_g(const DevScene& s, const DevMedium& m, f3 p) {
+    if (m.g_grid < 0) return m.g;
+    float g;
+    grid_sample<1>(s.density + m.g_grid, m.gnx, m.gny, m.gnz, grid_local(m, p), &g);
+    return fminf(fmaxf(g, -1.0f + kEps), 1.0f - kEps);
 }
 
 // Ratio-tracking transmittance of a grid medium over [0, dist) along d

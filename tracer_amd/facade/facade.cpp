@@ -230,12 +230,13 @@ RC<Texture3D> create_image_texture3d(const RC<Image3D<real>>& image) {
 RC<Medium> create_heterogeneous_medium(const Transform& local_to_world, RC<const Texture3D> density,
                                        RC<const Texture3D> albedo, RC<const Texture3D> g, int max_sc, bool white) {
     if (!density || !albedo || !g) throw std::runtime_error("create_heterogeneous_medium: texture must not be null");
-    if (!albedo->is_constant() || !g->is_constant())
-        throw std::runtime_error("create_heterogeneous_medium: the GPU path takes a constant albedo and g");
-    if (!density->is_constant() && !dynamic_cast<const ImageTexture3D*>(density.get()))
-        throw std::runtime_error("create_heterogeneous_medium: density must come from create_image_texture3d");
-    const real gv = g->evaluate_s(Point3f(0, 0, 0));
-    if (!(gv > -1 && gv < 1)) throw std::runtime_error("create_heterogeneous_medium: g must lie in (-1, 1)");
+    for (const Texture3D* t : {density.get(), albedo.get(), g.get()})
+        if (!t->is_constant() && !dynamic_cast<const ImageTexture3D*>(t))
+            throw std::runtime_error("create_heterogeneous_medium: textures must be constant or from create_image_texture3d");
+    if (g->is_constant()) {
+        const real gv = g->evaluate_s(Point3f(0, 0, 0));
+        if (!(gv > -1 && gv < 1)) throw std::runtime_error("create_heterogeneous_medium: g must lie in (-1, 1)");
+    }
     auto m = newRC<HeterogeneousMedium>();
     m->local_to_world = local_to_world;
     m->density = std::move(density);
@@ -463,9 +464,30 @@ FlatScene flatten(const Scene& scene) {
         if (it != med_id.end()) return it->second;
         tr_medium tm{};
         if (const auto* hm = dynamic_cast<const HeterogeneousMedium*>(m.get())) {
-            const Spectrum a = hm->albedo->evaluate(Point3f(0, 0, 0));
-            tm.sigma_s[0] = a.r; tm.sigma_s[1] = a.g; tm.sigma_s[2] = a.b;
-            tm.g = hm->g->evaluate_s(Point3f(0, 0, 0));
+            // albedo and g: constants, or grids looked up per collision (a grey
+            // image gives the same albedo in every channel)
+            if (const auto* ia = dynamic_cast<const ImageTexture3D*>(hm->albedo.get())) {
+                const Image3D<real>& im = *ia->image;
+                tm.anx = im.width(); tm.any = im.height(); tm.anz = im.depth();
+                const size_t n = size_t(tm.anx) * size_t(tm.any) * size_t(tm.anz);
+                f.grids.emplace_back(3 * n);
+                for (size_t k = 0; k < n; ++k)
+                    for (int c = 0; c < 3; ++c) f.grids.back()[3 * k + size_t(c)] = im.get_raw_data()[k];
+                tm.albedo_grid = f.grids.back().data();
+                tm.sigma_s[0] = tm.sigma_s[1] = tm.sigma_s[2] = 1.0f;
+            } else {
+                const Spectrum a = hm->albedo->evaluate(Point3f(0, 0, 0));
+                tm.sigma_s[0] = a.r; tm.sigma_s[1] = a.g; tm.sigma_s[2] = a.b;
+            }
+            if (const auto* ig = dynamic_cast<const ImageTexture3D*>(hm->g.get())) {
+                const Image3D<real>& im = *ig->image;
+                tm.gnx = im.width(); tm.gny = im.height(); tm.gnz = im.depth();
+                f.grids.emplace_back(im.get_raw_data(), im.get_raw_data() + size_t(tm.gnx) * size_t(tm.gny) * size_t(tm.gnz));
+                tm.g_grid = f.grids.back().data();
+                tm.g = 0.0f;
+            } else {
+                tm.g = hm->g->evaluate_s(Point3f(0, 0, 0));
+            }
             tm.max_scattering_count = hm->max_scattering_count;
             tm.white_for_indirect = hm->white_for_indirect ? 1 : 0;
             // the flat scene keeps its own copy of the voxels (a constant density

@@ -503,6 +503,8 @@ struct GridFog {   // a heterogeneous medium for obj_scene's outside
     float local_to_world[12];
     float albedo[3];
     float g = 0;
+    const float* albedo_grid = nullptr;   // grey albedo per voxel (same size as the density), or NULL
+    const float* g_grid = nullptr;        // g per voxel, or NULL
     int max_scattering_count = 0;
 };
 
@@ -527,10 +529,18 @@ static void obj_scene(Config& c, const std::string& path, const tr_camera& tc, b
         for (int i = 0; i < 3; ++i)
             for (int j = 0; j < 4; ++j) m[i][j] = grid->local_to_world[4 * i + j];
         m[3][3] = 1;
+        const size_t n = size_t(grid->nx) * size_t(grid->ny) * size_t(grid->nz);
+        auto image_of = [&](const float* v) {
+            auto img = newRC<Image3D<real>>(grid->nx, grid->ny, grid->nz);
+            std::copy(v, v + n, img->get_raw_data());
+            return create_image_texture3d(img);
+        };
         b.mi.outside = create_heterogeneous_medium(
             Transform(m), create_image_texture3d(im),
-            create_constant_texture3d(Spectrum(grid->albedo[0], grid->albedo[1], grid->albedo[2])),
-            create_constant_texture3d(Spectrum(grid->g)), grid->max_scattering_count, false);
+            grid->albedo_grid ? image_of(grid->albedo_grid)
+                              : create_constant_texture3d(Spectrum(grid->albedo[0], grid->albedo[1], grid->albedo[2])),
+            grid->g_grid ? image_of(grid->g_grid) : create_constant_texture3d(Spectrum(grid->g)),
+            grid->max_scattering_count, false);
     }
     for (const mesh_t& mesh : model.mesh) {
         auto tris = create_triangle_mesh(mesh, Transform());
@@ -622,7 +632,8 @@ trs_scene* trs_load_obj(const char* path, const tr_camera* camera, int32_t outsi
 }
 trs_scene* trs_load_obj_grid(const char* path, const tr_camera* camera, const float* density, int32_t nx, int32_t ny,
                              int32_t nz, const float* local_to_world, const float* albedo, float g,
-                             int32_t max_scattering_count, const char** err) {
+                             const float* albedo_grid, const float* g_grid, int32_t max_scattering_count,
+                             const char** err) {
     static thread_local std::string msg;
     try {
         if (!path || !camera || !density || !local_to_world || !albedo)
@@ -634,6 +645,8 @@ trs_scene* trs_load_obj_grid(const char* path, const tr_camera* camera, const fl
         std::copy(local_to_world, local_to_world + 12, gf.local_to_world);
         std::copy(albedo, albedo + 3, gf.albedo);
         gf.g = g;
+        gf.albedo_grid = albedo_grid;
+        gf.g_grid = g_grid;
         gf.max_scattering_count = max_scattering_count;
         tracer::scenes::Config c;
         tracer::scenes::obj_scene(c, path, *camera, false, &gf);

@@ -70,6 +70,7 @@ def hip_lib() -> C.CDLL:
         lib.tr_film_to_ldr.argtypes = [VP, C.c_float, C.c_float, P(C.c_uint8), I64]
         lib.tr_env_eval_batch.argtypes = [VP, P(C.c_float), I64, P(C.c_float), P(C.c_float)]
         lib.tr_medium_density_batch.argtypes = [VP, C.c_int32, P(C.c_float), I64, P(C.c_float)]
+        lib.tr_medium_albedo_g_batch.argtypes = [VP, C.c_int32, P(C.c_float), I64, P(C.c_float), P(C.c_float)]
         lib.tr_texture_eval_batch.argtypes = [VP, C.c_int32, P(C.c_float), I64, P(C.c_float)]
         F = P(C.c_float)
         lib.tr_camera_ray_batch.argtypes = [VP, F, F, I64, F]
@@ -109,7 +110,8 @@ def scenes_lib() -> C.CDLL:
         lib.trs_load_obj.restype = C.c_void_p
         lib.trs_load_obj_grid.argtypes = [C.c_char_p, C.POINTER(abi.TrCamera), C.POINTER(C.c_float), C.c_int32,
                                           C.c_int32, C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_float),
-                                          C.c_float, C.c_int32, C.POINTER(C.c_char_p)]
+                                          C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int32,
+                                          C.POINTER(C.c_char_p)]
         lib.trs_load_obj_grid.restype = C.c_void_p
         lib.trs_read_hdr.argtypes = [C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_float),
                                      C.c_int64, C.POINTER(C.c_char_p)]
@@ -281,17 +283,30 @@ def obj_grid_scene(path, camera: abi.TrCamera, density, local_to_world, albedo=(
     """obj_scene with a heterogeneous medium outside every surface, built by
     the C++ facade through create_image_texture3d and
     create_heterogeneous_medium; density is indexed [z, y, x] and
-    local_to_world (3x4 or 4x4) maps the unit grid cube into the world."""
+    local_to_world (3x4 or 4x4) maps the unit grid cube into the world.
+    albedo is an RGB triple or a grey grid, g a number or a grid, both grids
+    of the density's shape."""
     lib = scenes_lib()
     err = C.c_char_p()
     grid = np.ascontiguousarray(density, np.float32)
     if grid.ndim != 3:
         raise ValueError("density must be a 3-D array indexed [z, y, x]")
     l2w = np.ascontiguousarray(np.asarray(local_to_world, np.float64)[:3, :4], np.float32)
-    alb = np.ascontiguousarray(albedo, np.float32)
+    alb_grid = g_grid = None
+    if np.ndim(albedo) == 3:
+        alb_grid = np.ascontiguousarray(albedo, np.float32)
+        alb = np.ones(3, np.float32)
+    else:
+        alb = np.ascontiguousarray(albedo, np.float32)
+    if np.ndim(g) == 3:
+        g_grid = np.ascontiguousarray(g, np.float32)
+        g = 0.0
+    for a in (alb_grid, g_grid):
+        if a is not None and a.shape != grid.shape:
+            raise ValueError("albedo and g grids must have the density's shape")
     nz, ny, nx = grid.shape
     h = lib.trs_load_obj_grid(str(path).encode(), C.byref(camera), fptr(grid), nx, ny, nz, fptr(l2w), fptr(alb),
-                              float(g), int(max_scattering_count), C.byref(err))
+                              float(g), fptr(alb_grid), fptr(g_grid), int(max_scattering_count), C.byref(err))
     if not h:
         raise TracerError(abi.TR_ERR_INVALID, (err.value or b"").decode())
     return _scene_from_handle(lib, h)
@@ -365,11 +380,21 @@ def _scene_from_handle(lib, h) -> Scene:
 
         mats = [TrMaterial.from_buffer_copy(d.materials[i]) for i in range(d.num_materials)]
         media = [TrMedium.from_buffer_copy(d.media[i]) for i in range(d.num_media)]
-        for m in media:   # copy each density grid out of the handle's memory
+        for m in media:   # copy each density, albedo and g grid out of the handle's memory
+            keep = []
             if m.density:
                 grid = np.ctypeslib.as_array(m.density, shape=(m.nz * m.ny * m.nx,)).reshape(m.nz, m.ny, m.nx).copy()
                 m.density = grid.ctypes.data_as(C.POINTER(C.c_float))
-                m._grid = grid
+                keep.append(grid)
+            if m.albedo_grid:
+                a = np.ctypeslib.as_array(m.albedo_grid, shape=(3 * m.anz * m.any * m.anx,)).copy()
+                m.albedo_grid = a.ctypes.data_as(C.POINTER(C.c_float))
+                keep.append(a)
+            if m.g_grid:
+                g = np.ctypeslib.as_array(m.g_grid, shape=(m.gnz * m.gny * m.gnx,)).copy()
+                m.g_grid = g.ctypes.data_as(C.POINTER(C.c_float))
+                keep.append(g)
+            m._grid = keep
         defaults = TrRenderParams()
         lib.trs_render_defaults(h, C.byref(defaults))
         textures = []
@@ -543,6 +568,14 @@ class Context:
         out = np.zeros(x.shape[0], np.float32)
         self._check(self.lib.tr_medium_density_batch(self.h, int(medium), fptr(x), x.shape[0], fptr(out)))
         return out
+
+    def medium_albedo_g(self, medium: int, pos: np.ndarray):
+        """Albedo (n, 3) and asymmetry (n,) of grid medium `medium` at world positions (n, 3)."""
+        x = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+        a = np.zeros((x.shape[0], 3), np.float32)
+        g = np.zeros(x.shape[0], np.float32)
+        self._check(self.lib.tr_medium_albedo_g_batch(self.h, int(medium), fptr(x), x.shape[0], fptr(a), fptr(g)))
+        return a, g
 
     def texture_eval(self, texture: int, uv: np.ndarray) -> np.ndarray:
         """Texture `texture` (1-based) of the loaded scene at uv (n, 2) -> (n, 3)."""
