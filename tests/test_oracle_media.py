@@ -88,7 +88,7 @@ def test_facade_grid_scene(tmp_path):
     assert len(s.media) == 1
     m = s.media[0]
     assert (m.nx, m.ny, m.nz) == (6, 5, 4) and m.max_scattering_count == 6
-    assert np.array_equal(m._grid, grid)
+    assert np.array_equal(m._grid[0], grid)   # the density grid, copied out of the handle
     assert np.allclose(np.array(m.local_to_world[:]).reshape(3, 4), ROOM)
     assert np.allclose(list(m.sigma_s), [0.7, 0.8, 0.9]) and m.g == pytest.approx(0.3)
     assert s.medium_outside is not None and set(np.unique(s.medium_outside)) == {0}

@@ -47,6 +47,13 @@ int trace_leaf_share_any() {
     static const int v = env_share("TRACER_HIP_LEAF_SHARE_ANY", 4);
     return v;
 }
+// Rounds a shade window must span before the plain shade kernels order it by
+// shading class (RenderConsts::shade_sort_rounds); TRACER_HIP_SHADE_SORT (0-8)
+// overrides it for tuning, 5 and above never sort.
+int shade_sort_rounds() {
+    static const int v = env_share("TRACER_HIP_SHADE_SORT", 2);
+    return v;
+}
 
 // DisneyBSDF constructor (src/material/disney.cpp:347-399) evaluated once per material.
 DevMaterial make_material_untextured(const tr_material& m);
@@ -566,6 +573,14 @@ void build_device_scene(tr_ctx* c, const tr_bvh_opts* opts) {
     upload(c->nodes, r.nodes.data(), r.nodes.size() * sizeof(Node4), st);
     upload(c->tris, r.tris.data(), r.tris.size() * sizeof(TriPacked), st);
     upload(c->tri_shade, shade.data(), shade.size() * sizeof(float4), st);
+    // shading class per leaf-ordered triangle (kept until the stream synchronises)
+    std::vector<unsigned char> cls(static_cast<size_t>(nt));
+    for (int64_t i = 0; i < nt; ++i) {
+        const int type = h.mats[size_t(h.mat[size_t(r.tris[size_t(i)].prim)])].type;
+        cls[size_t(i)] = (unsigned char)(type == TR_MAT_LAMBERT ? kShadeClassLambert
+                                         : type == TR_MAT_DISNEY ? kShadeClassDisney : kShadeClassOther);
+    }
+    upload(c->tri_cls, cls.data(), cls.size(), st);
     upload(c->mats, dm.data(), dm.size() * sizeof(DevMaterial), st);
     upload(c->media, dmed.data(), dmed.size() * sizeof(DevMedium), st);
     if (!voxels.empty()) upload(c->density, voxels.data(), voxels.size() * sizeof(float), st);
@@ -678,6 +693,7 @@ void build_device_scene(tr_ctx* c, const tr_bvh_opts* opts) {
     s.tr.leaf_share = trace_leaf_share();
     s.tr.leaf_share_any = trace_leaf_share_any();
     s.tri_shade = c->tri_shade.as<float4>();
+    s.tri_cls = c->tri_cls.as<unsigned char>();
     s.mats = c->mats.as<DevMaterial>();
     s.media = c->media.as<DevMedium>();
     s.lights = c->lights.as<float4>();
@@ -1510,6 +1526,7 @@ tr_status tr_render(tr_ctx* c, const tr_render_params* prm, double* film_sums, i
         rc.filter = P.filter_type; rc.filter_radius = P.filter_radius; rc.filter_alpha = P.filter_alpha;
         rc.shadow_tr = P.shadow_transmittance;
         rc.nls = nls;
+        rc.shade_sort_rounds = shade_sort_rounds();
         rc.seed = P.seed;
         const float guide_frac = guided ? (P.guide_fraction > 0.0f ? P.guide_fraction : 0.5f) : 0.0f;
         if (guided && !(P.guide_train_fraction >= 0.0f && P.guide_train_fraction < 1.0f))

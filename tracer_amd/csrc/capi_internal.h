@@ -145,7 +145,7 @@ struct tr_ctx {
     bool has_delta = false;   // a material samples specular (delta) lobes: paths may take extra bounces
     int film_w = 0, film_h = 0;   // size of the film of the last tr_render (kept on the device)
     // scene on device
-    trcapi::DevBuf nodes, tris, tri_shade, mats, media, lights, light_cdf, light_pmf, light_guide;
+    trcapi::DevBuf nodes, tris, tri_shade, tri_cls, mats, media, lights, light_cdf, light_pmf, light_guide;
     trcapi::DevBuf env_tex, env_cond_cdf, env_cond_func, env_marg_cdf, env_marg_func;
     trcapi::DevBuf tri_uv, tex_info, tex8, texf;   // material textures
     trcapi::DevBuf tri_tan;                        // normal derivatives (anisotropic materials)

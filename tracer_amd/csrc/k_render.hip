@@ -567,73 +567,155 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
 // with the combined pdf.  A vertex
 // whose head outputs are not finite uses the plain BSDF estimator, drawing
 // the same random numbers as the plain kernel.
+// A workgroup takes a window of up to kShadeSortRounds x kShadeBlock queue
+// entries, orders them by the shading class of their hit (kShadeClass*, a
+// counting sort in LDS that keeps queue order within a class) and shades the
+// window in that order, kShadeBlock paths per round.  Waves then hold paths
+// that take the same branches: rays that left the scene finish in a few
+// instructions instead of idling beside a Disney evaluation.  A path's result
+// does not depend on which thread shades it, so the film is unchanged.
+constexpr int kShadeSortRounds = 4;
+template <int BLOCK>
+struct ShadeSortLds {
+    int slot[kShadeSortRounds * BLOCK];
+    unsigned int cnt[kShadeClasses * kShadeSortRounds * (BLOCK / 64)];   // (class, round, wave)
+};
+static_assert(kShadeClasses * kShadeSortRounds * (kShadeBlock / 64) == 64, "one wave scans the class counts");
+
+// Fills sort.slot with the window [base, base + rounds * BLOCK) of queue q
+// (entries at or past n hold -1), ordered by class.  Every thread of the
+// block calls it; it ends with a barrier.
+template <int BLOCK>
+TR_D void shade_sort_window(ShadeSortLds<BLOCK>& sort, const DevScene& s, const DevPaths& p, const int* q,
+                            unsigned int n, unsigned int base, int rounds) {
+    constexpr int WAVES = BLOCK / 64;
+    const int lane = __lane_id(), wave = int(threadIdx.x) >> 6;
+    const unsigned long long below = lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
+    int slot[kShadeSortRounds], key[kShadeSortRounds];
+    unsigned int rank[kShadeSortRounds];
+#pragma unroll
+    for (int r = 0; r < kShadeSortRounds; ++r) {
+        slot[r] = -1;
+        key[r] = r < rounds ? int(kShadeClassMiss) : -1;   // -1: outside the window
+        const unsigned int i = base + unsigned(r * BLOCK) + threadIdx.x;
+        if (r < rounds && i < n) {
+            slot[r] = q[i];
+            const int tri = __float_as_int(p.hit[slot[r]].y);
+            if (tri >= 0) key[r] = int(s.tri_cls[tri]);
+        }
+        rank[r] = 0;
+#pragma unroll
+        for (int k = 0; k < kShadeClasses; ++k) {
+            const unsigned long long m = __ballot(key[r] == k);
+            if (key[r] == k) rank[r] = (unsigned int)__popcll(m & below);
+            if (lane == 0) sort.cnt[(k * kShadeSortRounds + r) * WAVES + wave] = (unsigned int)__popcll(m);
+        }
+    }
+    __syncthreads();
+    if (wave == 0) {
+        // exclusive prefix over (class, round, wave): where each wave's run of a class starts
+        const unsigned int own = sort.cnt[lane];
+        unsigned int v = own;
+        for (int off = 1; off < 64; off <<= 1) {
+            const unsigned int t = __shfl_up(v, off);
+            if (lane >= off) v += t;
+        }
+        sort.cnt[lane] = v - own;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kShadeSortRounds; ++r)
+        if (key[r] >= 0) sort.slot[sort.cnt[(key[r] * kShadeSortRounds + r) * WAVES + wave] + rank[r]] = slot[r];
+    __syncthreads();
+}
+
+// The guided kernels shade in queue order: the window sort's registers would
+// halve their occupancy.
 template <bool GUIDE, bool EXT, int LCM, bool MULTI = false>
 TR_D void shade_queue(const DevScene& s, const DevPaths& p, const RenderConsts& c, int cur) {
+    constexpr bool SORT = !GUIDE;
     __shared__ BlockAppendLds<kShadeBlock> app;
+    __shared__ ShadeSortLds<kShadeBlock> sort;
     const unsigned int n = *queue_counter(p, cur);
     const int* q = p.queue[cur];
     int* qn = p.queue[1 - cur];
-    const unsigned int stride = gridDim.x * blockDim.x;
     unsigned int n_guided = 0, n_fallback = 0;
-    // The loop bound is uniform across the block: every thread reaches the
-    // block-wide appends.
-    const unsigned int base0 = blockIdx.x * blockDim.x;
-    for (unsigned int base = base0; base < n; base += stride) {
-        const unsigned int i = base + threadIdx.x;
-        const bool active = i < n;
-        ShadeOut o;
-        o.cont = false;
-        o.sh.emit = false;
-        o.g_used = o.g_fallback = o.want_rec = o.want_nee_rec = false;
-        int slot = 0;
-        if (active) {
-            slot = q[i];
-            shade_path<GUIDE, EXT, LCM, MULTI>(s, p, c, slot, o);
-        }
-        const bool cont = o.cont, g_used = o.g_used, g_fallback = o.g_fallback;
-        const bool want_rec = o.want_rec, want_nee_rec = o.want_nee_rec;
-        const ShadowOut& sh = o.sh;
-        const float4 &rec0 = o.rec0, &rec1 = o.rec1, &rec2 = o.rec2, &rec3 = o.rec3, &aux0 = o.aux0;
-        const float4 &nrec0 = o.nrec0, &nrec1 = o.nrec1, &nrec2 = o.nrec2, &nrec3 = o.nrec3;
-        const float nee_weight = o.nee_weight;
-        n_guided += g_used ? 1u : 0u;
-        n_fallback += g_fallback ? 1u : 0u;
-        int nee_ri = -1;
-        if (GUIDE && c.guide_record) {
-            unsigned int ri, nri;
-            block_append2(app, p.rec_count, want_rec, p.rec_count, want_nee_rec, ri, nri);
-            {
-                const bool stored = want_rec && ri < (unsigned int)p.rec_cap;
-                if (stored) {
-                    p.rec[4 * ri] = rec0;
-                    p.rec[4 * ri + 1] = rec1;
-                    p.rec[4 * ri + 2] = rec2;
-                    p.rec[4 * ri + 3] = rec3;
-                    p.rec_aux[2 * ri] = aux0;
-                    p.rec_aux[2 * ri + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    // Rounds per window: as many as keep every workgroup of the grid busy.
+    const unsigned int per_block = n / (gridDim.x * unsigned(kShadeBlock));
+    const int rounds = per_block < 1u ? 1 : (per_block > unsigned(kShadeSortRounds) ? kShadeSortRounds : int(per_block));
+    const unsigned int window = unsigned(rounds) * kShadeBlock;
+    const unsigned int stride = gridDim.x * window;
+    const bool sorted = SORT && rounds >= c.shade_sort_rounds;
+    // The loop bounds are uniform across the block: every thread reaches the
+    // block-wide sort and appends.
+    for (unsigned int base = blockIdx.x * window; base < n; base += stride) {
+        if (sorted) shade_sort_window<kShadeBlock>(sort, s, p, q, n, base, rounds);
+        for (int r = 0; r < rounds; ++r) {
+            int slot_in;
+            if (sorted) {
+                // the round's waves take its 64-entry runs in a rotated order, so
+                // that the costly classes (last in the window) do not gather on one SIMD
+                const int rot = int(blockIdx.x + unsigned(r)) & (kShadeBlock / 64 - 1);
+                slot_in = sort.slot[r * kShadeBlock + ((int(threadIdx.x) + 64 * rot) & (kShadeBlock - 1))];
+            } else {
+                const unsigned int i = base + unsigned(r * kShadeBlock) + threadIdx.x;
+                slot_in = i < n ? q[i] : -1;
+            }
+            const bool active = slot_in >= 0;
+            ShadeOut o;
+            o.cont = false;
+            o.sh.emit = false;
+            o.g_used = o.g_fallback = o.want_rec = o.want_nee_rec = false;
+            int slot = 0;
+            if (active) {
+                slot = slot_in;
+                shade_path<GUIDE, EXT, LCM, MULTI>(s, p, c, slot, o);
+            }
+            const bool cont = o.cont, g_used = o.g_used, g_fallback = o.g_fallback;
+            const bool want_rec = o.want_rec, want_nee_rec = o.want_nee_rec;
+            const ShadowOut& sh = o.sh;
+            const float4 &rec0 = o.rec0, &rec1 = o.rec1, &rec2 = o.rec2, &rec3 = o.rec3, &aux0 = o.aux0;
+            const float4 &nrec0 = o.nrec0, &nrec1 = o.nrec1, &nrec2 = o.nrec2, &nrec3 = o.nrec3;
+            const float nee_weight = o.nee_weight;
+            n_guided += g_used ? 1u : 0u;
+            n_fallback += g_fallback ? 1u : 0u;
+            int nee_ri = -1;
+            if (GUIDE && c.guide_record) {
+                unsigned int ri, nri;
+                block_append2(app, p.rec_count, want_rec, p.rec_count, want_nee_rec, ri, nri);
+                {
+                    const bool stored = want_rec && ri < (unsigned int)p.rec_cap;
+                    if (stored) {
+                        p.rec[4 * ri] = rec0;
+                        p.rec[4 * ri + 1] = rec1;
+                        p.rec[4 * ri + 2] = rec2;
+                        p.rec[4 * ri + 3] = rec3;
+                        p.rec_aux[2 * ri] = aux0;
+                        p.rec_aux[2 * ri + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    }
+                    if (cont) p.grec[slot] = stored ? int(ri) : -1;
                 }
-                if (cont) p.grec[slot] = stored ? int(ri) : -1;
+                if (want_nee_rec && nri < (unsigned int)p.rec_cap) {
+                    nee_ri = int(nri);
+                    p.rec[4 * nri] = nrec0;
+                    p.rec[4 * nri + 1] = nrec1;
+                    p.rec[4 * nri + 2] = nrec2;
+                    p.rec[4 * nri + 3] = nrec3;
+                    p.rec_aux[2 * nri] = make_float4(nee_weight, 0.0f, 0.0f, 0.0f);
+                    p.rec_aux[2 * nri + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                }
             }
-            if (want_nee_rec && nri < (unsigned int)p.rec_cap) {
-                nee_ri = int(nri);
-                p.rec[4 * nri] = nrec0;
-                p.rec[4 * nri + 1] = nrec1;
-                p.rec[4 * nri + 2] = nrec2;
-                p.rec[4 * nri + 3] = nrec3;
-                p.rec_aux[2 * nri] = make_float4(nee_weight, 0.0f, 0.0f, 0.0f);
-                p.rec_aux[2 * nri + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            unsigned int qi, si;
+            block_append2(app, queue_counter(p, 1 - cur), cont, queue_counter(p, kShadowCounter), sh.emit, qi, si);
+            if (cont) qn[qi] = slot;
+            if (sh.emit) {
+                p.sh_o[si] = f4(sh.o, sh.tmin);
+                p.sh_d[si] = f4(sh.d, sh.tmax);
+                p.sh_c[si] = f4(sh.contrib, __int_as_float(slot));
+                if (GUIDE && c.guide_record) p.sh_rec[si] = nee_ri;
+                if (lc_train<LCM>(p))
+                    p.sh_g[si] = make_float4(__int_as_float(o.lc_cell), __int_as_float(o.lc_bin), o.lc_val, 0.0f);
             }
-        }
-        unsigned int qi, si;
-        block_append2(app, queue_counter(p, 1 - cur), cont, queue_counter(p, kShadowCounter), sh.emit, qi, si);
-        if (cont) qn[qi] = slot;
-        if (sh.emit) {
-            p.sh_o[si] = f4(sh.o, sh.tmin);
-            p.sh_d[si] = f4(sh.d, sh.tmax);
-            p.sh_c[si] = f4(sh.contrib, __int_as_float(slot));
-            if (GUIDE && c.guide_record) p.sh_rec[si] = nee_ri;
-            if (lc_train<LCM>(p))
-                p.sh_g[si] = make_float4(__int_as_float(o.lc_cell), __int_as_float(o.lc_bin), o.lc_val, 0.0f);
         }
     }
     if (GUIDE) {

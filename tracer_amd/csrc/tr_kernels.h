@@ -104,6 +104,7 @@ struct RenderConsts {
     float filter_radius, filter_alpha;
     int shadow_tr;       // TR_TR_*
     int nls;             // light samples per vertex
+    int shade_sort_rounds;   // shade kernels order a window by class when it spans at least this many rounds
     unsigned long long seed;
     float guide_frac;    // probability of sampling the guide mixture at a surface vertex
     int guide_max_depth; // guide only vertices of depth < this (0 = every depth)

@@ -53,11 +53,18 @@ struct DevEnvMap {
 };
 
 constexpr int kMatTexRow = 16;
+// Shading classes, in the order the shade kernels visit them: paths whose ray
+// left the scene (and unused window entries), Lambert surfaces, Disney
+// surfaces, every other material.
+enum : int { kShadeClassMiss = 0, kShadeClassLambert = 1, kShadeClassDisney = 2, kShadeClassOther = 3, kShadeClasses = 4 };
 constexpr int kMatRawRow = 20;
 
 struct DevScene {
     TraceScene tr;
     const float4* tri_shade;   // 3 float4 per leaf-ordered triangle: (n0, material), (n1, light), (n2, media)
+    // shading class per leaf-ordered triangle (kShadeClassLambert, _Disney,
+    // _Other): the shade kernels group the paths of a window by it
+    const unsigned char* tri_cls;
     const DevMaterial* mats;
     const DevMedium* media;
     const float4* lights;      // 4 float4 per area light: (L, area), (A, pmf), (B, tri), (C, 0)
