@@ -106,13 +106,19 @@ def test_bias_loads_in_bounds(H):
     assert hi == bias_bytes
 
 
-def test_inference_lds_staging_fits():
-    """k_guide_infer<64, 2, true> stages fwd_size / 8 fragments of 16 bytes in
-    the dynamic LDS that launch_guide_infer passes (fwd_size * 2 bytes)."""
-    d = Dims(64)
+@pytest.mark.parametrize("H, limit", [(64, 64 * 1024), (128, 160 * 1024)])
+def test_inference_lds_staging_fits(H, limit):
+    """k_guide_infer<H, NT, true, BLOCK> stages fwd_size / 8 fragments of 16
+    bytes in the dynamic LDS that launch_guide_infer passes (fwd_size * 2
+    bytes): width 64 within 64 KB (three workgroups on a CU's 160 KB), width
+    128 within the 160 KB a gfx950 workgroup may hold (the launcher checks the
+    device's limit and otherwise reads the fragments from global memory)."""
+    d = Dims(H)
     staged = d.fwd_size() // 8 * 16
     assert d.fwd_size() % 8 == 0 and staged == d.fwd_size() * 2
-    assert staged <= 64 * 1024
+    assert staged <= limit
+    if H == 64:
+        assert 3 * staged <= 160 * 1024
 
 
 @pytest.mark.parametrize("H", [64, 128])

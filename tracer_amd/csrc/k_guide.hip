@@ -175,19 +175,26 @@ TR_D void forward_all(const GuideBox& box, const Src& src, const f3 (&x)[NT], co
 // ---- inference over the wavefront queue --------------------------------------
 // Head outputs for every queued path whose extension ray hit a surface.  With
 // WLDS the forward fragments are staged in LDS once per workgroup.
-template <int H, int NT, bool WLDS>
-__global__ __launch_bounds__(256) void k_guide_infer(GuideNet net, DevScene s, DevPaths p, int cur, float* raw_out) {
-    extern __shared__ bf16x8 lds_w[];
+// With WLDS: the workgroup copies the forward fragments (fwd_size * 2 bytes,
+// the dynamic LDS its launcher passes) into LDS.
+template <int H, bool WLDS>
+TR_D void stage_forward(const GuideNet& net, bf16x8* lds_w) {
     if (WLDS) {
         const int nf = GuideDims{H}.fwd_size() / 8;
         for (int i = threadIdx.x; i < nf; i += blockDim.x) lds_w[i] = net.fwd[i];
         __syncthreads();
     }
+}
+
+template <int H, int NT, bool WLDS, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_guide_infer(GuideNet net, DevScene s, DevPaths p, int cur, float* raw_out) {
+    extern __shared__ bf16x8 lds_w[];
+    stage_forward<H, WLDS>(net, lds_w);
     const WeightSrc<WLDS> src = forward_src<WLDS>(net, H, lds_w);
     const unsigned int n = *queue_counter(p, cur);
     const int* q = p.queue[cur];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4;
-    const unsigned int per_block = 4 * 16 * NT;
+    const unsigned int per_block = (BLOCK / 64) * 16 * NT;
     unsigned long long evals = 0;   // network evaluations of this wave, added to stats[7] once
     for (unsigned int bbase = blockIdx.x * per_block; bbase < n; bbase += gridDim.x * per_block) {
         keep_loads_in_loop();
@@ -235,11 +242,15 @@ __global__ __launch_bounds__(256) void k_guide_infer(GuideNet net, DevScene s, D
 }
 
 // ---- inference on explicit points (x, n, wo: 9 floats each) -------------------
-template <int H, int NT>
-__global__ __launch_bounds__(256) void k_guide_forward_points(GuideNet net, const float* pts, int n, float* raw_out) {
-    const WeightSrc<false> src = forward_src<false>(net, H, nullptr);
+// The same forward pass and weight staging as k_guide_infer, so that its
+// checks against the oracle cover the kernels the renderer runs.
+template <int H, int NT, bool WLDS, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_guide_forward_points(GuideNet net, const float* pts, int n, float* raw_out) {
+    extern __shared__ bf16x8 lds_w[];
+    stage_forward<H, WLDS>(net, lds_w);
+    const WeightSrc<WLDS> src = forward_src<WLDS>(net, H, lds_w);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4;
-    const int per_block = 4 * 16 * NT;
+    const int per_block = (BLOCK / 64) * 16 * NT;
     for (int bbase = blockIdx.x * per_block; bbase < n; bbase += gridDim.x * per_block) {
         keep_loads_in_loop();
         const int base = bbase + wave * 16 * NT;
@@ -701,25 +712,58 @@ static int guide_blocks(int64_t n, int per_block) {
     return int(b < 1 ? 1 : (b > 8192 ? 8192 : b));
 }
 
+// Compute units and the LDS a workgroup may allocate on the current device.
+static void device_limits(int& cus, int& lds_per_block) {
+    int dev = 0;
+    cus = 0;
+    lds_per_block = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 0;
+    if (hipDeviceGetAttribute(&lds_per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+        lds_per_block = 0;
+}
+
+// The forward fragments are staged in LDS once per workgroup, and the
+// workgroups stay resident and stride over the queue, so that the staging is
+// paid once per workgroup rather than once per tile: width 64 (43 KB) with
+// three 256-thread workgroups per CU, width 128 (135 KB) with one 1024-thread
+// workgroup per CU where the device lets a workgroup hold that much LDS.
 void launch_guide_infer(const GuideNet& net, int H, const DevScene& s, const DevPaths& p, int cur, int max_paths,
                         float* raw_out, hipStream_t st) {
+    int cus = 0, lds_max = 0;
+    device_limits(cus, lds_max);
+    const size_t lds = size_t(GuideDims{H}.fwd_size()) * 2;
     if (H == 64) {
-        const size_t lds = size_t(GuideDims{64}.fwd_size()) * 2;
-        hipLaunchKernelGGL((k_guide_infer<64, 2, true>), dim3(guide_blocks(max_paths, 128)), dim3(256), lds, st, net, s, p,
-                           cur, raw_out);
+        int blocks = guide_blocks(max_paths, 128);
+        if (cus > 0) blocks = std::min(blocks, 3 * cus);
+        hipLaunchKernelGGL((k_guide_infer<64, 2, true, 256>), dim3(blocks), dim3(256), lds, st, net, s, p, cur, raw_out);
+    } else if (lds_max > 0 && size_t(lds_max) >= lds && cus > 0) {
+        const int blocks = std::min(guide_blocks(max_paths, 256), cus);
+        hipLaunchKernelGGL((k_guide_infer<128, 1, true, 1024>), dim3(blocks), dim3(1024), lds, st, net, s, p, cur,
+                           raw_out);
     } else {
-        hipLaunchKernelGGL((k_guide_infer<128, 1, false>), dim3(guide_blocks(max_paths, 64)), dim3(256), 0, st, net, s, p,
-                           cur, raw_out);
+        hipLaunchKernelGGL((k_guide_infer<128, 1, false, 256>), dim3(guide_blocks(max_paths, 64)), dim3(256), 0, st, net,
+                           s, p, cur, raw_out);
     }
 }
 void launch_guide_forward_points(const GuideNet& net, int H, const float* pts, int n, float* raw_out, hipStream_t st) {
     if (n <= 0) return;
-    if (H == 64)
-        hipLaunchKernelGGL((k_guide_forward_points<64, 2>), dim3(guide_blocks(n, 128)), dim3(256), 0, st, net, pts, n,
+    int cus = 0, lds_max = 0;
+    device_limits(cus, lds_max);
+    const size_t lds = size_t(GuideDims{H}.fwd_size()) * 2;
+    if (H == 64) {
+        int blocks = guide_blocks(n, 128);
+        if (cus > 0) blocks = std::min(blocks, 3 * cus);
+        hipLaunchKernelGGL((k_guide_forward_points<64, 2, true, 256>), dim3(blocks), dim3(256), lds, st, net, pts, n,
                            raw_out);
-    else
-        hipLaunchKernelGGL((k_guide_forward_points<128, 1>), dim3(guide_blocks(n, 64)), dim3(256), 0, st, net, pts, n,
+    } else if (lds_max > 0 && size_t(lds_max) >= lds && cus > 0) {
+        const int blocks = std::min(guide_blocks(n, 256), cus);
+        hipLaunchKernelGGL((k_guide_forward_points<128, 1, true, 1024>), dim3(blocks), dim3(1024), lds, st, net, pts, n,
                            raw_out);
+    } else {
+        hipLaunchKernelGGL((k_guide_forward_points<128, 1, false, 256>), dim3(guide_blocks(n, 64)), dim3(256), 0, st, net,
+                           pts, n, raw_out);
+    }
 }
 void launch_guide_train(const GuideNet& net, int H, const GuideBatch& b, const GuideScratch& s, hipStream_t st) {
     if (H == 64)
