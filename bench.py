@@ -159,12 +159,15 @@ def cpu_baseline(scene, threads: int, target_s: float = 8.0, runs: int = 3) -> d
 
 
 def config5_guided(ctx, scene, hidden: int = 128, light_grid: int = 20, train_spp: int = 8, train_passes: int = 4,
-                   seeds: int = 2) -> dict:
+                   seeds: int = 12) -> dict:
     """Configuration 5 at its full film and 256 spp with the guide network
     widened to `hidden` (and learned light sampling on a light_grid^3 grid),
     against the plain renderer: device time of the full frame, and relMSE at
     equal spp and equal time on a 480x270 crop of the scene (the same camera
-    at a quarter of the resolution) against a 16384-spp plain reference."""
+    at a quarter of the resolution) against a 16384-spp plain reference.  The
+    plain estimator's error is heavy-tailed on this scene (a rare bright path
+    in one 256-spp image can raise its relMSE twentyfold), so the comparison
+    runs `seeds` independent renders and reports the mean and the median."""
     P = scene.render_params(seed=0)
     W, H, spp = P.width, P.height, P.spp
 
@@ -204,10 +207,14 @@ def config5_guided(ctx, scene, hidden: int = 128, light_grid: int = 20, train_sp
         p_err.append(relmse(sp))
         p_t.append(st.ms_total)
     pe, pt_, ge, gt = map(lambda v: float(np.mean(v)), (p_err, p_t, g_err, g_t))
+    pm, gm = float(np.median(p_err)), float(np.median(g_err))
     return {"full_frame": full,
-            "crop": {"film": f"{w}x{h}", "reference": "plain, 16384 spp", "spp": spp, "plain_relmse": pe, "plain_ms": pt_,
-                     "guided_relmse": ge, "guided_ms": gt, "equal_spp_ratio": pe / ge,
-                     "equal_time_ratio_plain_over_guided": (pe * pt_) / (ge * gt)},
+            "crop": {"film": f"{w}x{h}", "reference": "plain, 16384 spp", "spp": spp, "seeds": seeds,
+                     "plain_relmse": pe, "plain_relmse_median": pm, "plain_relmse_max": float(np.max(p_err)),
+                     "plain_ms": pt_, "guided_relmse": ge, "guided_relmse_median": gm,
+                     "guided_relmse_max": float(np.max(g_err)), "guided_ms": gt, "equal_spp_ratio": pe / ge,
+                     "equal_time_ratio_plain_over_guided": (pe * pt_) / (ge * gt),
+                     "equal_time_ratio_median": (pm * pt_) / (gm * gt)},
             "guide": f"network width {hidden}, {train_passes} training passes of {train_spp} spp, learned light "
                      f"sampling on a {light_grid}^3 grid"}
 
