@@ -449,7 +449,7 @@ void build_device_scene(tr_ctx* c, const tr_bvh_opts* opts) {
     // leaf-ordered shading records; the light table lists the emissive
     // triangles in scene order, as the reference's scene.lights does, so that
     // the power distribution over it samples the same light for the same u
-    std::vector<float4> shade(size_t(nt) * 3);
+    std::vector<float4> shade(size_t(nt) * kShadeRecord);   // the fourth float4 (geometric normal) is filled on the device
     std::vector<int> tri_to_light(size_t(nt), -1);
     int nA = 0;
     {
@@ -485,9 +485,11 @@ void build_device_scene(tr_ctx* c, const tr_bvh_opts* opts) {
         const int mi = h.med_in.empty() ? -1 : h.med_in[size_t(prim)];
         const int mo = h.med_out.empty() ? -1 : h.med_out[size_t(prim)];
         const int med = (mi & 0xffff) | (mo << 16);
-        shade[3 * size_t(i)] = f4(n[0], __builtin_bit_cast(float, int32_t(h.mat[size_t(prim)])));
-        shade[3 * size_t(i) + 1] = f4(n[1], __builtin_bit_cast(float, int32_t(tri_to_light[size_t(i)])));
-        shade[3 * size_t(i) + 2] = f4(n[2], __builtin_bit_cast(float, int32_t(med)));
+        float4* rec = shade.data() + kShadeRecord * size_t(i);
+        rec[0] = f4(n[0], __builtin_bit_cast(float, int32_t(h.mat[size_t(prim)])));
+        rec[1] = f4(n[1], __builtin_bit_cast(float, int32_t(tri_to_light[size_t(i)])));
+        rec[2] = f4(n[2], __builtin_bit_cast(float, int32_t(med)));
+        rec[3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         const int li = tri_to_light[size_t(i)];
         if (li >= 0) {
             const f3 L = mk(h.emission[3 * prim], h.emission[3 * prim + 1], h.emission[3 * prim + 2]);
@@ -573,6 +575,7 @@ void build_device_scene(tr_ctx* c, const tr_bvh_opts* opts) {
     upload(c->nodes, r.nodes.data(), r.nodes.size() * sizeof(Node4), st);
     upload(c->tris, r.tris.data(), r.tris.size() * sizeof(TriPacked), st);
     upload(c->tri_shade, shade.data(), shade.size() * sizeof(float4), st);
+    launch_geometric_normals(c->tris.as<float4>(), c->tri_shade.as<float4>(), nt, st);
     // shading class per leaf-ordered triangle (kept until the stream synchronises)
     std::vector<unsigned char> cls(static_cast<size_t>(nt));
     for (int64_t i = 0; i < nt; ++i) {
@@ -645,7 +648,7 @@ void build_device_scene(tr_ctx* c, const tr_bvh_opts* opts) {
             float u[3], v[3];
             for (int k = 0; k < 3; ++k) {
                 const int vi = h.idx[3 * size_t(prim) + k];
-                n[k] = xyz(shade[3 * size_t(i) + k]);
+                n[k] = xyz(shade[kShadeRecord * size_t(i) + k]);
                 u[k] = h.UV[2 * size_t(vi)];
                 v[k] = h.UV[2 * size_t(vi) + 1];
             }

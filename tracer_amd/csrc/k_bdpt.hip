@@ -383,10 +383,8 @@ TR_D bool extend_one(const DevScene& s, const BdPaths& b, const BdConsts& k, int
     const uint32_t pixel = uint32_t(slot % k.npix);
     Pcg rng = load_pcg(b.grng[ray], (uint64_t(pixel) << 1) | 1u);
     const float t = h.x;
-    const float4* tv = s.tr.tris + size_t(tri) * 3;
-    const f3 v0 = xyz(tv[0]), v1 = xyz(tv[1]), v2 = xyz(tv[2]);
-    const f3 ng = normalize(cross(v1 - v0, v2 - v0));
     const TriShade sr = load_shade(s, tri);
+    const f3 ng = sr.ng;
     const int mid = dot(d, ng) <= 0.0f ? sr.med_out : sr.med_in;
     int ev = 0;
     f3 x;

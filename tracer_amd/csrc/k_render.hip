@@ -257,10 +257,8 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
         alive = false;
     } else {
         const float t = h.x;
-        const float4* tv = s.tr.tris + size_t(tri) * 3;
-        const f3 v0 = xyz(tv[0]), v1 = xyz(tv[1]), v2 = xyz(tv[2]);
-        const f3 ng = normalize(cross(v1 - v0, v2 - v0));
         const TriShade sr = load_shade(s, tri);
+        const f3 ng = sr.ng;
         const float dn = dot(d, ng);
         // segment medium: the side of the hit surface the ray arrives from
         const int mid = dn <= 0.0f ? sr.med_out : sr.med_in;
@@ -1116,6 +1114,21 @@ void launch_lc_build(const DevLightCache& lc, const float* light_pmf, float eps,
                      int* work, hipStream_t st) {
     hipLaunchKernelGGL(k_lc_build, dim3((cells + 255) / 256), dim3(256), 0, st, lc, light_pmf, eps, cells, pmf, table,
                        work);
+}
+
+// Geometric normal of every leaf-ordered triangle into its shading record
+// (the fourth float4), computed on the device as the shading code computed it
+// from the vertices.
+__global__ void k_geometric_normals(const float4* tris, float4* tri_shade, int64_t nt) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= nt) return;
+    const float4* tv = tris + size_t(i) * 3;
+    const f3 v0 = xyz(tv[0]), v1 = xyz(tv[1]), v2 = xyz(tv[2]);
+    tri_shade[size_t(i) * kShadeRecord + 3] = f4(normalize(cross(v1 - v0, v2 - v0)), 0.0f);
+}
+void launch_geometric_normals(const float4* tris, float4* tri_shade, int64_t nt, hipStream_t st) {
+    if (nt <= 0) return;
+    hipLaunchKernelGGL(k_geometric_normals, dim3(unsigned((nt + 255) / 256)), dim3(256), 0, st, tris, tri_shade, nt);
 }
 
 // Test batches through the renderer's own device functions: camera rays

@@ -162,6 +162,7 @@ void launch_tex_eval(const DevScene& s, int texture, const float* uv, int64_t n,
 // Light-cache sampling tables (pmf, alias table) of every cell from the training sums.
 void launch_lc_build(const DevLightCache& lc, const float* light_pmf, float eps, int cells, float* pmf, float2* table,
                      int* work, hipStream_t st);
+void launch_geometric_normals(const float4* tris, float4* tri_shade, int64_t nt, hipStream_t st);
 void launch_shading_frames(const DevScene& s, const int* tri, const float* bary, int64_t n, float* frames,
                            hipStream_t st);
 void launch_camera_rays(const DevScene& s, const float* film_uv, const float* lens_uv, int64_t n, float4* rays,

@@ -61,7 +61,10 @@ constexpr int kMatRawRow = 20;
 
 struct DevScene {
     TraceScene tr;
-    const float4* tri_shade;   // 3 float4 per leaf-ordered triangle: (n0, material), (n1, light), (n2, media)
+    // 4 float4 (one 64-byte record) per leaf-ordered triangle: (n0, material),
+    // (n1, light), (n2, media), (geometric normal, 0); a surface hit reads its
+    // record and not the triangle's vertices
+    const float4* tri_shade;
     // shading class per leaf-ordered triangle (kShadeClassLambert, _Disney,
     // _Other): the shade kernels group the paths of a window by it
     const unsigned char* tri_cls;
@@ -104,15 +107,18 @@ struct DevScene {
     DevCamera cam;
 };
 
+constexpr int kShadeRecord = 4;   // float4 per shading record
 struct TriShade {
     f3 n0, n1, n2;
+    f3 ng;   // normalize(cross(v1 - v0, v2 - v0)), written by k_geometric_normals
     int material, light, med_in, med_out;
 };
 TR_D TriShade load_shade(const DevScene& s, int tri) {
-    const float4* p = s.tri_shade + size_t(tri) * 3;
-    const float4 a = p[0], b = p[1], c = p[2];
+    const float4* p = s.tri_shade + size_t(tri) * kShadeRecord;
+    const float4 a = p[0], b = p[1], c = p[2], e = p[3];
     TriShade r;
     r.n0 = xyz(a); r.n1 = xyz(b); r.n2 = xyz(c);
+    r.ng = xyz(e);
     r.material = __float_as_int(a.w);
     r.light = __float_as_int(b.w);
     const int med = __float_as_int(c.w);
