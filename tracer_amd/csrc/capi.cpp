@@ -1582,6 +1582,7 @@ tr_status tr_render(tr_ctx* c, const tr_render_params* prm, double* film_sums, i
             // glass needs up to kMaxSpecularDepth more rounds
             const int rounds = P.max_depth + 1 + (c->has_delta ? kMaxSpecularDepth : 0);
             for (int b = 0; b < rounds; ++b) {
+                rc.first_bounce = b == 0 ? 1 : 0;
                 if (use_tail) timed(6, [&] { launch_tail(c->ds, p, rc, cur, std::min(tail, num_paths), st); });
                 timed(0, [&] { launch_extend(c->ds.tr, p, cur, num_paths, P.profile == 2, st); });
                 if (guided && rc.guide_frac > 0.0f) timed(4, [&] { guide_infer(c, p, cur, num_paths); });

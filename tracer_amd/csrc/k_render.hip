@@ -66,9 +66,9 @@ __global__ __launch_bounds__(256) void k_generate(DevScene s, DevPaths p, Render
     camera_ray(s.cam, fx, fy, lu, lv, o, d);
     p.ray_o[i] = f4(o, 0.0f);
     p.ray_d[i] = f4(d, 3.0e38f);
-    p.thr[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-    p.rad[i] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-1));
-    p.info[i] = make_uint2(pixel, pack_info(0, 0, 0));
+    // throughput (1, 1, 1, pdf 0), radiance (0, 0, 0, no cell) and (pixel,
+    // depth 0) are not stored: the first shading of path i starts from them
+    // (first_path_state)
     store_rng(p, i, rng);
     p.queue[cur][i] = i;
     if (p.grec) p.grec[i] = -1;
@@ -201,12 +201,22 @@ TR_D void put_shadow(const DevPaths& p, size_t e, const ShadowOut& sh) {
     p.sh_c[e] = f4(sh.contrib, 0.0f);
 }
 
+// The state k_generate leaves implicit for path `slot` of a pass (its slot is
+// its generation index): throughput (1, 1, 1) with no BSDF pdf, no radiance
+// and no light-cache cell yet, its pixel, depth 0.
+TR_D void first_path_state(const RenderConsts& c, int slot, float4& tp, float4& rp, uint2& info) {
+    tp = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    rp = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-1));
+    info = make_uint2(uint32_t(slot) % uint32_t(c.width * c.height), pack_info(0, 0, 0));
+}
+
 // One vertex of the path in `slot`: reads its state and hit, adds emission,
 // samples the continuation and a light, and stores the updated state.
+// first: the path's first vertex, whose state is implicit (first_path_state).
 // EXT: the scene has an image environment or textured materials (their
 // code is compiled in only then).
 template <bool GUIDE, bool EXT, int LCM, bool MULTI = false>
-TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c, int slot, ShadeOut& out) {
+TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c, int slot, bool first, ShadeOut& out) {
     bool& cont = out.cont;
     ShadowOut& sh = out.sh;
     bool &g_used = out.g_used, &g_fallback = out.g_fallback, &want_rec = out.want_rec, &want_nee_rec = out.want_nee_rec;
@@ -229,9 +239,14 @@ TR_D void shade_path(const DevScene& s, const DevPaths& p, const RenderConsts& c
         for (int k = 0; k < nls; ++k) p.sh_d[size_t(slot) * nls + k] = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
     float4 next_o = make_float4(0, 0, 0, 0), next_d = make_float4(0, 0, 0, 0);
     const float4 ro = p.ray_o[slot], rd = p.ray_d[slot], h = p.hit[slot];
-    float4 tp = p.thr[slot];
-    float4 rp = p.rad[slot];
-    const uint2 info = p.info[slot];
+    float4 tp, rp;
+    uint2 info;
+    if (first) first_path_state(c, slot, tp, rp, info);
+    else {
+        tp = p.thr[slot];
+        rp = p.rad[slot];
+        info = p.info[slot];
+    }
     const int depth = int(info.y & 0xffu);
     int sdepth = int((info.y >> 8) & 0xffu);
     int scatter = int(info.y >> 16);
@@ -667,7 +682,7 @@ TR_D void shade_queue(const DevScene& s, const DevPaths& p, const RenderConsts& 
             int slot = 0;
             if (active) {
                 slot = slot_in;
-                shade_path<GUIDE, EXT, LCM, MULTI>(s, p, c, slot, o);
+                shade_path<GUIDE, EXT, LCM, MULTI>(s, p, c, slot, c.first_bounce != 0, o);
             }
             const bool cont = o.cont, g_used = o.g_used, g_fallback = o.g_fallback;
             const bool want_rec = o.want_rec, want_nee_rec = o.want_nee_rec;
@@ -771,6 +786,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_tail(DevScene s, DevPaths p, Re
     for (unsigned int i = gt; i < n; i += stride) {
         const int slot = q[i];
         bool cont = true;
+        bool first = c.first_bounce != 0;
         while (cont) {
             const float4 a = p.ray_o[slot], b = p.ray_d[slot];
             TraceHit h;
@@ -779,7 +795,8 @@ __global__ __launch_bounds__(kShadeBlock) void k_tail(DevScene s, DevPaths p, Re
             p.hit[slot] = make_float4(h.t, __int_as_float(h.tri), h.b1, h.b2);
             ++n_ext;
             ShadeOut o;
-            shade_path<false, EXT, LCM>(s, p, c, slot, o);
+            shade_path<false, EXT, LCM>(s, p, c, slot, first, o);
+            first = false;
             if (o.sh.emit) {
                 ++n_sh;
                 TraceHit hs;

@@ -105,6 +105,8 @@ struct RenderConsts {
     int shadow_tr;       // TR_TR_*
     int nls;             // light samples per vertex
     int shade_sort_rounds;   // shade kernels order a window by class when it spans at least this many rounds
+    int first_bounce;        // the queue holds the camera paths of k_generate, whose throughput, radiance
+                             // and depth are implicit (1, 0, 0) rather than stored
     unsigned long long seed;
     float guide_frac;    // probability of sampling the guide mixture at a surface vertex
     int guide_max_depth; // guide only vertices of depth < this (0 = every depth)
