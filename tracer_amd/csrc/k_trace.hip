@@ -95,6 +95,9 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_counted(TraceScene sc, co
 // per lane at a time) 25.6 ms of k_extend and 8.9 ms of k_shadow; refilling
 // below 48 busy lanes 22.2 and 7.7 ms; below 32, 22.1 and 8.0 ms; refilling
 // any idle lane at once, 24.6 and 7.8 ms (the wave's rays drift apart).
+// Re-swept with leaf postponement and class-ordered shading (40, 48 or 56
+// lanes, separately for closest-hit and any-hit rays): configuration 2 and 5
+// frames all within 1% of each other.
 // Fetching single rays from a global work counter was much slower (the
 // atomics contend), and so was dealing the queue out in chunks of 256 entries
 // from a device counter once a wave's chunk ran out (configuration 2:
