@@ -762,6 +762,14 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(3))
     shade_queue<false, false, LCM>(s, p, c, cur);
 }
 
+// The plain kernel of scenes with an image environment, textures or density
+// grids, held to three waves per SIMD like the plain one.
+template <int LCM>
+__global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_shade3x(DevScene s, DevPaths p,
+                                                                                                  RenderConsts c, int cur) {
+    shade_queue<false, true, LCM>(s, p, c, cur);
+}
+
 // The last bounces of a plain pass in one launch: once the queue holds at
 // most max_tail paths, each thread takes one and carries it to its end
 // (closest hit, shading, shadow ray, next bounce).  Late bounces hold few
@@ -974,7 +982,7 @@ static int plain_lc_mode(const DevPaths& p) {
 }
 template <bool EXT, int LCM>
 static void launch_shade_plain(const DevScene& s, const DevPaths& p, const RenderConsts& c, int cur, dim3 g, hipStream_t st) {
-    if constexpr (EXT) hipLaunchKernelGGL((k_shade<false, EXT, LCM>), g, dim3(kShadeBlock), 0, st, s, p, c, cur);
+    if constexpr (EXT) hipLaunchKernelGGL((k_shade3x<LCM>), g, dim3(kShadeBlock), 0, st, s, p, c, cur);
     else hipLaunchKernelGGL((k_shade3<LCM>), g, dim3(kShadeBlock), 0, st, s, p, c, cur);
 }
 void launch_shade(const DevScene& s, const DevPaths& p, const RenderConsts& c, int cur, int max_paths, bool guide,
